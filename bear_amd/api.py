@@ -1,0 +1,264 @@
+"""Python binding of the C ABI in include/bear_hip.h (ctypes).
+
+Mirrors the pyBeAR surface that the likelihood path uses
+(python/src/bear_python_wrapper.cpp:27-373 of the reference):
+``GlobalConfig``, ``ObservationInput``, ``Retrieval.computeLikelihood``,
+``Retrieval.computeModel`` (returning a ``ForwardModelOutput``) and
+``ForwardModel.calcModelGPU`` on device buffers.  The library is the in-tree
+``bear_amd/libbear_hip.so``; importing this module on a machine without it
+raises, there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbear_hip.so")
+
+_ERRORS = {-1: "InvalidInput", -2: "FileNotFound", -3: "DeviceError", -4: "OutOfMemory", -5: "InternalError"}
+
+
+class BearError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _GlobalConfig(C.Structure):
+    _fields_ = [("use_gpu", C.c_int), ("device", C.c_int), ("retrieval_folder_path", C.c_char_p),
+                ("cross_section_file_path", C.c_char_p), ("wavenumber_file_path", C.c_char_p),
+                ("spectral_discretisation", C.c_int), ("spectral_resolution", C.c_double),
+                ("use_error_inflation", C.c_int)]
+
+
+class _ModelConfig(C.Structure):
+    _fields_ = [("nb_grid_points", C.c_size_t), ("atmos_bottom_pressure", C.c_double),
+                ("atmos_top_pressure", C.c_double), ("temperature_profile", C.c_char_p),
+                ("radiative_transfer", C.c_char_p), ("chemistry", C.c_char_p), ("opacity_species", C.c_char_p)]
+
+
+_dp = C.POINTER(C.c_double)
+
+
+class _ObservationInput(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_char_p), ("nb_points", C.c_size_t),
+                ("wavelengths", _dp), ("bin_edges", _dp), ("data", _dp), ("data_error", _dp),
+                ("likelihood_weight", _dp), ("instrument_profile_fwhm", _dp),
+                ("filter_nb_points", C.c_size_t), ("filter_wavelength", _dp), ("filter_response", _dp),
+                ("filter_detector_type", C.c_char_p), ("spectrum_modifier", C.c_char_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbear_hip.so (built by __graft_entry__.build / make -C bear_amd/csrc)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"bear_amd native library is not built: {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        vp, sz = C.c_void_p, C.c_size_t
+        L.bear_last_error.restype = C.c_char_p
+        L.bear_version.restype = C.c_char_p
+        L.bear_opacity_data_register.argtypes = [C.c_char_p, C.c_double, C.c_int, sz, _dp, _dp, sz,
+                                                 C.POINTER(C.c_float)]
+        L.bear_retrieval_create.argtypes = [C.POINTER(_GlobalConfig), C.c_char_p, C.POINTER(_ModelConfig),
+                                            C.POINTER(_ObservationInput), sz, C.POINTER(C.c_char_p),
+                                            C.POINTER(C.c_char_p), sz, C.POINTER(vp)]
+        L.bear_retrieval_destroy.argtypes = [vp]
+        for f in ("bear_retrieval_nb_parameters", "bear_retrieval_nb_spectral_points",
+                  "bear_retrieval_nb_observations"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp]
+        L.bear_retrieval_nb_observation_points.restype = sz
+        L.bear_retrieval_nb_observation_points.argtypes = [vp, sz]
+        L.bear_retrieval_spectral_grid.argtypes = [vp, _dp, _dp]
+        L.bear_retrieval_compute_likelihood.argtypes = [vp, _dp, sz, _dp]
+        L.bear_retrieval_compute_model.argtypes = [vp, _dp, sz, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+        L.bear_forward_model_calc_model_gpu.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BearError(rc, lib().bear_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+@dataclass
+class GlobalConfig:
+    """GlobalConfig (config/global_config.h:28-71): the fields the likelihood path reads."""
+    retrieval_folder_path: str = ""
+    cross_section_file_path: str = ""
+    wavenumber_file_path: str = ""
+    spectral_disecretisation: str = "const_wavenumber"
+    spectral_resolution: float = 1.0
+    use_error_inflation: bool = False
+    use_gpu: bool = True
+    device: int = 0
+
+    def _c(self):
+        kinds = {"const_wavenumber": 0, "const_wavelength": 1, "const_resolution": 2}
+        if self.spectral_disecretisation not in kinds:
+            raise BearError(-1, f"unknown spectral discretisation {self.spectral_disecretisation}")
+        self._keep = [_b(self.retrieval_folder_path), _b(self.cross_section_file_path), _b(self.wavenumber_file_path)]
+        return _GlobalConfig(int(self.use_gpu), self.device, *self._keep, kinds[self.spectral_disecretisation],
+                             float(self.spectral_resolution), int(self.use_error_inflation))
+
+
+@dataclass
+class ModelConfig:
+    """Forward-model configuration in forward_model.config terms."""
+    nb_grid_points: int
+    atmos_bottom_pressure: float
+    atmos_top_pressure: float
+    temperature_profile: str                 # "poly 6 1" | "const"
+    chemistry: list                          # ["iso H2O CH4", "bg H2He"]
+    opacity_species: list                    # [("H2O", "Molecules/H2O"), ...]
+    radiative_transfer: str = "scm"
+
+    def _c(self):
+        chem = "\n".join(self.chemistry)
+        opac = "\n".join(f"{s} {f}" for s, f in self.opacity_species)
+        self._keep = [_b(self.temperature_profile), _b(self.radiative_transfer), _b(chem), _b(opac)]
+        return _ModelConfig(int(self.nb_grid_points), float(self.atmos_bottom_pressure),
+                            float(self.atmos_top_pressure), self._keep[0], self._keep[1], self._keep[2], self._keep[3])
+
+
+@dataclass
+class ObservationInput:
+    """ObservationInput (observations/observations.h:43-83)."""
+    name: str
+    type: str
+    data: np.ndarray
+    data_error: np.ndarray
+    wavelengths: np.ndarray | None = None
+    bin_wavelength_edges: np.ndarray | None = None      # [n, 2]
+    likelihood_weight: np.ndarray | None = None
+    instrument_profile_fwhm: np.ndarray | None = None
+    filter_response: tuple | None = None                # (wavelength[], response[])
+    filter_detector_type: str = ""
+    spectrum_modifier_id: str = "none"
+
+    def _c(self):
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        arrs = dict(w=f64(self.wavelengths), e=f64(self.bin_wavelength_edges), d=f64(self.data),
+                    s=f64(self.data_error), lw=f64(self.likelihood_weight), fw=f64(self.instrument_profile_fwhm))
+        fl = fr = None
+        if self.filter_response is not None:
+            fl, fr = f64(self.filter_response[0]), f64(self.filter_response[1])
+        self._keep = (arrs, fl, fr, _b(self.name), _b(self.type), _b(self.filter_detector_type),
+                      _b(self.spectrum_modifier_id))
+        return _ObservationInput(self._keep[3], self._keep[4], arrs["d"].size, _ptr(arrs["w"]), _ptr(arrs["e"]),
+                                 _ptr(arrs["d"]), _ptr(arrs["s"]), _ptr(arrs["lw"]), _ptr(arrs["fw"]),
+                                 0 if fl is None else fl.size, _ptr(fl), _ptr(fr), self._keep[5], self._keep[6])
+
+
+@dataclass
+class ForwardModelOutput:
+    neglect_model: bool = False
+    spectrum: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    spectrum_obs: list = field(default_factory=list)
+
+
+def register_opacity_data(folder, temperatures, pressures, rows, species_mass=0.0, log_storage=False):
+    """Hold one species' table in memory under ``folder`` (instead of a
+    filelist.dat folder): rows [nb_rows, nb_wavenumbers] float32 over the full
+    wavenumber list, with the (T, p) of each row."""
+    rows = np.ascontiguousarray(rows, dtype=np.float32)
+    t = np.ascontiguousarray(temperatures, dtype=np.float64)
+    p = np.ascontiguousarray(pressures, dtype=np.float64)
+    _check(lib().bear_opacity_data_register(_b(folder), float(species_mass), int(log_storage), rows.shape[0],
+                                            _ptr(t), _ptr(p), rows.shape[1], rows.ctypes.data_as(C.POINTER(C.c_float))))
+
+
+def clear_opacity_data():
+    _check(lib().bear_opacity_data_clear())
+
+
+class Retrieval:
+    """Retrieval restricted to the likelihood path (retrieval/retrieval.h)."""
+
+    def __init__(self, config: GlobalConfig, model_config: ModelConfig, observations=(),
+                 observation_files=(), observation_modifiers=None, forward_model_type="emission"):
+        L = lib()
+        gc, mc = config._c(), model_config._c()
+        obs = list(observations)
+        arr = (_ObservationInput * max(len(obs), 1))(*[o._c() for o in obs])
+        files = [_b(f) for f in observation_files]
+        mods = [_b(m) for m in (observation_modifiers or ["none"] * len(files))]
+        f_arr = (C.c_char_p * max(len(files), 1))(*files)
+        m_arr = (C.c_char_p * max(len(mods), 1))(*mods)
+        h = C.c_void_p()
+        _check(L.bear_retrieval_create(C.byref(gc), _b(forward_model_type), C.byref(mc), arr, len(obs), f_arr, m_arr,
+                                       len(files), C.byref(h)))
+        self._L, self._h = L, h
+        self.nb_parameters = L.bear_retrieval_nb_parameters(h)
+        self.nb_spectral_points = L.bear_retrieval_nb_spectral_points(h)
+        self.nb_observations = L.bear_retrieval_nb_observations(h)
+        self.nb_observation_points = [L.bear_retrieval_nb_observation_points(h, i) for i in range(self.nb_observations)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bear_retrieval_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def nbParameters(self):
+        return self.nb_parameters
+
+    def spectral_grid(self):
+        nu = np.empty(self.nb_spectral_points)
+        lam = np.empty(self.nb_spectral_points)
+        _check(self._L.bear_retrieval_spectral_grid(self._h, _ptr(nu), _ptr(lam)))
+        return nu, lam
+
+    def computeLikelihood(self, physical_parameters) -> float:
+        p = np.ascontiguousarray(physical_parameters, dtype=np.float64)
+        out = C.c_double()
+        _check(self._L.bear_retrieval_compute_likelihood(self._h, _ptr(p), p.size, C.byref(out)))
+        return out.value
+
+    def computeModel(self, physical_parameters, return_high_res_spectrum=True) -> ForwardModelOutput:
+        p = np.ascontiguousarray(physical_parameters, dtype=np.float64)
+        spec = np.empty(self.nb_spectral_points) if return_high_res_spectrum else None
+        obs = np.empty(sum(self.nb_observation_points))
+        neg = C.c_int(0)
+        _check(self._L.bear_retrieval_compute_model(self._h, _ptr(p), p.size, int(return_high_res_spectrum),
+                                                    _ptr(spec), _ptr(obs), C.byref(neg)))
+        out, k = [], 0
+        for n in self.nb_observation_points:
+            out.append(obs[k:k + n].copy())
+            k += n
+        return ForwardModelOutput(bool(neg.value), spec if spec is not None else np.zeros(0), out)
+
+    def calcModelGPU(self, physical_parameters, spectrum_ptr, spectrum_obs_ptrs):
+        """ForwardModel::calcModelGPU on caller-owned device buffers (raw
+        addresses, e.g. torch tensors' data_ptr()); returns the neglect flag."""
+        p = np.ascontiguousarray(physical_parameters, dtype=np.float64)
+        ptrs = (C.c_void_p * max(len(spectrum_obs_ptrs), 1))(*[C.c_void_p(int(x)) for x in spectrum_obs_ptrs])
+        neg = C.c_int(0)
+        _check(self._L.bear_forward_model_calc_model_gpu(self._h, _ptr(p), p.size,
+                                                         C.c_void_p(int(spectrum_ptr)) if spectrum_ptr else None,
+                                                         ptrs, C.byref(neg)))
+        return bool(neg.value)
+
+
+def version():
+    return lib().bear_version().decode()

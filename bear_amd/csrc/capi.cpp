@@ -1,0 +1,184 @@
+// C ABI (include/bear_hip.h) over the host C++ classes.
+#include "bear_hip.h"
+
+#include <cstring>
+#include <new>
+#include <sstream>
+#include <string>
+
+#include "model.h"
+
+struct bear_retrieval {
+  std::unique_ptr<bear_amd::Retrieval> impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return BEAR_OK;
+  } catch (const bear_amd::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "out of host memory";
+    return BEAR_ERROR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BEAR_ERROR_INTERNAL;
+  }
+}
+
+std::string str(const char* s) { return s ? std::string(s) : std::string(); }
+
+std::vector<double> vec(const double* p, size_t n) { return p ? std::vector<double>(p, p + n) : std::vector<double>(); }
+
+}  // namespace
+
+extern "C" {
+
+const char* bear_last_error(void) { return g_last_error.c_str(); }
+
+const char* bear_version(void) { return "bear-mi355x 0.1 (gfx950)"; }
+
+int bear_opacity_data_register(const char* folder, double species_mass, int log_storage, size_t nb_rows,
+                               const double* temperatures, const double* pressures, size_t nb_wavenumbers,
+                               const float* rows) {
+  return guarded([&] {
+    if (!folder || !temperatures || !pressures || !rows || nb_rows == 0 || nb_wavenumbers == 0)
+      bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "bear_opacity_data_register: empty table");
+    bear_amd::registerTable(folder, species_mass, log_storage != 0, nb_rows, temperatures, pressures, nb_wavenumbers,
+                            rows);
+  });
+}
+
+int bear_opacity_data_clear(void) {
+  return guarded([] { bear_amd::clearRegisteredTables(); });
+}
+
+int bear_retrieval_create(const bear_global_config* config, const char* forward_model_type,
+                          const bear_model_config* model_config, const bear_observation_input* observations,
+                          size_t nb_observations, const char* const* obs_files, const char* const* obs_modifiers,
+                          size_t nb_obs_files, bear_retrieval** out) {
+  return guarded([&] {
+    if (!config || !model_config || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    if (!config->use_gpu) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "this engine runs on the GPU only (use_gpu = 1)");
+    bear_amd::GlobalSettings g;
+    g.device = config->device;
+    g.retrieval_folder = str(config->retrieval_folder_path);
+    g.cross_section_path = str(config->cross_section_file_path);
+    g.wavenumber_file = str(config->wavenumber_file_path);
+    g.discretisation = config->spectral_discretisation;
+    g.resolution = config->spectral_resolution;
+    g.error_inflation = config->use_error_inflation != 0;
+
+    bear_amd::ModelSettings m;
+    m.nb_levels = model_config->nb_grid_points;
+    m.bottom_pressure = model_config->atmos_bottom_pressure;
+    m.top_pressure = model_config->atmos_top_pressure;
+    m.temperature = str(model_config->temperature_profile);
+    m.radiative_transfer = str(model_config->radiative_transfer);
+    m.chemistry = str(model_config->chemistry);
+    std::istringstream op(str(model_config->opacity_species));
+    std::string line;
+    while (std::getline(op, line)) {
+      std::istringstream in(line);
+      std::string sym, folder;
+      if (!(in >> sym)) continue;
+      if (!(in >> folder)) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "opacity species " + sym + " without a folder");
+      m.opacity.emplace_back(sym, folder);
+    }
+
+    std::vector<bear_amd::ObservationInputView> inputs;
+    for (size_t i = 0; i < nb_observations; ++i) {
+      const bear_observation_input& o = observations[i];
+      bear_amd::ObservationInputView v;
+      v.name = str(o.name);
+      v.type = str(o.type);
+      v.modifier = str(o.spectrum_modifier);
+      const size_t n = o.nb_points;
+      if (!o.data || !o.data_error || n == 0) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "observation without data");
+      v.data = vec(o.data, n);
+      v.error = vec(o.data_error, n);
+      v.weight = vec(o.likelihood_weight, n);
+      v.fwhm = vec(o.instrument_profile_fwhm, n);
+      if (v.type == "spectroscopy") {
+        if (!o.wavelengths) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "spectroscopy needs wavelengths");
+        v.wavelengths = vec(o.wavelengths, n);
+      } else {
+        if (!o.bin_edges) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, v.type + " needs bin edges");
+        for (size_t b = 0; b < n; ++b) v.bin_edges.push_back({o.bin_edges[2 * b], o.bin_edges[2 * b + 1]});
+      }
+      if (o.filter_nb_points > 0) {
+        v.filter_wavelength = vec(o.filter_wavelength, o.filter_nb_points);
+        v.filter_response = vec(o.filter_response, o.filter_nb_points);
+        v.detector = str(o.filter_detector_type);
+      }
+      inputs.push_back(std::move(v));
+    }
+    std::vector<std::string> files, mods;
+    for (size_t i = 0; i < nb_obs_files; ++i) {
+      files.push_back(str(obs_files[i]));
+      mods.push_back(obs_modifiers ? str(obs_modifiers[i]) : std::string());
+    }
+    auto* r = new bear_retrieval;
+    try {
+      r->impl.reset(new bear_amd::Retrieval(g, str(forward_model_type), m, inputs, files, mods));
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    *out = r;
+  });
+}
+
+void bear_retrieval_destroy(bear_retrieval* r) { delete r; }
+
+size_t bear_retrieval_nb_parameters(const bear_retrieval* r) { return r ? r->impl->nbParameters() : 0; }
+size_t bear_retrieval_nb_spectral_points(const bear_retrieval* r) { return r ? r->impl->grid().size() : 0; }
+size_t bear_retrieval_nb_observations(const bear_retrieval* r) { return r ? r->impl->observations().size() : 0; }
+size_t bear_retrieval_nb_observation_points(const bear_retrieval* r, size_t i) {
+  return (r && i < r->impl->observations().size()) ? r->impl->observations()[i].nbPoints() : 0;
+}
+
+int bear_retrieval_spectral_grid(const bear_retrieval* r, double* wavenumbers, double* wavelengths) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    const auto& g = r->impl->grid();
+    if (wavenumbers) std::memcpy(wavenumbers, g.wavenumber.data(), g.size() * sizeof(double));
+    if (wavelengths) std::memcpy(wavelengths, g.wavelength.data(), g.size() * sizeof(double));
+  });
+}
+
+int bear_retrieval_compute_likelihood(bear_retrieval* r, const double* parameters, size_t n, double* log_likelihood) {
+  return guarded([&] {
+    if (!r || !parameters || !log_likelihood) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    *log_likelihood = r->impl->computeLikelihood(parameters, n);
+  });
+}
+
+int bear_retrieval_compute_model(bear_retrieval* r, const double* parameters, size_t n, int return_high_res,
+                                 double* spectrum, double* spectrum_obs, int* neglect_model) {
+  return guarded([&] {
+    if (!r || !parameters || !spectrum_obs) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    const bool neglect = r->impl->computeModel(parameters, n, return_high_res != 0 && spectrum, spectrum, spectrum_obs);
+    if (neglect_model) *neglect_model = neglect;
+  });
+}
+
+int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameters, size_t n, double* spectrum_dev,
+                                      double* const* spectrum_obs_dev, int* neglect_model) {
+  return guarded([&] {
+    if (!r || !parameters) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    const bool neglect = r->impl->calcModelGPU(parameters, n, spectrum_dev, spectrum_obs_dev);
+    if (neglect_model) *neglect_model = neglect;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// Device side of the engine: memory owned on one GPU and the kernels of one
+// likelihood evaluation.  No HIP types appear here, so host-only C++ can
+// drive it.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "host.h"
+
+namespace bear_amd {
+
+// Per level: temperature and altitude of the model atmosphere.
+struct LevelState {
+  double temperature;
+  double inv_temperature;
+  double altitude;
+  double pad;
+};
+
+// Per (level, opacity species): effective number density (n_s, or n_s times
+// the CIA partners' densities) and the table stencil.  n == 0 skips the
+// species, as opacity_species.cpp:409-412 does.
+struct LevelSpecies {
+  double n;
+  double f_p_lower, f_p_upper, f_t;
+  uint32_t row[4];
+};
+
+// Per bin of every observation, in observation order: the folded operator
+// segment and the likelihood inputs.
+struct BinDesc {
+  uint64_t weight_offset;
+  uint32_t start;
+  uint32_t length;
+  uint32_t observation;
+  uint32_t pad;
+  double data, error, weight;
+};
+
+class DeviceEngine {
+ public:
+  explicit DeviceEngine(int device);
+  ~DeviceEngine();
+  DeviceEngine(const DeviceEngine&) = delete;
+  DeviceEngine& operator=(const DeviceEngine&) = delete;
+
+  void uploadGrid(const std::vector<double>& wavenumber, const std::vector<double>& wavelength);
+  void uploadTables(const TableStore& store);
+  void uploadObservations(const std::vector<ObservationSetup>& observations);
+  void configure(size_t nb_levels, size_t nb_species);
+
+  // pinned staging of the per-evaluation inputs; filled by the caller
+  LevelState* levels() { return levels_host_; }
+  LevelSpecies* levelSpecies() { return species_host_; }
+  double* shifts() { return shift_host_; }
+
+  // One emission evaluation: opacity + short characteristics -> flux on the
+  // grid (kept on the device), optional high-resolution output in
+  // W m^-2 micron^-1 (spectrum_dev), bins (bins_dev or the engine's own
+  // buffer), per-bin likelihood terms and their fixed-order sum.
+  void evaluateEmission(double scaling, double error_inflation, double* spectrum_dev, double* const* bins_dev);
+  double logLikelihood();                        // waits for the evaluation
+  void copyBins(std::vector<double>& out);       // internal (descending) order, all observations
+  void copySpectrum(double* host, const double* spectrum_dev);
+  void synchronize();
+  double* spectrumBuffer();                      // engine-owned [N], allocated on first use
+
+  size_t nbPoints() const { return n_points_; }
+  size_t nbBins() const { return n_bins_; }
+  int device() const { return device_; }
+  void* stream() const { return stream_; }
+
+ private:
+  int device_ = 0;
+  void* stream_ = nullptr;
+  size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0;
+  size_t nb_levels_ = 0, nb_species_ = 0;
+  double* wavenumber_ = nullptr;
+  double* wavelength_ = nullptr;
+  double* tables_ = nullptr;
+  double* flux_ = nullptr;
+  double* spectrum_ = nullptr;
+  double* weights_ = nullptr;
+  BinDesc* bins_desc_ = nullptr;
+  double* bins_ = nullptr;
+  double* ll_terms_ = nullptr;
+  double* ll_ = nullptr;
+  std::vector<size_t> obs_bin_offset_;
+  // per-evaluation inputs: one pinned block, one device block
+  unsigned char* staging_host_ = nullptr;
+  unsigned char* staging_dev_ = nullptr;
+  size_t staging_bytes_ = 0;
+  LevelState* levels_host_ = nullptr;
+  LevelSpecies* species_host_ = nullptr;
+  double* shift_host_ = nullptr;
+  double* ll_host_ = nullptr;
+};
+
+}  // namespace bear_amd

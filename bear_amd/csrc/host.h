@@ -1,0 +1,191 @@
+// Host side of the engine: spectral grid, observations, opacity tables,
+// atmosphere and the forward models that drive the device kernels.
+//
+// Reference citations are relative to the reference repository's src/.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bear_amd {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void raise(int code, const std::string& msg);
+
+constexpr int kInvalidInput = -1, kFileNotFound = -2, kDevice = -3, kOutOfMemory = -4;
+
+// ------------------------------------------------------------------ constants
+// cgs constants as in additional/physical_const.h:28-52
+namespace phys {
+constexpr double planck_h = 6.62606896e-27;
+constexpr double light_c = 2.99792458e10;
+constexpr double boltzmann_k = 1.3806504e-16;
+constexpr double gas_constant = 8.314472e7;
+constexpr double radius_jupiter = 7.1492e9;
+constexpr double pi = 3.14159265358979323846;
+constexpr double avogadro = 6.022140857e23;
+}  // namespace phys
+
+// Chemical species ids: indices into a level's number densities, in the
+// order of chemistry/chem_species.h:30-90 (with its molecular weights).
+int speciesId(const std::string& symbol);           // throws InvalidInput
+int nbSpeciesIds();
+double molecularWeight(int id);
+namespace sid { constexpr int total = 0, H = 1, He = 2, H2 = 10, Na = 19, K = 20; }
+
+// ------------------------------------------------------------------ grid
+// SpectralGrid (spectral_grid/spectral_grid.cpp): the full list the tables
+// are tabulated on and the sampled subset the models are computed on.
+class ObservationSetup;
+
+class SpectralGrid {
+ public:
+  std::vector<double> full_wavenumber, full_wavelength;
+  std::vector<size_t> index;               // positions of the sampled points in the full list
+  std::vector<double> wavenumber, wavelength;
+  int discretisation = 0;
+  double resolution = 0;
+
+  void loadFullList(const std::string& path);
+  void sample(std::vector<ObservationSetup>& observations);
+  size_t size() const { return wavenumber.size(); }
+  // nearest element of a monotone list, searching from position start
+  static size_t nearest(double x, const std::vector<double>& list, size_t start);
+};
+
+// ------------------------------------------------------------------ observations
+enum class BandType { spectroscopy, band_spectroscopy, photometry };
+
+// One observation after set-up, plus its folded observation operator: each
+// bin's value is a dot product of the high-resolution spectrum with a
+// contiguous weight segment.  The operator composes the bandpass
+// (observations_filter_response.cpp:166-181), the Gaussian line-spread
+// convolution (spectral_band_convolve.cpp:100-144) and the trapezoid band
+// integral (spectral_band_integrate.cpp:39-107), which are linear maps of a
+// non-negative spectrum.
+class ObservationSetup {
+ public:
+  std::string name;
+  BandType type = BandType::spectroscopy;
+  std::vector<double> data, error, weight, fwhm;
+  std::vector<std::array<double, 2>> edges_wavelength;   // descending, [left > right]
+  std::vector<std::array<double, 2>> edges_wavenumber;
+  std::vector<double> centres;
+  bool ascending = false;
+  double range_wavelength[2] = {0, 0};
+  int modifier_parameters = 0;
+  std::string detector;                                   // "energy" | "photon" | "" (no bandpass)
+  std::vector<double> filter_wavelength, filter_response;
+
+  // products of set-up on the sampled grid
+  std::vector<std::array<size_t, 2>> edge_index;
+  size_t range_index[2] = {0, 0};
+  std::vector<double> profile_sigma;                      // [N] or empty
+  std::vector<std::array<size_t, 2>> convolution_window;  // [range] or empty
+  std::vector<double> filter, filter_weight;              // [N] or empty
+  double filter_normalisation = 0;
+
+  // folded operator: bin b covers grid points [segment_start[b], segment_start[b] + segment_length[b])
+  std::vector<size_t> segment_start;
+  std::vector<size_t> segment_length;
+  std::vector<double> segment_weight;                     // concatenated
+  std::vector<size_t> segment_offset;                     // into segment_weight
+
+  size_t nbPoints() const { return data.size(); }
+  void setupOnGrid(const SpectralGrid& grid);
+  void foldOperator(const SpectralGrid& grid, bool is_flux);
+};
+
+struct ObservationInputView {                 // plain copy of bear_observation_input
+  std::string name, type, detector, modifier;
+  std::vector<double> wavelengths, data, error, weight, fwhm, filter_wavelength, filter_response;
+  std::vector<std::array<double, 2>> bin_edges;
+};
+ObservationSetup observationFromInput(const ObservationInputView& in);
+ObservationSetup observationFromFile(const std::string& file, const std::string& modifier,
+                                     const std::string& retrieval_folder);
+
+// ------------------------------------------------------------------ opacity
+// One table row's place in the device table store plus its grid point.
+struct TableRow { double temperature, pressure, log_pressure; uint32_t id; };
+
+// Bracketing of a (p, T) point inside one species' table
+// (opacity_species.cpp:160-273) and the interpolation weights of
+// opacity_species.cpp:277-345: rows r[0..3] = (T-, p-), (T-, p+), (T+, p-), (T+, p+).
+struct Stencil {
+  uint32_t row[4];
+  double f_p_lower, f_p_upper, f_t;
+};
+
+class OpacitySpecies {
+ public:
+  std::string symbol, folder;
+  int species_id = 0;
+  std::vector<int> cia_partners;
+  bool rayleigh = false;
+  std::vector<double> rayleigh_cross_section;        // [N], cm^2
+  std::vector<std::vector<TableRow>> by_temperature;  // ascending T, each ascending p
+
+  bool hasTable() const { return !by_temperature.empty(); }
+  Stencil bracket(double pressure, double temperature) const;
+};
+
+// All species' sampled rows, log10 cross-sections in FP64, [rows][N].
+struct TableStore {
+  size_t nb_points = 0;
+  std::vector<double> log_cross_sections;
+  uint32_t nb_rows = 0;
+};
+
+void loadSpeciesTable(OpacitySpecies& s, const std::string& cross_section_path, const SpectralGrid& grid,
+                      TableStore& store);
+OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& folder, const SpectralGrid& grid);
+
+// ------------------------------------------------------------------ atmosphere
+// Piecewise Lagrange polynomial over log10(p) (additional/piecewise_poly.cpp:49-229)
+class PiecewisePolynomial {
+ public:
+  PiecewisePolynomial(size_t nb_elements, size_t order, double bottom_pressure, double top_pressure);
+  void setValues(const std::vector<double>& dof_values);
+  double value(double x) const;
+  size_t nbDof() const { return vertices_.size(); }
+ private:
+  struct Element { std::vector<double> ref, vertex, value; };
+  std::vector<Element> elements_;
+  std::vector<double> vertices_, values_;
+};
+
+struct ChemistryModule { bool background = false; std::vector<int> species; bool sodium_free = false; };
+
+// Atmosphere (forward_model/atmosphere/atmosphere.cpp) with the temperature
+// profiles "poly" and "const" and the chemistry modules "iso" and "bg H2He".
+class Atmosphere {
+ public:
+  Atmosphere(size_t nb_levels, double bottom_pressure, double top_pressure,
+             const std::string& temperature_desc, const std::string& chemistry_desc);
+  size_t nb_levels;
+  std::vector<double> pressure, temperature, altitude, mean_molecular_weight;
+  std::vector<double> number_density;       // [L][nbSpeciesIds()]
+  size_t nbTemperatureParameters() const { return nb_temperature_param_; }
+  size_t nbChemistryParameters() const { return nb_chemistry_param_; }
+  // Atmosphere::calcAtmosphereStructure (atmosphere.cpp:63-122); returns "neglect"
+  bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param);
+  double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
+ private:
+  size_t n_ids_;
+  std::unique_ptr<PiecewisePolynomial> poly_;
+  bool isothermal_ = false;
+  size_t nb_temperature_param_ = 0, nb_chemistry_param_ = 0;
+  std::vector<ChemistryModule> chemistry_;
+};
+
+}  // namespace bear_amd

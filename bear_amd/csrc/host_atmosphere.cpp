@@ -1,0 +1,192 @@
+// Atmosphere of the host side: pressure grid, temperature profile,
+// chemistry and hydrostatic altitude, evaluated per model in FP64.
+#include <cmath>
+#include <sstream>
+
+#include "host.h"
+
+namespace bear_amd {
+
+// ---------------------------------------------------------------- piecewise polynomial
+// Gauss-Lobatto nodes on [-1, 1] per element order (additional/piecewise_poly.cpp:35-43;
+// orders above 5 are rejected: the reference's order-6 list is malformed)
+PiecewisePolynomial::PiecewisePolynomial(size_t nb_elements, size_t order, double bottom, double top) {
+  static const std::vector<std::vector<double>> nodes = {
+    {0.0}, {-1.0, 1.0}, {-1.0, 0.0, 1.0}, {-1.0, -0.447214, 0.447214, 1.0},
+    {-1.0, -0.654654, 0.654654, 1.0}, {-1.0, -0.654654, 0.0, 0.654654, 1.0}};
+  if (order > 5) raise(kInvalidInput, "piecewise polynomials above order 5 are not available");
+  if (nb_elements < 1) nb_elements = 1;
+  if (order < 1) order = 1;
+  // element edges from log10(p_bottom) downwards (piecewise_poly.cpp:89-132)
+  const double x0 = std::log10(bottom), x1 = std::log10(top);
+  const double width = (x0 - x1) / nb_elements;
+  std::vector<double> edge(nb_elements + 1);
+  edge.front() = x0;
+  edge.back() = x1;
+  for (size_t i = 1; i < nb_elements; ++i) edge[i] = edge[i - 1] - width;
+  for (size_t i = 0; i < nb_elements; ++i) {
+    Element e;
+    e.ref = nodes[order];
+    e.vertex.resize(e.ref.size());
+    e.value.assign(e.ref.size(), 0.0);
+    for (size_t j = 0; j < e.ref.size(); ++j) e.vertex[j] = edge[i] + (1.0 + e.ref[j]) / 2.0 * (edge[i + 1] - edge[i]);
+    elements_.push_back(e);
+  }
+  for (auto& e : elements_)
+    for (size_t j = 0; j + 1 < e.vertex.size(); ++j) vertices_.push_back(e.vertex[j]);
+  vertices_.push_back(elements_.back().vertex.back());
+  values_.assign(vertices_.size(), 0.0);
+}
+
+void PiecewisePolynomial::setValues(const std::vector<double>& v) {   // :149-173, shared element ends
+  values_ = v;
+  size_t g = 0;
+  for (auto& e : elements_) {
+    for (auto& x : e.value) x = values_[g++];
+    --g;
+  }
+}
+
+// :176-200 and the element's Lagrange sum (:210-229); outside the domain 0
+double PiecewisePolynomial::value(double x) const {
+  if (x > vertices_.front() || x < vertices_.back()) return 0.0;
+  for (size_t i = 0; i < vertices_.size(); ++i) if (vertices_[i] == x) return values_[i];
+  size_t k = 0;
+  for (size_t i = 0; i < elements_.size(); ++i)
+    if (elements_[i].vertex.front() > x && elements_[i].vertex.back() < x) { k = i; break; }
+  const Element& e = elements_[k];
+  const double r = 2.0 * (x - e.vertex.front()) / (e.vertex.back() - e.vertex.front()) - 1.0;
+  double y = 0;
+  for (size_t i = 0; i < e.ref.size(); ++i) {
+    double l = 1;
+    for (size_t j = 0; j < e.ref.size(); ++j)
+      if (i != j) l *= (r - e.ref[j]) / (e.ref[i] - e.ref[j]);
+    y += e.value[i] * l;
+  }
+  return y;
+}
+
+// ---------------------------------------------------------------- atmosphere
+Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& temperature_desc,
+                       const std::string& chemistry_desc)
+    : nb_levels(L), n_ids_(size_t(nbSpeciesIds())) {
+  if (L < 2) raise(kInvalidInput, "the atmosphere needs at least two levels");
+  // log-equidistant pressures from the bottom (atmosphere.cpp:365-383)
+  pressure.assign(L, 0.0);
+  pressure.front() = std::log10(bottom);
+  const double step = (std::log10(bottom) - std::log10(top)) / (L - 1.0);
+  for (size_t i = 1; i < L - 1; ++i) pressure[i] = pressure[i - 1] - step;
+  for (size_t i = 0; i < L - 1; ++i) pressure[i] = std::pow(10.0, pressure[i]);
+  pressure.back() = top;
+
+  std::istringstream t(temperature_desc);
+  std::string kind;
+  t >> kind;
+  if (kind == "poly") {                              // temperature/select_temperature_profile.h:86-104
+    size_t nb_el = 0, order = 0;
+    if (!(t >> nb_el >> order)) raise(kInvalidInput, "the poly temperature profile needs two parameters");
+    poly_.reset(new PiecewisePolynomial(nb_el, order, bottom, top));
+    nb_temperature_param_ = nb_el * order + 1;
+    if (poly_->nbDof() != nb_temperature_param_)
+      raise(kInvalidInput, "inconsistent piecewise-polynomial degrees of freedom");
+  } else if (kind == "const") {
+    isothermal_ = true;
+    nb_temperature_param_ = 1;
+  } else {
+    raise(kInvalidInput, "temperature profile " + kind + " is not available");
+  }
+
+  std::istringstream c(chemistry_desc);
+  std::string line;
+  while (std::getline(c, line)) {
+    std::istringstream in(line);
+    std::string model, sp;
+    if (!(in >> model)) continue;
+    ChemistryModule m;
+    if (model == "iso" || model == "isoprofile") {   // isoprofile_chemistry.cpp:33-80
+      while (in >> sp) m.species.push_back(speciesId(sp));
+      for (int s : m.species) if (s == sid::Na) m.sodium_free = true;
+      nb_chemistry_param_ += m.species.size();
+    } else if (model == "bg" || model == "background") {
+      in >> sp;
+      if (sp != "H2He") raise(kInvalidInput, "background chemistry " + sp + " is not available");
+      m.background = true;
+    } else {
+      raise(kInvalidInput, "chemistry model " + model + " is not available");
+    }
+    chemistry_.push_back(m);
+  }
+  temperature.assign(L, 0.0);
+  altitude.assign(L, 0.0);
+  mean_molecular_weight.assign(L, 0.0);
+  number_density.assign(L * n_ids_, 0.0);
+}
+
+bool Atmosphere::compute(double g, const double* tp, const double* cp) {
+  const size_t L = nb_levels;
+  const int S = int(n_ids_);
+  bool neglect = false;
+  // temperature (piecewise_poly_temperature.cpp:49-84; constant_temperature.h:34-46)
+  if (isothermal_) {
+    temperature.assign(L, tp[0]);
+  } else {
+    std::vector<double> dof(nb_temperature_param_);
+    dof[0] = tp[0];
+    for (size_t i = 1; i < dof.size(); ++i) dof[i] = dof[i - 1] * tp[i];
+    poly_->setValues(dof);
+    for (size_t i = 0; i < L; ++i) temperature[i] = poly_->value(std::log10(pressure[i]));
+  }
+  for (auto& t : temperature) if (t < 50) { t = 50; neglect = true; }
+
+  // chemistry modules in order (isoprofile_chemistry.cpp:87-119,
+  // background_chemistry.cpp:72-137, chemistry.h:47-76)
+  std::fill(number_density.begin(), number_density.end(), 0.0);
+  auto n = [&](size_t l, int id) -> double& { return number_density[l * n_ids_ + id]; };
+  auto meanWeight = [&]() {
+    for (size_t l = 0; l < L; ++l) {
+      double mu = 0;
+      for (int j = 0; j < S; ++j) mu += n(l, j) / n(l, sid::total) * molecularWeight(j);
+      mean_molecular_weight[l] = mu;
+    }
+  };
+  size_t k = 0;
+  for (auto& m : chemistry_) {
+    if (!m.background) {
+      for (size_t l = 0; l < L; ++l) {
+        n(l, sid::total) = pressure[l] * 1.e6 / phys::boltzmann_k / temperature[l];
+        for (size_t j = 0; j < m.species.size(); ++j) n(l, m.species[j]) = n(l, sid::total) * cp[k + j];
+        if (!m.sodium_free) n(l, sid::Na) = n(l, sid::K) * 16.2181;
+      }
+      k += m.species.size();
+      meanWeight();
+      for (size_t l = 0; l < L; ++l) {
+        double sum = 0;
+        for (int j = 1; j < S; ++j) sum += n(l, j) / n(l, sid::total);
+        if (sum > 1 + 1e-10) { neglect = true; break; }
+      }
+    } else {
+      const double h2 = 0.5, he = 0.085114, h2he = h2 + he;
+      for (size_t l = 0; l < L; ++l) n(l, sid::total) = pressure[l] * 1.e6 / phys::boltzmann_k / temperature[l];
+      for (size_t l = 0; l < L; ++l) {
+        double bg = 1.0;
+        for (int j = 1; j < S; ++j)
+          if (j != sid::H2 && j != sid::He) bg -= n(l, j) / n(l, sid::total);
+        if (bg < 0) { neglect = true; bg = 0; }
+        n(l, sid::H2) = n(l, sid::total) * (bg * h2 / h2he);
+        n(l, sid::He) = n(l, sid::total) * (bg * he / h2he);
+      }
+      meanWeight();
+    }
+  }
+  // hydrostatic altitude, trapezoid in 1/(rho g) (atmosphere.cpp:288-307)
+  std::vector<double> rho(L);
+  for (size_t l = 0; l < L; ++l)
+    rho[l] = mean_molecular_weight[l] * pressure[l] * 1e6 / (phys::gas_constant * temperature[l]);
+  altitude[0] = 0.0;
+  for (size_t l = 1; l < L; ++l)
+    altitude[l] = altitude[l - 1] +
+                  (1.0 / (rho[l] * g) + 1.0 / (rho[l - 1] * g)) * 0.5 * (pressure[l - 1] * 1e6 - pressure[l] * 1e6);
+  return neglect;
+}
+
+}  // namespace bear_amd

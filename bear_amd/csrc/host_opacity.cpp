@@ -1,0 +1,214 @@
+// Opacity species of the host side: table ingest into the device-ordered
+// row store, (p, T) bracketing, and Rayleigh cross-sections.
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "host.h"
+
+namespace bear_amd {
+
+// ---------------------------------------------------------------- in-memory tables
+namespace {
+struct RegisteredTable {
+  double mass = 0;
+  bool log_storage = false;
+  std::vector<double> temperature, pressure;
+  size_t nb_wavenumbers = 0;
+  std::vector<float> rows;
+};
+std::mutex g_registry_mutex;
+std::map<std::string, RegisteredTable>& registry() {
+  static std::map<std::string, RegisteredTable> r;
+  return r;
+}
+std::string normalise(std::string folder) {
+  while (!folder.empty() && folder.back() == '/') folder.pop_back();
+  return folder;
+}
+}  // namespace
+
+void registerTable(const std::string& folder, double mass, bool log_storage, size_t nb_rows,
+                   const double* temperatures, const double* pressures, size_t nb_wavenumbers, const float* rows) {
+  RegisteredTable t;
+  t.mass = mass;
+  t.log_storage = log_storage;
+  t.temperature.assign(temperatures, temperatures + nb_rows);
+  t.pressure.assign(pressures, pressures + nb_rows);
+  t.nb_wavenumbers = nb_wavenumbers;
+  t.rows.assign(rows, rows + nb_rows * nb_wavenumbers);
+  std::lock_guard<std::mutex> lock(g_registry_mutex);
+  registry()[normalise(folder)] = std::move(t);
+}
+
+void clearRegisteredTables() {
+  std::lock_guard<std::mutex> lock(g_registry_mutex);
+  registry().clear();
+}
+
+// ---------------------------------------------------------------- species
+// TransportCoefficients::addOpacitySpecies (transport_coeff.cpp:98-237):
+// CIA pairs, H2 Rayleigh and generic line absorbers
+OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& folder, const SpectralGrid& grid) {
+  OpacitySpecies s;
+  s.symbol = symbol;
+  s.folder = folder;
+  if (symbol == "CIA-H2-H2") { s.species_id = sid::H2; s.cia_partners = {sid::H2}; }
+  else if (symbol == "CIA-H2-He") { s.species_id = sid::H2; s.cia_partners = {sid::He}; }
+  else if (symbol == "CIA-H-He") { s.species_id = sid::H; s.cia_partners = {sid::He}; }
+  else if (folder == "Rayleigh") {
+    if (symbol != "H2") raise(kInvalidInput, "Rayleigh scattering of " + symbol + " is not available");
+    s.species_id = sid::H2;
+    s.rayleigh = true;
+    // species_rayleigh_cross_sections.cpp:22-45 with opacity_species.cpp:528-537
+    s.rayleigh_cross_section.resize(grid.size());
+    for (size_t j = 0; j < grid.size(); ++j) {
+      const double lam = grid.wavelength[j], nu = grid.wavenumber[j];
+      const double n = 13.58e-5 * (1. + 7.52e-3 / (lam * lam)) + 1.;
+      const double n_ref = 2.651629e19;
+      const double f = (n * n - 1) / (n * n + 2);
+      s.rayleigh_cross_section[j] = 24. * phys::pi * phys::pi * phys::pi * nu * nu * nu * nu / (n_ref * n_ref) * f * f;
+    }
+  } else if (symbol == "H-") {
+    raise(kInvalidInput, "the H- continuum is not available in this engine");
+  } else {
+    s.species_id = speciesId(symbol);
+  }
+  return s;
+}
+
+// filelist.dat + float32 row files (opacity_species.cpp:66-157,
+// cross_section_file.cpp:33-62, sampled_data.cpp:48-87): every row is sampled
+// at the grid's index list, scaled by mass/N_A when a mass is given, floored
+// at 1e-200 and stored as log10 in FP64.
+void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const SpectralGrid& grid, TableStore& store) {
+  if (s.rayleigh) return;
+  const std::string dir = path + s.folder;
+  struct Entry { double t, p; std::string file; };
+  std::vector<Entry> entries;
+  double mass = 0;
+  bool log_storage = false;
+  const RegisteredTable* reg = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_registry_mutex);
+    auto it = registry().find(normalise(s.folder));
+    if (it == registry().end()) it = registry().find(normalise(dir));
+    if (it != registry().end()) reg = &it->second;
+  }
+  if (reg) {
+    mass = reg->mass;
+    log_storage = reg->log_storage;
+    for (size_t r = 0; r < reg->temperature.size(); ++r) entries.push_back({reg->temperature[r], reg->pressure[r], ""});
+  } else {
+    std::ifstream f(dir + "/filelist.dat");
+    if (!f) raise(kFileNotFound, "no cross-section data for " + s.symbol + " (" + dir + "/filelist.dat)");
+    std::string line, storage;
+    std::getline(f, line);
+    std::istringstream head(line);
+    head >> mass >> storage;
+    log_storage = storage == "log";
+    while (std::getline(f, line)) {
+      std::istringstream in(line);
+      Entry e{0, 0, ""};
+      in >> e.p >> e.t >> e.file;
+      if (!e.file.empty()) entries.push_back(e);
+    }
+  }
+  if (entries.empty()) raise(kInvalidInput, "empty cross-section table for " + s.symbol);
+
+  const size_t N = grid.size();
+  const uint32_t first_row = store.nb_rows;
+  store.log_cross_sections.resize((size_t(first_row) + entries.size()) * N);
+  std::vector<float> raw;
+  for (size_t r = 0; r < entries.size(); ++r) {
+    const float* src;
+    size_t n_full;
+    if (reg) {
+      src = reg->rows.data() + r * reg->nb_wavenumbers;
+      n_full = reg->nb_wavenumbers;
+    } else {
+      std::ifstream bf(dir + "/" + entries[r].file, std::ios::binary | std::ios::ate);
+      if (!bf) raise(kFileNotFound, "cross-section file not found: " + dir + "/" + entries[r].file);
+      n_full = size_t(bf.tellg()) / sizeof(float);
+      raw.resize(n_full);
+      bf.seekg(0);
+      bf.read(reinterpret_cast<char*>(raw.data()), std::streamsize(n_full * sizeof(float)));
+      src = raw.data();
+    }
+    double* dst = store.log_cross_sections.data() + (size_t(first_row) + r) * N;
+    const double factor = mass / phys::avogadro;
+    for (size_t i = 0; i < N; ++i) {
+      double v = 0.0;
+      if (grid.index[i] < n_full) {
+        v = src[grid.index[i]];
+        if (log_storage) v = std::pow(10.0, v);
+      }
+      if (mass > 0) v *= factor;
+      if (v < 1e-200) v = 1e-200;
+      dst[i] = std::log10(v);
+    }
+  }
+  // order rows by temperature, then pressure (opacity_species.cpp:129-157)
+  std::map<double, std::vector<TableRow>> by_t;
+  for (size_t r = 0; r < entries.size(); ++r)
+    by_t[entries[r].t].push_back(TableRow{entries[r].t, entries[r].p, std::log10(entries[r].p), uint32_t(first_row + r)});
+  s.by_temperature.clear();
+  for (auto& kv : by_t) {
+    auto rows = kv.second;
+    std::sort(rows.begin(), rows.end(), [](const TableRow& a, const TableRow& b) { return a.pressure < b.pressure; });
+    s.by_temperature.push_back(rows);
+  }
+  store.nb_rows = uint32_t(first_row + entries.size());
+  store.nb_points = N;
+}
+
+// OpacitySpecies::findClosestDataPoints (opacity_species.cpp:160-273) and
+// the weights of calcAbsorptionCrossSections (:277-345).  A weight is zero
+// where the reference skips that interpolation (equal rows), so the device
+// formula  lo + (hi - lo) f_t  with  lo = r0 + (r1 - r0) f_p_lower  and
+// hi = r2 + (r3 - r2) f_p_upper  reproduces every branch.
+Stencil OpacitySpecies::bracket(double p, double t) const {
+  const std::vector<TableRow>* lo = &by_temperature.front();
+  const std::vector<TableRow>* hi = lo;
+  if (t < by_temperature.front().front().temperature) {
+  } else if (t > by_temperature.back().front().temperature) {
+    lo = hi = &by_temperature.back();
+  } else {
+    for (size_t i = 0; i < by_temperature.size(); ++i) {
+      const double ti = by_temperature[i].front().temperature;
+      if (ti == t) { lo = hi = &by_temperature[i]; break; }
+      if (ti < t && by_temperature[i + 1].front().temperature > t) {
+        lo = &by_temperature[i];
+        hi = &by_temperature[i + 1];
+        break;
+      }
+    }
+  }
+  auto pick = [p](const std::vector<TableRow>& rows, const TableRow*& a, const TableRow*& b) {
+    if (p < rows.front().pressure) { a = b = &rows.front(); return; }
+    if (p > rows.back().pressure) { a = b = &rows.back(); return; }
+    for (size_t i = 0; i < rows.size(); ++i) {
+      if (rows[i].pressure == p) { a = b = &rows[i]; return; }
+      if (rows[i].pressure < p && rows[i + 1].pressure > p) { a = &rows[i]; b = &rows[i + 1]; return; }
+    }
+    a = b = &rows.front();
+  };
+  const TableRow *r0, *r1, *r2, *r3;
+  pick(*lo, r0, r1);
+  pick(*hi, r2, r3);
+  Stencil s;
+  s.row[0] = r0->id;
+  s.row[1] = r1->id;
+  s.row[2] = r2->id;
+  s.row[3] = r3->id;
+  const double lp = std::log10(p);
+  s.f_p_lower = r0 != r1 ? (lp - r0->log_pressure) / (r1->log_pressure - r0->log_pressure) : 0.0;
+  s.f_p_upper = r2 != r3 ? (lp - r2->log_pressure) / (r3->log_pressure - r2->log_pressure) : 0.0;
+  s.f_t = r0 != r2 ? (t - r0->temperature) / (r2->temperature - r0->temperature) : 0.0;
+  return s;
+}
+
+}  // namespace bear_amd

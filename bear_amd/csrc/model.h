@@ -1,0 +1,87 @@
+// Forward models and the retrieval's likelihood path, mirroring the
+// reference's ForwardModel interface (forward_model/forward_model.h:61-144)
+// on top of the device engine.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "host.h"
+
+namespace bear_amd {
+
+struct GlobalSettings {                 // GlobalConfig (config/global_config.h:28-71)
+  int device = 0;
+  std::string retrieval_folder, cross_section_path, wavenumber_file;
+  int discretisation = 0;
+  double resolution = 0;
+  bool error_inflation = false;
+};
+
+struct ModelSettings {                  // EmissionModelConfig (emission.h:51-102)
+  size_t nb_levels = 0;
+  double bottom_pressure = 0, top_pressure = 0;
+  std::string temperature, radiative_transfer, chemistry;
+  std::vector<std::pair<std::string, std::string>> opacity;   // (symbol, folder)
+};
+
+class ForwardModel {
+ public:
+  virtual ~ForwardModel() = default;
+  // parameters of the model itself (ForwardModel::parametersNumber)
+  virtual size_t parametersNumber() const = 0;
+  // ForwardModel::calcModelGPU (forward_model.h:76-79) with the retrieval's
+  // error inflation forwarded to the likelihood terms; returns "neglect"
+  virtual bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
+                            double* const* spectrum_obs_dev) = 0;
+};
+
+// EmissionModel (forward_model/emission/emission.h:127-250)
+class EmissionModel : public ForwardModel {
+ public:
+  EmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
+                const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
+  size_t parametersNumber() const override { return nb_general_ + nb_chem_ + nb_temp_ + nb_modifier_; }
+  bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
+                    double* const* spectrum_obs_dev) override;
+  const Atmosphere& atmosphere() const { return atmosphere_; }
+
+ private:
+  const std::vector<ObservationSetup>& observations_;
+  DeviceEngine& engine_;
+  Atmosphere atmosphere_;
+  std::vector<OpacitySpecies> species_;     // absorbers with tables (Rayleigh is not used by emission)
+  size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
+};
+
+// Retrieval (retrieval/retrieval.h) restricted to the likelihood path
+class Retrieval {
+ public:
+  Retrieval(const GlobalSettings& global, const std::string& model_type, const ModelSettings& model,
+            const std::vector<ObservationInputView>& inputs, const std::vector<std::string>& files,
+            const std::vector<std::string>& modifiers);
+  size_t nbParameters() const { return model_->parametersNumber() + (global_.error_inflation ? 1 : 0); }
+  const SpectralGrid& grid() const { return grid_; }
+  const std::vector<ObservationSetup>& observations() const { return observations_; }
+  double computeLikelihood(const double* parameters, size_t n);
+  // ForwardModel::calcModel (forward_model.cpp:60-127) with host outputs
+  bool computeModel(const double* parameters, size_t n, bool high_res, double* spectrum, double* spectrum_obs);
+  bool calcModelGPU(const double* parameters, size_t n, double* spectrum_dev, double* const* spectrum_obs_dev);
+
+ private:
+  double errorInflation(const double* parameters, size_t n) const;
+  void checkSize(size_t n) const;
+  GlobalSettings global_;
+  SpectralGrid grid_;
+  std::vector<ObservationSetup> observations_;
+  std::unique_ptr<DeviceEngine> engine_;
+  std::unique_ptr<ForwardModel> model_;
+};
+
+void registerTable(const std::string& folder, double mass, bool log_storage, size_t nb_rows,
+                   const double* temperatures, const double* pressures, size_t nb_wavenumbers, const float* rows);
+void clearRegisteredTables();
+
+}  // namespace bear_amd

@@ -1,0 +1,568 @@
+"""Seeded synthetic workloads for the BeAR forward-model engine.
+
+Everything here is test/benchmark input generation: opacity tables built from
+seeded Lorentzian line lists, smooth collision-induced-absorption (CIA) tables,
+observations (spectroscopy with a Gaussian line-spread function, band
+spectroscopy, photometry with bandpass files) and parameter vectors.
+
+A case can be written to disk in the reference's own file formats, so that
+the same bytes drive the product, the CPU oracle and the reference build:
+
+* ``wavenumber_full.dat``  count, then one wavenumber per line
+  (read by ``spectral_grid.cpp:87-119``);
+* ``<species folder>/filelist.dat``  first line ``mass [log]``, then one
+  ``pressure temperature file`` line per table row (``opacity_species.cpp:66-122``);
+* raw float32 row files over the full wavenumber grid, linear or log10
+  (``cross_section_file.cpp:33-62``, ``sampled_data.cpp:48-88``);
+* observation files with the ``#type`` / ``#filter transmission function``
+  layout (``observations_load_data.cpp:40-330``) and bandpass files with the
+  detector type on line 4 (``observations_filter_response.cpp:42-117``);
+* ``observations.list`` (``retrieval_load_observations.cpp:103-141``).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# cgs constants (same values as the reference's physical_const.h)
+PLANCK_H = 6.62606896e-27
+LIGHT_C = 2.99792458e10
+BOLTZMANN_K = 1.3806504e-16
+RADIUS_JUPITER = 7.1492e9
+RADIUS_SUN = 6.96342e10
+PARSEC = 3.08567758135e18
+AVOGADRO = 6.022140857e23
+
+MOLECULAR_WEIGHT = {
+    "H2O": 18.01528, "CO2": 44.01, "CO": 28.0101, "CH4": 16.04246,
+    "HCN": 27.0253, "NH3": 17.03052, "C2H2": 26.04, "H2S": 34.09099,
+    "K": 39.0983, "Na": 22.98977, "TiO": 63.8664, "VO": 66.9409,
+    "FeH": 56.853, "SH": 34.08, "OH": 17.008, "SO2": 64.066,
+}
+
+
+@dataclass
+class SpeciesTable:
+    """One opacity species: a (T, p) grid of rows over the full wavenumber grid.
+
+    ``values`` holds exactly what the row files contain (float32): linear
+    cross-sections in cm^2 (``mass == 0``), opacities in cm^2/g (``mass > 0``)
+    or log10 of either when ``log_storage`` is set.
+    """
+    symbol: str
+    folder: str
+    temperatures: np.ndarray          # ascending, K
+    pressures: np.ndarray             # ascending, bar
+    values: np.ndarray                # float32 [nT, nP, N_full]
+    mass: float = 0.0
+    log_storage: bool = False
+    kind: str = "line"                # "line" | "cia"
+
+
+@dataclass
+class ObservationSpec:
+    name: str
+    type: str                         # spectroscopy | band-spectroscopy | photometry
+    wavelengths: np.ndarray | None = None     # spectroscopy: centres (micron)
+    bin_edges: np.ndarray | None = None       # band types: [nb, 2] (micron)
+    data: np.ndarray | None = None
+    error: np.ndarray | None = None
+    fwhm: np.ndarray | None = None            # optional instrument profile FWHM (micron)
+    weight: np.ndarray | None = None          # optional likelihood weights
+    filter: tuple | None = None               # (detector "Photon"|"Energy", wavelength[], response[])
+    modifier: str = "none"
+
+
+@dataclass
+class Case:
+    name: str
+    wavenumber_full: np.ndarray
+    spectral_resolution: float
+    nb_levels: int
+    p_bottom: float
+    p_top: float
+    temperature_model: tuple           # ("poly", nb_elements, degree) | ("const",)
+    chemistry: list                    # [("iso", [species...]), ("bg", ["H2He"])]
+    species: list                      # [SpeciesTable]
+    rayleigh: list                     # ["H2", ...]  (opacity entries with folder "Rayleigh")
+    observations: list                 # [ObservationSpec]
+    parameters: np.ndarray             # physical parameter vector
+    use_error_inflation: bool = False
+    model: str = "emission"
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def opacity_entries(self):
+        """(symbol, folder) pairs in the order a forward_model.config lists them."""
+        out = []
+        for s in self.species:
+            out.append((s.symbol, s.folder))
+        for r in self.rayleigh:
+            out.append((r, "Rayleigh"))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# tables
+# ---------------------------------------------------------------------------
+
+def _line_list(rng, n_lines, nu_lo, nu_hi):
+    centres = rng.uniform(nu_lo, nu_hi, n_lines)
+    strength = 10.0 ** rng.uniform(-24.0, -18.0, n_lines)      # cm / molecule
+    lower_energy = rng.uniform(0.0, 4000.0, n_lines)            # cm^-1
+    return centres, strength, lower_energy
+
+
+def _lorentz_rows(nu, centres, weights_per_row, widths, backend="numpy"):
+    """Sum of Lorentzians of one width per row, by FFT on a padded grid.
+
+    ``weights_per_row`` is [R, n_lines] (line strengths at each row's
+    temperature), ``widths`` is [R] (HWHM in cm^-1).  Line centres snap to
+    the nearest grid point; the profile is the exact Lorentzian transform
+    exp(-2 pi gamma |k|), so rows are smooth and strictly positive.
+    """
+    n = nu.size
+    step = (nu[-1] - nu[0]) / (n - 1)
+    m = 1 << int(math.ceil(math.log2(2 * n)))
+    idx = np.clip(np.rint((centres - nu[0]) / step).astype(np.int64), 0, n - 1)
+    freqs = np.fft.rfftfreq(m, d=step)
+    out = np.empty((weights_per_row.shape[0], n), dtype=np.float64)
+    if backend == "torch":
+        import torch
+        dev = torch.device("cuda")
+        idx_t = torch.as_tensor(idx, device=dev)
+        f_t = torch.as_tensor(freqs, device=dev)
+        for r in range(weights_per_row.shape[0]):
+            hist = torch.zeros(m, dtype=torch.float64, device=dev)
+            hist.index_add_(0, idx_t, torch.as_tensor(weights_per_row[r], device=dev))
+            spec = torch.fft.rfft(hist)
+            spec *= torch.exp(-2.0 * math.pi * widths[r] * f_t)
+            out[r] = (torch.fft.irfft(spec, n=m)[:n] / step).cpu().numpy()
+        return out
+    for r in range(weights_per_row.shape[0]):
+        hist = np.bincount(idx, weights=weights_per_row[r], minlength=m)
+        spec = np.fft.rfft(hist)
+        spec *= np.exp(-2.0 * math.pi * widths[r] * freqs)
+        out[r] = np.fft.irfft(spec, n=m)[:n] / step
+    return out
+
+
+def line_table(rng, symbol, folder, nu, temperatures, pressures, n_lines=3000,
+               mass=0.0, log_storage=False, backend="numpy"):
+    """log10 sigma = log10(1e-30 + sum_k S_k(T) L(nu; nu_k, gamma(p, T)))."""
+    centres, s0, e_low = _line_list(rng, n_lines, nu[0], nu[-1])
+    c2 = PLANCK_H * LIGHT_C / BOLTZMANN_K
+    t_grid, p_grid = np.meshgrid(temperatures, pressures, indexing="ij")
+    t_rows, p_rows = t_grid.ravel(), p_grid.ravel()
+    # line strengths at each row temperature (Boltzmann factor around 1000 K)
+    w = s0[None, :] * (1000.0 / t_rows[:, None]) ** 1.5 \
+        * np.exp(-c2 * e_low[None, :] * (1.0 / t_rows[:, None] - 1.0 / 1000.0))
+    gamma = 0.01 * np.sqrt(t_rows / 1000.0) + 0.07 * p_rows * (296.0 / t_rows) ** 0.7
+    sigma = 1e-30 + np.maximum(_lorentz_rows(nu, centres, w, gamma, backend), 0.0)
+    sigma = sigma.reshape(temperatures.size, pressures.size, nu.size)
+    if mass > 0:
+        sigma = sigma / (mass / AVOGADRO)            # store as cm^2/g
+    if log_storage:
+        sigma = np.log10(sigma)
+    return SpeciesTable(symbol, folder, np.asarray(temperatures, float),
+                        np.asarray(pressures, float), sigma.astype(np.float32),
+                        mass=mass, log_storage=log_storage, kind="line")
+
+
+def cia_table(rng, symbol, folder, nu, temperatures):
+    """Smooth, T-dependent CIA cross-section (cm^5), one pressure row per T."""
+    a = rng.uniform(0.5, 1.5)
+    phase = rng.uniform(0, 2 * math.pi)
+    centre = rng.uniform(nu[0], nu[-1])
+    x = (nu - centre) / (nu[-1] - nu[0])
+    rows = []
+    for t in temperatures:
+        lg = (-45.5 + 0.6 * a * np.sin(2 * math.pi * nu / 3000.0 + phase)
+              - 3.0 * x * x + 0.9 * np.log10(t / 1000.0) - 0.2 * x * np.log10(t / 1000.0))
+        rows.append(10.0 ** lg)
+    values = np.asarray(rows)[:, None, :].astype(np.float32)
+    return SpeciesTable(symbol, folder, np.asarray(temperatures, float),
+                        np.array([1.0]), values, mass=0.0, log_storage=False, kind="cia")
+
+
+# ---------------------------------------------------------------------------
+# observations
+# ---------------------------------------------------------------------------
+
+def blackbody_flux_lambda(wavelength_um, t_eff, scale):
+    """pi B_lambda(T) * scale in W m^-2 micron^-1 (synthetic 'observed' data)."""
+    nu = 1e4 / wavelength_um
+    b_nu = 2 * PLANCK_H * LIGHT_C ** 2 * nu ** 3 / np.expm1(PLANCK_H * LIGHT_C * nu / BOLTZMANN_K / t_eff)
+    # erg/s/cm^2/sr/cm^-1 -> W m^-2 cm (x1e-3), then per micron: / lambda^2 * 1e4
+    return math.pi * b_nu * 1e-3 * scale / wavelength_um ** 2 * 1e4
+
+
+def noisy(rng, model, frac=0.02):
+    data = model * (1.0 + frac * rng.standard_normal(model.shape))
+    return data, frac * np.abs(data)
+
+
+def bandpass(rng, lo, hi, n=50):
+    """Seeded smooth bandpass over [lo, hi] micron, ascending wavelengths."""
+    w = np.linspace(lo, hi, n)
+    x = (w - lo) / (hi - lo)
+    resp = np.clip(np.sin(math.pi * x) ** rng.uniform(0.5, 2.0)
+                   * (1.0 + 0.2 * np.sin(2 * math.pi * rng.uniform(1, 3) * x + rng.uniform(0, 6))), 0, None)
+    resp[0] = 0.0
+    resp[-1] = 0.0
+    return w, resp
+
+
+# ---------------------------------------------------------------------------
+# parameters
+# ---------------------------------------------------------------------------
+
+def poly_temperature_params(rng, nb_dof, t_lo=500.0, t_hi=2500.0):
+    """Control temperatures drawn in [t_lo, t_hi] and turned into the DOF
+    ratios of the piecewise-polynomial profile (T0 = theta0, Ti = Ti-1 thetai)."""
+    temps = np.sort(rng.uniform(t_lo, t_hi, nb_dof))[::-1]     # hot at the bottom
+    theta = np.empty(nb_dof)
+    theta[0] = temps[0]
+    theta[1:] = temps[1:] / temps[:-1]
+    return theta
+
+
+def emission_parameters(rng, nb_absorbers, nb_dof):
+    log_g = rng.uniform(4.0, 5.5)
+    scale_f = 1.0
+    distance = 10.0 * PARSEC
+    mixing = 10.0 ** rng.uniform(-8.0, -3.0, nb_absorbers)
+    theta = poly_temperature_params(rng, nb_dof)
+    return np.concatenate([[log_g, scale_f, distance], mixing, theta])
+
+
+# ---------------------------------------------------------------------------
+# the configured cases
+# ---------------------------------------------------------------------------
+
+ABSORBERS = ["H2O", "CH4", "CO", "CO2", "NH3", "H2S", "HCN", "K", "TiO", "VO", "FeH", "SO2"]
+
+
+def _grid(nu_lo, nu_hi, step):
+    n = int(round((nu_hi - nu_lo) / step)) + 1
+    return nu_lo + step * np.arange(n, dtype=np.float64)
+
+
+def tiny_case(seed=7):
+    """Small emission case exercising every observation path in seconds:
+    spectroscopy with an LSF (ascending input), band spectroscopy with a
+    shift modifier, photometry with photon and energy bandpasses."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(2000.0, 6000.0, 0.5)
+    temps = np.array([300.0, 700.0, 1100.0, 1500.0, 1900.0, 2300.0, 2700.0])
+    press = 10.0 ** np.arange(-5.0, 2.5, 1.5)
+    species = [
+        line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=400, mass=MOLECULAR_WEIGHT["H2O"]),
+        line_table(rng, "CH4", "CH4", nu, temps, press, n_lines=300, log_storage=True),
+        cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+        cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps),
+    ]
+    scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
+    obs = []
+    # spectroscopy, ascending wavelengths, with an instrument profile (R ~ 150)
+    wl = np.linspace(1.80, 4.60, 24)
+    model = blackbody_flux_lambda(wl, 1200.0, scale)
+    d, e = noisy(rng, model)
+    obs.append(ObservationSpec("spec_lsf", "spectroscopy", wavelengths=wl, data=d, error=e,
+                               fwhm=wl / 150.0, weight=np.where(np.arange(wl.size) % 5 == 0, 0.5, 1.0)))
+    # band spectroscopy, descending order, no profile, shift modifier
+    edges = np.linspace(4.9, 2.1, 9)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    model = blackbody_flux_lambda(be.mean(axis=1), 1200.0, scale)
+    d, e = noisy(rng, model)
+    obs.append(ObservationSpec("bands", "band-spectroscopy", bin_edges=be, data=d, error=e,
+                               modifier="shift_const"))
+    # photometry with photon-counting and energy bandpasses
+    for i, (lo, hi, det) in enumerate([(2.2, 2.6, "Photon"), (3.3, 3.9, "Energy")]):
+        w, r = bandpass(rng, lo, hi, 30)
+        model = blackbody_flux_lambda(np.array([(lo + hi) / 2]), 1200.0, scale)
+        d, e = noisy(rng, model)
+        obs.append(ObservationSpec(f"phot{i}", "photometry", bin_edges=np.array([[lo, hi]]),
+                                   data=d, error=e, filter=(det, w, r)))
+    params = emission_parameters(rng, 2, 7)
+    params = np.concatenate([params, [0.0]])          # shift_const of the band observation
+    return Case("tiny", nu, 1.5, 14, 100.0, 1e-5, ("poly", 6, 1),
+                [("iso", ["H2O", "CH4"]), ("bg", ["H2He"])], species, ["H2"], obs, params)
+
+
+def emission_c1(seed=11):
+    """Plain CPU reference emission model: 60 levels, 1e2..1e-6 bar, poly 6 1,
+    2 absorbers on 30 T x 9 p tables over 1000-10000 cm^-1 (20,001 points),
+    one 100-point spectrum with a Gaussian LSF at R = 100, 2 % noise."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(1000.0, 10000.0, 0.45)
+    temps = np.arange(100.0, 3001.0, 100.0)
+    press = 10.0 ** np.arange(-6.0, 2.5, 1.0)
+    species = [
+        line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=3000),
+        line_table(rng, "CH4", "CH4", nu, temps, press, n_lines=3000, log_storage=True),
+    ]
+    r = 100.0
+    lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
+    # keep the 5-sigma LSF wings inside the grid
+    lo = lam_lo * (1 + 6.0 / (2.355 * r)) * 1.01
+    hi = lam_hi * (1 - 6.0 / (2.355 * r)) * 0.99
+    wl = np.linspace(lo, hi, 100)
+    scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
+    d, e = noisy(rng, blackbody_flux_lambda(wl, 1100.0, scale))
+    obs = [ObservationSpec("spectrum_R100", "spectroscopy", wavelengths=wl[::-1].copy(),
+                           data=d[::-1].copy(), error=e[::-1].copy(), fwhm=wl[::-1] / r)]
+    params = emission_parameters(rng, 2, 7)
+    return Case("emission_c1", nu, 0.45, 60, 100.0, 1e-6, ("poly", 6, 1),
+                [("iso", ["H2O", "CH4"]), ("bg", ["H2He"])], species, [], obs, params)
+
+
+def emission_c2(seed=23, backend="numpy", n_wavenumbers=400_000, nb_bins=300):
+    """Plain GPU drop-in emission model: 100 levels, 6 absorbers + H2-H2/H2-He
+    CIA + H2 Rayleigh, 400,000 wavenumbers at 0.025 cm^-1, 30 T x 9 p tables,
+    300 photometric bins each with a seeded 50-point bandpass."""
+    rng = np.random.default_rng(seed)
+    step = 0.025
+    nu = 1000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
+    temps = np.arange(100.0, 3001.0, 100.0)
+    press = 10.0 ** np.arange(-6.0, 2.5, 1.0)
+    names = ABSORBERS[:6]
+    species = [line_table(rng, s, s, nu, temps, press, n_lines=3000, backend=backend) for s in names]
+    species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+                cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
+    lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
+    edges = np.linspace(lam_lo * 1.0005, lam_hi * 0.9995, nb_bins + 1)
+    scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
+    obs = []
+    for b in range(nb_bins):
+        lo, hi = edges[b], edges[b + 1]
+        w, r = bandpass(rng, lo, hi, 50)
+        d, e = noisy(rng, blackbody_flux_lambda(np.array([(lo + hi) / 2]), 1100.0, scale))
+        obs.append(ObservationSpec(f"band{b:03d}", "photometry", bin_edges=np.array([[lo, hi]]),
+                                   data=d, error=e, filter=("Photon" if b % 2 else "Energy", w, r)))
+    params = emission_parameters(rng, 6, 7)
+    return Case("emission_c2", nu, step, 100, 100.0, 1e-6, ("poly", 6, 1),
+                [("iso", names), ("bg", ["H2He"])], species, ["H2"], obs, params)
+
+
+CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2}
+
+
+# ---------------------------------------------------------------------------
+# writing the reference's file formats
+# ---------------------------------------------------------------------------
+
+def _fmt(x):
+    return repr(float(x))
+
+
+def write_case(case: Case, folder: str) -> str:
+    """Write ``case`` as a retrieval folder in the reference's formats.
+
+    Returns the folder path (with a trailing slash).  Besides the reference
+    files, ``case.json`` records the configuration a driver needs
+    (forward-model settings, opacity entries, parameter vector)."""
+    folder = os.path.join(folder, "")
+    os.makedirs(folder, exist_ok=True)
+    opac = os.path.join(folder, "opacity", "")
+    with open(folder + "wavenumber_full.dat", "w") as f:
+        f.write(f"{case.wavenumber_full.size}\n")
+        f.write("\n".join(_fmt(v) for v in case.wavenumber_full))
+        f.write("\n")
+    for s in case.species:
+        d = os.path.join(opac, s.folder)
+        os.makedirs(d, exist_ok=True)
+        lines = [f"{_fmt(s.mass)} {'log' if s.log_storage else 'lin'}"]
+        for it, t in enumerate(s.temperatures):
+            for ip, p in enumerate(s.pressures):
+                fname = f"row_T{it:02d}_p{ip:02d}.bin"
+                s.values[it, ip].astype("<f4").tofile(os.path.join(d, fname))
+                lines.append(f"{_fmt(p)} {_fmt(t)} {fname}")
+        # rows listed in a shuffled-by-construction order (p-major) to exercise
+        # the table ordering of the reader
+        head, rows = lines[0], lines[1:]
+        rows = rows[1::2] + rows[0::2]
+        with open(os.path.join(d, "filelist.dat"), "w") as f:
+            f.write(head + "\n" + "\n".join(rows) + "\n")
+    names = []
+    for i, o in enumerate(case.observations):
+        fname = f"obs_{i:03d}.dat"
+        names.append((fname, o.modifier))
+        with open(folder + fname, "w") as f:
+            f.write("#observation description\n#name\n" + o.name + "\n\n#type\n" + o.type + "\n\n")
+            f.write("#filter transmission function\n")
+            if o.filter is not None:
+                fil = f"filter_{i:03d}.dat"
+                f.write(fil + "\n\n")
+                det, w, r = o.filter
+                with open(folder + fil, "w") as ff:
+                    ff.write("# synthetic bandpass\n\n#Detector Type\n" +
+                             ("Photon counter" if det == "Photon" else "Energy counter") +
+                             "\n\n# Wave (microns)     Filter Response\n")
+                    for a, b in zip(w, r):
+                        ff.write(f"{_fmt(a)}\t{_fmt(b)}\n")
+            else:
+                f.write("None\n\n")
+            if o.type == "spectroscopy":
+                f.write("#wl  flux  error  fwhm  weight\n")
+                for j in range(o.wavelengths.size):
+                    cols = [o.wavelengths[j], o.data[j], o.error[j]]
+                    if o.fwhm is not None or o.weight is not None:
+                        cols.append(o.fwhm[j] if o.fwhm is not None else 0.0)
+                    if o.weight is not None:
+                        cols.append(o.weight[j])
+                    f.write("  ".join(_fmt(c) for c in cols) + "\n")
+            elif o.type == "band-spectroscopy":
+                f.write("#left  right  value  error  fwhm  weight\n")
+                for j in range(o.bin_edges.shape[0]):
+                    cols = [o.bin_edges[j, 0], o.bin_edges[j, 1], o.data[j], o.error[j]]
+                    if o.fwhm is not None or o.weight is not None:
+                        cols.append(o.fwhm[j] if o.fwhm is not None else 0.0)
+                    if o.weight is not None:
+                        cols.append(o.weight[j])
+                    f.write("  ".join(_fmt(c) for c in cols) + "\n")
+            else:
+                f.write("#lower  upper  value  error\n")
+                cols = [o.bin_edges[0, 0], o.bin_edges[0, 1], o.data[0], o.error[0]]
+                if o.weight is not None:
+                    cols.append(o.weight[0])
+                f.write("  ".join(_fmt(c) for c in cols) + "\n")
+    with open(folder + "observations.list", "w") as f:
+        for fname, mod in names:
+            f.write(f"{fname} {mod}\n")
+    meta = {
+        "name": case.name,
+        "model": case.model,
+        "spectral_resolution": case.spectral_resolution,
+        "nb_levels": case.nb_levels,
+        "p_bottom": case.p_bottom,
+        "p_top": case.p_top,
+        "temperature_model": list(case.temperature_model),
+        "chemistry": [[c[0], list(c[1])] for c in case.chemistry],
+        "opacity": [[s, f] for s, f in case.opacity_entries],
+        "observations": [n for n, _ in names],
+        "modifiers": [m for _, m in names],
+        "parameters": [float(v) for v in case.parameters],
+        "use_error_inflation": case.use_error_inflation,
+        "extra": case.extra,
+    }
+    with open(folder + "case.json", "w") as f:
+        json.dump(meta, f, indent=1)
+    # the same settings as plain "key values..." lines for C/C++ drivers
+    with open(folder + "case.txt", "w") as f:
+        f.write(f"model {case.model}\n")
+        f.write(f"spectral_resolution {_fmt(case.spectral_resolution)}\n")
+        f.write(f"nb_levels {case.nb_levels}\n")
+        f.write(f"p_bottom {_fmt(case.p_bottom)}\n")
+        f.write(f"p_top {_fmt(case.p_top)}\n")
+        f.write("temperature " + " ".join(str(v) for v in case.temperature_model) + "\n")
+        for kind, sp in case.chemistry:
+            f.write(f"chemistry {kind} " + " ".join(sp) + "\n")
+        for s, fo in case.opacity_entries:
+            f.write(f"opacity {s} {fo}\n")
+        f.write(f"error_inflation {int(case.use_error_inflation)}\n")
+        f.write(f"parameters {case.parameters.size} " + " ".join(_fmt(v) for v in case.parameters) + "\n")
+    return folder
+
+
+# ---------------------------------------------------------------------------
+# engine construction from a case
+# ---------------------------------------------------------------------------
+
+def _model_config(meta):
+    from . import api
+    chem = [" ".join([c[0]] + list(c[1])) for c in meta["chemistry"]]
+    return api.ModelConfig(meta["nb_levels"], meta["p_bottom"], meta["p_top"],
+                           " ".join(str(v) for v in meta["temperature_model"]), chem,
+                           [tuple(o) for o in meta["opacity"]])
+
+
+def open_retrieval(folder, device=0):
+    """Engine Retrieval over a case folder written by write_case (files only)."""
+    from . import api
+    folder = os.path.join(folder, "")
+    meta = json.load(open(folder + "case.json"))
+    cfg = api.GlobalConfig(retrieval_folder_path=folder, cross_section_file_path=folder + "opacity/",
+                           wavenumber_file_path=folder + "wavenumber_full.dat",
+                           spectral_resolution=meta["spectral_resolution"],
+                           use_error_inflation=meta["use_error_inflation"], device=device)
+    return api.Retrieval(cfg, _model_config(meta), observation_files=meta["observations"],
+                         observation_modifiers=meta["modifiers"])
+
+
+def case_meta(case: Case):
+    return {
+        "spectral_resolution": case.spectral_resolution, "nb_levels": case.nb_levels,
+        "p_bottom": case.p_bottom, "p_top": case.p_top,
+        "temperature_model": list(case.temperature_model),
+        "chemistry": [[c[0], list(c[1])] for c in case.chemistry],
+        "opacity": [[s, f] for s, f in case.opacity_entries],
+        "use_error_inflation": case.use_error_inflation,
+    }
+
+
+def observation_inputs(case: Case):
+    from . import api
+    out = []
+    for o in case.observations:
+        kw = dict(name=o.name, type=o.type, data=o.data, data_error=o.error,
+                  likelihood_weight=o.weight, instrument_profile_fwhm=o.fwhm,
+                  spectrum_modifier_id=o.modifier)
+        if o.type == "spectroscopy":
+            kw["wavelengths"] = o.wavelengths
+        else:
+            kw["bin_wavelength_edges"] = o.bin_edges
+        if o.filter is not None:
+            kw["filter_response"] = (o.filter[1], o.filter[2])
+            kw["filter_detector_type"] = o.filter[0].lower()
+        out.append(api.ObservationInput(**kw))
+    return out
+
+
+def open_retrieval_in_memory(case: Case, scratch_folder, device=0):
+    """Engine Retrieval with tables registered in memory and observations
+    passed as ObservationInput (only the wavenumber list is written)."""
+    from . import api
+    os.makedirs(scratch_folder, exist_ok=True)
+    wn = os.path.join(scratch_folder, "wavenumber_full.dat")
+    with open(wn, "w") as f:
+        f.write(f"{case.wavenumber_full.size}\n")
+        f.write("\n".join(_fmt(v) for v in case.wavenumber_full))
+        f.write("\n")
+    api.clear_opacity_data()
+    for s in case.species:
+        t = np.repeat(s.temperatures, s.pressures.size)
+        p = np.tile(s.pressures, s.temperatures.size)
+        api.register_opacity_data(s.folder, t, p, s.values.reshape(-1, s.values.shape[-1]),
+                                  species_mass=s.mass, log_storage=s.log_storage)
+    meta = case_meta(case)
+    cfg = api.GlobalConfig(retrieval_folder_path=scratch_folder, cross_section_file_path="",
+                           wavenumber_file_path=wn, spectral_resolution=meta["spectral_resolution"],
+                           use_error_inflation=meta["use_error_inflation"], device=device)
+    r = api.Retrieval(cfg, _model_config(meta), observations=observation_inputs(case))
+    api.clear_opacity_data()
+    return r
+
+
+def inject_data(case: Case, model_bins, seed=99, frac=0.02):
+    """Replace the observed data by the given model bins (one array per
+    observation, in each observation's input order) with 2 % Gaussian noise."""
+    rng = np.random.default_rng(seed)
+    for o, m in zip(case.observations, model_bins):
+        o.data, o.error = noisy(rng, np.asarray(m, dtype=np.float64), frac)
+
+
+def perturbed(params, case: Case, seed=5, scale=1.0):
+    """A nearby live point: small shifts of every physical parameter."""
+    rng = np.random.default_rng(seed)
+    p = np.array(params, dtype=np.float64)
+    nb_mix = sum(len(c[1]) for c in case.chemistry if c[0] == "iso")
+    p[0] += 0.02 * scale * rng.standard_normal()
+    p[3:3 + nb_mix] *= np.exp(0.1 * scale * rng.standard_normal(nb_mix))
+    p[3 + nb_mix] *= 1 + 0.01 * scale * rng.standard_normal()
+    return p

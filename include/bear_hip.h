@@ -1,0 +1,165 @@
+/*
+ * bear_hip.h -- C ABI of the MI355X forward-model and likelihood engine.
+ *
+ * A drop-in for the part of BeAR that one likelihood evaluation runs through:
+ * the forward models' calcModel / calcModelGPU, the retrieval's
+ * computeLikelihood / computeModel, and the pyBeAR bindings of those calls.
+ * Everything crosses this boundary as plain C: scalars, NUL-terminated
+ * strings, and pointers with explicit sizes.  No C++ types, no exceptions;
+ * every call returns a status code and bear_last_error() describes the last
+ * failure of the calling thread.
+ *
+ * Reference citations are relative to the reference repository's src/.
+ *
+ * Threading: a bear_retrieval binds to one device and one HIP stream.  Calls
+ * on one bear_retrieval are not re-entrant; different objects may be used
+ * from different threads.
+ */
+#ifndef BEAR_HIP_H
+#define BEAR_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status
+ * The reference reports errors by exceptions (additional/exceptions.h:32-49:
+ * InvalidInput, FileNotFound), std::bad_alloc (data_management_kernels.cu:85-91)
+ * and exit() on device errors (error_check.h:29-37).  Here each becomes a code. */
+#define BEAR_OK                    0
+#define BEAR_ERROR_INVALID_INPUT  -1   /* InvalidInput */
+#define BEAR_ERROR_FILE_NOT_FOUND -2   /* FileNotFound */
+#define BEAR_ERROR_DEVICE         -3   /* a HIP runtime error (the reference exits) */
+#define BEAR_ERROR_OUT_OF_MEMORY  -4   /* std::bad_alloc / hipErrorOutOfMemory */
+#define BEAR_ERROR_INTERNAL       -5
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* bear_last_error(void);
+
+/* Library version string. */
+const char* bear_version(void);
+
+/* ---------------------------------------------------------------- configuration
+ * GlobalConfig (config/global_config.h:28-71), the fields the forward-model
+ * path reads.  Paths follow the reference: the opacity folders named by the
+ * model configuration are relative to cross_section_file_path; observation
+ * files and bandpass files are relative to retrieval_folder_path.  A missing
+ * wavenumber file selects the default 0..90000 cm^-1 grid at 0.01 cm^-1
+ * (spectral_grid.cpp:70-84). */
+typedef struct bear_global_config {
+  int use_gpu;                      /* must be 1: this engine has no CPU path */
+  int device;                       /* HIP device ordinal */
+  const char* retrieval_folder_path;
+  const char* cross_section_file_path;
+  const char* wavenumber_file_path;
+  int spectral_discretisation;      /* 0 const_wavenumber, 1 const_wavelength, 2 const_resolution */
+  double spectral_resolution;
+  int use_error_inflation;          /* last parameter is log10 of the added variance */
+} bear_global_config;
+
+/* Forward-model configuration (EmissionModelConfig, forward_model/emission/emission.h:51-102;
+ * TransmissionModelConfig and OccultationModelConfig alike).  The string
+ * fields use the line syntax of the reference's forward_model.config:
+ *   temperature_profile  "poly 6 1" | "const"
+ *   radiative_transfer   "scm"
+ *   chemistry            one module per line: "iso H2O CH4\nbg H2He"
+ *   opacity_species      one "symbol folder" pair per line: "H2O Molecules/H2O\nCIA-H2-H2 CIA/H2-H2\nH2 Rayleigh" */
+typedef struct bear_model_config {
+  size_t nb_grid_points;
+  double atmos_bottom_pressure;     /* bar */
+  double atmos_top_pressure;        /* bar */
+  const char* temperature_profile;
+  const char* radiative_transfer;
+  const char* chemistry;
+  const char* opacity_species;
+} bear_model_config;
+
+/* ObservationInput (observations/observations.h:43-83).  For "spectroscopy"
+ * give wavelengths[nb_points]; for "band-spectroscopy" and "photometry" give
+ * bin_edges[nb_points][2] (micron).  Optional arrays may be NULL. */
+typedef struct bear_observation_input {
+  const char* name;
+  const char* type;                          /* spectroscopy | band-spectroscopy | photometry */
+  size_t nb_points;
+  const double* wavelengths;
+  const double* bin_edges;
+  const double* data;
+  const double* data_error;
+  const double* likelihood_weight;           /* NULL: all 1 */
+  const double* instrument_profile_fwhm;     /* NULL: no line-spread function */
+  size_t filter_nb_points;                   /* 0: no bandpass */
+  const double* filter_wavelength;
+  const double* filter_response;
+  const char* filter_detector_type;          /* "energy" | "photon" */
+  const char* spectrum_modifier;             /* NULL, "none" or "shift_const" */
+} bear_observation_input;
+
+/* Tables held in memory instead of on disk.  A registered folder is used in
+ * place of <cross_section_file_path><folder>/filelist.dat and its row files
+ * (opacity_species.cpp:66-122, cross_section_file.cpp:33-62): mass and
+ * log_storage mean what the first filelist.dat line means, and each row holds
+ * nb_wavenumbers float32 values over the full wavenumber grid exactly as a row
+ * file would.  The data are copied; registration is process-wide. */
+int bear_opacity_data_register(const char* folder, double species_mass, int log_storage,
+                               size_t nb_rows, const double* temperatures, const double* pressures,
+                               size_t nb_wavenumbers, const float* rows);
+int bear_opacity_data_clear(void);
+
+/* ---------------------------------------------------------------- retrieval
+ * Retrieval (retrieval/retrieval.h) restricted to the likelihood path:
+ * observations set up (retrieval_load_observations.cpp:38-92), the
+ * spectral grid sampled from them (spectral_grid.cpp:123-304) and a forward
+ * model selected (retrieval/select_forward_model.cpp:38-164, here "emission").
+ * Observations are the in-memory inputs followed by the files named in
+ * obs_files (entries of observations.list, with their modifiers). */
+typedef struct bear_retrieval bear_retrieval;
+
+int bear_retrieval_create(const bear_global_config* config,
+                          const char* forward_model_type,
+                          const bear_model_config* model_config,
+                          const bear_observation_input* observations, size_t nb_observations,
+                          const char* const* obs_files, const char* const* obs_modifiers, size_t nb_obs_files,
+                          bear_retrieval** out);
+void bear_retrieval_destroy(bear_retrieval* r);
+
+/* Sizes: physical parameters (ForwardModel::parametersNumber + 1 with error
+ * inflation), high-resolution points, observations and their points. */
+size_t bear_retrieval_nb_parameters(const bear_retrieval* r);
+size_t bear_retrieval_nb_spectral_points(const bear_retrieval* r);
+size_t bear_retrieval_nb_observations(const bear_retrieval* r);
+size_t bear_retrieval_nb_observation_points(const bear_retrieval* r, size_t observation);
+
+/* Wavenumbers (cm^-1, ascending) and wavelengths (micron) of the sampled grid
+ * (SpectralGrid::wavenumber_list / wavelength_list), each [nb_spectral_points]. */
+int bear_retrieval_spectral_grid(const bear_retrieval* r, double* wavenumbers, double* wavelengths);
+
+/* Retrieval::computeLikelihood (retrieval.cpp:295-309, 374-420): one
+ * evaluation of the forward model on the device and the Gaussian
+ * log-likelihood; -1e30 for a neglected model. */
+int bear_retrieval_compute_likelihood(bear_retrieval* r, const double* parameters, size_t nb_parameters,
+                                      double* log_likelihood);
+
+/* Retrieval::computeModel -> ForwardModel::calcModel (forward_model.cpp:60-127):
+ * spectrum [nb_spectral_points] (may be NULL when return_high_res is 0) in the
+ * model's high-resolution units; spectrum_obs holds every observation's points,
+ * concatenated in observation order, each in the order of its input
+ * (ascending inputs are returned ascending, forward_model.cpp:119-123). */
+int bear_retrieval_compute_model(bear_retrieval* r, const double* parameters, size_t nb_parameters,
+                                 int return_high_res, double* spectrum, double* spectrum_obs,
+                                 int* neglect_model);
+
+/* ForwardModel::calcModelGPU (forward_model.h:76-79): device buffers owned by
+ * the caller.  spectrum_dev [nb_spectral_points] doubles; spectrum_obs_dev[i]
+ * [nb_observation_points(i)] doubles, in the internal descending-wavelength
+ * order of each observation.  Returns after the work is complete. */
+int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameters, size_t nb_parameters,
+                                      double* spectrum_dev, double* const* spectrum_obs_dev,
+                                      int* neglect_model);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAR_HIP_H */

@@ -1,0 +1,154 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by the tests, the smoke check and the
+CPU-baseline leg of bench.py.  Also runs the reference build
+(oracle/_ref/bear_ref_emission) when it exists.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "bear_ref_emission")
+NB_SPECIES_IDS = 46
+
+_lib = None
+
+
+def build():
+    """Compile the oracle library (and nothing of the product)."""
+    root = os.path.dirname(HERE)
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile"], cwd=root, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, dp, ip = C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_emission_create.restype = vp
+        L.oracle_emission_create.argtypes = [C.c_char_p]
+        L.oracle_emission_destroy.argtypes = [vp]
+        for f in ("oracle_nb_points", "oracle_nb_levels", "oracle_nb_parameters", "oracle_nb_observations"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp]
+        for f in ("oracle_nb_bins", "oracle_nb_conv"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp, sz]
+        L.oracle_grid.argtypes = [vp, dp, dp, ip]
+        L.oracle_observation.restype = C.c_int
+        L.oracle_observation.argtypes = [vp, sz, ip, ip, dp, ip, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]
+        L.oracle_emission_eval.restype = C.c_int
+        L.oracle_emission_eval.argtypes = [vp, dp, sz, dp, dp, dp, dp, C.POINTER(C.c_int)]
+        L.oracle_atmosphere.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.oracle_coefficients.argtypes = [vp, dp, dp]
+        L.oracle_planck.restype = C.c_double
+        L.oracle_planck.argtypes = [C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class EmissionOracle:
+    """The oracle's emission model, loaded from a retrieval folder."""
+
+    def __init__(self, folder: str):
+        self._L = lib()
+        self._h = self._L.oracle_emission_create(folder.encode())
+        if not self._h:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+        self.nb_points = self._L.oracle_nb_points(self._h)
+        self.nb_levels = self._L.oracle_nb_levels(self._h)
+        self.nb_parameters = self._L.oracle_nb_parameters(self._h)
+        self.nb_observations = self._L.oracle_nb_observations(self._h)
+        self.nb_bins = [self._L.oracle_nb_bins(self._h, i) for i in range(self.nb_observations)]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.oracle_emission_destroy(self._h)
+            self._h = None
+
+    def grid(self):
+        n = self.nb_points
+        nu, lam, idx = np.empty(n), np.empty(n), np.empty(n, np.int64)
+        self._L.oracle_grid(self._h, _p(nu), _p(lam), _p(idx, C.c_int64))
+        return nu, lam, idx
+
+    def observation(self, i):
+        n, nb, nc = self.nb_points, self.nb_bins[i], self._L.oracle_nb_conv(self._h, i)
+        edges = np.empty((nb, 2), np.int64)
+        rng = np.empty(2, np.int64)
+        sigma, fil, fw = np.empty(n), np.empty(n), np.empty(n)
+        conv = np.empty((max(nc, 1), 2), np.int64)
+        norm = np.empty(1)
+        data, err, w = np.empty(nb), np.empty(nb), np.empty(nb)
+        asc = C.c_int(0)
+        self._L.oracle_observation(self._h, i, _p(edges, C.c_int64), _p(rng, C.c_int64), _p(sigma),
+                                   _p(conv, C.c_int64), _p(fil), _p(fw), _p(norm), _p(data), _p(err),
+                                   _p(w), C.byref(asc))
+        return dict(edge_indices=edges, index_range=rng, sigma=sigma, conv=conv[:nc], filter=fil,
+                    filter_weight=fw, filter_norm=norm[0], data=data, error=err, weight=w,
+                    ascending=bool(asc.value))
+
+    def evaluate(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        n = self.nb_points
+        flux, spec = np.empty(n), np.empty(n)
+        bins = np.empty(sum(self.nb_bins))
+        ll = C.c_double(0)
+        neg = C.c_int(0)
+        rc = self._L.oracle_emission_eval(self._h, _p(params), params.size, _p(flux), _p(spec), _p(bins),
+                                          C.byref(ll), C.byref(neg))
+        if rc != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+        out, k = [], 0
+        for nb in self.nb_bins:
+            out.append(bins[k:k + nb].copy())
+            k += nb
+        return dict(spectrum_flux=flux, spectrum=spec, bins=out, loglike=ll.value, neglect=bool(neg.value))
+
+    def atmosphere(self):
+        L = self.nb_levels
+        t, z, p, mmw = np.empty(L), np.empty(L), np.empty(L), np.empty(L)
+        nd = np.empty((L, NB_SPECIES_IDS))
+        self._L.oracle_atmosphere(self._h, _p(t), _p(z), _p(p), _p(nd), _p(mmw))
+        return dict(temperature=t, altitude=z, pressure=p, number_densities=nd, mean_molecular_weight=mmw)
+
+    def coefficients(self):
+        k = np.empty((self.nb_levels, self.nb_points))
+        s = np.empty_like(k)
+        self._L.oracle_coefficients(self._h, _p(k), _p(s))
+        return k, s
+
+
+def planck(t, nu):
+    return lib().oracle_planck(float(t), float(nu))
+
+
+def run_reference(case_folder: str, out_folder: str, n_timed: int = 0, threads: int | None = None):
+    """Run the reference build on a case folder; returns its outputs as a dict
+    of arrays, or None when the reference build is absent."""
+    if not os.path.exists(REF_BIN):
+        return None
+    os.makedirs(out_folder, exist_ok=True)
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    res = subprocess.run([REF_BIN, case_folder, out_folder, str(n_timed)], capture_output=True, text=True, env=env)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference build failed ({res.returncode}): {res.stderr[-2000:]}")
+    out = {f[:-4]: np.load(os.path.join(out_folder, f)) for f in os.listdir(out_folder) if f.endswith(".npy")}
+    out["stdout"] = res.stdout
+    return out

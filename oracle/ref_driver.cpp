@@ -1,0 +1,271 @@
+// Test infrastructure: a driver around the reference's own CPU sources.
+//
+// Compiled only by oracle/ref.mk against the reference tree; it is never part
+// of the product.  It composes the reference's CPU emission path in the same
+// order as EmissionModel::calcModelCPU (forward_model/emission/emission.cpp:194-228)
+// and Retrieval::logLikelihood (retrieval/retrieval.cpp:314-370), after the
+// set-up sequence of Retrieval::loadObservations
+// (retrieval/retrieval_load_observations.cpp:68-92).  The log-likelihood sum is
+// restated here because retrieval.cpp needs Boost headers this image lacks.
+//
+// usage: bear_ref_emission <case folder> <output folder> [n_timed_evals]
+// Outputs are .npy arrays (little-endian float64 / int64) in the output folder.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "config/global_config.h"
+#include "spectral_grid/spectral_grid.h"
+#include "observations/observations.h"
+#include "forward_model/atmosphere/atmosphere.h"
+#include "transport_coeff/opacity_calc.h"
+#include "temperature/piecewise_poly_temperature.h"
+#include "chemistry/isoprofile_chemistry.h"
+#include "chemistry/background_chemistry.h"
+#include "radiative_transfer/short_characteristics.h"
+#include "additional/physical_const.h"
+
+namespace {
+
+void writeNpy(const std::string& path, const std::vector<double>& v, size_t rows = 0) {
+  std::string shape = rows ? "(" + std::to_string(rows) + ", " + std::to_string(v.size() / rows) + ")"
+                           : "(" + std::to_string(v.size()) + ",)";
+  std::string header = "{'descr': '<f8', 'fortran_order': False, 'shape': " + shape + ", }";
+  while ((10 + header.size() + 1) % 64 != 0) header += ' ';
+  header += '\n';
+  std::ofstream f(path, std::ios::binary);
+  const char magic[] = "\x93NUMPY\x01\x00";
+  f.write(magic, 8);
+  uint16_t len = static_cast<uint16_t>(header.size());
+  f.write(reinterpret_cast<const char*>(&len), 2);
+  f.write(header.data(), header.size());
+  f.write(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(double));
+}
+
+template <class T>
+std::vector<double> asDouble(const std::vector<T>& v) { return std::vector<double>(v.begin(), v.end()); }
+
+struct CaseText {
+  size_t nb_levels = 0;
+  double resolution = 0, p_bottom = 0, p_top = 0;
+  std::vector<std::string> temperature;
+  std::vector<std::vector<std::string>> chemistry;
+  std::vector<std::string> opac_symbol, opac_folder;
+  bool error_inflation = false;
+  std::vector<double> parameters;
+};
+
+CaseText readCase(const std::string& folder) {
+  CaseText c;
+  std::ifstream f(folder + "case.txt");
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    std::string key;
+    in >> key;
+    std::vector<std::string> rest;
+    std::string tok;
+    while (in >> tok) rest.push_back(tok);
+    if (key == "spectral_resolution") c.resolution = std::stod(rest[0]);
+    else if (key == "nb_levels") c.nb_levels = std::stoul(rest[0]);
+    else if (key == "p_bottom") c.p_bottom = std::stod(rest[0]);
+    else if (key == "p_top") c.p_top = std::stod(rest[0]);
+    else if (key == "temperature") c.temperature = rest;
+    else if (key == "chemistry") c.chemistry.push_back(rest);
+    else if (key == "opacity") { c.opac_symbol.push_back(rest[0]); c.opac_folder.push_back(rest[1]); }
+    else if (key == "error_inflation") c.error_inflation = rest[0] == "1";
+    else if (key == "parameters") {
+      size_t n = std::stoul(rest[0]);
+      for (size_t i = 0; i < n; ++i) c.parameters.push_back(std::stod(rest[1 + i]));
+    }
+  }
+  return c;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  using namespace bear;
+  if (argc < 3) { std::cerr << "usage: bear_ref_emission <case> <out> [n_timed]\n"; return 2; }
+  std::string folder = argv[1];
+  if (folder.back() != '/') folder += '/';
+  std::string out = argv[2];
+  if (out.back() != '/') out += '/';
+  const int n_timed = argc > 3 ? std::atoi(argv[3]) : 0;
+
+  CaseText cs = readCase(folder);
+
+  GlobalConfig config;
+  config.use_gpu = false;
+  config.retrieval_folder_path = folder;
+  config.wavenumber_file_path = folder + "wavenumber_full.dat";
+  config.cross_section_file_path = folder + "opacity/";
+  config.spectral_disecretisation = 0;
+  config.spectral_resolution = cs.resolution;
+  config.use_error_inflation = cs.error_inflation;
+
+  SpectralGrid grid(&config);
+
+  // observations, in the order of Retrieval::loadObservations
+  std::vector<std::string> files, modifiers;
+  {
+    std::ifstream f(folder + "observations.list");
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream in(line);
+      std::string a, b;
+      in >> a >> b;
+      if (a.empty()) continue;
+      files.push_back(a);
+      modifiers.push_back(b);
+    }
+  }
+  std::vector<Observation> observations(files.size(), Observation(&config, &grid));
+  for (size_t i = 0; i < files.size(); ++i) observations[i].init(folder + files[i], modifiers[i]);
+  grid.sampleSpectralGrid(observations);
+  for (auto& o : observations) {
+    o.spectral_bands.setInstrumentProfileFWHW(o.instrument_profile_fwhm);
+    o.setFilterResponseFunction();
+  }
+
+  // the emission model's modules (EmissionModel ctor + initModules)
+  const std::vector<double> boundaries = {cs.p_bottom, cs.p_top};
+  Atmosphere atmosphere(cs.nb_levels, boundaries, false);
+  OpacityCalculation opacity(&config, &grid, &atmosphere, cs.opac_symbol, cs.opac_folder, false, false);
+  ShortCharacteristics rt(&grid);
+  PiecewisePolynomialTemperature temperature(std::stoul(cs.temperature[1]), std::stoul(cs.temperature[2]), boundaries);
+  std::vector<Chemistry*> chemistry;
+  size_t nb_chem_param = 0;
+  for (auto& c : cs.chemistry) {
+    if (c[0] == "iso") chemistry.push_back(new IsoprofileChemistry(std::vector<std::string>(c.begin() + 1, c.end())));
+    else chemistry.push_back(new BackgroundChemistry(c[1]));
+    nb_chem_param += chemistry.back()->nbParameters();
+  }
+  size_t nb_modifier = 0;
+  for (auto& o : observations) nb_modifier += o.nb_modifier_param;
+
+  const size_t nb_temp_param = temperature.nbParameters();
+  const std::vector<double>& p = cs.parameters;
+  const std::vector<double> model_param(p.begin(), p.begin() + 3);
+  const std::vector<double> chem_param(p.begin() + 3, p.begin() + 3 + nb_chem_param);
+  const std::vector<double> temp_param(p.begin() + 3 + nb_chem_param, p.begin() + 3 + nb_chem_param + nb_temp_param);
+  const std::vector<double> modifier_param(p.begin() + 3 + nb_chem_param + nb_temp_param,
+                                           p.begin() + 3 + nb_chem_param + nb_temp_param + nb_modifier);
+
+  std::vector<double> spectrum, spectrum_high_res_flux;
+  std::vector<std::vector<double>> spectrum_obs(observations.size());
+  bool neglect = false;
+  double log_like = 0;
+  std::vector<CloudModel*> no_clouds;
+
+  auto evaluate = [&]() {
+    // EmissionModel::calcAtmosphereStructure (emission.cpp:158-189)
+    const double surface_gravity = std::pow(10, model_param[0]);
+    neglect = atmosphere.calcAtmosphereStructure(surface_gravity, 1.0, false, &temperature, temp_param,
+                                                 chemistry, chem_param);
+    opacity.calculate(no_clouds, std::vector<double>{});
+    spectrum.assign(grid.nbSpectralPoints(), 0.0);
+    // EmissionModel::radiusDistanceScaling (emission.cpp:108-120)
+    double scaling = constants::radius_jupiter / model_param[2];
+    scaling = scaling * scaling * model_param[1];
+    rt.calcSpectrum(atmosphere, opacity.absorption_coeff, opacity.absorption_coeff, opacity.cloud_optical_depths,
+                    opacity.cloud_single_scattering, opacity.cloud_asym_param, scaling, spectrum);
+    spectrum_high_res_flux = spectrum;
+    for (size_t i = 0; i < observations.size(); ++i)
+      spectrum_obs[i] = observations[i].processModelSpectrum(spectrum, true);
+    // ForwardModel::applyObservationModifier (forward_model.cpp:161-180)
+    auto it = modifier_param.begin();
+    for (size_t i = 0; i < observations.size(); ++i)
+      if (observations[i].nb_modifier_param != 0) {
+        const double shift = *it;
+        if (shift != 0) for (auto& s : spectrum_obs[i]) s += shift;
+        it += observations[i].nb_modifier_param;
+      }
+    rt.changeSpectrumUnits(spectrum);
+    // Gaussian log-likelihood, restating retrieval.cpp:330-362
+    const double inflation = config.use_error_inflation ? std::pow(10, p.back()) : 0.0;
+    log_like = 0;
+    for (size_t i = 0; i < observations.size(); ++i)
+      for (size_t j = 0; j < observations[i].nbPoints(); ++j) {
+        const double e2 = observations[i].data_error[j] * observations[i].data_error[j] + inflation;
+        const double d = observations[i].data[j] - spectrum_obs[i][j];
+        log_like += (-0.5 * std::log(e2 * 2.0 * constants::pi) - 0.5 * d * d / e2) * observations[i].likelihood_weight[j];
+      }
+    if (neglect) log_like = -1e30;
+  };
+
+  evaluate();
+
+  std::vector<double> timings;
+  for (int k = 0; k < n_timed; ++k) {
+    auto t0 = std::chrono::steady_clock::now();
+    evaluate();
+    auto t1 = std::chrono::steady_clock::now();
+    timings.push_back(std::chrono::duration<double>(t1 - t0).count());
+  }
+
+  // outputs
+  writeNpy(out + "wavenumber.npy", grid.wavenumber_list);
+  writeNpy(out + "wavelength.npy", grid.wavelength_list);
+  writeNpy(out + "index_list.npy", asDouble(grid.spectralIndexList()));
+  writeNpy(out + "temperature.npy", atmosphere.temperature);
+  writeNpy(out + "pressure.npy", atmosphere.pressure);
+  writeNpy(out + "altitude.npy", atmosphere.altitude);
+  {
+    std::vector<double> nd;
+    for (auto& l : atmosphere.number_densities) nd.insert(nd.end(), l.begin(), l.end());
+    writeNpy(out + "number_densities.npy", nd, atmosphere.number_densities.size());
+  }
+  if (grid.nbSpectralPoints() * cs.nb_levels <= 4000000) {
+    std::vector<double> k;
+    for (auto& row : opacity.absorption_coeff) k.insert(k.end(), row.begin(), row.end());
+    writeNpy(out + "absorption_coeff.npy", k, opacity.absorption_coeff.size());
+    std::vector<double> s;
+    for (auto& row : opacity.scattering_coeff) s.insert(s.end(), row.begin(), row.end());
+    writeNpy(out + "scattering_coeff.npy", s, opacity.scattering_coeff.size());
+  }
+  writeNpy(out + "spectrum_flux.npy", spectrum_high_res_flux);
+  writeNpy(out + "spectrum.npy", spectrum);
+  for (size_t i = 0; i < observations.size(); ++i) {
+    const std::string tag = std::to_string(i);
+    std::vector<double> bins = spectrum_obs[i];
+    writeNpy(out + "obs" + tag + "_bins.npy", bins);
+    std::vector<double> edges;
+    for (auto& e : observations[i].spectral_bands.edge_indices) { edges.push_back(e[0]); edges.push_back(e[1]); }
+    writeNpy(out + "obs" + tag + "_edge_indices.npy", edges, observations[i].spectral_bands.edge_indices.size());
+    auto& sb = observations[i].spectral_bands;
+    writeNpy(out + "obs" + tag + "_index_range.npy", {double(sb.obs_index_range.first), double(sb.obs_index_range.second)});
+    if (!sb.instrument_profile_sigma.empty()) {
+      writeNpy(out + "obs" + tag + "_sigma.npy", sb.instrument_profile_sigma);
+      std::vector<double> iv;
+      for (auto& e : sb.convolution_quadrature_intervals) { iv.push_back(e[0]); iv.push_back(e[1]); }
+      writeNpy(out + "obs" + tag + "_conv_intervals.npy", iv, sb.convolution_quadrature_intervals.size());
+    }
+    if (!observations[i].filter_response.empty()) {
+      writeNpy(out + "obs" + tag + "_filter.npy", observations[i].filter_response);
+      writeNpy(out + "obs" + tag + "_filter_norm.npy", {observations[i].filter_response_normalisation});
+    }
+    writeNpy(out + "obs" + tag + "_data.npy", observations[i].data);
+    writeNpy(out + "obs" + tag + "_error.npy", observations[i].data_error);
+    writeNpy(out + "obs" + tag + "_weight.npy", observations[i].likelihood_weight);
+    writeNpy(out + "obs" + tag + "_ascending.npy", {double(observations[i].ascending_wavelengths)});
+  }
+  writeNpy(out + "loglike.npy", {log_like, double(neglect)});
+  if (!timings.empty()) writeNpy(out + "timings.npy", timings);
+
+  for (auto c : chemistry) delete c;
+  std::printf("N=%zu loglike=%.17g neglect=%d\n", grid.nbSpectralPoints(), log_like, int(neglect));
+  if (!timings.empty()) {
+    std::sort(timings.begin(), timings.end());
+    std::printf("median_eval_s=%.6g\n", timings[timings.size() / 2]);
+  }
+  return 0;
+}

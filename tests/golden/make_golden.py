@@ -1,0 +1,93 @@
+"""Writes the golden fixtures of the emission path from the reference's own
+CPU code (the build of oracle/ref.mk, compiled from /root/reference/src).
+
+For each seeded case (tests/cases.py) the observed data are injected from
+the reference build's own model at the truth parameters, then the reference
+build evaluates every parameter vector of the case.  The fixtures hold the
+inputs that matter (parameter vectors, injected data, a digest of the table
+files) and the reference's outputs: sampled grid, band-edge and convolution
+indices, atmosphere, spectra, bins and log-likelihoods.
+
+    make -f oracle/ref.mk && python tests/golden/make_golden.py
+"""
+
+import hashlib
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from oracle import api  # noqa: E402
+import cases  # noqa: E402
+
+
+def table_digest(folder):
+    h = hashlib.sha256()
+    for dirpath, _, files in sorted(os.walk(os.path.join(folder, "opacity"))):
+        for f in sorted(files):
+            with open(os.path.join(dirpath, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()
+
+
+def reference_bins(folder, params, work):
+    import json
+    meta = json.load(open(os.path.join(folder, "case.json")))
+    meta_txt = open(os.path.join(folder, "case.txt")).read().splitlines()
+    meta_txt = [l for l in meta_txt if not l.startswith("parameters ")]
+    meta_txt.append(f"parameters {len(params)} " + " ".join(repr(float(v)) for v in params))
+    with open(os.path.join(folder, "case.txt"), "w") as f:
+        f.write("\n".join(meta_txt) + "\n")
+    out = api.run_reference(folder, work)
+    bins = []
+    for i in range(len(meta["observations"])):
+        b = out[f"obs{i}_bins"]
+        bins.append(b[::-1].copy() if out[f"obs{i}_ascending"][0] else b)
+    return out, bins
+
+
+def main():
+    if not os.path.exists(api.REF_BIN):
+        sys.exit("build the reference first: make -f oracle/ref.mk")
+    tmp = tempfile.mkdtemp()
+    try:
+        for name in cases.CASES:
+            folder = os.path.join(tmp, name)
+            work = os.path.join(tmp, name + "_out")
+            folder, params = cases.build(name, folder, model_fn=lambda f, p: reference_bins(f, p, work)[1])
+            fixture = {"parameters": np.array(params), "table_digest": np.array(table_digest(folder))}
+            import json
+            meta = json.load(open(os.path.join(folder, "case.json")))
+            for k, p in enumerate(params):
+                shutil.rmtree(work, ignore_errors=True)
+                out, _ = reference_bins(folder, p, work)
+                pre = f"p{k}_"
+                for key in ("temperature", "altitude", "loglike"):
+                    fixture[pre + key] = out[key]
+                # the high-resolution flux in full for the first vector, every 10th point otherwise
+                fixture[pre + "spectrum_flux"] = out["spectrum_flux"] if k == 0 else out["spectrum_flux"][::10]
+                for i in range(len(meta["observations"])):
+                    fixture[pre + f"obs{i}_bins"] = out[f"obs{i}_bins"]
+                if k == 0:
+                    for key in ("wavenumber", "index_list", "pressure", "number_densities"):
+                        fixture[key] = out[key]
+                    for i in range(len(meta["observations"])):
+                        for key in ("edge_indices", "index_range", "conv_intervals", "sigma", "filter",
+                                    "filter_norm", "data", "error", "weight", "ascending"):
+                            if f"obs{i}_{key}" in out:
+                                fixture[f"obs{i}_{key}"] = out[f"obs{i}_{key}"]
+            path = os.path.join(ROOT, "tests", "golden", f"emission_{name.replace('emission_', '')}.npz")
+            np.savez_compressed(path, **fixture)
+            print(path, os.path.getsize(path), "bytes")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()
