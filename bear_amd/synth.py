@@ -309,9 +309,9 @@ def emission_c1(seed=11):
     ]
     r = 100.0
     lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
-    # keep the 5-sigma LSF wings inside the grid
-    lo = lam_lo * (1 + 6.0 / (2.355 * r)) * 1.01
-    hi = lam_hi * (1 - 6.0 / (2.355 * r)) * 0.99
+    # the observation range (bins + 5 sigma of the profile) stays inside the grid
+    lo = lam_lo * 1.12
+    hi = lam_hi * 0.88
     wl = np.linspace(lo, hi, 100)
     scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
     d, e = noisy(rng, blackbody_flux_lambda(wl, 1100.0, scale))

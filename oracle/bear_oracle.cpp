@@ -131,6 +131,9 @@ struct Grid {
     for (size_t i = 0; i < edges.size(); ++i) {
       auto l = std::lower_bound(nu_full.begin(), nu_full.end(), edges[i][0]);
       auto r = std::lower_bound(nu_full.begin(), nu_full.end(), edges[i][1]);
+      // beyond the list the reference indexes out of bounds; refuse instead
+      if (l == nu_full.end() || r == nu_full.end() || (l == nu_full.begin() && *l > edges[i][0]))
+        throw std::runtime_error("observation range outside the wavenumber list");
       out[i][0] = l - nu_full.begin();
       if (nu_full[out[i][0]] > edges[i][0]) out[i][0] -= 1;
       out[i][1] = r - nu_full.begin();
