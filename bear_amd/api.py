@@ -53,6 +53,12 @@ class _ObservationInput(C.Structure):
                 ("filter_detector_type", C.c_char_p), ("spectrum_modifier", C.c_char_p)]
 
 
+class EvalStats(C.Structure):
+    _fields_ = [("nb_spectral_points", C.c_size_t), ("nb_levels", C.c_size_t), ("nb_species", C.c_size_t),
+                ("nb_bins", C.c_size_t), ("distinct_table_rows", C.c_size_t), ("table_value_bytes", C.c_size_t),
+                ("kernel_launches", C.c_size_t), ("kernel_ms_total", C.c_double)]
+
+
 _lib = None
 
 
@@ -81,6 +87,8 @@ def lib():
         L.bear_retrieval_spectral_grid.argtypes = [vp, _dp, _dp]
         L.bear_retrieval_compute_likelihood.argtypes = [vp, _dp, sz, _dp]
         L.bear_retrieval_compute_model.argtypes = [vp, _dp, sz, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+        L.bear_retrieval_set_profiling.argtypes = [vp, C.c_int]
+        L.bear_retrieval_stats.argtypes = [vp, C.POINTER(EvalStats)]
         L.bear_forward_model_calc_model_gpu.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -247,6 +255,14 @@ class Retrieval:
             out.append(obs[k:k + n].copy())
             k += n
         return ForwardModelOutput(bool(neg.value), spec if spec is not None else np.zeros(0), out)
+
+    def set_profiling(self, enable=True):
+        _check(self._L.bear_retrieval_set_profiling(self._h, int(enable)))
+
+    def stats(self) -> dict:
+        st = EvalStats()
+        _check(self._L.bear_retrieval_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in EvalStats._fields_}
 
     def calcModelGPU(self, physical_parameters, spectrum_ptr, spectrum_obs_ptrs):
         """ForwardModel::calcModelGPU on caller-owned device buffers (raw

@@ -181,4 +181,26 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
   });
 }
 
+int bear_retrieval_set_profiling(bear_retrieval* r, int enable) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    r->impl->engine().setProfiling(enable != 0);
+  });
+}
+
+int bear_retrieval_stats(const bear_retrieval* r, bear_eval_stats* out) {
+  return guarded([&] {
+    if (!r || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    const bear_amd::DeviceEngine& e = r->impl->engine();
+    out->nb_spectral_points = e.nbPoints();
+    out->nb_levels = e.nbLevels();
+    out->nb_species = e.nbSpecies();
+    out->nb_bins = e.nbBins();
+    out->distinct_table_rows = e.distinctRows();
+    out->table_value_bytes = sizeof(double);
+    out->kernel_launches = e.timedLaunches();
+    out->kernel_ms_total = e.timedMilliseconds();
+  });
+}
+
 }  // extern "C"

@@ -20,12 +20,16 @@ struct LevelState {
 };
 
 // Per (level, opacity species): effective number density (n_s, or n_s times
-// the CIA partners' densities) and the table stencil.  n == 0 skips the
-// species, as opacity_species.cpp:409-412 does.
+// the CIA partners' densities) and the table stencil.  n == 0 removes the
+// species' term, as opacity_species.cpp:409-412 skips it.  reload != 0 when
+// the stencil rows differ from the previous level's (always at level 0):
+// the kernel keeps the four row values in registers otherwise.
 struct LevelSpecies {
   double n;
   double f_p_lower, f_p_upper, f_t;
   uint32_t row[4];
+  uint32_t reload;
+  uint32_t pad;
 };
 
 // Per bin of every observation, in observation order: the folded operator
@@ -53,7 +57,8 @@ class DeviceEngine {
 
   // pinned staging of the per-evaluation inputs; filled by the caller
   LevelState* levels() { return levels_host_; }
-  LevelSpecies* levelSpecies() { return species_host_; }
+  LevelSpecies* levelSpecies() { return species_host_; }    // [level][speciesStride()]
+  size_t speciesStride() const { return species_stride_; }
   double* shifts() { return shift_host_; }
 
   // One emission evaluation: opacity + short characteristics -> flux on the
@@ -66,6 +71,12 @@ class DeviceEngine {
   void copySpectrum(double* host, const double* spectrum_dev);
   void synchronize();
   double* spectrumBuffer();                      // engine-owned [N], allocated on first use
+  void setProfiling(bool on);
+  size_t timedLaunches() const { return timed_launches_; }
+  double timedMilliseconds() const { return timed_ms_; }
+  size_t distinctRows() const { return distinct_rows_; }
+  size_t nbLevels() const { return nb_levels_; }
+  size_t nbSpecies() const { return nb_species_; }
 
   size_t nbPoints() const { return n_points_; }
   size_t nbBins() const { return n_bins_; }
@@ -75,8 +86,8 @@ class DeviceEngine {
  private:
   int device_ = 0;
   void* stream_ = nullptr;
-  size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0;
-  size_t nb_levels_ = 0, nb_species_ = 0;
+  size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0, n_rows_ = 0;
+  size_t nb_levels_ = 0, nb_species_ = 0, species_stride_ = 0;
   double* wavenumber_ = nullptr;
   double* wavelength_ = nullptr;
   double* tables_ = nullptr;
@@ -96,6 +107,15 @@ class DeviceEngine {
   LevelSpecies* species_host_ = nullptr;
   double* shift_host_ = nullptr;
   double* ll_host_ = nullptr;
+  // measurement
+  bool profiling_ = false, event_pending_ = false;
+  void* event_start_ = nullptr;
+  void* event_stop_ = nullptr;
+  size_t timed_launches_ = 0, distinct_rows_ = 0;
+  double timed_ms_ = 0;
+  std::vector<uint8_t> row_seen_;
+  void collectTiming();
+  void countRows();
 };
 
 }  // namespace bear_amd

@@ -39,74 +39,177 @@ constexpr double kW = 0.5;
 // (additional/aux_functions.cpp:272-278): 2 h c c nu^3 / (exp(h nu c / k / T) - 1)
 constexpr double kPlanckC1 = 2. * 6.62606896e-27 * 2.99792458e10 * 2.99792458e10;
 
+// 2^t in FP64 with relative error below 2e-15: t = n + f, |f| <= 1/2,
+// 2^f from its degree-10 Chebyshev interpolant on [-1/2, 1/2] (coefficients
+// in monomial form), then an exponent shift.  t is clamped so that the shift
+// stays in range: results below 2^-1100 flush to 0, above 2^1100 become inf.
+__device__ __forceinline__ double pow2(double t) {
+  t = fmin(fmax(t, -1100.0), 1100.0);
+  const double n = __builtin_rint(t);
+  const double f = t - n;
+  double p = 6.767963482574983e-09;
+  p = fma(p, f, 1.0206977921453389e-07);
+  p = fma(p, f, 1.3217210256367588e-06);
+  p = fma(p, f, 1.525266622014565e-05);
+  p = fma(p, f, 0.00015403526882354606);
+  p = fma(p, f, 0.0013333558214738293);
+  p = fma(p, f, 0.00961812911053657);
+  p = fma(p, f, 0.055504108664544756);
+  p = fma(p, f, 0.24022650695901254);
+  p = fma(p, f, 0.6931471805599453);
+  p = fma(p, f, 1.0);
+  return __builtin_amdgcn_ldexp(p, int(n));
+}
+
+constexpr double kLog2Ten = 3.321928094887362347870319429489390175864831393;
+constexpr double kLog2E = 1.442695040888963407359924681001892137426645954;
+
+// 1/d for finite non-zero d: hardware estimate and two Newton steps
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return r;
+}
+
+// kappa_l(nu) = sum_s n_s 10^x_s, x_s the bilinear log10 cross-section of
+// species s at this level (opacity_species.cpp:277-345, 400-433).  Species
+// come in chunks of kChunk with every row of the chunk loaded before any is
+// used, so the loads of a chunk are in flight together.  Rows are always
+// valid; a zero density or a zero weight removes a term exactly.
+constexpr int kChunk = 4;
+
 __device__ __forceinline__ double levelKappa(const double* __restrict__ tab, size_t stride,
-                                             const LevelSpecies* __restrict__ d, int nb_species) {
+                                             const LevelSpecies* __restrict__ d, int nb_chunks) {
   double k = 0.0;
-  for (int s = 0; s < nb_species; ++s) {
-    const double n = d[s].n;
-    if (n == 0.0) continue;
-    const uint32_t r0 = d[s].row[0], r1 = d[s].row[1], r2 = d[s].row[2], r3 = d[s].row[3];
-    const double c0 = tab[size_t(r0) * stride];
-    double lo = c0;
-    if (r1 != r0) lo = c0 + (tab[size_t(r1) * stride] - c0) * d[s].f_p_lower;
-    double x = lo;
-    if (r2 != r0) {
-      const double c2 = tab[size_t(r2) * stride];
-      double hi = c2;
-      if (r3 != r2) hi = c2 + (tab[size_t(r3) * stride] - c2) * d[s].f_p_upper;
-      x = lo + (hi - lo) * d[s].f_t;
+  for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
+    double v[kChunk][4];
+#pragma unroll
+    for (int s = 0; s < kChunk; ++s)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+#pragma unroll
+    for (int s = 0; s < kChunk; ++s) {
+      const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
+      const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
+      const double x = fma(hi - lo, d[s].f_t, lo);
+      k = fma(d[s].n, pow2(x * kLog2Ten), k);
     }
-    k += exp10(x) * n;
   }
   return k;
 }
 
-// Fused opacity + emission radiative transfer, one wavenumber per thread.
-// Per level: kappa_l(nu) = sum_s n_s 10^(bilinear log10 sigma_s)
-// (opacity_species.cpp:277-345, 400-433); per layer the two-angle short
-// characteristics of short_characteristics.cpp:39-125, bottom to top.
-// kappa never leaves registers.
-__global__ __launch_bounds__(kBlock) void emissionKernel(
-    const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
-    const double* __restrict__ tables, size_t stride,
-    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
-    int nb_levels, int nb_species, int n, double scaling,
-    double* __restrict__ flux, double* __restrict__ spectrum) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double nu = wavenumber[i];
-  const double* tab = tables + i;
-  const double b_num = kPlanckC1 * nu * nu * nu;
-  const double x_nu = 6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16;
+__device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv_t) {
+  const double q = pow2(x_nu * inv_t);
+  return b_num * q * recip(1.0 - q);
+}
 
-  double k_prev = levelKappa(tab, stride, species, nb_species);
-  double b_prev = b_num / (exp(x_nu * levels[0].inv_temperature) - 1.0);
-  double z_prev = levels[0].altitude;
-  double i1 = b_prev, i2 = b_prev;
-  for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = levelKappa(tab, stride, species + size_t(l) * nb_species, nb_species);
-    const double b_next = b_num / (exp(x_nu * levels[l].inv_temperature) - 1.0);
-    const double z_next = levels[l].altitude;
-    const double tau = (z_next - z_prev) * (k_next + k_prev) / 2.;
-    if (tau != 0) {
-      const double r = 1.0 / tau;
-      const double a1 = exp(-tau / kMu1);
-      const double q1 = (a1 - 1.0) * (kMu1 * r);
-      i1 = i1 * a1 + ((1.0 + q1) * b_next + (-a1 - q1) * b_prev);
-      const double a2 = exp(-tau / kMu2);
-      const double q2 = (a2 - 1.0) * (kMu2 * r);
-      i2 = i2 * a2 + ((1.0 + q2) * b_next + (-a2 - q2) * b_prev);
-    }
-    k_prev = k_next;
-    b_prev = b_next;
-    z_prev = z_next;
+// One layer of the two-angle short characteristics (short_characteristics.cpp:88-114)
+__device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_prev, double b_next,
+                                                          double& i1, double& i2) {
+  if (tau != 0) {
+    const double r = recip(tau);
+    const double a1 = pow2(tau * (-kLog2E / kMu1));
+    const double q1 = (a1 - 1.0) * (kMu1 * r);
+    i1 = fma(i1, a1, (1.0 + q1) * b_next + (-a1 - q1) * b_prev);
+    const double a2 = pow2(tau * (-kLog2E / kMu2));
+    const double q2 = (a2 - 1.0) * (kMu2 * r);
+    i2 = fma(i2, a2, (1.0 + q2) * b_next + (-a2 - q2) * b_prev);
   }
+}
+
+__device__ __forceinline__ void writeFlux(int i, double i1, double i2, double scaling, const double* wavelength,
+                                          double* flux, double* spectrum) {
   const double f = 2.0 * 3.14159265358979323846 * (i1 * kMu1 * kW + i2 * kMu2 * kW) * 1e-3 * scaling;
   flux[i] = f;
   if (spectrum) {                      // radiative_transfer.h:58-60, per micron
     const double lam = wavelength[i];
     spectrum[i] = f / lam / lam * 10000.0;
   }
+}
+
+// Fused opacity + emission radiative transfer, one wavenumber per thread,
+// for NS (padded) species.  The four stencil values of every species stay
+// in registers from level to level and are reloaded only where the host
+// marks a stencil change, so each table row is read about once per thread
+// instead of once per level.  Planck as c1 nu^3 q / (1 - q),
+// q = e^(-h nu c / k T) (aux_functions.cpp:272-278).  kappa never leaves
+// registers.
+template <int NS>
+__global__ __launch_bounds__(kBlock) void emissionKernelCached(
+    const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
+    const double* __restrict__ tables, size_t stride,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    int nb_levels, int n, double scaling, double* __restrict__ flux, double* __restrict__ spectrum) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double nu = wavenumber[i];
+  const double* tab = tables + i;
+  const double b_num = kPlanckC1 * nu * nu * nu;
+  const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+  double v[NS][4];
+  auto kappa = [&](const LevelSpecies* __restrict__ d) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (d[s].reload) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+      }
+    double k = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
+      const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
+      const double x = fma(hi - lo, d[s].f_t, lo);
+      k = fma(d[s].n, pow2(x * kLog2Ten), k);
+    }
+    return k;
+  };
+  double k_prev = kappa(species);
+  double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
+  double z_prev = levels[0].altitude;
+  double i1 = b_prev, i2 = b_prev;
+  for (int l = 1; l < nb_levels; ++l) {
+    const double k_next = kappa(species + size_t(l) * NS);
+    const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
+    const double z_next = levels[l].altitude;
+    shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+    k_prev = k_next;
+    b_prev = b_next;
+    z_prev = z_next;
+  }
+  writeFlux(i, i1, i2, scaling, wavelength, flux, spectrum);
+}
+
+// The same for any number of species: chunks of kChunk species, every row
+// of a chunk loaded before any is used, no register cache.
+__global__ __launch_bounds__(kBlock) void emissionKernel(
+    const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
+    const double* __restrict__ tables, size_t stride,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    int nb_levels, int nb_chunks, int n, double scaling,
+    double* __restrict__ flux, double* __restrict__ spectrum) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double nu = wavenumber[i];
+  const double* tab = tables + i;
+  const double b_num = kPlanckC1 * nu * nu * nu;
+  const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+  const int stride_species = nb_chunks * kChunk;
+  double k_prev = levelKappa(tab, stride, species, nb_chunks);
+  double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
+  double z_prev = levels[0].altitude;
+  double i1 = b_prev, i2 = b_prev;
+  for (int l = 1; l < nb_levels; ++l) {
+    const double k_next = levelKappa(tab, stride, species + size_t(l) * stride_species, nb_chunks);
+    const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
+    const double z_next = levels[l].altitude;
+    shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+    k_prev = k_next;
+    b_prev = b_next;
+    z_prev = z_next;
+  }
+  writeFlux(i, i1, i2, scaling, wavelength, flux, spectrum);
 }
 
 __device__ __forceinline__ double waveSum(double v) {
@@ -190,7 +293,50 @@ DeviceEngine::~DeviceEngine() {
                   (void*)spectrum_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
+  if (event_start_) (void)hipEventDestroy(static_cast<hipEvent_t>(event_start_));
+  if (event_stop_) (void)hipEventDestroy(static_cast<hipEvent_t>(event_stop_));
   if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
+}
+
+void DeviceEngine::setProfiling(bool on) {
+  check(hipSetDevice(device_), "hipSetDevice");
+  if (on && !event_start_) {
+    hipEvent_t a, b;
+    check(hipEventCreate(&a), "hipEventCreate");
+    check(hipEventCreate(&b), "hipEventCreate");
+    event_start_ = a;
+    event_stop_ = b;
+  }
+  profiling_ = on;
+  timed_launches_ = 0;
+  timed_ms_ = 0;
+  event_pending_ = false;
+}
+
+// after the stream has drained: add the last timed launch
+void DeviceEngine::collectTiming() {
+  if (!event_pending_) return;
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(event_start_), static_cast<hipEvent_t>(event_stop_)),
+        "hipEventElapsedTime");
+  timed_ms_ += ms;
+  ++timed_launches_;
+  event_pending_ = false;
+}
+
+// distinct table rows referenced by the staged stencils
+void DeviceEngine::countRows() {
+  row_seen_.assign(n_rows_, 0);
+  size_t count = 0;
+  for (size_t k = 0; k < nb_levels_ * species_stride_; ++k) {
+    const LevelSpecies& d = species_host_[k];
+    if (d.n == 0.0) continue;
+    const uint32_t r[4] = {d.row[0], d.row[1], d.row[2], d.row[3]};
+    const bool use[4] = {true, r[1] != r[0], r[2] != r[0], r[2] != r[0] && r[3] != r[2]};
+    for (int q = 0; q < 4; ++q)
+      if (use[q] && r[q] < n_rows_ && !row_seen_[r[q]]) { row_seen_[r[q]] = 1; ++count; }
+  }
+  distinct_rows_ = count;
 }
 
 void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<double>& lam) {
@@ -203,8 +349,15 @@ void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<d
   wavenumber_ = deviceAlloc<double>(n_points_, "wavenumbers");
   wavelength_ = deviceAlloc<double>(n_points_, "wavelengths");
   flux_ = deviceAlloc<double>(n_points_, "flux");
-  check(hipMemcpy(wavenumber_, nu.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice), "copy wavenumbers");
-  check(hipMemcpy(wavelength_, lam.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice), "copy wavelengths");
+  // set-up copies go through the engine's own (non-blocking) stream and are
+  // waited for: a pageable hipMemcpy may return before its DMA has landed,
+  // and nothing would order it before kernels on this stream
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(wavenumber_, nu.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy wavenumbers");
+  check(hipMemcpyAsync(wavelength_, lam.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy wavelengths");
+  check(hipStreamSynchronize(s), "copy grid");
 }
 
 void DeviceEngine::uploadTables(const TableStore& store) {
@@ -212,10 +365,14 @@ void DeviceEngine::uploadTables(const TableStore& store) {
   if (store.nb_rows > 0 && store.nb_points != n_points_) raise(kInvalidInput, "table rows do not match the grid");
   deviceFree(tables_);
   tables_ = deviceAlloc<double>(size_t(store.nb_rows) * stride_, "cross-section tables");
+  n_rows_ = store.nb_rows;
+  hipStream_t s = static_cast<hipStream_t>(stream_);
   if (store.nb_rows > 0)
-    check(hipMemcpy2D(tables_, stride_ * sizeof(double), store.log_cross_sections.data(), n_points_ * sizeof(double),
-                      n_points_ * sizeof(double), store.nb_rows, hipMemcpyHostToDevice),
+    check(hipMemcpy2DAsync(tables_, stride_ * sizeof(double), store.log_cross_sections.data(),
+                           n_points_ * sizeof(double), n_points_ * sizeof(double), store.nb_rows,
+                           hipMemcpyHostToDevice, s),
           "copy cross-section tables");
+  check(hipStreamSynchronize(s), "copy cross-section tables");
 }
 
 void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) {
@@ -251,16 +408,21 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
   bins_desc_ = deviceAlloc<BinDesc>(desc.size(), "bin descriptors");
   bins_ = deviceAlloc<double>(n_bins_, "bins");
   ll_terms_ = deviceAlloc<double>(n_bins_, "likelihood terms");
-  check(hipMemcpy(weights_, weights.data(), weights.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
-  check(hipMemcpy(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice), "copy bins");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(weights_, weights.data(), weights.size() * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy weights");
+  check(hipMemcpyAsync(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, s),
+        "copy bins");
+  check(hipStreamSynchronize(s), "copy observations");
 }
 
 void DeviceEngine::configure(size_t nb_levels, size_t nb_species) {
   check(hipSetDevice(device_), "hipSetDevice");
   nb_levels_ = nb_levels;
   nb_species_ = nb_species;
+  species_stride_ = (nb_species + kChunk - 1) / kChunk * kChunk;   // padding entries keep n = 0, row 0
   const size_t lv = nb_levels * sizeof(LevelState);
-  const size_t sp = nb_levels * nb_species * sizeof(LevelSpecies);
+  const size_t sp = nb_levels * species_stride_ * sizeof(LevelSpecies);
   const size_t sh = (n_obs_ + 1) * sizeof(double);
   staging_bytes_ = lv + sp + sh;
   if (staging_host_) (void)hipHostFree(staging_host_);
@@ -272,6 +434,7 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species) {
   staging_dev_ = deviceAlloc<unsigned char>(staging_bytes_, "per-evaluation inputs");
   levels_host_ = reinterpret_cast<LevelState*>(staging_host_);
   species_host_ = reinterpret_cast<LevelSpecies*>(staging_host_ + lv);
+  for (size_t s = 0; s < species_stride_; ++s) species_host_[s].reload = 1;   // level 0 always loads
   shift_host_ = reinterpret_cast<double*>(staging_host_ + lv + sp);
   ll_host_ = reinterpret_cast<double*>(staging_host_ + staging_bytes_);
 }
@@ -281,15 +444,37 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
   hipStream_t s = static_cast<hipStream_t>(stream_);
   check(hipMemcpyAsync(staging_dev_, staging_host_, staging_bytes_, hipMemcpyHostToDevice, s), "upload inputs");
   const size_t lv = nb_levels_ * sizeof(LevelState);
-  const size_t sp = nb_levels_ * nb_species_ * sizeof(LevelSpecies);
+  const size_t sp = nb_levels_ * species_stride_ * sizeof(LevelSpecies);
   const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
   const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + lv);
   const double* shifts = reinterpret_cast<const double*>(staging_dev_ + lv + sp);
   const int n = int(n_points_);
   const int blocks = (n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, wavenumber_, wavelength_, tables_, stride_,
-                     lev, spc, int(nb_levels_), int(nb_species_), n, scaling, flux_, spectrum_dev);
+  if (profiling_) {
+    countRows();
+    check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
+  }
+  switch (species_stride_) {
+#define BEAR_CACHED(NS)                                                                                         \
+  case NS:                                                                                                      \
+    hipLaunchKernelGGL(emissionKernelCached<NS>, dim3(blocks), dim3(kBlock), 0, s, wavenumber_, wavelength_,    \
+                       tables_, stride_, lev, spc, int(nb_levels_), n, scaling, flux_, spectrum_dev);           \
+    break;
+    BEAR_CACHED(4)
+    BEAR_CACHED(8)
+    BEAR_CACHED(12)
+    BEAR_CACHED(16)
+#undef BEAR_CACHED
+    default:
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, wavenumber_, wavelength_, tables_,
+                         stride_, lev, spc, int(nb_levels_), int(species_stride_ / kChunk), n, scaling, flux_,
+                         spectrum_dev);
+  }
   check(hipGetLastError(), "emission kernel");
+  if (profiling_) {
+    check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), s), "hipEventRecord");
+    event_pending_ = true;
+  }
   if (n_bins_ > 0) {
     hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlock), 0, s, flux_, weights_, bins_desc_, shifts,
                        inflation, bins_, ll_terms_);
@@ -310,6 +495,7 @@ double DeviceEngine::logLikelihood() {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   check(hipMemcpyAsync(ll_host_, ll_, sizeof(double), hipMemcpyDeviceToHost, s), "read log-likelihood");
   check(hipStreamSynchronize(s), "evaluation");
+  collectTiming();
   return *ll_host_;
 }
 
@@ -318,6 +504,7 @@ void DeviceEngine::copyBins(std::vector<double>& out) {
   out.resize(n_bins_);
   if (n_bins_) check(hipMemcpyAsync(out.data(), bins_, n_bins_ * sizeof(double), hipMemcpyDeviceToHost, s), "read bins");
   check(hipStreamSynchronize(s), "evaluation");
+  collectTiming();
 }
 
 void DeviceEngine::copySpectrum(double* host, const double* spectrum_dev) {
@@ -336,6 +523,7 @@ double* DeviceEngine::spectrumBuffer() {
 
 void DeviceEngine::synchronize() {
   check(hipStreamSynchronize(static_cast<hipStream_t>(stream_)), "evaluation");
+  collectTiming();
 }
 
 }  // namespace bear_amd

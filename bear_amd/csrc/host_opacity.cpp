@@ -122,8 +122,12 @@ void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const Spectral
   const size_t N = grid.size();
   const uint32_t first_row = store.nb_rows;
   store.log_cross_sections.resize((size_t(first_row) + entries.size()) * N);
-  std::vector<float> raw;
+  // rows are independent: sample, scale, floor and take log10 in parallel
+  bool missing = false;
+  std::string missing_name;
+  #pragma omp parallel for schedule(dynamic, 1)
   for (size_t r = 0; r < entries.size(); ++r) {
+    std::vector<float> raw;
     const float* src;
     size_t n_full;
     if (reg) {
@@ -131,7 +135,11 @@ void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const Spectral
       n_full = reg->nb_wavenumbers;
     } else {
       std::ifstream bf(dir + "/" + entries[r].file, std::ios::binary | std::ios::ate);
-      if (!bf) raise(kFileNotFound, "cross-section file not found: " + dir + "/" + entries[r].file);
+      if (!bf) {
+        #pragma omp critical
+        { missing = true; missing_name = entries[r].file; }
+        continue;
+      }
       n_full = size_t(bf.tellg()) / sizeof(float);
       raw.resize(n_full);
       bf.seekg(0);
@@ -151,6 +159,7 @@ void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const Spectral
       dst[i] = std::log10(v);
     }
   }
+  if (missing) raise(kFileNotFound, "cross-section file not found: " + dir + "/" + missing_name);
   // order rows by temperature, then pressure (opacity_species.cpp:129-157)
   std::map<double, std::vector<TableRow>> by_t;
   for (size_t r = 0; r < entries.size(); ++r)

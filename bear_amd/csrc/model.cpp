@@ -40,7 +40,7 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
   const double* mods = temp + nb_temp_;
   const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
 
-  const size_t L = atmosphere_.nb_levels, S = species_.size();
+  const size_t L = atmosphere_.nb_levels, S = species_.size(), stride = engine_.speciesStride();
   LevelState* lev = engine_.levels();
   LevelSpecies* spc = engine_.levelSpecies();
   for (size_t l = 0; l < L; ++l) {
@@ -50,14 +50,19 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
       const OpacitySpecies& sp = species_[s];
       double n = atmosphere_.nd(l, sp.species_id);                 // opacity_species.cpp:407-412
       for (int q : sp.cia_partners) n *= atmosphere_.nd(l, q);
-      LevelSpecies& d = spc[l * S + s];
-      d.n = n;
-      if (n == 0) continue;
+      LevelSpecies& d = spc[l * stride + s];
+      d.n = n;                                   // rows stay valid when n == 0: the kernel loads them
       const Stencil st = sp.bracket(atmosphere_.pressure[l], t);
       d.f_p_lower = st.f_p_lower;
       d.f_p_upper = st.f_p_upper;
       d.f_t = st.f_t;
-      for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
+      // reload the rows only where they differ from the previous level's
+      uint32_t changed = l == 0;
+      for (int k = 0; k < 4; ++k) {
+        if (l > 0) changed |= spc[(l - 1) * stride + s].row[k] != st.row[k];
+        d.row[k] = st.row[k];
+      }
+      d.reload = changed;
     }
   }
   size_t k = 0;                                                   // forward_model.cpp:184-204

@@ -69,6 +69,7 @@ class Retrieval {
   // ForwardModel::calcModel (forward_model.cpp:60-127) with host outputs
   bool computeModel(const double* parameters, size_t n, bool high_res, double* spectrum, double* spectrum_obs);
   bool calcModelGPU(const double* parameters, size_t n, double* spectrum_dev, double* const* spectrum_obs_dev);
+  DeviceEngine& engine() const { return *engine_; }
 
  private:
   double errorInflation(const double* parameters, size_t n) const;

@@ -135,11 +135,10 @@ def _lorentz_rows(nu, centres, weights_per_row, widths, backend="numpy"):
     if backend == "torch":
         import torch
         dev = torch.device("cuda")
-        idx_t = torch.as_tensor(idx, device=dev)
         f_t = torch.as_tensor(freqs, device=dev)
         for r in range(weights_per_row.shape[0]):
-            hist = torch.zeros(m, dtype=torch.float64, device=dev)
-            hist.index_add_(0, idx_t, torch.as_tensor(weights_per_row[r], device=dev))
+            # histogram on the CPU (deterministic), transforms on the GPU
+            hist = torch.as_tensor(np.bincount(idx, weights=weights_per_row[r], minlength=m), device=dev)
             spec = torch.fft.rfft(hist)
             spec *= torch.exp(-2.0 * math.pi * widths[r] * f_t)
             out[r] = (torch.fft.irfft(spec, n=m)[:n] / step).cpu().numpy()

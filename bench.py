@@ -1,0 +1,221 @@
+#!/usr/bin/env python
+"""Likelihood evaluations per second of the MI355X engine on one GPU.
+
+Workload (one "step" = one likelihood evaluation, the plain single-model
+GPU emission configuration): 100 levels log-spaced 1e2..1e-6 bar, poly 6 1
+temperature profile, 6 absorbers with 30 T x 9 p log10 cross-section tables
+plus H2-H2 / H2-He CIA and H2 Rayleigh, 400,000 wavenumbers at 0.025 cm^-1,
+300 photometric bins each with a seeded 50-point bandpass.  Tables are
+synthetic (seeded Lorentzian line lists), resident in HBM before timing.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (fused opacity + radiative transfer, timed with HIP events on its own
+stream) and a CPU baseline timed on this host's cores.
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "likelihood evals/sec (batched+adjoint vs plain GPU) at ≤10 ppm spectral error"
+HBM_PEAK_GBS = 8000.0
+FULL_N = 400_000
+SAMPLE_N = 20_000
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_case(n_wavenumbers, backend):
+    from bear_amd import synth
+    return synth.emission_c2(backend=backend, n_wavenumbers=n_wavenumbers)
+
+
+def open_with_injected_data(case, scratch):
+    """Retrieval whose observed data are the engine's own model at the case's
+    truth parameters plus 2 % noise; returns (retrieval, nearby live point)."""
+    from bear_amd import synth
+    r = synth.open_retrieval_in_memory(case, scratch)
+    truth = case.parameters.copy()
+    synth.inject_data(case, r.computeModel(truth, False).spectrum_obs)
+    r.close()
+    r = synth.open_retrieval_in_memory(case, scratch)
+    return r, synth.perturbed(truth, case, seed=5)
+
+
+def isothermal_check(r, params):
+    """Size-independent property at full size: an isothermal atmosphere emits
+    pi B(T) 1e-3 scale at every wavenumber (returns the max relative error)."""
+    from bear_amd import synth
+    p = params.copy()
+    nb_mix = 6
+    p[3 + nb_mix + 1:3 + nb_mix + 7] = 1.0
+    out = r.computeModel(p, True)
+    nu, lam = r.spectral_grid()
+    h, c, kb = synth.PLANCK_H, synth.LIGHT_C, synth.BOLTZMANN_K
+    t = p[3 + nb_mix]
+    b = 2 * h * c * c * nu ** 3 / np.expm1(h * c * nu / (kb * t))
+    scale = (synth.RADIUS_JUPITER / p[2]) ** 2 * p[1]
+    expected = np.pi * b * 1e-3 * scale / lam ** 2 * 1e4
+    return float(np.max(np.abs(out.spectrum - expected) / expected))
+
+
+def traffic_from_profiles(kernel="emissionKernel"):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch,
+    scaled by the copy-kernel calibration recorded with them)."""
+    path = os.path.join(ROOT, "profiles", "emission_c2_counters.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return float(d[kernel]["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(threads, scratch):
+    """The reference's own CPU path (oracle/_ref build) on a bounded sample:
+    the same configuration at 20,000 wavenumbers, scaled to 400,000 by the
+    per-wavenumber cost; the oracle restatement stands in when the reference
+    build is absent."""
+    from bear_amd import synth
+    from oracle import api as oracle_api
+    case = build_case(SAMPLE_N, "numpy")
+    folder = synth.write_case(case, os.path.join(scratch, "cpu_sample"))
+    truth = case.parameters.copy()
+    o = oracle_api.EmissionOracle(folder)
+    synth.inject_data(case, o.evaluate(truth)["bins"])
+    folder = synth.write_case(case, folder)
+    params = synth.perturbed(truth, case, seed=5)
+    with open(os.path.join(folder, "case.txt")) as f:
+        lines = [l for l in f.read().splitlines() if not l.startswith("parameters ")]
+    lines.append(f"parameters {params.size} " + " ".join(repr(float(v)) for v in params))
+    with open(os.path.join(folder, "case.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    n_evals = 12
+    if os.path.exists(oracle_api.REF_BIN):
+        out = oracle_api.run_reference(folder, os.path.join(scratch, "cpu_out"), n_timed=n_evals, threads=threads)
+        t = float(np.median(out["timings"]))
+        kind = "reference"
+    else:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        o = oracle_api.EmissionOracle(folder)
+        o.evaluate(params)
+        ts = []
+        for _ in range(n_evals):
+            t0 = time.perf_counter()
+            o.evaluate(params)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        kind = "port"
+    n_sample = o.nb_points if kind == "port" else oracle_api.EmissionOracle(folder).nb_points
+    return {
+        "value": (1.0 / t) * n_sample / FULL_N,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"same emission configuration at {n_sample} wavenumbers (1000-1500 cm^-1), median of "
+                  f"{n_evals} evaluations = {t * 1e3:.1f} ms, scaled by {n_sample}/{FULL_N} wavenumbers",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the likelihood path on a single GPU "
+              "(the multi-GPU sharding is a design, not a run); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    from bear_amd import api
+
+    with tempfile.TemporaryDirectory() as scratch:
+        log("generating the 400,000-wavenumber tables")
+        t0 = time.time()
+        case = build_case(FULL_N, "torch")
+        log(f"tables ready in {time.time() - t0:.1f} s; building the engine")
+        r, params = open_with_injected_data(case, os.path.join(scratch, "gpu"))
+        del case
+        iso_err = isothermal_check(r, params)
+        log(f"isothermal identity at full size: max rel err {iso_err:.2e}")
+
+        for _ in range(args.warmup):
+            r.computeLikelihood(params)
+        r.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ll = r.computeLikelihood(params)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        st = r.stats()
+        r.set_profiling(False)
+        log(f"{args.steps} evaluations in {elapsed:.3f} s, log L = {ll:.6f}")
+
+        n = st["nb_spectral_points"]
+        kernel_s = st["kernel_ms_total"] / 1e3 / max(st["kernel_launches"], 1)
+        # compulsory bytes of one launch: every distinct table row once, the
+        # wavenumbers, the flux written; per-level inputs are a few tens of KB
+        algorithmic = st["distinct_table_rows"] * n * st["table_value_bytes"] + n * 8 + n * 8 \
+            + st["nb_levels"] * (32 + st["nb_species"] * 48)
+        achieved = algorithmic / kernel_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                    "kernel": "emissionKernel", "kernel_ms": round(kernel_s * 1e3, 4),
+                    "algorithmic_bytes": int(algorithmic), "distinct_table_rows": st["distinct_table_rows"]}
+
+        cpu = None
+        if not args.no_cpu_baseline:
+            threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+            log(f"CPU baseline on {threads} threads")
+            try:
+                cpu = cpu_baseline(threads, scratch)
+            except Exception as e:  # the GPU figure stands without it
+                log(f"CPU baseline failed: {e}")
+
+    value = args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "emission_c2: one emission likelihood evaluation, 100 levels, 6 absorbers + "
+                               "H2-H2/H2-He CIA + H2 Rayleigh, 400,000 wavenumbers, 30T x 9p tables, "
+                               "300 bandpass bins", "nb_spectral_points": n, "batch": 1,
+                   "table_dtype": "f64", "levels": st["nb_levels"], "bins": st["nb_bins"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "checks": {"isothermal_max_rel_err": iso_err},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -158,6 +158,26 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
                                       double* spectrum_dev, double* const* spectrum_obs_dev,
                                       int* neglect_model);
 
+/* ---------------------------------------------------------------- measurement
+ * No counterpart in the reference (it has no timing code).  With profiling
+ * on, HIP events on the retrieval's stream time every launch of the fused
+ * opacity + radiative-transfer kernel; the statistics also report how many
+ * distinct table rows the last evaluation's stencils touched, which fixes
+ * the kernel's compulsory HBM traffic. */
+typedef struct bear_eval_stats {
+  size_t nb_spectral_points;
+  size_t nb_levels;
+  size_t nb_species;             /* absorbers with tables in the evaluation */
+  size_t nb_bins;                /* observation points, all observations */
+  size_t distinct_table_rows;    /* rows referenced by the last evaluation */
+  size_t table_value_bytes;      /* bytes per stored table value */
+  size_t kernel_launches;        /* timed launches since profiling was enabled */
+  double kernel_ms_total;        /* their summed duration */
+} bear_eval_stats;
+
+int bear_retrieval_set_profiling(bear_retrieval* r, int enable);
+int bear_retrieval_stats(const bear_retrieval* r, bear_eval_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

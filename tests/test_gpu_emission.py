@@ -120,3 +120,17 @@ def test_repeated_evaluations_are_deterministic(case_factory):
     r, _, params, _ = retrieval(case_factory, "emission_c1")
     lls = [r.computeLikelihood(params[1]) for _ in range(5)]
     assert len(set(lls)) == 1
+
+
+def test_first_evaluation_after_setup(case_factory):
+    """The first evaluation of a fresh retrieval sees the uploaded tables
+    (set-up copies and kernels share one stream, and set-up waits for them)."""
+    folder, params = case_factory("emission_c1")
+    o = oracle_api.EmissionOracle(folder)
+    for k in range(2):
+        r = synth.open_retrieval(folder)
+        out = r.computeModel(params[1], True)          # first call on this retrieval
+        ref = o.evaluate(params[1])
+        rel = np.max(np.abs(out.spectrum - ref["spectrum"]) / np.abs(ref["spectrum"]))
+        assert rel <= SPECTRUM_RTOL, (k, rel)
+        r.close()
