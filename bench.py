@@ -79,9 +79,12 @@ def traffic_from_profiles(kernel="emissionKernel"):
         return None
     try:
         d = json.load(open(path))
-        return float(d[kernel]["hbm_bytes_per_launch"])
+        for name, v in d.items():
+            if name.startswith(kernel):
+                return float(v["hbm_bytes_per_launch"])
     except Exception:
-        return None
+        pass
+    return None
 
 
 def cpu_baseline(threads, scratch):
