@@ -39,7 +39,9 @@ class _GlobalConfig(C.Structure):
 class _ModelConfig(C.Structure):
     _fields_ = [("nb_grid_points", C.c_size_t), ("atmos_bottom_pressure", C.c_double),
                 ("atmos_top_pressure", C.c_double), ("temperature_profile", C.c_char_p),
-                ("radiative_transfer", C.c_char_p), ("chemistry", C.c_char_p), ("opacity_species", C.c_char_p)]
+                ("radiative_transfer", C.c_char_p), ("chemistry", C.c_char_p), ("opacity_species", C.c_char_p),
+                ("fit_mean_molecular_weight", C.c_int), ("fit_scale_height", C.c_int),
+                ("use_variable_gravity", C.c_int), ("stellar_spectrum_model", C.c_char_p)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -138,13 +140,20 @@ class ModelConfig:
     chemistry: list                          # ["iso H2O CH4", "bg H2He"]
     opacity_species: list                    # [("H2O", "Molecules/H2O"), ...]
     radiative_transfer: str = "scm"
+    fit_mean_molecular_weight: bool = False  # transmission
+    fit_scale_height: bool = False           # transmission
+    use_variable_gravity: bool = False       # transmission
+    stellar_spectrum_model: str = "blackbody"  # secondary eclipse
 
     def _c(self):
         chem = "\n".join(self.chemistry)
         opac = "\n".join(f"{s} {f}" for s, f in self.opacity_species)
-        self._keep = [_b(self.temperature_profile), _b(self.radiative_transfer), _b(chem), _b(opac)]
+        self._keep = [_b(self.temperature_profile), _b(self.radiative_transfer), _b(chem), _b(opac),
+                      _b(self.stellar_spectrum_model)]
         return _ModelConfig(int(self.nb_grid_points), float(self.atmos_bottom_pressure),
-                            float(self.atmos_top_pressure), self._keep[0], self._keep[1], self._keep[2], self._keep[3])
+                            float(self.atmos_top_pressure), self._keep[0], self._keep[1], self._keep[2], self._keep[3],
+                            int(self.fit_mean_molecular_weight), int(self.fit_scale_height),
+                            int(self.use_variable_gravity), self._keep[4])
 
 
 @dataclass

@@ -85,6 +85,10 @@ int bear_retrieval_create(const bear_global_config* config, const char* forward_
     m.temperature = str(model_config->temperature_profile);
     m.radiative_transfer = str(model_config->radiative_transfer);
     m.chemistry = str(model_config->chemistry);
+    m.fit_mean_molecular_weight = model_config->fit_mean_molecular_weight != 0;
+    m.fit_scale_height = model_config->fit_scale_height != 0;
+    m.use_variable_gravity = model_config->use_variable_gravity != 0;
+    if (model_config->stellar_spectrum_model) m.stellar_model = model_config->stellar_spectrum_model;
     std::istringstream op(str(model_config->opacity_species));
     std::string line;
     while (std::getline(op, line)) {

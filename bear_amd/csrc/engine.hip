@@ -4,12 +4,14 @@
 //   wavenumber, wavelength   [N] FP64
 //   tables                   [rows][stride] FP64 log10 cross-sections, every
 //                            species' (T, p) rows, wavenumber contiguous
-//   flux                     [N] FP64, the emergent flux of the last evaluation
+//   rayleigh                 [N] FP64 Rayleigh cross-section (transmission)
+//   flux                     [N] FP64, the high-resolution spectrum of the
+//                            last evaluation (flux, or depth in ppm)
 //   weights                  folded observation operator, one contiguous
 //                            segment per bin
 //   bins, ll_terms           [bins] FP64
-// Per evaluation the host fills one pinned block (levels, stencils, shifts)
-// that is copied in one transfer.
+// Per evaluation the host fills one pinned block (levels, stencils, shifts,
+// chord lengths) that is copied in one transfer.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,8 +40,11 @@ constexpr double kW = 0.5;
 // Planck constants in the reference's association order
 // (additional/aux_functions.cpp:272-278): 2 h c c nu^3 / (exp(h nu c / k / T) - 1)
 constexpr double kPlanckC1 = 2. * 6.62606896e-27 * 2.99792458e10 * 2.99792458e10;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kLog2Ten = 3.321928094887362347870319429489390175864831393;
+constexpr double kLog2E = 1.442695040888963407359924681001892137426645954;
 
-// 2^t in FP64 with relative error below 2e-15: t = n + f, |f| <= 1/2,
+// 2^t in FP64 with relative error below 3.1e-15: t = n + f, |f| <= 1/2,
 // 2^f from its degree-10 Chebyshev interpolant on [-1/2, 1/2] (coefficients
 // in monomial form), then an exponent shift.  t is clamped so that the shift
 // stays in range: results below 2^-1100 flush to 0, above 2^1100 become inf.
@@ -61,9 +66,6 @@ __device__ __forceinline__ double pow2(double t) {
   return __builtin_amdgcn_ldexp(p, int(n));
 }
 
-constexpr double kLog2Ten = 3.321928094887362347870319429489390175864831393;
-constexpr double kLog2E = 1.442695040888963407359924681001892137426645954;
-
 // 1/d for finite non-zero d: hardware estimate and two Newton steps
 __device__ __forceinline__ double recip(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -72,15 +74,64 @@ __device__ __forceinline__ double recip(double d) {
   return r;
 }
 
+// Planck function per wavenumber written as c1 nu^3 q / (1 - q),
+// q = e^(-h nu c / k T): the reference's expression without exp overflow
+__device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv_t) {
+  const double q = pow2(x_nu * inv_t);
+  return b_num * q * recip(1.0 - q);
+}
+
+// blackbody star flux pi B(T) 1e-3 (star_blackbody.cpp:40-54)
+__device__ __forceinline__ double starFlux(double nu, double inv_t) {
+  const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+  return planckAt(kPlanckC1 * nu * nu * nu, x_nu, inv_t) * kPi * 1e-3;
+}
+
+// One layer of the two-angle short characteristics (short_characteristics.cpp:88-114)
+__device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_prev, double b_next,
+                                                          double& i1, double& i2) {
+  if (tau != 0) {
+    const double r = recip(tau);
+    const double a1 = pow2(tau * (-kLog2E / kMu1));
+    const double q1 = (a1 - 1.0) * (kMu1 * r);
+    i1 = fma(i1, a1, (1.0 + q1) * b_next + (-a1 - q1) * b_prev);
+    const double a2 = pow2(tau * (-kLog2E / kMu2));
+    const double q2 = (a2 - 1.0) * (kMu2 * r);
+    i2 = fma(i2, a2, (1.0 + q2) * b_next + (-a2 - q2) * b_prev);
+  }
+}
+
 // kappa_l(nu) = sum_s n_s 10^x_s, x_s the bilinear log10 cross-section of
-// species s at this level (opacity_species.cpp:277-345, 400-433).  Species
-// come in chunks of kChunk with every row of the chunk loaded before any is
-// used, so the loads of a chunk are in flight together.  Rows are always
-// valid; a zero density or a zero weight removes a term exactly.
+// species s at this level (opacity_species.cpp:277-345, 400-433), over NS
+// (padded) species whose four stencil values v[s][] stay in registers; rows
+// are reloaded where `reload` is set.  Rows are always valid; a zero density
+// or a zero weight removes a term exactly.
+template <int NS>
+__device__ __forceinline__ double cachedKappa(const double* __restrict__ tab, size_t stride,
+                                              const LevelSpecies* __restrict__ d, bool force, double (&v)[NS][4]) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    if (force || d[s].reload) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+    }
+  double k = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
+    const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
+    const double x = fma(hi - lo, d[s].f_t, lo);
+    k = fma(d[s].n, pow2(x * kLog2Ten), k);
+  }
+  return k;
+}
+
+// The same without a register cache, for any number of species: chunks of
+// kChunk species with every row of a chunk loaded before any is used.
 constexpr int kChunk = 4;
 
-__device__ __forceinline__ double levelKappa(const double* __restrict__ tab, size_t stride,
-                                             const LevelSpecies* __restrict__ d, int nb_chunks) {
+__device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, size_t stride,
+                                               const LevelSpecies* __restrict__ d, int nb_chunks) {
   double k = 0.0;
   for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
     double v[kChunk][4];
@@ -99,48 +150,43 @@ __device__ __forceinline__ double levelKappa(const double* __restrict__ tab, siz
   return k;
 }
 
-__device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv_t) {
-  const double q = pow2(x_nu * inv_t);
-  return b_num * q * recip(1.0 - q);
-}
+// Emission spectrum epilogue: the flux F = 2 pi (mu1 w1 I1 + mu2 w2 I2) 1e-3 s
+// (short_characteristics.cpp:117-122) is kept for the bins; the optional
+// high-resolution output is F per micron (radiative_transfer.h:58-60) for
+// emission (mode 0) or the eclipse depth F / F_star (Rp/Rs)^2 1e6
+// (secondary_eclipse.cpp:249-251, mode 1).
+struct SpectrumOut {
+  int mode;
+  double star_inv_temperature;
+  double depth_scale;
+};
 
-// One layer of the two-angle short characteristics (short_characteristics.cpp:88-114)
-__device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_prev, double b_next,
-                                                          double& i1, double& i2) {
-  if (tau != 0) {
-    const double r = recip(tau);
-    const double a1 = pow2(tau * (-kLog2E / kMu1));
-    const double q1 = (a1 - 1.0) * (kMu1 * r);
-    i1 = fma(i1, a1, (1.0 + q1) * b_next + (-a1 - q1) * b_prev);
-    const double a2 = pow2(tau * (-kLog2E / kMu2));
-    const double q2 = (a2 - 1.0) * (kMu2 * r);
-    i2 = fma(i2, a2, (1.0 + q2) * b_next + (-a2 - q2) * b_prev);
-  }
-}
-
-__device__ __forceinline__ void writeFlux(int i, double i1, double i2, double scaling, const double* wavelength,
-                                          double* flux, double* spectrum) {
-  const double f = 2.0 * 3.14159265358979323846 * (i1 * kMu1 * kW + i2 * kMu2 * kW) * 1e-3 * scaling;
+__device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2, double scaling,
+                                          const double* wavelength, const SpectrumOut& out, double* flux,
+                                          double* spectrum) {
+  const double f = 2.0 * kPi * (i1 * kMu1 * kW + i2 * kMu2 * kW) * 1e-3 * scaling;
   flux[i] = f;
-  if (spectrum) {                      // radiative_transfer.h:58-60, per micron
-    const double lam = wavelength[i];
-    spectrum[i] = f / lam / lam * 10000.0;
+  if (spectrum) {
+    if (out.mode == 1) {
+      spectrum[i] = f * recip(starFlux(nu, out.star_inv_temperature)) * out.depth_scale;
+    } else {
+      const double lam = wavelength[i];
+      spectrum[i] = f / lam / lam * 10000.0;
+    }
   }
 }
 
 // Fused opacity + emission radiative transfer, one wavenumber per thread,
-// for NS (padded) species.  The four stencil values of every species stay
-// in registers from level to level and are reloaded only where the host
-// marks a stencil change, so each table row is read about once per thread
-// instead of once per level.  Planck as c1 nu^3 q / (1 - q),
-// q = e^(-h nu c / k T) (aux_functions.cpp:272-278).  kappa never leaves
-// registers.
+// levels bottom to top (short_characteristics.cpp:39-125).  The four stencil
+// values of every species stay in registers from level to level, so each
+// table row is read about once per thread; kappa and the intensities never
+// leave registers.
 template <int NS>
 __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
     const double* __restrict__ tables, size_t stride,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
-    int nb_levels, int n, double scaling, double* __restrict__ flux, double* __restrict__ spectrum) {
+    int nb_levels, int n, double scaling, SpectrumOut out, double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const double nu = wavenumber[i];
@@ -148,29 +194,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   double v[NS][4];
-  auto kappa = [&](const LevelSpecies* __restrict__ d) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-      if (d[s].reload) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
-      }
-    double k = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-      const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-      const double x = fma(hi - lo, d[s].f_t, lo);
-      k = fma(d[s].n, pow2(x * kLog2Ten), k);
-    }
-    return k;
-  };
-  double k_prev = kappa(species);
+  double k_prev = cachedKappa<NS>(tab, stride, species, true, v);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = kappa(species + size_t(l) * NS);
+    const double k_next = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, false, v);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -178,16 +207,15 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     b_prev = b_next;
     z_prev = z_next;
   }
-  writeFlux(i, i1, i2, scaling, wavelength, flux, spectrum);
+  writeFlux(i, nu, i1, i2, scaling, wavelength, out, flux, spectrum);
 }
 
-// The same for any number of species: chunks of kChunk species, every row
-// of a chunk loaded before any is used, no register cache.
+// The same for any number of species, without the register cache.
 __global__ __launch_bounds__(kBlock) void emissionKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
     const double* __restrict__ tables, size_t stride,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
-    int nb_levels, int nb_chunks, int n, double scaling,
+    int nb_levels, int nb_chunks, int n, double scaling, SpectrumOut out,
     double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -196,12 +224,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   const int stride_species = nb_chunks * kChunk;
-  double k_prev = levelKappa(tab, stride, species, nb_chunks);
+  double k_prev = chunkedKappa(tab, stride, species, nb_chunks);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = levelKappa(tab, stride, species + size_t(l) * stride_species, nb_chunks);
+    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, nb_chunks);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -209,7 +237,80 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
     b_prev = b_next;
     z_prev = z_next;
   }
-  writeFlux(i, i1, i2, scaling, wavelength, flux, spectrum);
+  writeFlux(i, nu, i1, i2, scaling, wavelength, out, flux, spectrum);
+}
+
+// Transmission: transit depth per wavenumber (transmission_spectrum.cpp:32-154).
+// A workgroup owns `tile` consecutive wavenumbers and stages their extinction
+// coefficients for every level in LDS (K[level][tile]).
+//  1. extinction: thread (level group g, wavenumber j) computes
+//     kappa_abs + sigma_R n_R for its contiguous range of levels, with the
+//     register-cached stencils;
+//  2. tangent optical depths: the (wavenumber, tangent level) pairs are spread
+//     over the workgroup; each sums chord length x (k_i + k_i+1) outward from
+//     its tangent level, in the reference's order, and stops once tau > 10
+//     (transmission_spectrum.cpp:112-139); exp(-tau) goes to LDS (T[level][tile]);
+//  3. per wavenumber, the effective tangent height (trapezoid in the
+//     reference's order) and the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
+// Chord lengths sqrt((R+z_i)^2 - (R+z_t)^2) differences come from the host.
+template <int NS>
+__global__ __launch_bounds__(kBlock) void transmissionKernel(
+    const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    const double* __restrict__ path, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
+    double* __restrict__ depth, double* __restrict__ spectrum) {
+  extern __shared__ double lds[];
+  double* K = lds;
+  double* T = lds + size_t(nb_levels) * tile;
+  const int groups = kBlock / tile;
+  const int j = threadIdx.x % tile;
+  const int g = threadIdx.x / tile;
+  const int i = blockIdx.x * tile + j;
+  const int chunk = (nb_levels + groups - 1) / groups;
+  const int l0 = g * chunk;
+  const int l1 = min(nb_levels, l0 + chunk);
+  if (i < n) {
+    const double* tab = tables + i;
+    const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
+    double v[NS][4];
+    for (int l = l0; l < l1; ++l) {
+      const double k = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, l == l0, v);
+      K[l * tile + j] = k + sigma_r * levels[l].rayleigh_density;
+    }
+  } else {
+    for (int l = l0; l < l1; ++l) K[l * tile + j] = 0.0;
+  }
+  __syncthreads();
+  const int nb_pairs = tile * (nb_levels - 1);
+  for (int p = threadIdx.x; p < nb_pairs; p += kBlock) {
+    const int jj = p % tile;
+    const int t = p / tile;
+    const double* chord = path + size_t(t) * nb_levels;
+    double tau = 0;
+    for (int q = t; q < nb_levels - 1; ++q) {
+      tau += chord[q] * (K[q * tile + jj] + K[(q + 1) * tile + jj]);
+      if (tau > 10) break;
+    }
+    T[t * tile + jj] = pow2(-tau * kLog2E);
+  }
+  if (threadIdx.x < tile) T[(nb_levels - 1) * tile + threadIdx.x] = 1.0;   // transparent top
+  __syncthreads();
+  if (threadIdx.x < tile && i < n) {
+    const double r0 = bottom_radius;
+    double h = 0;
+    double z_prev = levels[0].altitude;
+    for (int l = 0; l < nb_levels - 1; ++l) {
+      const double z_next = levels[l + 1].altitude;
+      h += 2. * ((r0 + z_prev) * (1. - T[l * tile + j]) + (r0 + z_next) * (1. - T[(l + 1) * tile + j])) *
+           (z_next - z_prev) * 0.5;
+      z_prev = z_next;
+    }
+    h = sqrt(h + r0 * r0) - r0;
+    const double r = h + r0;
+    const double d = r * r / star_radius / star_radius * 1e6;
+    depth[i] = d;
+    if (spectrum) spectrum[i] = d;
+  }
 }
 
 __device__ __forceinline__ double waveSum(double v) {
@@ -217,31 +318,41 @@ __device__ __forceinline__ double waveSum(double v) {
   return v;
 }
 
-// One workgroup per bin: dot product of the flux with the bin's folded
-// weights (fixed order: strided per thread, then a fixed tree), the
-// observation shift (forward_model.cpp:161-180), and the bin's Gaussian
+// One workgroup per bin: dot product of the high-resolution spectrum with the
+// bin's folded weights (fixed order: strided per thread, then a fixed tree).
+// Mode 1 (eclipse) also folds the blackbody star's flux and returns
+// planet / star (Rp/Rs)^2 1e6 (secondary_eclipse.cpp:231-246).  Then the
+// observation's shift (forward_model.cpp:161-180) and the bin's Gaussian
 // log-likelihood term (retrieval.cpp:330-358).
 __global__ __launch_bounds__(kBlock) void binKernel(
-    const double* __restrict__ flux, const double* __restrict__ weights, const BinDesc* __restrict__ bins,
-    const double* __restrict__ shifts, double error_inflation, double* __restrict__ out,
-    double* __restrict__ ll_terms) {
-  __shared__ double partial[kBlock / 64];
+    const double* __restrict__ flux, const double* __restrict__ wavenumber, const double* __restrict__ weights,
+    const BinDesc* __restrict__ bins, const double* __restrict__ shifts, int mode, double star_inv_temperature,
+    double depth_scale, double error_inflation, double* __restrict__ out, double* __restrict__ ll_terms) {
+  __shared__ double partial[2][kBlock / 64];
   const BinDesc b = bins[blockIdx.x];
   const double* w = weights + b.weight_offset;
   const double* f = flux + b.start;
-  double acc = 0.0;
-  for (uint32_t j = threadIdx.x; j < b.length; j += kBlock) acc += w[j] * f[j];
+  double acc = 0.0, acc_star = 0.0;
+  for (uint32_t j = threadIdx.x; j < b.length; j += kBlock) {
+    acc += w[j] * f[j];
+    if (mode == 1) acc_star += w[j] * starFlux(wavenumber[b.start + j], star_inv_temperature);
+  }
   acc = waveSum(acc);
-  if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = acc;
+  acc_star = waveSum(acc_star);
+  if ((threadIdx.x & 63) == 0) {
+    partial[0][threadIdx.x >> 6] = acc;
+    partial[1][threadIdx.x >> 6] = acc_star;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double v = (partial[0] + partial[1]) + (partial[2] + partial[3]);
+    double v = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
+    if (mode == 1) v = v / ((partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3])) * depth_scale;
     const double shift = shifts[b.observation];
     if (shift != 0) v += shift;
     out[blockIdx.x] = v;
     const double e2 = b.error * b.error + error_inflation;
     const double d = b.data - v;
-    ll_terms[blockIdx.x] = (-0.5 * log(e2 * 2.0 * 3.14159265358979323846) - 0.5 * d * d / e2) * b.weight;
+    ll_terms[blockIdx.x] = (-0.5 * log(e2 * 2.0 * kPi) - 0.5 * d * d / e2) * b.weight;
   }
 }
 
@@ -272,6 +383,9 @@ void deviceFree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+constexpr size_t kMaxSpecies = 16;                    // register-cached kernels up to 16 species
+constexpr size_t kTransmissionLds = 64 * 1024;        // LDS budget of one transmission workgroup
+
 }  // namespace
 
 DeviceEngine::DeviceEngine(int device) : device_(device) {
@@ -288,9 +402,9 @@ DeviceEngine::DeviceEngine(int device) : device_(device) {
 DeviceEngine::~DeviceEngine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
-  for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)flux_, (void*)weights_,
-                  (void*)bins_desc_, (void*)bins_, (void*)ll_terms_, (void*)ll_, (void*)staging_dev_,
-                  (void*)spectrum_})
+  for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
+                  (void*)weights_, (void*)bins_desc_, (void*)bins_, (void*)ll_terms_, (void*)ll_,
+                  (void*)staging_dev_, (void*)spectrum_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
   if (event_start_) (void)hipEventDestroy(static_cast<hipEvent_t>(event_start_));
@@ -339,6 +453,18 @@ void DeviceEngine::countRows() {
   distinct_rows_ = count;
 }
 
+void DeviceEngine::beginTimed() {
+  if (!profiling_) return;
+  countRows();
+  check(hipEventRecord(static_cast<hipEvent_t>(event_start_), static_cast<hipStream_t>(stream_)), "hipEventRecord");
+}
+
+void DeviceEngine::endTimed() {
+  if (!profiling_) return;
+  check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), static_cast<hipStream_t>(stream_)), "hipEventRecord");
+  event_pending_ = true;
+}
+
 void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<double>& lam) {
   check(hipSetDevice(device_), "hipSetDevice");
   n_points_ = nu.size();
@@ -373,6 +499,19 @@ void DeviceEngine::uploadTables(const TableStore& store) {
                            hipMemcpyHostToDevice, s),
           "copy cross-section tables");
   check(hipStreamSynchronize(s), "copy cross-section tables");
+}
+
+void DeviceEngine::uploadRayleigh(const std::vector<double>& sigma) {
+  check(hipSetDevice(device_), "hipSetDevice");
+  deviceFree(rayleigh_);
+  rayleigh_ = nullptr;
+  if (sigma.empty()) return;
+  if (sigma.size() != n_points_) raise(kInvalidInput, "Rayleigh cross-sections do not match the grid");
+  rayleigh_ = deviceAlloc<double>(n_points_, "Rayleigh cross-sections");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(rayleigh_, sigma.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy Rayleigh cross-sections");
+  check(hipStreamSynchronize(s), "copy Rayleigh cross-sections");
 }
 
 void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) {
@@ -416,15 +555,25 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
   check(hipStreamSynchronize(s), "copy observations");
 }
 
-void DeviceEngine::configure(size_t nb_levels, size_t nb_species) {
+void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmission) {
   check(hipSetDevice(device_), "hipSetDevice");
   nb_levels_ = nb_levels;
   nb_species_ = nb_species;
+  transmission_ = transmission;
   species_stride_ = (nb_species + kChunk - 1) / kChunk * kChunk;   // padding entries keep n = 0, row 0
+  if (species_stride_ == 0) species_stride_ = kChunk;
+  if (transmission && species_stride_ > kMaxSpecies)
+    raise(kInvalidInput, "the transmission model takes at most 16 tabulated species");
+  if (transmission && 2 * nb_levels * 8 * sizeof(double) > kTransmissionLds)
+    raise(kInvalidInput, "too many levels for the transmission kernel");
   const size_t lv = nb_levels * sizeof(LevelState);
   const size_t sp = nb_levels * species_stride_ * sizeof(LevelSpecies);
   const size_t sh = (n_obs_ + 1) * sizeof(double);
-  staging_bytes_ = lv + sp + sh;
+  const size_t pa = transmission ? nb_levels * nb_levels * sizeof(double) : 0;
+  off_species_ = lv;
+  off_shift_ = lv + sp;
+  off_path_ = lv + sp + sh;
+  staging_bytes_ = lv + sp + sh + pa;
   if (staging_host_) (void)hipHostFree(staging_host_);
   deviceFree(staging_dev_);
   void* h = nullptr;
@@ -433,51 +582,28 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species) {
   std::memset(staging_host_, 0, staging_bytes_ + sizeof(double));
   staging_dev_ = deviceAlloc<unsigned char>(staging_bytes_, "per-evaluation inputs");
   levels_host_ = reinterpret_cast<LevelState*>(staging_host_);
-  species_host_ = reinterpret_cast<LevelSpecies*>(staging_host_ + lv);
+  species_host_ = reinterpret_cast<LevelSpecies*>(staging_host_ + off_species_);
   for (size_t s = 0; s < species_stride_; ++s) species_host_[s].reload = 1;   // level 0 always loads
-  shift_host_ = reinterpret_cast<double*>(staging_host_ + lv + sp);
+  shift_host_ = reinterpret_cast<double*>(staging_host_ + off_shift_);
+  path_host_ = transmission ? reinterpret_cast<double*>(staging_host_ + off_path_) : nullptr;
   ll_host_ = reinterpret_cast<double*>(staging_host_ + staging_bytes_);
 }
 
-void DeviceEngine::evaluateEmission(double scaling, double inflation, double* spectrum_dev, double* const* bins_dev) {
+void DeviceEngine::upload() {
   check(hipSetDevice(device_), "hipSetDevice");
+  check(hipMemcpyAsync(staging_dev_, staging_host_, staging_bytes_, hipMemcpyHostToDevice,
+                       static_cast<hipStream_t>(stream_)),
+        "upload inputs");
+}
+
+void DeviceEngine::launchBins(int mode, double star_temperature, double depth_scale, double inflation,
+                              double* const* bins_dev) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  check(hipMemcpyAsync(staging_dev_, staging_host_, staging_bytes_, hipMemcpyHostToDevice, s), "upload inputs");
-  const size_t lv = nb_levels_ * sizeof(LevelState);
-  const size_t sp = nb_levels_ * species_stride_ * sizeof(LevelSpecies);
-  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
-  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + lv);
-  const double* shifts = reinterpret_cast<const double*>(staging_dev_ + lv + sp);
-  const int n = int(n_points_);
-  const int blocks = (n + kBlock - 1) / kBlock;
-  if (profiling_) {
-    countRows();
-    check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
-  }
-  switch (species_stride_) {
-#define BEAR_CACHED(NS)                                                                                         \
-  case NS:                                                                                                      \
-    hipLaunchKernelGGL(emissionKernelCached<NS>, dim3(blocks), dim3(kBlock), 0, s, wavenumber_, wavelength_,    \
-                       tables_, stride_, lev, spc, int(nb_levels_), n, scaling, flux_, spectrum_dev);           \
-    break;
-    BEAR_CACHED(4)
-    BEAR_CACHED(8)
-    BEAR_CACHED(12)
-    BEAR_CACHED(16)
-#undef BEAR_CACHED
-    default:
-      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, wavenumber_, wavelength_, tables_,
-                         stride_, lev, spc, int(nb_levels_), int(species_stride_ / kChunk), n, scaling, flux_,
-                         spectrum_dev);
-  }
-  check(hipGetLastError(), "emission kernel");
-  if (profiling_) {
-    check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), s), "hipEventRecord");
-    event_pending_ = true;
-  }
+  const double* shifts = reinterpret_cast<const double*>(staging_dev_ + off_shift_);
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlock), 0, s, flux_, weights_, bins_desc_, shifts,
-                       inflation, bins_, ll_terms_);
+    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlock), 0, s, flux_, wavenumber_, weights_,
+                       bins_desc_, shifts, mode, mode == 1 ? 1.0 / star_temperature : 0.0, depth_scale, inflation,
+                       bins_, ll_terms_);
     check(hipGetLastError(), "bin kernel");
     hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, ll_terms_, int(n_bins_), ll_);
     check(hipGetLastError(), "likelihood kernel");
@@ -489,6 +615,94 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
         check(hipMemcpyAsync(bins_dev[o], bins_ + a, (b - a) * sizeof(double), hipMemcpyDeviceToDevice, s),
               "copy bins");
     }
+}
+
+namespace {
+void launchEmission(hipStream_t s, size_t species_stride, int blocks, const double* nu, const double* lam,
+                    const double* tables, size_t stride, const LevelState* lev, const LevelSpecies* spc, int L, int n,
+                    double scaling, SpectrumOut out, double* flux, double* spectrum) {
+  switch (species_stride) {
+#define BEAR_CACHED(NS)                                                                                             \
+  case NS:                                                                                                          \
+    hipLaunchKernelGGL(emissionKernelCached<NS>, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev,   \
+                       spc, L, n, scaling, out, flux, spectrum);                                                    \
+    break;
+    BEAR_CACHED(4)
+    BEAR_CACHED(8)
+    BEAR_CACHED(12)
+    BEAR_CACHED(16)
+#undef BEAR_CACHED
+    default:
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, L,
+                         int(species_stride / kChunk), n, scaling, out, flux, spectrum);
+  }
+}
+}  // namespace
+
+void DeviceEngine::evaluateEmission(double scaling, double inflation, double* spectrum_dev, double* const* bins_dev) {
+  if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
+  upload();
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
+  const int n = int(n_points_);
+  beginTimed();
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
+                 int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0}, flux_, spectrum_dev);
+  check(hipGetLastError(), "emission kernel");
+  endTimed();
+  launchBins(0, 0.0, 0.0, inflation, bins_dev);
+}
+
+void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio, double inflation,
+                                   double* spectrum_dev, double* const* bins_dev) {
+  if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
+  upload();
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
+  const int n = int(n_points_);
+  const double depth_scale = radius_ratio * radius_ratio * 1e6;
+  beginTimed();
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
+                 int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale}, flux_, spectrum_dev);
+  check(hipGetLastError(), "emission kernel");
+  endTimed();
+  launchBins(1, star_temperature, depth_scale, inflation, bins_dev);
+}
+
+void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius, double inflation,
+                                        double* spectrum_dev, double* const* bins_dev) {
+  if (!transmission_) raise(kInvalidInput, "the engine is not configured for transmission");
+  upload();
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
+  const double* path = reinterpret_cast<const double*>(staging_dev_ + off_path_);
+  const int n = int(n_points_), L = int(nb_levels_);
+  // the widest wavenumber tile whose two LDS arrays fit the budget
+  int tile = 64;
+  while (tile > 8 && size_t(2) * L * tile * sizeof(double) > kTransmissionLds) tile /= 2;
+  const size_t lds = size_t(2) * L * tile * sizeof(double);
+  const int blocks = (n + tile - 1) / tile;
+  beginTimed();
+  switch (species_stride_) {
+#define BEAR_TRANSMISSION(NS)                                                                                       \
+  case NS:                                                                                                          \
+    hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kBlock), lds, s, tables_, stride_, rayleigh_,    \
+                       lev, spc, path, L, n, tile, bottom_radius, star_radius, flux_, spectrum_dev);               \
+    break;
+    BEAR_TRANSMISSION(4)
+    BEAR_TRANSMISSION(8)
+    BEAR_TRANSMISSION(12)
+    BEAR_TRANSMISSION(16)
+#undef BEAR_TRANSMISSION
+    default:
+      raise(kInvalidInput, "unsupported number of species for transmission");
+  }
+  check(hipGetLastError(), "transmission kernel");
+  endTimed();
+  launchBins(0, 0.0, 0.0, inflation, bins_dev);
 }
 
 double DeviceEngine::logLikelihood() {

@@ -166,6 +166,15 @@ class PiecewisePolynomial {
 
 struct ChemistryModule { bool background = false; std::vector<int> species; bool sodium_free = false; };
 
+// Variants of the altitude computation (atmosphere.cpp:63-241): radius-
+// dependent gravity, a fixed mean molecular weight, or a fixed scale height.
+struct AltitudeOptions {
+  double bottom_radius = 1.0;
+  bool variable_gravity = false;
+  double fixed_mean_molecular_weight = 0;   // > 0: replaces the chemistry's
+  double scale_height_km = 0;               // > 0: z = -H ln(p / p0)
+};
+
 // Atmosphere (forward_model/atmosphere/atmosphere.cpp) with the temperature
 // profiles "poly" and "const" and the chemistry modules "iso" and "bg H2He".
 class Atmosphere {
@@ -178,7 +187,8 @@ class Atmosphere {
   size_t nbTemperatureParameters() const { return nb_temperature_param_; }
   size_t nbChemistryParameters() const { return nb_chemistry_param_; }
   // Atmosphere::calcAtmosphereStructure (atmosphere.cpp:63-122); returns "neglect"
-  bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param);
+  bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param,
+               const AltitudeOptions& options = AltitudeOptions());
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
  private:
   size_t n_ids_;

@@ -122,7 +122,7 @@ Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& t
   number_density.assign(L * n_ids_, 0.0);
 }
 
-bool Atmosphere::compute(double g, const double* tp, const double* cp) {
+bool Atmosphere::compute(double g, const double* tp, const double* cp, const AltitudeOptions& opt) {
   const size_t L = nb_levels;
   const int S = int(n_ids_);
   bool neglect = false;
@@ -136,7 +136,10 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp) {
     poly_->setValues(dof);
     for (size_t i = 0; i < L; ++i) temperature[i] = poly_->value(std::log10(pressure[i]));
   }
-  for (auto& t : temperature) if (t < 50) { t = 50; neglect = true; }
+  // the polynomial profile floors at 50 K and flags the model
+  // (piecewise_poly_temperature.cpp:77-81); the isothermal one does not
+  if (!isothermal_)
+    for (auto& t : temperature) if (t < 50) { t = 50; neglect = true; }
 
   // chemistry modules in order (isoprofile_chemistry.cpp:87-119,
   // background_chemistry.cpp:72-137, chemistry.h:47-76)
@@ -178,14 +181,28 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp) {
       meanWeight();
     }
   }
-  // hydrostatic altitude, trapezoid in 1/(rho g) (atmosphere.cpp:288-307)
+  altitude[0] = 0.0;
+  if (opt.scale_height_km > 0) {                   // constant scale height (atmosphere.cpp:338-350)
+    const double h = opt.scale_height_km * 100000.0;
+    for (size_t l = 1; l < L; ++l) altitude[l] = -h * std::log(pressure[l] / pressure[0]);
+    return neglect;
+  }
+  if (opt.fixed_mean_molecular_weight > 0)          // atmosphere.cpp:168
+    std::fill(mean_molecular_weight.begin(), mean_molecular_weight.end(), opt.fixed_mean_molecular_weight);
+  // hydrostatic altitude, trapezoid in 1/(rho g), with g falling off as
+  // 1/r^2 when asked (atmosphere.cpp:288-334)
   std::vector<double> rho(L);
   for (size_t l = 0; l < L; ++l)
     rho[l] = mean_molecular_weight[l] * pressure[l] * 1e6 / (phys::gas_constant * temperature[l]);
-  altitude[0] = 0.0;
-  for (size_t l = 1; l < L; ++l)
+  for (size_t l = 1; l < L; ++l) {
+    double gl = g;
+    if (opt.variable_gravity) {
+      const double local_radius = opt.bottom_radius + altitude[l - 1];
+      gl = g * std::pow(opt.bottom_radius / local_radius, 2.0);
+    }
     altitude[l] = altitude[l - 1] +
-                  (1.0 / (rho[l] * g) + 1.0 / (rho[l - 1] * g)) * 0.5 * (pressure[l - 1] * 1e6 - pressure[l] * 1e6);
+                  (1.0 / (rho[l] * gl) + 1.0 / (rho[l - 1] * gl)) * 0.5 * (pressure[l - 1] * 1e6 - pressure[l] * 1e6);
+  }
   return neglect;
 }
 

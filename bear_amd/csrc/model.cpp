@@ -6,52 +6,54 @@
 
 namespace bear_amd {
 
-EmissionModel::EmissionModel(const GlobalSettings& global, const ModelSettings& m, const SpectralGrid& grid,
-                             const std::vector<ObservationSetup>& observations, DeviceEngine& engine)
+// opacity species and tables (TransportCoefficients ctor, transport_coeff.cpp:40-93)
+TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings& m, const SpectralGrid& grid,
+                               const std::vector<ObservationSetup>& observations, DeviceEngine& engine,
+                               bool transmission)
     : observations_(observations),
       engine_(engine),
       atmosphere_(m.nb_levels, m.bottom_pressure, m.top_pressure, m.temperature, m.chemistry) {
-  if (!m.radiative_transfer.empty() && m.radiative_transfer != "scm")
-    raise(kInvalidInput, "radiative transfer " + m.radiative_transfer + " is not available (only scm)");
-  // opacity species (TransportCoefficients ctor, transport_coeff.cpp:40-93)
   TableStore store;
   store.nb_points = grid.size();
+  std::vector<double> rayleigh;
   for (auto& e : m.opacity) {
     OpacitySpecies s = makeOpacitySpecies(e.first, e.second, grid);
-    // emission's short characteristics use absorption only (emission.cpp:210-218),
-    // so scattering species do not enter this model's evaluation
-    if (s.rayleigh) continue;
+    if (s.rayleigh) {
+      // scattering enters the transmission model only: emission's short
+      // characteristics take the absorption coefficients alone
+      // (emission.cpp:210-218, secondary_eclipse.cpp:200-209)
+      if (!transmission) continue;
+      if (rayleigh_species_ >= 0) raise(kInvalidInput, "one Rayleigh scatterer at most");
+      rayleigh_species_ = s.species_id;
+      rayleigh = s.rayleigh_cross_section;
+      continue;
+    }
     loadSpeciesTable(s, global.cross_section_path, grid, store);
     species_.push_back(std::move(s));
   }
   engine_.uploadTables(store);
-  engine_.configure(m.nb_levels, species_.size());
+  engine_.uploadRayleigh(rayleigh);
+  engine_.configure(m.nb_levels, species_.size(), transmission);
   nb_chem_ = atmosphere_.nbChemistryParameters();
   nb_temp_ = atmosphere_.nbTemperatureParameters();
   for (auto& o : observations_) nb_modifier_ += size_t(o.modifier_parameters);
 }
 
-// EmissionModel::calcModelGPU (emission.cpp:234-263): parameters
-// [log g, scaling f, distance d, chemistry..., temperature..., modifiers...]
-bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
-                                 double* const* spectrum_obs_dev) {
-  const double* chem = p + nb_general_;
-  const double* temp = chem + nb_chem_;
-  const double* mods = temp + nb_temp_;
-  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
-
+void TabulatedModel::stage(const double* mods) {
   const size_t L = atmosphere_.nb_levels, S = species_.size(), stride = engine_.speciesStride();
   LevelState* lev = engine_.levels();
   LevelSpecies* spc = engine_.levelSpecies();
   for (size_t l = 0; l < L; ++l) {
     const double t = atmosphere_.temperature[l];
-    lev[l] = LevelState{t, 1.0 / t, atmosphere_.altitude[l], 0.0};
+    // opacity_species.cpp:443-446: scattering uses the scatterer's own density
+    const double n_r = rayleigh_species_ >= 0 ? atmosphere_.nd(l, rayleigh_species_) : 0.0;
+    lev[l] = LevelState{t, 1.0 / t, atmosphere_.altitude[l], n_r};
     for (size_t s = 0; s < S; ++s) {
       const OpacitySpecies& sp = species_[s];
       double n = atmosphere_.nd(l, sp.species_id);                 // opacity_species.cpp:407-412
       for (int q : sp.cia_partners) n *= atmosphere_.nd(l, q);
       LevelSpecies& d = spc[l * stride + s];
-      d.n = n;                                   // rows stay valid when n == 0: the kernel loads them
+      d.n = n;                                   // rows stay valid when n == 0: the kernels load them
       const Stencil st = sp.bracket(atmosphere_.pressure[l], t);
       d.f_p_lower = st.f_p_lower;
       d.f_p_upper = st.f_p_upper;
@@ -70,19 +72,109 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
     engine_.shifts()[o] = observations_[o].modifier_parameters ? mods[k] : 0.0;
     k += size_t(observations_[o].modifier_parameters);
   }
+}
+
+// ---------------------------------------------------------------- emission
+EmissionModel::EmissionModel(const GlobalSettings& global, const ModelSettings& m, const SpectralGrid& grid,
+                             const std::vector<ObservationSetup>& observations, DeviceEngine& engine)
+    : TabulatedModel(global, m, grid, observations, engine, false) {
+  if (!m.radiative_transfer.empty() && m.radiative_transfer != "scm")
+    raise(kInvalidInput, "radiative transfer " + m.radiative_transfer + " is not available (only scm)");
+}
+
+// EmissionModel::calcModelGPU (emission.cpp:234-263)
+bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
+                                 double* const* spectrum_obs_dev) {
+  const double* chem = p + 3;
+  const double* temp = chem + nb_chem_;
+  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
+  stage(temp + nb_temp_);
   double scaling = phys::radius_jupiter / p[2];                   // emission.cpp:108-120
   scaling = scaling * scaling * p[1];
   engine_.evaluateEmission(scaling, inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
 }
 
-// Retrieval set-up: observations, grid sampling, per-observation operators,
-// forward model (retrieval.cpp:70-104, retrieval_load_observations.cpp:38-92)
+// ---------------------------------------------------------------- secondary eclipse
+OccultationModel::OccultationModel(const GlobalSettings& global, const ModelSettings& m, const SpectralGrid& grid,
+                                   const std::vector<ObservationSetup>& observations, DeviceEngine& engine)
+    : TabulatedModel(global, m, grid, observations, engine, false) {
+  if (!m.radiative_transfer.empty() && m.radiative_transfer != "scm")
+    raise(kInvalidInput, "radiative transfer " + m.radiative_transfer + " is not available (only scm)");
+  if (m.stellar_model != "blackbody")
+    raise(kInvalidInput, "stellar model " + m.stellar_model + " is not available (only blackbody)");
+}
+
+// OccultationModel::calcModelGPU (secondary_eclipse.cpp:262-355): planet
+// emission at scale 1, blackbody star, depth = F_p / F_* (Rp/Rs)^2 1e6 per bin
+bool OccultationModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
+                                    double* const* spectrum_obs_dev) {
+  const double* chem = p + 3;
+  const double* temp = chem + nb_chem_;
+  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
+  stage(temp + nb_temp_);
+  engine_.evaluateEclipse(p[2], p[1], inflation, spectrum_dev, spectrum_obs_dev);
+  return neglect;
+}
+
+// ---------------------------------------------------------------- transmission
+TransmissionModel::TransmissionModel(const GlobalSettings& global, const ModelSettings& m, const SpectralGrid& grid,
+                                     const std::vector<ObservationSetup>& observations, DeviceEngine& engine)
+    : TabulatedModel(global, m, grid, observations, engine, true),
+      fit_mmw_(m.fit_mean_molecular_weight),
+      fit_scale_height_(m.fit_scale_height),
+      variable_gravity_(m.use_variable_gravity) {
+  if (fit_mmw_ && fit_scale_height_)
+    raise(kInvalidInput, "fit either the mean molecular weight or the scale height, not both");
+  nb_general_ = 3 + ((fit_mmw_ || fit_scale_height_) ? 1 : 0);    // transmission.cpp:69-72
+}
+
+// TransmissionModel::calcModelGPU (transmission.cpp:257-321) with the
+// atmosphere variants of calcAtmosphereStructure (:155-195)
+bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
+                                     double* const* spectrum_obs_dev) {
+  const double* chem = p + nb_general_;
+  const double* temp = chem + nb_chem_;
+  AltitudeOptions opt;
+  opt.bottom_radius = p[1];
+  opt.variable_gravity = variable_gravity_;
+  if (fit_mmw_) opt.fixed_mean_molecular_weight = p[3];
+  if (fit_scale_height_) opt.scale_height_km = p[3];
+  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem, opt);
+  stage(temp + nb_temp_);
+  // chord lengths between successive levels for every tangent level
+  // (transmission_spectrum.cpp:117-118, 145-154): path[t][i] =
+  // sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2)
+  const size_t L = atmosphere_.nb_levels;
+  const double r0 = p[1];
+  const std::vector<double>& z = atmosphere_.altitude;
+  double* path = engine_.pathLengths();
+  for (size_t t = 0; t + 1 < L; ++t) {
+    const double b = r0 + z[t];
+    double previous = std::sqrt(b * b - b * b);
+    for (size_t i = t; i + 1 < L; ++i) {
+      const double a = r0 + z[i + 1];
+      const double next = std::sqrt(a * a - b * b);
+      path[t * L + i] = next - previous;
+      previous = next;
+    }
+  }
+  engine_.evaluateTransmission(p[1], p[2], inflation, spectrum_dev, spectrum_obs_dev);
+  return neglect;
+}
+
+// ---------------------------------------------------------------- retrieval
+// Set-up: observations, grid sampling, per-observation operators, forward
+// model (retrieval.cpp:70-104, retrieval_load_observations.cpp:38-92,
+// select_forward_model.cpp:38-164)
 Retrieval::Retrieval(const GlobalSettings& global, const std::string& model_type, const ModelSettings& model,
                      const std::vector<ObservationInputView>& inputs, const std::vector<std::string>& files,
                      const std::vector<std::string>& modifiers)
     : global_(global) {
-  if (model_type != "emission")
+  const bool emission = model_type == "emission";
+  const bool eclipse = model_type == "secondary_eclipse" || model_type == "se";
+  const bool transmission = model_type == "transmission";
+  if (!emission && !eclipse && !transmission)
     raise(kInvalidInput, "forward model " + model_type + " is not available");
   std::string folder = global_.retrieval_folder;
   if (!folder.empty() && folder.back() != '/') folder += '/';
@@ -95,15 +187,16 @@ Retrieval::Retrieval(const GlobalSettings& global, const std::string& model_type
   for (size_t i = 0; i < files.size(); ++i)
     observations_.push_back(observationFromFile(folder + files[i], i < modifiers.size() ? modifiers[i] : "", folder));
   grid_.sample(observations_);
-  const bool is_flux = true;                                      // emission bins are fluxes
   for (auto& o : observations_) {
     o.setupOnGrid(grid_);
-    o.foldOperator(grid_, is_flux);
+    o.foldOperator(grid_, !transmission);                         // transmission bins are not fluxes
   }
   engine_.reset(new DeviceEngine(global_.device));
   engine_->uploadGrid(grid_.wavenumber, grid_.wavelength);
   engine_->uploadObservations(observations_);
-  model_.reset(new EmissionModel(global_, model, grid_, observations_, *engine_));
+  if (emission) model_.reset(new EmissionModel(global_, model, grid_, observations_, *engine_));
+  if (eclipse) model_.reset(new OccultationModel(global_, model, grid_, observations_, *engine_));
+  if (transmission) model_.reset(new TransmissionModel(global_, model, grid_, observations_, *engine_));
 }
 
 void Retrieval::checkSize(size_t n) const {

@@ -20,11 +20,15 @@ struct GlobalSettings {                 // GlobalConfig (config/global_config.h:
   bool error_inflation = false;
 };
 
-struct ModelSettings {                  // EmissionModelConfig (emission.h:51-102)
+// EmissionModelConfig (emission.h:51-102), TransmissionModelConfig
+// (transmission.h:55-130), OccultationModelConfig (secondary_eclipse.h:55-120)
+struct ModelSettings {
   size_t nb_levels = 0;
   double bottom_pressure = 0, top_pressure = 0;
   std::string temperature, radiative_transfer, chemistry;
   std::vector<std::pair<std::string, std::string>> opacity;   // (symbol, folder)
+  bool fit_mean_molecular_weight = false, fit_scale_height = false, use_variable_gravity = false;
+  std::string stellar_model = "blackbody";
 };
 
 class ForwardModel {
@@ -36,24 +40,66 @@ class ForwardModel {
   // error inflation forwarded to the likelihood terms; returns "neglect"
   virtual bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                             double* const* spectrum_obs_dev) = 0;
+  // the binned spectrum is a flux (emission, eclipse) or not (transmission)
+  virtual bool binsAreFluxes() const = 0;
 };
 
-// EmissionModel (forward_model/emission/emission.h:127-250)
-class EmissionModel : public ForwardModel {
- public:
-  EmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
-                const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
-  size_t parametersNumber() const override { return nb_general_ + nb_chem_ + nb_temp_ + nb_modifier_; }
-  bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
-                    double* const* spectrum_obs_dev) override;
-  const Atmosphere& atmosphere() const { return atmosphere_; }
-
- private:
+// What the three models share: atmosphere, opacity species with their
+// tables on the device, and the per-level stencils of one evaluation.
+class TabulatedModel : public ForwardModel {
+ protected:
+  TabulatedModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
+                 const std::vector<ObservationSetup>& observations, DeviceEngine& engine, bool transmission);
+  // fill the engine's staging: levels, stencils and observation shifts
+  void stage(const double* modifiers);
   const std::vector<ObservationSetup>& observations_;
   DeviceEngine& engine_;
   Atmosphere atmosphere_;
-  std::vector<OpacitySpecies> species_;     // absorbers with tables (Rayleigh is not used by emission)
-  size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
+  std::vector<OpacitySpecies> species_;     // tabulated absorbers
+  int rayleigh_species_ = -1;               // species id of the Rayleigh scatterer, -1: none
+  size_t nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
+};
+
+// EmissionModel (forward_model/emission/emission.h:127-250): parameters
+// [log g, f, d, chemistry..., temperature..., modifiers...]
+class EmissionModel : public TabulatedModel {
+ public:
+  EmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
+                const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
+  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_modifier_; }
+  bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
+                    double* const* spectrum_obs_dev) override;
+  bool binsAreFluxes() const override { return true; }
+};
+
+// OccultationModel (forward_model/secondary_eclipse/secondary_eclipse.h) with
+// the blackbody star: parameters [log g, Rp/Rs, T_star, chemistry...,
+// temperature..., modifiers...]
+class OccultationModel : public TabulatedModel {
+ public:
+  OccultationModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
+                   const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
+  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_modifier_; }
+  bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
+                    double* const* spectrum_obs_dev) override;
+  bool binsAreFluxes() const override { return true; }
+};
+
+// TransmissionModel (forward_model/transmission/transmission.h): parameters
+// [log g, R_p (cm), R_* (cm), (mu | H in km), chemistry..., temperature...,
+// modifiers...]
+class TransmissionModel : public TabulatedModel {
+ public:
+  TransmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
+                    const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
+  size_t parametersNumber() const override { return nb_general_ + nb_chem_ + nb_temp_ + nb_modifier_; }
+  bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
+                    double* const* spectrum_obs_dev) override;
+  bool binsAreFluxes() const override { return false; }
+
+ private:
+  size_t nb_general_ = 3;
+  bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;
 };
 
 // Retrieval (retrieval/retrieval.h) restricted to the likelihood path

@@ -349,7 +349,94 @@ def emission_c2(seed=23, backend="numpy", n_wavenumbers=400_000, nb_bins=300):
                 [("iso", names), ("bg", ["H2He"])], species, ["H2"], obs, params)
 
 
-CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2}
+def transmission_parameters(rng, nb_absorbers, isothermal=True):
+    log_g = rng.uniform(2.5, 4.0)
+    radius = RADIUS_JUPITER * rng.uniform(0.9, 1.1)
+    star = RADIUS_SUN
+    mixing = 10.0 ** rng.uniform(-8.0, -3.0, nb_absorbers)
+    t = [rng.uniform(500.0, 2500.0)] if isothermal else list(poly_temperature_params(rng, 7))
+    return np.concatenate([[log_g, radius, star], mixing, t])
+
+
+def _transit_data(lo, hi, nb, r, rng):
+    edges = np.linspace(hi, lo, nb + 1)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    centres = be.mean(axis=1)
+    depth = 1.2e4 * np.ones(nb)
+    return be, depth, 50.0 * np.ones(nb), centres / r
+
+
+def transmission_small(seed=31, variant="plain"):
+    """Transmission case for the tests: 40 levels 1e2..1e-6 bar, isothermal,
+    2 absorbers + H2-H2 / H2-He CIA + H2 Rayleigh, 3000-8000 cm^-1, three
+    band-spectroscopy instruments with Gaussian line-spread functions.
+    variant: "plain", "variable_gravity", "fit_mmw" (mean molecular weight
+    as a parameter) or "fit_scale_height" (scale height in km as a parameter)."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(3000.0, 8000.0, 1.0)
+    temps = np.arange(100.0, 3001.0, 300.0)
+    press = 10.0 ** np.arange(-6.0, 2.5, 2.0)
+    species = [
+        line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=600),
+        line_table(rng, "CH4", "CH4", nu, temps, press, n_lines=500, log_storage=True),
+        cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+        cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps),
+    ]
+    obs = []
+    for k, (lo, hi, nb, r) in enumerate([(1.30, 1.65, 8, 70.0), (1.75, 2.45, 15, 130.0), (2.55, 3.20, 25, 300.0)]):
+        be, d, e, fw = _transit_data(lo, hi, nb, r, rng)
+        obs.append(ObservationSpec(f"instrument{k}", "band-spectroscopy", bin_edges=be, data=d, error=e, fwhm=fw,
+                                   modifier="shift_const" if k == 1 else "none"))
+    params = transmission_parameters(rng, 2)
+    extra = {"variant": variant}
+    if variant == "fit_mmw":
+        params = np.concatenate([params[:3], [2.3], params[3:]])
+    if variant == "fit_scale_height":
+        params = np.concatenate([params[:3], [400.0], params[3:]])
+    params = np.concatenate([params, [0.0]])
+    c = Case(f"transmission_{variant}", nu, 2.0, 40, 100.0, 1e-6, ("const",),
+             [("iso", ["H2O", "CH4"]), ("bg", ["H2He"])], species, ["H2"], obs, params, model="transmission",
+             extra=extra)
+    return c
+
+
+def eclipse_small(seed=41):
+    """Secondary-eclipse case for the tests: 30 levels, poly 6 1, 2 absorbers
+    + CIA, a blackbody star, three photometric bandpasses and a 20-bin
+    band spectrum with a line-spread function."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(2500.0, 7000.0, 1.0)
+    temps = np.arange(100.0, 3001.0, 300.0)
+    press = 10.0 ** np.arange(-6.0, 2.5, 2.0)
+    species = [
+        line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=600),
+        line_table(rng, "CO", "CO", nu, temps, press, n_lines=300, mass=MOLECULAR_WEIGHT["CO"]),
+        cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+        cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps),
+    ]
+    obs = []
+    for k, (lo, hi, det) in enumerate([(1.6, 1.9, "Photon"), (2.1, 2.4, "Energy"), (3.2, 3.8, "Photon")]):
+        w, r = bandpass(rng, lo, hi, 50)
+        obs.append(ObservationSpec(f"phot{k}", "photometry", bin_edges=np.array([[lo, hi]]),
+                                   data=np.array([1500.0]), error=np.array([80.0]), filter=(det, w, r)))
+    edges = np.linspace(1.55, 3.6, 21)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    obs.append(ObservationSpec("spectrum", "band-spectroscopy", bin_edges=be, data=np.full(20, 1500.0),
+                               error=np.full(20, 80.0), fwhm=be.mean(axis=1) / 200.0))
+    rng2 = np.random.default_rng(seed + 1)
+    params = np.concatenate([[rng2.uniform(3.0, 4.0), rng2.uniform(0.08, 0.15), rng2.uniform(4500, 6500)],
+                             10.0 ** rng2.uniform(-8.0, -3.0, 2), poly_temperature_params(rng2, 7, 800.0, 2800.0)])
+    return Case("eclipse_small", nu, 2.0, 30, 100.0, 1e-6, ("poly", 6, 1),
+                [("iso", ["H2O", "CO"]), ("bg", ["H2He"])], species, [], obs, params, model="secondary_eclipse",
+                extra={"stellar_model": "blackbody"})
+
+
+CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2,
+         "transmission_plain": lambda: transmission_small(variant="plain"),
+         "transmission_variable_gravity": lambda: transmission_small(variant="variable_gravity"),
+         "transmission_fit_mmw": lambda: transmission_small(variant="fit_mmw"),
+         "transmission_fit_scale_height": lambda: transmission_small(variant="fit_scale_height"),
+         "eclipse_small": eclipse_small}
 
 
 # ---------------------------------------------------------------------------
@@ -417,13 +504,15 @@ def write_case(case: Case, folder: str) -> str:
                         cols.append(o.weight[j])
                     f.write("  ".join(_fmt(c) for c in cols) + "\n")
             elif o.type == "band-spectroscopy":
+                # with a profile column, write the weight column too: the
+                # reader takes a 5-column line's last field as a weight
+                # (observations_load_data.cpp:180-191), which would be 0 here
                 f.write("#left  right  value  error  fwhm  weight\n")
                 for j in range(o.bin_edges.shape[0]):
                     cols = [o.bin_edges[j, 0], o.bin_edges[j, 1], o.data[j], o.error[j]]
                     if o.fwhm is not None or o.weight is not None:
                         cols.append(o.fwhm[j] if o.fwhm is not None else 0.0)
-                    if o.weight is not None:
-                        cols.append(o.weight[j])
+                        cols.append(o.weight[j] if o.weight is not None else 1.0)
                     f.write("  ".join(_fmt(c) for c in cols) + "\n")
             else:
                 f.write("#lower  upper  value  error\n")
@@ -465,6 +554,12 @@ def write_case(case: Case, folder: str) -> str:
         for s, fo in case.opacity_entries:
             f.write(f"opacity {s} {fo}\n")
         f.write(f"error_inflation {int(case.use_error_inflation)}\n")
+        variant = case.extra.get("variant", "plain")
+        f.write(f"use_variable_gravity {int(variant == 'variable_gravity')}\n")
+        f.write(f"fit_mean_molecular_weight {int(variant == 'fit_mmw')}\n")
+        f.write(f"fit_scale_height {int(variant == 'fit_scale_height')}\n")
+        if "stellar_model" in case.extra:
+            f.write(f"stellar_model {case.extra['stellar_model']}\n")
         f.write(f"parameters {case.parameters.size} " + " ".join(_fmt(v) for v in case.parameters) + "\n")
     return folder
 
@@ -476,9 +571,13 @@ def write_case(case: Case, folder: str) -> str:
 def _model_config(meta):
     from . import api
     chem = [" ".join([c[0]] + list(c[1])) for c in meta["chemistry"]]
+    variant = meta.get("extra", {}).get("variant", "plain")
     return api.ModelConfig(meta["nb_levels"], meta["p_bottom"], meta["p_top"],
                            " ".join(str(v) for v in meta["temperature_model"]), chem,
-                           [tuple(o) for o in meta["opacity"]])
+                           [tuple(o) for o in meta["opacity"]],
+                           fit_mean_molecular_weight=variant == "fit_mmw",
+                           fit_scale_height=variant == "fit_scale_height",
+                           use_variable_gravity=variant == "variable_gravity")
 
 
 def open_retrieval(folder, device=0):
@@ -491,7 +590,7 @@ def open_retrieval(folder, device=0):
                            spectral_resolution=meta["spectral_resolution"],
                            use_error_inflation=meta["use_error_inflation"], device=device)
     return api.Retrieval(cfg, _model_config(meta), observation_files=meta["observations"],
-                         observation_modifiers=meta["modifiers"])
+                         observation_modifiers=meta["modifiers"], forward_model_type=meta["model"])
 
 
 def case_meta(case: Case):
@@ -502,6 +601,8 @@ def case_meta(case: Case):
         "chemistry": [[c[0], list(c[1])] for c in case.chemistry],
         "opacity": [[s, f] for s, f in case.opacity_entries],
         "use_error_inflation": case.use_error_inflation,
+        "model": case.model,
+        "extra": case.extra,
     }
 
 
@@ -543,7 +644,7 @@ def open_retrieval_in_memory(case: Case, scratch_folder, device=0):
     cfg = api.GlobalConfig(retrieval_folder_path=scratch_folder, cross_section_file_path="",
                            wavenumber_file_path=wn, spectral_resolution=meta["spectral_resolution"],
                            use_error_inflation=meta["use_error_inflation"], device=device)
-    r = api.Retrieval(cfg, _model_config(meta), observations=observation_inputs(case))
+    r = api.Retrieval(cfg, _model_config(meta), observations=observation_inputs(case), forward_model_type=case.model)
     api.clear_opacity_data()
     return r
 
@@ -556,12 +657,25 @@ def inject_data(case: Case, model_bins, seed=99, frac=0.02):
         o.data, o.error = noisy(rng, np.asarray(m, dtype=np.float64), frac)
 
 
+def nb_general_parameters(case: Case):
+    """General parameters ahead of the chemistry: emission [log g, f, d],
+    transmission [log g, R_p, R_*, (mu | H)], eclipse [log g, Rp/Rs, T_star]."""
+    if case.model == "transmission":
+        return 3 + (1 if case.extra.get("variant") in ("fit_mmw", "fit_scale_height") else 0)
+    return 3
+
+
 def perturbed(params, case: Case, seed=5, scale=1.0):
-    """A nearby live point: small shifts of every physical parameter."""
+    """A nearby live point: small shifts of the physical parameters."""
     rng = np.random.default_rng(seed)
     p = np.array(params, dtype=np.float64)
+    g0 = nb_general_parameters(case)
     nb_mix = sum(len(c[1]) for c in case.chemistry if c[0] == "iso")
     p[0] += 0.02 * scale * rng.standard_normal()
-    p[3:3 + nb_mix] *= np.exp(0.1 * scale * rng.standard_normal(nb_mix))
-    p[3 + nb_mix] *= 1 + 0.01 * scale * rng.standard_normal()
+    if case.model == "transmission":
+        p[1] *= 1 + 0.002 * scale * rng.standard_normal()
+    if case.model == "secondary_eclipse":
+        p[1] *= 1 + 0.01 * scale * rng.standard_normal()
+    p[g0:g0 + nb_mix] *= np.exp(0.1 * scale * rng.standard_normal(nb_mix))
+    p[g0 + nb_mix] *= 1 + 0.01 * scale * rng.standard_normal()
     return p

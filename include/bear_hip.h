@@ -74,6 +74,14 @@ typedef struct bear_model_config {
   const char* radiative_transfer;
   const char* chemistry;
   const char* opacity_species;
+  /* transmission (TransmissionModelConfig, transmission.h:55-70): a 4th
+   * general parameter fixes the mean molecular weight or the scale height
+   * (km); gravity may fall off as 1/r^2 with altitude */
+  int fit_mean_molecular_weight;
+  int fit_scale_height;
+  int use_variable_gravity;
+  /* secondary eclipse (OccultationModelConfig): "blackbody" (NULL: blackbody) */
+  const char* stellar_spectrum_model;
 } bear_model_config;
 
 /* ObservationInput (observations/observations.h:43-83).  For "spectroscopy"
@@ -111,7 +119,8 @@ int bear_opacity_data_clear(void);
  * Retrieval (retrieval/retrieval.h) restricted to the likelihood path:
  * observations set up (retrieval_load_observations.cpp:38-92), the
  * spectral grid sampled from them (spectral_grid.cpp:123-304) and a forward
- * model selected (retrieval/select_forward_model.cpp:38-164, here "emission").
+ * model selected (retrieval/select_forward_model.cpp:38-164): "emission",
+ * "transmission" or "secondary_eclipse".
  * Observations are the in-memory inputs followed by the files named in
  * obs_files (entries of observations.list, with their modifiers). */
 typedef struct bear_retrieval bear_retrieval;

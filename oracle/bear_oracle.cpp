@@ -703,6 +703,8 @@ struct Chemistry { bool background = false; std::vector<int> species; bool sodiu
 
 // ---------------------------------------------------------------- the model
 struct EmissionOracle {
+  std::string model = "emission";
+  bool variable_gravity = false, fit_mmw = false, fit_scale_height = false;
   Grid grid;
   std::vector<Observation> obs;
   size_t nb_levels = 0;
@@ -741,7 +743,15 @@ struct EmissionOracle {
       in >> key;
       std::vector<std::string> r;
       while (in >> tok) r.push_back(tok);
-      if (key == "model" && r[0] != "emission") throw std::runtime_error("only the emission model is restated");
+      if (key == "model") {
+        model = r[0];
+        if (model != "emission" && model != "transmission" && model != "secondary_eclipse")
+          throw std::runtime_error("forward model not restated: " + model);
+      }
+      if (key == "use_variable_gravity") variable_gravity = r[0] == "1";
+      if (key == "fit_mean_molecular_weight") fit_mmw = r[0] == "1";
+      if (key == "fit_scale_height") fit_scale_height = r[0] == "1";
+      if (key == "stellar_model" && r[0] != "blackbody") throw std::runtime_error("stellar model not restated: " + r[0]);
       if (key == "spectral_resolution") grid.resolution = std::stod(r[0]);
       if (key == "nb_levels") nb_levels = std::stoul(r[0]);
       if (key == "p_bottom") p_bottom = std::stod(r[0]);
@@ -818,9 +828,23 @@ struct EmissionOracle {
     }
   }
 
-  size_t nbParameters() const { return 3 + nb_chem_param + nb_temp_param + nb_modifier + (error_inflation ? 1 : 0); }
+  // general parameters: emission [log g, f, d] (emission.cpp:68-69); transmission
+  // [log g, R_p, R_*, (mu or H)] (transmission.cpp:69-72); eclipse [log g, Rp/Rs]
+  // then the blackbody star's T_eff (secondary_eclipse.cpp:72, star_blackbody.h)
+  size_t nbGeneral() const {
+    if (model == "transmission") return 3 + ((fit_mmw || fit_scale_height) ? 1 : 0);
+    if (model == "secondary_eclipse") return 3;
+    return 3;
+  }
+  size_t nbParameters() const {
+    return nbGeneral() + nb_chem_param + nb_temp_param + nb_modifier + (error_inflation ? 1 : 0);
+  }
 
-  bool atmosphereStructure(double g, const std::vector<double>& tp, const std::vector<double>& cp) {
+  // Atmosphere::calcAtmosphereStructure, its three overloads (atmosphere.cpp:63-241):
+  // fixed_mmw > 0 replaces the mean molecular weight, scale_height_km > 0
+  // replaces the hydrostatic altitude by z = -H ln(p / p0)
+  bool atmosphereStructure(double g, const std::vector<double>& tp, const std::vector<double>& cp,
+                           double bottom_radius = 1.0, double fixed_mmw = 0.0, double scale_height_km = 0.0) {
     bool neg = false;
     const size_t L = nb_levels;
     // temperature (piecewise_poly_temperature.cpp:49-84, constant_temperature.h:34-46)
@@ -834,7 +858,10 @@ struct EmissionOracle {
       tpoly->set(dof);
       for (size_t i = 0; i < L; ++i) temperature[i] = tpoly->value(std::log10(pressure[i]));
     }
-    for (auto& t : temperature) if (t < 50) { t = 50; neg = true; }
+    // the piecewise polynomial floors at 50 K and flags the model
+    // (piecewise_poly_temperature.cpp:77-81); the isothermal profile does not
+    if (!const_temperature)
+      for (auto& t : temperature) if (t < 50) { t = 50; neg = true; }
     // chemistry (isoprofile_chemistry.cpp:87-119, background_chemistry.cpp:72-137, chemistry.h:47-76)
     nd.assign(L, std::vector<double>(kNbSpecies, 0.0));
     mmw.assign(L, 0.0);
@@ -874,12 +901,23 @@ struct EmissionOracle {
         meanWeight();
       }
     }
-    // altitude (atmosphere.cpp:288-307)
     altitude.assign(L, 0.0);
+    if (scale_height_km > 0) {                                 // atmosphere.cpp:338-350
+      const double h = scale_height_km * 100000.0;
+      for (size_t i = 1; i < L; ++i) altitude[i] = -h * std::log(pressure[i] / pressure[0]);
+      return neg;
+    }
+    if (fixed_mmw > 0) mmw.assign(L, fixed_mmw);               // atmosphere.cpp:168
+    // altitude (atmosphere.cpp:288-334)
     std::vector<double> rho(L);
     for (size_t i = 0; i < L; ++i) rho[i] = mmw[i] * pressure[i] * 1e6 / (kGasConstant * temperature[i]);
     for (size_t i = 1; i < L; ++i) {
-      const double dz = (1.0 / (rho[i] * g) + 1.0 / (rho[i - 1] * g)) * 0.5 * (pressure[i - 1] * 1e6 - pressure[i] * 1e6);
+      double gl = g;
+      if (variable_gravity) {
+        const double local_radius = bottom_radius + altitude[i - 1];
+        gl = g * std::pow(bottom_radius / local_radius, 2.0);
+      }
+      const double dz = (1.0 / (rho[i] * gl) + 1.0 / (rho[i - 1] * gl)) * 0.5 * (pressure[i - 1] * 1e6 - pressure[i] * 1e6);
       altitude[i] = altitude[i - 1] + dz;
     }
     return neg;
@@ -936,27 +974,99 @@ struct EmissionOracle {
     for (auto& s : spectrum_flux) s *= scaling;
   }
 
-  // EmissionModel::calcModelCPU (emission.cpp:194-228) + Retrieval::logLikelihood (retrieval.cpp:314-370)
-  void evaluate(const std::vector<double>& p) {
-    if (p.size() < nbParameters()) throw std::runtime_error("too few parameters");
-    const std::vector<double> tp(p.begin() + 3 + nb_chem_param, p.begin() + 3 + nb_chem_param + nb_temp_param);
-    const std::vector<double> cp(p.begin() + 3, p.begin() + 3 + nb_chem_param);
-    neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp);
-    opacity();
-    double scaling = kRadiusJupiter / p[2];                          // emission.cpp:108-120
-    scaling = scaling * scaling * p[1];
-    radiativeTransfer(scaling);
-    bins.resize(obs.size());
-    for (size_t i = 0; i < obs.size(); ++i) bins[i] = processSpectrum(grid, obs[i], spectrum_flux, true);
-    size_t m = 3 + nb_chem_param + nb_temp_param;                     // forward_model.cpp:161-180
+  // TransmissionModel::calcTransmissionSpectrum .. distanceToTangentCenter
+  // (transmission_spectrum.cpp:32-154): per wavenumber, the tangent optical
+  // depth of every level (trapezoid along the chord, both hemispheres, stop
+  // once tau > 10), the effective tangent height and the transit depth in ppm
+  void transmissionSpectrum(double bottom_radius, double star_radius) {
+    const size_t N = grid.nu.size(), L = nb_levels;
+    spectrum_flux.assign(N, 0.0);
+    auto dist = [&](size_t tangent, size_t level) {
+      const double a = bottom_radius + altitude[level];
+      const double b = bottom_radius + altitude[tangent];
+      return std::sqrt(a * a - b * b);
+    };
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (size_t w = 0; w < N; ++w) {
+      std::vector<double> trans(L, 0.0);
+      trans.back() = 1;
+      for (int t = int(L) - 2; t > -1; --t) {
+        double tau = 0;
+        for (size_t i = size_t(t); i < L - 1; ++i) {
+          const double path = dist(size_t(t), i + 1) - dist(size_t(t), i);
+          const double e1 = kappa[i][w] + scattering[i][w];
+          const double e2 = kappa[i + 1][w] + scattering[i + 1][w];
+          tau += path * (e1 + e2);
+          if (tau > 10) break;
+        }
+        trans[size_t(t)] = std::exp(-tau);
+      }
+      double h = 0;
+      for (size_t i = 0; i < L - 1; i++)
+        h += 2. * ((bottom_radius + altitude[i]) * (1. - trans[i]) + (bottom_radius + altitude[i + 1]) * (1. - trans[i + 1]))
+             * (altitude[i + 1] - altitude[i]) * 0.5;
+      h = std::sqrt(h + bottom_radius * bottom_radius) - bottom_radius;
+      const double r = h + bottom_radius;
+      spectrum_flux[w] = r * r / star_radius / star_radius * 1e6;
+    }
+  }
+
+  void applyModifiers(const std::vector<double>& p, size_t m) {      // forward_model.cpp:161-180
     for (size_t i = 0; i < obs.size(); ++i)
       if (obs[i].nb_modifier != 0) {
         const double shift = p[m];
         if (shift != 0) for (auto& v : bins[i]) v += shift;
         m += obs[i].nb_modifier;
       }
-    spectrum.resize(spectrum_flux.size());                           // radiative_transfer.h:58-60
-    for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] = spectrum_flux[i] / grid.lam[i] / grid.lam[i] * 10000.0;
+  }
+
+  // EmissionModel::calcModelCPU (emission.cpp:194-228), TransmissionModel::calcModelCPU
+  // (transmission.cpp:199-251), OccultationModel::calcModelCPU (secondary_eclipse.cpp:181-256)
+  // + Retrieval::logLikelihood (retrieval.cpp:314-370)
+  void evaluate(const std::vector<double>& p) {
+    if (p.size() < nbParameters()) throw std::runtime_error("too few parameters");
+    const size_t g0 = nbGeneral();
+    const std::vector<double> cp(p.begin() + g0, p.begin() + g0 + nb_chem_param);
+    const std::vector<double> tp(p.begin() + g0 + nb_chem_param, p.begin() + g0 + nb_chem_param + nb_temp_param);
+    const size_t mods = g0 + nb_chem_param + nb_temp_param;
+    bins.resize(obs.size());
+    if (model == "transmission") {
+      const double fit = (fit_mmw || fit_scale_height) ? p[3] : 0.0;
+      neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp, p[1], fit_mmw ? fit : 0.0,
+                                    fit_scale_height ? fit : 0.0);
+      opacity();
+      transmissionSpectrum(p[1], p[2]);
+      for (size_t i = 0; i < obs.size(); ++i) bins[i] = processSpectrum(grid, obs[i], spectrum_flux, false);
+      applyModifiers(p, mods);
+      spectrum = spectrum_flux;
+    } else if (model == "secondary_eclipse") {
+      neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp);
+      opacity();
+      radiativeTransfer(1.0);
+      std::vector<double> star(grid.nu.size());                     // star_blackbody.cpp:40-54
+      for (size_t i = 0; i < star.size(); ++i) star[i] = planck(p[2], grid.nu[i]) * kPi * 1e-3;
+      const double ratio = p[1];
+      for (size_t i = 0; i < obs.size(); ++i) {
+        const std::vector<double> planet_bins = processSpectrum(grid, obs[i], spectrum_flux, true);
+        const std::vector<double> star_bins = processSpectrum(grid, obs[i], star, true);
+        bins[i].assign(planet_bins.size(), 0.0);
+        for (size_t j = 0; j < planet_bins.size(); ++j)
+          bins[i][j] = planet_bins[j] / star_bins[j] * ratio * ratio * 1e6;
+      }
+      applyModifiers(p, mods);
+      spectrum.resize(star.size());
+      for (size_t i = 0; i < star.size(); ++i) spectrum[i] = spectrum_flux[i] / star[i] * ratio * ratio * 1e6;
+    } else {
+      neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp);
+      opacity();
+      double scaling = kRadiusJupiter / p[2];                          // emission.cpp:108-120
+      scaling = scaling * scaling * p[1];
+      radiativeTransfer(scaling);
+      for (size_t i = 0; i < obs.size(); ++i) bins[i] = processSpectrum(grid, obs[i], spectrum_flux, true);
+      applyModifiers(p, mods);
+      spectrum.resize(spectrum_flux.size());                           // radiative_transfer.h:58-60
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] = spectrum_flux[i] / grid.lam[i] / grid.lam[i] * 10000.0;
+    }
     const double inflation = error_inflation ? std::pow(10, p.back()) : 0.0;
     log_like = 0;
     for (size_t i = 0; i < obs.size(); ++i)

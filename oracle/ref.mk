@@ -31,7 +31,8 @@ SRCS := config/global_config.cpp \
         temperature/piecewise_poly_temperature.cpp \
         chemistry/isoprofile_chemistry.cpp chemistry/background_chemistry.cpp \
         chemistry/fixed_chemistry.cpp \
-        radiative_transfer/short_characteristics.cpp
+        radiative_transfer/short_characteristics.cpp \
+        forward_model/stellar_spectrum/star_blackbody.cpp
 
 OBJS := $(addprefix $(OBJ)/,$(subst /,_,$(SRCS:.cpp=.o)))
 

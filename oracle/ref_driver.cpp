@@ -8,6 +8,14 @@
 // (retrieval/retrieval_load_observations.cpp:68-92).  The log-likelihood sum is
 // restated here because retrieval.cpp needs Boost headers this image lacks.
 //
+// The secondary-eclipse case composes the same pieces with the blackbody
+// star as OccultationModel::calcModelCPU does (secondary_eclipse.cpp:181-256),
+// restating its 10-line ratio loop.  For the transmission case the driver
+// runs the reference's atmosphere variants and opacity
+// (TransmissionModel::calcAtmosphereStructure, transmission.cpp:155-195) and
+// writes them out; the tangent-path integral is a TransmissionModel member
+// whose class cannot be built here.
+//
 // usage: bear_ref_emission <case folder> <output folder> [n_timed_evals]
 // Outputs are .npy arrays (little-endian float64 / int64) in the output folder.
 
@@ -18,6 +26,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -28,9 +37,11 @@
 #include "forward_model/atmosphere/atmosphere.h"
 #include "transport_coeff/opacity_calc.h"
 #include "temperature/piecewise_poly_temperature.h"
+#include "temperature/constant_temperature.h"
 #include "chemistry/isoprofile_chemistry.h"
 #include "chemistry/background_chemistry.h"
 #include "radiative_transfer/short_characteristics.h"
+#include "forward_model/stellar_spectrum/star_blackbody.h"
 #include "additional/physical_const.h"
 
 namespace {
@@ -54,6 +65,8 @@ template <class T>
 std::vector<double> asDouble(const std::vector<T>& v) { return std::vector<double>(v.begin(), v.end()); }
 
 struct CaseText {
+  std::string model = "emission";
+  bool variable_gravity = false, fit_mmw = false, fit_scale_height = false;
   size_t nb_levels = 0;
   double resolution = 0, p_bottom = 0, p_top = 0;
   std::vector<std::string> temperature;
@@ -74,7 +87,11 @@ CaseText readCase(const std::string& folder) {
     std::vector<std::string> rest;
     std::string tok;
     while (in >> tok) rest.push_back(tok);
-    if (key == "spectral_resolution") c.resolution = std::stod(rest[0]);
+    if (key == "model") c.model = rest[0];
+    else if (key == "use_variable_gravity") c.variable_gravity = rest[0] == "1";
+    else if (key == "fit_mean_molecular_weight") c.fit_mmw = rest[0] == "1";
+    else if (key == "fit_scale_height") c.fit_scale_height = rest[0] == "1";
+    else if (key == "spectral_resolution") c.resolution = std::stod(rest[0]);
     else if (key == "nb_levels") c.nb_levels = std::stoul(rest[0]);
     else if (key == "p_bottom") c.p_bottom = std::stod(rest[0]);
     else if (key == "p_top") c.p_top = std::stod(rest[0]);
@@ -141,7 +158,11 @@ int main(int argc, char** argv) {
   Atmosphere atmosphere(cs.nb_levels, boundaries, false);
   OpacityCalculation opacity(&config, &grid, &atmosphere, cs.opac_symbol, cs.opac_folder, false, false);
   ShortCharacteristics rt(&grid);
-  PiecewisePolynomialTemperature temperature(std::stoul(cs.temperature[1]), std::stoul(cs.temperature[2]), boundaries);
+  std::unique_ptr<Temperature> temperature_profile;
+  if (cs.temperature[0] == "const") temperature_profile.reset(new ConstantTemperature());
+  else temperature_profile.reset(new PiecewisePolynomialTemperature(std::stoul(cs.temperature[1]),
+                                                                    std::stoul(cs.temperature[2]), boundaries));
+  Temperature& temperature = *temperature_profile;
   std::vector<Chemistry*> chemistry;
   size_t nb_chem_param = 0;
   for (auto& c : cs.chemistry) {
@@ -153,12 +174,17 @@ int main(int argc, char** argv) {
   for (auto& o : observations) nb_modifier += o.nb_modifier_param;
 
   const size_t nb_temp_param = temperature.nbParameters();
+  const bool transmission = cs.model == "transmission", eclipse = cs.model == "secondary_eclipse";
+  // general parameters (emission.cpp:68-69, transmission.cpp:69-72, secondary_eclipse.cpp:72 + star)
+  const size_t nb_general = transmission ? 3 + ((cs.fit_mmw || cs.fit_scale_height) ? 1 : 0) : 3;
   const std::vector<double>& p = cs.parameters;
-  const std::vector<double> model_param(p.begin(), p.begin() + 3);
-  const std::vector<double> chem_param(p.begin() + 3, p.begin() + 3 + nb_chem_param);
-  const std::vector<double> temp_param(p.begin() + 3 + nb_chem_param, p.begin() + 3 + nb_chem_param + nb_temp_param);
-  const std::vector<double> modifier_param(p.begin() + 3 + nb_chem_param + nb_temp_param,
-                                           p.begin() + 3 + nb_chem_param + nb_temp_param + nb_modifier);
+  const std::vector<double> model_param(p.begin(), p.begin() + nb_general);
+  const std::vector<double> chem_param(p.begin() + nb_general, p.begin() + nb_general + nb_chem_param);
+  const std::vector<double> temp_param(p.begin() + nb_general + nb_chem_param,
+                                       p.begin() + nb_general + nb_chem_param + nb_temp_param);
+  const std::vector<double> modifier_param(p.begin() + nb_general + nb_chem_param + nb_temp_param,
+                                           p.begin() + nb_general + nb_chem_param + nb_temp_param + nb_modifier);
+  StarBlackBody star(&grid);
 
   std::vector<double> spectrum, spectrum_high_res_flux;
   std::vector<std::vector<double>> spectrum_obs(observations.size());
@@ -167,20 +193,52 @@ int main(int argc, char** argv) {
   std::vector<CloudModel*> no_clouds;
 
   auto evaluate = [&]() {
-    // EmissionModel::calcAtmosphereStructure (emission.cpp:158-189)
     const double surface_gravity = std::pow(10, model_param[0]);
+    if (transmission) {
+      // TransmissionModel::calcAtmosphereStructure (transmission.cpp:155-195)
+      if (cs.fit_mmw)
+        neglect = atmosphere.calcAtmosphereStructure(surface_gravity, model_param[1], cs.variable_gravity,
+                                                     &temperature, temp_param, chemistry, chem_param, model_param[3]);
+      else if (cs.fit_scale_height)
+        neglect = atmosphere.calcAtmosphereStructure(surface_gravity, model_param[3], &temperature, temp_param,
+                                                     chemistry, chem_param);
+      else
+        neglect = atmosphere.calcAtmosphereStructure(surface_gravity, model_param[1], cs.variable_gravity,
+                                                     &temperature, temp_param, chemistry, chem_param);
+      opacity.calculate(no_clouds, std::vector<double>{});
+      spectrum.clear();
+      return;
+    }
+    // EmissionModel::calcAtmosphereStructure (emission.cpp:158-189); the eclipse
+    // model's is the same call (secondary_eclipse.cpp:159-176)
     neglect = atmosphere.calcAtmosphereStructure(surface_gravity, 1.0, false, &temperature, temp_param,
                                                  chemistry, chem_param);
     opacity.calculate(no_clouds, std::vector<double>{});
     spectrum.assign(grid.nbSpectralPoints(), 0.0);
-    // EmissionModel::radiusDistanceScaling (emission.cpp:108-120)
-    double scaling = constants::radius_jupiter / model_param[2];
-    scaling = scaling * scaling * model_param[1];
+    double scaling = 1.0;
+    if (!eclipse) {
+      // EmissionModel::radiusDistanceScaling (emission.cpp:108-120)
+      scaling = constants::radius_jupiter / model_param[2];
+      scaling = scaling * scaling * model_param[1];
+    }
     rt.calcSpectrum(atmosphere, opacity.absorption_coeff, opacity.absorption_coeff, opacity.cloud_optical_depths,
                     opacity.cloud_single_scattering, opacity.cloud_asym_param, scaling, spectrum);
     spectrum_high_res_flux = spectrum;
-    for (size_t i = 0; i < observations.size(); ++i)
-      spectrum_obs[i] = observations[i].processModelSpectrum(spectrum, true);
+    if (eclipse) {
+      std::vector<double> stellar = star.calcFlux(std::vector<double>{model_param[2]});
+      const double ratio = model_param[1];
+      for (size_t i = 0; i < observations.size(); ++i) {
+        std::vector<double> planet_obs = observations[i].processModelSpectrum(spectrum, true);
+        std::vector<double> star_obs = observations[i].processModelSpectrum(stellar, true);
+        spectrum_obs[i].assign(observations[i].nbPoints(), 0.0);
+        for (size_t j = 0; j < spectrum_obs[i].size(); ++j)
+          spectrum_obs[i][j] = planet_obs[j] / star_obs[j] * ratio * ratio * 1e6;
+      }
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] = spectrum[i] / stellar[i] * ratio * ratio * 1e6;
+    } else {
+      for (size_t i = 0; i < observations.size(); ++i)
+        spectrum_obs[i] = observations[i].processModelSpectrum(spectrum, true);
+    }
     // ForwardModel::applyObservationModifier (forward_model.cpp:161-180)
     auto it = modifier_param.begin();
     for (size_t i = 0; i < observations.size(); ++i)
@@ -189,7 +247,7 @@ int main(int argc, char** argv) {
         if (shift != 0) for (auto& s : spectrum_obs[i]) s += shift;
         it += observations[i].nb_modifier_param;
       }
-    rt.changeSpectrumUnits(spectrum);
+    if (!eclipse) rt.changeSpectrumUnits(spectrum);
     // Gaussian log-likelihood, restating retrieval.cpp:330-362
     const double inflation = config.use_error_inflation ? std::pow(10, p.back()) : 0.0;
     log_like = 0;

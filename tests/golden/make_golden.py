@@ -55,34 +55,55 @@ def reference_bins(folder, params, work):
 def main():
     if not os.path.exists(api.REF_BIN):
         sys.exit("build the reference first: make -f oracle/ref.mk")
+    import json
     tmp = tempfile.mkdtemp()
     try:
         for name in cases.CASES:
             folder = os.path.join(tmp, name)
             work = os.path.join(tmp, name + "_out")
-            folder, params = cases.build(name, folder, model_fn=lambda f, p: reference_bins(f, p, work)[1])
+            transmission = name.startswith("transmission")
+            # the reference build has no transmission integral: its data are
+            # injected from the oracle, and its fixtures hold the reference's
+            # atmosphere and opacity (the integral is pinned by closed forms)
+            model_fn = None if transmission else (lambda f, p: reference_bins(f, p, work)[1])
+            folder, params = cases.build(name, folder, model_fn=model_fn)
             fixture = {"parameters": np.array(params), "table_digest": np.array(table_digest(folder))}
-            import json
             meta = json.load(open(os.path.join(folder, "case.json")))
             for k, p in enumerate(params):
                 shutil.rmtree(work, ignore_errors=True)
-                out, _ = reference_bins(folder, p, work)
+                out, _ = reference_bins(folder, p, work) if not transmission else (None, None)
+                if transmission:
+                    shutil.rmtree(work, ignore_errors=True)
+                    txt = [l for l in open(os.path.join(folder, "case.txt")).read().splitlines()
+                           if not l.startswith("parameters ")]
+                    txt.append(f"parameters {len(p)} " + " ".join(repr(float(v)) for v in p))
+                    open(os.path.join(folder, "case.txt"), "w").write("\n".join(txt) + "\n")
+                    out = api.run_reference(folder, work)
                 pre = f"p{k}_"
-                for key in ("temperature", "altitude", "loglike"):
+                for key in ("temperature", "altitude"):
                     fixture[pre + key] = out[key]
-                # the high-resolution flux in full for the first vector, every 10th point otherwise
+                if transmission:
+                    if name == "transmission_plain" and k == 0:
+                        # extinction inputs on every 5th wavenumber
+                        fixture[pre + "absorption_coeff"] = out["absorption_coeff"][::5]
+                        fixture[pre + "scattering_coeff"] = out["scattering_coeff"][::5]
+                    continue
+                fixture[pre + "loglike"] = out["loglike"]
+                # the high-resolution spectrum in full for the first vector, every 10th point otherwise
                 fixture[pre + "spectrum_flux"] = out["spectrum_flux"] if k == 0 else out["spectrum_flux"][::10]
+                fixture[pre + "spectrum"] = out["spectrum"][::10]
                 for i in range(len(meta["observations"])):
                     fixture[pre + f"obs{i}_bins"] = out[f"obs{i}_bins"]
-                if k == 0:
-                    for key in ("wavenumber", "index_list", "pressure", "number_densities"):
-                        fixture[key] = out[key]
-                    for i in range(len(meta["observations"])):
-                        for key in ("edge_indices", "index_range", "conv_intervals", "sigma", "filter",
-                                    "filter_norm", "data", "error", "weight", "ascending"):
-                            if f"obs{i}_{key}" in out:
-                                fixture[f"obs{i}_{key}"] = out[f"obs{i}_{key}"]
-            path = os.path.join(ROOT, "tests", "golden", f"emission_{name.replace('emission_', '')}.npz")
+            for key in ("wavenumber", "index_list", "pressure", "number_densities"):
+                fixture[key] = out[key]
+            for i in range(len(meta["observations"])):
+                for key in ("edge_indices", "index_range", "conv_intervals", "sigma", "filter",
+                            "filter_norm", "data", "error", "weight", "ascending"):
+                    if f"obs{i}_{key}" in out:
+                        fixture[f"obs{i}_{key}"] = out[f"obs{i}_{key}"]
+            path = os.path.join(ROOT, "tests", "golden", f"{name if not name.startswith('emission_') else name}.npz")
+            if name == "tiny":
+                path = os.path.join(ROOT, "tests", "golden", "emission_tiny.npz")
             np.savez_compressed(path, **fixture)
             print(path, os.path.getsize(path), "bytes")
     finally:

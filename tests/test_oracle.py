@@ -16,16 +16,22 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def _golden(name):
-    return np.load(os.path.join(GOLDEN, f"emission_{name.replace('emission_', '')}.npz"))
+    fname = "emission_tiny.npz" if name == "tiny" else f"{name}.npz"
+    return np.load(os.path.join(GOLDEN, fname))
 
 
-@pytest.mark.parametrize("name", ["tiny", "emission_c1"])
+ALL_CASES = ["tiny", "emission_c1", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
+             "transmission_fit_mmw", "transmission_fit_scale_height"]
+
+
+@pytest.mark.parametrize("name", ALL_CASES)
 def test_oracle_matches_reference_fixtures(case_factory, name):
     from golden.make_golden import table_digest
     folder, params = case_factory(name)
     g = _golden(name)
     assert str(g["table_digest"]) == table_digest(folder), "regenerated tables differ from the fixture's"
     np.testing.assert_array_equal(np.array(params), g["parameters"])
+    transmission = name.startswith("transmission")
     o = api.EmissionOracle(folder)
     nu, lam, idx = o.grid()
     np.testing.assert_array_equal(nu, g["wavenumber"])
@@ -35,6 +41,7 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
         np.testing.assert_array_equal(ob["edge_indices"], g[f"obs{i}_edge_indices"].astype(np.int64))
         np.testing.assert_array_equal(ob["index_range"], g[f"obs{i}_index_range"].astype(np.int64))
         np.testing.assert_array_equal(ob["data"], g[f"obs{i}_data"])
+        np.testing.assert_array_equal(ob["weight"], g[f"obs{i}_weight"])
         if f"obs{i}_conv_intervals" in g:
             np.testing.assert_array_equal(ob["conv"], g[f"obs{i}_conv_intervals"].astype(np.int64))
             np.testing.assert_array_equal(ob["sigma"], g[f"obs{i}_sigma"])
@@ -44,11 +51,18 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
     for k, p in enumerate(params):
         r = o.evaluate(p)
         pre = f"p{k}_"
-        flux = r["spectrum_flux"] if k == 0 else r["spectrum_flux"][::10]
-        np.testing.assert_allclose(flux, g[pre + "spectrum_flux"], rtol=1e-13, atol=0)
         a = o.atmosphere()
         np.testing.assert_allclose(a["temperature"], g[pre + "temperature"], rtol=1e-14)
         np.testing.assert_allclose(a["altitude"], g[pre + "altitude"], rtol=1e-13)
+        if transmission:
+            if pre + "absorption_coeff" in g:
+                kap, sca = o.coefficients()
+                np.testing.assert_allclose(kap.T[::5], g[pre + "absorption_coeff"], rtol=1e-13)
+                np.testing.assert_allclose(sca.T[::5], g[pre + "scattering_coeff"], rtol=1e-13)
+            continue
+        flux = r["spectrum_flux"] if k == 0 else r["spectrum_flux"][::10]
+        np.testing.assert_allclose(flux, g[pre + "spectrum_flux"], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(r["spectrum"][::10], g[pre + "spectrum"], rtol=1e-13, atol=0)
         for i in range(o.nb_observations):
             np.testing.assert_allclose(r["bins"][i], g[pre + f"obs{i}_bins"], rtol=1e-13)
         ref_ll, ref_neglect = g[pre + "loglike"]
@@ -57,8 +71,9 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
 
 
 @pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
-def test_oracle_intermediates_match_live_reference(case_factory, tmp_path):
-    folder, params = case_factory("tiny")
+@pytest.mark.parametrize("name", ["tiny", "eclipse_small", "transmission_plain", "transmission_variable_gravity"])
+def test_oracle_intermediates_match_live_reference(case_factory, tmp_path, name):
+    folder, params = case_factory(name)
     ref = api.run_reference(folder, str(tmp_path / "ref"))
     o = api.EmissionOracle(folder)
     meta = json.load(open(os.path.join(folder, "case.json")))
@@ -67,7 +82,9 @@ def test_oracle_intermediates_match_live_reference(case_factory, tmp_path):
     np.testing.assert_array_equal(k.T, ref["absorption_coeff"])
     np.testing.assert_array_equal(s.T, ref["scattering_coeff"])
     np.testing.assert_array_equal(o.atmosphere()["number_densities"], ref["number_densities"])
-    np.testing.assert_array_equal(r["spectrum"], ref["spectrum"])
+    np.testing.assert_array_equal(o.atmosphere()["altitude"], ref["altitude"])
+    if not name.startswith("transmission"):
+        np.testing.assert_array_equal(r["spectrum"], ref["spectrum"])
 
 
 def _planck(t, nu):
@@ -164,3 +181,68 @@ def test_affine_log_table_is_interpolated_exactly(tmp_path):
         assert abs(k[l, 0] - expected) <= 1e-12 * expected
         # and the affine function itself, to float32 table precision
         assert abs(np.log10(k[l, 0] / (nd[l, 11] + nd[l, 14])) - (a0 + at * T[l] + ap * np.log10(P[l]))) < 1e-5
+
+
+# ---------------------------------------------------------------- transmission closed forms
+def _affine_transmission_case(tmp_path, log_sigma, mixing=1e-4, tag="t"):
+    """Transmission case whose two absorbers share a table
+    log10 sigma = log_sigma(T, p) (affine in log10 p, so interpolated
+    exactly), isothermal, no CIA and no Rayleigh."""
+    case = synth.transmission_small()
+    t, pr = case.species[0].temperatures, case.species[0].pressures
+    nu = case.wavenumber_full
+    lg = np.asarray(log_sigma(t[:, None, None], pr[None, :, None]) + 0.0 * nu[None, None, :])
+    for s in case.species[:2]:
+        s.values = lg.astype(np.float32)
+        s.log_storage, s.mass = True, 0.0
+    case.species = case.species[:2]
+    case.rayleigh = []
+    folder = synth.write_case(case, str(tmp_path / tag))
+    p = case.parameters.copy()
+    p[3] = p[4] = mixing
+    return folder, p
+
+
+def test_transparent_atmosphere_depth(tmp_path):
+    """No absorbers: every chord transmits, h = 0 and depth = (R_p / R_*)^2 1e6."""
+    folder, p = _affine_transmission_case(tmp_path, lambda t, pr: -30.0 + 0 * t * pr, mixing=0.0)
+    r = api.EmissionOracle(folder).evaluate(p)
+    np.testing.assert_allclose(r["spectrum"], (p[1] / p[2]) ** 2 * 1e6, rtol=1e-14)
+
+
+def test_opaque_atmosphere_depth(tmp_path):
+    """Every chord below the top level opaque: the trapezoid of 2 r dr is
+    exact, so (R + h)^2 = (R + z_{L-2})^2 + (R + z_{L-2}) dz_{L-2} (the top
+    level transmits)."""
+    folder, p = _affine_transmission_case(tmp_path, lambda t, pr: -8.0 + 0 * t * pr, mixing=1e-2)
+    o = api.EmissionOracle(folder)
+    r = o.evaluate(p)
+    z = o.atmosphere()["altitude"]
+    R = p[1]
+    r2 = (R + z[-2]) ** 2 + (R + z[-2]) * (z[-1] - z[-2])
+    np.testing.assert_allclose(r["spectrum"], r2 / p[2] ** 2 * 1e6, rtol=1e-12)
+
+
+def test_constant_extinction_telescopes(tmp_path):
+    """log10 sigma = a - log10 p with an isothermal atmosphere gives the same
+    extinction k at every level, so the chord sum telescopes: tau_t =
+    2 k sqrt((R + z_top)^2 - (R + z_t)^2) (no cutoff reached)."""
+    a = -24.0
+    folder, p = _affine_transmission_case(tmp_path, lambda t, pr: a - np.log10(pr) + 0 * t, mixing=1e-4)
+    o = api.EmissionOracle(folder)
+    r = o.evaluate(p)
+    atm = o.atmosphere()
+    z, T, P = atm["altitude"], atm["temperature"][0], atm["pressure"]
+    kap, _ = o.coefficients()
+    n_tot = P * 1e6 / synth.BOLTZMANN_K / T
+    k_expected = 2 * 1e-4 * n_tot * 10 ** a / P             # two absorbers, sigma = 10^a / p
+    np.testing.assert_allclose(kap[:, 0], k_expected, rtol=2e-6)   # float32 table nodes
+    k = kap[:, 0].mean()
+    R = p[1]
+    b = R + z
+    tau = 2 * k * np.sqrt((R + z[-1]) ** 2 - b ** 2)
+    assert tau.max() < 10
+    trans = np.exp(-tau)
+    H = np.sum(2 * ((R + z[:-1]) * (1 - trans[:-1]) + (R + z[1:]) * (1 - trans[1:])) * np.diff(z) * 0.5)
+    depth = (np.sqrt(H + R * R)) ** 2 / p[2] ** 2 * 1e6
+    np.testing.assert_allclose(r["spectrum"], depth, rtol=1e-6)
