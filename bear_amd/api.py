@@ -61,6 +61,13 @@ class EvalStats(C.Structure):
                 ("kernel_launches", C.c_size_t), ("kernel_ms_total", C.c_double)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile", C.c_int), ("grid", C.c_int),
+                ("used_graph", C.c_int), ("evaluations", C.c_size_t), ("gpu_ms_total", C.c_double)]
+
+
+TABLE_PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+
 _lib = None
 
 
@@ -92,6 +99,11 @@ def lib():
         L.bear_retrieval_set_profiling.argtypes = [vp, C.c_int]
         L.bear_retrieval_stats.argtypes = [vp, C.POINTER(EvalStats)]
         L.bear_forward_model_calc_model_gpu.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
+        L.bear_retrieval_set_table_precision.argtypes = [vp, C.c_int]
+        L.bear_retrieval_set_batch_options.argtypes = [vp, C.c_int, C.c_int]
+        L.bear_retrieval_compute_likelihood_batch.argtypes = [vp, _dp, sz, sz, _dp]
+        L.bear_retrieval_compute_model_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp, C.POINTER(C.c_int)]
+        L.bear_retrieval_batch_stats.argtypes = [vp, C.POINTER(BatchStats)]
         _lib = L
     return _lib
 
@@ -264,6 +276,44 @@ class Retrieval:
             out.append(obs[k:k + n].copy())
             k += n
         return ForwardModelOutput(bool(neg.value), spec if spec is not None else np.zeros(0), out)
+
+    # ---- batches: parameters [batch, nb_parameters]
+    def set_table_precision(self, precision="fp64"):
+        """Table storage of the batched path: "fp64", "fp32", "fp16" or "bf16"."""
+        if precision not in TABLE_PRECISIONS:
+            raise BearError(-1, f"unknown table precision {precision}")
+        _check(self._L.bear_retrieval_set_table_precision(self._h, TABLE_PRECISIONS[precision]))
+
+    def set_batch_options(self, use_graph=True, autotune=True):
+        _check(self._L.bear_retrieval_set_batch_options(self._h, int(use_graph), int(autotune)))
+
+    def computeLikelihoodBatch(self, physical_parameters) -> np.ndarray:
+        """log-likelihoods of a batch of live points, one launch sequence."""
+        p = np.ascontiguousarray(np.atleast_2d(physical_parameters), dtype=np.float64)
+        out = np.empty(p.shape[0])
+        _check(self._L.bear_retrieval_compute_likelihood_batch(self._h, _ptr(p), p.shape[0], p.shape[1], _ptr(out)))
+        return out
+
+    def computeModelBatch(self, physical_parameters, return_high_res_spectrum=True):
+        """(spectra [batch, N] or None, per observation bins [batch, points], neglect [batch])."""
+        p = np.ascontiguousarray(np.atleast_2d(physical_parameters), dtype=np.float64)
+        b = p.shape[0]
+        spec = np.empty((b, self.nb_spectral_points)) if return_high_res_spectrum else None
+        total = sum(self.nb_observation_points)
+        obs = np.empty((b, total))
+        neg = np.zeros(b, dtype=np.int32)
+        _check(self._L.bear_retrieval_compute_model_batch(self._h, _ptr(p), b, p.shape[1], _ptr(spec), _ptr(obs),
+                                                          neg.ctypes.data_as(C.POINTER(C.c_int))))
+        out, k = [], 0
+        for n in self.nb_observation_points:
+            out.append(obs[:, k:k + n].copy())
+            k += n
+        return spec, out, neg.astype(bool)
+
+    def batch_stats(self) -> dict:
+        st = BatchStats()
+        _check(self._L.bear_retrieval_batch_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
 
     def set_profiling(self, enable=True):
         _check(self._L.bear_retrieval_set_profiling(self._h, int(enable)))

@@ -1,0 +1,740 @@
+// Batched likelihood evaluation on gfx950: many live points of one retrieval
+// per launch sequence.
+//
+// One work item is (wavenumber tile, live point).  The spectrum kernels are
+// persistent: a grid sized by the autotuner walks the items, ordered so that
+// every live point of one tile runs on the same XCD (workgroups go to XCDs
+// round-robin by id, and the grid is a multiple of 8), where the tile's
+// table rows stay in that XCD's L2 while the points reuse them.
+//
+// Per batch, in one hipGraph when enabled: the pinned per-point records go to
+// the device in one copy; (transmission) the chord lengths of every point are
+// computed; the spectrum kernel writes [point][wavenumber] FP64; one
+// workgroup per (bin, point) folds the observation operator and the Gaussian
+// term; one workgroup per point sums its terms; the log-likelihoods come back
+// in one copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "device_math.h"
+#include "engine.h"
+
+namespace bear_amd {
+
+using namespace dev;
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? kOutOfMemory : kDevice;
+  raise(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* deviceAlloc(size_t count, const char* what) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  check(hipMalloc(&p, count * sizeof(T)), what);
+  return static_cast<T*>(p);
+}
+
+void deviceFree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+constexpr int kBatchBlock = 256;
+constexpr float kLog2TenF = 3.32192809488736234787f;
+constexpr size_t kLdsLimit = 64 * 1024;
+
+// ---------------------------------------------------------------- storage types
+struct TabF32 {
+  using T = float;
+  static __device__ __forceinline__ float load(const T* p) { return *p; }
+  static __device__ __forceinline__ T store(double v) { return float(v); }
+};
+struct TabF16 {
+  using T = _Float16;
+  static __device__ __forceinline__ float load(const T* p) { return float(*p); }
+  static __device__ __forceinline__ T store(double v) { return _Float16(float(v)); }
+};
+struct TabBF16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float load(const T* p) { return __uint_as_float(uint32_t(*p) << 16); }
+  static __device__ __forceinline__ T store(double v) {   // round to nearest even
+    const uint32_t u = __float_as_uint(float(v));
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
+};
+
+// Species s's rows [r0, r1) at wavenumber i: base = FP32 midrange of the
+// values, residual = value - base.  Nothing is floored: log-linear
+// interpolation between a row at the reference's 1e-200 floor and a row far
+// above it yields values in between that can dominate a level's opacity.
+template <class Tab>
+__global__ __launch_bounds__(kBatchBlock) void splitTableKernel(const double* __restrict__ tables, size_t stride, int n,
+                                                                uint32_t r0, uint32_t r1, int slot,
+                                                                float* __restrict__ base,
+                                                                typename Tab::T* __restrict__ low) {
+  const int i = blockIdx.x * kBatchBlock + threadIdx.x;
+  if (i >= n) return;
+  double hi = -INFINITY, lo = INFINITY;
+  for (uint32_t r = r0; r < r1; ++r) {
+    const double v = tables[size_t(r) * stride + i];
+    hi = fmax(hi, v);
+    lo = fmin(lo, v);
+  }
+  const float b = float(0.5 * (hi + lo));
+  base[size_t(slot) * stride + i] = b;
+  for (uint32_t r = r0; r < r1; ++r)
+    low[size_t(r) * stride + i] = Tab::store(tables[size_t(r) * stride + i] - double(b));
+}
+
+// ---------------------------------------------------------------- opacity per level
+// FP64 tables and arithmetic: the single-model path's interpolation.
+template <int NS>
+struct Opacity64 {
+  static constexpr int kNS = NS;
+  using Species = LevelSpecies;
+  using K = double;
+  const double* tab;
+  size_t stride;
+  double v[NS][4];
+  __device__ __forceinline__ void init(const void* tables, const float*, size_t s, int i) {
+    tab = static_cast<const double*>(tables) + i;
+    stride = s;
+  }
+  __device__ __forceinline__ double kappa(const LevelSpecies* d, bool force) {
+    return cachedKappa<NS>(tab, stride, d, force, v);
+  }
+};
+
+// Reduced-precision tables: x = log2(10) base + log2 n + sum_k w_k v_k in
+// FP32 and sigma n = 2^x with the hardware exponential.
+template <int NS, class Tab>
+struct Opacity32 {
+  static constexpr int kNS = NS;
+  using Species = LevelSpecies32;
+  using K = float;
+  const typename Tab::T* tab;
+  size_t stride;
+  float bz[NS];
+  float v[NS][4];
+  __device__ __forceinline__ void init(const void* tables, const float* base, size_t s, int i) {
+    tab = static_cast<const typename Tab::T*>(tables) + i;
+    stride = s;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) bz[q] = base[size_t(q) * s + i] * kLog2TenF;
+  }
+  __device__ __forceinline__ float kappa(const LevelSpecies32* d, bool force) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (force || d[s].reload) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[s][q] = Tab::load(tab + size_t(d[s].row[q]) * stride);
+      }
+    float k = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      float x = bz[s] + d[s].log2_n;
+      x = fmaf(d[s].w[0], v[s][0], x);
+      x = fmaf(d[s].w[1], v[s][1], x);
+      x = fmaf(d[s].w[2], v[s][2], x);
+      x = fmaf(d[s].w[3], v[s][3], x);
+      k += __builtin_amdgcn_exp2f(x);
+    }
+    return k;
+  }
+};
+
+// ---------------------------------------------------------------- work items
+struct Records {
+  const unsigned char* base;
+  size_t bytes, levels, species, shifts;
+  __device__ __forceinline__ const unsigned char* at(int m) const { return base + size_t(m) * bytes; }
+};
+
+// item -> (tile, point): XCD x = item mod 8 owns tiles x, x + 8, ...; its
+// items run through every point of one tile before the next tile
+__device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile) {
+  const size_t k = item / 8;
+  point = int(k % size_t(batch));
+  tile = int(k / size_t(batch)) * 8 + int(item % 8);
+  return tile < tiles;
+}
+
+__host__ __device__ inline size_t nbItems(int tiles, int batch) { return size_t((tiles + 7) / 8) * 8 * size_t(batch); }
+
+// ---------------------------------------------------------------- emission
+// Fused opacity + two-angle short characteristics per (wavenumber, point),
+// as the single-model kernel; the flux F = 2 pi sum mu w I 1e-3 scaling goes
+// to flux[point][i].
+template <class Op>
+__global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
+    const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
+    size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux) {
+  using Species = typename Op::Species;
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;
+    const int i = tile * int(blockDim.x) + int(threadIdx.x);
+    if (i >= n) continue;
+    const unsigned char* r = rec.at(m);
+    const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+    const Species* species = reinterpret_cast<const Species*>(r + rec.species);
+    Op op;
+    op.init(tables, base, stride, i);
+    const double nu = wavenumber[i];
+    const double b_num = kPlanckC1 * nu * nu * nu;
+    const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+    double k_prev = double(op.kappa(species, true));
+    double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
+    double z_prev = levels[0].altitude;
+    double i1 = b_prev, i2 = b_prev;
+    for (int l = 1; l < nb_levels; ++l) {
+      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, false));
+      const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
+      const double z_next = levels[l].altitude;
+      shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+      k_prev = k_next;
+      b_prev = b_next;
+      z_prev = z_next;
+    }
+    flux[size_t(m) * stride + i] = 2.0 * kPi * (i1 * kMu1 * kW + i2 * kMu2 * kW) * 1e-3 * bm->scaling;
+  }
+}
+
+// ---------------------------------------------------------------- transmission
+// Chord lengths of every point: chord[point][t][q] =
+// sqrt((R+z_{q+1})^2 - (R+z_t)^2) - sqrt((R+z_q)^2 - (R+z_t)^2), q >= t
+// (transmission_spectrum.cpp:117-118, 145-154); one workgroup per (t, point).
+// Evaluated without contraction, as the host does: fused, a*a - b*b at a == b
+// leaves the rounding error of b*b, which can be negative.
+__global__ __launch_bounds__(kBatchBlock) void chordKernel(Records rec, int nb_levels, double* __restrict__ chord,
+                                                           float* __restrict__ chord32) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x, m = blockIdx.y;
+  const unsigned char* r = rec.at(m);
+  const double r0 = reinterpret_cast<const BatchModel*>(r)->bottom_radius;
+  const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+  const double b = r0 + levels[t].altitude;
+  const size_t row = (size_t(m) * nb_levels + t) * nb_levels;
+  for (int q = t + int(threadIdx.x); q < nb_levels - 1; q += kBatchBlock) {
+    const double a1 = r0 + levels[q + 1].altitude, a0 = r0 + levels[q].altitude;
+    const double c = sqrt(a1 * a1 - b * b) - sqrt(a0 * a0 - b * b);
+    chord[row + q] = c;
+    if (chord32) chord32[row + q] = float(c);
+  }
+}
+
+__device__ __forceinline__ double transmittance(double tau) { return pow2(-tau * kLog2E); }
+__device__ __forceinline__ double transmittance(float tau) {
+  return double(__builtin_amdgcn_exp2f(-tau * 1.44269504088896340736f));
+}
+
+// Transit depth per (wavenumber, point) (transmission_spectrum.cpp:32-154).
+// One work item = `tile` wavenumbers of one point:
+//  1. extinction K[level][tile] in LDS (Op::K: FP64 or FP32), threads split
+//     over level ranges with register-cached stencils;
+//  2. the (tangent level t, wavenumber) pairs: tau summed outward from t in
+//     the reference's order, stopping once tau > 10; each pair adds
+//     c_t (1 - e^-tau) to its thread's FP64 partial, where
+//     c_t = (R+z_t)(z_{t+1} - z_{t-1}) is the coefficient of (1 - T_t) in the
+//     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
+//  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R
+//     and depth (R + h)^2 / R_*^2 1e6 in FP64.
+template <class Op>
+__global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
+    const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
+    Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
+    int tiles, int tile, int batch, double* __restrict__ depth) {
+  using K = typename Op::K;
+  using Species = typename Op::Species;
+  extern __shared__ double lds[];
+  K* kx = reinterpret_cast<K*>(lds);
+  double* part = lds + (size_t(nb_levels) * tile * sizeof(K) + 7) / 8;
+  const int L = nb_levels;
+  const int groups = kBatchBlock / tile;
+  const int j = int(threadIdx.x) % tile;
+  const int g = int(threadIdx.x) / tile;
+  const int chunk = (L + groups - 1) / groups;
+  const int l0 = g * chunk;
+  const int l1 = min(L, l0 + chunk);
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tl;
+    if (!workItem(item, batch, tiles, m, tl)) continue;      // uniform over the workgroup
+    const unsigned char* r = rec.at(m);
+    const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+    const Species* species = reinterpret_cast<const Species*>(r + rec.species);
+    const int i = tl * tile + j;
+    if (i < n) {
+      Op op;
+      op.init(tables, base, stride, i);
+      const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
+      for (int l = l0; l < l1; ++l) {
+        const double k = double(op.kappa(species + size_t(l) * Op::kNS, l == l0));
+        kx[l * tile + j] = K(k + sigma_r * levels[l].rayleigh_density);
+      }
+    } else {
+      for (int l = l0; l < l1; ++l) kx[l * tile + j] = K(0);
+    }
+    __syncthreads();
+    const double r0 = bm->bottom_radius;
+    double h = 0.0;
+    for (int p = int(threadIdx.x); p < tile * (L - 1); p += kBatchBlock) {
+      const int t = p / tile;                 // p % tile == j
+      const size_t row = (size_t(m) * L + t) * L;
+      K tau = K(0);
+      K kq = kx[t * tile + j];
+      for (int q = t; q < L - 1; ++q) {
+        const K kn = kx[(q + 1) * tile + j];
+        K c;
+        if constexpr (sizeof(K) == 8) c = chord64[row + q];
+        else c = chord32[row + q];
+        tau += c * (kq + kn);
+        kq = kn;
+        if (tau > K(10)) break;
+      }
+      const double zl = levels[t > 0 ? t - 1 : 0].altitude;
+      const double zu = levels[t + 1].altitude;
+      h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - transmittance(tau));
+    }
+    part[g * tile + j] = h;
+    __syncthreads();
+    if (int(threadIdx.x) < tile && i < n) {
+      double hs = 0.0;
+      for (int q = 0; q < groups; ++q) hs += part[q * tile + j];
+      const double hh = sqrt(hs + r0 * r0) - r0;
+      const double rr = hh + r0;
+      depth[size_t(m) * stride + i] = rr * rr / bm->star_radius / bm->star_radius * 1e6;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- bins and likelihood
+// One workgroup per (bin, point); mode 1 (eclipse) divides by the folded
+// blackbody star of that point.
+__global__ __launch_bounds__(kBlockBins) void binBatchKernel(
+    const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
+    const double* __restrict__ weights, const BinDesc* __restrict__ bins, int nb_bins, Records rec, int mode,
+    double* __restrict__ out, double* __restrict__ terms) {
+  const int m = blockIdx.y;
+  const unsigned char* r = rec.at(m);
+  const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+  const double* shifts = reinterpret_cast<const double*>(r + rec.shifts);
+  const size_t o = size_t(m) * nb_bins + blockIdx.x;
+  binBody(flux + size_t(m) * stride, wavenumber, weights, bins[blockIdx.x], mode, bm->star_inv_temperature,
+          bm->depth_scale, shifts, bm->inflation, out + o, terms + o);
+}
+
+__global__ __launch_bounds__(1024) void sumBatchKernel(const double* __restrict__ terms, int n,
+                                                       double* __restrict__ out) {
+  rowSum(terms + size_t(blockIdx.x) * n, n, out + blockIdx.x);
+}
+
+// high-resolution output per point: emission F per micron, eclipse depth
+// F / F_* (Rp/Rs)^2 1e6, transmission depth
+__global__ __launch_bounds__(kBatchBlock) void batchSpectrumKernel(
+    const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
+    const double* __restrict__ wavelength, Records rec, int mode, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * kBatchBlock + threadIdx.x;
+  const int m = blockIdx.y;
+  if (i >= n) return;
+  const BatchModel* bm = reinterpret_cast<const BatchModel*>(rec.at(m));
+  const double f = flux[size_t(m) * stride + i];
+  double v = f;
+  if (mode == 0) {
+    const double lam = wavelength[i];
+    v = f / lam / lam * 10000.0;
+  } else if (mode == 1) {
+    v = f * recip(starFlux(wavenumber[i], bm->star_inv_temperature)) * bm->depth_scale;
+  }
+  out[size_t(m) * n + i] = v;
+}
+
+// ---------------------------------------------------------------- launch helpers
+template <int NS>
+void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const void* tab64,
+                    const void* tab_low, const float* base, size_t stride, Records rec, int L, int n, int tiles,
+                    int batch, double* flux) {
+  switch (p) {
+    case TablePrecision::fp64:
+      hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, tab64, base, stride, rec, L, n,
+                         tiles, batch, flux);
+      break;
+    case TablePrecision::fp32:
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, 0, s, nu, tab_low, base, stride,
+                         rec, L, n, tiles, batch, flux);
+      break;
+    case TablePrecision::fp16:
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, 0, s, nu, tab_low, base, stride,
+                         rec, L, n, tiles, batch, flux);
+      break;
+    case TablePrecision::bf16:
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, tab_low, base, stride,
+                         rec, L, n, tiles, batch, flux);
+      break;
+  }
+}
+
+template <int NS>
+void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
+                        const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
+                        const double* chord, const float* chord32, int L, int n, int tiles, int tile, int batch,
+                        double* depth) {
+  const dim3 block(kBatchBlock);
+  switch (p) {
+    case TablePrecision::fp64:
+      hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
+                         rec, chord, chord32, L, n, tiles, tile, batch, depth);
+      break;
+    case TablePrecision::fp32:
+      hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+      break;
+    case TablePrecision::fp16:
+      hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+      break;
+    case TablePrecision::bf16:
+      hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+      break;
+  }
+}
+
+size_t align(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+size_t transmissionLds(int L, int tile, bool wide) {
+  const size_t k = size_t(L) * tile * (wide ? 8 : 4);
+  return align(k, 8) + size_t(kBatchBlock / tile) * tile * sizeof(double);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- precision
+void DeviceEngine::setTablePrecision(TablePrecision p,
+                                     const std::vector<std::pair<uint32_t, uint32_t>>& species_rows) {
+  check(hipSetDevice(device_), "hipSetDevice");
+  releaseGraph();
+  deviceFree(tables_low_);
+  deviceFree(table_base_);
+  tables_low_ = nullptr;
+  table_base_ = nullptr;
+  precision_ = p;
+  batch_capacity_ = 0;                                  // record layout depends on the precision
+  tuned_.clear();
+  if (p == TablePrecision::fp64) return;
+  if (species_rows.size() > species_stride_) raise(kInvalidInput, "more species than stencil slots");
+  const size_t elem = p == TablePrecision::fp32 ? 4 : 2;
+  void* low = nullptr;
+  check(hipMalloc(&low, std::max<size_t>(1, n_rows_) * stride_ * elem), "reduced-precision tables");
+  tables_low_ = low;
+  table_base_ = deviceAlloc<float>(species_stride_ * stride_, "table base");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  // padding slots keep a zero base (their density is zero)
+  check(hipMemsetAsync(table_base_, 0, species_stride_ * stride_ * sizeof(float), s), "clear table base");
+  const int n = int(n_points_);
+  const dim3 grid((n + kBatchBlock - 1) / kBatchBlock), block(kBatchBlock);
+  for (size_t q = 0; q < species_rows.size(); ++q) {
+    const uint32_t r0 = species_rows[q].first, r1 = species_rows[q].second;
+    if (r1 <= r0 || r1 > n_rows_) raise(kInternal, "species rows out of range");
+    switch (p) {
+      case TablePrecision::fp32:
+        hipLaunchKernelGGL(splitTableKernel<TabF32>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
+                           table_base_, static_cast<float*>(low));
+        break;
+      case TablePrecision::fp16:
+        hipLaunchKernelGGL(splitTableKernel<TabF16>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
+                           table_base_, static_cast<_Float16*>(low));
+        break;
+      default:
+        hipLaunchKernelGGL(splitTableKernel<TabBF16>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
+                           table_base_, static_cast<uint16_t*>(low));
+    }
+    check(hipGetLastError(), "table split kernel");
+  }
+  check(hipStreamSynchronize(s), "table split");
+}
+
+// ---------------------------------------------------------------- staging
+LevelSpecies* DeviceEngine::batchSpecies(size_t b) {
+  if (precision_ != TablePrecision::fp64) raise(kInternal, "FP64 stencils requested with reduced-precision tables");
+  return reinterpret_cast<LevelSpecies*>(record(b) + rec_species_);
+}
+
+LevelSpecies32* DeviceEngine::batchSpecies32(size_t b) {
+  if (precision_ == TablePrecision::fp64) raise(kInternal, "FP32 stencils requested with FP64 tables");
+  return reinterpret_cast<LevelSpecies32*>(record(b) + rec_species_);
+}
+
+void DeviceEngine::reserveBatch(size_t batch) {
+  if (batch == 0) raise(kInvalidInput, "empty batch");
+  if (batch > 65535) raise(kInvalidInput, "at most 65535 live points per batch");
+  if (batch <= batch_capacity_) return;
+  check(hipSetDevice(device_), "hipSetDevice");
+  check(hipStreamSynchronize(static_cast<hipStream_t>(stream_)), "batch");
+  releaseGraph();
+  const size_t L = nb_levels_, NS = species_stride_;
+  const size_t spc = precision_ == TablePrecision::fp64 ? sizeof(LevelSpecies) : sizeof(LevelSpecies32);
+  rec_levels_ = align(sizeof(BatchModel), 16);
+  rec_species_ = align(rec_levels_ + L * sizeof(LevelState), 16);
+  rec_shifts_ = align(rec_species_ + L * NS * spc, 16);
+  rec_bytes_ = align(rec_shifts_ + (n_obs_ + 1) * sizeof(double), 256);
+  if (batch_host_) (void)hipHostFree(batch_host_);
+  if (batch_ll_host_) (void)hipHostFree(batch_ll_host_);
+  batch_host_ = nullptr;
+  batch_ll_host_ = nullptr;
+  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_terms_, (void*)batch_ll_,
+                  (void*)batch_chord_, (void*)batch_chord32_})
+    deviceFree(p);
+  batch_dev_ = nullptr;
+  batch_flux_ = batch_bins_ = batch_terms_ = batch_ll_ = batch_chord_ = nullptr;
+  batch_chord32_ = nullptr;
+  batch_capacity_ = 0;
+  void* h = nullptr;
+  check(hipHostMalloc(&h, batch * rec_bytes_, hipHostMallocDefault), "pinned batch records");
+  batch_host_ = static_cast<unsigned char*>(h);
+  std::memset(batch_host_, 0, batch * rec_bytes_);
+  check(hipHostMalloc(&h, batch * sizeof(double), hipHostMallocDefault), "pinned batch log-likelihoods");
+  batch_ll_host_ = static_cast<double*>(h);
+  batch_dev_ = deviceAlloc<unsigned char>(batch * rec_bytes_, "batch records");
+  batch_flux_ = deviceAlloc<double>(batch * stride_, "batch spectra");
+  batch_bins_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "batch bins");
+  batch_terms_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "batch likelihood terms");
+  batch_ll_ = deviceAlloc<double>(batch, "batch log-likelihoods");
+  if (transmission_) {
+    batch_chord_ = deviceAlloc<double>(batch * L * L, "batch chord lengths");
+    if (precision_ != TablePrecision::fp64) batch_chord32_ = deviceAlloc<float>(batch * L * L, "batch chord lengths");
+  }
+  batch_capacity_ = batch;
+}
+
+// ---------------------------------------------------------------- launches
+void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
+  const bool wide = precision_ == TablePrecision::fp64;
+  if (kind == ModelKind::transmission) {
+    hipLaunchKernelGGL(chordKernel, dim3(unsigned(L - 1), unsigned(B)), dim3(kBatchBlock), 0, s, rec, L, batch_chord_,
+                       wide ? nullptr : batch_chord32_);
+    check(hipGetLastError(), "chord kernel");
+    const int tiles = (n + lc.tile - 1) / lc.tile;
+    const size_t lds = transmissionLds(L, lc.tile, wide);
+    switch (species_stride_) {
+#define BEAR_T(NS)                                                                                                  \
+  case NS:                                                                                                          \
+    launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
+                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, batch_flux_);     \
+    break;
+      BEAR_T(4)
+      BEAR_T(8)
+      BEAR_T(12)
+      BEAR_T(16)
+#undef BEAR_T
+      default:
+        raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
+    }
+    check(hipGetLastError(), "transmission batch kernel");
+    return;
+  }
+  const int tiles = (n + lc.block - 1) / lc.block;
+  switch (species_stride_) {
+#define BEAR_E(NS)                                                                                                   \
+  case NS:                                                                                                           \
+    launchEmission<NS>(precision_, dim3(unsigned(lc.grid)), dim3(unsigned(lc.block)), s, wavenumber_, tables_,       \
+                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, batch_flux_);                         \
+    break;
+    BEAR_E(4)
+    BEAR_E(8)
+    BEAR_E(12)
+    BEAR_E(16)
+#undef BEAR_E
+    default:
+      raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
+  }
+  check(hipGetLastError(), "emission batch kernel");
+}
+
+void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  if (n_bins_ > 0) {
+    hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s, batch_flux_,
+                       stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
+                       kind == ModelKind::eclipse ? 1 : 0, batch_bins_, batch_terms_);
+    check(hipGetLastError(), "batch bin kernel");
+    hipLaunchKernelGGL(sumBatchKernel, dim3(unsigned(batch)), dim3(1024), 0, s, batch_terms_, int(n_bins_),
+                       batch_ll_);
+    check(hipGetLastError(), "batch likelihood kernel");
+    check(hipMemcpyAsync(batch_ll_host_, batch_ll_, batch * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read batch log-likelihoods");
+  }
+}
+
+// Autotuner: times the spectrum kernel of the first min(batch, 64) staged
+// points for each candidate launch shape and keeps the fastest.
+//   emission / eclipse: workgroup of 64, 128 or 256 wavenumbers; persistent
+//     grid of 8, 16 or 32 waves per CU
+//   transmission: tile of 16, 32 or 64 wavenumbers within the LDS budget;
+//     grid of 1, 2 or 4 workgroups per CU
+BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const size_t b = std::min<size_t>(batch, 64);
+  std::vector<BatchLaunch> cand;
+  if (kind == ModelKind::transmission) {
+    const bool wide = precision_ == TablePrecision::fp64;
+    for (int tile : {16, 32, 64}) {
+      if (transmissionLds(int(nb_levels_), tile, wide) > kLdsLimit) continue;
+      for (int per_cu : {1, 2, 4}) cand.push_back(BatchLaunch{kBatchBlock, tile, std::max(8, nb_cus_ * per_cu)});
+    }
+    if (cand.empty()) raise(kInvalidInput, "too many levels for the transmission batch kernel");
+  } else {
+    for (int block : {64, 128, 256})
+      for (int waves : {8, 16, 32}) cand.push_back(BatchLaunch{block, block, std::max(8, nb_cus_ * waves / (block / 64))});
+  }
+  for (auto& c : cand) c.grid = int(align(size_t(c.grid), 8));
+  if (!autotune_) return cand[cand.size() / 2];
+  check(hipMemcpyAsync(batch_dev_, batch_host_, b * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "hipEventCreate");
+  check(hipEventCreate(&e1), "hipEventCreate");
+  BatchLaunch best = cand.front();
+  float best_ms = std::numeric_limits<float>::max();
+  try {
+    for (const auto& c : cand) {
+      launchBatchSpectrum(kind, b, c);                    // warm-up
+      check(hipEventRecord(e0, s), "hipEventRecord");
+      launchBatchSpectrum(kind, b, c);
+      check(hipEventRecord(e1, s), "hipEventRecord");
+      check(hipEventSynchronize(e1), "tuning");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+void DeviceEngine::releaseGraph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec_));
+  graph_exec_ = nullptr;
+  graph_key_ = TuneKey{-1, -1, 0};
+}
+
+void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
+  if (batch == 0 || batch > batch_capacity_) raise(kInternal, "batch larger than the staged records");
+  if ((kind == ModelKind::transmission) != transmission_) raise(kInvalidInput, "model kind does not match the engine");
+  check(hipSetDevice(device_), "hipSetDevice");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const TuneKey key{int(kind), int(precision_), batch};
+  auto same = [](const TuneKey& a, const TuneKey& b) {
+    return a.kind == b.kind && a.precision == b.precision && a.batch == b.batch;
+  };
+  BatchLaunch launch;
+  bool found = false;
+  for (auto& t : tuned_)
+    if (same(t.first, key)) { launch = t.second; found = true; }
+  if (!found) {
+    launch = tune(kind, batch);
+    tuned_.push_back({key, launch});
+  }
+  last_launch_ = launch;
+  auto sequence = [&] {
+    check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+    launchBatchSpectrum(kind, batch, launch);
+    launchBatchRest(kind, batch);
+  };
+  if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
+  if (use_graph_) {
+    if (!graph_exec_ || !same(graph_key_, key)) {
+      releaseGraph();
+      check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+      hipGraph_t graph = nullptr;
+      try {
+        sequence();
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      check(hipStreamEndCapture(s, &graph), "end graph capture");
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      check(e, "instantiate graph");
+      graph_exec_ = exec;
+      graph_key_ = key;
+    }
+    check(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec_), s), "launch graph");
+  } else {
+    sequence();
+  }
+  last_used_graph_ = use_graph_;
+  if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), s), "hipEventRecord");
+  check(hipStreamSynchronize(s), "batch evaluation");
+  if (profiling_) {
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(event_start_), static_cast<hipEvent_t>(event_stop_)),
+          "hipEventElapsedTime");
+    batch_ms_ += ms;
+  }
+  batch_evaluations_ += batch;
+}
+
+void DeviceEngine::copyBatchBins(size_t batch, std::vector<double>& out) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  out.resize(batch * n_bins_);
+  if (!out.empty())
+    check(hipMemcpyAsync(out.data(), batch_bins_, out.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read batch bins");
+  check(hipStreamSynchronize(s), "read batch bins");
+}
+
+void DeviceEngine::copyBatchSpectrum(ModelKind kind, size_t batch, double* host) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const int n = int(n_points_);
+  double* out = deviceAlloc<double>(batch * n_points_, "batch spectrum output");
+  try {
+    hipLaunchKernelGGL(batchSpectrumKernel, dim3(unsigned((n + kBatchBlock - 1) / kBatchBlock), unsigned(batch)),
+                       dim3(kBatchBlock), 0, s, batch_flux_, stride_, wavenumber_, wavelength_, rec,
+                       kind == ModelKind::emission ? 0 : (kind == ModelKind::eclipse ? 1 : 2), n, out);
+    check(hipGetLastError(), "batch spectrum kernel");
+    check(hipMemcpyAsync(host, out, batch * n_points_ * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read batch spectrum");
+    check(hipStreamSynchronize(s), "read batch spectrum");
+  } catch (...) {
+    deviceFree(out);
+    throw;
+  }
+  deviceFree(out);
+}
+
+void DeviceEngine::releaseBatch() {
+  releaseGraph();
+  if (batch_host_) (void)hipHostFree(batch_host_);
+  if (batch_ll_host_) (void)hipHostFree(batch_ll_host_);
+  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_terms_, (void*)batch_ll_,
+                  (void*)batch_chord_, (void*)batch_chord32_, tables_low_, (void*)table_base_})
+    deviceFree(p);
+}
+
+}  // namespace bear_amd

@@ -185,6 +185,54 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
   });
 }
 
+int bear_retrieval_set_table_precision(bear_retrieval* r, int precision) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    if (precision < BEAR_TABLE_FP64 || precision > BEAR_TABLE_BF16)
+      bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "unknown table precision " + std::to_string(precision));
+    r->impl->setTablePrecision(static_cast<bear_amd::TablePrecision>(precision));
+  });
+}
+
+int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autotune) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    r->impl->engine().setBatchOptions(use_graph != 0, autotune != 0);
+  });
+}
+
+int bear_retrieval_compute_likelihood_batch(bear_retrieval* r, const double* parameters, size_t batch,
+                                            size_t nb_parameters, double* log_likelihood) {
+  return guarded([&] {
+    if (!r || (batch && (!parameters || !log_likelihood))) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    r->impl->computeLikelihoodBatch(parameters, batch, nb_parameters, log_likelihood);
+  });
+}
+
+int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* parameters, size_t batch,
+                                       size_t nb_parameters, double* spectrum, double* spectrum_obs,
+                                       int* neglect_model) {
+  return guarded([&] {
+    if (!r || (batch && (!parameters || !spectrum_obs))) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    r->impl->computeModelBatch(parameters, batch, nb_parameters, spectrum, spectrum_obs, neglect_model);
+  });
+}
+
+int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out) {
+  return guarded([&] {
+    if (!r || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    const bear_amd::DeviceEngine& e = r->impl->engine();
+    const bear_amd::BatchLaunch lc = e.batchLaunch();
+    out->precision = int(e.tablePrecision());
+    out->block = lc.block;
+    out->tile = lc.tile;
+    out->grid = lc.grid;
+    out->used_graph = e.batchUsedGraph();
+    out->evaluations = e.batchEvaluations();
+    out->gpu_ms_total = e.batchMilliseconds();
+  });
+}
+
 int bear_retrieval_set_profiling(bear_retrieval* r, int enable) {
   return guarded([&] {
     if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");

@@ -32,6 +32,45 @@ struct LevelSpecies {
   uint32_t pad;
 };
 
+// Reduced-precision form of LevelSpecies: the bilinear weights
+// (1-f_t)(1-f_p-), (1-f_t) f_p-, f_t (1-f_p+), f_t f_p+ times log2(10), and
+// log2 of the effective density (-inf when it is zero), so that
+// sigma n = 2^(log2_n + base + sum_k w[k] v[k]) for the stored residuals v.
+struct LevelSpecies32 {
+  float w[4];
+  float log2_n;
+  uint32_t row[4];
+  uint32_t reload;
+};
+
+// Storage of the log10 cross-section tables read by the batched kernels.
+// fp64: the tables as loaded, FP64 arithmetic throughout (the single-model
+// path's arithmetic).  fp32 / fp16 / bf16: per (species, wavenumber) an FP32
+// base, the midrange of that species' rows there, and per row the residual
+// log10 sigma - base in the named type; interpolation and exponentials in
+// FP32, radiative transfer, bins and likelihood in FP64.
+enum class TablePrecision : int { fp64 = 0, fp32 = 1, fp16 = 2, bf16 = 3 };
+
+enum class ModelKind : int { emission = 0, eclipse = 1, transmission = 2 };
+
+// Scalars of one live point of a batch.
+struct BatchModel {
+  double scaling;                // emission: (R_J / d)^2 f; eclipse: 1
+  double star_inv_temperature;   // eclipse: 1 / T_*
+  double depth_scale;            // eclipse: (Rp / Rs)^2 1e6
+  double bottom_radius;          // transmission: R_p (cm)
+  double star_radius;            // transmission: R_* (cm)
+  double inflation;              // added variance (error inflation), or 0
+  double pad[2];
+};
+
+// Launch shape of the batched spectrum kernel, picked by the autotuner.
+struct BatchLaunch {
+  int block = 256;               // threads per workgroup
+  int tile = 256;                // wavenumbers per work item
+  int grid = 0;                  // persistent workgroups
+};
+
 // Per bin of every observation, in observation order: the folded operator
 // segment and the likelihood inputs.
 struct BinDesc {
@@ -80,6 +119,29 @@ class DeviceEngine {
   void copySpectrum(double* host, const double* spectrum_dev);
   void synchronize();
   double* spectrumBuffer();                      // engine-owned [N], allocated on first use
+
+  // ---- batched evaluation: many live points per launch
+  // Table storage of the batched kernels; species_rows[s] = [first, end) of
+  // species s's rows in the table store.
+  void setTablePrecision(TablePrecision p, const std::vector<std::pair<uint32_t, uint32_t>>& species_rows);
+  TablePrecision tablePrecision() const { return precision_; }
+  void setBatchOptions(bool use_graph, bool autotune) { use_graph_ = use_graph; autotune_ = autotune; }
+  // pinned staging for up to `batch` live points, one record per point
+  void reserveBatch(size_t batch);
+  BatchModel* batchModel(size_t b) { return reinterpret_cast<BatchModel*>(record(b)); }
+  LevelState* batchLevels(size_t b) { return reinterpret_cast<LevelState*>(record(b) + rec_levels_); }
+  LevelSpecies* batchSpecies(size_t b);          // fp64 tables
+  LevelSpecies32* batchSpecies32(size_t b);      // reduced-precision tables
+  double* batchShifts(size_t b) { return reinterpret_cast<double*>(record(b) + rec_shifts_); }
+  // spectra, bins and log-likelihoods of live points [0, batch)
+  void evaluateBatch(ModelKind kind, size_t batch);
+  const double* batchLogLikelihood() const { return batch_ll_host_; }
+  void copyBatchBins(size_t batch, std::vector<double>& out);          // [batch][bins], internal order
+  void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
+  BatchLaunch batchLaunch() const { return last_launch_; }
+  size_t batchEvaluations() const { return batch_evaluations_; }
+  double batchMilliseconds() const { return batch_ms_; }
+  bool batchUsedGraph() const { return last_used_graph_; }
 
   void setProfiling(bool on);
   size_t timedLaunches() const { return timed_launches_; }
@@ -133,6 +195,37 @@ class DeviceEngine {
   std::vector<uint8_t> row_seen_;
   void collectTiming();
   void countRows();
+
+  // batched evaluation
+  unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
+  void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch);
+  void launchBatchRest(ModelKind kind, size_t batch);
+  BatchLaunch tune(ModelKind kind, size_t batch);
+  void releaseGraph();
+  void releaseBatch();
+  int nb_cus_ = 0;
+  TablePrecision precision_ = TablePrecision::fp64;
+  void* tables_low_ = nullptr;                   // residuals [rows][stride] (fp32 / fp16 / bf16)
+  float* table_base_ = nullptr;                  // [species stride][stride]
+  bool use_graph_ = true, autotune_ = true;
+  size_t batch_capacity_ = 0, rec_bytes_ = 0, rec_levels_ = 0, rec_species_ = 0, rec_shifts_ = 0;
+  unsigned char* batch_host_ = nullptr;          // pinned records
+  unsigned char* batch_dev_ = nullptr;
+  double* batch_flux_ = nullptr;                 // [capacity][stride]
+  double* batch_bins_ = nullptr;                 // [capacity][bins]
+  double* batch_terms_ = nullptr;                // [capacity][bins]
+  double* batch_ll_ = nullptr;                   // [capacity]
+  double* batch_ll_host_ = nullptr;              // pinned [capacity]
+  double* batch_chord_ = nullptr;                // transmission: [capacity][L][L] FP64
+  float* batch_chord32_ = nullptr;               // and FP32
+  struct TuneKey { int kind, precision; size_t batch; };
+  std::vector<std::pair<TuneKey, BatchLaunch>> tuned_;
+  void* graph_exec_ = nullptr;
+  TuneKey graph_key_{-1, -1, 0};
+  BatchLaunch last_launch_;
+  bool last_used_graph_ = false;
+  size_t batch_evaluations_ = 0;
+  double batch_ms_ = 0;
 };
 
 }  // namespace bear_amd

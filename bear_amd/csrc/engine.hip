@@ -18,9 +18,12 @@
 #include <cstring>
 #include <string>
 
+#include "device_math.h"
 #include "engine.h"
 
 namespace bear_amd {
+
+using namespace dev;
 
 namespace {
 
@@ -31,124 +34,6 @@ void check(hipError_t e, const char* what) {
 }
 
 constexpr int kBlock = 256;
-
-// Short-characteristics Gauss nodes and weights (short_characteristics.h:70-71)
-constexpr double kMu1 = 0.211324865405187;
-constexpr double kMu2 = 0.788675134594813;
-constexpr double kW = 0.5;
-
-// Planck constants in the reference's association order
-// (additional/aux_functions.cpp:272-278): 2 h c c nu^3 / (exp(h nu c / k / T) - 1)
-constexpr double kPlanckC1 = 2. * 6.62606896e-27 * 2.99792458e10 * 2.99792458e10;
-constexpr double kPi = 3.14159265358979323846;
-constexpr double kLog2Ten = 3.321928094887362347870319429489390175864831393;
-constexpr double kLog2E = 1.442695040888963407359924681001892137426645954;
-
-// 2^t in FP64 with relative error below 3.1e-15: t = n + f, |f| <= 1/2,
-// 2^f from its degree-10 Chebyshev interpolant on [-1/2, 1/2] (coefficients
-// in monomial form), then an exponent shift.  t is clamped so that the shift
-// stays in range: results below 2^-1100 flush to 0, above 2^1100 become inf.
-__device__ __forceinline__ double pow2(double t) {
-  t = fmin(fmax(t, -1100.0), 1100.0);
-  const double n = __builtin_rint(t);
-  const double f = t - n;
-  double p = 6.767963482574983e-09;
-  p = fma(p, f, 1.0206977921453389e-07);
-  p = fma(p, f, 1.3217210256367588e-06);
-  p = fma(p, f, 1.525266622014565e-05);
-  p = fma(p, f, 0.00015403526882354606);
-  p = fma(p, f, 0.0013333558214738293);
-  p = fma(p, f, 0.00961812911053657);
-  p = fma(p, f, 0.055504108664544756);
-  p = fma(p, f, 0.24022650695901254);
-  p = fma(p, f, 0.6931471805599453);
-  p = fma(p, f, 1.0);
-  return __builtin_amdgcn_ldexp(p, int(n));
-}
-
-// 1/d for finite non-zero d: hardware estimate and two Newton steps
-__device__ __forceinline__ double recip(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  r = fma(fma(-d, r, 1.0), r, r);
-  r = fma(fma(-d, r, 1.0), r, r);
-  return r;
-}
-
-// Planck function per wavenumber written as c1 nu^3 q / (1 - q),
-// q = e^(-h nu c / k T): the reference's expression without exp overflow
-__device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv_t) {
-  const double q = pow2(x_nu * inv_t);
-  return b_num * q * recip(1.0 - q);
-}
-
-// blackbody star flux pi B(T) 1e-3 (star_blackbody.cpp:40-54)
-__device__ __forceinline__ double starFlux(double nu, double inv_t) {
-  const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
-  return planckAt(kPlanckC1 * nu * nu * nu, x_nu, inv_t) * kPi * 1e-3;
-}
-
-// One layer of the two-angle short characteristics (short_characteristics.cpp:88-114)
-__device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_prev, double b_next,
-                                                          double& i1, double& i2) {
-  if (tau != 0) {
-    const double r = recip(tau);
-    const double a1 = pow2(tau * (-kLog2E / kMu1));
-    const double q1 = (a1 - 1.0) * (kMu1 * r);
-    i1 = fma(i1, a1, (1.0 + q1) * b_next + (-a1 - q1) * b_prev);
-    const double a2 = pow2(tau * (-kLog2E / kMu2));
-    const double q2 = (a2 - 1.0) * (kMu2 * r);
-    i2 = fma(i2, a2, (1.0 + q2) * b_next + (-a2 - q2) * b_prev);
-  }
-}
-
-// kappa_l(nu) = sum_s n_s 10^x_s, x_s the bilinear log10 cross-section of
-// species s at this level (opacity_species.cpp:277-345, 400-433), over NS
-// (padded) species whose four stencil values v[s][] stay in registers; rows
-// are reloaded where `reload` is set.  Rows are always valid; a zero density
-// or a zero weight removes a term exactly.
-template <int NS>
-__device__ __forceinline__ double cachedKappa(const double* __restrict__ tab, size_t stride,
-                                              const LevelSpecies* __restrict__ d, bool force, double (&v)[NS][4]) {
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-    if (force || d[s].reload) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
-    }
-  double k = 0.0;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-    const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-    const double x = fma(hi - lo, d[s].f_t, lo);
-    k = fma(d[s].n, pow2(x * kLog2Ten), k);
-  }
-  return k;
-}
-
-// The same without a register cache, for any number of species: chunks of
-// kChunk species with every row of a chunk loaded before any is used.
-constexpr int kChunk = 4;
-
-__device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, size_t stride,
-                                               const LevelSpecies* __restrict__ d, int nb_chunks) {
-  double k = 0.0;
-  for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
-    double v[kChunk][4];
-#pragma unroll
-    for (int s = 0; s < kChunk; ++s)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
-#pragma unroll
-    for (int s = 0; s < kChunk; ++s) {
-      const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-      const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-      const double x = fma(hi - lo, d[s].f_t, lo);
-      k = fma(d[s].n, pow2(x * kLog2Ten), k);
-    }
-  }
-  return k;
-}
 
 // Emission spectrum epilogue: the flux F = 2 pi (mu1 w1 I1 + mu2 w2 I2) 1e-3 s
 // (short_characteristics.cpp:117-122) is kept for the bins; the optional
@@ -313,62 +198,18 @@ __global__ __launch_bounds__(kBlock) void transmissionKernel(
   }
 }
 
-__device__ __forceinline__ double waveSum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-
-// One workgroup per bin: dot product of the high-resolution spectrum with the
-// bin's folded weights (fixed order: strided per thread, then a fixed tree).
-// Mode 1 (eclipse) also folds the blackbody star's flux and returns
-// planet / star (Rp/Rs)^2 1e6 (secondary_eclipse.cpp:231-246).  Then the
-// observation's shift (forward_model.cpp:161-180) and the bin's Gaussian
-// log-likelihood term (retrieval.cpp:330-358).
-__global__ __launch_bounds__(kBlock) void binKernel(
+// One workgroup per bin (binBody).
+__global__ __launch_bounds__(kBlockBins) void binKernel(
     const double* __restrict__ flux, const double* __restrict__ wavenumber, const double* __restrict__ weights,
     const BinDesc* __restrict__ bins, const double* __restrict__ shifts, int mode, double star_inv_temperature,
     double depth_scale, double error_inflation, double* __restrict__ out, double* __restrict__ ll_terms) {
-  __shared__ double partial[2][kBlock / 64];
-  const BinDesc b = bins[blockIdx.x];
-  const double* w = weights + b.weight_offset;
-  const double* f = flux + b.start;
-  double acc = 0.0, acc_star = 0.0;
-  for (uint32_t j = threadIdx.x; j < b.length; j += kBlock) {
-    acc += w[j] * f[j];
-    if (mode == 1) acc_star += w[j] * starFlux(wavenumber[b.start + j], star_inv_temperature);
-  }
-  acc = waveSum(acc);
-  acc_star = waveSum(acc_star);
-  if ((threadIdx.x & 63) == 0) {
-    partial[0][threadIdx.x >> 6] = acc;
-    partial[1][threadIdx.x >> 6] = acc_star;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double v = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
-    if (mode == 1) v = v / ((partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3])) * depth_scale;
-    const double shift = shifts[b.observation];
-    if (shift != 0) v += shift;
-    out[blockIdx.x] = v;
-    const double e2 = b.error * b.error + error_inflation;
-    const double d = b.data - v;
-    ll_terms[blockIdx.x] = (-0.5 * log(e2 * 2.0 * kPi) - 0.5 * d * d / e2) * b.weight;
-  }
+  binBody(flux, wavenumber, weights, bins[blockIdx.x], mode, star_inv_temperature, depth_scale, shifts,
+          error_inflation, out + blockIdx.x, ll_terms + blockIdx.x);
 }
 
 // Fixed-order sum of the per-bin terms in one workgroup.
 __global__ __launch_bounds__(1024) void sumKernel(const double* __restrict__ terms, int n, double* __restrict__ out) {
-  __shared__ double partial[16];
-  double acc = 0.0;
-  for (int j = threadIdx.x; j < n; j += 1024) acc += terms[j];
-  acc = waveSum(acc);
-  if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < 16; ++k) s += partial[k];
-    *out = s;
-  }
+  rowSum(terms, n, out);
 }
 
 template <class T>
@@ -397,11 +238,15 @@ DeviceEngine::DeviceEngine(int device) : device_(device) {
   check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
   stream_ = s;
   ll_ = deviceAlloc<double>(1, "log-likelihood");
+  hipDeviceProp_t prop;
+  check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  nb_cus_ = prop.multiProcessorCount;
 }
 
 DeviceEngine::~DeviceEngine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
+  releaseBatch();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
                   (void*)weights_, (void*)bins_desc_, (void*)bins_, (void*)ll_terms_, (void*)ll_,
                   (void*)staging_dev_, (void*)spectrum_})
@@ -424,6 +269,8 @@ void DeviceEngine::setProfiling(bool on) {
   profiling_ = on;
   timed_launches_ = 0;
   timed_ms_ = 0;
+  batch_evaluations_ = 0;
+  batch_ms_ = 0;
   event_pending_ = false;
 }
 
@@ -601,7 +448,7 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const double* shifts = reinterpret_cast<const double*>(staging_dev_ + off_shift_);
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlock), 0, s, flux_, wavenumber_, weights_,
+    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlockBins), 0, s, flux_, wavenumber_, weights_,
                        bins_desc_, shifts, mode, mode == 1 ? 1.0 / star_temperature : 0.0, depth_scale, inflation,
                        bins_, ll_terms_);
     check(hipGetLastError(), "bin kernel");

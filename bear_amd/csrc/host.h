@@ -21,7 +21,7 @@ struct Error : std::runtime_error {
 };
 [[noreturn]] void raise(int code, const std::string& msg);
 
-constexpr int kInvalidInput = -1, kFileNotFound = -2, kDevice = -3, kOutOfMemory = -4;
+constexpr int kInvalidInput = -1, kFileNotFound = -2, kDevice = -3, kOutOfMemory = -4, kInternal = -5;
 
 // ------------------------------------------------------------------ constants
 // cgs constants as in additional/physical_const.h:28-52

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace bear_amd {
 
@@ -39,39 +40,91 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
   for (auto& o : observations_) nb_modifier_ += size_t(o.modifier_parameters);
 }
 
-void TabulatedModel::stage(const double* mods) {
-  const size_t L = atmosphere_.nb_levels, S = species_.size(), stride = engine_.speciesStride();
-  LevelState* lev = engine_.levels();
-  LevelSpecies* spc = engine_.levelSpecies();
+void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, LevelState* lev, LevelSpecies* spc,
+                                 LevelSpecies32* spc32, double* shifts) const {
+  const size_t L = atm.nb_levels, S = species_.size(), stride = engine_.speciesStride();
+  const double log2_ten = 3.321928094887362347870319429489390175864831393;
   for (size_t l = 0; l < L; ++l) {
-    const double t = atmosphere_.temperature[l];
+    const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
-    const double n_r = rayleigh_species_ >= 0 ? atmosphere_.nd(l, rayleigh_species_) : 0.0;
-    lev[l] = LevelState{t, 1.0 / t, atmosphere_.altitude[l], n_r};
-    for (size_t s = 0; s < S; ++s) {
-      const OpacitySpecies& sp = species_[s];
-      double n = atmosphere_.nd(l, sp.species_id);                 // opacity_species.cpp:407-412
-      for (int q : sp.cia_partners) n *= atmosphere_.nd(l, q);
-      LevelSpecies& d = spc[l * stride + s];
-      d.n = n;                                   // rows stay valid when n == 0: the kernels load them
-      const Stencil st = sp.bracket(atmosphere_.pressure[l], t);
-      d.f_p_lower = st.f_p_lower;
-      d.f_p_upper = st.f_p_upper;
-      d.f_t = st.f_t;
+    const double n_r = rayleigh_species_ >= 0 ? atm.nd(l, rayleigh_species_) : 0.0;
+    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r};
+    for (size_t s = 0; s < stride; ++s) {
+      // padding slots: zero density, row 0 (loaded once at level 0)
+      double n = 0.0;
+      Stencil st{{0, 0, 0, 0}, 0.0, 0.0, 0.0};
+      if (s < S) {
+        const OpacitySpecies& sp = species_[s];
+        n = atm.nd(l, sp.species_id);                                 // opacity_species.cpp:407-412
+        for (int q : sp.cia_partners) n *= atm.nd(l, q);
+        st = sp.bracket(atm.pressure[l], t);                         // rows stay valid when n == 0
+      }
       // reload the rows only where they differ from the previous level's
       uint32_t changed = l == 0;
-      for (int k = 0; k < 4; ++k) {
-        if (l > 0) changed |= spc[(l - 1) * stride + s].row[k] != st.row[k];
-        d.row[k] = st.row[k];
+      if (l > 0) {
+        const uint32_t* prev = spc ? spc[(l - 1) * stride + s].row : spc32[(l - 1) * stride + s].row;
+        for (int k = 0; k < 4; ++k) changed |= prev[k] != st.row[k];
       }
-      d.reload = changed;
+      if (spc) {
+        LevelSpecies& d = spc[l * stride + s];
+        d.n = n;
+        d.f_p_lower = st.f_p_lower;
+        d.f_p_upper = st.f_p_upper;
+        d.f_t = st.f_t;
+        for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
+        d.reload = changed;
+        d.pad = 0;
+      }
+      if (spc32) {
+        LevelSpecies32& d = spc32[l * stride + s];
+        const double ft = st.f_t, fl = st.f_p_lower, fu = st.f_p_upper;
+        d.w[0] = float((1.0 - ft) * (1.0 - fl) * log2_ten);
+        d.w[1] = float((1.0 - ft) * fl * log2_ten);
+        d.w[2] = float(ft * (1.0 - fu) * log2_ten);
+        d.w[3] = float(ft * fu * log2_ten);
+        d.log2_n = n > 0 ? float(std::log2(n)) : -std::numeric_limits<float>::infinity();
+        for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
+        d.reload = changed;
+      }
     }
   }
   size_t k = 0;                                                   // forward_model.cpp:184-204
   for (size_t o = 0; o < observations_.size(); ++o) {
-    engine_.shifts()[o] = observations_[o].modifier_parameters ? mods[k] : 0.0;
+    shifts[o] = observations_[o].modifier_parameters ? mods[k] : 0.0;
     k += size_t(observations_[o].modifier_parameters);
   }
+}
+
+bool TabulatedModel::stage(const double* p, BatchModel& scalars) {
+  scalars = BatchModel{};
+  const bool neglect = setup(p, atmosphere_, scalars);
+  stageLevels(atmosphere_, modifiers(p), engine_.levels(), engine_.levelSpecies(), nullptr, engine_.shifts());
+  return neglect;
+}
+
+bool TabulatedModel::stageBatch(const double* p, double inflation, size_t b) {
+  BatchModel& scalars = *engine_.batchModel(b);
+  scalars = BatchModel{};
+  const bool neglect = setup(p, atmosphere_, scalars);
+  scalars.inflation = inflation;
+  const bool wide = engine_.tablePrecision() == TablePrecision::fp64;
+  stageLevels(atmosphere_, modifiers(p), engine_.batchLevels(b), wide ? engine_.batchSpecies(b) : nullptr,
+              wide ? nullptr : engine_.batchSpecies32(b), engine_.batchShifts(b));
+  return neglect;
+}
+
+std::vector<std::pair<uint32_t, uint32_t>> TabulatedModel::speciesRows() const {
+  std::vector<std::pair<uint32_t, uint32_t>> out;
+  for (const auto& s : species_) {
+    uint32_t lo = std::numeric_limits<uint32_t>::max(), hi = 0;
+    for (const auto& row : s.by_temperature)
+      for (const auto& r : row) {
+        lo = std::min(lo, r.id);
+        hi = std::max(hi, r.id + 1);
+      }
+    out.push_back({lo, hi});
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------- emission
@@ -82,16 +135,22 @@ EmissionModel::EmissionModel(const GlobalSettings& global, const ModelSettings& 
     raise(kInvalidInput, "radiative transfer " + m.radiative_transfer + " is not available (only scm)");
 }
 
+// parameters and atmosphere (emission.cpp:108-178); scale (R_J / d)^2 f
+bool EmissionModel::setup(const double* p, Atmosphere& atm, BatchModel& scalars) const {
+  const double* chem = p + 3;
+  const double* temp = chem + nb_chem_;
+  const bool neglect = atm.compute(std::pow(10, p[0]), temp, chem);
+  double scaling = phys::radius_jupiter / p[2];                   // emission.cpp:108-120
+  scalars.scaling = scaling * scaling * p[1];
+  return neglect;
+}
+
 // EmissionModel::calcModelGPU (emission.cpp:234-263)
 bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
                                  double* const* spectrum_obs_dev) {
-  const double* chem = p + 3;
-  const double* temp = chem + nb_chem_;
-  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
-  stage(temp + nb_temp_);
-  double scaling = phys::radius_jupiter / p[2];                   // emission.cpp:108-120
-  scaling = scaling * scaling * p[1];
-  engine_.evaluateEmission(scaling, inflation, spectrum_dev, spectrum_obs_dev);
+  BatchModel scalars;
+  const bool neglect = stage(p, scalars);
+  engine_.evaluateEmission(scalars.scaling, inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
 }
 
@@ -105,14 +164,23 @@ OccultationModel::OccultationModel(const GlobalSettings& global, const ModelSett
     raise(kInvalidInput, "stellar model " + m.stellar_model + " is not available (only blackbody)");
 }
 
+// parameters [log g, Rp/Rs, T_*, ...] (secondary_eclipse.cpp:117-153)
+bool OccultationModel::setup(const double* p, Atmosphere& atm, BatchModel& scalars) const {
+  const double* chem = p + 3;
+  const double* temp = chem + nb_chem_;
+  const bool neglect = atm.compute(std::pow(10, p[0]), temp, chem);
+  scalars.scaling = 1.0;
+  scalars.star_inv_temperature = 1.0 / p[2];
+  scalars.depth_scale = p[1] * p[1] * 1e6;
+  return neglect;
+}
+
 // OccultationModel::calcModelGPU (secondary_eclipse.cpp:262-355): planet
 // emission at scale 1, blackbody star, depth = F_p / F_* (Rp/Rs)^2 1e6 per bin
 bool OccultationModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
                                     double* const* spectrum_obs_dev) {
-  const double* chem = p + 3;
-  const double* temp = chem + nb_chem_;
-  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem);
-  stage(temp + nb_temp_);
+  BatchModel scalars;
+  const bool neglect = stage(p, scalars);
   engine_.evaluateEclipse(p[2], p[1], inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
 }
@@ -129,10 +197,8 @@ TransmissionModel::TransmissionModel(const GlobalSettings& global, const ModelSe
   nb_general_ = 3 + ((fit_mmw_ || fit_scale_height_) ? 1 : 0);    // transmission.cpp:69-72
 }
 
-// TransmissionModel::calcModelGPU (transmission.cpp:257-321) with the
-// atmosphere variants of calcAtmosphereStructure (:155-195)
-bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
-                                     double* const* spectrum_obs_dev) {
+// parameters and atmosphere variants of calcAtmosphereStructure (transmission.cpp:150-195)
+bool TransmissionModel::setup(const double* p, Atmosphere& atm, BatchModel& scalars) const {
   const double* chem = p + nb_general_;
   const double* temp = chem + nb_chem_;
   AltitudeOptions opt;
@@ -140,8 +206,16 @@ bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* 
   opt.variable_gravity = variable_gravity_;
   if (fit_mmw_) opt.fixed_mean_molecular_weight = p[3];
   if (fit_scale_height_) opt.scale_height_km = p[3];
-  const bool neglect = atmosphere_.compute(std::pow(10, p[0]), temp, chem, opt);
-  stage(temp + nb_temp_);
+  scalars.bottom_radius = p[1];
+  scalars.star_radius = p[2];
+  return atm.compute(std::pow(10, p[0]), temp, chem, opt);
+}
+
+// TransmissionModel::calcModelGPU (transmission.cpp:257-321)
+bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
+                                     double* const* spectrum_obs_dev) {
+  BatchModel scalars;
+  const bool neglect = stage(p, scalars);
   // chord lengths between successive levels for every tangent level
   // (transmission_spectrum.cpp:117-118, 145-154): path[t][i] =
   // sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2)
@@ -222,6 +296,50 @@ bool Retrieval::calcModelGPU(const double* p, size_t n, double* spectrum_dev, do
   engine_->synchronize();
   return neglect;
 }
+
+std::vector<char> Retrieval::stageBatch(const double* params, size_t batch, size_t n) {
+  checkSize(n);
+  engine_->reserveBatch(batch);
+  std::vector<char> neglect(batch);
+  for (size_t b = 0; b < batch; ++b) {
+    const double* p = params + b * n;
+    neglect[b] = model_->stageBatch(p, errorInflation(p, n), b);
+  }
+  return neglect;
+}
+
+void Retrieval::computeLikelihoodBatch(const double* params, size_t batch, size_t n, double* ll) {
+  if (batch == 0) return;
+  const std::vector<char> neglect = stageBatch(params, batch, n);
+  engine_->evaluateBatch(model_->kind(), batch);
+  const double* v = engine_->batchLogLikelihood();
+  for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
+}
+
+void Retrieval::computeModelBatch(const double* params, size_t batch, size_t n, double* spectrum,
+                                  double* spectrum_obs, int* neglect_out) {
+  if (batch == 0) return;
+  const std::vector<char> neglect = stageBatch(params, batch, n);
+  engine_->evaluateBatch(model_->kind(), batch);
+  std::vector<double> bins;
+  engine_->copyBatchBins(batch, bins);
+  const size_t nb = engine_->nbBins();
+  for (size_t b = 0; b < batch; ++b) {
+    size_t k = 0;
+    for (auto& o : observations_) {                               // input order (forward_model.cpp:119-123)
+      const size_t m = o.nbPoints();
+      const double* src = bins.data() + b * nb + k;
+      double* dst = spectrum_obs + b * nb + k;
+      if (o.ascending) std::reverse_copy(src, src + m, dst);
+      else std::copy(src, src + m, dst);
+      k += m;
+    }
+    if (neglect_out) neglect_out[b] = neglect[b];
+  }
+  if (spectrum) engine_->copyBatchSpectrum(model_->kind(), batch, spectrum);
+}
+
+void Retrieval::setTablePrecision(TablePrecision p) { engine_->setTablePrecision(p, model_->speciesRows()); }
 
 bool Retrieval::computeModel(const double* p, size_t n, bool high_res, double* spectrum, double* spectrum_obs) {
   checkSize(n);

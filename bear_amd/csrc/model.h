@@ -42,22 +42,39 @@ class ForwardModel {
                             double* const* spectrum_obs_dev) = 0;
   // the binned spectrum is a flux (emission, eclipse) or not (transmission)
   virtual bool binsAreFluxes() const = 0;
+  virtual ModelKind kind() const = 0;
+  // one live point of a batch: its scalars, atmosphere and stencils into the
+  // engine's batch record b; returns "neglect"
+  virtual bool stageBatch(const double* parameters, double error_inflation, size_t b) = 0;
+  // [first, end) of each tabulated species' rows in the table store
+  virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
 };
 
 // What the three models share: atmosphere, opacity species with their
 // tables on the device, and the per-level stencils of one evaluation.
 class TabulatedModel : public ForwardModel {
+ public:
+  bool stageBatch(const double* parameters, double error_inflation, size_t b) override;
+  std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const override;
+
  protected:
   TabulatedModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
                  const std::vector<ObservationSetup>& observations, DeviceEngine& engine, bool transmission);
-  // fill the engine's staging: levels, stencils and observation shifts
-  void stage(const double* modifiers);
+  // the atmosphere of one parameter vector and the model's scalars; returns "neglect"
+  virtual bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const = 0;
+  // levels, stencils (FP64 and/or FP32 form) and observation shifts of one
+  // computed atmosphere
+  void stageLevels(const Atmosphere& atmosphere, const double* modifiers, LevelState* levels, LevelSpecies* species,
+                   LevelSpecies32* species32, double* shifts) const;
+  // the single-model path: setup + stageLevels into the engine's staging
+  bool stage(const double* parameters, BatchModel& scalars);
+  const double* modifiers(const double* p) const { return p + nb_general_ + nb_chem_ + nb_temp_; }
   const std::vector<ObservationSetup>& observations_;
   DeviceEngine& engine_;
   Atmosphere atmosphere_;
   std::vector<OpacitySpecies> species_;     // tabulated absorbers
   int rayleigh_species_ = -1;               // species id of the Rayleigh scatterer, -1: none
-  size_t nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
+  size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
 };
 
 // EmissionModel (forward_model/emission/emission.h:127-250): parameters
@@ -70,6 +87,10 @@ class EmissionModel : public TabulatedModel {
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
+  ModelKind kind() const override { return ModelKind::emission; }
+
+ protected:
+  bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
 };
 
 // OccultationModel (forward_model/secondary_eclipse/secondary_eclipse.h) with
@@ -83,6 +104,10 @@ class OccultationModel : public TabulatedModel {
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
+  ModelKind kind() const override { return ModelKind::eclipse; }
+
+ protected:
+  bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
 };
 
 // TransmissionModel (forward_model/transmission/transmission.h): parameters
@@ -96,9 +121,12 @@ class TransmissionModel : public TabulatedModel {
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return false; }
+  ModelKind kind() const override { return ModelKind::transmission; }
+
+ protected:
+  bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
 
  private:
-  size_t nb_general_ = 3;
   bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;
 };
 
@@ -115,11 +143,17 @@ class Retrieval {
   // ForwardModel::calcModel (forward_model.cpp:60-127) with host outputs
   bool computeModel(const double* parameters, size_t n, bool high_res, double* spectrum, double* spectrum_obs);
   bool calcModelGPU(const double* parameters, size_t n, double* spectrum_dev, double* const* spectrum_obs_dev);
+  // batches: parameters [batch][n] row-major
+  void computeLikelihoodBatch(const double* parameters, size_t batch, size_t n, double* log_likelihood);
+  void computeModelBatch(const double* parameters, size_t batch, size_t n, double* spectrum, double* spectrum_obs,
+                         int* neglect);
+  void setTablePrecision(TablePrecision p);
   DeviceEngine& engine() const { return *engine_; }
 
  private:
   double errorInflation(const double* parameters, size_t n) const;
   void checkSize(size_t n) const;
+  std::vector<char> stageBatch(const double* parameters, size_t batch, size_t n);
   GlobalSettings global_;
   SpectralGrid grid_;
   std::vector<ObservationSetup> observations_;

@@ -167,6 +167,60 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
                                       double* spectrum_dev, double* const* spectrum_obs_dev,
                                       int* neglect_model);
 
+/* ---------------------------------------------------------------- batches
+ * Many live points per launch sequence.  The reference evaluates one point
+ * per call (the sampler callback, retrieval.h:118-129 and
+ * multinest_loglike.cpp:131-186, calls Retrieval::computeLikelihood once per
+ * point); these calls take `batch` parameter vectors at once, row-major
+ * [batch][nb_parameters] (nb_parameters >= bear_retrieval_nb_parameters), and
+ * return what that many single calls would.
+ *
+ * Table precision of the batched kernels (the single-point calls above
+ * always use the FP64 tables):
+ *   BEAR_TABLE_FP64  FP64 tables and arithmetic: the single-point results
+ *   BEAR_TABLE_FP32  per (species, wavenumber) an FP32 base plus per-row
+ *   BEAR_TABLE_FP16  residuals log10 sigma - base stored in FP32, FP16 or
+ *   BEAR_TABLE_BF16  BF16; interpolation and exponentials in FP32,
+ *                    radiative transfer, transit-depth integral, bins and
+ *                    log-likelihood in FP64.  The residuals' precision
+ *                    follows each species' range of log10 sigma over its
+ *                    rows at a wavenumber. */
+#define BEAR_TABLE_FP64 0
+#define BEAR_TABLE_FP32 1
+#define BEAR_TABLE_FP16 2
+#define BEAR_TABLE_BF16 3
+
+int bear_retrieval_set_table_precision(bear_retrieval* r, int precision);
+
+/* use_graph: replay each (model, precision, batch size)'s launch sequence
+ * from a captured hipGraph.  autotune: time the candidate launch shapes on
+ * the first batch of each size and keep the fastest (otherwise a fixed
+ * shape).  Both default to 1. */
+int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autotune);
+
+/* log_likelihood [batch]; -1e30 for neglected points. */
+int bear_retrieval_compute_likelihood_batch(bear_retrieval* r, const double* parameters, size_t batch,
+                                            size_t nb_parameters, double* log_likelihood);
+
+/* spectrum [batch][nb_spectral_points] (NULL: not returned), spectrum_obs
+ * [batch][total observation points] (each point's observations as in
+ * bear_retrieval_compute_model), neglect_model [batch] (may be NULL). */
+int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* parameters, size_t batch,
+                                       size_t nb_parameters, double* spectrum, double* spectrum_obs,
+                                       int* neglect_model);
+
+typedef struct bear_batch_stats {
+  int precision;                 /* BEAR_TABLE_* */
+  int block;                     /* launch shape of the last batch: threads per workgroup */
+  int tile;                      /* wavenumbers per work item */
+  int grid;                      /* persistent workgroups */
+  int used_graph;
+  size_t evaluations;            /* live points evaluated in batches */
+  double gpu_ms_total;           /* device time of those batches (with profiling on) */
+} bear_batch_stats;
+
+int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out);
+
 /* ---------------------------------------------------------------- measurement
  * No counterpart in the reference (it has no timing code).  With profiling
  * on, HIP events on the retrieval's stream time every launch of the fused

@@ -65,3 +65,14 @@ def test_missing_device_is_a_device_error(case_factory):
     with pytest.raises(api.BearError) as e:
         synth.open_retrieval(folder)
     assert e.value.code == -3
+
+
+def test_batch_entry_points_refuse_null_arguments():
+    from bear_amd import api
+    L = api.lib()
+    assert L.bear_retrieval_set_table_precision(None, 0) == -1
+    assert L.bear_retrieval_set_batch_options(None, 1, 1) == -1
+    assert L.bear_retrieval_compute_likelihood_batch(None, None, 1, 4, None) == -1
+    assert L.bear_retrieval_compute_model_batch(None, None, 1, 4, None, None, None) == -1
+    assert L.bear_retrieval_batch_stats(None, None) == -1
+    assert b"null" in L.bear_last_error()
