@@ -190,6 +190,8 @@ class Atmosphere {
   bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param,
                const AltitudeOptions& options = AltitudeOptions());
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
+  // copies are independent (each owns its temperature profile's state)
+  Atmosphere(const Atmosphere& other);
  private:
   size_t n_ids_;
   std::unique_ptr<PiecewisePolynomial> poly_;

@@ -122,6 +122,20 @@ Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& t
   number_density.assign(L * n_ids_, 0.0);
 }
 
+Atmosphere::Atmosphere(const Atmosphere& o)
+    : nb_levels(o.nb_levels),
+      pressure(o.pressure),
+      temperature(o.temperature),
+      altitude(o.altitude),
+      mean_molecular_weight(o.mean_molecular_weight),
+      number_density(o.number_density),
+      n_ids_(o.n_ids_),
+      poly_(o.poly_ ? new PiecewisePolynomial(*o.poly_) : nullptr),
+      isothermal_(o.isothermal_),
+      nb_temperature_param_(o.nb_temperature_param_),
+      nb_chemistry_param_(o.nb_chemistry_param_),
+      chemistry_(o.chemistry_) {}
+
 bool Atmosphere::compute(double g, const double* tp, const double* cp, const AltitudeOptions& opt) {
   const size_t L = nb_levels;
   const int S = int(n_ids_);

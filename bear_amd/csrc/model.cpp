@@ -1,6 +1,8 @@
 // Forward models and retrieval of the host side.
 #include "model.h"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -102,15 +104,33 @@ bool TabulatedModel::stage(const double* p, BatchModel& scalars) {
   return neglect;
 }
 
-bool TabulatedModel::stageBatch(const double* p, double inflation, size_t b) {
-  BatchModel& scalars = *engine_.batchModel(b);
-  scalars = BatchModel{};
-  const bool neglect = setup(p, atmosphere_, scalars);
-  scalars.inflation = inflation;
+void TabulatedModel::stageBatch(const double* params, size_t batch, size_t stride, const double* inflation,
+                                char* neglect) {
+  const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
+  while (atmosphere_pool_.size() < size_t(threads)) atmosphere_pool_.emplace_back(new Atmosphere(atmosphere_));
   const bool wide = engine_.tablePrecision() == TablePrecision::fp64;
-  stageLevels(atmosphere_, modifiers(p), engine_.batchLevels(b), wide ? engine_.batchSpecies(b) : nullptr,
-              wide ? nullptr : engine_.batchSpecies32(b), engine_.batchShifts(b));
-  return neglect;
+  std::string failure;
+  int failure_code = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (size_t b = 0; b < batch; ++b) {
+    try {
+      Atmosphere& atm = *atmosphere_pool_[omp_get_thread_num()];
+      const double* p = params + b * stride;
+      BatchModel& scalars = *engine_.batchModel(b);
+      scalars = BatchModel{};
+      neglect[b] = setup(p, atm, scalars);
+      scalars.inflation = inflation[b];
+      stageLevels(atm, modifiers(p), engine_.batchLevels(b), wide ? engine_.batchSpecies(b) : nullptr,
+                  wide ? nullptr : engine_.batchSpecies32(b), engine_.batchShifts(b));
+    } catch (const Error& e) {
+#pragma omp critical(bear_stage_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = e.code; }
+    } catch (const std::exception& e) {
+#pragma omp critical(bear_stage_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = kInternal; }
+    }
+  }
+  if (!failure.empty()) raise(failure_code, failure);
 }
 
 std::vector<std::pair<uint32_t, uint32_t>> TabulatedModel::speciesRows() const {
@@ -301,10 +321,9 @@ std::vector<char> Retrieval::stageBatch(const double* params, size_t batch, size
   checkSize(n);
   engine_->reserveBatch(batch);
   std::vector<char> neglect(batch);
-  for (size_t b = 0; b < batch; ++b) {
-    const double* p = params + b * n;
-    neglect[b] = model_->stageBatch(p, errorInflation(p, n), b);
-  }
+  std::vector<double> inflation(batch);
+  for (size_t b = 0; b < batch; ++b) inflation[b] = errorInflation(params + b * n, n);
+  model_->stageBatch(params, batch, n, inflation.data(), neglect.data());
   return neglect;
 }
 

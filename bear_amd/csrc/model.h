@@ -43,9 +43,11 @@ class ForwardModel {
   // the binned spectrum is a flux (emission, eclipse) or not (transmission)
   virtual bool binsAreFluxes() const = 0;
   virtual ModelKind kind() const = 0;
-  // one live point of a batch: its scalars, atmosphere and stencils into the
-  // engine's batch record b; returns "neglect"
-  virtual bool stageBatch(const double* parameters, double error_inflation, size_t b) = 0;
+  // live points [0, batch) (parameters [batch][stride]): scalars, atmosphere
+  // and stencils into the engine's batch records, in parallel over points;
+  // neglect[b] set per point
+  virtual void stageBatch(const double* parameters, size_t batch, size_t stride, const double* error_inflation,
+                          char* neglect) = 0;
   // [first, end) of each tabulated species' rows in the table store
   virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
 };
@@ -54,7 +56,8 @@ class ForwardModel {
 // tables on the device, and the per-level stencils of one evaluation.
 class TabulatedModel : public ForwardModel {
  public:
-  bool stageBatch(const double* parameters, double error_inflation, size_t b) override;
+  void stageBatch(const double* parameters, size_t batch, size_t stride, const double* error_inflation,
+                  char* neglect) override;
   std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const override;
 
  protected:
@@ -72,6 +75,7 @@ class TabulatedModel : public ForwardModel {
   const std::vector<ObservationSetup>& observations_;
   DeviceEngine& engine_;
   Atmosphere atmosphere_;
+  std::vector<std::unique_ptr<Atmosphere>> atmosphere_pool_;   // one per staging thread
   std::vector<OpacitySpecies> species_;     // tabulated absorbers
   int rayleigh_species_ = -1;               // species id of the Rayleigh scatterer, -1: none
   size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;

@@ -87,6 +87,46 @@ def traffic_from_profiles(kernel="emissionKernel"):
     return None
 
 
+def batched_leg(r, points, batch_steps, plain_rate):
+    """Batches of live points per launch sequence at each table precision:
+    throughput, speedup over the single-point path, and the error against the
+    FP64 single-point path (pinned to the oracle by the parity tests) on the
+    first four points: high-resolution spectrum and bins (relative),
+    log-likelihood (absolute)."""
+    import torch
+    pts = np.array(points)
+    ref = [r.computeModel(p, True) for p in pts[:4]]
+    ref_ll = [r.computeLikelihood(p) for p in pts[:4]]
+    out = {"batch": len(pts)}
+    for precision in ("fp64", "fp32", "fp16", "bf16"):
+        r.set_table_precision(precision)
+        r.computeLikelihoodBatch(pts)                   # tuning + graph capture
+        r.computeLikelihoodBatch(pts)
+        r.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(batch_steps):
+            ll = r.computeLikelihoodBatch(pts)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        st = r.batch_stats()
+        r.set_profiling(False)
+        spec, bins, _ = r.computeModelBatch(pts[:4], True)
+        err_spec = max(float(np.max(np.abs(spec[k] - ref[k].spectrum) / np.abs(ref[k].spectrum))) for k in range(4))
+        err_bins = max(float(np.max(np.abs(bins[i][k] - ref[k].spectrum_obs[i]) / np.abs(ref[k].spectrum_obs[i])))
+                       for k in range(4) for i in range(len(bins)))
+        err_ll = max(abs(float(ll[k]) - ref_ll[k]) for k in range(4))
+        rate = batch_steps * len(pts) / elapsed
+        out[precision] = {"evals_per_s": round(rate, 1), "ms_per_batch": round(elapsed / batch_steps * 1e3, 3),
+                          "gpu_ms_per_batch": round(st["gpu_ms_total"] / batch_steps, 3),
+                          "speedup_vs_single": round(rate / plain_rate, 2),
+                          "max_spectrum_rel_err": err_spec, "max_bin_rel_err": err_bins, "max_abs_dloglike": err_ll,
+                          "launch": {k: st[k] for k in ("block", "tile", "grid", "used_graph")}}
+        log(f"batch {precision}: {rate:.1f} evals/s, spectrum err {err_spec:.2e}, |dlogL| {err_ll:.2e}")
+    r.set_table_precision("fp64")
+    return out
+
+
 def cpu_baseline(threads, scratch):
     """The reference's own CPU path (oracle/_ref build) on a bounded sample:
     the same configuration at 20,000 wavenumbers, scaled to 400,000 by the
@@ -139,6 +179,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=128, help="live points per batch in the batched leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,6 +199,8 @@ def main():
         case = build_case(FULL_N, "torch")
         log(f"tables ready in {time.time() - t0:.1f} s; building the engine")
         r, params = open_with_injected_data(case, os.path.join(scratch, "gpu"))
+        from bear_amd import synth
+        points = [params] + [synth.perturbed(params, case, seed=1000 + k) for k in range(max(args.batch, 1) - 1)]
         del case
         iso_err = isothermal_check(r, params)
         log(f"isothermal identity at full size: max rel err {iso_err:.2e}")
@@ -185,6 +229,10 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                     "kernel": "emissionKernel", "kernel_ms": round(kernel_s * 1e3, 4),
                     "algorithmic_bytes": int(algorithmic), "distinct_table_rows": st["distinct_table_rows"]}
+
+        batched = None
+        if args.batch > 0:
+            batched = batched_leg(r, points[:args.batch], args.batch_steps, args.steps / elapsed)
 
         cpu = None
         if not args.no_cpu_baseline:
@@ -215,6 +263,7 @@ def main():
                    "table_dtype": "f64", "levels": st["nb_levels"], "bins": st["nb_bins"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "checks": {"isothermal_max_rel_err": iso_err},
     }
     print(json.dumps(out), flush=True)
