@@ -104,6 +104,7 @@ def lib():
         L.bear_retrieval_compute_likelihood_batch.argtypes = [vp, _dp, sz, sz, _dp]
         L.bear_retrieval_compute_model_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp, C.POINTER(C.c_int)]
         L.bear_retrieval_batch_stats.argtypes = [vp, C.POINTER(BatchStats)]
+        L.bear_retrieval_gradient_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp]
         _lib = L
     return _lib
 
@@ -309,6 +310,15 @@ class Retrieval:
             out.append(obs[:, k:k + n].copy())
             k += n
         return spec, out, neg.astype(bool)
+
+    def gradientBatch(self, physical_parameters):
+        """(log-likelihoods [batch], gradients [batch, nb_parameters]) from the
+        hand-written adjoint (emission model, FP64 tables)."""
+        p = np.ascontiguousarray(np.atleast_2d(physical_parameters), dtype=np.float64)
+        ll = np.empty(p.shape[0])
+        grad = np.empty_like(p)
+        _check(self._L.bear_retrieval_gradient_batch(self._h, _ptr(p), p.shape[0], p.shape[1], _ptr(ll), _ptr(grad)))
+        return ll, grad
 
     def batch_stats(self) -> dict:
         st = BatchStats()

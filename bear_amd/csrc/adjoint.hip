@@ -1,0 +1,357 @@
+// Adjoint of the batched emission likelihood on gfx950: the gradient of
+// log L with respect to every level's temperature, altitude and effective
+// species densities (and the flux scaling), for every live point of the last
+// batch.  The host carries it on to the retrieval parameters.
+//
+// Reverse order of the forward chain:
+//   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
+//   d logL / d F(nu) = sum_j d logL / d bin_j  W_j(nu)          (fluxAdjointKernel, the
+//                      folded operator transposed, per wavenumber)
+//   short characteristics, Planck function, table interpolation (emissionAdjointKernel)
+//   sums over wavenumbers                                       (adjointReduceKernel)
+//
+// emissionAdjointKernel: one wavefront per work item (64 wavenumbers of one
+// point).  A forward sweep recomputes the levels and keeps both angles'
+// intensities I_l in a per-workgroup scratch ([levels][2][64] FP64, reused by
+// every item the workgroup takes); the backward sweep recomputes each
+// level's opacity from the tables (register-cached stencils, reloaded where
+// the rows differ from the level above) and runs the recurrence backwards.
+// Per level, the wavefront sums its lanes' contributions and lane 0 stores
+// them to partial[point][level][slot][tile]; a second kernel sums the tiles in
+// a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "device_math.h"
+#include "engine.h"
+
+namespace bear_amd {
+
+using namespace dev;
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? kOutOfMemory : kDevice;
+  raise(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* deviceAlloc(size_t count, const char* what) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  check(hipMalloc(&p, count * sizeof(T)), what);
+  return static_cast<T*>(p);
+}
+
+void deviceFree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr double kLn10 = 2.302585092994045684017991454684364207601101488;
+constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
+
+// d logL / d bin_j for every (bin, point)
+__global__ __launch_bounds__(kBlock) void gbarKernel(const double* __restrict__ bins, const BinDesc* __restrict__ desc,
+                                                     int nb_bins, Records rec, double* __restrict__ gbar) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int m = blockIdx.y;
+  if (j >= nb_bins) return;
+  const double inflation = reinterpret_cast<const BatchModel*>(rec.at(m))->inflation;
+  const BinDesc d = desc[j];
+  const double e2 = d.error * d.error + inflation;
+  gbar[size_t(m) * nb_bins + j] = d.weight * (d.data - bins[size_t(m) * nb_bins + j]) / e2;
+}
+
+// d logL / d F(nu) through the transposed operator
+__global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __restrict__ ptr,
+                                                            const uint32_t* __restrict__ bin,
+                                                            const double* __restrict__ weight,
+                                                            const double* __restrict__ gbar, int nb_bins, int n,
+                                                            size_t stride, double* __restrict__ fbar) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int m = blockIdx.y;
+  if (i >= n) return;
+  const double* g = gbar + size_t(m) * nb_bins;
+  double acc = 0.0;
+  for (uint64_t e = ptr[i]; e < ptr[i + 1]; ++e) acc = fma(g[bin[e]], weight[e], acc);
+  fbar[size_t(m) * stride + i] = acc;
+}
+
+// Opacity of one level as in the forward kernels (same operations in the same
+// order), plus each species' cross-section sigma_s = 10^x_s and
+// d kappa / dT = ln 10 sum_s n_s sigma_s (x_hi - x_lo) d f_t / dT.  Rows are
+// reloaded where they differ from the level above (or always, with force).
+template <int NS>
+__device__ __forceinline__ void opacityAdjoint(const double* __restrict__ tab, size_t stride,
+                                               const LevelSpecies* __restrict__ d,
+                                               const LevelSpecies* __restrict__ above, bool force,
+                                               double (&v)[NS][4], double (&sigma)[NS], double& kappa,
+                                               double& dkdt) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    bool reload = force;
+    if (!force)
+      reload = d[s].row[0] != above[s].row[0] || d[s].row[1] != above[s].row[1] ||
+               d[s].row[2] != above[s].row[2] || d[s].row[3] != above[s].row[3];
+    if (reload) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+    }
+  }
+  double k = 0.0, dk = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
+    const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
+    const double x = fma(hi - lo, d[s].f_t, lo);
+    const double sg = pow2(x * kLog2Ten);
+    sigma[s] = sg;
+    k = fma(d[s].n, sg, k);
+    dk = fma(d[s].n * sg, (hi - lo) * d[s].dft, dk);
+  }
+  kappa = k;
+  dkdt = dk * kLn10;
+}
+
+// B(T) and dB/dT = B X / (T (1 - q)), X = h c nu / (k T), q = e^-X
+__device__ __forceinline__ void planckAdjoint(double b_num, double x_nu, double inv_t, double& b, double& dbdt) {
+  const double q = pow2(x_nu * inv_t);
+  const double r = recip(1.0 - q);
+  b = b_num * q * r;
+  dbdt = b * (-x_nu * inv_t * kLn2) * inv_t * r;
+}
+
+// One angle of one layer backwards.  Forward: a = e^(-tau/mu),
+// q = (a - 1) mu / tau, I_l = a I_{l-1} + (1 + q) B_l + (-a - q) B_{l-1}.
+// dq/dtau = (mu (1 - a) - a tau) / tau^2, from its series in delta = tau/mu
+// below 1e-2 (the closed form cancels there).
+__device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, double i_prev, double b_lo, double b_hi,
+                                             double& ib, double& tbar, double& bbar_hi, double& bbar_lo) {
+  const double a = pow2(tau * (-kLog2E / mu));
+  const double q = (a - 1.0) * (mu * r);
+  const double delta = tau / mu;
+  double dq;
+  if (delta < 1e-2)
+    dq = fma(delta, fma(delta, fma(delta, fma(delta, 1.0 / 144.0, -1.0 / 30.0), 1.0 / 8.0), -1.0 / 3.0), 0.5) / mu;
+  else
+    dq = (mu * (1.0 - a) - a * tau) * r * r;
+  tbar = fma(ib, (i_prev - b_lo) * (-a / mu) + (b_hi - b_lo) * dq, tbar);
+  bbar_hi = fma(ib, 1.0 + q, bbar_hi);
+  bbar_lo = fma(ib, -a - q, bbar_lo);
+  ib *= a;
+}
+
+// The wavefront's sums of one level's slots -> partial[(slot) * tiles + tile]
+template <int NS>
+__device__ __forceinline__ void emitLevel(double* __restrict__ out, size_t tiles, int lane, double kbar,
+                                          const double (&sigma)[NS], double tbar, double zbar, double extra) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double v = waveSum(kbar * sigma[s]);
+    if (lane == 0) out[size_t(s) * tiles] = v;
+  }
+  const double t = waveSum(tbar);
+  const double z = waveSum(zbar);
+  const double e = waveSum(extra);
+  if (lane == 0) {
+    out[size_t(NS) * tiles] = t;
+    out[size_t(NS + 1) * tiles] = z;
+    out[size_t(NS + 2) * tiles] = e;
+  }
+}
+
+template <int NS>
+__global__ __launch_bounds__(kWave) void emissionAdjointKernel(
+    const double* __restrict__ wavenumber, const double* __restrict__ tables, size_t stride, Records rec,
+    int nb_levels, int n, int tiles, int batch, const double* __restrict__ fbar, double* __restrict__ scratch,
+    double* __restrict__ partial) {
+  constexpr int K = NS + 3;
+  const int L = nb_levels;
+  const int lane = int(threadIdx.x);
+  double* store = scratch + size_t(blockIdx.x) * size_t(L) * 2 * kWave;
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    const int i = tile * kWave + lane;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;                                // idle lanes: zero adjoints
+    const unsigned char* r = rec.at(m);
+    const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+    const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
+    double* out = partial + size_t(m) * L * K * size_t(tiles) + tile;
+    const double* tab = tables + ic;
+    const double nu = wavenumber[ic];
+    const double b_num = kPlanckC1 * nu * nu * nu;
+    const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+
+    // forward sweep, keeping I_l (l < L-1)
+    double v[NS][4];
+    double k_prev = cachedKappa<NS>(tab, stride, species, true, v);
+    double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
+    double z_prev = levels[0].altitude;
+    double i1 = b_prev, i2 = b_prev;
+    store[lane] = i1;
+    store[kWave + lane] = i2;
+    for (int l = 1; l < L; ++l) {
+      const double k_next = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, false, v);
+      const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
+      const double z_next = levels[l].altitude;
+      shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+      if (l < L - 1) {
+        store[(size_t(l) * 2) * kWave + lane] = i1;
+        store[(size_t(l) * 2 + 1) * kWave + lane] = i2;
+      }
+      k_prev = k_next;
+      b_prev = b_next;
+      z_prev = z_next;
+    }
+
+    // F = 2 pi (mu1 w I1 + mu2 w I2) 1e-3 scaling
+    const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
+    const double unit = 2.0 * kPi * 1e-3;
+    const double s_bar = fb * unit * (i1 * kMu1 * kW + i2 * kMu2 * kW);
+    double ib1 = fb * unit * bm->scaling * kMu1 * kW;
+    double ib2 = fb * unit * bm->scaling * kMu2 * kW;
+
+    // backward sweep
+    double sig_hi[NS], sig_lo[NS];
+    double k_hi, dk_hi, b_hi, db_hi;
+    opacityAdjoint<NS>(tab, stride, species + size_t(L - 1) * NS, nullptr, true, v, sig_hi, k_hi, dk_hi);
+    planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
+    double z_hi = levels[L - 1].altitude;
+    double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
+    for (int l = L - 1; l >= 1; --l) {
+      double k_lo, dk_lo, b_lo, db_lo;
+      opacityAdjoint<NS>(tab, stride, species + size_t(l - 1) * NS, species + size_t(l) * NS, false, v, sig_lo, k_lo,
+                         dk_lo);
+      planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
+      const double z_lo = levels[l - 1].altitude;
+      const double dz = z_hi - z_lo;
+      const double tau = dz * (k_hi + k_lo) * 0.5;
+      double tbar = 0.0, bbar_lo = 0.0;
+      if (tau != 0) {
+        const double rt = recip(tau);
+        const double i1p = store[(size_t(l - 1) * 2) * kWave + lane];
+        const double i2p = store[(size_t(l - 1) * 2 + 1) * kWave + lane];
+        angleAdjoint(tau, rt, kMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
+      }
+      const double kbar_lo = tbar * dz * 0.5;
+      kbar_hi += kbar_lo;
+      zbar_hi += tbar * (k_hi + k_lo) * 0.5;
+      const double zbar_lo = -tbar * (k_hi + k_lo) * 0.5;
+      // every contribution to level l is in: emit it
+      emitLevel<NS>(out + size_t(l) * K * tiles, size_t(tiles), lane, kbar_hi, sig_hi,
+                    fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, 0.0);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) sig_hi[s] = sig_lo[s];
+      k_hi = k_lo;
+      dk_hi = dk_lo;
+      b_hi = b_lo;
+      db_hi = db_lo;
+      z_hi = z_lo;
+      kbar_hi = kbar_lo;
+      zbar_hi = zbar_lo;
+      bbar_hi = bbar_lo;
+    }
+    bbar_hi += ib1 + ib2;                                            // I_0 = B_0
+    emitLevel<NS>(out, size_t(tiles), lane, kbar_hi, sig_hi, fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, s_bar);
+  }
+}
+
+// out[point][row] = sum over tiles of partial[point][row][tile], fixed order
+__global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __restrict__ partial, int rows,
+                                                              int tiles, double* __restrict__ out) {
+  __shared__ double part[kBlock / 64];
+  const int row = blockIdx.x, m = blockIdx.y;
+  const double* p = partial + (size_t(m) * rows + row) * size_t(tiles);
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += kBlock) acc += p[t];
+  acc = waveSum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[size_t(m) * rows + row] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+}  // namespace
+
+void DeviceEngine::releaseAdjoint() {
+  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_scratch_, (void*)adj_partial_, (void*)adj_out_})
+    deviceFree(p);
+  adj_fbar_ = adj_gbar_ = adj_scratch_ = adj_partial_ = adj_out_ = nullptr;
+  adj_capacity_ = adj_grid_ = adj_partial_size_ = 0;
+}
+
+void DeviceEngine::adjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
+                                std::vector<double>& bins) {
+  if (transmission_) raise(kInvalidInput, "the adjoint covers the emission model");
+  if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
+  if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
+  check(hipSetDevice(device_), "hipSetDevice");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const int n = int(n_points_), L = int(nb_levels_), B = int(batch), nb = int(n_bins_);
+  const int NS = int(species_stride_), K = NS + 3;
+  const int tiles = (n + kWave - 1) / kWave;
+  const int grid = int(std::max<size_t>(8, (size_t(nb_cus_) * 8 + 7) / 8 * 8));
+  const size_t partial_size = size_t(B) * L * K * size_t(tiles);
+  if (batch > adj_capacity_ || size_t(grid) != adj_grid_ || partial_size > adj_partial_size_) {
+    releaseAdjoint();
+    adj_fbar_ = deviceAlloc<double>(batch * stride_, "adjoint flux");
+    adj_gbar_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "adjoint bins");
+    adj_scratch_ = deviceAlloc<double>(size_t(grid) * L * 2 * kWave, "adjoint intensities");
+    adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
+    adj_out_ = deviceAlloc<double>(size_t(B) * L * K, "adjoint level gradients");
+    adj_capacity_ = batch;
+    adj_grid_ = size_t(grid);
+    adj_partial_size_ = partial_size;
+  }
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  if (nb > 0) {
+    hipLaunchKernelGGL(gbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(B)), dim3(kBlock), 0, s,
+                       batch_bins_, bins_desc_, nb, rec, adj_gbar_);
+    check(hipGetLastError(), "adjoint bin kernel");
+  }
+  hipLaunchKernelGGL(fluxAdjointKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(B)), dim3(kBlock), 0, s,
+                     adj_ptr_, adj_bin_, adj_weight_, adj_gbar_, nb, n, stride_, adj_fbar_);
+  check(hipGetLastError(), "adjoint flux kernel");
+  switch (NS) {
+#define BEAR_ADJ(NSV)                                                                                            \
+  case NSV:                                                                                                      \
+    hipLaunchKernelGGL(emissionAdjointKernel<NSV>, dim3(unsigned(grid)), dim3(kWave), 0, s, wavenumber_, tables_, \
+                       stride_, rec, L, n, tiles, B, adj_fbar_, adj_scratch_, adj_partial_);                     \
+    break;
+    BEAR_ADJ(4)
+    BEAR_ADJ(8)
+    BEAR_ADJ(12)
+    BEAR_ADJ(16)
+#undef BEAR_ADJ
+    default:
+      raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
+  }
+  check(hipGetLastError(), "adjoint kernel");
+  hipLaunchKernelGGL(adjointReduceKernel, dim3(unsigned(L * K), unsigned(B)), dim3(kBlock), 0, s, adj_partial_,
+                     L * K, tiles, adj_out_);
+  check(hipGetLastError(), "adjoint reduction kernel");
+  level_grad.resize(size_t(B) * L * K);
+  bin_grad.resize(size_t(B) * nb);
+  bins.resize(size_t(B) * nb);
+  check(hipMemcpyAsync(level_grad.data(), adj_out_, level_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+        "read level gradients");
+  if (nb > 0) {
+    check(hipMemcpyAsync(bin_grad.data(), adj_gbar_, bin_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read bin gradients");
+    check(hipMemcpyAsync(bins.data(), batch_bins_, bins.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read bins");
+  }
+  check(hipStreamSynchronize(s), "adjoint");
+}
+
+}  // namespace bear_amd

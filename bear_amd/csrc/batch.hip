@@ -152,24 +152,6 @@ struct Opacity32 {
   }
 };
 
-// ---------------------------------------------------------------- work items
-struct Records {
-  const unsigned char* base;
-  size_t bytes, levels, species, shifts;
-  __device__ __forceinline__ const unsigned char* at(int m) const { return base + size_t(m) * bytes; }
-};
-
-// item -> (tile, point): XCD x = item mod 8 owns tiles x, x + 8, ...; its
-// items run through every point of one tile before the next tile
-__device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile) {
-  const size_t k = item / 8;
-  point = int(k % size_t(batch));
-  tile = int(k / size_t(batch)) * 8 + int(item % 8);
-  return tile < tiles;
-}
-
-__host__ __device__ inline size_t nbItems(int tiles, int batch) { return size_t((tiles + 7) / 8) * 8 * size_t(batch); }
-
 // ---------------------------------------------------------------- emission
 // Fused opacity + two-angle short characteristics per (wavenumber, point),
 // as the single-model kernel; the flux F = 2 pi sum mu w I 1e-3 scaling goes

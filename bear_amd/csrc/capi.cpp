@@ -218,6 +218,15 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
   });
 }
 
+int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, size_t batch, size_t nb_parameters,
+                                  double* log_likelihood, double* gradient) {
+  return guarded([&] {
+    if (!r || (batch && (!parameters || !log_likelihood || !gradient)))
+      bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    r->impl->gradientBatch(parameters, batch, nb_parameters, log_likelihood, gradient);
+  });
+}
+
 int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out) {
   return guarded([&] {
     if (!r || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");

@@ -173,6 +173,25 @@ __device__ __forceinline__ void binBody(const double* __restrict__ flux, const d
   }
 }
 
+// ---------------------------------------------------------------- batch work items
+// Per-point records of a batch: [BatchModel][LevelState L][stencils L x NS][shifts]
+struct Records {
+  const unsigned char* base;
+  size_t bytes, levels, species, shifts;
+  __device__ __forceinline__ const unsigned char* at(int m) const { return base + size_t(m) * bytes; }
+};
+
+// item -> (tile, point): XCD x = item mod 8 owns tiles x, x + 8, ...; its
+// items run through every point of one tile before the next tile
+__device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile) {
+  const size_t k = item / 8;
+  point = int(k % size_t(batch));
+  tile = int(k / size_t(batch)) * 8 + int(item % 8);
+  return tile < tiles;
+}
+
+__host__ __device__ inline size_t nbItems(int tiles, int batch) { return size_t((tiles + 7) / 8) * 8 * size_t(batch); }
+
 // Fixed-order sum of n terms by one workgroup of 1024 threads.
 __device__ __forceinline__ void rowSum(const double* __restrict__ terms, int n, double* __restrict__ out) {
   __shared__ double partial[16];

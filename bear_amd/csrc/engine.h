@@ -27,6 +27,7 @@ struct LevelState {
 struct LevelSpecies {
   double n;
   double f_p_lower, f_p_upper, f_t;
+  double dft;                // d f_t / dT (the adjoint's temperature derivative)
   uint32_t row[4];
   uint32_t reload;
   uint32_t pad;
@@ -138,6 +139,13 @@ class DeviceEngine {
   const double* batchLogLikelihood() const { return batch_ll_host_; }
   void copyBatchBins(size_t batch, std::vector<double>& out);          // [batch][bins], internal order
   void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
+  // Adjoint of the last evaluateBatch (emission, FP64 tables): per point
+  // dL/d(level quantities) [batch][levels][speciesStride() + 3] = the
+  // effective density of each stencil slot, the temperature (its direct
+  // effects: Planck function and table interpolation), the altitude, and at
+  // level 0 the flux scaling; also dL/d(bin) and the bins, [batch][bins].
+  void adjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
+                    std::vector<double>& bins);
   BatchLaunch batchLaunch() const { return last_launch_; }
   size_t batchEvaluations() const { return batch_evaluations_; }
   double batchMilliseconds() const { return batch_ms_; }
@@ -226,6 +234,18 @@ class DeviceEngine {
   bool last_used_graph_ = false;
   size_t batch_evaluations_ = 0;
   double batch_ms_ = 0;
+  // adjoint: the observation operator transposed (per wavenumber: bins and
+  // weights) and work buffers
+  uint64_t* adj_ptr_ = nullptr;                  // [N + 1]
+  uint32_t* adj_bin_ = nullptr;
+  double* adj_weight_ = nullptr;
+  double* adj_fbar_ = nullptr;                   // [capacity][stride]
+  double* adj_gbar_ = nullptr;                   // [capacity][bins]
+  double* adj_scratch_ = nullptr;                // [grid][levels][2][64]
+  double* adj_partial_ = nullptr;                // [capacity][levels][slots][tiles]
+  double* adj_out_ = nullptr;                    // [capacity][levels][slots]
+  size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0;
+  void releaseAdjoint();
 };
 
 }  // namespace bear_amd

@@ -247,9 +247,10 @@ DeviceEngine::~DeviceEngine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
   releaseBatch();
+  releaseAdjoint();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
                   (void*)weights_, (void*)bins_desc_, (void*)bins_, (void*)ll_terms_, (void*)ll_,
-                  (void*)staging_dev_, (void*)spectrum_})
+                  (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
   if (event_start_) (void)hipEventDestroy(static_cast<hipEvent_t>(event_start_));
@@ -399,6 +400,35 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
         "copy weights");
   check(hipMemcpyAsync(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, s),
         "copy bins");
+  // the operator transposed, per wavenumber (bins in increasing order), for the adjoint
+  std::vector<uint64_t> ptr(n_points_ + 1, 0);
+  for (const BinDesc& d : desc)
+    for (uint32_t k = 0; k < d.length; ++k) ++ptr[d.start + k + 1];
+  for (size_t i = 0; i < n_points_; ++i) ptr[i + 1] += ptr[i];
+  std::vector<uint32_t> adj_bin(ptr.back());
+  std::vector<double> adj_w(ptr.back());
+  std::vector<uint64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (size_t j = 0; j < desc.size(); ++j)
+    for (uint32_t k = 0; k < desc[j].length; ++k) {
+      const uint64_t e = fill[desc[j].start + k]++;
+      adj_bin[e] = uint32_t(j);
+      adj_w[e] = weights[desc[j].weight_offset + k];
+    }
+  releaseAdjoint();
+  deviceFree(adj_ptr_);
+  deviceFree(adj_bin_);
+  deviceFree(adj_weight_);
+  adj_ptr_ = deviceAlloc<uint64_t>(ptr.size(), "transposed operator");
+  adj_bin_ = deviceAlloc<uint32_t>(adj_bin.size(), "transposed operator");
+  adj_weight_ = deviceAlloc<double>(adj_w.size(), "transposed operator");
+  check(hipMemcpyAsync(adj_ptr_, ptr.data(), ptr.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s),
+        "copy transposed operator");
+  if (!adj_bin.empty()) {
+    check(hipMemcpyAsync(adj_bin_, adj_bin.data(), adj_bin.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s),
+          "copy transposed operator");
+    check(hipMemcpyAsync(adj_weight_, adj_w.data(), adj_w.size() * sizeof(double), hipMemcpyHostToDevice, s),
+          "copy transposed operator");
+  }
   check(hipStreamSynchronize(s), "copy observations");
 }
 

@@ -124,6 +124,7 @@ struct TableRow { double temperature, pressure, log_pressure; uint32_t id; };
 struct Stencil {
   uint32_t row[4];
   double f_p_lower, f_p_upper, f_t;
+  double dft;                  // d f_t / dT: 1 / (T+ - T-), 0 when the T rows coincide
 };
 
 class OpacitySpecies {
@@ -190,6 +191,13 @@ class Atmosphere {
   bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param,
                const AltitudeOptions& options = AltitudeOptions());
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
+  // Reverse-mode derivative of the last compute() (constant gravity, the
+  // chemistry's own mean molecular weight): from dL/dT[l] (direct effects),
+  // dL/dn[l][id] and dL/dz[l] (consumed) to dL/dg and dL/d(parameters).
+  void adjoint(double surface_gravity, const double* temperature_param, const double* chemistry_param,
+               std::vector<double>& temperature_bar, std::vector<double>& number_density_bar,
+               std::vector<double>& altitude_bar, double& gravity_bar, double* temperature_param_bar,
+               double* chemistry_param_bar) const;
   // copies are independent (each owns its temperature profile's state)
   Atmosphere(const Atmosphere& other);
  private:
@@ -198,6 +206,8 @@ class Atmosphere {
   bool isothermal_ = false;
   size_t nb_temperature_param_ = 0, nb_chemistry_param_ = 0;
   std::vector<ChemistryModule> chemistry_;
+  std::vector<char> clamped_;               // levels floored at 50 K by the last compute()
+  bool bg_clamped_ = false;
 };
 
 }  // namespace bear_amd

@@ -152,8 +152,11 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
   }
   // the polynomial profile floors at 50 K and flags the model
   // (piecewise_poly_temperature.cpp:77-81); the isothermal one does not
+  clamped_.assign(L, 0);
+  bg_clamped_ = false;
   if (!isothermal_)
-    for (auto& t : temperature) if (t < 50) { t = 50; neglect = true; }
+    for (size_t l = 0; l < L; ++l)
+      if (temperature[l] < 50) { temperature[l] = 50; clamped_[l] = 1; neglect = true; }
 
   // chemistry modules in order (isoprofile_chemistry.cpp:87-119,
   // background_chemistry.cpp:72-137, chemistry.h:47-76)
@@ -188,7 +191,7 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
         double bg = 1.0;
         for (int j = 1; j < S; ++j)
           if (j != sid::H2 && j != sid::He) bg -= n(l, j) / n(l, sid::total);
-        if (bg < 0) { neglect = true; bg = 0; }
+        if (bg < 0) { neglect = true; bg = 0; bg_clamped_ = true; }
         n(l, sid::H2) = n(l, sid::total) * (bg * h2 / h2he);
         n(l, sid::He) = n(l, sid::total) * (bg * he / h2he);
       }
@@ -218,6 +221,115 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
                   (1.0 / (rho[l] * gl) + 1.0 / (rho[l - 1] * gl)) * 0.5 * (pressure[l - 1] * 1e6 - pressure[l] * 1e6);
   }
   return neglect;
+}
+
+// Reverse sweep through compute(), last operation first:
+//   z_l = z_{l-1} + (u_l + u_{l-1}) / (2 g) (p_{l-1} - p_l) 1e6, u = 1/rho = R T / (mu p 1e6)
+//   mu_l = sum_j n_j m_j / n_tot
+//   background: n_H2,He = n_tot bg r_H2,He, bg = 1 - sum_{j != H2,He} n_j / n_tot
+//   isoprofile: n_j = n_tot x_j, then Na = 16.2181 K unless Na is free
+//   n_tot = p 1e6 / (k T);  T_l = poly(log10 p_l; dof), dof_i = dof_{i-1} theta_i
+void Atmosphere::adjoint(double g, const double* tp, const double* cp, std::vector<double>& t_bar,
+                         std::vector<double>& n_bar, std::vector<double>& z_bar, double& g_bar, double* tp_bar,
+                         double* cp_bar) const {
+  const size_t L = nb_levels;
+  const int S = int(n_ids_);
+  auto n = [&](size_t l, int id) { return number_density[l * n_ids_ + id]; };
+  auto nb = [&](size_t l, int id) -> double& { return n_bar[l * n_ids_ + id]; };
+  std::vector<double> mu_bar(L, 0.0), ntot_bar(L, 0.0);
+  for (size_t l = 0; l < L; ++l) ntot_bar[l] = nb(l, sid::total);
+  g_bar = 0;
+  // altitude: Z_k = sum_{l >= k} dL/dz_l is the adjoint of layer k's increment h_k
+  double suffix = 0;
+  std::vector<double> u_bar(L, 0.0);
+  for (size_t k = L - 1; k >= 1; --k) {
+    suffix += z_bar[k];
+    const double rho_k = mean_molecular_weight[k] * pressure[k] * 1e6 / (phys::gas_constant * temperature[k]);
+    const double rho_p = mean_molecular_weight[k - 1] * pressure[k - 1] * 1e6 /
+                         (phys::gas_constant * temperature[k - 1]);
+    const double dp = 0.5 * (pressure[k - 1] * 1e6 - pressure[k] * 1e6);
+    const double h = (1.0 / (rho_k * g) + 1.0 / (rho_p * g)) * dp;
+    u_bar[k] += suffix * dp / g;
+    u_bar[k - 1] += suffix * dp / g;
+    g_bar -= suffix * h / g;
+  }
+  for (size_t l = 0; l < L; ++l) {
+    const double u = phys::gas_constant * temperature[l] / (mean_molecular_weight[l] * pressure[l] * 1e6);
+    t_bar[l] += u_bar[l] * u / temperature[l];
+    mu_bar[l] -= u_bar[l] * u / mean_molecular_weight[l];
+  }
+  // the last mean-weight evaluation
+  for (size_t l = 0; l < L; ++l) {
+    const double nt = n(l, sid::total);
+    for (int j = 1; j < S; ++j) nb(l, j) += mu_bar[l] * molecularWeight(j) / nt;
+    ntot_bar[l] -= mu_bar[l] * mean_molecular_weight[l] / nt;
+  }
+  // chemistry modules in reverse order
+  size_t k_end = nb_chemistry_param_;
+  for (auto m = chemistry_.rbegin(); m != chemistry_.rend(); ++m) {
+    if (m->background) {
+      const double h2 = 0.5, he = 0.085114, h2he = h2 + he;
+      for (size_t l = 0; l < L; ++l) {
+        const double nt = n(l, sid::total);
+        double bg = 1.0;
+        for (int j = 1; j < S; ++j)
+          if (j != sid::H2 && j != sid::He) bg -= n(l, j) / nt;
+        if (bg < 0) bg = 0;
+        ntot_bar[l] += nb(l, sid::H2) * bg * h2 / h2he + nb(l, sid::He) * bg * he / h2he;
+        const double bg_bar = bg_clamped_ ? 0.0 : nt * (nb(l, sid::H2) * h2 + nb(l, sid::He) * he) / h2he;
+        nb(l, sid::H2) = 0;
+        nb(l, sid::He) = 0;
+        for (int j = 1; j < S; ++j)
+          if (j != sid::H2 && j != sid::He) {
+            nb(l, j) -= bg_bar / nt;
+            ntot_bar[l] += bg_bar * n(l, j) / (nt * nt);
+          }
+      }
+    } else {
+      const size_t k0 = k_end - m->species.size();
+      for (size_t l = 0; l < L; ++l) {
+        if (!m->sodium_free) {
+          nb(l, sid::K) += 16.2181 * nb(l, sid::Na);
+          nb(l, sid::Na) = 0;
+        }
+        const double nt = n(l, sid::total);
+        for (size_t j = 0; j < m->species.size(); ++j) {
+          double& v = nb(l, m->species[j]);
+          cp_bar[k0 + j] += v * nt;
+          ntot_bar[l] += v * cp[k0 + j];
+          v = 0;
+        }
+      }
+      k_end = k0;
+    }
+  }
+  // n_tot = p 1e6 / (k T)
+  for (size_t l = 0; l < L; ++l) t_bar[l] -= ntot_bar[l] * n(l, sid::total) / temperature[l];
+  // temperature profile
+  if (isothermal_) {
+    double s = 0;
+    for (size_t l = 0; l < L; ++l) s += t_bar[l];
+    tp_bar[0] += s;
+    return;
+  }
+  const size_t nd = nb_temperature_param_;
+  std::vector<double> dof(nd), dof_bar(nd, 0.0), unit(nd, 0.0);
+  dof[0] = tp[0];
+  for (size_t i = 1; i < nd; ++i) dof[i] = dof[i - 1] * tp[i];
+  PiecewisePolynomial basis(*poly_);                  // T_l is linear in the degrees of freedom
+  for (size_t i = 0; i < nd; ++i) {
+    unit.assign(nd, 0.0);
+    unit[i] = 1.0;
+    basis.setValues(unit);
+    for (size_t l = 0; l < L; ++l)
+      if (!clamped_[l]) dof_bar[i] += t_bar[l] * basis.value(std::log10(pressure[l]));
+  }
+  // dof_i = tp_0 prod_{k=1..i} tp_k
+  double carry = 0;
+  for (size_t i = nd; i-- > 0;) {
+    carry += dof_bar[i] * dof[i];                     // sum_{j >= i} dof_bar_j dof_j
+    tp_bar[i] += carry / tp[i];
+  }
 }
 
 }  // namespace bear_amd

@@ -217,6 +217,7 @@ Stencil OpacitySpecies::bracket(double p, double t) const {
   s.f_p_lower = r0 != r1 ? (lp - r0->log_pressure) / (r1->log_pressure - r0->log_pressure) : 0.0;
   s.f_p_upper = r2 != r3 ? (lp - r2->log_pressure) / (r3->log_pressure - r2->log_pressure) : 0.0;
   s.f_t = r0 != r2 ? (t - r0->temperature) / (r2->temperature - r0->temperature) : 0.0;
+  s.dft = r0 != r2 ? 1.0 / (r2->temperature - r0->temperature) : 0.0;
   return s;
 }
 

@@ -54,7 +54,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
       double n = 0.0;
-      Stencil st{{0, 0, 0, 0}, 0.0, 0.0, 0.0};
+      Stencil st{{0, 0, 0, 0}, 0.0, 0.0, 0.0, 0.0};
       if (s < S) {
         const OpacitySpecies& sp = species_[s];
         n = atm.nd(l, sp.species_id);                                 // opacity_species.cpp:407-412
@@ -73,6 +73,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.f_p_lower = st.f_p_lower;
         d.f_p_upper = st.f_p_upper;
         d.f_t = st.f_t;
+        d.dft = st.dft;
         for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
         d.reload = changed;
         d.pad = 0;
@@ -172,6 +173,76 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
   const bool neglect = stage(p, scalars);
   engine_.evaluateEmission(scalars.scaling, inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
+}
+
+// Chain rule from the device's level gradients to the parameters
+// [log g, f, d, chemistry..., temperature..., modifiers...]: effective
+// densities to species densities (CIA products), then the atmosphere's
+// reverse sweep, then g = 10^log g and scale = (R_J / d)^2 f.
+void EmissionModel::gradientBatch(const double* params, size_t batch, size_t stride,
+                                  const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
+                                  const char* neglect, double* gradient) {
+  const size_t L = atmosphere_.nb_levels, S = species_.size(), NS = engine_.speciesStride(), K = NS + 3;
+  const size_t ids = size_t(nbSpeciesIds());
+  const size_t nb = engine_.nbBins();
+  const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
+  while (atmosphere_pool_.size() < size_t(threads)) atmosphere_pool_.emplace_back(new Atmosphere(atmosphere_));
+  std::string failure;
+  int failure_code = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (size_t b = 0; b < batch; ++b) {
+    if (neglect[b]) continue;
+    try {
+      Atmosphere& atm = *atmosphere_pool_[omp_get_thread_num()];
+      const double* p = params + b * stride;
+      const double* chem = p + 3;
+      const double* temp = chem + nb_chem_;
+      const double g = std::pow(10, p[0]);
+      atm.compute(g, temp, chem);
+      const double* G = level_grad.data() + b * L * K;
+      std::vector<double> t_bar(L), z_bar(L), n_bar(L * ids, 0.0);
+      for (size_t l = 0; l < L; ++l) {
+        t_bar[l] = G[l * K + NS];
+        z_bar[l] = G[l * K + NS + 1];
+        for (size_t s = 0; s < S; ++s) {
+          const OpacitySpecies& sp = species_[s];
+          const double gs = G[l * K + s];
+          double others = 1.0;
+          for (int q : sp.cia_partners) others *= atm.nd(l, q);
+          n_bar[l * ids + sp.species_id] += gs * others;
+          for (size_t a = 0; a < sp.cia_partners.size(); ++a) {
+            double rest = atm.nd(l, sp.species_id);
+            for (size_t c = 0; c < sp.cia_partners.size(); ++c)
+              if (c != a) rest *= atm.nd(l, sp.cia_partners[c]);
+            n_bar[l * ids + sp.cia_partners[a]] += gs * rest;
+          }
+        }
+      }
+      double* out = gradient + b * stride;
+      double g_bar = 0;
+      atm.adjoint(g, temp, chem, t_bar, n_bar, z_bar, g_bar, out + 3 + nb_chem_, out + 3);
+      out[0] = g_bar * g * std::log(10.0);
+      const double s_bar = G[NS + 2];
+      const double base = (phys::radius_jupiter / p[2]) * (phys::radius_jupiter / p[2]);
+      out[1] = s_bar * base;
+      out[2] = s_bar * (-2.0 * base * p[1] / p[2]);
+      // shift_const: one parameter per observation, added to its bins
+      size_t k = 3 + nb_chem_ + nb_temp_, j = 0;
+      for (const auto& o : observations_) {
+        double sum = 0;
+        for (size_t q = 0; q < o.nbPoints(); ++q) sum += bin_grad[b * nb + j + q];
+        if (o.modifier_parameters) out[k++] = sum;
+        j += o.nbPoints();
+      }
+    } catch (const Error& e) {
+#pragma omp critical(bear_gradient_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = e.code; }
+    } catch (const std::exception& e) {
+#pragma omp critical(bear_gradient_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = kInternal; }
+    }
+  }
+  if (!failure.empty()) raise(failure_code, failure);
 }
 
 // ---------------------------------------------------------------- secondary eclipse
@@ -359,6 +430,40 @@ void Retrieval::computeModelBatch(const double* params, size_t batch, size_t n, 
 }
 
 void Retrieval::setTablePrecision(TablePrecision p) { engine_->setTablePrecision(p, model_->speciesRows()); }
+
+// Value and gradient of the log-likelihood (retrieval.cpp:314-370) for a
+// batch: the forward batch, the device adjoint, the model's chain rule, and
+// the error-inflation term d/de sum_j w_j (-ln(2 pi s_j^2) - r_j^2 / s_j^2) / 2
+// with s_j^2 = sigma_j^2 + 10^e.
+void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, double* ll, double* gradient) {
+  if (batch == 0) return;
+  if (model_->kind() != ModelKind::emission) raise(kInvalidInput, "the gradient is available for the emission model");
+  if (engine_->tablePrecision() != TablePrecision::fp64)
+    raise(kInvalidInput, "the gradient needs the FP64 tables (table precision fp64)");
+  const std::vector<char> neglect = stageBatch(params, batch, n);
+  engine_->evaluateBatch(model_->kind(), batch);
+  const double* v = engine_->batchLogLikelihood();
+  for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
+  std::vector<double> level_grad, bin_grad, bins;
+  engine_->adjointBatch(batch, level_grad, bin_grad, bins);
+  std::fill(gradient, gradient + batch * n, 0.0);
+  model_->gradientBatch(params, batch, n, level_grad, bin_grad, neglect.data(), gradient);
+  if (!global_.error_inflation) return;
+  const size_t nb = engine_->nbBins();
+  for (size_t b = 0; b < batch; ++b) {
+    if (neglect[b]) continue;
+    const double inflation = errorInflation(params + b * n, n);
+    double sum = 0;
+    size_t j = 0;
+    for (const auto& o : observations_)
+      for (size_t q = 0; q < o.nbPoints(); ++q, ++j) {
+        const double e2 = o.error[q] * o.error[q] + inflation;
+        const double r = o.data[q] - bins[b * nb + j];
+        sum += o.weight[q] * (-0.5 / e2 + 0.5 * r * r / (e2 * e2));
+      }
+    gradient[b * n + n - 1] = sum * inflation * std::log(10.0);
+  }
+}
 
 bool Retrieval::computeModel(const double* p, size_t n, bool high_res, double* spectrum, double* spectrum_obs) {
   checkSize(n);

@@ -50,6 +50,15 @@ class ForwardModel {
                           char* neglect) = 0;
   // [first, end) of each tabulated species' rows in the table store
   virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
+  // gradient rows [batch][stride] of the model's parameters (and observation
+  // modifiers) from the engine's adjoint: level gradients [batch][levels][slots]
+  // and bin gradients [batch][bins]; rows of neglected points are left alone
+  virtual void gradientBatch(const double* parameters, size_t batch, size_t stride,
+                             const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
+                             const char* neglect, double* gradient) {
+    (void)parameters, (void)batch, (void)stride, (void)level_grad, (void)bin_grad, (void)neglect, (void)gradient;
+    raise(kInvalidInput, "the gradient is available for the emission model");
+  }
 };
 
 // What the three models share: atmosphere, opacity species with their
@@ -92,6 +101,8 @@ class EmissionModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
   ModelKind kind() const override { return ModelKind::emission; }
+  void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
+                     const std::vector<double>& bin_grad, const char* neglect, double* gradient) override;
 
  protected:
   bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
@@ -152,6 +163,8 @@ class Retrieval {
   void computeModelBatch(const double* parameters, size_t batch, size_t n, double* spectrum, double* spectrum_obs,
                          int* neglect);
   void setTablePrecision(TablePrecision p);
+  // log-likelihoods [batch] and their gradients [batch][n] (emission, FP64 tables)
+  void gradientBatch(const double* parameters, size_t batch, size_t n, double* log_likelihood, double* gradient);
   DeviceEngine& engine() const { return *engine_; }
 
  private:

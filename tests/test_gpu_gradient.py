@@ -1,0 +1,149 @@
+"""The hand-written adjoint of the emission likelihood, through the C ABI and
+the torch.autograd.Function.
+
+Bar: every gradient component within 1e-4 relative of central finite
+differences (step 1e-4 relative) of the GPU FP64 likelihood, and of the FP64
+oracle's gradient (central differences at steps h and h/2 combined by
+Richardson extrapolation), each component on its own scale (the parameters'
+units differ by tens of decades), plus the finite differences' own error
+estimate |D(h) - D(h/2)|."""
+
+import os
+
+import numpy as np
+import pytest
+
+from bear_amd import api, synth
+from oracle import api as oracle_api
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+STEP = 1e-4
+
+
+def build_inflated_tiny(folder):
+    """The tiny emission case with error inflation (last parameter log10 of
+    the added variance), data injected from the oracle."""
+    case = synth.tiny_case()
+    case.use_error_inflation = True
+    # inflation of a third of the median variance
+    errs = np.concatenate([ob.error for ob in case.observations])
+    truth = np.concatenate([case.parameters, [np.log10(0.3 * np.median(errs) ** 2)]])
+    case.parameters = truth
+    folder = synth.write_case(case, folder)
+    o = oracle_api.EmissionOracle(folder)
+    r = o.evaluate(truth)
+    bins = [b[::-1].copy() if o.observation(i)["ascending"] else b for i, b in enumerate(r["bins"])]
+    synth.inject_data(case, bins)
+    folder = synth.write_case(case, folder)
+    return folder, [truth, synth.perturbed(truth, case, seed=21)]
+
+
+_CASES = {}
+
+
+def gradient_case(case_factory, tmp_path_factory, name):
+    if name not in _CASES:
+        if name == "tiny_inflated":
+            folder, params = build_inflated_tiny(str(tmp_path_factory.mktemp(name)))
+        else:
+            folder, params = case_factory(name)
+            params = params[:2]
+        _CASES[name] = (folder, params, synth.open_retrieval(folder))
+    return _CASES[name]
+
+
+def steps(p):
+    return STEP * np.maximum(np.abs(p), 1e-12)
+
+
+def central(f, p, h):
+    g = np.empty_like(p)
+    for k in range(p.size):
+        e = np.zeros_like(p)
+        e[k] = h[k]
+        g[k] = (f(p + e) - f(p - e)) / (2 * h[k])
+    return g
+
+
+def check(grad, ref, label, fd_error):
+    tol = RTOL * np.abs(ref) + 10 * fd_error + 1e-300
+    rel = np.abs(grad - ref) / np.maximum(np.abs(ref), 1e-300)
+    print(f"[gpu] {label}: max relative difference {rel.max():.2e} (component {rel.argmax()})", flush=True)
+    bad = np.abs(grad - ref) > tol
+    for k in np.where(bad)[0]:
+        print(f"[gpu] {label} component {k}: adjoint {grad[k]:.10e} reference {ref[k]:.10e}", flush=True)
+    assert not bad.any(), label
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1"])
+def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, name):
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
+    r.set_table_precision("fp64")
+    pts = np.array(params)
+    ll, grad = r.gradientBatch(pts)
+    for k, p in enumerate(pts):
+        print(f"[gpu] gradient {name} point {k}", flush=True)
+        assert abs(ll[k] - r.computeLikelihood(p)) <= 1e-9 * max(1.0, abs(ll[k]))
+        h = steps(p)
+        fd = central(r.computeLikelihood, p, h)
+        fd2 = central(r.computeLikelihood, p, h / 2)
+        check(grad[k], fd2, f"{name} point {k} vs GPU central differences", np.abs(fd - fd2))
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated"])
+def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, name):
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
+    r.set_table_precision("fp64")
+    o = oracle_api.EmissionOracle(folder)
+    f = lambda q: o.evaluate(q)["loglike"]
+    p = np.array(params[0])
+    _, grad = r.gradientBatch(p[None, :])
+    h = steps(p)
+    d1, d2 = central(f, p, h), central(f, p, h / 2)
+    ref = (4 * d2 - d1) / 3
+    check(grad[0], ref, f"{name} vs oracle Richardson gradient", np.abs(ref - d2))
+
+
+def test_autograd_function_backward(case_factory, tmp_path_factory):
+    import torch
+    from bear_amd.autograd import log_likelihood
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, "tiny")
+    r.set_table_precision("fp64")
+    pts = np.array(params)
+    x = torch.tensor(pts, dtype=torch.float64, requires_grad=True)
+    ll = log_likelihood(r, x)
+    w = torch.tensor([0.7, -1.3], dtype=torch.float64)
+    (ll * w).sum().backward()
+    ll_ref, grad_ref = r.gradientBatch(pts)
+    np.testing.assert_array_equal(ll.detach().numpy(), ll_ref)
+    np.testing.assert_allclose(x.grad.numpy(), w.numpy()[:, None] * grad_ref, rtol=1e-15, atol=0)
+    plain = log_likelihood(r, torch.tensor(pts, dtype=torch.float64))
+    np.testing.assert_array_equal(plain.numpy(), r.computeLikelihoodBatch(pts))
+
+
+def test_gradient_refuses_other_models_and_precisions(case_factory, tmp_path_factory):
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, "tiny")
+    r.set_table_precision("fp32")
+    with pytest.raises(api.BearError) as e:
+        r.gradientBatch(np.array(params))
+    assert e.value.code == -1
+    r.set_table_precision("fp64")
+    folder_t, params_t = case_factory("transmission_plain")
+    rt = synth.open_retrieval(folder_t)
+    with pytest.raises(api.BearError) as e:
+        rt.gradientBatch(np.array(params_t[:1]))
+    assert e.value.code == -1
+    rt.close()
+
+
+def test_neglected_point_has_zero_gradient(case_factory, tmp_path_factory):
+    folder, params = case_factory("tiny")
+    r = gradient_case(case_factory, tmp_path_factory, "tiny")[2]
+    r.set_table_precision("fp64")
+    cold = params[4]
+    ll, grad = r.gradientBatch(np.array([params[0], cold]))
+    assert ll[1] == -1e30
+    assert np.all(grad[1] == 0)
+    assert np.any(grad[0] != 0)
