@@ -124,6 +124,30 @@ def batched_leg(r, points, batch_steps, plain_rate):
                           "launch": {k: st[k] for k in ("block", "tile", "grid", "used_graph")}}
         log(f"batch {precision}: {rate:.1f} evals/s, spectrum err {err_spec:.2e}, |dlogL| {err_ll:.2e}")
     r.set_table_precision("fp64")
+    # value and gradient (hand-written adjoint) of the same batch
+    r.gradientBatch(pts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(batch_steps):
+        ll, grad = r.gradientBatch(pts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    rate = batch_steps * len(pts) / elapsed
+    # spot check at full size: central differences (step 1e-4 relative) of
+    # three components of the first point through the single-point path
+    p0 = pts[0]
+    fd_rel = {}
+    for k, label in ((0, "log_g"), (3, "first_mixing_ratio"), (9, "first_temperature")):
+        h = 1e-4 * abs(p0[k])
+        e = np.zeros_like(p0)
+        e[k] = h
+        fd = (r.computeLikelihood(p0 + e) - r.computeLikelihood(p0 - e)) / (2 * h)
+        fd_rel[label] = abs(grad[0][k] - fd) / abs(fd)
+    out["adjoint"] = {"evals_per_s": round(rate, 1), "ms_per_batch": round(elapsed / batch_steps * 1e3, 3),
+                      "cost_vs_fp64_forward": round(rate and out["fp64"]["evals_per_s"] / rate, 3),
+                      "table_dtype": "f64", "fd_check_rel_diff": fd_rel}
+    log(f"adjoint: {rate:.1f} value+gradient/s, {out['adjoint']['cost_vs_fp64_forward']}x the FP64 forward, "
+        f"FD check {fd_rel}")
     return out
 
 
