@@ -90,30 +90,31 @@ __global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __re
 template <int NS>
 __device__ __forceinline__ void opacityAdjoint(const double* __restrict__ tab, size_t stride,
                                                const LevelSpecies* __restrict__ d,
-                                               const LevelSpecies* __restrict__ above, bool force,
+                                               const LevelSpecies* __restrict__ above,
+                                               const LevelSpeciesAdjoint* __restrict__ a, bool force,
                                                double (&v)[NS][4], double (&sigma)[NS], double& kappa,
                                                double& dkdt) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     bool reload = force;
     if (!force)
-      reload = d[s].row[0] != above[s].row[0] || d[s].row[1] != above[s].row[1] ||
-               d[s].row[2] != above[s].row[2] || d[s].row[3] != above[s].row[3];
+      reload = d[s].block[0] != above[s].block[0] || d[s].block[1] != above[s].block[1] ||
+               d[s].block[2] != above[s].block[2] || d[s].block[3] != above[s].block[3];
     if (reload) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
     }
   }
   double k = 0.0, dk = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-    const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-    const double x = fma(hi - lo, d[s].f_t, lo);
+    const double lo = fma(v[s][1] - v[s][0], a[s].f_p_lower, v[s][0]);
+    const double hi = fma(v[s][3] - v[s][2], a[s].f_p_upper, v[s][2]);
+    const double x = fma(hi - lo, a[s].f_t, lo);
     const double sg = pow2(x * kLog2Ten);
     sigma[s] = sg;
-    k = fma(d[s].n, sg, k);
-    dk = fma(d[s].n * sg, (hi - lo) * d[s].dft, dk);
+    k = fma(a[s].n, sg, k);
+    dk = fma(a[s].n * sg, (hi - lo) * a[s].dft, dk);
   }
   kappa = k;
   dkdt = dk * kLn10;
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
+    const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
     double* out = partial + size_t(m) * L * K * size_t(tiles) + tile;
     const double* tab = tables + ic;
     const double nu = wavenumber[ic];
@@ -194,14 +196,14 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 
     // forward sweep, keeping I_l (l < L-1)
     double v[NS][4];
-    double k_prev = cachedKappa<NS>(tab, stride, species, true, v);
+    double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
     store[lane] = i1;
     store[kWave + lane] = i2;
     for (int l = 1; l < L; ++l) {
-      const double k_next = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, false, v);
+      const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -224,14 +226,15 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     // backward sweep
     double sig_hi[NS], sig_lo[NS];
     double k_hi, dk_hi, b_hi, db_hi;
-    opacityAdjoint<NS>(tab, stride, species + size_t(L - 1) * NS, nullptr, true, v, sig_hi, k_hi, dk_hi);
+    opacityAdjoint<NS>(tab, stride, species + size_t(L - 1) * NS, nullptr, extra + size_t(L - 1) * NS, true, v,
+                       sig_hi, k_hi, dk_hi);
     planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
     double z_hi = levels[L - 1].altitude;
     double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
     for (int l = L - 1; l >= 1; --l) {
       double k_lo, dk_lo, b_lo, db_lo;
-      opacityAdjoint<NS>(tab, stride, species + size_t(l - 1) * NS, species + size_t(l) * NS, false, v, sig_lo, k_lo,
-                         dk_lo);
+      opacityAdjoint<NS>(tab, stride, species + size_t(l - 1) * NS, species + size_t(l) * NS,
+                         extra + size_t(l - 1) * NS, false, v, sig_lo, k_lo, dk_lo);
       planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
       const double z_lo = levels[l - 1].altitude;
       const double dz = z_hi - z_lo;
@@ -313,7 +316,7 @@ void DeviceEngine::adjointBatch(size_t batch, std::vector<double>& level_grad, s
     adj_grid_ = size_t(grid);
     adj_partial_size_ = partial_size;
   }
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
   if (nb > 0) {
     hipLaunchKernelGGL(gbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(B)), dim3(kBlock), 0, s,
                        batch_bins_, bins_desc_, nb, rec, adj_gbar_);

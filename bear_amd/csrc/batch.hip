@@ -105,12 +105,12 @@ struct Opacity64 {
   const double* tab;
   size_t stride;
   double v[NS][4];
-  __device__ __forceinline__ void init(const void* tables, const float*, size_t s, int i) {
+  __device__ __forceinline__ void init(const void* tables, const float*, size_t s, int i, int) {
     tab = static_cast<const double*>(tables) + i;
     stride = s;
   }
   __device__ __forceinline__ double kappa(const LevelSpecies* d, bool force) {
-    return cachedKappa<NS>(tab, stride, d, force, v);
+    return kappaTerms<NS>(tab, stride, d, force, v);
   }
 };
 
@@ -125,18 +125,19 @@ struct Opacity32 {
   size_t stride;
   float bz[NS];
   float v[NS][4];
-  __device__ __forceinline__ void init(const void* tables, const float* base, size_t s, int i) {
+  // slot0: the first stencil slot this thread handles (its base row)
+  __device__ __forceinline__ void init(const void* tables, const float* base, size_t s, int i, int slot0) {
     tab = static_cast<const typename Tab::T*>(tables) + i;
     stride = s;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) bz[q] = base[size_t(q) * s + i] * kLog2TenF;
+    for (int q = 0; q < NS; ++q) bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
   }
   __device__ __forceinline__ float kappa(const LevelSpecies32* d, bool force) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
       if (force || d[s].reload) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[s][q] = Tab::load(tab + size_t(d[s].row[q]) * stride);
+        for (int q = 0; q < 4; ++q) v[s][q] = Tab::load(tab + (size_t(d[s].block[q]) << 6));
       }
     float k = 0.0f;
 #pragma unroll
@@ -154,8 +155,8 @@ struct Opacity32 {
 
 // ---------------------------------------------------------------- emission
 // Fused opacity + two-angle short characteristics per (wavenumber, point),
-// as the single-model kernel; the flux F = 2 pi sum mu w I 1e-3 scaling goes
-// to flux[point][i].
+// as the single-point kernel; a work item is blockDim wavenumbers of one
+// point and the flux F = 2 pi sum mu w I 1e-3 scaling goes to flux[point][i].
 template <class Op>
 __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const Species* species = reinterpret_cast<const Species*>(r + rec.species);
     Op op;
-    op.init(tables, base, stride, i);
+    op.init(tables, base, stride, i, 0);
     const double nu = wavenumber[i];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
     const int i = tl * tile + j;
     if (i < n) {
       Op op;
-      op.init(tables, base, stride, i);
+      op.init(tables, base, stride, i, 0);
       const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
       for (int l = l0; l < l1; ++l) {
         const double k = double(op.kappa(species + size_t(l) * Op::kNS, l == l0));
@@ -455,6 +456,11 @@ LevelSpecies* DeviceEngine::batchSpecies(size_t b) {
   return reinterpret_cast<LevelSpecies*>(record(b) + rec_species_);
 }
 
+LevelSpeciesAdjoint* DeviceEngine::batchSpeciesAdjoint(size_t b) {
+  if (precision_ != TablePrecision::fp64) raise(kInternal, "adjoint stencils requested with reduced-precision tables");
+  return reinterpret_cast<LevelSpeciesAdjoint*>(record(b) + rec_adjoint_);
+}
+
 LevelSpecies32* DeviceEngine::batchSpecies32(size_t b) {
   if (precision_ == TablePrecision::fp64) raise(kInternal, "FP32 stencils requested with FP64 tables");
   return reinterpret_cast<LevelSpecies32*>(record(b) + rec_species_);
@@ -469,9 +475,11 @@ void DeviceEngine::reserveBatch(size_t batch) {
   releaseGraph();
   const size_t L = nb_levels_, NS = species_stride_;
   const size_t spc = precision_ == TablePrecision::fp64 ? sizeof(LevelSpecies) : sizeof(LevelSpecies32);
-  rec_levels_ = align(sizeof(BatchModel), 16);
-  rec_species_ = align(rec_levels_ + L * sizeof(LevelState), 16);
-  rec_shifts_ = align(rec_species_ + L * NS * spc, 16);
+  const bool wide = precision_ == TablePrecision::fp64;
+  rec_levels_ = align(sizeof(BatchModel), 64);
+  rec_species_ = align(rec_levels_ + L * sizeof(LevelState), 64);
+  rec_adjoint_ = wide ? align(rec_species_ + L * NS * spc, 64) : 0;
+  rec_shifts_ = align(wide ? rec_adjoint_ + L * NS * sizeof(LevelSpeciesAdjoint) : rec_species_ + L * NS * spc, 64);
   rec_bytes_ = align(rec_shifts_ + (n_obs_ + 1) * sizeof(double), 256);
   if (batch_host_) (void)hipHostFree(batch_host_);
   if (batch_ll_host_) (void)hipHostFree(batch_ll_host_);
@@ -505,7 +513,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
 // ---------------------------------------------------------------- launches
 void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
   const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
   const bool wide = precision_ == TablePrecision::fp64;
   if (kind == ModelKind::transmission) {
@@ -551,7 +559,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 
 void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
   if (n_bins_ > 0) {
     hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s, batch_flux_,
                        stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
@@ -692,7 +700,7 @@ void DeviceEngine::copyBatchBins(size_t batch, std::vector<double>& out) {
 
 void DeviceEngine::copyBatchSpectrum(ModelKind kind, size_t batch, double* host) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_shifts_};
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
   const int n = int(n_points_);
   double* out = deviceAlloc<double>(batch * n_points_, "batch spectrum output");
   try {

@@ -43,6 +43,30 @@ __device__ __forceinline__ double pow2(double t) {
   return __builtin_amdgcn_ldexp(p, int(n));
 }
 
+// 2^t with two polynomial terms fewer: degree-8 Chebyshev interpolant of 2^f
+// on [-1/2, 1/2] (relative error 1.1e-12), exponent shift.  For quantities
+// that enter the result linearly: the opacity terms (they add, nothing
+// cancels) and the Planck function.  The attenuation factors e^-delta of the
+// radiative transfer keep pow2: (e^-delta - 1) / delta cancels for thin
+// layers and magnifies the exponential's error by 1 / delta.
+// t >= -1100 (so -inf, a zero density, gives 0); t is bounded above by the
+// physics (sigma n < 2^1000, Planck arguments <= 0).
+__device__ __forceinline__ double pow2Lean(double t) {
+  t = fmax(t, -1100.0);
+  const double n = __builtin_rint(t);
+  const double f = t - n;
+  double p = 1.3255225648015585e-06;
+  p = fma(p, f, 1.5310079150997826e-05);
+  p = fma(p, f, 0.00015403455848948458);
+  p = fma(p, f, 0.001333345056925233);
+  p = fma(p, f, 0.009618129159399412);
+  p = fma(p, f, 0.055504109412040106);
+  p = fma(p, f, 0.24022650695813164);
+  p = fma(p, f, 0.6931471805459343);
+  p = fma(p, f, 0.9999999999999999);
+  return __builtin_amdgcn_ldexp(p, int(n));
+}
+
 // 1/d for finite non-zero d: hardware estimate and two Newton steps
 __device__ __forceinline__ double recip(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -54,7 +78,7 @@ __device__ __forceinline__ double recip(double d) {
 // Planck function per wavenumber written as c1 nu^3 q / (1 - q),
 // q = e^(-h nu c / k T): the reference's expression without exp overflow
 __device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv_t) {
-  const double q = pow2(x_nu * inv_t);
+  const double q = pow2Lean(x_nu * inv_t);
   return b_num * q * recip(1.0 - q);
 }
 
@@ -78,33 +102,35 @@ __device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_p
   }
 }
 
-// kappa_l(nu) = sum_s n_s 10^x_s, x_s the bilinear log10 cross-section of
-// species s at this level (opacity_species.cpp:277-345, 400-433), over NS
-// (padded) species whose four stencil values v[s][] stay in registers; rows
-// are reloaded where `reload` is set.  Rows are always valid; a zero density
-// or a zero weight removes a term exactly.
-template <int NS>
-__device__ __forceinline__ double cachedKappa(const double* __restrict__ tab, size_t stride,
-                                              const LevelSpecies* __restrict__ d, bool force, double (&v)[NS][4]) {
+// The opacity terms of NH consecutive stencil slots at one level,
+// sum_s 2^(log2 n_s + sum_k w_sk v_sk) with the slots' four table values v
+// kept in registers between levels and reloaded where `reload` is set
+// (opacity_species.cpp:277-345, 400-433: the bilinear log10 interpolation in
+// T and log p, times the effective density).  Rows are always valid; a zero
+// density (log2 n = -inf) removes a term exactly.
+template <int NH>
+__device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, size_t stride,
+                                             const LevelSpecies* __restrict__ d, bool force, double (&v)[NH][4]) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
+  for (int s = 0; s < NH; ++s)
     if (force || d[s].reload) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
+      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
     }
   double k = 0.0;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-    const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-    const double x = fma(hi - lo, d[s].f_t, lo);
-    k = fma(d[s].n, pow2(x * kLog2Ten), k);
+  for (int s = 0; s < NH; ++s) {
+    double x = fma(d[s].w[0], v[s][0], d[s].log2_n);
+    x = fma(d[s].w[1], v[s][1], x);
+    x = fma(d[s].w[2], v[s][2], x);
+    x = fma(d[s].w[3], v[s][3], x);
+    k += pow2Lean(x);
   }
   return k;
 }
 
 // The same without a register cache, for any number of species: chunks of
-// kChunk species with every row of a chunk loaded before any is used.
+// kChunk slots with every row of a chunk loaded before any is used.
 constexpr int kChunk = 4;
 
 __device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, size_t stride,
@@ -112,17 +138,7 @@ __device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, s
   double k = 0.0;
   for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
     double v[kChunk][4];
-#pragma unroll
-    for (int s = 0; s < kChunk; ++s)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].row[q]) * stride];
-#pragma unroll
-    for (int s = 0; s < kChunk; ++s) {
-      const double lo = fma(v[s][1] - v[s][0], d[s].f_p_lower, v[s][0]);
-      const double hi = fma(v[s][3] - v[s][2], d[s].f_p_upper, v[s][2]);
-      const double x = fma(hi - lo, d[s].f_t, lo);
-      k = fma(d[s].n, pow2(x * kLog2Ten), k);
-    }
+    k += kappaTerms<kChunk>(tab, stride, d, true, v);
   }
   return k;
 }
@@ -174,10 +190,11 @@ __device__ __forceinline__ void binBody(const double* __restrict__ flux, const d
 }
 
 // ---------------------------------------------------------------- batch work items
-// Per-point records of a batch: [BatchModel][LevelState L][stencils L x NS][shifts]
+// Per-point records of a batch: [BatchModel][LevelState L][stencils L x NS]
+// [adjoint stencils L x NS (FP64 tables)][shifts]
 struct Records {
   const unsigned char* base;
-  size_t bytes, levels, species, shifts;
+  size_t bytes, levels, species, adjoint, shifts;
   __device__ __forceinline__ const unsigned char* at(int m) const { return base + size_t(m) * bytes; }
 };
 

@@ -19,29 +19,45 @@ struct LevelState {
   double rayleigh_density;   // number density of the Rayleigh scatterer (transmission)
 };
 
-// Per (level, opacity species): effective number density (n_s, or n_s times
-// the CIA partners' densities) and the table stencil.  n == 0 removes the
-// species' term, as opacity_species.cpp:409-412 skips it.  reload != 0 when
-// the stencil rows differ from the previous level's (always at level 0):
-// the kernels keep the four row values in registers otherwise.
-struct LevelSpecies {
-  double n;
-  double f_p_lower, f_p_upper, f_t;
-  double dft;                // d f_t / dT (the adjoint's temperature derivative)
-  uint32_t row[4];
+// Per (level, opacity species) for the forward kernels: the table stencil
+// and the effective number density n (n_s, or n_s times the CIA partners'
+// densities) in the form the kernels use,
+//   sigma n = 2^(log2_n + sum_k w_k v_k),
+// v the four stencil rows' log10 cross-sections, w the bilinear weights
+// log2(10) {(1-f_t)(1-f_p-), (1-f_t) f_p-, f_t (1-f_p+), f_t f_p+}
+// (opacity_species.cpp:277-345), log2_n = -inf for n == 0, which removes the
+// species' term as opacity_species.cpp:409-412 skips it.  block[k] locates
+// stencil row k in the tables: row * stride / 64 (the stride is a multiple of
+// 64), so an address is one shift.  reload != 0 when the stencil rows differ
+// from the previous level's (always at level 0): the kernels keep the four
+// row values in registers otherwise.  64 bytes, aligned.
+struct alignas(64) LevelSpecies {
+  double w[4];
+  double log2_n;
+  uint32_t block[4];
   uint32_t reload;
   uint32_t pad;
+};
+
+// The same stencil as the adjoint reads it: n, the interpolation fractions
+// and d f_t / dT = 1 / (T+ - T-) (0 when the T rows coincide).
+struct alignas(64) LevelSpeciesAdjoint {
+  double n;
+  double f_p_lower, f_p_upper, f_t;
+  double dft;
+  double pad[3];
 };
 
 // Reduced-precision form of LevelSpecies: the bilinear weights
 // (1-f_t)(1-f_p-), (1-f_t) f_p-, f_t (1-f_p+), f_t f_p+ times log2(10), and
 // log2 of the effective density (-inf when it is zero), so that
 // sigma n = 2^(log2_n + base + sum_k w[k] v[k]) for the stored residuals v.
-struct LevelSpecies32 {
+struct alignas(16) LevelSpecies32 {
   float w[4];
   float log2_n;
-  uint32_t row[4];
+  uint32_t block[4];                 // row * stride / 64, as in LevelSpecies
   uint32_t reload;
+  uint32_t pad[2];
 };
 
 // Storage of the log10 cross-section tables read by the batched kernels.
@@ -100,6 +116,7 @@ class DeviceEngine {
   LevelState* levels() { return levels_host_; }
   LevelSpecies* levelSpecies() { return species_host_; }    // [level][speciesStride()]
   size_t speciesStride() const { return species_stride_; }
+  size_t tableStride() const { return stride_; }          // elements per table row, a multiple of 64
   double* shifts() { return shift_host_; }
   double* pathLengths() { return path_host_; }              // [level][level] (transmission)
 
@@ -132,6 +149,7 @@ class DeviceEngine {
   BatchModel* batchModel(size_t b) { return reinterpret_cast<BatchModel*>(record(b)); }
   LevelState* batchLevels(size_t b) { return reinterpret_cast<LevelState*>(record(b) + rec_levels_); }
   LevelSpecies* batchSpecies(size_t b);          // fp64 tables
+  LevelSpeciesAdjoint* batchSpeciesAdjoint(size_t b);   // fp64 tables, for the adjoint
   LevelSpecies32* batchSpecies32(size_t b);      // reduced-precision tables
   double* batchShifts(size_t b) { return reinterpret_cast<double*>(record(b) + rec_shifts_); }
   // spectra, bins and log-likelihoods of live points [0, batch)
@@ -216,7 +234,7 @@ class DeviceEngine {
   void* tables_low_ = nullptr;                   // residuals [rows][stride] (fp32 / fp16 / bf16)
   float* table_base_ = nullptr;                  // [species stride][stride]
   bool use_graph_ = true, autotune_ = true;
-  size_t batch_capacity_ = 0, rec_bytes_ = 0, rec_levels_ = 0, rec_species_ = 0, rec_shifts_ = 0;
+  size_t batch_capacity_ = 0, rec_bytes_ = 0, rec_levels_ = 0, rec_species_ = 0, rec_adjoint_ = 0, rec_shifts_ = 0;
   unsigned char* batch_host_ = nullptr;          // pinned records
   unsigned char* batch_dev_ = nullptr;
   double* batch_flux_ = nullptr;                 // [capacity][stride]

@@ -62,10 +62,11 @@ __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2
 }
 
 // Fused opacity + emission radiative transfer, one wavenumber per thread,
-// levels bottom to top (short_characteristics.cpp:39-125).  The four stencil
-// values of every species stay in registers from level to level, so each
-// table row is read about once per thread; kappa and the intensities never
-// leave registers.
+// levels bottom to top (short_characteristics.cpp:39-125).  The four table
+// values of every stencil slot stay in registers from level to level and are
+// reloaded only where the rows change, so each table row is read about once
+// per thread; kappa and the intensities never leave registers.  The stencil
+// fields are wave-uniform (scalar loads).
 template <int NS>
 __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
@@ -79,12 +80,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   double v[NS][4];
-  double k_prev = cachedKappa<NS>(tab, stride, species, true, v);
+  double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, false, v);
+    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void transmissionKernel(
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
     for (int l = l0; l < l1; ++l) {
-      const double k = cachedKappa<NS>(tab, stride, species + size_t(l) * NS, l == l0, v);
+      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, l == l0, v);
       K[l * tile + j] = k + sigma_r * levels[l].rayleigh_density;
     }
   } else {
@@ -292,8 +293,9 @@ void DeviceEngine::countRows() {
   size_t count = 0;
   for (size_t k = 0; k < nb_levels_ * species_stride_; ++k) {
     const LevelSpecies& d = species_host_[k];
-    if (d.n == 0.0) continue;
-    const uint32_t r[4] = {d.row[0], d.row[1], d.row[2], d.row[3]};
+    if (std::isinf(d.log2_n) && d.log2_n < 0) continue;            // zero density: term removed
+    uint32_t r[4];
+    for (int q = 0; q < 4; ++q) r[q] = uint32_t(size_t(d.block[q]) * 64 / stride_);
     const bool use[4] = {true, r[1] != r[0], r[2] != r[0], r[2] != r[0] && r[3] != r[2]};
     for (int q = 0; q < 4; ++q)
       if (use[q] && r[q] < n_rows_ && !row_seen_[r[q]]) { row_seen_[r[q]] = 1; ++count; }
@@ -337,6 +339,8 @@ void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<d
 void DeviceEngine::uploadTables(const TableStore& store) {
   check(hipSetDevice(device_), "hipSetDevice");
   if (store.nb_rows > 0 && store.nb_points != n_points_) raise(kInvalidInput, "table rows do not match the grid");
+  if (size_t(store.nb_rows) * (stride_ / 64) > size_t(UINT32_MAX))   // stencils hold 32-bit block offsets
+    raise(kInvalidInput, "tables too large: rows x wavenumbers exceeds 2^38 values");
   deviceFree(tables_);
   tables_ = deviceAlloc<double>(size_t(store.nb_rows) * stride_, "cross-section tables");
   n_rows_ = store.nb_rows;

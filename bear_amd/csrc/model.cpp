@@ -43,9 +43,10 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
 }
 
 void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, LevelState* lev, LevelSpecies* spc,
-                                 LevelSpecies32* spc32, double* shifts) const {
+                                 LevelSpeciesAdjoint* adj, LevelSpecies32* spc32, double* shifts) const {
   const size_t L = atm.nb_levels, S = species_.size(), stride = engine_.speciesStride();
   const double log2_ten = 3.321928094887362347870319429489390175864831393;
+  const size_t blocks_per_row = engine_.tableStride() / 64;     // row -> block offset of the device tables
   for (size_t l = 0; l < L; ++l) {
     const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
@@ -61,20 +62,32 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         for (int q : sp.cia_partners) n *= atm.nd(l, q);
         st = sp.bracket(atm.pressure[l], t);                         // rows stay valid when n == 0
       }
+      uint32_t block[4];
+      for (int k = 0; k < 4; ++k) block[k] = uint32_t(st.row[k] * blocks_per_row);
       // reload the rows only where they differ from the previous level's
       uint32_t changed = l == 0;
       if (l > 0) {
-        const uint32_t* prev = spc ? spc[(l - 1) * stride + s].row : spc32[(l - 1) * stride + s].row;
-        for (int k = 0; k < 4; ++k) changed |= prev[k] != st.row[k];
+        const uint32_t* prev = spc ? spc[(l - 1) * stride + s].block : spc32[(l - 1) * stride + s].block;
+        for (int k = 0; k < 4; ++k) changed |= prev[k] != block[k];
+      }
+      if (adj) {
+        LevelSpeciesAdjoint& a = adj[l * stride + s];
+        a = LevelSpeciesAdjoint{};
+        a.n = n;
+        a.f_p_lower = st.f_p_lower;
+        a.f_p_upper = st.f_p_upper;
+        a.f_t = st.f_t;
+        a.dft = st.dft;
       }
       if (spc) {
         LevelSpecies& d = spc[l * stride + s];
-        d.n = n;
-        d.f_p_lower = st.f_p_lower;
-        d.f_p_upper = st.f_p_upper;
-        d.f_t = st.f_t;
-        d.dft = st.dft;
-        for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
+        const double ft = st.f_t, fl = st.f_p_lower, fu = st.f_p_upper;
+        d.w[0] = (1.0 - ft) * (1.0 - fl) * log2_ten;
+        d.w[1] = (1.0 - ft) * fl * log2_ten;
+        d.w[2] = ft * (1.0 - fu) * log2_ten;
+        d.w[3] = ft * fu * log2_ten;
+        d.log2_n = n > 0 ? std::log2(n) : -std::numeric_limits<double>::infinity();
+        for (int k = 0; k < 4; ++k) d.block[k] = block[k];
         d.reload = changed;
         d.pad = 0;
       }
@@ -86,8 +99,9 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.w[2] = float(ft * (1.0 - fu) * log2_ten);
         d.w[3] = float(ft * fu * log2_ten);
         d.log2_n = n > 0 ? float(std::log2(n)) : -std::numeric_limits<float>::infinity();
-        for (int k = 0; k < 4; ++k) d.row[k] = st.row[k];
+        for (int k = 0; k < 4; ++k) d.block[k] = block[k];
         d.reload = changed;
+        d.pad[0] = d.pad[1] = 0;
       }
     }
   }
@@ -101,7 +115,8 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
 bool TabulatedModel::stage(const double* p, BatchModel& scalars) {
   scalars = BatchModel{};
   const bool neglect = setup(p, atmosphere_, scalars);
-  stageLevels(atmosphere_, modifiers(p), engine_.levels(), engine_.levelSpecies(), nullptr, engine_.shifts());
+  stageLevels(atmosphere_, modifiers(p), engine_.levels(), engine_.levelSpecies(), nullptr, nullptr,
+              engine_.shifts());
   return neglect;
 }
 
@@ -122,7 +137,8 @@ void TabulatedModel::stageBatch(const double* params, size_t batch, size_t strid
       neglect[b] = setup(p, atm, scalars);
       scalars.inflation = inflation[b];
       stageLevels(atm, modifiers(p), engine_.batchLevels(b), wide ? engine_.batchSpecies(b) : nullptr,
-                  wide ? nullptr : engine_.batchSpecies32(b), engine_.batchShifts(b));
+                  wide ? engine_.batchSpeciesAdjoint(b) : nullptr, wide ? nullptr : engine_.batchSpecies32(b),
+                  engine_.batchShifts(b));
     } catch (const Error& e) {
 #pragma omp critical(bear_stage_failure)
       if (failure.empty()) { failure = e.what(); failure_code = e.code; }

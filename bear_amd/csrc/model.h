@@ -77,7 +77,7 @@ class TabulatedModel : public ForwardModel {
   // levels, stencils (FP64 and/or FP32 form) and observation shifts of one
   // computed atmosphere
   void stageLevels(const Atmosphere& atmosphere, const double* modifiers, LevelState* levels, LevelSpecies* species,
-                   LevelSpecies32* species32, double* shifts) const;
+                   LevelSpeciesAdjoint* species_adjoint, LevelSpecies32* species32, double* shifts) const;
   // the single-model path: setup + stageLevels into the engine's staging
   bool stage(const double* parameters, BatchModel& scalars);
   const double* modifiers(const double* p) const { return p + nb_general_ + nb_chem_ + nb_temp_; }
