@@ -63,7 +63,8 @@ class EvalStats(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile", C.c_int), ("grid", C.c_int),
-                ("used_graph", C.c_int), ("evaluations", C.c_size_t), ("gpu_ms_total", C.c_double)]
+                ("used_graph", C.c_int), ("evaluations", C.c_size_t), ("gpu_ms_total", C.c_double),
+                ("adjoint_chunk", C.c_size_t)]
 
 
 TABLE_PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
@@ -105,6 +106,7 @@ def lib():
         L.bear_retrieval_compute_model_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp, C.POINTER(C.c_int)]
         L.bear_retrieval_batch_stats.argtypes = [vp, C.POINTER(BatchStats)]
         L.bear_retrieval_gradient_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp]
+        L.bear_retrieval_set_adjoint_memory.argtypes = [vp, sz]
         _lib = L
     return _lib
 
@@ -319,6 +321,11 @@ class Retrieval:
         grad = np.empty_like(p)
         _check(self._L.bear_retrieval_gradient_batch(self._h, _ptr(p), p.shape[0], p.shape[1], _ptr(ll), _ptr(grad)))
         return ll, grad
+
+    def set_adjoint_memory(self, nbytes):
+        """Device memory the gradient may hold for stored forward intensities;
+        0 recomputes them inside the backward kernel."""
+        _check(self._L.bear_retrieval_set_adjoint_memory(self._h, int(nbytes)))
 
     def batch_stats(self) -> dict:
         st = BatchStats()

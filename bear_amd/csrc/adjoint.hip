@@ -1,7 +1,7 @@
 // Adjoint of the batched emission likelihood on gfx950: the gradient of
 // log L with respect to every level's temperature, altitude and effective
-// species densities (and the flux scaling), for every live point of the last
-// batch.  The host carries it on to the retrieval parameters.
+// species densities (and the flux scaling), for every live point of a batch.
+// The host carries it on to the retrieval parameters.
 //
 // Reverse order of the forward chain:
 //   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
@@ -10,15 +10,21 @@
 //   short characteristics, Planck function, table interpolation (emissionAdjointKernel)
 //   sums over wavenumbers                                       (adjointReduceKernel)
 //
+// Intensities: the forward batch kernel stores both angles' intensity at
+// every level ([point][level][angle][wavenumber], FP64) for as many points at
+// a time as the intensity budget holds, and emissionAdjointKernel<NS, true>
+// reads them back.  When one point's intensities exceed the budget,
+// emissionAdjointKernel<NS, false> recomputes each work item's forward sweep
+// into a per-workgroup scratch first.
+//
 // emissionAdjointKernel: one wavefront per work item (64 wavenumbers of one
-// point).  A forward sweep recomputes the levels and keeps both angles'
-// intensities I_l in a per-workgroup scratch ([levels][2][64] FP64, reused by
-// every item the workgroup takes); the backward sweep recomputes each
-// level's opacity from the tables (register-cached stencils, reloaded where
-// the rows differ from the level above) and runs the recurrence backwards.
-// Per level, the wavefront sums its lanes' contributions and lane 0 stores
-// them to partial[point][level][slot][tile]; a second kernel sums the tiles in
-// a fixed order.
+// point).  The backward sweep recomputes each level's opacity from the tables
+// (register-cached stencil rows, reloaded where the level above brought new
+// rows), keeps every species' cross-section of the last two levels in LDS
+// (each lane its own column) and runs the recurrence backwards.  Once a level
+// has all its contributions the wavefront sums them with DPP moves and one
+// lane stores them to partial[point][level][slot][tile]; a second kernel sums
+// the tiles in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,8 +59,12 @@ void deviceFree(void* p) {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr double kLn10 = 2.302585092994045684017991454684364207601101488;
+constexpr int kAdjointWavesPerCu = 16;                 // persistent grid of the adjoint kernel
+constexpr size_t kPartialBudget = size_t(4) << 30;     // partial sums per chunk when recomputing
 constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
+constexpr double kLn10 = 2.302585092994045684017991454684364207601101488;
+constexpr double kInvMu1 = 1.0 / kMu1;
+constexpr double kInvMu2 = 1.0 / kMu2;
 
 // d logL / d bin_j for every (bin, point)
 __global__ __launch_bounds__(kBlock) void gbarKernel(const double* __restrict__ bins, const BinDesc* __restrict__ desc,
@@ -83,46 +93,43 @@ __global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __re
   fbar[size_t(m) * stride + i] = acc;
 }
 
-// Opacity of one level as in the forward kernels (same operations in the same
-// order), plus each species' cross-section sigma_s = 10^x_s and
-// d kappa / dT = ln 10 sum_s n_s sigma_s (x_hi - x_lo) d f_t / dT.  Rows are
-// reloaded where they differ from the level above (or always, with force).
+// Opacity of one level for the backward sweep: per species the bilinear
+// log10 interpolation x_s (opacity_species.cpp:277-345),
+// sigma_s = 10^x_s, kappa = sum_s n_s sigma_s and
+// d kappa / dT = ln 10 sum_s n_s sigma_s (x_hi - x_lo) d f_t / dT.  sigma_s
+// goes to sigma[s * 64] (this lane's LDS column).  All rows that change are
+// loaded before any is used: where `above` (the level processed before this
+// one) brought new rows, or everywhere without it.
 template <int NS>
-__device__ __forceinline__ void opacityAdjoint(const double* __restrict__ tab, size_t stride,
-                                               const LevelSpecies* __restrict__ d,
-                                               const LevelSpecies* __restrict__ above,
-                                               const LevelSpeciesAdjoint* __restrict__ a, bool force,
-                                               double (&v)[NS][4], double (&sigma)[NS], double& kappa,
-                                               double& dkdt) {
+__device__ __forceinline__ void levelOpacity(const double* __restrict__ tab, const LevelSpecies* __restrict__ d,
+                                             const LevelSpeciesAdjoint* __restrict__ a,
+                                             const LevelSpecies* __restrict__ above, double (&v)[NS][4],
+                                             double* __restrict__ sigma, double& kappa, double& dkdt) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    bool reload = force;
-    if (!force)
-      reload = d[s].block[0] != above[s].block[0] || d[s].block[1] != above[s].block[1] ||
-               d[s].block[2] != above[s].block[2] || d[s].block[3] != above[s].block[3];
-    if (reload) {
+  for (int s = 0; s < NS; ++s)
+    if (!above || above[s].reload) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
     }
-  }
   double k = 0.0, dk = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const double lo = fma(v[s][1] - v[s][0], a[s].f_p_lower, v[s][0]);
     const double hi = fma(v[s][3] - v[s][2], a[s].f_p_upper, v[s][2]);
-    const double x = fma(hi - lo, a[s].f_t, lo);
-    const double sg = pow2(x * kLog2Ten);
-    sigma[s] = sg;
-    k = fma(a[s].n, sg, k);
-    dk = fma(a[s].n * sg, (hi - lo) * a[s].dft, dk);
+    const double span = hi - lo;
+    const double sg = pow2Lean(fma(span, a[s].f_t, lo) * kLog2Ten);
+    sigma[s * kWave] = sg;
+    const double t = a[s].n * sg;
+    k += t;
+    dk = fma(t, span * a[s].dft, dk);
   }
   kappa = k;
   dkdt = dk * kLn10;
 }
 
-// B(T) and dB/dT = B X / (T (1 - q)), X = h c nu / (k T), q = e^-X
+// B(T) as planckAt and dB/dT = B X / (T (1 - q)), X = h c nu / (k T), q = e^-X
 __device__ __forceinline__ void planckAdjoint(double b_num, double x_nu, double inv_t, double& b, double& dbdt) {
-  const double q = pow2(x_nu * inv_t);
+  const double q = pow2Lean(x_nu * inv_t);
   const double r = recip(1.0 - q);
   b = b_num * q * r;
   dbdt = b * (-x_nu * inv_t * kLn2) * inv_t * r;
@@ -131,51 +138,65 @@ __device__ __forceinline__ void planckAdjoint(double b_num, double x_nu, double 
 // One angle of one layer backwards.  Forward: a = e^(-tau/mu),
 // q = (a - 1) mu / tau, I_l = a I_{l-1} + (1 + q) B_l + (-a - q) B_{l-1}.
 // dq/dtau = (mu (1 - a) - a tau) / tau^2, from its series in delta = tau/mu
-// below 1e-2 (the closed form cancels there).
-__device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, double i_prev, double b_lo, double b_hi,
-                                             double& ib, double& tbar, double& bbar_hi, double& bbar_lo) {
+// below 1e-2 (the closed form cancels there).  r = 1 / tau, inv_mu = 1 / mu.
+__device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, double inv_mu, double i_prev,
+                                             double b_lo, double b_hi, double& ib, double& tbar, double& bbar_hi,
+                                             double& bbar_lo) {
   const double a = pow2(tau * (-kLog2E / mu));
   const double q = (a - 1.0) * (mu * r);
-  const double delta = tau / mu;
+  const double delta = tau * inv_mu;
   double dq;
   if (delta < 1e-2)
-    dq = fma(delta, fma(delta, fma(delta, fma(delta, 1.0 / 144.0, -1.0 / 30.0), 1.0 / 8.0), -1.0 / 3.0), 0.5) / mu;
+    dq = fma(delta, fma(delta, fma(delta, fma(delta, 1.0 / 144.0, -1.0 / 30.0), 1.0 / 8.0), -1.0 / 3.0), 0.5) *
+         inv_mu;
   else
     dq = (mu * (1.0 - a) - a * tau) * r * r;
-  tbar = fma(ib, (i_prev - b_lo) * (-a / mu) + (b_hi - b_lo) * dq, tbar);
+  tbar = fma(ib, (i_prev - b_lo) * (-a * inv_mu) + (b_hi - b_lo) * dq, tbar);
   bbar_hi = fma(ib, 1.0 + q, bbar_hi);
   bbar_lo = fma(ib, -a - q, bbar_lo);
   ib *= a;
 }
 
-// The wavefront's sums of one level's slots -> partial[(slot) * tiles + tile]
+// The wavefront's sums of one level's slots -> out[slot * tiles]: each
+// species' sigma times d logL / d kappa, the temperature, the altitude and
+// the flux scaling (0 except at level 0); the last lane of each row stores
+// the totals that row holds.
 template <int NS>
 __device__ __forceinline__ void emitLevel(double* __restrict__ out, size_t tiles, int lane, double kbar,
-                                          const double (&sigma)[NS], double tbar, double zbar, double extra) {
+                                          const double* __restrict__ sigma, double tbar, double zbar,
+                                          double scaling) {
+  constexpr int K = NS + 3;
+  constexpr int K4 = (K + 3) / 4 * 4;
+  double x[K4];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double v = waveSum(kbar * sigma[s]);
-    if (lane == 0) out[size_t(s) * tiles] = v;
-  }
-  const double t = waveSum(tbar);
-  const double z = waveSum(zbar);
-  const double e = waveSum(extra);
-  if (lane == 0) {
-    out[size_t(NS) * tiles] = t;
-    out[size_t(NS + 1) * tiles] = z;
-    out[size_t(NS + 2) * tiles] = e;
+  for (int s = 0; s < NS; ++s) x[s] = kbar * sigma[s * kWave];
+  x[NS] = tbar;
+  x[NS + 1] = zbar;
+  x[NS + 2] = scaling;
+#pragma unroll
+  for (int s = K; s < K4; ++s) x[s] = 0.0;
+  waveTotals<K4>(x);
+  if ((lane & 15) == 15) {
+    const int r = rowSlot(lane >> 4);
+#pragma unroll
+    for (int b = 0; b < K4 / 4; ++b)
+      if (4 * b + r < K) out[size_t(4 * b + r) * tiles] = x[b];
   }
 }
 
-template <int NS>
+// kStored: intensity = [batch][levels][2][stride], written by the forward
+// batch kernel.  Otherwise intensity = [grid][levels][2][64], this kernel's
+// scratch for the recomputed forward sweep.
+template <int NS, bool kStored>
 __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ tables, size_t stride, Records rec,
-    int nb_levels, int n, int tiles, int batch, const double* __restrict__ fbar, double* __restrict__ scratch,
+    int nb_levels, int n, int tiles, int batch, const double* __restrict__ fbar, double* __restrict__ intensity,
     double* __restrict__ partial) {
   constexpr int K = NS + 3;
+  __shared__ double sig[2 * NS * kWave];
   const int L = nb_levels;
   const int lane = int(threadIdx.x);
-  double* store = scratch + size_t(blockIdx.x) * size_t(L) * 2 * kWave;
+  double* my_sig = sig + lane;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
@@ -193,80 +214,89 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     const double nu = wavenumber[ic];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
-
-    // forward sweep, keeping I_l (l < L-1)
     double v[NS][4];
-    double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
-    double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
-    double z_prev = levels[0].altitude;
-    double i1 = b_prev, i2 = b_prev;
-    store[lane] = i1;
-    store[kWave + lane] = i2;
-    for (int l = 1; l < L; ++l) {
-      const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
-      const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
-      const double z_next = levels[l].altitude;
-      shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
-      if (l < L - 1) {
-        store[(size_t(l) * 2) * kWave + lane] = i1;
-        store[(size_t(l) * 2 + 1) * kWave + lane] = i2;
+
+    // I_l of angle a at I[l * level_step + a * angle_step]
+    const double* I;
+    size_t level_step, angle_step;
+    if (kStored) {
+      I = intensity + size_t(m) * L * 2 * stride + ic;
+      level_step = 2 * stride;
+      angle_step = stride;
+    } else {
+      double* store = intensity + size_t(blockIdx.x) * size_t(L) * 2 * kWave + lane;
+      double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
+      double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
+      double z_prev = levels[0].altitude;
+      double i1 = b_prev, i2 = b_prev;
+      store[0] = i1;
+      store[kWave] = i2;
+      for (int l = 1; l < L; ++l) {
+        const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
+        const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
+        const double z_next = levels[l].altitude;
+        shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+        store[size_t(l) * 2 * kWave] = i1;
+        store[(size_t(l) * 2 + 1) * kWave] = i2;
+        k_prev = k_next;
+        b_prev = b_next;
+        z_prev = z_next;
       }
-      k_prev = k_next;
-      b_prev = b_next;
-      z_prev = z_next;
+      I = store;
+      level_step = 2 * kWave;
+      angle_step = kWave;
     }
 
-    // F = 2 pi (mu1 w I1 + mu2 w I2) 1e-3 scaling
+    // F = 2 pi (mu1 w I1 + mu2 w I2) 1e-3 scaling at the top level
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double unit = 2.0 * kPi * 1e-3;
-    const double s_bar = fb * unit * (i1 * kMu1 * kW + i2 * kMu2 * kW);
-    double ib1 = fb * unit * bm->scaling * kMu1 * kW;
-    double ib2 = fb * unit * bm->scaling * kMu2 * kW;
+    const double* top = I + size_t(L - 1) * level_step;
+    const double s_bar = fb * unit * (top[0] * (kMu1 * kW) + top[angle_step] * (kMu2 * kW));
+    double ib1 = fb * unit * bm->scaling * (kMu1 * kW);
+    double ib2 = fb * unit * bm->scaling * (kMu2 * kW);
 
-    // backward sweep
-    double sig_hi[NS], sig_lo[NS];
+    // backward sweep; level l's cross-sections in LDS half (l & 1)
     double k_hi, dk_hi, b_hi, db_hi;
-    opacityAdjoint<NS>(tab, stride, species + size_t(L - 1) * NS, nullptr, extra + size_t(L - 1) * NS, true, v,
-                       sig_hi, k_hi, dk_hi);
+    levelOpacity<NS>(tab, species + size_t(L - 1) * NS, extra + size_t(L - 1) * NS, nullptr, v,
+                     my_sig + ((L - 1) & 1) * NS * kWave, k_hi, dk_hi);
     planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
     double z_hi = levels[L - 1].altitude;
     double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
     for (int l = L - 1; l >= 1; --l) {
+      const double* below = I + size_t(l - 1) * level_step;
+      const double i1p = below[0];
+      const double i2p = below[angle_step];
       double k_lo, dk_lo, b_lo, db_lo;
-      opacityAdjoint<NS>(tab, stride, species + size_t(l - 1) * NS, species + size_t(l) * NS,
-                         extra + size_t(l - 1) * NS, false, v, sig_lo, k_lo, dk_lo);
+      levelOpacity<NS>(tab, species + size_t(l - 1) * NS, extra + size_t(l - 1) * NS, species + size_t(l) * NS, v,
+                       my_sig + ((l - 1) & 1) * NS * kWave, k_lo, dk_lo);
       planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
       const double z_lo = levels[l - 1].altitude;
       const double dz = z_hi - z_lo;
-      const double tau = dz * (k_hi + k_lo) * 0.5;
+      const double k_mean = (k_hi + k_lo) * 0.5;
+      const double tau = dz * k_mean;
       double tbar = 0.0, bbar_lo = 0.0;
       if (tau != 0) {
         const double rt = recip(tau);
-        const double i1p = store[(size_t(l - 1) * 2) * kWave + lane];
-        const double i2p = store[(size_t(l - 1) * 2 + 1) * kWave + lane];
-        angleAdjoint(tau, rt, kMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
-        angleAdjoint(tau, rt, kMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu1, kInvMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu2, kInvMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
       }
       const double kbar_lo = tbar * dz * 0.5;
       kbar_hi += kbar_lo;
-      zbar_hi += tbar * (k_hi + k_lo) * 0.5;
-      const double zbar_lo = -tbar * (k_hi + k_lo) * 0.5;
+      zbar_hi = fma(tbar, k_mean, zbar_hi);
       // every contribution to level l is in: emit it
-      emitLevel<NS>(out + size_t(l) * K * tiles, size_t(tiles), lane, kbar_hi, sig_hi,
+      emitLevel<NS>(out + size_t(l) * K * tiles, size_t(tiles), lane, kbar_hi, my_sig + (l & 1) * NS * kWave,
                     fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, 0.0);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) sig_hi[s] = sig_lo[s];
       k_hi = k_lo;
       dk_hi = dk_lo;
       b_hi = b_lo;
       db_hi = db_lo;
       z_hi = z_lo;
       kbar_hi = kbar_lo;
-      zbar_hi = zbar_lo;
+      zbar_hi = -tbar * k_mean;
       bbar_hi = bbar_lo;
     }
     bbar_hi += ib1 + ib2;                                            // I_0 = B_0
-    emitLevel<NS>(out, size_t(tiles), lane, kbar_hi, sig_hi, fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, s_bar);
+    emitLevel<NS>(out, size_t(tiles), lane, kbar_hi, my_sig, fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, s_bar);
   }
 }
 
@@ -284,68 +314,107 @@ __global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __re
   if (threadIdx.x == 0) out[size_t(m) * rows + row] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
+template <int NS>
+void launchAdjoint(bool stored, dim3 grid, hipStream_t s, const double* nu, const double* tables, size_t stride,
+                   Records rec, int L, int n, int tiles, int batch, const double* fbar, double* intensity,
+                   double* partial) {
+  if (stored)
+    hipLaunchKernelGGL((emissionAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n,
+                       tiles, batch, fbar, intensity, partial);
+  else
+    hipLaunchKernelGGL((emissionAdjointKernel<NS, false>), grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n,
+                       tiles, batch, fbar, intensity, partial);
+}
+
+size_t align8(size_t x) { return (x + 7) / 8 * 8; }
+
 }  // namespace
 
 void DeviceEngine::releaseAdjoint() {
-  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_scratch_, (void*)adj_partial_, (void*)adj_out_})
+  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_scratch_, (void*)adj_intensity_,
+                  (void*)adj_partial_, (void*)adj_out_})
     deviceFree(p);
-  adj_fbar_ = adj_gbar_ = adj_scratch_ = adj_partial_ = adj_out_ = nullptr;
-  adj_capacity_ = adj_grid_ = adj_partial_size_ = 0;
+  adj_fbar_ = adj_gbar_ = adj_scratch_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
+  adj_capacity_ = adj_grid_ = adj_partial_size_ = adj_intensity_size_ = 0;
 }
 
-void DeviceEngine::adjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
-                                std::vector<double>& bins) {
+void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
+                                        std::vector<double>& bins) {
   if (transmission_) raise(kInvalidInput, "the adjoint covers the emission model");
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
+  const int NS = int(species_stride_);
+  if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
   check(hipSetDevice(device_), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const int n = int(n_points_), L = int(nb_levels_), B = int(batch), nb = int(n_bins_);
-  const int NS = int(species_stride_), K = NS + 3;
+  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + 3;
   const int tiles = (n + kWave - 1) / kWave;
-  const int grid = int(std::max<size_t>(8, (size_t(nb_cus_) * 8 + 7) / 8 * 8));
-  const size_t partial_size = size_t(B) * L * K * size_t(tiles);
-  if (batch > adj_capacity_ || size_t(grid) != adj_grid_ || partial_size > adj_partial_size_) {
+  const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
+
+  // points per chunk: stored intensities within the budget, else recomputed
+  const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
+  const size_t stored_point = size_t(L) * 2 * stride_ + partial_point + stride_;
+  size_t free_bytes = 0, total_bytes = 0;
+  check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+  const size_t held = (adj_intensity_size_ + adj_partial_size_) * sizeof(double);
+  const size_t budget = std::min(adjoint_budget_, (free_bytes + held) / 10 * 8);
+  size_t chunk = std::min(batch, budget / (stored_point * sizeof(double)));
+  const bool stored = chunk > 0;
+  if (!stored) chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
+  const size_t intensity_size = stored ? chunk * size_t(L) * 2 * stride_ : size_t(grid) * L * 2 * kWave;
+  const size_t partial_size = chunk * partial_point;
+  if (batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ || partial_size > adj_partial_size_ ||
+      intensity_size > adj_intensity_size_) {
     releaseAdjoint();
-    adj_fbar_ = deviceAlloc<double>(batch * stride_, "adjoint flux");
-    adj_gbar_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "adjoint bins");
-    adj_scratch_ = deviceAlloc<double>(size_t(grid) * L * 2 * kWave, "adjoint intensities");
-    adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
-    adj_out_ = deviceAlloc<double>(size_t(B) * L * K, "adjoint level gradients");
-    adj_capacity_ = batch;
+    adj_capacity_ = batch_capacity_;
     adj_grid_ = size_t(grid);
     adj_partial_size_ = partial_size;
+    adj_intensity_size_ = intensity_size;
+    adj_gbar_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint bins");
+    adj_out_ = deviceAlloc<double>(adj_capacity_ * L * K, "adjoint level gradients");
+    adj_fbar_ = deviceAlloc<double>(adj_capacity_ * stride_, "adjoint flux");
+    adj_intensity_ = deviceAlloc<double>(intensity_size, "adjoint intensities");
+    adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
   }
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
-  if (nb > 0) {
-    hipLaunchKernelGGL(gbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(B)), dim3(kBlock), 0, s,
-                       batch_bins_, bins_desc_, nb, rec, adj_gbar_);
-    check(hipGetLastError(), "adjoint bin kernel");
-  }
-  hipLaunchKernelGGL(fluxAdjointKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(B)), dim3(kBlock), 0, s,
-                     adj_ptr_, adj_bin_, adj_weight_, adj_gbar_, nb, n, stride_, adj_fbar_);
-  check(hipGetLastError(), "adjoint flux kernel");
-  switch (NS) {
-#define BEAR_ADJ(NSV)                                                                                            \
-  case NSV:                                                                                                      \
-    hipLaunchKernelGGL(emissionAdjointKernel<NSV>, dim3(unsigned(grid)), dim3(kWave), 0, s, wavenumber_, tables_, \
-                       stride_, rec, L, n, tiles, B, adj_fbar_, adj_scratch_, adj_partial_);                     \
+  const BatchLaunch launch = tuned(ModelKind::emission, chunk);
+  if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
+  check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+  for (size_t first = 0; first < batch; first += chunk) {
+    const int cb = int(std::min(chunk, batch - first));
+    const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
+                      rec_shifts_};
+    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, stored ? adj_intensity_ : nullptr);
+    launchBatchRest(ModelKind::emission, size_t(cb), first);
+    double* gbar = adj_gbar_ + first * n_bins_;
+    if (nb > 0) {
+      hipLaunchKernelGGL(gbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock), 0, s,
+                         batch_bins_ + first * n_bins_, bins_desc_, nb, rec, gbar);
+      check(hipGetLastError(), "adjoint bin kernel");
+    }
+    hipLaunchKernelGGL(fluxAdjointKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock), 0,
+                       s, adj_ptr_, adj_bin_, adj_weight_, gbar, nb, n, stride_, adj_fbar_);
+    check(hipGetLastError(), "adjoint flux kernel");
+    switch (NS) {
+#define BEAR_ADJ(NSV)                                                                                              \
+  case NSV:                                                                                                        \
+    launchAdjoint<NSV>(stored, dim3(unsigned(grid)), s, wavenumber_, tables_, stride_, rec, L, n, tiles, cb,       \
+                       adj_fbar_, adj_intensity_, adj_partial_);                                                   \
     break;
-    BEAR_ADJ(4)
-    BEAR_ADJ(8)
-    BEAR_ADJ(12)
-    BEAR_ADJ(16)
+      BEAR_ADJ(4)
+      BEAR_ADJ(8)
+      BEAR_ADJ(12)
+      BEAR_ADJ(16)
 #undef BEAR_ADJ
-    default:
-      raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
+    }
+    check(hipGetLastError(), "adjoint kernel");
+    hipLaunchKernelGGL(adjointReduceKernel, dim3(unsigned(L * K), unsigned(cb)), dim3(kBlock), 0, s, adj_partial_,
+                       L * K, tiles, adj_out_ + first * L * K);
+    check(hipGetLastError(), "adjoint reduction kernel");
   }
-  check(hipGetLastError(), "adjoint kernel");
-  hipLaunchKernelGGL(adjointReduceKernel, dim3(unsigned(L * K), unsigned(B)), dim3(kBlock), 0, s, adj_partial_,
-                     L * K, tiles, adj_out_);
-  check(hipGetLastError(), "adjoint reduction kernel");
-  level_grad.resize(size_t(B) * L * K);
-  bin_grad.resize(size_t(B) * nb);
-  bins.resize(size_t(B) * nb);
+  if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), s), "hipEventRecord");
+  level_grad.resize(batch * L * K);
+  bin_grad.resize(batch * n_bins_);
+  bins.resize(batch * n_bins_);
   check(hipMemcpyAsync(level_grad.data(), adj_out_, level_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
         "read level gradients");
   if (nb > 0) {
@@ -355,6 +424,15 @@ void DeviceEngine::adjointBatch(size_t batch, std::vector<double>& level_grad, s
           "read bins");
   }
   check(hipStreamSynchronize(s), "adjoint");
+  if (profiling_) {
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(event_start_), static_cast<hipEvent_t>(event_stop_)),
+          "hipEventElapsedTime");
+    batch_ms_ += ms;
+  }
+  batch_evaluations_ += batch;
+  last_used_graph_ = false;
+  adj_last_chunk_ = stored ? chunk : 0;
 }
 
 }  // namespace bear_amd

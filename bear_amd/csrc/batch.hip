@@ -157,10 +157,13 @@ struct Opacity32 {
 // Fused opacity + two-angle short characteristics per (wavenumber, point),
 // as the single-point kernel; a work item is blockDim wavenumbers of one
 // point and the flux F = 2 pi sum mu w I 1e-3 scaling goes to flux[point][i].
+// With `intensity`, both angles' intensity at every level also goes to
+// intensity[point][level][angle][i], for the adjoint's backward sweep.
 template <class Op>
 __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
-    size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux) {
+    size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux,
+    double* __restrict__ intensity) {
   using Species = typename Op::Species;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -181,11 +184,20 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
+    double* keep = intensity ? intensity + size_t(m) * nb_levels * 2 * stride + i : nullptr;
+    if (keep) {
+      keep[0] = i1;
+      keep[stride] = i2;
+    }
     for (int l = 1; l < nb_levels; ++l) {
       const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, false));
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+      if (keep) {
+        keep[size_t(l) * 2 * stride] = i1;
+        keep[(size_t(l) * 2 + 1) * stride] = i2;
+      }
       k_prev = k_next;
       b_prev = b_next;
       z_prev = z_next;
@@ -349,23 +361,23 @@ __global__ __launch_bounds__(kBatchBlock) void batchSpectrumKernel(
 template <int NS>
 void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const void* tab64,
                     const void* tab_low, const float* base, size_t stride, Records rec, int L, int n, int tiles,
-                    int batch, double* flux) {
+                    int batch, double* flux, double* intensity) {
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, tab64, base, stride, rec, L, n,
-                         tiles, batch, flux);
+                         tiles, batch, flux, intensity);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux);
+                         rec, L, n, tiles, batch, flux, intensity);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux);
+                         rec, L, n, tiles, batch, flux, intensity);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux);
+                         rec, L, n, tiles, batch, flux, intensity);
       break;
   }
 }
@@ -511,9 +523,12 @@ void DeviceEngine::reserveBatch(size_t batch) {
 }
 
 // ---------------------------------------------------------------- launches
-void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc) {
+void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc, size_t first,
+                                       double* intensity) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
+  const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
+                    rec_shifts_};
+  double* flux = batch_flux_ + first * stride_;
   const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
   const bool wide = precision_ == TablePrecision::fp64;
   if (kind == ModelKind::transmission) {
@@ -526,7 +541,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_T(NS)                                                                                                  \
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
-                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, batch_flux_);     \
+                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, flux);            \
     break;
       BEAR_T(4)
       BEAR_T(8)
@@ -544,7 +559,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_E(NS)                                                                                                   \
   case NS:                                                                                                           \
     launchEmission<NS>(precision_, dim3(unsigned(lc.grid)), dim3(unsigned(lc.block)), s, wavenumber_, tables_,       \
-                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, batch_flux_);                         \
+                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, flux, intensity);                     \
     break;
     BEAR_E(4)
     BEAR_E(8)
@@ -557,18 +572,21 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
   check(hipGetLastError(), "emission batch kernel");
 }
 
-void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch) {
+void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch, size_t first) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
+  const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
+                    rec_shifts_};
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s, batch_flux_,
-                       stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
-                       kind == ModelKind::eclipse ? 1 : 0, batch_bins_, batch_terms_);
+    hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s,
+                       batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
+                       kind == ModelKind::eclipse ? 1 : 0, batch_bins_ + first * n_bins_,
+                       batch_terms_ + first * n_bins_);
     check(hipGetLastError(), "batch bin kernel");
-    hipLaunchKernelGGL(sumBatchKernel, dim3(unsigned(batch)), dim3(1024), 0, s, batch_terms_, int(n_bins_),
-                       batch_ll_);
+    hipLaunchKernelGGL(sumBatchKernel, dim3(unsigned(batch)), dim3(1024), 0, s, batch_terms_ + first * n_bins_,
+                       int(n_bins_), batch_ll_ + first);
     check(hipGetLastError(), "batch likelihood kernel");
-    check(hipMemcpyAsync(batch_ll_host_, batch_ll_, batch * sizeof(double), hipMemcpyDeviceToHost, s),
+    check(hipMemcpyAsync(batch_ll_host_ + first, batch_ll_ + first, batch * sizeof(double), hipMemcpyDeviceToHost,
+                         s),
           "read batch log-likelihoods");
   }
 }
@@ -623,6 +641,20 @@ BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
   return best;
 }
 
+// The launch shape of (kind, table precision, batch size): tuned on first use
+BatchLaunch DeviceEngine::tuned(ModelKind kind, size_t batch) {
+  const TuneKey key{int(kind), int(precision_), batch};
+  for (auto& t : tuned_)
+    if (t.first.kind == key.kind && t.first.precision == key.precision && t.first.batch == key.batch) {
+      last_launch_ = t.second;
+      return t.second;
+    }
+  const BatchLaunch launch = tune(kind, batch);
+  tuned_.push_back({key, launch});
+  last_launch_ = launch;
+  return launch;
+}
+
 void DeviceEngine::releaseGraph() {
   if (graph_exec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec_));
   graph_exec_ = nullptr;
@@ -638,15 +670,7 @@ void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
   auto same = [](const TuneKey& a, const TuneKey& b) {
     return a.kind == b.kind && a.precision == b.precision && a.batch == b.batch;
   };
-  BatchLaunch launch;
-  bool found = false;
-  for (auto& t : tuned_)
-    if (same(t.first, key)) { launch = t.second; found = true; }
-  if (!found) {
-    launch = tune(kind, batch);
-    tuned_.push_back({key, launch});
-  }
-  last_launch_ = launch;
+  const BatchLaunch launch = tuned(kind, batch);
   auto sequence = [&] {
     check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
     launchBatchSpectrum(kind, batch, launch);

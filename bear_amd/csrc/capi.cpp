@@ -201,6 +201,13 @@ int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autot
   });
 }
 
+int bear_retrieval_set_adjoint_memory(bear_retrieval* r, size_t bytes) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    r->impl->engine().setAdjointMemory(bytes);
+  });
+}
+
 int bear_retrieval_compute_likelihood_batch(bear_retrieval* r, const double* parameters, size_t batch,
                                             size_t nb_parameters, double* log_likelihood) {
   return guarded([&] {
@@ -239,6 +246,7 @@ int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out) {
     out->used_graph = e.batchUsedGraph();
     out->evaluations = e.batchEvaluations();
     out->gpu_ms_total = e.batchMilliseconds();
+    out->adjoint_chunk = e.adjointChunk();
   });
 }
 

@@ -149,6 +149,59 @@ __device__ __forceinline__ double waveSum(double v) {
   return v;
 }
 
+// v of another lane through DPP moves of both halves (every row, every bank)
+template <int kCtrl>
+__device__ __forceinline__ double dppMove(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp(int(unsigned(b)), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(int(unsigned(b >> 32)), kCtrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (unsigned long long)unsigned(lo) | ((unsigned long long)unsigned(hi) << 32));
+}
+
+__device__ __forceinline__ double joinHalves(unsigned lo, unsigned hi) {
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+// Sums over the 64 lanes of a wavefront of K values at once (K a multiple of
+// 4), all in VALU registers.  Two halving steps first: v_permlane32_swap
+// exchanges the upper 32 lanes of value 2a with the lower 32 of value 2a+1,
+// so one add leaves value 2a summed over lane pairs (j, j+32) in lanes 0-31
+// and value 2a+1 in lanes 32-63; v_permlane16_swap does the same between
+// rows.  Then each remaining value is summed within its rows of 16 (DPP
+// quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror).  On return,
+// every lane of row r holds in x[b] the total of value 4b + kRowSlot[r].
+template <int K>
+__device__ __forceinline__ void waveTotals(double (&x)[K]) {
+  static_assert(K % 4 == 0, "values in groups of four");
+#pragma unroll
+  for (int a = 0; a < K / 2; ++a) {
+    const unsigned long long p = __builtin_bit_cast(unsigned long long, x[2 * a]);
+    const unsigned long long q = __builtin_bit_cast(unsigned long long, x[2 * a + 1]);
+    const auto lo = __builtin_amdgcn_permlane32_swap(unsigned(p), unsigned(q), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(unsigned(p >> 32), unsigned(q >> 32), false, false);
+    x[a] = joinHalves(lo[0], hi[0]) + joinHalves(lo[1], hi[1]);
+  }
+#pragma unroll
+  for (int b = 0; b < K / 4; ++b) {
+    const unsigned long long p = __builtin_bit_cast(unsigned long long, x[2 * b]);
+    const unsigned long long q = __builtin_bit_cast(unsigned long long, x[2 * b + 1]);
+    const auto lo = __builtin_amdgcn_permlane16_swap(unsigned(p), unsigned(q), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(unsigned(p >> 32), unsigned(q >> 32), false, false);
+    x[b] = joinHalves(lo[0], hi[0]) + joinHalves(lo[1], hi[1]);
+  }
+#pragma unroll
+  for (int b = 0; b < K / 4; ++b) {
+    double v = x[b];
+    v += dppMove<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dppMove<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dppMove<0x141>(v);       // row_half_mirror
+    v += dppMove<0x140>(v);       // row_mirror
+    x[b] = v;
+  }
+}
+// value of row r after waveTotals: 4b + kRowSlot[r]
+__device__ __forceinline__ int rowSlot(int row) { return row == 1 ? 2 : row == 2 ? 1 : row; }
+
 // Bins: one workgroup of kBlockBins threads per bin.  The dot product of the
 // spectrum with the bin's folded weights (strided per thread, then a fixed
 // tree); mode 1 (eclipse) also folds the blackbody star's flux and returns

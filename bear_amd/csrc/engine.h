@@ -39,8 +39,9 @@ struct alignas(64) LevelSpecies {
   uint32_t pad;
 };
 
-// The same stencil as the adjoint reads it: n, the interpolation fractions
-// and d f_t / dT = 1 / (T+ - T-) (0 when the T rows coincide).
+// The same stencil as the adjoint reads it: the effective density n, the
+// interpolation fractions and d f_t / dT = 1 / (T+ - T-) (0 when the T rows
+// coincide or T is clamped to the table).
 struct alignas(64) LevelSpeciesAdjoint {
   double n;
   double f_p_lower, f_p_upper, f_t;
@@ -157,13 +158,21 @@ class DeviceEngine {
   const double* batchLogLikelihood() const { return batch_ll_host_; }
   void copyBatchBins(size_t batch, std::vector<double>& out);          // [batch][bins], internal order
   void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
-  // Adjoint of the last evaluateBatch (emission, FP64 tables): per point
-  // dL/d(level quantities) [batch][levels][speciesStride() + 3] = the
-  // effective density of each stencil slot, the temperature (its direct
-  // effects: Planck function and table interpolation), the altitude, and at
-  // level 0 the flux scaling; also dL/d(bin) and the bins, [batch][bins].
-  void adjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
-                    std::vector<double>& bins);
+  // Value and adjoint of the staged batch (emission, FP64 tables): the
+  // log-likelihoods (batchLogLikelihood()) and per point dL/d(level
+  // quantities) [batch][levels][speciesStride() + 3] = the effective density
+  // of each stencil slot, the temperature (its direct effects: Planck
+  // function and table interpolation), the altitude, and at level 0 the flux
+  // scaling; also dL/d(bin) and the bins, [batch][bins].  The forward pass
+  // stores every level's intensities for the backward pass, in chunks of
+  // points that fit the intensity budget; a point that does not fit alone is
+  // recomputed inside the backward kernel instead.
+  void evaluateAdjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
+                            std::vector<double>& bins);
+  // HBM budget (bytes) of the stored intensities; 0: always recompute
+  void setAdjointMemory(size_t bytes) { adjoint_budget_ = bytes; }
+  size_t adjointMemory() const { return adjoint_budget_; }
+  size_t adjointChunk() const { return adj_last_chunk_; }   // points per chunk of the last call; 0: recomputed
   BatchLaunch batchLaunch() const { return last_launch_; }
   size_t batchEvaluations() const { return batch_evaluations_; }
   double batchMilliseconds() const { return batch_ms_; }
@@ -224,8 +233,12 @@ class DeviceEngine {
 
   // batched evaluation
   unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
-  void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch);
-  void launchBatchRest(ModelKind kind, size_t batch);
+  // points [first, first + batch) of the staged records; intensity: also
+  // store every level's intensities [batch][levels][2][stride] (emission)
+  void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
+                           double* intensity = nullptr);
+  void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);
+  BatchLaunch tuned(ModelKind kind, size_t batch);
   BatchLaunch tune(ModelKind kind, size_t batch);
   void releaseGraph();
   void releaseBatch();
@@ -259,10 +272,12 @@ class DeviceEngine {
   double* adj_weight_ = nullptr;
   double* adj_fbar_ = nullptr;                   // [capacity][stride]
   double* adj_gbar_ = nullptr;                   // [capacity][bins]
-  double* adj_scratch_ = nullptr;                // [grid][levels][2][64]
-  double* adj_partial_ = nullptr;                // [capacity][levels][slots][tiles]
+  double* adj_scratch_ = nullptr;                // recomputing: [grid][levels][2][64]
+  double* adj_intensity_ = nullptr;              // storing: [chunk][levels][2][stride]
+  double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]
-  size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0;
+  size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0, adj_intensity_size_ = 0;
+  size_t adjoint_budget_ = size_t(32) << 30, adj_last_chunk_ = 0;
   void releaseAdjoint();
 };
 

@@ -457,11 +457,10 @@ void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, doub
   if (engine_->tablePrecision() != TablePrecision::fp64)
     raise(kInvalidInput, "the gradient needs the FP64 tables (table precision fp64)");
   const std::vector<char> neglect = stageBatch(params, batch, n);
-  engine_->evaluateBatch(model_->kind(), batch);
+  std::vector<double> level_grad, bin_grad, bins;
+  engine_->evaluateAdjointBatch(batch, level_grad, bin_grad, bins);
   const double* v = engine_->batchLogLikelihood();
   for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
-  std::vector<double> level_grad, bin_grad, bins;
-  engine_->adjointBatch(batch, level_grad, bin_grad, bins);
   std::fill(gradient, gradient + batch * n, 0.0);
   model_->gradientBatch(params, batch, n, level_grad, bin_grad, neglect.data(), gradient);
   if (!global_.error_inflation) return;

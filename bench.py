@@ -126,12 +126,15 @@ def batched_leg(r, points, batch_steps, plain_rate):
     r.set_table_precision("fp64")
     # value and gradient (hand-written adjoint) of the same batch
     r.gradientBatch(pts)
+    r.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(batch_steps):
         ll, grad = r.gradientBatch(pts)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    st = r.batch_stats()
+    r.set_profiling(False)
     rate = batch_steps * len(pts) / elapsed
     # spot check at full size: central differences (step 1e-4 relative) of
     # three components of the first point through the single-point path
@@ -144,7 +147,9 @@ def batched_leg(r, points, batch_steps, plain_rate):
         fd = (r.computeLikelihood(p0 + e) - r.computeLikelihood(p0 - e)) / (2 * h)
         fd_rel[label] = abs(grad[0][k] - fd) / abs(fd)
     out["adjoint"] = {"evals_per_s": round(rate, 1), "ms_per_batch": round(elapsed / batch_steps * 1e3, 3),
+                      "gpu_ms_per_batch": round(st["gpu_ms_total"] / batch_steps, 3),
                       "cost_vs_fp64_forward": round(rate and out["fp64"]["evals_per_s"] / rate, 3),
+                      "points_per_chunk": st["adjoint_chunk"],
                       "table_dtype": "f64", "fd_check_rel_diff": fd_rel}
     log(f"adjoint: {rate:.1f} value+gradient/s, {out['adjoint']['cost_vs_fp64_forward']}x the FP64 forward, "
         f"FD check {fd_rel}")

@@ -219,6 +219,15 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
 int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, size_t batch, size_t nb_parameters,
                                   double* log_likelihood, double* gradient);
 
+/* Device memory (bytes) the gradient may hold for the forward intensities
+ * of every level that its backward pass reads back (8 bytes x 2 angles x
+ * levels x spectral points per live point, plus per-point partial sums).
+ * Batches are processed in chunks of as many points as fit; with 0, or when
+ * a single point does not fit, the backward kernel recomputes the forward
+ * pass instead.  Default 32 GiB, never more than 80 % of the free memory.
+ * No counterpart in the reference. */
+int bear_retrieval_set_adjoint_memory(bear_retrieval* r, size_t bytes);
+
 typedef struct bear_batch_stats {
   int precision;                 /* BEAR_TABLE_* */
   int block;                     /* launch shape of the last batch: threads per workgroup */
@@ -227,6 +236,7 @@ typedef struct bear_batch_stats {
   int used_graph;
   size_t evaluations;            /* live points evaluated in batches */
   double gpu_ms_total;           /* device time of those batches (with profiling on) */
+  size_t adjoint_chunk;          /* last gradient: points per chunk with stored intensities, 0: recomputed */
 } bear_batch_stats;
 
 int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out);

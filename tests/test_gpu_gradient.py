@@ -106,6 +106,44 @@ def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, na
     check(grad[0], ref, f"{name} vs oracle Richardson gradient", np.abs(ref - d2))
 
 
+def stored_bytes_per_point(r):
+    """Device bytes one live point takes when its intensities are stored:
+    2 angles x levels x padded spectral points, the per-tile partial sums of
+    every (level, slot) and the flux adjoint."""
+    st = r.stats()
+    n, L = st["nb_spectral_points"], st["nb_levels"]
+    stride = (n + 63) // 64 * 64
+    slots = (st["nb_species"] + 3) // 4 * 4 + 3
+    tiles = (n + 63) // 64
+    return 8 * (L * 2 * stride + L * slots * tiles + stride)
+
+
+@pytest.mark.parametrize("name", ["tiny", "emission_c1"])
+def test_stored_chunked_and_recomputed_intensities_agree(case_factory, tmp_path_factory, name):
+    """The backward pass over stored intensities (whole batch, and one point
+    per chunk) and over intensities recomputed inside the kernel."""
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
+    r.set_table_precision("fp64")
+    pts = np.array(params)
+    try:
+        ll_s, g_s = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == len(pts)
+        r.set_adjoint_memory(stored_bytes_per_point(r) * 3 // 2)
+        ll_c, g_c = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == 1
+        r.set_adjoint_memory(0)
+        ll_r, g_r = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == 0
+    finally:
+        r.set_adjoint_memory(32 << 30)
+    print(f"[gpu] {name}: chunked max |diff| {np.max(np.abs(g_c - g_s)):.3e}, "
+          f"recomputed max rel diff {np.max(np.abs(g_r - g_s) / np.maximum(np.abs(g_s), 1e-300)):.3e}", flush=True)
+    np.testing.assert_array_equal(ll_c, ll_s)
+    np.testing.assert_array_equal(g_c, g_s)
+    np.testing.assert_array_equal(ll_r, ll_s)
+    np.testing.assert_allclose(g_r, g_s, rtol=1e-12, atol=0)
+
+
 def test_autograd_function_backward(case_factory, tmp_path_factory):
     import torch
     from bear_amd.autograd import log_likelihood

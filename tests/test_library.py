@@ -75,4 +75,5 @@ def test_batch_entry_points_refuse_null_arguments():
     assert L.bear_retrieval_compute_likelihood_batch(None, None, 1, 4, None) == -1
     assert L.bear_retrieval_compute_model_batch(None, None, 1, 4, None, None, None) == -1
     assert L.bear_retrieval_batch_stats(None, None) == -1
+    assert L.bear_retrieval_set_adjoint_memory(None, 0) == -1
     assert b"null" in L.bear_last_error()
