@@ -351,20 +351,32 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
   const int tiles = (n + kWave - 1) / kWave;
   const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
 
-  // points per chunk: stored intensities within the budget, else recomputed
+  // points per chunk: stored intensities within the budget (and 80 % of the
+  // free memory, checked when the buffers grow), else recomputed
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
   const size_t stored_point = size_t(L) * 2 * stride_ + partial_point + stride_;
-  size_t free_bytes = 0, total_bytes = 0;
-  check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
-  const size_t held = (adj_intensity_size_ + adj_partial_size_) * sizeof(double);
-  const size_t budget = std::min(adjoint_budget_, (free_bytes + held) / 10 * 8);
-  size_t chunk = std::min(batch, budget / (stored_point * sizeof(double)));
-  const bool stored = chunk > 0;
-  if (!stored) chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
-  const size_t intensity_size = stored ? chunk * size_t(L) * 2 * stride_ : size_t(grid) * L * 2 * kWave;
-  const size_t partial_size = chunk * partial_point;
-  if (batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ || partial_size > adj_partial_size_ ||
-      intensity_size > adj_intensity_size_) {
+  size_t budget = adjoint_budget_;
+  size_t chunk = 0, intensity_size = 0, partial_size = 0;
+  bool stored = false;
+  auto plan = [&] {
+    chunk = std::min(batch, budget / (stored_point * sizeof(double)));
+    stored = chunk > 0;
+    if (!stored)
+      chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
+    intensity_size = stored ? chunk * size_t(L) * 2 * stride_ : size_t(grid) * L * 2 * kWave;
+    partial_size = chunk * partial_point;
+  };
+  plan();
+  const bool grow = batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ ||
+                    partial_size > adj_partial_size_ || intensity_size > adj_intensity_size_;
+  if (grow) {
+    size_t free_bytes = 0, total_bytes = 0;
+    check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+    const size_t held = (adj_intensity_size_ + adj_partial_size_) * sizeof(double);
+    budget = std::min(adjoint_budget_, (free_bytes + held) / 10 * 8);
+    plan();
+  }
+  if (grow) {
     releaseAdjoint();
     adj_capacity_ = batch_capacity_;
     adj_grid_ = size_t(grid);
