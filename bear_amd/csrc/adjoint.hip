@@ -7,24 +7,29 @@
 //   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
 //   d logL / d F(nu) = sum_j d logL / d bin_j  W_j(nu)          (fluxAdjointKernel, the
 //                      folded operator transposed, per wavenumber)
-//   short characteristics, Planck function, table interpolation (emissionAdjointKernel)
+//   short characteristics and Planck function                   (rtAdjointKernel)
+//   table interpolation                                         (opacityAdjointKernel)
 //   sums over wavenumbers                                       (adjointReduceKernel)
 //
-// Intensities: the forward batch kernel stores both angles' intensity at
-// every level ([point][level][angle][wavenumber], FP64) for as many points at
-// a time as the intensity budget holds, and emissionAdjointKernel<NS, true>
-// reads them back.  When one point's intensities exceed the budget,
-// emissionAdjointKernel<NS, false> recomputes each work item's forward sweep
-// into a per-workgroup scratch first.
+// The forward batch kernel keeps both angles' intensity and the opacity at
+// every level ([point][level][3][wavenumber], FP64) for as many points at a
+// time as the budget holds.  rtAdjointKernel runs the short-characteristics
+// recurrence backwards over them (no tables, few registers, many waves) and
+// overwrites each level's opacity with d logL / d kappa; opacityAdjointKernel
+// then recomputes each level's cross-sections from the tables as the forward
+// kernels do and forms d logL / d n_eff and d logL / dT through the tables.
+// When one point's kept quantities exceed the budget, emissionAdjointKernel
+// does all of it for each work item, recomputing its forward sweep into a
+// per-workgroup scratch first.
 //
-// emissionAdjointKernel: one wavefront per work item (64 wavenumbers of one
-// point).  The backward sweep recomputes each level's opacity from the tables
-// (register-cached stencil rows, reloaded where the level above brought new
-// rows), keeps every species' cross-section of the last two levels in LDS
-// (each lane its own column) and runs the recurrence backwards.  Once a level
-// has all its contributions the wavefront sums them with DPP moves and one
-// lane stores them to partial[point][level][slot][tile]; a second kernel sums
-// the tiles in a fixed order.
+// Each kernel takes one wavefront per work item (64 wavenumbers of one point);
+// once a level has all its contributions the wavefront sums them with
+// permlane swaps and DPP moves and the last lane of each row stores its
+// row's totals to partial[point][level][tile][slot]; a last kernel sums the
+// tiles in a fixed order.  Slots per level: [0, NS) the stencil slots'
+// effective densities, NS temperature through the tables (recomputing: also
+// through the Planck function), NS+1 altitude, NS+2 flux scaling (level 0),
+// NS+3 temperature through the Planck function.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,7 +64,8 @@ void deviceFree(void* p) {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kAdjointWavesPerCu = 16;                 // persistent grid of the adjoint kernel
+constexpr int kAdjointWavesPerCu = 16;                 // persistent grids of the table kernels
+constexpr int kRtWavesPerCu = 16;                      // and of the recurrence kernel
 constexpr size_t kPartialBudget = size_t(4) << 30;     // partial sums per chunk when recomputing
 constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
 constexpr double kLn10 = 2.302585092994045684017991454684364207601101488;
@@ -157,42 +163,45 @@ __device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, do
   ib *= a;
 }
 
-// The wavefront's sums of one level's slots -> out[slot * tiles]: each
-// species' sigma times d logL / d kappa, the temperature, the altitude and
-// the flux scaling (0 except at level 0); the last lane of each row stores
-// the totals that row holds.
+// The wavefront's sums of values x[0, K) (K a multiple of 4) to
+// out[slot0 + k] for k < valid: the last lane of each row stores the totals
+// its row holds.
+template <int K>
+__device__ __forceinline__ void emitSums(double* __restrict__ out, int lane, double (&x)[K], int slot0, int valid) {
+  waveTotals<K>(x);
+  if ((lane & 15) == 15) {
+    const int r = rowSlot(lane >> 4);
+#pragma unroll
+    for (int b = 0; b < K / 4; ++b)
+      if (4 * b + r < valid) out[slot0 + 4 * b + r] = x[b];
+  }
+}
+
+// All slots of one level from the fused kernel: each species' sigma times
+// d logL / d kappa, the temperature (tables and Planck function), the
+// altitude, the flux scaling (0 except at level 0) and 0 in the last slot.
 template <int NS>
-__device__ __forceinline__ void emitLevel(double* __restrict__ out, size_t tiles, int lane, double kbar,
+__device__ __forceinline__ void emitLevel(double* __restrict__ out, int lane, double kbar,
                                           const double* __restrict__ sigma, double tbar, double zbar,
                                           double scaling) {
-  constexpr int K = NS + 3;
-  constexpr int K4 = (K + 3) / 4 * 4;
-  double x[K4];
+  double x[NS + 4];
 #pragma unroll
   for (int s = 0; s < NS; ++s) x[s] = kbar * sigma[s * kWave];
   x[NS] = tbar;
   x[NS + 1] = zbar;
   x[NS + 2] = scaling;
-#pragma unroll
-  for (int s = K; s < K4; ++s) x[s] = 0.0;
-  waveTotals<K4>(x);
-  if ((lane & 15) == 15) {
-    const int r = rowSlot(lane >> 4);
-#pragma unroll
-    for (int b = 0; b < K4 / 4; ++b)
-      if (4 * b + r < K) out[size_t(4 * b + r) * tiles] = x[b];
-  }
+  x[NS + 3] = 0.0;
+  emitSums<NS + 4>(out, lane, x, 0, NS + 4);
 }
 
-// kStored: intensity = [batch][levels][2][stride], written by the forward
-// batch kernel.  Otherwise intensity = [grid][levels][2][64], this kernel's
-// scratch for the recomputed forward sweep.
-template <int NS, bool kStored>
+// Forward sweep into scratch = [grid][levels][2][64], then the backward
+// sweep, per work item.
+template <int NS>
 __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ tables, size_t stride, Records rec,
-    int nb_levels, int n, int tiles, int batch, const double* __restrict__ fbar, double* __restrict__ intensity,
+    int nb_levels, int n, int tiles, int batch, const double* __restrict__ fbar, double* __restrict__ scratch,
     double* __restrict__ partial) {
-  constexpr int K = NS + 3;
+  constexpr int K = NS + 4;
   __shared__ double sig[2 * NS * kWave];
   const int L = nb_levels;
   const int lane = int(threadIdx.x);
@@ -209,22 +218,16 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
     const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
-    double* out = partial + size_t(m) * L * K * size_t(tiles) + tile;
+    double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double* tab = tables + ic;
     const double nu = wavenumber[ic];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
     double v[NS][4];
 
-    // I_l of angle a at I[l * level_step + a * angle_step]
-    const double* I;
-    size_t level_step, angle_step;
-    if (kStored) {
-      I = intensity + size_t(m) * L * 2 * stride + ic;
-      level_step = 2 * stride;
-      angle_step = stride;
-    } else {
-      double* store = intensity + size_t(blockIdx.x) * size_t(L) * 2 * kWave + lane;
+    // forward sweep; I_l of angle a at I[l * 2 * 64 + a * 64]
+    double* store = scratch + size_t(blockIdx.x) * size_t(L) * 2 * kWave + lane;
+    {
       double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
       double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
       double z_prev = levels[0].altitude;
@@ -242,10 +245,9 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
         b_prev = b_next;
         z_prev = z_next;
       }
-      I = store;
-      level_step = 2 * kWave;
-      angle_step = kWave;
     }
+    const double* I = store;
+    constexpr size_t level_step = 2 * kWave, angle_step = kWave;
 
     // F = 2 pi (mu1 w I1 + mu2 w I2) 1e-3 scaling at the top level
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       kbar_hi += kbar_lo;
       zbar_hi = fma(tbar, k_mean, zbar_hi);
       // every contribution to level l is in: emit it
-      emitLevel<NS>(out + size_t(l) * K * tiles, size_t(tiles), lane, kbar_hi, my_sig + (l & 1) * NS * kWave,
+      emitLevel<NS>(out + size_t(l) * tiles * K, lane, kbar_hi, my_sig + (l & 1) * NS * kWave,
                     fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, 0.0);
       k_hi = k_lo;
       dk_hi = dk_lo;
@@ -296,34 +298,195 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       bbar_hi = bbar_lo;
     }
     bbar_hi += ib1 + ib2;                                            // I_0 = B_0
-    emitLevel<NS>(out, size_t(tiles), lane, kbar_hi, my_sig, fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, s_bar);
+    emitLevel<NS>(out, lane, kbar_hi, my_sig, fma(bbar_hi, db_hi, kbar_hi * dk_hi), zbar_hi, s_bar);
   }
 }
 
-// out[point][row] = sum over tiles of partial[point][row][tile], fixed order
-__global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __restrict__ partial, int rows,
+// Backward short characteristics over the kept forward quantities of a
+// chunk, keep = [point][level][3][stride] (I_1, I_2, kappa).  Emits per level
+// the altitude, the flux scaling (level 0) and the temperature through the
+// Planck function, and overwrites each level's kappa with d logL / d kappa
+// for opacityAdjointKernel.  ns: stencil slots per level.
+__global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restrict__ wavenumber, Records rec,
+                                                         int nb_levels, int n, size_t stride, int tiles, int batch,
+                                                         int ns, const double* __restrict__ fbar,
+                                                         double* __restrict__ keep, double* __restrict__ partial) {
+  const int L = nb_levels;
+  const int K = ns + 4;
+  const int lane = int(threadIdx.x);
+  const size_t level_step = 3 * stride;
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    const int i = tile * kWave + lane;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;                                // idle lanes: zero adjoints, no stores
+    const unsigned char* r = rec.at(m);
+    const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+    double* kp = keep + size_t(m) * L * level_step + ic;
+    double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
+    const double nu = wavenumber[ic];
+    const double b_num = kPlanckC1 * nu * nu * nu;
+    const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+    const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
+    const double unit = 2.0 * kPi * 1e-3;
+    const double* top = kp + size_t(L - 1) * level_step;
+    const double s_bar = fb * unit * (top[0] * (kMu1 * kW) + top[stride] * (kMu2 * kW));
+    double ib1 = fb * unit * bm->scaling * (kMu1 * kW);
+    double ib2 = fb * unit * bm->scaling * (kMu2 * kW);
+    double k_hi = top[2 * stride];
+    double b_hi, db_hi;
+    planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
+    double z_hi = levels[L - 1].altitude;
+    double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
+    const double* below = kp + size_t(L - 2) * level_step;         // the level below, read one step ahead
+    double i1n = below[0], i2n = below[stride], kn = below[2 * stride];
+    for (int l = L - 1; l >= 1; --l) {
+      const double i1p = i1n, i2p = i2n, k_lo = kn;
+      if (l >= 2) {
+        below = kp + size_t(l - 2) * level_step;
+        i1n = below[0];
+        i2n = below[stride];
+        kn = below[2 * stride];
+      }
+      double b_lo, db_lo;
+      planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
+      const double z_lo = levels[l - 1].altitude;
+      const double dz = z_hi - z_lo;
+      const double k_mean = (k_hi + k_lo) * 0.5;
+      const double tau = dz * k_mean;
+      double tbar = 0.0, bbar_lo = 0.0;
+      if (tau != 0) {
+        const double rt = recip(tau);
+        angleAdjoint(tau, rt, kMu1, kInvMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu2, kInvMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
+      }
+      const double kbar_lo = tbar * dz * 0.5;
+      kbar_hi += kbar_lo;
+      zbar_hi = fma(tbar, k_mean, zbar_hi);
+      // every contribution to level l is in
+      if (valid) kp[size_t(l) * level_step + 2 * stride] = kbar_hi;
+      double x[4] = {zbar_hi, 0.0, bbar_hi * db_hi, 0.0};
+      emitSums<4>(out + size_t(l) * tiles * K, lane, x, ns + 1, 3);
+      k_hi = k_lo;
+      b_hi = b_lo;
+      db_hi = db_lo;
+      z_hi = z_lo;
+      kbar_hi = kbar_lo;
+      zbar_hi = -tbar * k_mean;
+      bbar_hi = bbar_lo;
+    }
+    bbar_hi += ib1 + ib2;                                            // I_0 = B_0
+    if (valid) kp[2 * stride] = kbar_hi;
+    double x[4] = {zbar_hi, s_bar, bbar_hi * db_hi, 0.0};
+    emitSums<4>(out, lane, x, ns + 1, 3);
+  }
+}
+
+// Per level, from d logL / d kappa (keep[point][level][2][i], written by
+// rtAdjointKernel): the cross-sections recomputed from the tables as the
+// forward kernels do (register-cached rows, reloaded where they change),
+// d logL / d n_eff,s = sum over wavenumbers of kappa-bar sigma_s and the
+// temperature's share through the tables, kappa-bar d kappa / dT.
+template <int NS>
+__global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __restrict__ tables, size_t stride,
+                                                              Records rec, int nb_levels, int n, int tiles,
+                                                              int batch, const double* __restrict__ keep,
+                                                              double* __restrict__ partial) {
+  constexpr int K = NS + 4;
+  const int L = nb_levels;
+  const int lane = int(threadIdx.x);
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    const int i = tile * kWave + lane;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;
+    const unsigned char* r = rec.at(m);
+    const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
+    const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
+    const double* kb = keep + size_t(m) * L * 3 * stride + 2 * stride + ic;
+    double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
+    const double* tab = tables + ic;
+    double v[NS][4];
+    // d logL / d kappa streams from HBM: read two levels ahead
+    double kb1 = kb[0], kb2 = L > 1 ? kb[3 * stride] : 0.0;
+    for (int l = 0; l < L; ++l) {
+      const double kbar = valid ? kb1 : 0.0;
+      const LevelSpecies* d = species + size_t(l) * NS;
+      const LevelSpeciesAdjoint* a = extra + size_t(l) * NS;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (l == 0 || d[s].reload) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
+        }
+      kb1 = kb2;
+      if (l + 2 < L) kb2 = kb[size_t(l + 2) * 3 * stride];
+      double x[K];
+      double dk = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const double lo = fma(v[s][1] - v[s][0], a[s].f_p_lower, v[s][0]);
+        const double hi = fma(v[s][3] - v[s][2], a[s].f_p_upper, v[s][2]);
+        const double span = hi - lo;
+        const double sg = pow2Lean(fma(span, a[s].f_t, lo) * kLog2Ten);
+        x[s] = kbar * sg;
+        dk = fma(a[s].n * sg, span * a[s].dft, dk);
+      }
+      x[NS] = kbar * dk * kLn10;
+#pragma unroll
+      for (int s = NS + 1; s < K; ++s) x[s] = 0.0;
+      emitSums<K>(out + size_t(l) * tiles * K, lane, x, 0, NS + 1);
+    }
+  }
+}
+
+// out[point][level][slot] = sum over tiles of partial[point][level][tile][slot]:
+// per thread a strided run of tiles (each a contiguous row of K slots), then
+// the wavefronts' sums, in a fixed order
+template <int K>
+__global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __restrict__ partial, int nb_levels,
                                                               int tiles, double* __restrict__ out) {
-  __shared__ double part[kBlock / 64];
-  const int row = blockIdx.x, m = blockIdx.y;
-  const double* p = partial + (size_t(m) * rows + row) * size_t(tiles);
-  double acc = 0.0;
-  for (int t = threadIdx.x; t < tiles; t += kBlock) acc += p[t];
-  acc = waveSum(acc);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __shared__ double part[kBlock / 64][K];
+  const int l = blockIdx.x, m = blockIdx.y;
+  const size_t row = size_t(m) * nb_levels + l;
+  const double* p = partial + row * size_t(tiles) * K;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += p[size_t(t) * K + k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double v = waveSum(acc[k]);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][k] = v;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) out[size_t(m) * rows + row] = (part[0] + part[1]) + (part[2] + part[3]);
+  if (threadIdx.x < K) {
+    const int k = int(threadIdx.x);
+    out[row * K + k] = (part[0][k] + part[1][k]) + (part[2][k] + part[3][k]);
+  }
 }
 
 template <int NS>
-void launchAdjoint(bool stored, dim3 grid, hipStream_t s, const double* nu, const double* tables, size_t stride,
-                   Records rec, int L, int n, int tiles, int batch, const double* fbar, double* intensity,
-                   double* partial) {
-  if (stored)
-    hipLaunchKernelGGL((emissionAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n,
-                       tiles, batch, fbar, intensity, partial);
-  else
-    hipLaunchKernelGGL((emissionAdjointKernel<NS, false>), grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n,
-                       tiles, batch, fbar, intensity, partial);
+void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const double* nu, const double* tables,
+                   size_t stride, Records rec, int L, int n, int tiles, int batch, const double* fbar,
+                   double* buffer, double* partial) {
+  if (kept) {
+    hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, fbar,
+                       buffer, partial);
+    hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch,
+                       buffer, partial);
+  } else {
+    hipLaunchKernelGGL(emissionAdjointKernel<NS>, grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n, tiles,
+                       batch, fbar, buffer, partial);
+  }
 }
 
 size_t align8(size_t x) { return (x + 7) / 8 * 8; }
@@ -331,10 +494,9 @@ size_t align8(size_t x) { return (x + 7) / 8 * 8; }
 }  // namespace
 
 void DeviceEngine::releaseAdjoint() {
-  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_scratch_, (void*)adj_intensity_,
-                  (void*)adj_partial_, (void*)adj_out_})
+  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_intensity_, (void*)adj_partial_, (void*)adj_out_})
     deviceFree(p);
-  adj_fbar_ = adj_gbar_ = adj_scratch_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
+  adj_fbar_ = adj_gbar_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
   adj_capacity_ = adj_grid_ = adj_partial_size_ = adj_intensity_size_ = 0;
 }
 
@@ -347,45 +509,44 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
   check(hipSetDevice(device_), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + 3;
+  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + 4;
   const int tiles = (n + kWave - 1) / kWave;
   const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
+  const int rt_grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kRtWavesPerCu)));
 
-  // points per chunk: stored intensities within the budget (and 80 % of the
-  // free memory, checked when the buffers grow), else recomputed
+  // points per chunk: kept forward quantities within the budget (and 80 % of
+  // the free memory, checked when the buffers grow), else recomputed
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
-  const size_t stored_point = size_t(L) * 2 * stride_ + partial_point + stride_;
+  const size_t kept_point = size_t(L) * 3 * stride_ + partial_point + stride_;
   size_t budget = adjoint_budget_;
-  size_t chunk = 0, intensity_size = 0, partial_size = 0;
-  bool stored = false;
+  size_t chunk = 0, buffer_size = 0, partial_size = 0;
+  bool kept = false;
   auto plan = [&] {
-    chunk = std::min(batch, budget / (stored_point * sizeof(double)));
-    stored = chunk > 0;
-    if (!stored)
+    chunk = std::min(batch, budget / (kept_point * sizeof(double)));
+    kept = chunk > 0;
+    if (!kept)
       chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
-    intensity_size = stored ? chunk * size_t(L) * 2 * stride_ : size_t(grid) * L * 2 * kWave;
+    buffer_size = kept ? chunk * size_t(L) * 3 * stride_ : size_t(grid) * L * 2 * kWave;
     partial_size = chunk * partial_point;
   };
   plan();
   const bool grow = batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ ||
-                    partial_size > adj_partial_size_ || intensity_size > adj_intensity_size_;
+                    partial_size > adj_partial_size_ || buffer_size > adj_intensity_size_;
   if (grow) {
     size_t free_bytes = 0, total_bytes = 0;
     check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
     const size_t held = (adj_intensity_size_ + adj_partial_size_) * sizeof(double);
     budget = std::min(adjoint_budget_, (free_bytes + held) / 10 * 8);
     plan();
-  }
-  if (grow) {
     releaseAdjoint();
     adj_capacity_ = batch_capacity_;
     adj_grid_ = size_t(grid);
     adj_partial_size_ = partial_size;
-    adj_intensity_size_ = intensity_size;
+    adj_intensity_size_ = buffer_size;
     adj_gbar_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint bins");
     adj_out_ = deviceAlloc<double>(adj_capacity_ * L * K, "adjoint level gradients");
     adj_fbar_ = deviceAlloc<double>(adj_capacity_ * stride_, "adjoint flux");
-    adj_intensity_ = deviceAlloc<double>(intensity_size, "adjoint intensities");
+    adj_intensity_ = deviceAlloc<double>(buffer_size, "adjoint forward quantities");
     adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
   }
   const BatchLaunch launch = tuned(ModelKind::emission, chunk);
@@ -395,7 +556,7 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
     const int cb = int(std::min(chunk, batch - first));
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                       rec_shifts_};
-    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, stored ? adj_intensity_ : nullptr);
+    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, kept ? adj_intensity_ : nullptr);
     launchBatchRest(ModelKind::emission, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
     if (nb > 0) {
@@ -409,8 +570,8 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
     switch (NS) {
 #define BEAR_ADJ(NSV)                                                                                              \
   case NSV:                                                                                                        \
-    launchAdjoint<NSV>(stored, dim3(unsigned(grid)), s, wavenumber_, tables_, stride_, rec, L, n, tiles, cb,       \
-                       adj_fbar_, adj_intensity_, adj_partial_);                                                   \
+    launchAdjoint<NSV>(kept, dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, wavenumber_, tables_, stride_, rec, \
+                       L, n, tiles, cb, adj_fbar_, adj_intensity_, adj_partial_);                                  \
     break;
       BEAR_ADJ(4)
       BEAR_ADJ(8)
@@ -418,9 +579,20 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
       BEAR_ADJ(16)
 #undef BEAR_ADJ
     }
-    check(hipGetLastError(), "adjoint kernel");
-    hipLaunchKernelGGL(adjointReduceKernel, dim3(unsigned(L * K), unsigned(cb)), dim3(kBlock), 0, s, adj_partial_,
-                       L * K, tiles, adj_out_ + first * L * K);
+    check(hipGetLastError(), "adjoint kernels");
+    double* level_out = adj_out_ + first * L * K;
+    switch (K) {
+#define BEAR_RED(KV)                                                                                               \
+  case KV:                                                                                                         \
+    hipLaunchKernelGGL(adjointReduceKernel<KV>, dim3(unsigned(L), unsigned(cb)), dim3(kBlock), 0, s, adj_partial_, \
+                       L, tiles, level_out);                                                                       \
+    break;
+      BEAR_RED(8)
+      BEAR_RED(12)
+      BEAR_RED(16)
+      BEAR_RED(20)
+#undef BEAR_RED
+    }
     check(hipGetLastError(), "adjoint reduction kernel");
   }
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_stop_), s), "hipEventRecord");
@@ -444,7 +616,7 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
   }
   batch_evaluations_ += batch;
   last_used_graph_ = false;
-  adj_last_chunk_ = stored ? chunk : 0;
+  adj_last_chunk_ = kept ? chunk : 0;
 }
 
 }  // namespace bear_amd

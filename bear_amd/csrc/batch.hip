@@ -157,13 +157,13 @@ struct Opacity32 {
 // Fused opacity + two-angle short characteristics per (wavenumber, point),
 // as the single-point kernel; a work item is blockDim wavenumbers of one
 // point and the flux F = 2 pi sum mu w I 1e-3 scaling goes to flux[point][i].
-// With `intensity`, both angles' intensity at every level also goes to
-// intensity[point][level][angle][i], for the adjoint's backward sweep.
+// With `keep`, both angles' intensity and the opacity at every level also go
+// to keep[point][level][0, 1, 2][i], for the adjoint's backward sweep.
 template <class Op>
 __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
     size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux,
-    double* __restrict__ intensity) {
+    double* __restrict__ keep) {
   using Species = typename Op::Species;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -184,19 +184,22 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
-    double* keep = intensity ? intensity + size_t(m) * nb_levels * 2 * stride + i : nullptr;
-    if (keep) {
-      keep[0] = i1;
-      keep[stride] = i2;
+    double* kept = keep ? keep + size_t(m) * nb_levels * 3 * stride + i : nullptr;
+    if (kept) {
+      kept[0] = i1;
+      kept[stride] = i2;
+      kept[2 * stride] = k_prev;
     }
     for (int l = 1; l < nb_levels; ++l) {
       const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, false));
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
-      if (keep) {
-        keep[size_t(l) * 2 * stride] = i1;
-        keep[(size_t(l) * 2 + 1) * stride] = i2;
+      if (kept) {
+        double* at = kept + size_t(l) * 3 * stride;
+        at[0] = i1;
+        at[stride] = i2;
+        at[2 * stride] = k_next;
       }
       k_prev = k_next;
       b_prev = b_next;
@@ -361,23 +364,23 @@ __global__ __launch_bounds__(kBatchBlock) void batchSpectrumKernel(
 template <int NS>
 void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const void* tab64,
                     const void* tab_low, const float* base, size_t stride, Records rec, int L, int n, int tiles,
-                    int batch, double* flux, double* intensity) {
+                    int batch, double* flux, double* keep) {
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, tab64, base, stride, rec, L, n,
-                         tiles, batch, flux, intensity);
+                         tiles, batch, flux, keep);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, intensity);
+                         rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, intensity);
+                         rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, intensity);
+                         rec, L, n, tiles, batch, flux, keep);
       break;
   }
 }
@@ -524,7 +527,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
 
 // ---------------------------------------------------------------- launches
 void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc, size_t first,
-                                       double* intensity) {
+                                       double* keep) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                     rec_shifts_};
@@ -559,7 +562,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_E(NS)                                                                                                   \
   case NS:                                                                                                           \
     launchEmission<NS>(precision_, dim3(unsigned(lc.grid)), dim3(unsigned(lc.block)), s, wavenumber_, tables_,       \
-                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, flux, intensity);                     \
+                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, flux, keep);                          \
     break;
     BEAR_E(4)
     BEAR_E(8)

@@ -160,16 +160,18 @@ class DeviceEngine {
   void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
   // Value and adjoint of the staged batch (emission, FP64 tables): the
   // log-likelihoods (batchLogLikelihood()) and per point dL/d(level
-  // quantities) [batch][levels][speciesStride() + 3] = the effective density
-  // of each stencil slot, the temperature (its direct effects: Planck
-  // function and table interpolation), the altitude, and at level 0 the flux
-  // scaling; also dL/d(bin) and the bins, [batch][bins].  The forward pass
-  // stores every level's intensities for the backward pass, in chunks of
-  // points that fit the intensity budget; a point that does not fit alone is
-  // recomputed inside the backward kernel instead.
+  // quantities) [batch][levels][speciesStride() + 4]: the effective density
+  // of each stencil slot, the temperature through the table interpolation
+  // (recomputing: and through the Planck function), the altitude, the flux
+  // scaling (level 0 only; 0 elsewhere), the temperature through the Planck
+  // function (0 when recomputing); also dL/d(bin) and the bins,
+  // [batch][bins].  The forward pass keeps every level's intensities and
+  // opacity for the backward pass, in chunks of points that fit the budget;
+  // a point that does not fit alone is recomputed inside the backward kernel
+  // instead.
   void evaluateAdjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
                             std::vector<double>& bins);
-  // HBM budget (bytes) of the stored intensities; 0: always recompute
+  // HBM budget (bytes) of the kept forward quantities; 0: always recompute
   void setAdjointMemory(size_t bytes) { adjoint_budget_ = bytes; }
   size_t adjointMemory() const { return adjoint_budget_; }
   size_t adjointChunk() const { return adj_last_chunk_; }   // points per chunk of the last call; 0: recomputed
@@ -233,10 +235,10 @@ class DeviceEngine {
 
   // batched evaluation
   unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
-  // points [first, first + batch) of the staged records; intensity: also
-  // store every level's intensities [batch][levels][2][stride] (emission)
+  // points [first, first + batch) of the staged records; keep: also store
+  // every level's intensities and opacity [batch][levels][3][stride] (emission)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
-                           double* intensity = nullptr);
+                           double* keep = nullptr);
   void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);
   BatchLaunch tuned(ModelKind kind, size_t batch);
   BatchLaunch tune(ModelKind kind, size_t batch);
@@ -272,8 +274,7 @@ class DeviceEngine {
   double* adj_weight_ = nullptr;
   double* adj_fbar_ = nullptr;                   // [capacity][stride]
   double* adj_gbar_ = nullptr;                   // [capacity][bins]
-  double* adj_scratch_ = nullptr;                // recomputing: [grid][levels][2][64]
-  double* adj_intensity_ = nullptr;              // storing: [chunk][levels][2][stride]
+  double* adj_intensity_ = nullptr;              // kept: [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]
   size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0, adj_intensity_size_ = 0;

@@ -198,7 +198,7 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
 void EmissionModel::gradientBatch(const double* params, size_t batch, size_t stride,
                                   const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
                                   const char* neglect, double* gradient) {
-  const size_t L = atmosphere_.nb_levels, S = species_.size(), NS = engine_.speciesStride(), K = NS + 3;
+  const size_t L = atmosphere_.nb_levels, S = species_.size(), NS = engine_.speciesStride(), K = NS + 4;
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
@@ -218,7 +218,7 @@ void EmissionModel::gradientBatch(const double* params, size_t batch, size_t str
       const double* G = level_grad.data() + b * L * K;
       std::vector<double> t_bar(L), z_bar(L), n_bar(L * ids, 0.0);
       for (size_t l = 0; l < L; ++l) {
-        t_bar[l] = G[l * K + NS];
+        t_bar[l] = G[l * K + NS] + G[l * K + NS + 3];
         z_bar[l] = G[l * K + NS + 1];
         for (size_t s = 0; s < S; ++s) {
           const OpacitySpecies& sp = species_[s];

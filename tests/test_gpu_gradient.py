@@ -107,21 +107,21 @@ def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, na
 
 
 def stored_bytes_per_point(r):
-    """Device bytes one live point takes when its intensities are stored:
-    2 angles x levels x padded spectral points, the per-tile partial sums of
-    every (level, slot) and the flux adjoint."""
+    """Device bytes one live point takes when its forward quantities are
+    kept: 2 intensities and the opacity x levels x padded spectral points,
+    the per-tile partial sums of every (level, slot) and the flux adjoint."""
     st = r.stats()
     n, L = st["nb_spectral_points"], st["nb_levels"]
     stride = (n + 63) // 64 * 64
-    slots = (st["nb_species"] + 3) // 4 * 4 + 3
+    slots = (st["nb_species"] + 3) // 4 * 4 + 4
     tiles = (n + 63) // 64
-    return 8 * (L * 2 * stride + L * slots * tiles + stride)
+    return 8 * (L * 3 * stride + L * slots * tiles + stride)
 
 
 @pytest.mark.parametrize("name", ["tiny", "emission_c1"])
-def test_stored_chunked_and_recomputed_intensities_agree(case_factory, tmp_path_factory, name):
-    """The backward pass over stored intensities (whole batch, and one point
-    per chunk) and over intensities recomputed inside the kernel."""
+def test_kept_chunked_and_recomputed_forward_agree(case_factory, tmp_path_factory, name):
+    """The backward pass over kept forward quantities (whole batch, and one
+    point per chunk) and over a forward sweep recomputed inside the kernel."""
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
     pts = np.array(params)
@@ -141,7 +141,10 @@ def test_stored_chunked_and_recomputed_intensities_agree(case_factory, tmp_path_
     np.testing.assert_array_equal(ll_c, ll_s)
     np.testing.assert_array_equal(g_c, g_s)
     np.testing.assert_array_equal(ll_r, ll_s)
-    np.testing.assert_allclose(g_r, g_s, rtol=1e-12, atol=0)
+    # the kept path takes the forward kernel's opacity (2^(log2 n + w.v)) where
+    # the fused kernel recomputes n 2^(x log2 10) (each to 1e-12): the two
+    # gradients agree far inside the 1e-4 bar, not bit for bit
+    np.testing.assert_allclose(g_r, g_s, rtol=1e-7, atol=1e-7 * np.max(np.abs(g_s)))
 
 
 def test_autograd_function_backward(case_factory, tmp_path_factory):
