@@ -53,19 +53,26 @@ constexpr float kLog2TenF = 3.32192809488736234787f;
 constexpr size_t kLdsLimit = 64 * 1024;
 
 // ---------------------------------------------------------------- storage types
+// raw(): the stored value as loaded; value(): its FP32 value
 struct TabF32 {
   using T = float;
-  static __device__ __forceinline__ float load(const T* p) { return *p; }
+  using R = float;
+  static __device__ __forceinline__ R raw(const T* p) { return *p; }
+  static __device__ __forceinline__ float value(R r) { return r; }
   static __device__ __forceinline__ T store(double v) { return float(v); }
 };
 struct TabF16 {
   using T = _Float16;
-  static __device__ __forceinline__ float load(const T* p) { return float(*p); }
+  using R = _Float16;
+  static __device__ __forceinline__ R raw(const T* p) { return *p; }
+  static __device__ __forceinline__ float value(R r) { return float(r); }
   static __device__ __forceinline__ T store(double v) { return _Float16(float(v)); }
 };
 struct TabBF16 {
   using T = uint16_t;
-  static __device__ __forceinline__ float load(const T* p) { return __uint_as_float(uint32_t(*p) << 16); }
+  using R = uint32_t;
+  static __device__ __forceinline__ R raw(const T* p) { return *p; }
+  static __device__ __forceinline__ float value(R r) { return __uint_as_float(r << 16); }
   static __device__ __forceinline__ T store(double v) {   // round to nearest even
     const uint32_t u = __float_as_uint(float(v));
     return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
@@ -124,7 +131,7 @@ struct Opacity32 {
   const typename Tab::T* tab;
   size_t stride;
   float bz[NS];
-  float v[NS][4];
+  typename Tab::R v[NS][4];       // as stored; converted where used (FP16 folds into v_fma_mix_f32)
   // slot0: the first stencil slot this thread handles (its base row)
   __device__ __forceinline__ void init(const void* tables, const float* base, size_t s, int i, int slot0) {
     tab = static_cast<const typename Tab::T*>(tables) + i;
@@ -137,16 +144,16 @@ struct Opacity32 {
     for (int s = 0; s < NS; ++s)
       if (force || d[s].reload) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[s][q] = Tab::load(tab + (size_t(d[s].block[q]) << 6));
+        for (int q = 0; q < 4; ++q) v[s][q] = Tab::raw(tab + (size_t(d[s].block[q]) << 6));
       }
     float k = 0.0f;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       float x = bz[s] + d[s].log2_n;
-      x = fmaf(d[s].w[0], v[s][0], x);
-      x = fmaf(d[s].w[1], v[s][1], x);
-      x = fmaf(d[s].w[2], v[s][2], x);
-      x = fmaf(d[s].w[3], v[s][3], x);
+      x = fmaf(d[s].w[0], Tab::value(v[s][0]), x);
+      x = fmaf(d[s].w[1], Tab::value(v[s][1]), x);
+      x = fmaf(d[s].w[2], Tab::value(v[s][2]), x);
+      x = fmaf(d[s].w[3], Tab::value(v[s][3]), x);
       k += __builtin_amdgcn_exp2f(x);
     }
     return k;
