@@ -103,20 +103,20 @@ __global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __re
 // log10 interpolation x_s (opacity_species.cpp:277-345),
 // sigma_s = 10^x_s, kappa = sum_s n_s sigma_s and
 // d kappa / dT = ln 10 sum_s n_s sigma_s (x_hi - x_lo) d f_t / dT.  sigma_s
-// goes to sigma[s * 64] (this lane's LDS column).  All rows that change are
-// loaded before any is used: where `above` (the level processed before this
-// one) brought new rows, or everywhere without it.
+// goes to sigma[s * 64] (this lane's LDS column).  Every slot's rows are
+// reloaded when `reload` (the level processed before this one, the one
+// above, brought new rows; or the first level of the sweep).
 template <int NS>
 __device__ __forceinline__ void levelOpacity(const double* __restrict__ tab, const LevelSpecies* __restrict__ d,
-                                             const LevelSpeciesAdjoint* __restrict__ a,
-                                             const LevelSpecies* __restrict__ above, double (&v)[NS][4],
-                                             double* __restrict__ sigma, double& kappa, double& dkdt) {
+                                             const LevelSpeciesAdjoint* __restrict__ a, bool reload,
+                                             double (&v)[NS][4], double* __restrict__ sigma, double& kappa,
+                                             double& dkdt) {
+  if (reload) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
-    if (!above || above[s].reload) {
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
-    }
+  }
   double k = 0.0, dk = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       store[0] = i1;
       store[kWave] = i2;
       for (int l = 1; l < L; ++l) {
-        const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
+        const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l].reload != 0, v);
         const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
         const double z_next = levels[l].altitude;
         shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 
     // backward sweep; level l's cross-sections in LDS half (l & 1)
     double k_hi, dk_hi, b_hi, db_hi;
-    levelOpacity<NS>(tab, species + size_t(L - 1) * NS, extra + size_t(L - 1) * NS, nullptr, v,
+    levelOpacity<NS>(tab, species + size_t(L - 1) * NS, extra + size_t(L - 1) * NS, true, v,
                      my_sig + ((L - 1) & 1) * NS * kWave, k_hi, dk_hi);
     planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
     double z_hi = levels[L - 1].altitude;
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       const double i1p = below[0];
       const double i2p = below[angle_step];
       double k_lo, dk_lo, b_lo, db_lo;
-      levelOpacity<NS>(tab, species + size_t(l - 1) * NS, extra + size_t(l - 1) * NS, species + size_t(l) * NS, v,
+      levelOpacity<NS>(tab, species + size_t(l - 1) * NS, extra + size_t(l - 1) * NS, levels[l].reload != 0, v,
                        my_sig + ((l - 1) & 1) * NS * kWave, k_lo, dk_lo);
       planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
       const double z_lo = levels[l - 1].altitude;
@@ -406,6 +406,7 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
     const unsigned char* r = rec.at(m);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
     const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
     const double* kb = keep + size_t(m) * L * 3 * stride + 2 * stride + ic;
@@ -418,12 +419,12 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
       const double kbar = valid ? kb1 : 0.0;
       const LevelSpecies* d = species + size_t(l) * NS;
       const LevelSpeciesAdjoint* a = extra + size_t(l) * NS;
+      if (l == 0 || levels[l].reload != 0) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        if (l == 0 || d[s].reload) {
+        for (int s = 0; s < NS; ++s)
 #pragma unroll
           for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
-        }
+      }
       kb1 = kb2;
       if (l + 2 < L) kb2 = kb[size_t(l + 2) * 3 * stride];
       double x[K];

@@ -116,8 +116,8 @@ struct Opacity64 {
     tab = static_cast<const double*>(tables) + i;
     stride = s;
   }
-  __device__ __forceinline__ double kappa(const LevelSpecies* d, bool force) {
-    return kappaTerms<NS>(tab, stride, d, force, v);
+  __device__ __forceinline__ double kappa(const LevelSpecies* d, bool reload) {
+    return kappaTerms<NS>(tab, stride, d, reload, v);
   }
 };
 
@@ -139,13 +139,13 @@ struct Opacity32 {
 #pragma unroll
     for (int q = 0; q < NS; ++q) bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
   }
-  __device__ __forceinline__ float kappa(const LevelSpecies32* d, bool force) {
+  __device__ __forceinline__ float kappa(const LevelSpecies32* d, bool reload) {
+    if (reload) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-      if (force || d[s].reload) {
+      for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[s][q] = Tab::raw(tab + (size_t(d[s].block[q]) << 6));
-      }
+    }
     float k = 0.0f;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
       kept[2 * stride] = k_prev;
     }
     for (int l = 1; l < nb_levels; ++l) {
-      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, false));
+      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l].reload != 0));
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
       op.init(tables, base, stride, i, 0);
       const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
       for (int l = l0; l < l1; ++l) {
-        const double k = double(op.kappa(species + size_t(l) * Op::kNS, l == l0));
+        const double k = double(op.kappa(species + size_t(l) * Op::kNS, l == l0 || levels[l].reload != 0));
         kx[l * tile + j] = K(k + sigma_r * levels[l].rayleigh_density);
       }
     } else {

@@ -104,19 +104,20 @@ __device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_p
 
 // The opacity terms of NH consecutive stencil slots at one level,
 // sum_s 2^(log2 n_s + sum_k w_sk v_sk) with the slots' four table values v
-// kept in registers between levels and reloaded where `reload` is set
+// kept in registers between levels and all reloaded when `reload` (the
+// level brings new rows for some slot, or the first level)
 // (opacity_species.cpp:277-345, 400-433: the bilinear log10 interpolation in
 // T and log p, times the effective density).  Rows are always valid; a zero
 // density (log2 n = -inf) removes a term exactly.
 template <int NH>
 __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, size_t stride,
-                                             const LevelSpecies* __restrict__ d, bool force, double (&v)[NH][4]) {
+                                             const LevelSpecies* __restrict__ d, bool reload, double (&v)[NH][4]) {
+  if (reload) {
 #pragma unroll
-  for (int s = 0; s < NH; ++s)
-    if (force || d[s].reload) {
+    for (int s = 0; s < NH; ++s)
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
-    }
+  }
   double k = 0.0;
 #pragma unroll
   for (int s = 0; s < NH; ++s) {

@@ -11,12 +11,17 @@
 
 namespace bear_amd {
 
-// Per level: temperature and altitude of the model atmosphere.
+// Per level: temperature and altitude of the model atmosphere, and whether
+// any stencil slot's table rows differ from the previous level's (always at
+// level 0): the kernels then reload every slot's rows at once, so one scalar
+// decides and the loads issue together.
 struct LevelState {
   double temperature;
   double inv_temperature;
   double altitude;
   double rayleigh_density;   // number density of the Rayleigh scatterer (transmission)
+  uint32_t reload;           // bit s: slot s's rows changed
+  uint32_t pad;
 };
 
 // Per (level, opacity species) for the forward kernels: the table stencil
@@ -29,8 +34,9 @@ struct LevelState {
 // species' term as opacity_species.cpp:409-412 skips it.  block[k] locates
 // stencil row k in the tables: row * stride / 64 (the stride is a multiple of
 // 64), so an address is one shift.  reload != 0 when the stencil rows differ
-// from the previous level's (always at level 0): the kernels keep the four
-// row values in registers otherwise.  64 bytes, aligned.
+// from the previous level's (always at level 0); the kernels keep the four
+// row values in registers until LevelState::reload says a level brings new
+// rows.  64 bytes, aligned.
 struct alignas(64) LevelSpecies {
   double w[4];
   double log2_n;

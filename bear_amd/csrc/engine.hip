@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, false, v);
+    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l].reload != 0, v);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void transmissionKernel(
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
     for (int l = l0; l < l1; ++l) {
-      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, l == l0, v);
+      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, l == l0 || levels[l].reload != 0, v);
       K[l * tile + j] = k + sigma_r * levels[l].rayleigh_density;
     }
   } else {
@@ -447,7 +447,7 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmiss
     raise(kInvalidInput, "the transmission model takes at most 16 tabulated species");
   if (transmission && 2 * nb_levels * 8 * sizeof(double) > kTransmissionLds)
     raise(kInvalidInput, "too many levels for the transmission kernel");
-  const size_t lv = nb_levels * sizeof(LevelState);
+  const size_t lv = (nb_levels * sizeof(LevelState) + 63) / 64 * 64;      // stencil records start 64-B aligned
   const size_t sp = nb_levels * species_stride_ * sizeof(LevelSpecies);
   const size_t sh = (n_obs_ + 1) * sizeof(double);
   const size_t pa = transmission ? nb_levels * nb_levels * sizeof(double) : 0;

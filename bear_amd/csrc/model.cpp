@@ -51,7 +51,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
     const double n_r = rayleigh_species_ >= 0 ? atm.nd(l, rayleigh_species_) : 0.0;
-    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r};
+    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r, 0u, 0u};
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
       double n = 0.0;
@@ -103,6 +103,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.reload = changed;
         d.pad[0] = d.pad[1] = 0;
       }
+      lev[l].reload |= changed << s;
     }
   }
   size_t k = 0;                                                   // forward_model.cpp:184-204
