@@ -253,11 +253,23 @@ struct Records {
 };
 
 // item -> (tile, point): XCD x = item mod 8 owns tiles x, x + 8, ...; its
-// items run through every point of one tile before the next tile
+// items take the points in groups of kPointGroup and run through every point
+// of a group for one tile before the next tile, so that the L2 of each XCD
+// holds the group's per-level records (about 50 KB per point) together with
+// the table rows of the tiles in flight, which every point of the group
+// reuses.
+constexpr int kPointGroup = 32;
+
 __device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile) {
-  const size_t k = item / 8;
-  point = int(k % size_t(batch));
-  tile = int(k / size_t(batch)) * 8 + int(item % 8);
+  const size_t k = item / 8;                                   // this XCD's sequence
+  const size_t per_xcd = size_t((tiles + 7) / 8);              // tiles per XCD, padded
+  const size_t span = size_t(kPointGroup) * per_xcd;           // items of a full group
+  const size_t g = k / span;
+  const int first = int(g) * kPointGroup;
+  const int points = min(kPointGroup, batch - first);          // the last group may be smaller
+  const size_t w = k - g * span;
+  point = first + int(w % size_t(points));
+  tile = int(w / size_t(points)) * 8 + int(item % 8);
   return tile < tiles;
 }
 
