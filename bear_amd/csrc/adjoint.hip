@@ -228,14 +228,15 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
     // forward sweep; I_l of angle a at I[l * 2 * 64 + a * 64]
     double* store = scratch + size_t(blockIdx.x) * size_t(L) * 2 * kWave + lane;
     {
-      double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
+      double k_prev = kappaTerms<NS>(tab, stride, species, levels[0], rec.groups, true, v);
       double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
       double z_prev = levels[0].altitude;
       double i1 = b_prev, i2 = b_prev;
       store[0] = i1;
       store[kWave] = i2;
       for (int l = 1; l < L; ++l) {
-        const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l].reload != 0, v);
+        const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], rec.groups,
+                                             levels[l].reload != 0, v);
         const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
         const double z_next = levels[l].altitude;
         shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -556,7 +557,7 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
   for (size_t first = 0; first < batch; first += chunk) {
     const int cb = int(std::min(chunk, batch - first));
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
-                      rec_shifts_};
+                      rec_shifts_, groups_};
     launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, kept ? adj_intensity_ : nullptr);
     launchBatchRest(ModelKind::emission, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;

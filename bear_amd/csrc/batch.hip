@@ -116,8 +116,9 @@ struct Opacity64 {
     tab = static_cast<const double*>(tables) + i;
     stride = s;
   }
-  __device__ __forceinline__ double kappa(const LevelSpecies* d, bool reload) {
-    return kappaTerms<NS>(tab, stride, d, reload, v);
+  __device__ __forceinline__ double kappa(const LevelSpecies* d, const LevelState& lev, StencilGroups groups,
+                                          bool reload) {
+    return kappaTerms<NS>(tab, stride, d, lev, groups, reload, v);
   }
 };
 
@@ -139,7 +140,7 @@ struct Opacity32 {
 #pragma unroll
     for (int q = 0; q < NS; ++q) bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
   }
-  __device__ __forceinline__ float kappa(const LevelSpecies32* d, bool reload) {
+  __device__ __forceinline__ float kappa(const LevelSpecies32* d, const LevelState&, StencilGroups, bool reload) {
     if (reload) {
 #pragma unroll
       for (int s = 0; s < NS; ++s)
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double nu = wavenumber[i];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
-    double k_prev = double(op.kappa(species, true));
+    double k_prev = double(op.kappa(species, levels[0], rec.groups, true));
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
       kept[2 * stride] = k_prev;
     }
     for (int l = 1; l < nb_levels; ++l) {
-      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l].reload != 0));
+      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups, levels[l].reload != 0));
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -286,7 +287,8 @@ __global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
       op.init(tables, base, stride, i, 0);
       const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
       for (int l = l0; l < l1; ++l) {
-        const double k = double(op.kappa(species + size_t(l) * Op::kNS, l == l0 || levels[l].reload != 0));
+        const double k = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
+                                         l == l0 || levels[l].reload != 0));
         kx[l * tile + j] = K(k + sigma_r * levels[l].rayleigh_density);
       }
     } else {
@@ -537,7 +539,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
                                        double* keep) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
-                    rec_shifts_};
+                    rec_shifts_, groups_};
   double* flux = batch_flux_ + first * stride_;
   const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
   const bool wide = precision_ == TablePrecision::fp64;
@@ -585,7 +587,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch, size_t first) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
-                    rec_shifts_};
+                    rec_shifts_, groups_};
   if (n_bins_ > 0) {
     hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s,
                        batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
@@ -734,7 +736,7 @@ void DeviceEngine::copyBatchBins(size_t batch, std::vector<double>& out) {
 
 void DeviceEngine::copyBatchSpectrum(ModelKind kind, size_t batch, double* host) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_};
+  const Records rec{batch_dev_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_, rec_shifts_, groups_};
   const int n = int(n_points_);
   double* out = deviceAlloc<double>(batch * n_points_, "batch spectrum output");
   try {

@@ -109,9 +109,13 @@ __device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_p
 // (opacity_species.cpp:277-345, 400-433: the bilinear log10 interpolation in
 // T and log p, times the effective density).  Rows are always valid; a zero
 // density (log2 n = -inf) removes a term exactly.
+// With grouped stencils (StencilGroups) the weights come from the level's two
+// shared sets, group1 holding these NH slots' group bits; otherwise from each
+// slot's record.
 template <int NH>
 __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, size_t stride,
-                                             const LevelSpecies* __restrict__ d, bool reload, double (&v)[NH][4]) {
+                                             const LevelSpecies* __restrict__ d, const LevelState& lev,
+                                             StencilGroups groups, bool reload, double (&v)[NH][4]) {
   if (reload) {
 #pragma unroll
     for (int s = 0; s < NH; ++s)
@@ -119,13 +123,31 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
       for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
   }
   double k = 0.0;
+  if (groups.grouped) {
+    double a[4], b[4];
 #pragma unroll
-  for (int s = 0; s < NH; ++s) {
-    double x = fma(d[s].w[0], v[s][0], d[s].log2_n);
-    x = fma(d[s].w[1], v[s][1], x);
-    x = fma(d[s].w[2], v[s][2], x);
-    x = fma(d[s].w[3], v[s][3], x);
-    k += pow2Lean(x);
+    for (int q = 0; q < 4; ++q) {
+      a[q] = lev.group_w[0][q];
+      b[q] = lev.group_w[1][q];
+    }
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+      const bool one = (groups.group1 >> s) & 1u;
+      double x = fma(one ? b[0] : a[0], v[s][0], d[s].log2_n);
+      x = fma(one ? b[1] : a[1], v[s][1], x);
+      x = fma(one ? b[2] : a[2], v[s][2], x);
+      x = fma(one ? b[3] : a[3], v[s][3], x);
+      k += pow2Lean(x);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+      double x = fma(d[s].w[0], v[s][0], d[s].log2_n);
+      x = fma(d[s].w[1], v[s][1], x);
+      x = fma(d[s].w[2], v[s][2], x);
+      x = fma(d[s].w[3], v[s][3], x);
+      k += pow2Lean(x);
+    }
   }
   return k;
 }
@@ -135,11 +157,12 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
 constexpr int kChunk = 4;
 
 __device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, size_t stride,
-                                               const LevelSpecies* __restrict__ d, int nb_chunks) {
+                                               const LevelSpecies* __restrict__ d, const LevelState& lev,
+                                               StencilGroups groups, int nb_chunks) {
   double k = 0.0;
   for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
     double v[kChunk][4];
-    k += kappaTerms<kChunk>(tab, stride, d, true, v);
+    k += kappaTerms<kChunk>(tab, stride, d, lev, StencilGroups{groups.grouped, groups.group1 >> (c * kChunk)}, true, v);
   }
   return k;
 }
@@ -249,6 +272,7 @@ __device__ __forceinline__ void binBody(const double* __restrict__ flux, const d
 struct Records {
   const unsigned char* base;
   size_t bytes, levels, species, adjoint, shifts;
+  StencilGroups groups;
   __device__ __forceinline__ const unsigned char* at(int m) const { return base + size_t(m) * bytes; }
 };
 

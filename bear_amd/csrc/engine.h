@@ -22,6 +22,20 @@ struct LevelState {
   double rayleigh_density;   // number density of the Rayleigh scatterer (transmission)
   uint32_t reload;           // bit s: slot s's rows changed
   uint32_t pad;
+  // bilinear weights x log2(10) (as LevelSpecies::w) shared by every slot of
+  // stencil group 0 and 1 (StencilGroups): slots whose species share a table
+  // grid have identical weights at every level
+  double group_w[2][4];
+};
+
+// Which stencil slots take the level's shared weights: grouped != 0 when the
+// tabulated species use at most two distinct (T, p) grids; slot s then uses
+// LevelState::group_w[(group1 >> s) & 1], else its own LevelSpecies::w.
+// Kernels read the shared weights with one scalar load per level instead of
+// one per slot.
+struct StencilGroups {
+  uint32_t grouped = 0;
+  uint32_t group1 = 0;
 };
 
 // Per (level, opacity species) for the forward kernels: the table stencil
@@ -118,6 +132,8 @@ class DeviceEngine {
   void uploadRayleigh(const std::vector<double>& cross_section);   // [N] cm^2, empty: no scatterer
   void uploadObservations(const std::vector<ObservationSetup>& observations);
   void configure(size_t nb_levels, size_t nb_species, bool transmission);
+  void setStencilGroups(StencilGroups g) { groups_ = g; }
+  StencilGroups stencilGroups() const { return groups_; }
 
   // pinned staging of the per-evaluation inputs; filled by the caller
   LevelState* levels() { return levels_host_; }
@@ -207,6 +223,7 @@ class DeviceEngine {
   size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0, n_rows_ = 0;
   size_t nb_levels_ = 0, nb_species_ = 0, species_stride_ = 0;
   bool transmission_ = false;
+  StencilGroups groups_;
   double* wavenumber_ = nullptr;
   double* wavelength_ = nullptr;
   double* tables_ = nullptr;

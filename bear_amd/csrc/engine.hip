@@ -71,7 +71,7 @@ template <int NS>
 __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
     const double* __restrict__ tables, size_t stride,
-    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups groups,
     int nb_levels, int n, double scaling, SpectrumOut out, double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -80,12 +80,13 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   double v[NS][4];
-  double k_prev = kappaTerms<NS>(tab, stride, species, true, v);
+  double k_prev = kappaTerms<NS>(tab, stride, species, levels[0], groups, true, v);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l].reload != 0, v);
+    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], groups,
+                                         levels[l].reload != 0, v);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
 __global__ __launch_bounds__(kBlock) void emissionKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
     const double* __restrict__ tables, size_t stride,
-    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups groups,
     int nb_levels, int nb_chunks, int n, double scaling, SpectrumOut out,
     double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -110,12 +111,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   const int stride_species = nb_chunks * kChunk;
-  double k_prev = chunkedKappa(tab, stride, species, nb_chunks);
+  double k_prev = chunkedKappa(tab, stride, species, levels[0], groups, nb_chunks);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, nb_chunks);
+    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], groups, nb_chunks);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
 template <int NS>
 __global__ __launch_bounds__(kBlock) void transmissionKernel(
     const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
-    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups stencil_groups,
     const double* __restrict__ path, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
     double* __restrict__ depth, double* __restrict__ spectrum) {
   extern __shared__ double lds[];
@@ -160,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void transmissionKernel(
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
     for (int l = l0; l < l1; ++l) {
-      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, l == l0 || levels[l].reload != 0, v);
+      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], stencil_groups,
+                                      l == l0 || levels[l].reload != 0, v);
       K[l * tile + j] = k + sigma_r * levels[l].rayleigh_density;
     }
   } else {
@@ -500,13 +502,14 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
 
 namespace {
 void launchEmission(hipStream_t s, size_t species_stride, int blocks, const double* nu, const double* lam,
-                    const double* tables, size_t stride, const LevelState* lev, const LevelSpecies* spc, int L, int n,
-                    double scaling, SpectrumOut out, double* flux, double* spectrum) {
+                    const double* tables, size_t stride, const LevelState* lev, const LevelSpecies* spc,
+                    StencilGroups groups, int L, int n, double scaling, SpectrumOut out, double* flux,
+                    double* spectrum) {
   switch (species_stride) {
 #define BEAR_CACHED(NS)                                                                                             \
   case NS:                                                                                                          \
     hipLaunchKernelGGL(emissionKernelCached<NS>, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev,   \
-                       spc, L, n, scaling, out, flux, spectrum);                                                    \
+                       spc, groups, L, n, scaling, out, flux, spectrum);                                            \
     break;
     BEAR_CACHED(4)
     BEAR_CACHED(8)
@@ -514,8 +517,8 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
     BEAR_CACHED(16)
 #undef BEAR_CACHED
     default:
-      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, L,
-                         int(species_stride / kChunk), n, scaling, out, flux, spectrum);
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, groups,
+                         L, int(species_stride / kChunk), n, scaling, out, flux, spectrum);
   }
 }
 }  // namespace
@@ -529,7 +532,7 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
   const int n = int(n_points_);
   beginTimed();
   launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0}, flux_, spectrum_dev);
+                 groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0}, flux_, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(0, 0.0, 0.0, inflation, bins_dev);
@@ -546,7 +549,7 @@ void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio,
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
   launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale}, flux_, spectrum_dev);
+                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale}, flux_, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(1, star_temperature, depth_scale, inflation, bins_dev);
@@ -571,7 +574,7 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
 #define BEAR_TRANSMISSION(NS)                                                                                       \
   case NS:                                                                                                          \
     hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kBlock), lds, s, tables_, stride_, rayleigh_,    \
-                       lev, spc, path, L, n, tile, bottom_radius, star_radius, flux_, spectrum_dev);               \
+                       lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, flux_, spectrum_dev);      \
     break;
     BEAR_TRANSMISSION(4)
     BEAR_TRANSMISSION(8)

@@ -138,6 +138,9 @@ class OpacitySpecies {
 
   bool hasTable() const { return !by_temperature.empty(); }
   Stencil bracket(double pressure, double temperature) const;
+  // same temperatures and, per temperature, the same pressures: bracket()
+  // then gives the same fractions for every (p, T)
+  bool sameGrid(const OpacitySpecies& other) const;
 };
 
 // All species' sampled rows, log10 cross-sections in FP64, [rows][N].

@@ -179,6 +179,20 @@ void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const Spectral
 // where the reference skips that interpolation (equal rows), so the device
 // formula  lo + (hi - lo) f_t  with  lo = r0 + (r1 - r0) f_p_lower  and
 // hi = r2 + (r3 - r2) f_p_upper  reproduces every branch.
+bool OpacitySpecies::sameGrid(const OpacitySpecies& o) const {
+  if (by_temperature.size() != o.by_temperature.size()) return false;
+  for (size_t i = 0; i < by_temperature.size(); ++i) {
+    const auto& a = by_temperature[i];
+    const auto& b = o.by_temperature[i];
+    if (a.size() != b.size()) return false;
+    for (size_t k = 0; k < a.size(); ++k)
+      if (a[k].temperature != b[k].temperature || a[k].pressure != b[k].pressure ||
+          a[k].log_pressure != b[k].log_pressure)
+        return false;
+  }
+  return true;
+}
+
 Stencil OpacitySpecies::bracket(double p, double t) const {
   const std::vector<TableRow>* lo = &by_temperature.front();
   const std::vector<TableRow>* hi = lo;

@@ -37,6 +37,20 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
   engine_.uploadTables(store);
   engine_.uploadRayleigh(rayleigh);
   engine_.configure(m.nb_levels, species_.size(), transmission);
+  // stencil groups: the species' table grids, at most two distinct ones
+  StencilGroups groups;
+  groups.grouped = 1;
+  for (size_t s = 0; s < species_.size() && groups.grouped; ++s) {
+    int g = -1;
+    for (int k = 0; k < 2 && g < 0; ++k) {
+      if (group_rep_[k] < 0) { group_rep_[k] = int(s); g = k; }
+      else if (species_[s].sameGrid(species_[size_t(group_rep_[k])])) g = k;
+    }
+    if (g < 0) groups.grouped = 0;
+    else groups.group1 |= uint32_t(g) << s;
+  }
+  if (!groups.grouped) groups.group1 = 0;
+  engine_.setStencilGroups(groups);
   nb_chem_ = atmosphere_.nbChemistryParameters();
   nb_temp_ = atmosphere_.nbTemperatureParameters();
   for (auto& o : observations_) nb_modifier_ += size_t(o.modifier_parameters);
@@ -51,7 +65,16 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
     const double n_r = rayleigh_species_ >= 0 ? atm.nd(l, rayleigh_species_) : 0.0;
-    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r, 0u, 0u};
+    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r, 0u, 0u, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+    for (int g = 0; g < 2; ++g) {
+      if (group_rep_[g] < 0) continue;
+      const Stencil st = species_[size_t(group_rep_[g])].bracket(atm.pressure[l], t);
+      const double ft = st.f_t, fl = st.f_p_lower, fu = st.f_p_upper;
+      lev[l].group_w[g][0] = (1.0 - ft) * (1.0 - fl) * log2_ten;
+      lev[l].group_w[g][1] = (1.0 - ft) * fl * log2_ten;
+      lev[l].group_w[g][2] = ft * (1.0 - fu) * log2_ten;
+      lev[l].group_w[g][3] = ft * fu * log2_ten;
+    }
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
       double n = 0.0;

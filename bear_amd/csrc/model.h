@@ -87,6 +87,7 @@ class TabulatedModel : public ForwardModel {
   std::vector<std::unique_ptr<Atmosphere>> atmosphere_pool_;   // one per staging thread
   std::vector<OpacitySpecies> species_;     // tabulated absorbers
   int rayleigh_species_ = -1;               // species id of the Rayleigh scatterer, -1: none
+  int group_rep_[2] = {-1, -1};             // a species of each stencil group (StencilGroups)
   size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
 };
 
