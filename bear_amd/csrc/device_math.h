@@ -110,8 +110,9 @@ __device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_p
 // T and log p, times the effective density).  Rows are always valid; a zero
 // density (log2 n = -inf) removes a term exactly.
 // With grouped stencils (StencilGroups) the weights come from the level's two
-// shared sets, group1 holding these NH slots' group bits; otherwise from each
-// slot's record.
+// shared sets, group1 holding these NH slots' group bits, and the densities
+// from the level's contiguous copy (NH <= 16, the first NH slots); otherwise
+// both from each slot's record.
 template <int NH>
 __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, size_t stride,
                                              const LevelSpecies* __restrict__ d, const LevelState& lev,
@@ -133,7 +134,7 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
 #pragma unroll
     for (int s = 0; s < NH; ++s) {
       const bool one = (groups.group1 >> s) & 1u;
-      double x = fma(one ? b[0] : a[0], v[s][0], d[s].log2_n);
+      double x = fma(one ? b[0] : a[0], v[s][0], lev.log2_n[s]);
       x = fma(one ? b[1] : a[1], v[s][1], x);
       x = fma(one ? b[2] : a[2], v[s][2], x);
       x = fma(one ? b[3] : a[3], v[s][3], x);
@@ -153,16 +154,17 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
 }
 
 // The same without a register cache, for any number of species: chunks of
-// kChunk slots with every row of a chunk loaded before any is used.
+// kChunk slots with every row of a chunk loaded before any is used; weights
+// and densities from each slot's record.
 constexpr int kChunk = 4;
 
 __device__ __forceinline__ double chunkedKappa(const double* __restrict__ tab, size_t stride,
                                                const LevelSpecies* __restrict__ d, const LevelState& lev,
-                                               StencilGroups groups, int nb_chunks) {
+                                               int nb_chunks) {
   double k = 0.0;
   for (int c = 0; c < nb_chunks; ++c, d += kChunk) {
     double v[kChunk][4];
-    k += kappaTerms<kChunk>(tab, stride, d, lev, StencilGroups{groups.grouped, groups.group1 >> (c * kChunk)}, true, v);
+    k += kappaTerms<kChunk>(tab, stride, d, lev, StencilGroups{0u, 0u}, true, v);      // per-slot records
   }
   return k;
 }

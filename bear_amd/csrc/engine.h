@@ -26,6 +26,8 @@ struct LevelState {
   // stencil group 0 and 1 (StencilGroups): slots whose species share a table
   // grid have identical weights at every level
   double group_w[2][4];
+  // LevelSpecies::log2_n of slots 0-15, contiguous: one scalar load per level
+  double log2_n[16];
 };
 
 // Which stencil slots take the level's shared weights: grouped != 0 when the

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
 __global__ __launch_bounds__(kBlock) void emissionKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
     const double* __restrict__ tables, size_t stride,
-    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups groups,
+    const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
     int nb_levels, int nb_chunks, int n, double scaling, SpectrumOut out,
     double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -111,12 +111,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   const int stride_species = nb_chunks * kChunk;
-  double k_prev = chunkedKappa(tab, stride, species, levels[0], groups, nb_chunks);
+  double k_prev = chunkedKappa(tab, stride, species, levels[0], nb_chunks);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], groups, nb_chunks);
+    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], nb_chunks);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -517,8 +517,8 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
     BEAR_CACHED(16)
 #undef BEAR_CACHED
     default:
-      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, groups,
-                         L, int(species_stride / kChunk), n, scaling, out, flux, spectrum);
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, L,
+                         int(species_stride / kChunk), n, scaling, out, flux, spectrum);
   }
 }
 }  // namespace

@@ -65,7 +65,11 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
     const double n_r = rayleigh_species_ >= 0 ? atm.nd(l, rayleigh_species_) : 0.0;
-    lev[l] = LevelState{t, 1.0 / t, atm.altitude[l], n_r, 0u, 0u, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+    lev[l] = LevelState{};
+    lev[l].temperature = t;
+    lev[l].inv_temperature = 1.0 / t;
+    lev[l].altitude = atm.altitude[l];
+    lev[l].rayleigh_density = n_r;
     for (int g = 0; g < 2; ++g) {
       if (group_rep_[g] < 0) continue;
       const Stencil st = species_[size_t(group_rep_[g])].bracket(atm.pressure[l], t);
@@ -110,6 +114,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.w[2] = ft * (1.0 - fu) * log2_ten;
         d.w[3] = ft * fu * log2_ten;
         d.log2_n = n > 0 ? std::log2(n) : -std::numeric_limits<double>::infinity();
+        if (s < 16) lev[l].log2_n[s] = d.log2_n;
         for (int k = 0; k < 4; ++k) d.block[k] = block[k];
         d.reload = changed;
         d.pad = 0;
