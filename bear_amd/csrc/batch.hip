@@ -140,7 +140,10 @@ struct Opacity32 {
 #pragma unroll
     for (int q = 0; q < NS; ++q) bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
   }
-  __device__ __forceinline__ float kappa(const LevelSpecies32* d, const LevelState&, StencilGroups, bool reload) {
+  // grouped stencils: the level's shared weights and contiguous densities
+  // (FP32 copies), as kappaTerms does for the FP64 tables
+  __device__ __forceinline__ float kappa(const LevelSpecies32* d, const LevelState& lev, StencilGroups groups,
+                                         bool reload) {
     if (reload) {
 #pragma unroll
       for (int s = 0; s < NS; ++s)
@@ -148,14 +151,33 @@ struct Opacity32 {
         for (int q = 0; q < 4; ++q) v[s][q] = Tab::raw(tab + (size_t(d[s].block[q]) << 6));
     }
     float k = 0.0f;
+    if (groups.grouped) {
+      float a[4], b[4];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      float x = bz[s] + d[s].log2_n;
-      x = fmaf(d[s].w[0], Tab::value(v[s][0]), x);
-      x = fmaf(d[s].w[1], Tab::value(v[s][1]), x);
-      x = fmaf(d[s].w[2], Tab::value(v[s][2]), x);
-      x = fmaf(d[s].w[3], Tab::value(v[s][3]), x);
-      k += __builtin_amdgcn_exp2f(x);
+      for (int q = 0; q < 4; ++q) {
+        a[q] = lev.group_w32[0][q];
+        b[q] = lev.group_w32[1][q];
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const bool one = (groups.group1 >> s) & 1u;
+        float x = bz[s] + lev.log2_n32[s];
+        x = fmaf(one ? b[0] : a[0], Tab::value(v[s][0]), x);
+        x = fmaf(one ? b[1] : a[1], Tab::value(v[s][1]), x);
+        x = fmaf(one ? b[2] : a[2], Tab::value(v[s][2]), x);
+        x = fmaf(one ? b[3] : a[3], Tab::value(v[s][3]), x);
+        k += __builtin_amdgcn_exp2f(x);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        float x = bz[s] + d[s].log2_n;
+        x = fmaf(d[s].w[0], Tab::value(v[s][0]), x);
+        x = fmaf(d[s].w[1], Tab::value(v[s][1]), x);
+        x = fmaf(d[s].w[2], Tab::value(v[s][2]), x);
+        x = fmaf(d[s].w[3], Tab::value(v[s][3]), x);
+        k += __builtin_amdgcn_exp2f(x);
+      }
     }
     return k;
   }

@@ -28,6 +28,9 @@ struct LevelState {
   double group_w[2][4];
   // LevelSpecies::log2_n of slots 0-15, contiguous: one scalar load per level
   double log2_n[16];
+  // the same two for the reduced-precision tables (LevelSpecies32 values)
+  float group_w32[2][4];
+  float log2_n32[16];
 };
 
 // Which stencil slots take the level's shared weights: grouped != 0 when the

@@ -78,6 +78,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
       lev[l].group_w[g][1] = (1.0 - ft) * fl * log2_ten;
       lev[l].group_w[g][2] = ft * (1.0 - fu) * log2_ten;
       lev[l].group_w[g][3] = ft * fu * log2_ten;
+      for (int k = 0; k < 4; ++k) lev[l].group_w32[g][k] = float(lev[l].group_w[g][k]);
     }
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
@@ -127,6 +128,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.w[2] = float(ft * (1.0 - fu) * log2_ten);
         d.w[3] = float(ft * fu * log2_ten);
         d.log2_n = n > 0 ? float(std::log2(n)) : -std::numeric_limits<float>::infinity();
+        if (s < 16) lev[l].log2_n32[s] = d.log2_n;
         for (int k = 0; k < 4; ++k) d.block[k] = block[k];
         d.reload = changed;
         d.pad[0] = d.pad[1] = 0;
