@@ -26,8 +26,8 @@
 // once a level has all its contributions the wavefront sums them with
 // permlane swaps and DPP moves and the last lane of each row stores its
 // row's totals to partial[point][level][tile][slot]; a last kernel sums the
-// tiles in a fixed order.  Slots per level: [0, NS) the stencil slots'
-// effective densities, NS temperature through the tables (recomputing: also
+// tiles in a fixed order.  Slots per level: [0, NS) n dlogL/dn for the
+// stencil slots' effective densities n, NS temperature through the tables (recomputing: also
 // through the Planck function), NS+1 altitude, NS+2 flux scaling (level 0),
 // NS+3 temperature through the Planck function.
 #include <hip/hip_runtime.h>
@@ -68,7 +68,6 @@ constexpr int kAdjointWavesPerCu = 16;                 // persistent grids of th
 constexpr int kRtWavesPerCu = 16;                      // and of the recurrence kernel
 constexpr size_t kPartialBudget = size_t(4) << 30;     // partial sums per chunk when recomputing
 constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
-constexpr double kLn10 = 2.302585092994045684017991454684364207601101488;
 constexpr double kInvMu1 = 1.0 / kMu1;
 constexpr double kInvMu2 = 1.0 / kMu2;
 
@@ -99,18 +98,63 @@ __global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __re
   fbar[size_t(m) * stride + i] = acc;
 }
 
-// Opacity of one level for the backward sweep: per species the bilinear
-// log10 interpolation x_s (opacity_species.cpp:277-345),
-// sigma_s = 10^x_s, kappa = sum_s n_s sigma_s and
-// d kappa / dT = ln 10 sum_s n_s sigma_s (x_hi - x_lo) d f_t / dT.  sigma_s
-// goes to sigma[s * 64] (this lane's LDS column).  Every slot's rows are
-// reloaded when `reload` (the level processed before this one, the one
-// above, brought new rows; or the first level of the sweep).
+// Per slot of one level: the opacity term t_s = n_s sigma_s =
+// 2^(log2 n_s + sum_k w_k v_k), exactly as the forward kernels form it, and
+// y_s = d log2 sigma_s / dT = sum_k u_k v_k; f(s, t_s, y_s) for each.  Weights
+// from the level's shared sets with grouped stencils, else from the slots'
+// records.
+template <int NS, class F>
+__device__ __forceinline__ void slotTerms(const LevelSpecies* __restrict__ d, const LevelSpeciesAdjoint* __restrict__ a,
+                                          const LevelState& lev, StencilGroups groups, const double (&v)[NS][4],
+                                          F&& f) {
+  if (groups.grouped) {
+    double wa[4], wb[4], ua[4], ub[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      wa[q] = lev.group_w[0][q];
+      wb[q] = lev.group_w[1][q];
+      ua[q] = lev.group_u[0][q];
+      ub[q] = lev.group_u[1][q];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const bool one = (groups.group1 >> s) & 1u;
+      double x = fma(one ? wb[0] : wa[0], v[s][0], lev.log2_n[s]);
+      x = fma(one ? wb[1] : wa[1], v[s][1], x);
+      x = fma(one ? wb[2] : wa[2], v[s][2], x);
+      x = fma(one ? wb[3] : wa[3], v[s][3], x);
+      double y = (one ? ub[0] : ua[0]) * v[s][0];
+      y = fma(one ? ub[1] : ua[1], v[s][1], y);
+      y = fma(one ? ub[2] : ua[2], v[s][2], y);
+      y = fma(one ? ub[3] : ua[3], v[s][3], y);
+      f(s, pow2Lean(x), y);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double x = fma(d[s].w[0], v[s][0], d[s].log2_n);
+      x = fma(d[s].w[1], v[s][1], x);
+      x = fma(d[s].w[2], v[s][2], x);
+      x = fma(d[s].w[3], v[s][3], x);
+      double y = a[s].u[0] * v[s][0];
+      y = fma(a[s].u[1], v[s][1], y);
+      y = fma(a[s].u[2], v[s][2], y);
+      y = fma(a[s].u[3], v[s][3], y);
+      f(s, pow2Lean(x), y);
+    }
+  }
+}
+
+// Opacity of one level for the backward sweep: kappa = sum_s t_s and
+// d kappa / dT = ln 2 sum_s t_s y_s (slotTerms); t_s goes to t[s * 64] (this
+// lane's LDS column).  Every slot's rows are reloaded when `reload` (the
+// level processed before this one, the one above, brought new rows; or the
+// first level of the sweep).
 template <int NS>
 __device__ __forceinline__ void levelOpacity(const double* __restrict__ tab, const LevelSpecies* __restrict__ d,
-                                             const LevelSpeciesAdjoint* __restrict__ a, bool reload,
-                                             double (&v)[NS][4], double* __restrict__ sigma, double& kappa,
-                                             double& dkdt) {
+                                             const LevelSpeciesAdjoint* __restrict__ a, const LevelState& lev,
+                                             StencilGroups groups, bool reload, double (&v)[NS][4],
+                                             double* __restrict__ t, double& kappa, double& dkdt) {
   if (reload) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -118,19 +162,13 @@ __device__ __forceinline__ void levelOpacity(const double* __restrict__ tab, con
       for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
   }
   double k = 0.0, dk = 0.0;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double lo = fma(v[s][1] - v[s][0], a[s].f_p_lower, v[s][0]);
-    const double hi = fma(v[s][3] - v[s][2], a[s].f_p_upper, v[s][2]);
-    const double span = hi - lo;
-    const double sg = pow2Lean(fma(span, a[s].f_t, lo) * kLog2Ten);
-    sigma[s * kWave] = sg;
-    const double t = a[s].n * sg;
-    k += t;
-    dk = fma(t, span * a[s].dft, dk);
-  }
+  slotTerms<NS>(d, a, lev, groups, v, [&](int s, double ts, double ys) {
+    t[s * kWave] = ts;
+    k += ts;
+    dk = fma(ts, ys, dk);
+  });
   kappa = k;
-  dkdt = dk * kLn10;
+  dkdt = dk * kLn2;
 }
 
 // B(T) as planckAt and dB/dT = B X / (T (1 - q)), X = h c nu / (k T), q = e^-X
@@ -177,16 +215,17 @@ __device__ __forceinline__ void emitSums(double* __restrict__ out, int lane, dou
   }
 }
 
-// All slots of one level from the fused kernel: each species' sigma times
-// d logL / d kappa, the temperature (tables and Planck function), the
-// altitude, the flux scaling (0 except at level 0) and 0 in the last slot.
+// All slots of one level from the fused kernel: each slot's opacity term
+// t_s = n_s sigma_s times d logL / d kappa (so n_s d logL / d n_s), the
+// temperature (tables and Planck function), the altitude, the flux scaling
+// (0 except at level 0) and 0 in the last slot.
 template <int NS>
 __device__ __forceinline__ void emitLevel(double* __restrict__ out, int lane, double kbar,
-                                          const double* __restrict__ sigma, double tbar, double zbar,
+                                          const double* __restrict__ t, double tbar, double zbar,
                                           double scaling) {
   double x[NS + 4];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) x[s] = kbar * sigma[s * kWave];
+  for (int s = 0; s < NS; ++s) x[s] = kbar * t[s * kWave];
   x[NS] = tbar;
   x[NS + 1] = zbar;
   x[NS + 2] = scaling;
@@ -260,7 +299,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 
     // backward sweep; level l's cross-sections in LDS half (l & 1)
     double k_hi, dk_hi, b_hi, db_hi;
-    levelOpacity<NS>(tab, species + size_t(L - 1) * NS, extra + size_t(L - 1) * NS, true, v,
+    levelOpacity<NS>(tab, species + size_t(L - 1) * NS, extra + size_t(L - 1) * NS, levels[L - 1], rec.groups, true, v,
                      my_sig + ((L - 1) & 1) * NS * kWave, k_hi, dk_hi);
     planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
     double z_hi = levels[L - 1].altitude;
@@ -270,7 +309,8 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       const double i1p = below[0];
       const double i2p = below[angle_step];
       double k_lo, dk_lo, b_lo, db_lo;
-      levelOpacity<NS>(tab, species + size_t(l - 1) * NS, extra + size_t(l - 1) * NS, levels[l].reload != 0, v,
+      levelOpacity<NS>(tab, species + size_t(l - 1) * NS, extra + size_t(l - 1) * NS, levels[l - 1], rec.groups,
+                       levels[l].reload != 0, v,
                        my_sig + ((l - 1) & 1) * NS * kWave, k_lo, dk_lo);
       planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
       const double z_lo = levels[l - 1].altitude;
@@ -387,9 +427,9 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
 }
 
 // Per level, from d logL / d kappa (keep[point][level][2][i], written by
-// rtAdjointKernel): the cross-sections recomputed from the tables as the
-// forward kernels do (register-cached rows, reloaded where they change),
-// d logL / d n_eff,s = sum over wavenumbers of kappa-bar sigma_s and the
+// rtAdjointKernel): the opacity terms recomputed from the tables as the
+// forward kernels do (register-cached rows, reloaded when they change),
+// n_s d logL / d n_s = sum over wavenumbers of kappa-bar t_s and the
 // temperature's share through the tables, kappa-bar d kappa / dT.
 template <int NS>
 __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __restrict__ tables, size_t stride,
@@ -430,16 +470,11 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
       if (l + 2 < L) kb2 = kb[size_t(l + 2) * 3 * stride];
       double x[K];
       double dk = 0.0;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const double lo = fma(v[s][1] - v[s][0], a[s].f_p_lower, v[s][0]);
-        const double hi = fma(v[s][3] - v[s][2], a[s].f_p_upper, v[s][2]);
-        const double span = hi - lo;
-        const double sg = pow2Lean(fma(span, a[s].f_t, lo) * kLog2Ten);
-        x[s] = kbar * sg;
-        dk = fma(a[s].n * sg, span * a[s].dft, dk);
-      }
-      x[NS] = kbar * dk * kLn10;
+      slotTerms<NS>(d, a, levels[l], rec.groups, v, [&](int s, double ts, double ys) {
+        x[s] = kbar * ts;
+        dk = fma(ts, ys, dk);
+      });
+      x[NS] = kbar * dk * kLn2;
 #pragma unroll
       for (int s = NS + 1; s < K; ++s) x[s] = 0.0;
       emitSums<K>(out + size_t(l) * tiles * K, lane, x, 0, NS + 1);

@@ -31,6 +31,8 @@ struct LevelState {
   // the same two for the reduced-precision tables (LevelSpecies32 values)
   float group_w32[2][4];
   float log2_n32[16];
+  // LevelSpeciesAdjoint::u shared by the slots of stencil group 0 and 1
+  double group_u[2][4];
 };
 
 // Which stencil slots take the level's shared weights: grouped != 0 when the
@@ -64,14 +66,13 @@ struct alignas(64) LevelSpecies {
   uint32_t pad;
 };
 
-// The same stencil as the adjoint reads it: the effective density n, the
-// interpolation fractions and d f_t / dT = 1 / (T+ - T-) (0 when the T rows
-// coincide or T is clamped to the table).
+// What the adjoint adds to a LevelSpecies: the weights of
+// d log2 sigma / dT = sum_k u_k v_k,
+// u = log2(10) (d f_t / dT) {-(1-f_p-), -f_p-, 1-f_p+, f_p+}, d f_t / dT =
+// 1 / (T+ - T-) (0 when the T rows coincide or T is clamped to the table).
 struct alignas(64) LevelSpeciesAdjoint {
-  double n;
-  double f_p_lower, f_p_upper, f_t;
-  double dft;
-  double pad[3];
+  double u[4];
+  double pad[4];
 };
 
 // Reduced-precision form of LevelSpecies: the bilinear weights
@@ -187,8 +188,8 @@ class DeviceEngine {
   void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
   // Value and adjoint of the staged batch (emission, FP64 tables): the
   // log-likelihoods (batchLogLikelihood()) and per point dL/d(level
-  // quantities) [batch][levels][speciesStride() + 4]: the effective density
-  // of each stencil slot, the temperature through the table interpolation
+  // quantities) [batch][levels][speciesStride() + 4]: n dL/dn for the
+  // effective density n of each stencil slot, the temperature through the table interpolation
   // (recomputing: and through the Planck function), the altitude, the flux
   // scaling (level 0 only; 0 elsewhere), the temperature through the Planck
   // function (0 when recomputing); also dL/d(bin) and the bins,

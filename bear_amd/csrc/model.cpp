@@ -79,6 +79,11 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
       lev[l].group_w[g][2] = ft * (1.0 - fu) * log2_ten;
       lev[l].group_w[g][3] = ft * fu * log2_ten;
       for (int k = 0; k < 4; ++k) lev[l].group_w32[g][k] = float(lev[l].group_w[g][k]);
+      const double du = st.dft * log2_ten;
+      lev[l].group_u[g][0] = -(1.0 - fl) * du;
+      lev[l].group_u[g][1] = -fl * du;
+      lev[l].group_u[g][2] = (1.0 - fu) * du;
+      lev[l].group_u[g][3] = fu * du;
     }
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
@@ -101,11 +106,11 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
       if (adj) {
         LevelSpeciesAdjoint& a = adj[l * stride + s];
         a = LevelSpeciesAdjoint{};
-        a.n = n;
-        a.f_p_lower = st.f_p_lower;
-        a.f_p_upper = st.f_p_upper;
-        a.f_t = st.f_t;
-        a.dft = st.dft;
+        const double du = st.dft * log2_ten;
+        a.u[0] = -(1.0 - st.f_p_lower) * du;
+        a.u[1] = -st.f_p_lower * du;
+        a.u[2] = (1.0 - st.f_p_upper) * du;
+        a.u[3] = st.f_p_upper * du;
       }
       if (spc) {
         LevelSpecies& d = spc[l * stride + s];
@@ -253,7 +258,10 @@ void EmissionModel::gradientBatch(const double* params, size_t batch, size_t str
         z_bar[l] = G[l * K + NS + 1];
         for (size_t s = 0; s < S; ++s) {
           const OpacitySpecies& sp = species_[s];
-          const double gs = G[l * K + s];
+          // the kernels give n dL/dn
+          double n_eff = atm.nd(l, sp.species_id);
+          for (int q : sp.cia_partners) n_eff *= atm.nd(l, q);
+          const double gs = n_eff > 0 ? G[l * K + s] / n_eff : 0.0;
           double others = 1.0;
           for (int q : sp.cia_partners) others *= atm.nd(l, q);
           n_bar[l * ids + sp.species_id] += gs * others;
