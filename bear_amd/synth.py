@@ -431,6 +431,78 @@ def eclipse_small(seed=41):
                 extra={"stellar_model": "blackbody"})
 
 
+def transmission_c3(seed=53, backend="numpy", n_wavenumbers=300_000, nb_levels=200):
+    """Batched transmission configuration: 200 levels 1e2..1e-6 bar (200
+    tangent paths), isothermal, 8 absorbers + H2-H2/H2-He CIA + H2 Rayleigh
+    on 30 T x 9 p tables, 300,000 wavenumbers at 0.025 cm^-1 from 2000 cm^-1,
+    three instruments of uneven size: 20, 147 and 400 bins with Gaussian
+    line-spread functions at R = 70, 130 and 1000."""
+    rng = np.random.default_rng(seed)
+    step = 0.025
+    nu = 2000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
+    temps = np.arange(100.0, 3001.0, 100.0)
+    press = 10.0 ** np.arange(-6.0, 2.5, 1.0)
+    names = ABSORBERS[:8]
+    species = [line_table(rng, s, s, nu, temps, press, n_lines=3000, backend=backend) for s in names]
+    species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+                cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
+    obs = []
+    for k, (lo, hi, nb, r) in enumerate([(1.10, 1.70, 20, 70.0), (1.75, 3.30, 147, 130.0),
+                                         (3.35, 4.85, 400, 1000.0)]):
+        be, d, e, fw = _transit_data(lo, hi, nb, r, rng)
+        obs.append(ObservationSpec(f"instrument{k}", "band-spectroscopy", bin_edges=be, data=d, error=e, fwhm=fw))
+    params = transmission_parameters(rng, 8)
+    return Case("transmission_c3", nu, step, nb_levels, 100.0, 1e-6, ("const",),
+                [("iso", names), ("bg", ["H2He"])], species, ["H2"], obs, params, model="transmission",
+                extra={"variant": "plain"})
+
+
+def emission_c4(seed=29, backend="numpy"):
+    """Adjoint configuration: the emission_c2 model (100 levels, 6 absorbers
+    + 2 CIA = 8 tabulated species, 7 temperature control points, 300 bins) at
+    200,000 wavenumbers; 16 parameters (log g, radius and distance scale, 6
+    log abundances, 7 temperature control points)."""
+    c = emission_c2(seed=seed, backend=backend, n_wavenumbers=200_000)
+    c.name = "emission_c4"
+    return c
+
+
+def eclipse_c5(seed=61, backend="numpy", n_wavenumbers=1_000_000):
+    """Reduced-precision batched secondary-eclipse configuration: 100 levels,
+    poly 6 1, 12 absorbers + H2-H2/H2-He CIA on 30 T x 12 p tables over
+    1,000,000 wavenumbers at 0.025 cm^-1 from 1000 cm^-1, a blackbody star,
+    a 9-bandpass photometric set and a 400-bin band spectrum with a Gaussian
+    line-spread function at R = 400."""
+    rng = np.random.default_rng(seed)
+    step = 0.025
+    nu = 1000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
+    temps = np.arange(100.0, 3001.0, 100.0)
+    press = 10.0 ** np.linspace(-6.0, 2.0, 12)
+    names = ABSORBERS[:12]
+    species = [line_table(rng, s, s, nu, temps, press, n_lines=3000, backend=backend) for s in names]
+    species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+                cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
+    lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
+    obs = []
+    centres = np.geomspace(lam_lo * 1.3, lam_hi * 0.8, 9)
+    for k, c in enumerate(centres):
+        lo, hi = c * 0.95, c * 1.05
+        w, r = bandpass(rng, lo, hi, 50)
+        obs.append(ObservationSpec(f"phot{k}", "photometry", bin_edges=np.array([[lo, hi]]),
+                                   data=np.array([1500.0]), error=np.array([80.0]),
+                                   filter=("Photon" if k % 2 else "Energy", w, r)))
+    edges = np.linspace(1.2, 5.0, 401)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    obs.append(ObservationSpec("spectrum", "band-spectroscopy", bin_edges=be, data=np.full(400, 1500.0),
+                               error=np.full(400, 80.0), fwhm=be.mean(axis=1) / 400.0))
+    rng2 = np.random.default_rng(seed + 1)
+    params = np.concatenate([[rng2.uniform(3.0, 4.0), rng2.uniform(0.08, 0.15), rng2.uniform(4500, 6500)],
+                             10.0 ** rng2.uniform(-8.0, -3.0, 12), poly_temperature_params(rng2, 7, 800.0, 2800.0)])
+    return Case("eclipse_c5", nu, step, 100, 100.0, 1e-6, ("poly", 6, 1),
+                [("iso", names), ("bg", ["H2He"])], species, [], obs, params, model="secondary_eclipse",
+                extra={"stellar_model": "blackbody"})
+
+
 CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2,
          "transmission_plain": lambda: transmission_small(variant="plain"),
          "transmission_variable_gravity": lambda: transmission_small(variant="variable_gravity"),

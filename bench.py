@@ -202,6 +202,107 @@ def cpu_baseline(threads, scratch):
     }
 
 
+CONFIG_WORKLOADS = {
+    # name: (builder, live points, table precisions of the batched legs, what a result is)
+    "transmission_c3": ("transmission_c3", 256, ("fp64", "fp32", "fp16"), "ppm of transit depth"),
+    "emission_c4": ("emission_c4", 64, ("fp64",), "W m^-2 micron^-1"),
+    "eclipse_c5": ("eclipse_c5", 512, ("fp64", "fp16", "bf16"), "ppm of eclipse depth"),
+}
+
+
+def config_workload(name, batch, steps):
+    """The other configurations at their full sizes (one JSON line): the FP64
+    single-point rate, each table precision's batched rate and its error
+    against the FP64 single-point path on the first four points (bins as
+    absolute differences in the result's unit, high-resolution spectrum
+    relative, log-likelihood absolute), and for emission_c4 value + gradient."""
+    import torch
+    from bear_amd import synth
+    builder, default_batch, precisions, unit = CONFIG_WORKLOADS[name]
+    nb = batch or default_batch
+    with tempfile.TemporaryDirectory() as scratch:
+        log(f"generating the {name} tables")
+        t0 = time.time()
+        case = getattr(synth, builder)(backend="torch")
+        log(f"tables ready in {time.time() - t0:.1f} s; building the engine")
+        r, params = open_with_injected_data(case, os.path.join(scratch, "gpu"))
+        points = np.array([params] + [synth.perturbed(params, case, seed=2000 + k) for k in range(nb - 1)])
+        del case
+        st = r.stats()
+        out = {"workload": name, "points": nb, "nb_spectral_points": st["nb_spectral_points"],
+               "levels": st["nb_levels"], "tabulated_species": st["nb_species"], "bins": st["nb_bins"],
+               "result_unit": unit, "data": "synthetic"}
+        for _ in range(3):
+            r.computeLikelihood(params)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            r.computeLikelihood(params)
+        torch.cuda.synchronize()
+        single = 20 / (time.perf_counter() - t0)
+        out["single_fp64_evals_per_s"] = round(single, 2)
+        log(f"single point FP64: {single:.1f} evals/s")
+        ref = [r.computeModel(p, True) for p in points[:4]]
+        ref_ll = [r.computeLikelihood(p) for p in points[:4]]
+        legs = {}
+        for precision in precisions:
+            r.set_table_precision(precision)
+            r.computeLikelihoodBatch(points)                 # tuning + graph capture
+            r.computeLikelihoodBatch(points)
+            r.set_profiling(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ll = r.computeLikelihoodBatch(points)
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
+            bst = r.batch_stats()
+            r.set_profiling(False)
+            spec, bins, _ = r.computeModelBatch(points[:4], True)
+            err_spec = max(float(np.max(np.abs(spec[k] - ref[k].spectrum) / np.abs(ref[k].spectrum)))
+                           for k in range(4))
+            err_bins = max(float(np.max(np.abs(bins[i][k] - ref[k].spectrum_obs[i])))
+                           for k in range(4) for i in range(len(bins)))
+            err_ll = max(abs(float(ll[k]) - ref_ll[k]) for k in range(4))
+            rate = steps * nb / elapsed
+            legs[precision] = {"evals_per_s": round(rate, 1), "gpu_ms_per_batch": round(bst["gpu_ms_total"] / steps, 3),
+                               "speedup_vs_single_fp64": round(rate / single, 2),
+                               "max_spectrum_rel_err": err_spec, "max_bin_abs_err": err_bins,
+                               "max_abs_dloglike": err_ll,
+                               "launch": {k: bst[k] for k in ("block", "tile", "grid", "used_graph")}}
+            log(f"{name} batch {precision}: {rate:.1f} evals/s, bins err {err_bins:.3e} {unit}, |dlogL| {err_ll:.2e}")
+        out["batched"] = legs
+        if name == "emission_c4":
+            r.set_table_precision("fp64")
+            r.gradientBatch(points)
+            r.set_profiling(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ll, grad = r.gradientBatch(points)
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
+            bst = r.batch_stats()
+            r.set_profiling(False)
+            rate = steps * nb / elapsed
+            p0 = points[0]
+            fd_rel = {}
+            for k in range(p0.size):
+                h = 1e-4 * abs(p0[k])
+                e = np.zeros_like(p0)
+                e[k] = h
+                fd = (r.computeLikelihood(p0 + e) - r.computeLikelihood(p0 - e)) / (2 * h)
+                fd_rel[k] = float(abs(grad[0][k] - fd) / max(abs(fd), 1e-300))
+            out["adjoint"] = {"evals_per_s": round(rate, 1), "gpu_ms_per_batch": round(bst["gpu_ms_total"] / steps, 3),
+                              "cost_vs_fp64_forward": round(legs["fp64"]["evals_per_s"] / rate, 3),
+                              "points_per_chunk": bst["adjoint_chunk"],
+                              "fd_rel_diff_per_parameter": fd_rel}
+            log(f"{name} value + gradient: {rate:.1f}/s, {out['adjoint']['cost_vs_fp64_forward']}x the forward; "
+                f"max FD rel diff {max(fd_rel.values()):.2e}")
+        r.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +312,9 @@ def main():
     ap.add_argument("--batch", type=int, default=128, help="live points per batch in the batched leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="emission_c2", choices=["emission_c2"] + sorted(CONFIG_WORKLOADS),
+                    help="emission_c2: the headline line; the others print their own line (batch size "
+                         "from --batch, default theirs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the likelihood path on a single GPU "
@@ -221,6 +325,10 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     from bear_amd import api
+    if args.workload != "emission_c2":
+        explicit = any(a == "--batch" or a.startswith("--batch=") for a in sys.argv[1:])
+        config_workload(args.workload, args.batch if explicit else 0, max(1, args.batch_steps // 4))
+        return
 
     with tempfile.TemporaryDirectory() as scratch:
         log("generating the 400,000-wavenumber tables")
