@@ -49,8 +49,8 @@ void deviceFree(void* p) {
 }
 
 constexpr int kBatchBlock = 256;
+constexpr int kTransmissionBlock = 512;          // 8 wavefronts share one tile's extinction in LDS
 constexpr float kLog2TenF = 3.32192809488736234787f;
-constexpr size_t kLdsLimit = 64 * 1024;
 
 // ---------------------------------------------------------------- storage types
 // raw(): the stored value as loaded; value(): its FP32 value
@@ -279,7 +279,7 @@ __device__ __forceinline__ double transmittance(float tau) {
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R
 //     and depth (R + h)^2 / R_*^2 1e6 in FP64.
 template <class Op>
-__global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
+__global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
     int tiles, int tile, int batch, double* __restrict__ depth) {
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
   K* kx = reinterpret_cast<K*>(lds);
   double* part = lds + (size_t(nb_levels) * tile * sizeof(K) + 7) / 8;
   const int L = nb_levels;
-  const int groups = kBatchBlock / tile;
+  const int groups = kTransmissionBlock / tile;
   const int j = int(threadIdx.x) % tile;
   const int g = int(threadIdx.x) / tile;
   const int chunk = (L + groups - 1) / groups;
@@ -318,21 +318,14 @@ __global__ __launch_bounds__(kBatchBlock) void transmissionBatchKernel(
     }
     __syncthreads();
     const double r0 = bm->bottom_radius;
+    const K* chords;
+    if constexpr (sizeof(K) == 8) chords = chord64;
+    else chords = chord32;
     double h = 0.0;
-    for (int p = int(threadIdx.x); p < tile * (L - 1); p += kBatchBlock) {
-      const int t = p / tile;                 // p % tile == j
-      const size_t row = (size_t(m) * L + t) * L;
-      K tau = K(0);
-      K kq = kx[t * tile + j];
-      for (int q = t; q < L - 1; ++q) {
-        const K kn = kx[(q + 1) * tile + j];
-        K c;
-        if constexpr (sizeof(K) == 8) c = chord64[row + q];
-        else c = chord32[row + q];
-        tau += c * (kq + kn);
-        kq = kn;
-        if (tau > K(10)) break;
-      }
+    for (int p = int(threadIdx.x); p < tile * (L - 1); p += kTransmissionBlock) {
+      // tile is a multiple of 64: one tangent per wavefront (p % tile == j)
+      const int t = __builtin_amdgcn_readfirstlane(p / tile);
+      const K tau = tangentDepth<K>(kx, tile, j, t, L, chords + (size_t(m) * L + t) * L);
       const double zl = levels[t > 0 ? t - 1 : 0].altitude;
       const double zu = levels[t + 1].altitude;
       h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - transmittance(tau));
@@ -421,7 +414,7 @@ void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, 
                         const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
                         const double* chord, const float* chord32, int L, int n, int tiles, int tile, int batch,
                         double* depth) {
-  const dim3 block(kBatchBlock);
+  const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
@@ -446,7 +439,7 @@ size_t align(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 size_t transmissionLds(int L, int tile, bool wide) {
   const size_t k = size_t(L) * tile * (wide ? 8 : 4);
-  return align(k, 8) + size_t(kBatchBlock / tile) * tile * sizeof(double);
+  return align(k, 8) + size_t(kTransmissionBlock / tile) * tile * sizeof(double);
 }
 
 }  // namespace
@@ -629,7 +622,8 @@ void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch, size_t first) {
 // points for each candidate launch shape and keeps the fastest.
 //   emission / eclipse: workgroup of 64, 128 or 256 wavenumbers; persistent
 //     grid of 8, 16 or 32 waves per CU
-//   transmission: tile of 16, 32 or 64 wavenumbers within the LDS budget;
+//   transmission: tile of 64 or 128 wavenumbers (one tangent path per
+//     wavefront) within the device's LDS per workgroup;
 //     grid of 1, 2 or 4 workgroups per CU
 BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
@@ -637,9 +631,9 @@ BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
   std::vector<BatchLaunch> cand;
   if (kind == ModelKind::transmission) {
     const bool wide = precision_ == TablePrecision::fp64;
-    for (int tile : {16, 32, 64}) {
-      if (transmissionLds(int(nb_levels_), tile, wide) > kLdsLimit) continue;
-      for (int per_cu : {1, 2, 4}) cand.push_back(BatchLaunch{kBatchBlock, tile, std::max(8, nb_cus_ * per_cu)});
+    for (int tile : {64, 128}) {
+      if (transmissionLds(int(nb_levels_), tile, wide) > lds_limit_) continue;
+      for (int per_cu : {1, 2, 4}) cand.push_back(BatchLaunch{kTransmissionBlock, tile, std::max(8, nb_cus_ * per_cu)});
     }
     if (cand.empty()) raise(kInvalidInput, "too many levels for the transmission batch kernel");
   } else {

@@ -228,6 +228,39 @@ __device__ __forceinline__ void waveTotals(double (&x)[K]) {
 // value of row r after waveTotals: 4b + kRowSlot[r]
 __device__ __forceinline__ int rowSlot(int row) { return row == 1 ? 2 : row == 2 ? 1 : row; }
 
+// Optical depth along tangent path t for this lane's wavenumber column of
+// K[level][tile] (transmission_spectrum.cpp:112-139): summed outward from t
+// in the reference's order, stopping once tau > 10.  t and chord_row
+// (chord lengths c[t][q], q >= t) must be uniform over the wavefront (tiles
+// a multiple of 64 wavenumbers): the chords arrive with scalar loads, eight
+// per round trip, and lanes whose sum has passed 10 keep their value while
+// the others go on (selects, no branches; the eight extinction values of a
+// round are read before any is used).
+template <class K>
+__device__ __forceinline__ K tangentDepth(const K* __restrict__ kx, int tile, int j, int t, int nb_levels,
+                                          const K* __restrict__ chord_row) {
+  const int last = nb_levels - 1;
+  K tau = K(0);
+  K kq = kx[t * tile + j];
+  bool done = false;
+  for (int q0 = t; q0 < last; q0 += 8) {
+    K c[8], kn[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = q0 + u < last ? chord_row[q0 + u] : K(0);   // 0 past the top
+#pragma unroll
+    for (int u = 0; u < 8; ++u) kn[u] = kx[min(q0 + u + 1, last) * tile + j];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const K next = fma(c[u], kq + kn[u], tau);
+      tau = done ? tau : next;
+      done = done || tau > K(10);
+      kq = kn[u];
+    }
+    if (__all(done)) break;
+  }
+  return tau;
+}
+
 // Bins: one workgroup of kBlockBins threads per bin.  The dot product of the
 // spectrum with the bin's folded weights (strided per thread, then a fixed
 // tree); mode 1 (eclipse) also folds the blackbody star's flux and returns

@@ -274,6 +274,7 @@ class DeviceEngine {
   void releaseGraph();
   void releaseBatch();
   int nb_cus_ = 0;
+  size_t lds_limit_ = 64 * 1024;                 // LDS bytes per workgroup (device attribute)
   TablePrecision precision_ = TablePrecision::fp64;
   void* tables_low_ = nullptr;                   // residuals [rows][stride] (fp32 / fp16 / bf16)
   float* table_base_ = nullptr;                  // [species stride][stride]

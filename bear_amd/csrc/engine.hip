@@ -34,6 +34,13 @@ void check(hipError_t e, const char* what) {
 }
 
 constexpr int kBlock = 256;
+// Single-point transmission: workgroups of 8 wavefronts share one tile's
+// extinction in LDS (K[level][tile] and the partial effective heights
+// [kTransmissionBlock / tile][tile], FP64)
+constexpr int kTransmissionBlock = 512;
+size_t transmissionLds(size_t nb_levels, size_t tile) {
+  return (nb_levels + kTransmissionBlock / tile) * tile * sizeof(double);
+}
 
 // Emission spectrum epilogue: the flux F = 2 pi (mu1 w1 I1 + mu2 w2 I2) 1e-3 s
 // (short_characteristics.cpp:117-122) is kept for the bins; the optional
@@ -128,34 +135,37 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
 }
 
 // Transmission: transit depth per wavenumber (transmission_spectrum.cpp:32-154).
-// A workgroup owns `tile` consecutive wavenumbers and stages their extinction
-// coefficients for every level in LDS (K[level][tile]).
+// A workgroup owns `tile` consecutive wavenumbers (a multiple of 64) and
+// stages their extinction coefficients for every level in LDS (K[level][tile]).
 //  1. extinction: thread (level group g, wavenumber j) computes
 //     kappa_abs + sigma_R n_R for its contiguous range of levels, with the
 //     register-cached stencils;
-//  2. tangent optical depths: the (wavenumber, tangent level) pairs are spread
-//     over the workgroup; each sums chord length x (k_i + k_i+1) outward from
-//     its tangent level, in the reference's order, and stops once tau > 10
-//     (transmission_spectrum.cpp:112-139); exp(-tau) goes to LDS (T[level][tile]);
-//  3. per wavenumber, the effective tangent height (trapezoid in the
-//     reference's order) and the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
+//  2. tangent optical depths: each wavefront takes one tangent level t at a
+//     time for 64 wavenumbers (tangentDepth: chord lengths by scalar loads,
+//     summed outward in the reference's order, stopping once tau > 10) and
+//     adds c_t (1 - e^-tau) to its thread's FP64 partial, where
+//     c_t = (R+z_t)(z_t+1 - z_t-1) is the coefficient of (1 - T_t) in the
+//     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
+//  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R and
+//     the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
 // Chord lengths sqrt((R+z_i)^2 - (R+z_t)^2) differences come from the host.
 template <int NS>
-__global__ __launch_bounds__(kBlock) void transmissionKernel(
+__global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups stencil_groups,
     const double* __restrict__ path, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
     double* __restrict__ depth, double* __restrict__ spectrum) {
   extern __shared__ double lds[];
   double* K = lds;
-  double* T = lds + size_t(nb_levels) * tile;
-  const int groups = kBlock / tile;
+  double* part = lds + size_t(nb_levels) * tile;
+  const int L = nb_levels;
+  const int groups = kTransmissionBlock / tile;
   const int j = threadIdx.x % tile;
   const int g = threadIdx.x / tile;
   const int i = blockIdx.x * tile + j;
-  const int chunk = (nb_levels + groups - 1) / groups;
+  const int chunk = (L + groups - 1) / groups;
   const int l0 = g * chunk;
-  const int l1 = min(nb_levels, l0 + chunk);
+  const int l1 = min(L, l0 + chunk);
   if (i < n) {
     const double* tab = tables + i;
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
@@ -169,32 +179,22 @@ __global__ __launch_bounds__(kBlock) void transmissionKernel(
     for (int l = l0; l < l1; ++l) K[l * tile + j] = 0.0;
   }
   __syncthreads();
-  const int nb_pairs = tile * (nb_levels - 1);
-  for (int p = threadIdx.x; p < nb_pairs; p += kBlock) {
-    const int jj = p % tile;
-    const int t = p / tile;
-    const double* chord = path + size_t(t) * nb_levels;
-    double tau = 0;
-    for (int q = t; q < nb_levels - 1; ++q) {
-      tau += chord[q] * (K[q * tile + jj] + K[(q + 1) * tile + jj]);
-      if (tau > 10) break;
-    }
-    T[t * tile + jj] = pow2(-tau * kLog2E);
+  const double r0 = bottom_radius;
+  double h = 0.0;
+  for (int p = threadIdx.x; p < tile * (L - 1); p += kTransmissionBlock) {
+    const int t = __builtin_amdgcn_readfirstlane(p / tile);          // p % tile == j
+    const double tau = tangentDepth<double>(K, tile, j, t, L, path + size_t(t) * L);
+    const double zl = levels[t > 0 ? t - 1 : 0].altitude;
+    const double zu = levels[t + 1].altitude;
+    h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - pow2(-tau * kLog2E));
   }
-  if (threadIdx.x < tile) T[(nb_levels - 1) * tile + threadIdx.x] = 1.0;   // transparent top
+  part[g * tile + j] = h;
   __syncthreads();
-  if (threadIdx.x < tile && i < n) {
-    const double r0 = bottom_radius;
-    double h = 0;
-    double z_prev = levels[0].altitude;
-    for (int l = 0; l < nb_levels - 1; ++l) {
-      const double z_next = levels[l + 1].altitude;
-      h += 2. * ((r0 + z_prev) * (1. - T[l * tile + j]) + (r0 + z_next) * (1. - T[(l + 1) * tile + j])) *
-           (z_next - z_prev) * 0.5;
-      z_prev = z_next;
-    }
-    h = sqrt(h + r0 * r0) - r0;
-    const double r = h + r0;
+  if (int(threadIdx.x) < tile && i < n) {
+    double hs = 0.0;
+    for (int q = 0; q < groups; ++q) hs += part[q * tile + j];
+    const double hh = sqrt(hs + r0 * r0) - r0;
+    const double r = hh + r0;
     const double d = r * r / star_radius / star_radius * 1e6;
     depth[i] = d;
     if (spectrum) spectrum[i] = d;
@@ -228,7 +228,6 @@ void deviceFree(void* p) {
 }
 
 constexpr size_t kMaxSpecies = 16;                    // register-cached kernels up to 16 species
-constexpr size_t kTransmissionLds = 64 * 1024;        // LDS budget of one transmission workgroup
 
 }  // namespace
 
@@ -244,6 +243,7 @@ DeviceEngine::DeviceEngine(int device) : device_(device) {
   hipDeviceProp_t prop;
   check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   nb_cus_ = prop.multiProcessorCount;
+  lds_limit_ = prop.sharedMemPerBlock;
 }
 
 DeviceEngine::~DeviceEngine() {
@@ -447,7 +447,7 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmiss
   if (species_stride_ == 0) species_stride_ = kChunk;
   if (transmission && species_stride_ > kMaxSpecies)
     raise(kInvalidInput, "the transmission model takes at most 16 tabulated species");
-  if (transmission && 2 * nb_levels * 8 * sizeof(double) > kTransmissionLds)
+  if (transmission && transmissionLds(nb_levels, 64) > lds_limit_)
     raise(kInvalidInput, "too many levels for the transmission kernel");
   const size_t lv = (nb_levels * sizeof(LevelState) + 63) / 64 * 64;      // stencil records start 64-B aligned
   const size_t sp = nb_levels * species_stride_ * sizeof(LevelSpecies);
@@ -564,17 +564,18 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
   const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
   const double* path = reinterpret_cast<const double*>(staging_dev_ + off_path_);
   const int n = int(n_points_), L = int(nb_levels_);
-  // the widest wavenumber tile whose two LDS arrays fit the budget
-  int tile = 64;
-  while (tile > 8 && size_t(2) * L * tile * sizeof(double) > kTransmissionLds) tile /= 2;
-  const size_t lds = size_t(2) * L * tile * sizeof(double);
+  // a wavefront per tangent path: tiles of 128 or 64 wavenumbers, the wider
+  // one when its LDS (K[level][tile] and the partials) fits the workgroup
+  const int tile = transmissionLds(L, 128) <= lds_limit_ ? 128 : 64;
+  const size_t lds = transmissionLds(L, tile);
   const int blocks = (n + tile - 1) / tile;
   beginTimed();
   switch (species_stride_) {
 #define BEAR_TRANSMISSION(NS)                                                                                       \
   case NS:                                                                                                          \
-    hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kBlock), lds, s, tables_, stride_, rayleigh_,    \
-                       lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, flux_, spectrum_dev);      \
+    hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_, stride_,    \
+                       rayleigh_, lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, flux_,       \
+                       spectrum_dev);                                                                               \
     break;
     BEAR_TRANSMISSION(4)
     BEAR_TRANSMISSION(8)
