@@ -64,7 +64,7 @@ class EvalStats(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile", C.c_int), ("grid", C.c_int),
                 ("used_graph", C.c_int), ("evaluations", C.c_size_t), ("gpu_ms_total", C.c_double),
-                ("adjoint_chunk", C.c_size_t)]
+                ("adjoint_chunk", C.c_size_t), ("point_group", C.c_int)]
 
 
 TABLE_PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}

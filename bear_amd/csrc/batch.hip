@@ -240,25 +240,30 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
 }
 
 // ---------------------------------------------------------------- transmission
-// Chord lengths of every point: chord[point][t][q] =
-// sqrt((R+z_{q+1})^2 - (R+z_t)^2) - sqrt((R+z_q)^2 - (R+z_t)^2), q >= t
-// (transmission_spectrum.cpp:117-118, 145-154); one workgroup per (t, point).
+// Chord lengths of every point, transposed (chord[point][q][t], zero for
+// q < t and in the padding): sqrt((R+z_{q+1})^2 - (R+z_t)^2) -
+// sqrt((R+z_q)^2 - (R+z_t)^2), q >= t (transmission_spectrum.cpp:117-118,
+// 145-154); one workgroup per (segment q, point), FP64 and/or FP32.
 // Evaluated without contraction, as the host does: fused, a*a - b*b at a == b
 // leaves the rounding error of b*b, which can be negative.
 __global__ __launch_bounds__(kBatchBlock) void chordKernel(Records rec, int nb_levels, double* __restrict__ chord,
                                                            float* __restrict__ chord32) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x, m = blockIdx.y;
+  const int q = blockIdx.x, m = blockIdx.y;
+  const int tangents = chordTangents(nb_levels);
   const unsigned char* r = rec.at(m);
   const double r0 = reinterpret_cast<const BatchModel*>(r)->bottom_radius;
   const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
-  const double b = r0 + levels[t].altitude;
-  const size_t row = (size_t(m) * nb_levels + t) * nb_levels;
-  for (int q = t + int(threadIdx.x); q < nb_levels - 1; q += kBatchBlock) {
-    const double a1 = r0 + levels[q + 1].altitude, a0 = r0 + levels[q].altitude;
-    const double c = sqrt(a1 * a1 - b * b) - sqrt(a0 * a0 - b * b);
-    chord[row + q] = c;
-    if (chord32) chord32[row + q] = float(c);
+  const size_t row = (size_t(m) * gridDim.x + q) * tangents;
+  for (int t = int(threadIdx.x); t < tangents; t += kBatchBlock) {
+    double c = 0.0;
+    if (q < nb_levels - 1 && t <= q) {
+      const double b = r0 + levels[t].altitude;
+      const double a1 = r0 + levels[q + 1].altitude, a0 = r0 + levels[q].altitude;
+      c = sqrt(a1 * a1 - b * b) - sqrt(a0 * a0 - b * b);
+    }
+    if (chord) chord[row + t] = c;
+    if (chord32) chord32[row + t] = float(c);
   }
 }
 
@@ -269,66 +274,87 @@ __device__ __forceinline__ double transmittance(float tau) {
 
 // Transit depth per (wavenumber, point) (transmission_spectrum.cpp:32-154).
 // One work item = `tile` wavenumbers of one point:
-//  1. extinction K[level][tile] in LDS (Op::K: FP64 or FP32), threads split
-//     over level ranges with register-cached stencils;
-//  2. the (tangent level t, wavenumber) pairs: tau summed outward from t in
-//     the reference's order, stopping once tau > 10; each pair adds
-//     c_t (1 - e^-tau) to its thread's FP64 partial, where
-//     c_t = (R+z_t)(z_{t+1} - z_{t-1}) is the coefficient of (1 - T_t) in the
-//     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
+//  1. extinction k_l (Op::K: FP64 or FP32), threads split over level ranges
+//     with register-cached stencils, stored as the segment sums
+//     S[q] = k_q + k_{q+1} in LDS (across a group boundary after the first
+//     barrier, from the next group's first level);
+//  2. blocks of eight tangent levels per thread group (tangentBlock: tau
+//     summed outward in the reference's order, stopping at the first sum
+//     above 10); each path adds c_t (1 - e^-tau) to its thread's FP64
+//     partial, where c_t = (R+z_t)(z_{t+1} - z_{t-1}) is the coefficient of
+//     (1 - T_t) in the reference's trapezoid (one-sided at t = 0; T = 1 at
+//     the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R
 //     and depth (R + h)^2 / R_*^2 1e6 in FP64.
 template <class Op>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
-    int tiles, int tile, int batch, double* __restrict__ depth) {
+    int tiles, int tile, int batch, int point_group, double* __restrict__ depth) {
   using K = typename Op::K;
   using Species = typename Op::Species;
   extern __shared__ double lds[];
-  K* kx = reinterpret_cast<K*>(lds);
-  double* part = lds + (size_t(nb_levels) * tile * sizeof(K) + 7) / 8;
   const int L = nb_levels;
+  const int segments = chordSegments(L), tangents = chordTangents(L);
+  K* S = reinterpret_cast<K*>(lds);
+  double* part = lds + (size_t(segments) * tile * sizeof(K) + 7) / 8;   // also the group-boundary levels
   const int groups = kTransmissionBlock / tile;
   const int j = int(threadIdx.x) % tile;
-  const int g = int(threadIdx.x) / tile;
+  const int g = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / tile);   // tile: a multiple of 64
   const int chunk = (L + groups - 1) / groups;
   const int l0 = g * chunk;
   const int l1 = min(L, l0 + chunk);
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tl;
-    if (!workItem(item, batch, tiles, m, tl)) continue;      // uniform over the workgroup
+    if (!workItem(item, batch, tiles, m, tl, point_group)) continue;      // uniform over the workgroup
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const Species* species = reinterpret_cast<const Species*>(r + rec.species);
     const int i = tl * tile + j;
+    K last = K(0);
     if (i < n) {
       Op op;
       op.init(tables, base, stride, i, 0);
       const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
       for (int l = l0; l < l1; ++l) {
-        const double k = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
-                                         l == l0 || levels[l].reload != 0));
-        kx[l * tile + j] = K(k + sigma_r * levels[l].rayleigh_density);
+        const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
+                                          l == l0 || levels[l].reload != 0));
+        const K k = K(kd + sigma_r * levels[l].rayleigh_density);
+        if (l == l0) part[g * tile + j] = double(k);
+        else S[(l - 1) * tile + j] = last + k;
+        last = k;
       }
     } else {
-      for (int l = l0; l < l1; ++l) kx[l * tile + j] = K(0);
+      for (int l = l0; l < l1; ++l) {
+        if (l == l0) part[g * tile + j] = 0.0;
+        else S[(l - 1) * tile + j] = K(0);
+      }
     }
+    __syncthreads();
+    if (l0 < l1 && l1 < L) S[(l1 - 1) * tile + j] = last + K(part[(g + 1) * tile + j]);
+    if (g == 0)
+      for (int q = L - 1; q < segments; ++q) S[q * tile + j] = K(0);
     __syncthreads();
     const double r0 = bm->bottom_radius;
     const K* chords;
     if constexpr (sizeof(K) == 8) chords = chord64;
     else chords = chord32;
+    chords += size_t(m) * segments * tangents;
     double h = 0.0;
-    for (int p = int(threadIdx.x); p < tile * (L - 1); p += kTransmissionBlock) {
-      // tile is a multiple of 64: one tangent per wavefront (p % tile == j)
-      const int t = __builtin_amdgcn_readfirstlane(p / tile);
-      const K tau = tangentDepth<K>(kx, tile, j, t, L, chords + (size_t(m) * L + t) * L);
-      const double zl = levels[t > 0 ? t - 1 : 0].altitude;
-      const double zu = levels[t + 1].altitude;
-      h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - transmittance(tau));
+    for (int t0 = kTangentBlock * g; t0 < tangents; t0 += kTangentBlock * groups) {
+      K tau[kTangentBlock];
+      tangentBlock<K>(S + j, tile, chords, tangents, t0, L - 1, tau);
+#pragma unroll
+      for (int k = 0; k < kTangentBlock; ++k) {
+        const int t = t0 + k;
+        if (t < L - 1) {
+          const double zl = levels[t > 0 ? t - 1 : 0].altitude;
+          const double zu = levels[t + 1].altitude;
+          h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - transmittance(tau[k]));
+        }
+      }
     }
     part[g * tile + j] = h;
     __syncthreads();
@@ -413,24 +439,24 @@ template <int NS>
 void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
                         const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
                         const double* chord, const float* chord32, int L, int n, int tiles, int tile, int batch,
-                        double* depth) {
+                        int group, double* depth) {
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                         rec, chord, chord32, L, n, tiles, tile, batch, depth);
+                         rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
       break;
   }
 }
@@ -438,7 +464,7 @@ void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, 
 size_t align(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 size_t transmissionLds(int L, int tile, bool wide) {
-  const size_t k = size_t(L) * tile * (wide ? 8 : 4);
+  const size_t k = size_t(chordSegments(L)) * tile * (wide ? 8 : 4);
   return align(k, 8) + size_t(kTransmissionBlock / tile) * tile * sizeof(double);
 }
 
@@ -543,8 +569,9 @@ void DeviceEngine::reserveBatch(size_t batch) {
   batch_terms_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "batch likelihood terms");
   batch_ll_ = deviceAlloc<double>(batch, "batch log-likelihoods");
   if (transmission_) {
-    batch_chord_ = deviceAlloc<double>(batch * L * L, "batch chord lengths");
-    if (precision_ != TablePrecision::fp64) batch_chord32_ = deviceAlloc<float>(batch * L * L, "batch chord lengths");
+    const size_t chord_size = size_t(chordSegments(int(L))) * chordTangents(int(L));
+    if (precision_ == TablePrecision::fp64) batch_chord_ = deviceAlloc<double>(batch * chord_size, "batch chord lengths");
+    else batch_chord32_ = deviceAlloc<float>(batch * chord_size, "batch chord lengths");
   }
   batch_capacity_ = batch;
 }
@@ -559,8 +586,8 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
   const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
   const bool wide = precision_ == TablePrecision::fp64;
   if (kind == ModelKind::transmission) {
-    hipLaunchKernelGGL(chordKernel, dim3(unsigned(L - 1), unsigned(B)), dim3(kBatchBlock), 0, s, rec, L, batch_chord_,
-                       wide ? nullptr : batch_chord32_);
+    hipLaunchKernelGGL(chordKernel, dim3(unsigned(chordSegments(L)), unsigned(B)), dim3(kBatchBlock), 0, s, rec, L,
+                       wide ? batch_chord_ : nullptr, wide ? nullptr : batch_chord32_);
     check(hipGetLastError(), "chord kernel");
     const int tiles = (n + lc.tile - 1) / lc.tile;
     const size_t lds = transmissionLds(L, lc.tile, wide);
@@ -568,7 +595,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_T(NS)                                                                                                  \
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
-                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, flux);            \
+                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, lc.group, flux);  \
     break;
       BEAR_T(4)
       BEAR_T(8)
@@ -622,9 +649,9 @@ void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch, size_t first) {
 // points for each candidate launch shape and keeps the fastest.
 //   emission / eclipse: workgroup of 64, 128 or 256 wavenumbers; persistent
 //     grid of 8, 16 or 32 waves per CU
-//   transmission: tile of 64 or 128 wavenumbers (one tangent path per
-//     wavefront) within the device's LDS per workgroup;
-//     grid of 1, 2 or 4 workgroups per CU
+//   transmission: tile of 64 or 128 wavenumbers within the device's LDS per
+//     workgroup; grid of 1, 2 or 4 workgroups per CU; points taken in groups
+//     of 1 or 4 (each point brings its own chord matrix)
 BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const size_t b = std::min<size_t>(batch, 64);
@@ -633,7 +660,9 @@ BatchLaunch DeviceEngine::tune(ModelKind kind, size_t batch) {
     const bool wide = precision_ == TablePrecision::fp64;
     for (int tile : {64, 128}) {
       if (transmissionLds(int(nb_levels_), tile, wide) > lds_limit_) continue;
-      for (int per_cu : {1, 2, 4}) cand.push_back(BatchLaunch{kTransmissionBlock, tile, std::max(8, nb_cus_ * per_cu)});
+      for (int group : {1, 4})
+        for (int per_cu : {1, 2, 4})
+          cand.push_back(BatchLaunch{kTransmissionBlock, tile, std::max(8, nb_cus_ * per_cu), group});
     }
     if (cand.empty()) raise(kInvalidInput, "too many levels for the transmission batch kernel");
   } else {

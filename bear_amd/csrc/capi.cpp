@@ -247,6 +247,7 @@ int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out) {
     out->evaluations = e.batchEvaluations();
     out->gpu_ms_total = e.batchMilliseconds();
     out->adjoint_chunk = e.adjointChunk();
+    out->point_group = lc.group;
   });
 }
 

@@ -228,37 +228,63 @@ __device__ __forceinline__ void waveTotals(double (&x)[K]) {
 // value of row r after waveTotals: 4b + kRowSlot[r]
 __device__ __forceinline__ int rowSlot(int row) { return row == 1 ? 2 : row == 2 ? 1 : row; }
 
-// Optical depth along tangent path t for this lane's wavenumber column of
-// K[level][tile] (transmission_spectrum.cpp:112-139): summed outward from t
-// in the reference's order, stopping once tau > 10.  t and chord_row
-// (chord lengths c[t][q], q >= t) must be uniform over the wavefront (tiles
-// a multiple of 64 wavenumbers): the chords arrive with scalar loads, eight
-// per round trip, and lanes whose sum has passed 10 keep their value while
-// the others go on (selects, no branches; the eight extinction values of a
-// round are read before any is used).
+// Optical depths of the tangent paths t0 .. t0+7 at this lane's wavenumber
+// (transmission_spectrum.cpp:107-139): tau_t = sum over q >= t of
+// c[t][q] (k_q + k_{q+1}) in increasing q, stopping at the first partial sum
+// above 10.  seg[q * tile] holds k_q + k_{q+1} for this lane's column (zero
+// in the padded rows), chord the transposed chord matrix with `tangents`
+// columns; t0 (a multiple of 8) and chord are uniform over the wavefront, so
+// the chords arrive by scalar loads.  All eight paths take four segments per
+// step from one set of LDS reads; a path that first passes 10 within a step
+// is summed again one segment at a time from its value before the step (the
+// same fused operations in the same order), so it keeps the partial sum at
+// which the reference stops.  Paths already past 10 keep accumulating, which
+// leaves their stored value untouched.
 template <class K>
-__device__ __forceinline__ K tangentDepth(const K* __restrict__ kx, int tile, int j, int t, int nb_levels,
-                                          const K* __restrict__ chord_row) {
-  const int last = nb_levels - 1;
-  K tau = K(0);
-  K kq = kx[t * tile + j];
-  bool done = false;
-  for (int q0 = t; q0 < last; q0 += 8) {
-    K c[8], kn[8];
+__device__ __forceinline__ void tangentBlock(const K* __restrict__ seg, int tile, const K* __restrict__ chord,
+                                             int tangents, int t0, int nb_segments, K (&out)[kTangentBlock]) {
+  // fin[k]: the stopping sum once path k has passed 10 (always above 10), 0 before
+  K tau[kTangentBlock], fin[kTangentBlock];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) c[u] = q0 + u < last ? chord_row[q0 + u] : K(0);   // 0 past the top
-#pragma unroll
-    for (int u = 0; u < 8; ++u) kn[u] = kx[min(q0 + u + 1, last) * tile + j];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const K next = fma(c[u], kq + kn[u], tau);
-      tau = done ? tau : next;
-      done = done || tau > K(10);
-      kq = kn[u];
-    }
-    if (__all(done)) break;
+  for (int k = 0; k < kTangentBlock; ++k) {
+    tau[k] = K(0);
+    fin[k] = K(0);
   }
-  return tau;
+  for (int q0 = t0; q0 < nb_segments; q0 += kSegmentStep) {
+    K sq[kSegmentStep];
+#pragma unroll
+    for (int u = 0; u < kSegmentStep; ++u) sq[u] = seg[(q0 + u) * tile];
+    const K* c = chord + size_t(q0) * tangents + t0;
+    K prev[kTangentBlock];
+#pragma unroll
+    for (int k = 0; k < kTangentBlock; ++k) prev[k] = tau[k];
+#pragma unroll
+    for (int u = 0; u < kSegmentStep; ++u)
+#pragma unroll
+      for (int k = 0; k < kTangentBlock; ++k) tau[k] = fma(c[u * tangents + k], sq[u], tau[k]);
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < kTangentBlock; ++k) {
+      const bool past = tau[k] > K(10);
+      const bool first = past && fin[k] == K(0);
+      if (__builtin_amdgcn_ballot_w64(first) != 0) {       // uniform branch: exact re-sum of path k
+        K x = prev[k], hit = K(0);
+        bool found = false;
+#pragma unroll
+        for (int u = 0; u < kSegmentStep; ++u) {
+          x = fma(c[u * tangents + k], sq[u], x);
+          const bool now = !found && x > K(10);
+          hit = now ? x : hit;
+          found = found || now;
+        }
+        fin[k] = first ? hit : fin[k];
+      }
+      all = all && past;
+    }
+    if (__builtin_amdgcn_ballot_w64(!all) == 0) break;
+  }
+#pragma unroll
+  for (int k = 0; k < kTangentBlock; ++k) out[k] = fin[k] != K(0) ? fin[k] : tau[k];
 }
 
 // Bins: one workgroup of kBlockBins threads per bin.  The dot product of the
@@ -312,20 +338,22 @@ struct Records {
 };
 
 // item -> (tile, point): XCD x = item mod 8 owns tiles x, x + 8, ...; its
-// items take the points in groups of kPointGroup and run through every point
-// of a group for one tile before the next tile, so that the L2 of each XCD
-// holds the group's per-level records (about 50 KB per point) together with
-// the table rows of the tiles in flight, which every point of the group
-// reuses.
+// items take the points in groups of `group` (kPointGroup unless the launch
+// says otherwise) and run through every point of a group for one tile before
+// the next tile, so that the L2 of each XCD holds the group's per-level
+// records (about 50 KB per point) together with the table rows of the tiles
+// in flight, which every point of the group reuses.  Transmission adds each
+// point's chord matrix (320 KB at 200 levels), for which smaller groups pay.
 constexpr int kPointGroup = 32;
 
-__device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile) {
+__device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int& point, int& tile,
+                                         int group = kPointGroup) {
   const size_t k = item / 8;                                   // this XCD's sequence
   const size_t per_xcd = size_t((tiles + 7) / 8);              // tiles per XCD, padded
-  const size_t span = size_t(kPointGroup) * per_xcd;           // items of a full group
+  const size_t span = size_t(group) * per_xcd;                 // items of a full group
   const size_t g = k / span;
-  const int first = int(g) * kPointGroup;
-  const int points = min(kPointGroup, batch - first);          // the last group may be smaller
+  const int first = int(g) * group;
+  const int points = min(group, batch - first);                // the last group may be smaller
   const size_t w = k - g * span;
   point = first + int(w % size_t(points));
   tile = int(w / size_t(points)) * 8 + int(item % 8);

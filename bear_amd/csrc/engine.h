@@ -35,6 +35,20 @@ struct LevelState {
   double group_u[2][4];
 };
 
+// Transmission geometry (transmission_spectrum.cpp:107-154): tangent path t
+// (t = 0 .. L-2) crosses the segments q = t .. L-2 (level q to q+1) with
+// chord lengths c[t][q].  The device keeps them transposed, chord[q][t]
+// (zero for q < t), with the segment count padded to kSegmentStep and the
+// tangent count to kTangentBlock, zeros in the padding.
+constexpr int kTangentBlock = 8;       // tangent paths per thread
+constexpr int kSegmentStep = 4;        // segments per step
+constexpr int chordSegments(int nb_levels) {
+  return (nb_levels - 1 + kSegmentStep - 1) / kSegmentStep * kSegmentStep;
+}
+constexpr int chordTangents(int nb_levels) {
+  return (nb_levels - 1 + kTangentBlock - 1) / kTangentBlock * kTangentBlock;
+}
+
 // Which stencil slots take the level's shared weights: grouped != 0 when the
 // tabulated species use at most two distinct (T, p) grids; slot s then uses
 // LevelState::group_w[(group1 >> s) & 1], else its own LevelSpecies::w.
@@ -113,6 +127,7 @@ struct BatchLaunch {
   int block = 256;               // threads per workgroup
   int tile = 256;                // wavenumbers per work item
   int grid = 0;                  // persistent workgroups
+  int group = 32;                // points per group of the work-item order (transmission)
 };
 
 // Per bin of every observation, in observation order: the folded operator
@@ -147,7 +162,7 @@ class DeviceEngine {
   size_t speciesStride() const { return species_stride_; }
   size_t tableStride() const { return stride_; }          // elements per table row, a multiple of 64
   double* shifts() { return shift_host_; }
-  double* pathLengths() { return path_host_; }              // [level][level] (transmission)
+  double* pathLengths() { return path_host_; }              // chord[segment][tangent] (transmission)
 
   // One evaluation: the high-resolution spectrum on the grid (kept on the
   // device; also written to spectrum_dev when given), the bins (also to

@@ -35,11 +35,12 @@ void check(hipError_t e, const char* what) {
 
 constexpr int kBlock = 256;
 // Single-point transmission: workgroups of 8 wavefronts share one tile's
-// extinction in LDS (K[level][tile] and the partial effective heights
-// [kTransmissionBlock / tile][tile], FP64)
+// segment extinction sums in LDS (S[segment][tile], rows padded to
+// chordSegments) and the partial effective heights [kTransmissionBlock /
+// tile][tile], FP64
 constexpr int kTransmissionBlock = 512;
 size_t transmissionLds(size_t nb_levels, size_t tile) {
-  return (nb_levels + kTransmissionBlock / tile) * tile * sizeof(double);
+  return (size_t(chordSegments(int(nb_levels))) + kTransmissionBlock / tile) * tile * sizeof(double);
 }
 
 // Emission spectrum epilogue: the flux F = 2 pi (mu1 w1 I1 + mu2 w2 I2) 1e-3 s
@@ -135,58 +136,77 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
 }
 
 // Transmission: transit depth per wavenumber (transmission_spectrum.cpp:32-154).
-// A workgroup owns `tile` consecutive wavenumbers (a multiple of 64) and
-// stages their extinction coefficients for every level in LDS (K[level][tile]).
+// A workgroup owns `tile` consecutive wavenumbers (a multiple of 64):
 //  1. extinction: thread (level group g, wavenumber j) computes
-//     kappa_abs + sigma_R n_R for its contiguous range of levels, with the
-//     register-cached stencils;
-//  2. tangent optical depths: each wavefront takes one tangent level t at a
-//     time for 64 wavenumbers (tangentDepth: chord lengths by scalar loads,
-//     summed outward in the reference's order, stopping once tau > 10) and
-//     adds c_t (1 - e^-tau) to its thread's FP64 partial, where
+//     k_l = kappa_abs + sigma_R n_R for its contiguous range of levels, with
+//     the register-cached stencils, and stores the segment sums
+//     S[q] = k_q + k_{q+1} in LDS (the sum across a group boundary after the
+//     first barrier, from the next group's first level);
+//  2. tangent optical depths: each thread group takes blocks of eight
+//     tangent levels (tangentBlock: chords by scalar loads, summed outward in
+//     the reference's order, stopping at the first sum above 10) and adds
+//     c_t (1 - e^-tau) to its thread's FP64 partial, where
 //     c_t = (R+z_t)(z_t+1 - z_t-1) is the coefficient of (1 - T_t) in the
 //     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R and
 //     the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
-// Chord lengths sqrt((R+z_i)^2 - (R+z_t)^2) differences come from the host.
+// The transposed chord matrix chord[q][t] comes from the host.
 template <int NS>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups stencil_groups,
-    const double* __restrict__ path, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
+    const double* __restrict__ chord, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
     double* __restrict__ depth, double* __restrict__ spectrum) {
   extern __shared__ double lds[];
-  double* K = lds;
-  double* part = lds + size_t(nb_levels) * tile;
   const int L = nb_levels;
+  const int segments = chordSegments(L), tangents = chordTangents(L);
+  double* S = lds;
+  double* part = lds + size_t(segments) * tile;             // also the group-boundary levels of step 1
   const int groups = kTransmissionBlock / tile;
   const int j = threadIdx.x % tile;
-  const int g = threadIdx.x / tile;
+  const int g = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / tile);   // tile: a multiple of 64
   const int i = blockIdx.x * tile + j;
   const int chunk = (L + groups - 1) / groups;
   const int l0 = g * chunk;
   const int l1 = min(L, l0 + chunk);
+  double last = 0.0;
   if (i < n) {
     const double* tab = tables + i;
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
     for (int l = l0; l < l1; ++l) {
       const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], stencil_groups,
-                                      l == l0 || levels[l].reload != 0, v);
-      K[l * tile + j] = k + sigma_r * levels[l].rayleigh_density;
+                                      l == l0 || levels[l].reload != 0, v) +
+                       sigma_r * levels[l].rayleigh_density;
+      if (l == l0) part[g * tile + j] = k;
+      else S[(l - 1) * tile + j] = last + k;
+      last = k;
     }
   } else {
-    for (int l = l0; l < l1; ++l) K[l * tile + j] = 0.0;
+    for (int l = l0; l < l1; ++l) {
+      if (l == l0) part[g * tile + j] = 0.0;
+      else S[(l - 1) * tile + j] = 0.0;
+    }
   }
+  __syncthreads();
+  if (l0 < l1 && l1 < L) S[(l1 - 1) * tile + j] = last + part[(g + 1) * tile + j];
+  if (g == 0)
+    for (int q = L - 1; q < segments; ++q) S[q * tile + j] = 0.0;
   __syncthreads();
   const double r0 = bottom_radius;
   double h = 0.0;
-  for (int p = threadIdx.x; p < tile * (L - 1); p += kTransmissionBlock) {
-    const int t = __builtin_amdgcn_readfirstlane(p / tile);          // p % tile == j
-    const double tau = tangentDepth<double>(K, tile, j, t, L, path + size_t(t) * L);
-    const double zl = levels[t > 0 ? t - 1 : 0].altitude;
-    const double zu = levels[t + 1].altitude;
-    h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - pow2(-tau * kLog2E));
+  for (int t0 = kTangentBlock * g; t0 < tangents; t0 += kTangentBlock * groups) {
+    double tau[kTangentBlock];
+    tangentBlock<double>(S + j, tile, chord, tangents, t0, L - 1, tau);
+#pragma unroll
+    for (int k = 0; k < kTangentBlock; ++k) {
+      const int t = t0 + k;
+      if (t < L - 1) {
+        const double zl = levels[t > 0 ? t - 1 : 0].altitude;
+        const double zu = levels[t + 1].altitude;
+        h += (r0 + levels[t].altitude) * (zu - zl) * (1.0 - pow2(-tau[k] * kLog2E));
+      }
+    }
   }
   part[g * tile + j] = h;
   __syncthreads();
@@ -452,7 +472,8 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmiss
   const size_t lv = (nb_levels * sizeof(LevelState) + 63) / 64 * 64;      // stencil records start 64-B aligned
   const size_t sp = nb_levels * species_stride_ * sizeof(LevelSpecies);
   const size_t sh = (n_obs_ + 1) * sizeof(double);
-  const size_t pa = transmission ? nb_levels * nb_levels * sizeof(double) : 0;
+  const size_t pa = transmission ? size_t(chordSegments(int(nb_levels))) * chordTangents(int(nb_levels)) *
+                                       sizeof(double) : 0;
   off_species_ = lv;
   off_shift_ = lv + sp;
   off_path_ = lv + sp + sh;
@@ -564,8 +585,8 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
   const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
   const double* path = reinterpret_cast<const double*>(staging_dev_ + off_path_);
   const int n = int(n_points_), L = int(nb_levels_);
-  // a wavefront per tangent path: tiles of 128 or 64 wavenumbers, the wider
-  // one when its LDS (K[level][tile] and the partials) fits the workgroup
+  // tiles of 128 or 64 wavenumbers, the wider one when its LDS (the segment
+  // sums and the partials) fits the workgroup
   const int tile = transmissionLds(L, 128) <= lds_limit_ ? 128 : 64;
   const size_t lds = transmissionLds(L, tile);
   const int blocks = (n + tile - 1) / tile;

@@ -363,19 +363,21 @@ bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* 
   BatchModel scalars;
   const bool neglect = stage(p, scalars);
   // chord lengths between successive levels for every tangent level
-  // (transmission_spectrum.cpp:117-118, 145-154): path[t][i] =
-  // sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2)
+  // (transmission_spectrum.cpp:117-118, 145-154), transposed: chord[i][t] =
+  // sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2), i >= t
+  // (the entries with i < t and the padding stay zero)
   const size_t L = atmosphere_.nb_levels;
+  const size_t tangents = size_t(chordTangents(int(L)));
   const double r0 = p[1];
   const std::vector<double>& z = atmosphere_.altitude;
-  double* path = engine_.pathLengths();
+  double* chord = engine_.pathLengths();
   for (size_t t = 0; t + 1 < L; ++t) {
     const double b = r0 + z[t];
     double previous = std::sqrt(b * b - b * b);
     for (size_t i = t; i + 1 < L; ++i) {
       const double a = r0 + z[i + 1];
       const double next = std::sqrt(a * a - b * b);
-      path[t * L + i] = next - previous;
+      chord[i * tangents + t] = next - previous;
       previous = next;
     }
   }

@@ -269,7 +269,7 @@ def config_workload(name, batch, steps):
                                "speedup_vs_single_fp64": round(rate / single, 2),
                                "max_spectrum_rel_err": err_spec, "max_bin_abs_err": err_bins,
                                "max_abs_dloglike": err_ll,
-                               "launch": {k: bst[k] for k in ("block", "tile", "grid", "used_graph")}}
+                               "launch": {k: bst[k] for k in ("block", "tile", "grid", "point_group", "used_graph")}}
             log(f"{name} batch {precision}: {rate:.1f} evals/s, bins err {err_bins:.3e} {unit}, |dlogL| {err_ll:.2e}")
         out["batched"] = legs
         if name == "emission_c4":

@@ -237,6 +237,7 @@ typedef struct bear_batch_stats {
   size_t evaluations;            /* live points evaluated in batches */
   double gpu_ms_total;           /* device time of those batches (with profiling on) */
   size_t adjoint_chunk;          /* last gradient: points per chunk with stored intensities, 0: recomputed */
+  int point_group;               /* launch shape: points per group of the work-item order */
 } bear_batch_stats;
 
 int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out);
