@@ -1,11 +1,13 @@
-// Adjoint of the batched emission likelihood on gfx950: the gradient of
-// log L with respect to every level's temperature, altitude and effective
-// species densities (and the flux scaling), for every live point of a batch.
-// The host carries it on to the retrieval parameters.
+// Adjoint of the batched emission and secondary-eclipse likelihoods on
+// gfx950: the gradient of log L with respect to every level's temperature,
+// altitude and effective species densities (and the flux scaling), for every
+// live point of a batch.  The host carries it on to the retrieval parameters.
 //
 // Reverse order of the forward chain:
 //   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
-//   d logL / d F(nu) = sum_j d logL / d bin_j  W_j(nu)          (fluxAdjointKernel, the
+//     eclipse: also d logL / d P_j for the planet's binned flux and the
+//     star's terms                                              (eclipseGbarKernel)
+//   d logL / d F(nu) = sum_j d logL / d P_j  W_j(nu)            (fluxAdjointKernel, the
 //                      folded operator transposed, per wavenumber)
 //   short characteristics and Planck function                   (rtAdjointKernel)
 //   table interpolation                                         (opacityAdjointKernel)
@@ -81,6 +83,56 @@ __global__ __launch_bounds__(kBlock) void gbarKernel(const double* __restrict__ 
   const BinDesc d = desc[j];
   const double e2 = d.error * d.error + inflation;
   gbar[size_t(m) * nb_bins + j] = d.weight * (d.data - bins[size_t(m) * nb_bins + j]) / e2;
+}
+
+// Secondary eclipse (binBody mode 1): depth_j = P_j / S_j (Rp/Rs)^2 1e6 (+
+// shift) with P_j = sum W_j F, S_j = sum W_j pi B(T_*) 1e-3.  One workgroup
+// per (bin, point) recomputes P_j, S_j and dS_j/dT_* and writes
+//   gbar[j]    d logL / d depth_j
+//   fscale[j]  d logL / d P_j = gbar_j (Rp/Rs)^2 1e6 / S_j
+//   terms[j]   {gbar_j (depth_j - shift), -gbar_j (depth_j - shift) dS_j/dT_* / S_j}
+// (secondary_eclipse.cpp:231-246, 262-355).
+__global__ __launch_bounds__(kBlockBins) void eclipseGbarKernel(
+    const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
+    const double* __restrict__ weights, const BinDesc* __restrict__ desc, int nb_bins, Records rec,
+    const double* __restrict__ bins, double* __restrict__ gbar, double* __restrict__ fscale,
+    double* __restrict__ terms) {
+  __shared__ double partial[3][kBlockBins / 64];
+  const int j = blockIdx.x, m = blockIdx.y;
+  const BatchModel* bm = reinterpret_cast<const BatchModel*>(rec.at(m));
+  const BinDesc d = desc[j];
+  const double* w = weights + d.weight_offset;
+  const double* f = flux + size_t(m) * stride + d.start;
+  double p = 0.0, st = 0.0, dst = 0.0;
+  for (uint32_t k = threadIdx.x; k < d.length; k += kBlockBins) {
+    double s, ds;
+    starFluxWithDerivative(wavenumber[d.start + k], bm->star_inv_temperature, s, ds);
+    p = fma(w[k], f[k], p);
+    st = fma(w[k], s, st);
+    dst = fma(w[k], ds, dst);
+  }
+  p = waveSum(p);
+  st = waveSum(st);
+  dst = waveSum(dst);
+  if ((threadIdx.x & 63) == 0) {
+    partial[0][threadIdx.x >> 6] = p;
+    partial[1][threadIdx.x >> 6] = st;
+    partial[2][threadIdx.x >> 6] = dst;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double pj = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
+    const double sj = (partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3]);
+    const double dsj = (partial[2][0] + partial[2][1]) + (partial[2][2] + partial[2][3]);
+    const size_t o = size_t(m) * nb_bins + j;
+    const double e2 = d.error * d.error + bm->inflation;
+    const double g = d.weight * (d.data - bins[o]) / e2;
+    const double depth = pj / sj * bm->depth_scale;
+    gbar[o] = g;
+    fscale[o] = g * bm->depth_scale / sj;
+    terms[2 * o] = g * depth;
+    terms[2 * o + 1] = -g * depth * dsj / sj;
+  }
 }
 
 // d logL / d F(nu) through the transposed operator
@@ -531,15 +583,19 @@ size_t align8(size_t x) { return (x + 7) / 8 * 8; }
 }  // namespace
 
 void DeviceEngine::releaseAdjoint() {
-  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_intensity_, (void*)adj_partial_, (void*)adj_out_})
+  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_fscale_, (void*)adj_terms_, (void*)adj_intensity_,
+                  (void*)adj_partial_, (void*)adj_out_})
     deviceFree(p);
-  adj_fbar_ = adj_gbar_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
+  adj_fbar_ = adj_gbar_ = adj_fscale_ = adj_terms_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
   adj_capacity_ = adj_grid_ = adj_partial_size_ = adj_intensity_size_ = 0;
 }
 
-void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
-                                        std::vector<double>& bins) {
-  if (transmission_) raise(kInvalidInput, "the adjoint covers the emission model");
+void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
+                                        std::vector<double>& bin_grad, std::vector<double>& bins,
+                                        std::vector<double>& bin_terms) {
+  if (transmission_ || kind == ModelKind::transmission)
+    raise(kInvalidInput, "the adjoint covers the emission and secondary-eclipse models");
+  const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
   const int NS = int(species_stride_);
@@ -581,6 +637,8 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
     adj_partial_size_ = partial_size;
     adj_intensity_size_ = buffer_size;
     adj_gbar_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint bins");
+    adj_fscale_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint planet bins");
+    adj_terms_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_) * 2, "adjoint eclipse terms");
     adj_out_ = deviceAlloc<double>(adj_capacity_ * L * K, "adjoint level gradients");
     adj_fbar_ = deviceAlloc<double>(adj_capacity_ * stride_, "adjoint flux");
     adj_intensity_ = deviceAlloc<double>(buffer_size, "adjoint forward quantities");
@@ -594,15 +652,21 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                       rec_shifts_, groups_};
     launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, kept ? adj_intensity_ : nullptr);
-    launchBatchRest(ModelKind::emission, size_t(cb), first);
+    launchBatchRest(kind, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
-    if (nb > 0) {
+    double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;
+    if (nb > 0 && eclipse) {
+      hipLaunchKernelGGL(eclipseGbarKernel, dim3(unsigned(nb), unsigned(cb)), dim3(kBlockBins), 0, s,
+                         batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bins_desc_, nb, rec,
+                         batch_bins_ + first * n_bins_, gbar, fscale, adj_terms_ + first * n_bins_ * 2);
+      check(hipGetLastError(), "adjoint eclipse bin kernel");
+    } else if (nb > 0) {
       hipLaunchKernelGGL(gbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock), 0, s,
                          batch_bins_ + first * n_bins_, bins_desc_, nb, rec, gbar);
       check(hipGetLastError(), "adjoint bin kernel");
     }
     hipLaunchKernelGGL(fluxAdjointKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock), 0,
-                       s, adj_ptr_, adj_bin_, adj_weight_, gbar, nb, n, stride_, adj_fbar_);
+                       s, adj_ptr_, adj_bin_, adj_weight_, fscale, nb, n, stride_, adj_fbar_);
     check(hipGetLastError(), "adjoint flux kernel");
     switch (NS) {
 #define BEAR_ADJ(NSV)                                                                                              \
@@ -636,8 +700,12 @@ void DeviceEngine::evaluateAdjointBatch(size_t batch, std::vector<double>& level
   level_grad.resize(batch * L * K);
   bin_grad.resize(batch * n_bins_);
   bins.resize(batch * n_bins_);
+  bin_terms.assign(eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   check(hipMemcpyAsync(level_grad.data(), adj_out_, level_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
         "read level gradients");
+  if (eclipse && nb > 0)
+    check(hipMemcpyAsync(bin_terms.data(), adj_terms_, bin_terms.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+          "read eclipse bin terms");
   if (nb > 0) {
     check(hipMemcpyAsync(bin_grad.data(), adj_gbar_, bin_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
           "read bin gradients");

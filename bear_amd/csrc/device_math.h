@@ -20,6 +20,7 @@ constexpr double kPlanckC1 = 2. * 6.62606896e-27 * 2.99792458e10 * 2.99792458e10
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kLog2Ten = 3.321928094887362347870319429489390175864831393;
 constexpr double kLog2E = 1.442695040888963407359924681001892137426645954;
+constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
 
 // 2^t in FP64 with relative error below 3.1e-15: t = n + f, |f| <= 1/2,
 // 2^f from its degree-10 Chebyshev interpolant on [-1/2, 1/2] (coefficients
@@ -86,6 +87,16 @@ __device__ __forceinline__ double planckAt(double b_num, double x_nu, double inv
 __device__ __forceinline__ double starFlux(double nu, double inv_t) {
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   return planckAt(kPlanckC1 * nu * nu * nu, x_nu, inv_t) * kPi * 1e-3;
+}
+
+// starFlux and its derivative with respect to the star's temperature:
+// d B / dT = B ln2 (-x) / T^2 / (1 - q) for q = 2^(x / T)
+__device__ __forceinline__ void starFluxWithDerivative(double nu, double inv_t, double& flux, double& d_flux) {
+  const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+  const double q = pow2Lean(x_nu * inv_t);
+  const double r = recip(1.0 - q);
+  flux = kPlanckC1 * nu * nu * nu * q * r * kPi * 1e-3;
+  d_flux = flux * (-x_nu * kLn2) * inv_t * inv_t * r;
 }
 
 // One layer of the two-angle short characteristics (short_characteristics.cpp:88-114)

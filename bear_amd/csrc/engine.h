@@ -211,9 +211,12 @@ class DeviceEngine {
   // [batch][bins].  The forward pass keeps every level's intensities and
   // opacity for the backward pass, in chunks of points that fit the budget;
   // a point that does not fit alone is recomputed inside the backward kernel
-  // instead.
-  void evaluateAdjointBatch(size_t batch, std::vector<double>& level_grad, std::vector<double>& bin_grad,
-                            std::vector<double>& bins);
+  // instead.  kind: emission, or eclipse (planet flux at scale 1 against the
+  // blackbody star; bin_terms then gets per (point, bin) dL/dbin times the
+  // depth without shift, and dL/dbin times d depth / dT_star,
+  // [batch][bins][2]).
+  void evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
+                            std::vector<double>& bin_grad, std::vector<double>& bins, std::vector<double>& bin_terms);
   // HBM budget (bytes) of the kept forward quantities; 0: always recompute
   void setAdjointMemory(size_t bytes) { adjoint_budget_ = bytes; }
   size_t adjointMemory() const { return adjoint_budget_; }
@@ -319,6 +322,8 @@ class DeviceEngine {
   double* adj_weight_ = nullptr;
   double* adj_fbar_ = nullptr;                   // [capacity][stride]
   double* adj_gbar_ = nullptr;                   // [capacity][bins]
+  double* adj_fscale_ = nullptr;                 // eclipse: dL/d(planet bin) [capacity][bins]
+  double* adj_terms_ = nullptr;                  // eclipse: [capacity][bins][2]
   double* adj_intensity_ = nullptr;              // kept: [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]

@@ -228,12 +228,13 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
 }
 
 // Chain rule from the device's level gradients to the parameters
-// [log g, f, d, chemistry..., temperature..., modifiers...]: effective
-// densities to species densities (CIA products), then the atmosphere's
-// reverse sweep, then g = 10^log g and scale = (R_J / d)^2 f.
-void EmissionModel::gradientBatch(const double* params, size_t batch, size_t stride,
-                                  const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
-                                  const char* neglect, double* gradient) {
+// [log g, p1, p2, chemistry..., temperature..., modifiers...] of the emission
+// and eclipse models: effective densities to species densities (CIA
+// products), then the atmosphere's reverse sweep, g = 10^log g, the model's
+// own p1, p2 (generalGradient) and the observations' shifts.
+void TabulatedModel::chainGradient(const double* params, size_t batch, size_t stride,
+                                   const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
+                                   const std::vector<double>& bin_terms, const char* neglect, double* gradient) {
   const size_t L = atmosphere_.nb_levels, S = species_.size(), NS = engine_.speciesStride(), K = NS + 4;
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
@@ -277,10 +278,7 @@ void EmissionModel::gradientBatch(const double* params, size_t batch, size_t str
       double g_bar = 0;
       atm.adjoint(g, temp, chem, t_bar, n_bar, z_bar, g_bar, out + 3 + nb_chem_, out + 3);
       out[0] = g_bar * g * std::log(10.0);
-      const double s_bar = G[NS + 2];
-      const double base = (phys::radius_jupiter / p[2]) * (phys::radius_jupiter / p[2]);
-      out[1] = s_bar * base;
-      out[2] = s_bar * (-2.0 * base * p[1] / p[2]);
+      generalGradient(p, G[NS + 2], bin_terms.empty() ? nullptr : bin_terms.data() + b * nb * 2, out);
       // shift_const: one parameter per observation, added to its bins
       size_t k = 3 + nb_chem_ + nb_temp_, j = 0;
       for (const auto& o : observations_) {
@@ -298,6 +296,13 @@ void EmissionModel::gradientBatch(const double* params, size_t batch, size_t str
     }
   }
   if (!failure.empty()) raise(failure_code, failure);
+}
+
+// emission: scale = (R_J / d)^2 f (emission.cpp:123-153)
+void EmissionModel::generalGradient(const double* p, double scale_bar, const double*, double* out) const {
+  const double base = (phys::radius_jupiter / p[2]) * (phys::radius_jupiter / p[2]);
+  out[1] = scale_bar * base;
+  out[2] = scale_bar * (-2.0 * base * p[1] / p[2]);
 }
 
 // ---------------------------------------------------------------- secondary eclipse
@@ -319,6 +324,19 @@ bool OccultationModel::setup(const double* p, Atmosphere& atm, BatchModel& scala
   scalars.star_inv_temperature = 1.0 / p[2];
   scalars.depth_scale = p[1] * p[1] * 1e6;
   return neglect;
+}
+
+// eclipse: depth = P / S (Rp/Rs)^2 1e6 per bin, S the binned blackbody star;
+// bin_terms per bin {dL/ddepth (depth - shift), dL/ddepth d depth / dT_*}
+void OccultationModel::generalGradient(const double* p, double, const double* bin_terms, double* out) const {
+  double ratio_bar = 0, t_bar = 0;
+  const size_t nb = engine_.nbBins();
+  for (size_t j = 0; bin_terms && j < nb; ++j) {
+    ratio_bar += bin_terms[2 * j];
+    t_bar += bin_terms[2 * j + 1];
+  }
+  out[1] = ratio_bar * 2.0 / p[1];
+  out[2] = t_bar;
 }
 
 // OccultationModel::calcModelGPU (secondary_eclipse.cpp:262-355): planet
@@ -494,16 +512,17 @@ void Retrieval::setTablePrecision(TablePrecision p) { engine_->setTablePrecision
 // with s_j^2 = sigma_j^2 + 10^e.
 void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, double* ll, double* gradient) {
   if (batch == 0) return;
-  if (model_->kind() != ModelKind::emission) raise(kInvalidInput, "the gradient is available for the emission model");
+  if (model_->kind() == ModelKind::transmission)
+    raise(kInvalidInput, "the gradient is available for the emission and secondary-eclipse models");
   if (engine_->tablePrecision() != TablePrecision::fp64)
     raise(kInvalidInput, "the gradient needs the FP64 tables (table precision fp64)");
   const std::vector<char> neglect = stageBatch(params, batch, n);
-  std::vector<double> level_grad, bin_grad, bins;
-  engine_->evaluateAdjointBatch(batch, level_grad, bin_grad, bins);
+  std::vector<double> level_grad, bin_grad, bins, bin_terms;
+  engine_->evaluateAdjointBatch(model_->kind(), batch, level_grad, bin_grad, bins, bin_terms);
   const double* v = engine_->batchLogLikelihood();
   for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
   std::fill(gradient, gradient + batch * n, 0.0);
-  model_->gradientBatch(params, batch, n, level_grad, bin_grad, neglect.data(), gradient);
+  model_->gradientBatch(params, batch, n, level_grad, bin_grad, bin_terms, neglect.data(), gradient);
   if (!global_.error_inflation) return;
   const size_t nb = engine_->nbBins();
   for (size_t b = 0; b < batch; ++b) {

@@ -51,13 +51,15 @@ class ForwardModel {
   // [first, end) of each tabulated species' rows in the table store
   virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
   // gradient rows [batch][stride] of the model's parameters (and observation
-  // modifiers) from the engine's adjoint: level gradients [batch][levels][slots]
-  // and bin gradients [batch][bins]; rows of neglected points are left alone
+  // modifiers) from the engine's adjoint: level gradients [batch][levels][slots],
+  // bin gradients [batch][bins] and (eclipse) the bin terms [batch][bins][2];
+  // rows of neglected points are left alone
   virtual void gradientBatch(const double* parameters, size_t batch, size_t stride,
                              const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
-                             const char* neglect, double* gradient) {
-    (void)parameters, (void)batch, (void)stride, (void)level_grad, (void)bin_grad, (void)neglect, (void)gradient;
-    raise(kInvalidInput, "the gradient is available for the emission model");
+                             const std::vector<double>& bin_terms, const char* neglect, double* gradient) {
+    (void)parameters, (void)batch, (void)stride, (void)level_grad, (void)bin_grad, (void)bin_terms, (void)neglect,
+        (void)gradient;
+    raise(kInvalidInput, "the gradient is available for the emission and secondary-eclipse models");
   }
 };
 
@@ -81,6 +83,16 @@ class TabulatedModel : public ForwardModel {
   // the single-model path: setup + stageLevels into the engine's staging
   bool stage(const double* parameters, BatchModel& scalars);
   const double* modifiers(const double* p) const { return p + nb_general_ + nb_chem_ + nb_temp_; }
+  // emission and eclipse: the chain from the level gradients through the
+  // atmosphere to [log g, chemistry..., temperature..., modifiers...];
+  // generalGradient fills the other two general parameters
+  void chainGradient(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
+                     const std::vector<double>& bin_grad, const std::vector<double>& bin_terms, const char* neglect,
+                     double* gradient);
+  // scale_bar: dL/d(flux scaling); bin_terms: this point's [bins][2] (eclipse)
+  virtual void generalGradient(const double* p, double scale_bar, const double* bin_terms, double* out) const {
+    (void)p, (void)scale_bar, (void)bin_terms, (void)out;
+  }
   const std::vector<ObservationSetup>& observations_;
   DeviceEngine& engine_;
   Atmosphere atmosphere_;
@@ -103,10 +115,14 @@ class EmissionModel : public TabulatedModel {
   bool binsAreFluxes() const override { return true; }
   ModelKind kind() const override { return ModelKind::emission; }
   void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
-                     const std::vector<double>& bin_grad, const char* neglect, double* gradient) override;
+                     const std::vector<double>& bin_grad, const std::vector<double>& bin_terms, const char* neglect,
+                     double* gradient) override {
+    chainGradient(parameters, batch, stride, level_grad, bin_grad, bin_terms, neglect, gradient);
+  }
 
  protected:
   bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
+  void generalGradient(const double* p, double scale_bar, const double* bin_terms, double* out) const override;
 };
 
 // OccultationModel (forward_model/secondary_eclipse/secondary_eclipse.h) with
@@ -121,9 +137,15 @@ class OccultationModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
   ModelKind kind() const override { return ModelKind::eclipse; }
+  void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
+                     const std::vector<double>& bin_grad, const std::vector<double>& bin_terms, const char* neglect,
+                     double* gradient) override {
+    chainGradient(parameters, batch, stride, level_grad, bin_grad, bin_terms, neglect, gradient);
+  }
 
  protected:
   bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
+  void generalGradient(const double* p, double scale_bar, const double* bin_terms, double* out) const override;
 };
 
 // TransmissionModel (forward_model/transmission/transmission.h): parameters

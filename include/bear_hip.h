@@ -209,11 +209,13 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
                                        size_t nb_parameters, double* spectrum, double* spectrum_obs,
                                        int* neglect_model);
 
-/* Value and gradient of the log-likelihood for a batch (emission model, FP64
- * tables).  No counterpart in the reference, which has no derivatives: a
- * hand-written adjoint of the whole chain (parameters -> temperature profile,
- * chemistry, altitudes -> opacity interpolation -> short characteristics ->
- * folded observation operator -> Gaussian likelihood).  log_likelihood
+/* Value and gradient of the log-likelihood for a batch (emission and
+ * secondary-eclipse models, FP64 tables; transmission returns
+ * BEAR_ERROR_INVALID_INPUT).  No counterpart in the reference, which has no
+ * derivatives: a hand-written adjoint of the whole chain (parameters ->
+ * temperature profile, chemistry, altitudes -> opacity interpolation ->
+ * short characteristics -> folded observation operator, for the eclipse
+ * against the blackbody star -> Gaussian likelihood).  log_likelihood
  * [batch] as bear_retrieval_compute_likelihood_batch; gradient
  * [batch][nb_parameters] (rows of neglected points are zero). */
 int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, size_t batch, size_t nb_parameters,

@@ -1,5 +1,5 @@
-"""The hand-written adjoint of the emission likelihood, through the C ABI and
-the torch.autograd.Function.
+"""The hand-written adjoint of the emission and secondary-eclipse
+likelihoods, through the C ABI and the torch.autograd.Function.
 
 Bar: every gradient component within 1e-4 relative of central finite
 differences (step 1e-4 relative) of the GPU FP64 likelihood, and of the FP64
@@ -77,7 +77,7 @@ def check(grad, ref, label, fd_error):
     assert not bad.any(), label
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small"])
 def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -92,7 +92,7 @@ def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, nam
         check(grad[k], fd2, f"{name} point {k} vs GPU central differences", np.abs(fd - fd2))
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small"])
 def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -118,7 +118,7 @@ def stored_bytes_per_point(r):
     return 8 * (L * 3 * stride + L * slots * tiles + stride)
 
 
-@pytest.mark.parametrize("name", ["tiny", "emission_c1"])
+@pytest.mark.parametrize("name", ["tiny", "emission_c1", "eclipse_small"])
 def test_kept_chunked_and_recomputed_forward_agree(case_factory, tmp_path_factory, name):
     """The backward pass over kept forward quantities (whole batch, and one
     point per chunk) and over a forward sweep recomputed inside the kernel."""
