@@ -314,11 +314,20 @@ __device__ __forceinline__ void binBody(const double* __restrict__ flux, const d
   __shared__ double partial[2][kBlockBins / 64];
   const double* w = weights + b.weight_offset;
   const double* f = flux + b.start;
-  double acc = 0.0, acc_star = 0.0;
-  for (uint32_t j = threadIdx.x; j < b.length; j += kBlockBins) {
-    acc += w[j] * f[j];
-    if (mode == 1) acc_star += w[j] * starFlux(wavenumber[b.start + j], star_inv_temperature);
+  // four independent products per thread per round, so that a bin of a few
+  // thousand wavenumbers costs one or two memory latencies, not one per
+  // kBlockBins elements
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, acc_star = 0.0;
+  uint32_t j = threadIdx.x;
+  for (; j + 3 * kBlockBins < b.length; j += 4 * kBlockBins) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = fma(w[j + u * kBlockBins], f[j + u * kBlockBins], a[u]);
   }
+  for (; j < b.length; j += kBlockBins) a[0] = fma(w[j], f[j], a[0]);
+  double acc = (a[0] + a[1]) + (a[2] + a[3]);
+  if (mode == 1)
+    for (uint32_t k = threadIdx.x; k < b.length; k += kBlockBins)
+      acc_star += w[k] * starFlux(wavenumber[b.start + k], star_inv_temperature);
   acc = waveSum(acc);
   acc_star = waveSum(acc_star);
   if ((threadIdx.x & 63) == 0) {

@@ -270,6 +270,7 @@ class DeviceEngine {
   double* shift_host_ = nullptr;
   double* path_host_ = nullptr;
   double* ll_host_ = nullptr;
+  double* ll_host_dev_ = nullptr;                // ll_host_ as the device addresses it
   // measurement
   bool profiling_ = false, event_pending_ = false;
   void* event_start_ = nullptr;

@@ -230,9 +230,12 @@ __global__ __launch_bounds__(kBlockBins) void binKernel(
           error_inflation, out + blockIdx.x, ll_terms + blockIdx.x);
 }
 
-// Fixed-order sum of the per-bin terms in one workgroup.
-__global__ __launch_bounds__(1024) void sumKernel(const double* __restrict__ terms, int n, double* __restrict__ out) {
+// Fixed-order sum of the per-bin terms in one workgroup; also stored to the
+// pinned host word the caller reads after the stream's synchronisation
+__global__ __launch_bounds__(1024) void sumKernel(const double* __restrict__ terms, int n, double* __restrict__ out,
+                                                  double* __restrict__ host_out) {
   rowSum(terms, n, out);
+  if (threadIdx.x == 0) *host_out = *out;
 }
 
 template <class T>
@@ -491,6 +494,9 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmiss
   shift_host_ = reinterpret_cast<double*>(staging_host_ + off_shift_);
   path_host_ = transmission ? reinterpret_cast<double*>(staging_host_ + off_path_) : nullptr;
   ll_host_ = reinterpret_cast<double*>(staging_host_ + staging_bytes_);
+  void* mapped = nullptr;
+  check(hipHostGetDevicePointer(&mapped, ll_host_, 0), "hipHostGetDevicePointer");
+  ll_host_dev_ = static_cast<double*>(mapped);
 }
 
 void DeviceEngine::upload() {
@@ -509,7 +515,7 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
                        bins_desc_, shifts, mode, mode == 1 ? 1.0 / star_temperature : 0.0, depth_scale, inflation,
                        bins_, ll_terms_);
     check(hipGetLastError(), "bin kernel");
-    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, ll_terms_, int(n_bins_), ll_);
+    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, ll_terms_, int(n_bins_), ll_, ll_host_dev_);
     check(hipGetLastError(), "likelihood kernel");
   }
   if (bins_dev)
@@ -613,10 +619,9 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
 
 double DeviceEngine::logLikelihood() {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  check(hipMemcpyAsync(ll_host_, ll_, sizeof(double), hipMemcpyDeviceToHost, s), "read log-likelihood");
-  check(hipStreamSynchronize(s), "evaluation");
+  check(hipStreamSynchronize(s), "evaluation");                      // sumKernel stored *ll_host_
   collectTiming();
-  return *ll_host_;
+  return n_bins_ > 0 ? *ll_host_ : 0.0;
 }
 
 void DeviceEngine::copyBins(std::vector<double>& out) {

@@ -125,6 +125,10 @@ struct Stencil {
   uint32_t row[4];
   double f_p_lower, f_p_upper, f_t;
   double dft;                  // d f_t / dT: 1 / (T+ - T-), 0 when the T rows coincide
+  // where the rows sit in by_temperature: [t_pos[k / 2]][p_pos[k]] for row k,
+  // so that a species with the same grid takes its own rows at once
+  uint16_t t_pos[2];
+  uint16_t p_pos[4];
 };
 
 class OpacitySpecies {
@@ -138,6 +142,10 @@ class OpacitySpecies {
 
   bool hasTable() const { return !by_temperature.empty(); }
   Stencil bracket(double pressure, double temperature) const;
+  // the rows of this species at a stencil bracketed on a species with the same grid
+  void takeRows(Stencil& st) const {
+    for (int k = 0; k < 4; ++k) st.row[k] = by_temperature[st.t_pos[k / 2]][st.p_pos[k]].id;
+  }
   // same temperatures and, per temperature, the same pressures: bracket()
   // then gives the same fractions for every (p, T)
   bool sameGrid(const OpacitySpecies& other) const;

@@ -194,35 +194,43 @@ bool OpacitySpecies::sameGrid(const OpacitySpecies& o) const {
 }
 
 Stencil OpacitySpecies::bracket(double p, double t) const {
-  const std::vector<TableRow>* lo = &by_temperature.front();
-  const std::vector<TableRow>* hi = lo;
+  size_t lo_i = 0, hi_i = 0;
   if (t < by_temperature.front().front().temperature) {
   } else if (t > by_temperature.back().front().temperature) {
-    lo = hi = &by_temperature.back();
+    lo_i = hi_i = by_temperature.size() - 1;
   } else {
     for (size_t i = 0; i < by_temperature.size(); ++i) {
       const double ti = by_temperature[i].front().temperature;
-      if (ti == t) { lo = hi = &by_temperature[i]; break; }
+      if (ti == t) { lo_i = hi_i = i; break; }
       if (ti < t && by_temperature[i + 1].front().temperature > t) {
-        lo = &by_temperature[i];
-        hi = &by_temperature[i + 1];
+        lo_i = i;
+        hi_i = i + 1;
         break;
       }
     }
   }
-  auto pick = [p](const std::vector<TableRow>& rows, const TableRow*& a, const TableRow*& b) {
-    if (p < rows.front().pressure) { a = b = &rows.front(); return; }
-    if (p > rows.back().pressure) { a = b = &rows.back(); return; }
+  auto pick = [p](const std::vector<TableRow>& rows, size_t& a, size_t& b) {
+    if (p < rows.front().pressure) { a = b = 0; return; }
+    if (p > rows.back().pressure) { a = b = rows.size() - 1; return; }
     for (size_t i = 0; i < rows.size(); ++i) {
-      if (rows[i].pressure == p) { a = b = &rows[i]; return; }
-      if (rows[i].pressure < p && rows[i + 1].pressure > p) { a = &rows[i]; b = &rows[i + 1]; return; }
+      if (rows[i].pressure == p) { a = b = i; return; }
+      if (rows[i].pressure < p && rows[i + 1].pressure > p) { a = i; b = i + 1; return; }
     }
-    a = b = &rows.front();
+    a = b = 0;
   };
-  const TableRow *r0, *r1, *r2, *r3;
-  pick(*lo, r0, r1);
-  pick(*hi, r2, r3);
+  const std::vector<TableRow>& lo = by_temperature[lo_i];
+  const std::vector<TableRow>& hi = by_temperature[hi_i];
+  size_t p0, p1, p2, p3;
+  pick(lo, p0, p1);
+  pick(hi, p2, p3);
+  const TableRow *r0 = &lo[p0], *r1 = &lo[p1], *r2 = &hi[p2], *r3 = &hi[p3];
   Stencil s;
+  s.t_pos[0] = uint16_t(lo_i);
+  s.t_pos[1] = uint16_t(hi_i);
+  s.p_pos[0] = uint16_t(p0);
+  s.p_pos[1] = uint16_t(p1);
+  s.p_pos[2] = uint16_t(p2);
+  s.p_pos[3] = uint16_t(p3);
   s.row[0] = r0->id;
   s.row[1] = r1->id;
   s.row[2] = r2->id;

@@ -61,7 +61,9 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
   const size_t L = atm.nb_levels, S = species_.size(), stride = engine_.speciesStride();
   const double log2_ten = 3.321928094887362347870319429489390175864831393;
   const size_t blocks_per_row = engine_.tableStride() / 64;     // row -> block offset of the device tables
+  const StencilGroups groups = engine_.stencilGroups();
   for (size_t l = 0; l < L; ++l) {
+    Stencil group_st[2] = {};
     const double t = atm.temperature[l];
     // opacity_species.cpp:443-446: scattering uses the scatterer's own density
     const double n_r = rayleigh_species_ >= 0 ? atm.nd(l, rayleigh_species_) : 0.0;
@@ -73,6 +75,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     for (int g = 0; g < 2; ++g) {
       if (group_rep_[g] < 0) continue;
       const Stencil st = species_[size_t(group_rep_[g])].bracket(atm.pressure[l], t);
+      group_st[g] = st;
       const double ft = st.f_t, fl = st.f_p_lower, fu = st.f_p_upper;
       lev[l].group_w[g][0] = (1.0 - ft) * (1.0 - fl) * log2_ten;
       lev[l].group_w[g][1] = (1.0 - ft) * fl * log2_ten;
@@ -88,12 +91,18 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     for (size_t s = 0; s < stride; ++s) {
       // padding slots: zero density, row 0 (loaded once at level 0)
       double n = 0.0;
-      Stencil st{{0, 0, 0, 0}, 0.0, 0.0, 0.0, 0.0};
+      Stencil st{{0, 0, 0, 0}, 0.0, 0.0, 0.0, 0.0, {0, 0}, {0, 0, 0, 0}};
       if (s < S) {
         const OpacitySpecies& sp = species_[s];
         n = atm.nd(l, sp.species_id);                                 // opacity_species.cpp:407-412
         for (int q : sp.cia_partners) n *= atm.nd(l, q);
-        st = sp.bracket(atm.pressure[l], t);                         // rows stay valid when n == 0
+        if (groups.grouped) {
+          // same grid as the group's representative: same bracket, own rows
+          st = group_st[(groups.group1 >> s) & 1u];
+          sp.takeRows(st);
+        } else {
+          st = sp.bracket(atm.pressure[l], t);                       // rows stay valid when n == 0
+        }
       }
       uint32_t block[4];
       for (int k = 0; k < 4; ++k) block[k] = uint32_t(st.row[k] * blocks_per_row);
