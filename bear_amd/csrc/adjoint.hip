@@ -13,10 +13,11 @@
 //   table interpolation                                         (opacityAdjointKernel)
 //   sums over wavenumbers                                       (adjointReduceKernel)
 //
-// The forward batch kernel keeps both angles' intensity and the opacity at
-// every level ([point][level][3][wavenumber], FP64) for as many points at a
-// time as the budget holds.  rtAdjointKernel runs the short-characteristics
-// recurrence backwards over them (no tables, few registers, many waves) and
+// The forward batch kernel keeps both angles' intensity less the Planck
+// function and the opacity at every level ([point][level][3][wavenumber],
+// FP32) for as many points at a time as the budget holds.  rtAdjointKernel
+// runs the short-characteristics recurrence backwards over them (no tables,
+// few registers, many waves) and
 // overwrites each level's opacity with d logL / d kappa; opacityAdjointKernel
 // then recomputes each level's cross-sections from the tables as the forward
 // kernels do and forms d logL / d n_eff and d logL / dT through the tables.
@@ -151,14 +152,16 @@ __global__ __launch_bounds__(kBlock) void fluxAdjointKernel(const uint64_t* __re
 }
 
 // Per slot of one level: the opacity term t_s = n_s sigma_s =
-// 2^(log2 n_s + sum_k w_k v_k), exactly as the forward kernels form it, and
+// 2^(log2 n_s + sum_k w_k v_k), as the forward kernels form it (with
+// kGradientExp the exponential to 1.2e-7, pow2Gradient), and
 // y_s = d log2 sigma_s / dT = sum_k u_k v_k; f(s, t_s, y_s) for each.  Weights
 // from the level's shared sets with grouped stencils, else from the slots'
 // records.
-template <int NS, class F>
+template <int NS, bool kGradientExp = false, class F>
 __device__ __forceinline__ void slotTerms(const LevelSpecies* __restrict__ d, const LevelSpeciesAdjoint* __restrict__ a,
                                           const LevelState& lev, StencilGroups groups, const double (&v)[NS][4],
                                           F&& f) {
+  auto term = [](double x) { return kGradientExp ? pow2Gradient(x) : pow2Lean(x); };
   if (groups.grouped) {
     double wa[4], wb[4], ua[4], ub[4];
 #pragma unroll
@@ -179,7 +182,7 @@ __device__ __forceinline__ void slotTerms(const LevelSpecies* __restrict__ d, co
       y = fma(one ? ub[1] : ua[1], v[s][1], y);
       y = fma(one ? ub[2] : ua[2], v[s][2], y);
       y = fma(one ? ub[3] : ua[3], v[s][3], y);
-      f(s, pow2Lean(x), y);
+      f(s, term(x), y);
     }
   } else {
 #pragma unroll
@@ -192,7 +195,7 @@ __device__ __forceinline__ void slotTerms(const LevelSpecies* __restrict__ d, co
       y = fma(a[s].u[1], v[s][1], y);
       y = fma(a[s].u[2], v[s][2], y);
       y = fma(a[s].u[3], v[s][3], y);
-      f(s, pow2Lean(x), y);
+      f(s, term(x), y);
     }
   }
 }
@@ -235,7 +238,7 @@ __device__ __forceinline__ void planckAdjoint(double b_num, double x_nu, double 
 // q = (a - 1) mu / tau, I_l = a I_{l-1} + (1 + q) B_l + (-a - q) B_{l-1}.
 // dq/dtau = (mu (1 - a) - a tau) / tau^2, from its series in delta = tau/mu
 // below 1e-2 (the closed form cancels there).  r = 1 / tau, inv_mu = 1 / mu.
-__device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, double inv_mu, double i_prev,
+__device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, double inv_mu, double d_prev,
                                              double b_lo, double b_hi, double& ib, double& tbar, double& bbar_hi,
                                              double& bbar_lo) {
   const double a = pow2(tau * (-kLog2E / mu));
@@ -247,7 +250,7 @@ __device__ __forceinline__ void angleAdjoint(double tau, double r, double mu, do
          inv_mu;
   else
     dq = (mu * (1.0 - a) - a * tau) * r * r;
-  tbar = fma(ib, (i_prev - b_lo) * (-a * inv_mu) + (b_hi - b_lo) * dq, tbar);
+  tbar = fma(ib, d_prev * (-a * inv_mu) + (b_hi - b_lo) * dq, tbar);
   bbar_hi = fma(ib, 1.0 + q, bbar_hi);
   bbar_lo = fma(ib, -a - q, bbar_lo);
   ib *= a;
@@ -372,8 +375,8 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
       double tbar = 0.0, bbar_lo = 0.0;
       if (tau != 0) {
         const double rt = recip(tau);
-        angleAdjoint(tau, rt, kMu1, kInvMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
-        angleAdjoint(tau, rt, kMu2, kInvMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu1, kInvMu1, i1p - b_lo, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu2, kInvMu2, i2p - b_lo, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
       }
       const double kbar_lo = tbar * dz * 0.5;
       kbar_hi += kbar_lo;
@@ -403,7 +406,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restrict__ wavenumber, Records rec,
                                                          int nb_levels, int n, size_t stride, int tiles, int batch,
                                                          int ns, const double* __restrict__ fbar,
-                                                         double* __restrict__ keep, double* __restrict__ partial) {
+                                                         float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels;
   const int K = ns + 4;
   const int lane = int(threadIdx.x);
@@ -418,30 +421,31 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
-    double* kp = keep + size_t(m) * L * level_step + ic;
+    float* kp = keep + size_t(m) * L * level_step + ic;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double nu = wavenumber[ic];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double unit = 2.0 * kPi * 1e-3;
-    const double* top = kp + size_t(L - 1) * level_step;
-    const double s_bar = fb * unit * (top[0] * (kMu1 * kW) + top[stride] * (kMu2 * kW));
+    // kept: d = I - B of both angles and kappa per level, FP32
+    const float* top = kp + size_t(L - 1) * level_step;
+    double b_hi, db_hi;
+    planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
+    const double s_bar = fb * unit * ((double(top[0]) + b_hi) * (kMu1 * kW) + (double(top[stride]) + b_hi) * (kMu2 * kW));
     double ib1 = fb * unit * bm->scaling * (kMu1 * kW);
     double ib2 = fb * unit * bm->scaling * (kMu2 * kW);
     double k_hi = top[2 * stride];
-    double b_hi, db_hi;
-    planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
     double z_hi = levels[L - 1].altitude;
     double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
-    const double* below = kp + size_t(L - 2) * level_step;         // the level below, read one step ahead
-    double i1n = below[0], i2n = below[stride], kn = below[2 * stride];
+    const float* below = kp + size_t(L - 2) * level_step;          // the level below, read one step ahead
+    double d1n = below[0], d2n = below[stride], kn = below[2 * stride];
     for (int l = L - 1; l >= 1; --l) {
-      const double i1p = i1n, i2p = i2n, k_lo = kn;
+      const double d1p = d1n, d2p = d2n, k_lo = kn;
       if (l >= 2) {
         below = kp + size_t(l - 2) * level_step;
-        i1n = below[0];
-        i2n = below[stride];
+        d1n = below[0];
+        d2n = below[stride];
         kn = below[2 * stride];
       }
       double b_lo, db_lo;
@@ -453,14 +457,14 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       double tbar = 0.0, bbar_lo = 0.0;
       if (tau != 0) {
         const double rt = recip(tau);
-        angleAdjoint(tau, rt, kMu1, kInvMu1, i1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
-        angleAdjoint(tau, rt, kMu2, kInvMu2, i2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu1, kInvMu1, d1p, b_lo, b_hi, ib1, tbar, bbar_hi, bbar_lo);
+        angleAdjoint(tau, rt, kMu2, kInvMu2, d2p, b_lo, b_hi, ib2, tbar, bbar_hi, bbar_lo);
       }
       const double kbar_lo = tbar * dz * 0.5;
       kbar_hi += kbar_lo;
       zbar_hi = fma(tbar, k_mean, zbar_hi);
       // every contribution to level l is in
-      if (valid) kp[size_t(l) * level_step + 2 * stride] = kbar_hi;
+      if (valid) kp[size_t(l) * level_step + 2 * stride] = float(kbar_hi);
       double x[4] = {zbar_hi, 0.0, bbar_hi * db_hi, 0.0};
       emitSums<4>(out + size_t(l) * tiles * K, lane, x, ns + 1, 3);
       k_hi = k_lo;
@@ -472,7 +476,7 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       bbar_hi = bbar_lo;
     }
     bbar_hi += ib1 + ib2;                                            // I_0 = B_0
-    if (valid) kp[2 * stride] = kbar_hi;
+    if (valid) kp[2 * stride] = float(kbar_hi);
     double x[4] = {zbar_hi, s_bar, bbar_hi * db_hi, 0.0};
     emitSums<4>(out, lane, x, ns + 1, 3);
   }
@@ -486,7 +490,7 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
 template <int NS>
 __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __restrict__ tables, size_t stride,
                                                               Records rec, int nb_levels, int n, int tiles,
-                                                              int batch, const double* __restrict__ keep,
+                                                              int batch, const float* __restrict__ keep,
                                                               double* __restrict__ partial) {
   constexpr int K = NS + 4;
   const int L = nb_levels;
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
     const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
-    const double* kb = keep + size_t(m) * L * 3 * stride + 2 * stride + ic;
+    const float* kb = keep + size_t(m) * L * 3 * stride + 2 * stride + ic;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double* tab = tables + ic;
     double v[NS][4];
@@ -522,7 +526,8 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
       if (l + 2 < L) kb2 = kb[size_t(l + 2) * 3 * stride];
       double x[K];
       double dk = 0.0;
-      slotTerms<NS>(d, a, levels[l], rec.groups, v, [&](int s, double ts, double ys) {
+      // the terms to 1.2e-7 (pow2Gradient): they only weight d logL / d kappa
+      slotTerms<NS, true>(d, a, levels[l], rec.groups, v, [&](int s, double ts, double ys) {
         x[s] = kbar * ts;
         dk = fma(ts, ys, dk);
       });
@@ -568,10 +573,11 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
                    size_t stride, Records rec, int L, int n, int tiles, int batch, const double* fbar,
                    double* buffer, double* partial) {
   if (kept) {
+    float* kept_values = reinterpret_cast<float*>(buffer);
     hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, fbar,
-                       buffer, partial);
+                       kept_values, partial);
     hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch,
-                       buffer, partial);
+                       kept_values, partial);
   } else {
     hipLaunchKernelGGL(emissionAdjointKernel<NS>, grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n, tiles,
                        batch, fbar, buffer, partial);
@@ -610,7 +616,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   // points per chunk: kept forward quantities within the budget (and 80 % of
   // the free memory, checked when the buffers grow), else recomputed
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
-  const size_t kept_point = size_t(L) * 3 * stride_ + partial_point + stride_;
+  const size_t kept_values = size_t(L) * 3 * stride_;                          // FP32 values per point
+  const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_;  // doubles per point
   size_t budget = adjoint_budget_;
   size_t chunk = 0, buffer_size = 0, partial_size = 0;
   bool kept = false;
@@ -619,7 +626,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     kept = chunk > 0;
     if (!kept)
       chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
-    buffer_size = kept ? chunk * size_t(L) * 3 * stride_ : size_t(grid) * L * 2 * kWave;
+    buffer_size = kept ? (chunk * kept_values + 1) / 2 : size_t(grid) * L * 2 * kWave;   // doubles
     partial_size = chunk * partial_point;
   };
   plan();
@@ -651,7 +658,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     const int cb = int(std::min(chunk, batch - first));
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                       rec_shifts_, groups_};
-    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first, kept ? adj_intensity_ : nullptr);
+    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first,
+                        kept ? reinterpret_cast<float*>(adj_intensity_) : nullptr);
     launchBatchRest(kind, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
     double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;

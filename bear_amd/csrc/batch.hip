@@ -187,13 +187,16 @@ struct Opacity32 {
 // Fused opacity + two-angle short characteristics per (wavenumber, point),
 // as the single-point kernel; a work item is blockDim wavenumbers of one
 // point and the flux F = 2 pi sum mu w I 1e-3 scaling goes to flux[point][i].
-// With `keep`, both angles' intensity and the opacity at every level also go
-// to keep[point][level][0, 1, 2][i], for the adjoint's backward sweep.
+// With `keep`, both angles' intensity less the level's Planck function,
+// I - B, and the opacity at every level also go to
+// keep[point][level][0, 1, 2][i] in FP32, for the adjoint's backward sweep
+// (which recomputes B in FP64: the difference it needs keeps its own
+// relative precision, and the gradient's bar is 1e-4).
 template <class Op>
 __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
     size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux,
-    double* __restrict__ keep) {
+    float* __restrict__ keep) {
   using Species = typename Op::Species;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -214,11 +217,11 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
-    double* kept = keep ? keep + size_t(m) * nb_levels * 3 * stride + i : nullptr;
+    float* kept = keep ? keep + size_t(m) * nb_levels * 3 * stride + i : nullptr;
     if (kept) {
-      kept[0] = i1;
-      kept[stride] = i2;
-      kept[2 * stride] = k_prev;
+      kept[0] = 0.0f;                                      // I = B at the bottom
+      kept[stride] = 0.0f;
+      kept[2 * stride] = float(k_prev);
     }
     for (int l = 1; l < nb_levels; ++l) {
       const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups, levels[l].reload != 0));
@@ -226,10 +229,10 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
       if (kept) {
-        double* at = kept + size_t(l) * 3 * stride;
-        at[0] = i1;
-        at[stride] = i2;
-        at[2 * stride] = k_next;
+        float* at = kept + size_t(l) * 3 * stride;
+        at[0] = float(i1 - b_next);
+        at[stride] = float(i2 - b_next);
+        at[2 * stride] = float(k_next);
       }
       k_prev = k_next;
       b_prev = b_next;
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchSpectrumKernel(
 template <int NS>
 void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const void* tab64,
                     const void* tab_low, const float* base, size_t stride, Records rec, int L, int n, int tiles,
-                    int batch, double* flux, double* keep) {
+                    int batch, double* flux, float* keep) {
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, tab64, base, stride, rec, L, n,
@@ -578,7 +581,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
 
 // ---------------------------------------------------------------- launches
 void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc, size_t first,
-                                       double* keep) {
+                                       float* keep) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                     rec_shifts_, groups_};

@@ -68,6 +68,16 @@ __device__ __forceinline__ double pow2Lean(double t) {
   return __builtin_amdgcn_ldexp(p, int(n));
 }
 
+// 2^t with relative error about 1.2e-7, for the gradient's opacity terms
+// (whose bar is 1e-4): t = n + f in FP64, 2^f by the hardware's FP32
+// exponential, then the exponent shift; t is clamped below as in pow2Lean
+__device__ __forceinline__ double pow2Gradient(double t) {
+  t = fmax(t, -1100.0);
+  const double n = __builtin_rint(t);
+  const float f = float(t - n);
+  return __builtin_amdgcn_ldexp(double(__builtin_amdgcn_exp2f(f)), int(n));
+}
+
 // 1/d for finite non-zero d: hardware estimate and two Newton steps
 __device__ __forceinline__ double recip(double d) {
   double r = __builtin_amdgcn_rcp(d);

@@ -284,9 +284,10 @@ class DeviceEngine {
   // batched evaluation
   unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
   // points [first, first + batch) of the staged records; keep: also store
-  // every level's intensities and opacity [batch][levels][3][stride] (emission)
+  // every level's I - B of both angles and opacity, FP32
+  // [batch][levels][3][stride] (emission)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
-                           double* keep = nullptr);
+                           float* keep = nullptr);
   void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);
   BatchLaunch tuned(ModelKind kind, size_t batch);
   BatchLaunch tune(ModelKind kind, size_t batch);
@@ -325,7 +326,7 @@ class DeviceEngine {
   double* adj_gbar_ = nullptr;                   // [capacity][bins]
   double* adj_fscale_ = nullptr;                 // eclipse: dL/d(planet bin) [capacity][bins]
   double* adj_terms_ = nullptr;                  // eclipse: [capacity][bins][2]
-  double* adj_intensity_ = nullptr;              // kept: [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
+  double* adj_intensity_ = nullptr;              // kept: FP32 [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]
   size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0, adj_intensity_size_ = 0;

@@ -221,9 +221,10 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
 int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, size_t batch, size_t nb_parameters,
                                   double* log_likelihood, double* gradient);
 
-/* Device memory (bytes) the gradient may hold for the forward intensities
- * of every level that its backward pass reads back (8 bytes x 2 angles x
- * levels x spectral points per live point, plus per-point partial sums).
+/* Device memory (bytes) the gradient may hold for the forward quantities of
+ * every level that its backward pass reads back (4 bytes x (2 angles'
+ * intensity less the Planck function + opacity) x levels x spectral points
+ * per live point, plus per-point partial sums).
  * Batches are processed in chunks of as many points as fit; with 0, or when
  * a single point does not fit, the backward kernel recomputes the forward
  * pass instead.  Default 32 GiB, never more than 80 % of the free memory.

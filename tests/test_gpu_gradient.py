@@ -108,14 +108,15 @@ def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, na
 
 def stored_bytes_per_point(r):
     """Device bytes one live point takes when its forward quantities are
-    kept: 2 intensities and the opacity x levels x padded spectral points,
-    the per-tile partial sums of every (level, slot) and the flux adjoint."""
+    kept: 2 intensities less the Planck function and the opacity x levels x
+    padded spectral points in FP32, the per-tile partial sums of every
+    (level, slot) and the flux adjoint in FP64."""
     st = r.stats()
     n, L = st["nb_spectral_points"], st["nb_levels"]
     stride = (n + 63) // 64 * 64
     slots = (st["nb_species"] + 3) // 4 * 4 + 4
     tiles = (n + 63) // 64
-    return 8 * (L * 3 * stride + L * slots * tiles + stride)
+    return 4 * L * 3 * stride + 8 * (L * slots * tiles + stride)
 
 
 @pytest.mark.parametrize("name", ["tiny", "emission_c1", "eclipse_small"])
@@ -141,10 +142,10 @@ def test_kept_chunked_and_recomputed_forward_agree(case_factory, tmp_path_factor
     np.testing.assert_array_equal(ll_c, ll_s)
     np.testing.assert_array_equal(g_c, g_s)
     np.testing.assert_array_equal(ll_r, ll_s)
-    # the kept path takes the forward kernel's opacity (2^(log2 n + w.v)) where
-    # the fused kernel recomputes n 2^(x log2 10) (each to 1e-12): the two
-    # gradients agree far inside the 1e-4 bar, not bit for bit
-    np.testing.assert_allclose(g_r, g_s, rtol=1e-7, atol=1e-7 * np.max(np.abs(g_s)))
+    # the kept path reads the forward's I - B and opacity in FP32 where the
+    # fused kernel recomputes them in FP64: the two gradients agree far
+    # inside the 1e-4 bar, not bit for bit
+    np.testing.assert_allclose(g_r, g_s, rtol=1e-5, atol=1e-6 * np.max(np.abs(g_s)))
 
 
 def test_autograd_function_backward(case_factory, tmp_path_factory):
