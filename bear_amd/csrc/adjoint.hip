@@ -626,6 +626,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     kept = chunk > 0;
     if (!kept)
       chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
+    const size_t chunks = (batch + chunk - 1) / chunk;                      // equal chunks, no small tail
+    chunk = (batch + chunks - 1) / chunks;
     buffer_size = kept ? (chunk * kept_values + 1) / 2 : size_t(grid) * L * 2 * kWave;   // doubles
     partial_size = chunk * partial_point;
   };

@@ -330,7 +330,7 @@ class DeviceEngine {
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]
   size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0, adj_intensity_size_ = 0;
-  size_t adjoint_budget_ = size_t(32) << 30, adj_last_chunk_ = 0;
+  size_t adjoint_budget_ = size_t(96) << 30, adj_last_chunk_ = 0;
   void releaseAdjoint();
 };
 

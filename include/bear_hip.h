@@ -227,7 +227,7 @@ int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, s
  * per live point, plus per-point partial sums).
  * Batches are processed in chunks of as many points as fit; with 0, or when
  * a single point does not fit, the backward kernel recomputes the forward
- * pass instead.  Default 32 GiB, never more than 80 % of the free memory.
+ * pass instead.  Default 96 GiB, never more than 80 % of the free memory.
  * No counterpart in the reference. */
 int bear_retrieval_set_adjoint_memory(bear_retrieval* r, size_t bytes);
 
