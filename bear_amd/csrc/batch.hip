@@ -375,22 +375,27 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
 // ---------------------------------------------------------------- bins and likelihood
 // One workgroup per (bin, point); mode 1 (eclipse) divides by the folded
 // blackbody star of that point.
+// One workgroup per (piece of a bin's segment, point) (binChunkBody)
 __global__ __launch_bounds__(kBlockBins) void binBatchKernel(
     const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
-    const double* __restrict__ weights, const BinDesc* __restrict__ bins, int nb_bins, Records rec, int mode,
-    double* __restrict__ out, double* __restrict__ terms) {
+    const double* __restrict__ weights, const BinChunk* __restrict__ chunks, int nb_chunks, Records rec, int mode,
+    double* __restrict__ chunk_sums) {
   const int m = blockIdx.y;
+  const BatchModel* bm = reinterpret_cast<const BatchModel*>(rec.at(m));
+  binChunkBody(flux + size_t(m) * stride, wavenumber, weights, chunks[blockIdx.x], mode, bm->star_inv_temperature,
+               chunk_sums + (size_t(m) * nb_chunks + blockIdx.x) * 2);
+}
+
+// Bins and log-likelihood of one point per workgroup (binFinalizeBody)
+__global__ __launch_bounds__(1024) void sumBatchKernel(const double* __restrict__ chunk_sums, int nb_chunks,
+                                                       const BinDesc* __restrict__ bins, int nb_bins, Records rec,
+                                                       int mode, double* __restrict__ out, double* __restrict__ ll) {
+  const int m = blockIdx.x;
   const unsigned char* r = rec.at(m);
   const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
   const double* shifts = reinterpret_cast<const double*>(r + rec.shifts);
-  const size_t o = size_t(m) * nb_bins + blockIdx.x;
-  binBody(flux + size_t(m) * stride, wavenumber, weights, bins[blockIdx.x], mode, bm->star_inv_temperature,
-          bm->depth_scale, shifts, bm->inflation, out + o, terms + o);
-}
-
-__global__ __launch_bounds__(1024) void sumBatchKernel(const double* __restrict__ terms, int n,
-                                                       double* __restrict__ out) {
-  rowSum(terms + size_t(blockIdx.x) * n, n, out + blockIdx.x);
+  binFinalizeBody(chunk_sums + size_t(m) * nb_chunks * 2, bins, nb_bins, mode, bm->depth_scale, shifts,
+                  bm->inflation, out + size_t(m) * nb_bins, ll + m, nullptr);
 }
 
 // high-resolution output per point: emission F per micron, eclipse depth
@@ -553,11 +558,11 @@ void DeviceEngine::reserveBatch(size_t batch) {
   if (batch_ll_host_) (void)hipHostFree(batch_ll_host_);
   batch_host_ = nullptr;
   batch_ll_host_ = nullptr;
-  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_terms_, (void*)batch_ll_,
+  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_chunk_sums_, (void*)batch_ll_,
                   (void*)batch_chord_, (void*)batch_chord32_})
     deviceFree(p);
   batch_dev_ = nullptr;
-  batch_flux_ = batch_bins_ = batch_terms_ = batch_ll_ = batch_chord_ = nullptr;
+  batch_flux_ = batch_bins_ = batch_chunk_sums_ = batch_ll_ = batch_chord_ = nullptr;
   batch_chord32_ = nullptr;
   batch_capacity_ = 0;
   void* h = nullptr;
@@ -569,7 +574,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
   batch_dev_ = deviceAlloc<unsigned char>(batch * rec_bytes_, "batch records");
   batch_flux_ = deviceAlloc<double>(batch * stride_, "batch spectra");
   batch_bins_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "batch bins");
-  batch_terms_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_bins_), "batch likelihood terms");
+  batch_chunk_sums_ = deviceAlloc<double>(batch * std::max<size_t>(1, n_chunks_) * 2, "batch bin piece sums");
   batch_ll_ = deviceAlloc<double>(batch, "batch log-likelihoods");
   if (transmission_) {
     const size_t chord_size = size_t(chordSegments(int(L))) * chordTangents(int(L));
@@ -634,13 +639,14 @@ void DeviceEngine::launchBatchRest(ModelKind kind, size_t batch, size_t first) {
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                     rec_shifts_, groups_};
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_bins_), unsigned(batch)), dim3(kBlockBins), 0, s,
-                       batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bins_desc_, int(n_bins_), rec,
-                       kind == ModelKind::eclipse ? 1 : 0, batch_bins_ + first * n_bins_,
-                       batch_terms_ + first * n_bins_);
+    const int mode = kind == ModelKind::eclipse ? 1 : 0;
+    double* sums = batch_chunk_sums_ + first * n_chunks_ * 2;
+    hipLaunchKernelGGL(binBatchKernel, dim3(unsigned(n_chunks_), unsigned(batch)), dim3(kBlockBins), 0, s,
+                       batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bin_chunks_, int(n_chunks_), rec,
+                       mode, sums);
     check(hipGetLastError(), "batch bin kernel");
-    hipLaunchKernelGGL(sumBatchKernel, dim3(unsigned(batch)), dim3(1024), 0, s, batch_terms_ + first * n_bins_,
-                       int(n_bins_), batch_ll_ + first);
+    hipLaunchKernelGGL(sumBatchKernel, dim3(unsigned(batch)), dim3(1024), 0, s, sums, int(n_chunks_), bins_desc_,
+                       int(n_bins_), rec, mode, batch_bins_ + first * n_bins_, batch_ll_ + first);
     check(hipGetLastError(), "batch likelihood kernel");
     check(hipMemcpyAsync(batch_ll_host_ + first, batch_ll_ + first, batch * sizeof(double), hipMemcpyDeviceToHost,
                          s),
@@ -806,7 +812,7 @@ void DeviceEngine::releaseBatch() {
   releaseGraph();
   if (batch_host_) (void)hipHostFree(batch_host_);
   if (batch_ll_host_) (void)hipHostFree(batch_ll_host_);
-  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_terms_, (void*)batch_ll_,
+  for (void* p : {(void*)batch_dev_, (void*)batch_flux_, (void*)batch_bins_, (void*)batch_chunk_sums_, (void*)batch_ll_,
                   (void*)batch_chord_, (void*)batch_chord32_, tables_low_, (void*)table_base_})
     deviceFree(p);
 }

@@ -308,36 +308,36 @@ __device__ __forceinline__ void tangentBlock(const K* __restrict__ seg, int tile
   for (int k = 0; k < kTangentBlock; ++k) out[k] = fin[k] != K(0) ? fin[k] : tau[k];
 }
 
-// Bins: one workgroup of kBlockBins threads per bin.  The dot product of the
-// spectrum with the bin's folded weights (strided per thread, then a fixed
-// tree); mode 1 (eclipse) also folds the blackbody star's flux and returns
-// planet / star (Rp/Rs)^2 1e6 (secondary_eclipse.cpp:231-246).  Then the
-// observation's shift (forward_model.cpp:161-180) and the bin's Gaussian
-// log-likelihood term (retrieval.cpp:330-358).
+// Bins (secondary_eclipse.cpp:231-246, forward_model.cpp:161-180,
+// retrieval.cpp:330-358), in two steps:
+//  1. binChunkBody, one workgroup of kBlockBins threads per piece of a bin's
+//     folded segment: the dot product of the spectrum with the weights (four
+//     products per thread per round, so that a piece costs one or two memory
+//     latencies), and with mode 1 (eclipse) the folded blackbody star; a
+//     fixed tree to out[0], out[1];
+//  2. binFinalizeBody, one workgroup of 1024 threads per evaluation: per bin
+//     its pieces' sums in order, the value (eclipse: planet / star
+//     (Rp/Rs)^2 1e6), the observation's shift and the Gaussian
+//     log-likelihood term, and the terms' fixed-order sum.
 constexpr int kBlockBins = 256;
 
-__device__ __forceinline__ void binBody(const double* __restrict__ flux, const double* __restrict__ wavenumber,
-                                        const double* __restrict__ weights, const BinDesc& b, int mode,
-                                        double star_inv_temperature, double depth_scale,
-                                        const double* __restrict__ shifts, double error_inflation,
-                                        double* __restrict__ out, double* __restrict__ term) {
+__device__ __forceinline__ void binChunkBody(const double* __restrict__ flux, const double* __restrict__ wavenumber,
+                                             const double* __restrict__ weights, const BinChunk& c, int mode,
+                                             double star_inv_temperature, double* __restrict__ out) {
   __shared__ double partial[2][kBlockBins / 64];
-  const double* w = weights + b.weight_offset;
-  const double* f = flux + b.start;
-  // four independent products per thread per round, so that a bin of a few
-  // thousand wavenumbers costs one or two memory latencies, not one per
-  // kBlockBins elements
+  const double* w = weights + c.weight_offset;
+  const double* f = flux + c.start;
   double a[4] = {0.0, 0.0, 0.0, 0.0}, acc_star = 0.0;
   uint32_t j = threadIdx.x;
-  for (; j + 3 * kBlockBins < b.length; j += 4 * kBlockBins) {
+  for (; j + 3 * kBlockBins < c.length; j += 4 * kBlockBins) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) a[u] = fma(w[j + u * kBlockBins], f[j + u * kBlockBins], a[u]);
   }
-  for (; j < b.length; j += kBlockBins) a[0] = fma(w[j], f[j], a[0]);
+  for (; j < c.length; j += kBlockBins) a[0] = fma(w[j], f[j], a[0]);
   double acc = (a[0] + a[1]) + (a[2] + a[3]);
   if (mode == 1)
-    for (uint32_t k = threadIdx.x; k < b.length; k += kBlockBins)
-      acc_star += w[k] * starFlux(wavenumber[b.start + k], star_inv_temperature);
+    for (uint32_t k = threadIdx.x; k < c.length; k += kBlockBins)
+      acc_star += w[k] * starFlux(wavenumber[c.start + k], star_inv_temperature);
   acc = waveSum(acc);
   acc_star = waveSum(acc_star);
   if ((threadIdx.x & 63) == 0) {
@@ -346,14 +346,45 @@ __device__ __forceinline__ void binBody(const double* __restrict__ flux, const d
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double v = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
-    if (mode == 1) v = v / ((partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3])) * depth_scale;
+    out[0] = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
+    out[1] = (partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3]);
+  }
+}
+
+// Step 2 for n bins with pieces' sums chunk_sums[piece][2]: bin values to
+// out[bin], the log-likelihood to *ll (and *ll_host when given).
+__device__ __forceinline__ void binFinalizeBody(const double* __restrict__ chunk_sums,
+                                                const BinDesc* __restrict__ bins, int n, int mode,
+                                                double depth_scale, const double* __restrict__ shifts,
+                                                double error_inflation, double* __restrict__ out,
+                                                double* __restrict__ ll, double* __restrict__ ll_host) {
+  __shared__ double partial[16];
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < n; j += 1024) {
+    const BinDesc b = bins[j];
+    const uint32_t pieces = (b.length + kBinChunk - 1) / kBinChunk;
+    const double* cs = chunk_sums + size_t(b.first_chunk) * 2;
+    double v = cs[0], star = cs[1];
+    for (uint32_t k = 1; k < pieces; ++k) {
+      v += cs[2 * k];
+      star += cs[2 * k + 1];
+    }
+    if (mode == 1) v = v / star * depth_scale;
     const double shift = shifts[b.observation];
     if (shift != 0) v += shift;
-    *out = v;
+    out[j] = v;
     const double e2 = b.error * b.error + error_inflation;
     const double d = b.data - v;
-    *term = (-0.5 * log(e2 * 2.0 * kPi) - 0.5 * d * d / e2) * b.weight;
+    acc += (-0.5 * log(e2 * 2.0 * kPi) - 0.5 * d * d / e2) * b.weight;
+  }
+  acc = waveSum(acc);
+  if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < 16; ++k) s += partial[k];
+    *ll = s;
+    if (ll_host) *ll_host = s;
   }
 }
 
@@ -391,21 +422,6 @@ __device__ __forceinline__ bool workItem(size_t item, int batch, int tiles, int&
 }
 
 __host__ __device__ inline size_t nbItems(int tiles, int batch) { return size_t((tiles + 7) / 8) * 8 * size_t(batch); }
-
-// Fixed-order sum of n terms by one workgroup of 1024 threads.
-__device__ __forceinline__ void rowSum(const double* __restrict__ terms, int n, double* __restrict__ out) {
-  __shared__ double partial[16];
-  double acc = 0.0;
-  for (int j = threadIdx.x; j < n; j += 1024) acc += terms[j];
-  acc = waveSum(acc);
-  if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < 16; ++k) s += partial[k];
-    *out = s;
-  }
-}
 
 }  // namespace dev
 }  // namespace bear_amd

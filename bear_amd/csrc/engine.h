@@ -131,14 +131,24 @@ struct BatchLaunch {
 };
 
 // Per bin of every observation, in observation order: the folded operator
-// segment and the likelihood inputs.
+// segment and the likelihood inputs.  The segment is summed in pieces of at
+// most kBinChunk weights, BinChunk records first_chunk .. first_chunk +
+// ceil(length / kBinChunk) - 1, so that one wide band (a photometric filter
+// over a million wavenumbers) is summed by many workgroups.
+constexpr uint32_t kBinChunk = 2048;
 struct BinDesc {
   uint64_t weight_offset;
   uint32_t start;
   uint32_t length;
   uint32_t observation;
-  uint32_t pad;
+  uint32_t first_chunk;
   double data, error, weight;
+};
+
+struct BinChunk {
+  uint64_t weight_offset;
+  uint32_t start;
+  uint32_t length;
 };
 
 class DeviceEngine {
@@ -244,7 +254,7 @@ class DeviceEngine {
 
   int device_ = 0;
   void* stream_ = nullptr;
-  size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0, n_rows_ = 0;
+  size_t n_points_ = 0, stride_ = 0, n_bins_ = 0, n_obs_ = 0, n_rows_ = 0, n_chunks_ = 0;
   size_t nb_levels_ = 0, nb_species_ = 0, species_stride_ = 0;
   bool transmission_ = false;
   StencilGroups groups_;
@@ -257,7 +267,8 @@ class DeviceEngine {
   double* weights_ = nullptr;
   BinDesc* bins_desc_ = nullptr;
   double* bins_ = nullptr;
-  double* ll_terms_ = nullptr;
+  BinChunk* bin_chunks_ = nullptr;               // [n_chunks_]
+  double* chunk_sums_ = nullptr;                 // [n_chunks_][2]
   double* ll_ = nullptr;
   std::vector<size_t> obs_bin_offset_;
   // per-evaluation inputs: one pinned block, one device block
@@ -304,7 +315,7 @@ class DeviceEngine {
   unsigned char* batch_dev_ = nullptr;
   double* batch_flux_ = nullptr;                 // [capacity][stride]
   double* batch_bins_ = nullptr;                 // [capacity][bins]
-  double* batch_terms_ = nullptr;                // [capacity][bins]
+  double* batch_chunk_sums_ = nullptr;           // [capacity][pieces][2]
   double* batch_ll_ = nullptr;                   // [capacity]
   double* batch_ll_host_ = nullptr;              // pinned [capacity]
   double* batch_chord_ = nullptr;                // transmission: [capacity][L][L] FP64

@@ -221,21 +221,25 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
   }
 }
 
-// One workgroup per bin (binBody).
-__global__ __launch_bounds__(kBlockBins) void binKernel(
-    const double* __restrict__ flux, const double* __restrict__ wavenumber, const double* __restrict__ weights,
-    const BinDesc* __restrict__ bins, const double* __restrict__ shifts, int mode, double star_inv_temperature,
-    double depth_scale, double error_inflation, double* __restrict__ out, double* __restrict__ ll_terms) {
-  binBody(flux, wavenumber, weights, bins[blockIdx.x], mode, star_inv_temperature, depth_scale, shifts,
-          error_inflation, out + blockIdx.x, ll_terms + blockIdx.x);
+// One workgroup per piece of a bin's segment (binChunkBody).
+__global__ __launch_bounds__(kBlockBins) void binKernel(const double* __restrict__ flux,
+                                                        const double* __restrict__ wavenumber,
+                                                        const double* __restrict__ weights,
+                                                        const BinChunk* __restrict__ chunks, int mode,
+                                                        double star_inv_temperature, double* __restrict__ chunk_sums) {
+  binChunkBody(flux, wavenumber, weights, chunks[blockIdx.x], mode, star_inv_temperature,
+               chunk_sums + size_t(blockIdx.x) * 2);
 }
 
-// Fixed-order sum of the per-bin terms in one workgroup; also stored to the
-// pinned host word the caller reads after the stream's synchronisation
-__global__ __launch_bounds__(1024) void sumKernel(const double* __restrict__ terms, int n, double* __restrict__ out,
-                                                  double* __restrict__ host_out) {
-  rowSum(terms, n, out);
-  if (threadIdx.x == 0) *host_out = *out;
+// Bins and the log-likelihood in one workgroup (binFinalizeBody); the sum is
+// also stored to the pinned host word the caller reads after the stream's
+// synchronisation
+__global__ __launch_bounds__(1024) void sumKernel(const double* __restrict__ chunk_sums,
+                                                  const BinDesc* __restrict__ bins, int n, int mode,
+                                                  double depth_scale, const double* __restrict__ shifts,
+                                                  double error_inflation, double* __restrict__ out,
+                                                  double* __restrict__ ll, double* __restrict__ ll_host) {
+  binFinalizeBody(chunk_sums, bins, n, mode, depth_scale, shifts, error_inflation, out, ll, ll_host);
 }
 
 template <class T>
@@ -275,7 +279,7 @@ DeviceEngine::~DeviceEngine() {
   releaseBatch();
   releaseAdjoint();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
-                  (void*)weights_, (void*)bins_desc_, (void*)bins_, (void*)ll_terms_, (void*)ll_,
+                  (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_, (void*)chunk_sums_, (void*)ll_,
                   (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
@@ -394,6 +398,7 @@ void DeviceEngine::uploadRayleigh(const std::vector<double>& sigma) {
 void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) {
   check(hipSetDevice(device_), "hipSetDevice");
   std::vector<BinDesc> desc;
+  std::vector<BinChunk> chunks;
   std::vector<double> weights;
   obs_bin_offset_.clear();
   for (size_t o = 0; o < obs.size(); ++o) {
@@ -408,6 +413,9 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
       d.data = s.data[b];
       d.error = s.error[b];
       d.weight = s.weight[b];
+      d.first_chunk = uint32_t(chunks.size());
+      for (uint32_t k = 0; k < d.length; k += kBinChunk)
+        chunks.push_back(BinChunk{d.weight_offset + k, d.start + k, std::min(kBinChunk, d.length - k)});
       weights.insert(weights.end(), s.segment_weight.begin() + s.segment_offset[b],
                      s.segment_weight.begin() + s.segment_offset[b] + s.segment_length[b]);
       desc.push_back(d);
@@ -415,20 +423,27 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
   }
   obs_bin_offset_.push_back(desc.size());
   n_bins_ = desc.size();
+  n_chunks_ = chunks.size();
   n_obs_ = obs.size();
   deviceFree(weights_);
   deviceFree(bins_desc_);
+  deviceFree(bin_chunks_);
   deviceFree(bins_);
-  deviceFree(ll_terms_);
+  deviceFree(chunk_sums_);
+  batch_capacity_ = 0;                                     // the batch's piece sums are sized by the pieces
   weights_ = deviceAlloc<double>(weights.size(), "observation operator");
   bins_desc_ = deviceAlloc<BinDesc>(desc.size(), "bin descriptors");
+  bin_chunks_ = deviceAlloc<BinChunk>(chunks.size(), "bin pieces");
   bins_ = deviceAlloc<double>(n_bins_, "bins");
-  ll_terms_ = deviceAlloc<double>(n_bins_, "likelihood terms");
+  chunk_sums_ = deviceAlloc<double>(n_chunks_ * 2, "bin piece sums");
   hipStream_t s = static_cast<hipStream_t>(stream_);
   check(hipMemcpyAsync(weights_, weights.data(), weights.size() * sizeof(double), hipMemcpyHostToDevice, s),
         "copy weights");
   check(hipMemcpyAsync(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, s),
         "copy bins");
+  if (!chunks.empty())
+    check(hipMemcpyAsync(bin_chunks_, chunks.data(), chunks.size() * sizeof(BinChunk), hipMemcpyHostToDevice, s),
+          "copy bin pieces");
   // the operator transposed, per wavenumber (bins in increasing order), for the adjoint
   std::vector<uint64_t> ptr(n_points_ + 1, 0);
   for (const BinDesc& d : desc)
@@ -511,11 +526,11 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const double* shifts = reinterpret_cast<const double*>(staging_dev_ + off_shift_);
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_bins_)), dim3(kBlockBins), 0, s, flux_, wavenumber_, weights_,
-                       bins_desc_, shifts, mode, mode == 1 ? 1.0 / star_temperature : 0.0, depth_scale, inflation,
-                       bins_, ll_terms_);
+    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_chunks_)), dim3(kBlockBins), 0, s, flux_, wavenumber_, weights_,
+                       bin_chunks_, mode, mode == 1 ? 1.0 / star_temperature : 0.0, chunk_sums_);
     check(hipGetLastError(), "bin kernel");
-    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, ll_terms_, int(n_bins_), ll_, ll_host_dev_);
+    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, chunk_sums_, bins_desc_, int(n_bins_), mode,
+                       depth_scale, shifts, inflation, bins_, ll_, ll_host_dev_);
     check(hipGetLastError(), "likelihood kernel");
   }
   if (bins_dev)
