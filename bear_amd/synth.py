@@ -503,7 +503,26 @@ def eclipse_c5(seed=61, backend="numpy", n_wavenumbers=1_000_000):
                 extra={"stellar_model": "blackbody"})
 
 
+def wide_bands(case: Case, seed, bands):
+    """The case with broad photometric bands added: [(lo, hi, detector)] in
+    micron, each covering thousands of grid points (bins whose folded segment
+    is summed in several pieces)."""
+    rng = np.random.default_rng(seed)
+    for k, (lo, hi, det) in enumerate(bands):
+        w, r = bandpass(rng, lo, hi, 50)
+        ref = float(np.median(np.abs(case.observations[0].data)))
+        case.observations.append(ObservationSpec(f"wide{k}", "photometry", bin_edges=np.array([[lo, hi]]),
+                                                 data=np.array([ref]), error=np.array([0.05 * ref]),
+                                                 filter=(det, w, r)))
+    case.name = case.name + "_wide"
+    return case
+
+
 CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2,
+         # emission_c1 grid: 1000-10000 cm^-1 at 0.45; bands of 14,800 and 2,560 points
+         "emission_wide": lambda: wide_bands(emission_c1(), 71, [(1.25, 7.5, "Photon"), (2.0, 2.6, "Energy")]),
+         # eclipse_small: 2500-7000 cm^-1 sampled every 2 cm^-1; a band of 2,180 points
+         "eclipse_wide": lambda: wide_bands(eclipse_small(), 73, [(1.45, 3.95, "Energy")]),
          "transmission_plain": lambda: transmission_small(variant="plain"),
          "transmission_variable_gravity": lambda: transmission_small(variant="variable_gravity"),
          "transmission_fit_mmw": lambda: transmission_small(variant="fit_mmw"),

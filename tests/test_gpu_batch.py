@@ -61,7 +61,7 @@ def oracle_bins(o, ref):
 
 
 @pytest.mark.parametrize("name", ["tiny", "emission_c1", "eclipse_small", "transmission_plain",
-                                  "transmission_fit_scale_height"])
+                                  "transmission_fit_scale_height", "emission_wide", "eclipse_wide"])
 def test_fp64_batch_matches_oracle_and_single_calls(case_factory, name):
     folder, pts = setup(case_factory, name, 5)
     r, o = opened(folder)

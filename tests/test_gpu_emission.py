@@ -41,7 +41,7 @@ def test_grid_matches_oracle(case_factory, name):
     assert r.nb_observation_points == o.nb_bins
 
 
-@pytest.mark.parametrize("name", ["tiny", "emission_c1"])
+@pytest.mark.parametrize("name", ["tiny", "emission_c1", "emission_wide"])
 def test_emission_parity(case_factory, name):
     r, o, params, _ = retrieval(case_factory, name)
     for k, p in enumerate(params):

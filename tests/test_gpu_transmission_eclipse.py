@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 SPECTRUM_RTOL = 1e-5
 LOGLIKE_ATOL = 1e-6
 
-MODELS = ["eclipse_small", "transmission_plain", "transmission_variable_gravity", "transmission_fit_mmw",
-          "transmission_fit_scale_height"]
+MODELS = ["eclipse_small", "eclipse_wide", "transmission_plain", "transmission_variable_gravity",
+          "transmission_fit_mmw", "transmission_fit_scale_height"]
 
 
 @pytest.mark.parametrize("name", MODELS)
