@@ -161,7 +161,7 @@ template <int NS, bool kGradientExp = false, class F>
 __device__ __forceinline__ void slotTerms(const LevelSpecies* __restrict__ d, const LevelSpeciesAdjoint* __restrict__ a,
                                           const LevelState& lev, StencilGroups groups, const double (&v)[NS][4],
                                           F&& f) {
-  auto term = [](double x) { return kGradientExp ? pow2Gradient(x) : pow2Lean(x); };
+  auto term = [](double x) { return kGradientExp ? pow2Gradient(x) : pow2Opacity(x); };
   if (groups.grouped) {
     double wa[4], wb[4], ua[4], ub[4];
 #pragma unroll

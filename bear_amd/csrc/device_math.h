@@ -24,10 +24,11 @@ constexpr double kLn2 = 0.693147180559945309417232121458176568075500134;
 
 // 2^t in FP64 with relative error below 3.1e-15: t = n + f, |f| <= 1/2,
 // 2^f from its degree-10 Chebyshev interpolant on [-1/2, 1/2] (coefficients
-// in monomial form), then an exponent shift.  t is clamped so that the shift
-// stays in range: results below 2^-1100 flush to 0, above 2^1100 become inf.
+// in monomial form), then an exponent shift.  For t <= 0 (the attenuation
+// and transmittance factors e^-tau); t is clamped below so that the shift
+// stays in range: results below 2^-1100 flush to 0.
 __device__ __forceinline__ double pow2(double t) {
-  t = fmin(fmax(t, -1100.0), 1100.0);
+  t = fmax(t, -1100.0);
   const double n = __builtin_rint(t);
   const double f = t - n;
   double p = 6.767963482574983e-09;
@@ -54,6 +55,23 @@ __device__ __forceinline__ double pow2(double t) {
 // physics (sigma n < 2^1000, Planck arguments <= 0).
 __device__ __forceinline__ double pow2Lean(double t) {
   t = fmax(t, -1100.0);
+  const double n = __builtin_rint(t);
+  const double f = t - n;
+  double p = 1.3255225648015585e-06;
+  p = fma(p, f, 1.5310079150997826e-05);
+  p = fma(p, f, 0.00015403455848948458);
+  p = fma(p, f, 0.001333345056925233);
+  p = fma(p, f, 0.009618129159399412);
+  p = fma(p, f, 0.055504109412040106);
+  p = fma(p, f, 0.24022650695813164);
+  p = fma(p, f, 0.6931471805459343);
+  p = fma(p, f, 0.9999999999999999);
+  return __builtin_amdgcn_ldexp(p, int(n));
+}
+
+// pow2Lean for the opacity terms 2^(log2 n + sum_k w_k v_k), whose argument
+// is finite and above -2300 (kZeroDensityLog2 for a zero density): no clamp
+__device__ __forceinline__ double pow2Opacity(double t) {
   const double n = __builtin_rint(t);
   const double f = t - n;
   double p = 1.3255225648015585e-06;
@@ -159,7 +177,7 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
       x = fma(one ? b[1] : a[1], v[s][1], x);
       x = fma(one ? b[2] : a[2], v[s][2], x);
       x = fma(one ? b[3] : a[3], v[s][3], x);
-      k += pow2Lean(x);
+      k += pow2Opacity(x);
     }
   } else {
 #pragma unroll
@@ -168,7 +186,7 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
       x = fma(d[s].w[1], v[s][1], x);
       x = fma(d[s].w[2], v[s][2], x);
       x = fma(d[s].w[3], v[s][3], x);
-      k += pow2Lean(x);
+      k += pow2Opacity(x);
     }
   }
   return k;

@@ -35,6 +35,12 @@ struct LevelState {
   double group_u[2][4];
 };
 
+// log2 of a zero effective density in the FP64 stencil records: finite, so
+// that the kernels' exponentials need no clamp, and low enough that
+// 2^(kZeroDensityLog2 + log2 sigma) underflows to exactly 0 for any
+// cross-section below 2^900 cm^2.
+constexpr double kZeroDensityLog2 = -2000.0;
+
 // Transmission geometry (transmission_spectrum.cpp:107-154): tangent path t
 // (t = 0 .. L-2) crosses the segments q = t .. L-2 (level q to q+1) with
 // chord lengths c[t][q].  The device keeps them transposed, chord[q][t]
@@ -65,8 +71,9 @@ struct StencilGroups {
 //   sigma n = 2^(log2_n + sum_k w_k v_k),
 // v the four stencil rows' log10 cross-sections, w the bilinear weights
 // log2(10) {(1-f_t)(1-f_p-), (1-f_t) f_p-, f_t (1-f_p+), f_t f_p+}
-// (opacity_species.cpp:277-345), log2_n = -inf for n == 0, which removes the
-// species' term as opacity_species.cpp:409-412 skips it.  block[k] locates
+// (opacity_species.cpp:277-345), log2_n = kZeroDensityLog2 for n == 0, whose
+// term 2^(log2_n + ...) is exactly 0, which removes the species' term as
+// opacity_species.cpp:409-412 skips it.  block[k] locates
 // stencil row k in the tables: row * stride / 64 (the stride is a multiple of
 // 64), so an address is one shift.  reload != 0 when the stencil rows differ
 // from the previous level's (always at level 0); the kernels keep the four

@@ -322,7 +322,7 @@ void DeviceEngine::countRows() {
   size_t count = 0;
   for (size_t k = 0; k < nb_levels_ * species_stride_; ++k) {
     const LevelSpecies& d = species_host_[k];
-    if (std::isinf(d.log2_n) && d.log2_n < 0) continue;            // zero density: term removed
+    if (d.log2_n <= kZeroDensityLog2) continue;                     // zero density: term removed
     uint32_t r[4];
     for (int q = 0; q < 4; ++q) r[q] = uint32_t(size_t(d.block[q]) * 64 / stride_);
     const bool use[4] = {true, r[1] != r[0], r[2] != r[0], r[2] != r[0] && r[3] != r[2]};

@@ -128,7 +128,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
         d.w[1] = (1.0 - ft) * fl * log2_ten;
         d.w[2] = ft * (1.0 - fu) * log2_ten;
         d.w[3] = ft * fu * log2_ten;
-        d.log2_n = n > 0 ? std::log2(n) : -std::numeric_limits<double>::infinity();
+        d.log2_n = n > 0 ? std::log2(n) : kZeroDensityLog2;
         if (s < 16) lev[l].log2_n[s] = d.log2_n;
         for (int k = 0; k < 4; ++k) d.block[k] = block[k];
         d.reload = changed;
