@@ -382,7 +382,7 @@ __device__ __forceinline__ void binFinalizeBody(const double* __restrict__ chunk
     const BinDesc b = bins[j];
     const uint32_t pieces = (b.length + kBinChunk - 1) / kBinChunk;
     const double* cs = chunk_sums + size_t(b.first_chunk) * 2;
-    double v = cs[0], star = cs[1];
+    double v = pieces > 0 ? cs[0] : 0.0, star = pieces > 0 ? cs[1] : 0.0;       // an empty segment sums to 0
     for (uint32_t k = 1; k < pieces; ++k) {
       v += cs[2 * k];
       star += cs[2 * k + 1];
