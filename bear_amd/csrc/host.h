@@ -170,6 +170,10 @@ class PiecewisePolynomial {
   void setValues(const std::vector<double>& dof_values);
   double value(double x) const;
   size_t nbDof() const { return vertices_.size(); }
+  // value(x) = sum_k dof[index[k]] weight[k], the same products summed in
+  // the same order as value() does, for an x fixed in advance
+  struct Basis { std::vector<size_t> index; std::vector<double> weight; };
+  Basis basis(double x) const;
  private:
   struct Element { std::vector<double> ref, vertex, value; };
   std::vector<Element> elements_;
@@ -217,6 +221,10 @@ class Atmosphere {
   bool isothermal_ = false;
   size_t nb_temperature_param_ = 0, nb_chemistry_param_ = 0;
   std::vector<ChemistryModule> chemistry_;
+  // ascending species ids (from 1) that a chemistry module can make non-zero:
+  // the sums over species skip the others, whose terms are exact zeros
+  std::vector<int> active_ids_;
+  std::vector<PiecewisePolynomial::Basis> temperature_basis_;   // per level, for the fixed pressures
   std::vector<char> clamped_;               // levels floored at 50 K by the last compute()
   bool bg_clamped_ = false;
 };

@@ -66,6 +66,32 @@ double PiecewisePolynomial::value(double x) const {
   return y;
 }
 
+PiecewisePolynomial::Basis PiecewisePolynomial::basis(double x) const {
+  Basis b;
+  if (x > vertices_.front() || x < vertices_.back()) return b;                  // outside: 0
+  for (size_t i = 0; i < vertices_.size(); ++i)
+    if (vertices_[i] == x) {
+      b.index.push_back(i);
+      b.weight.push_back(1.0);
+      return b;
+    }
+  size_t k = 0;
+  for (size_t i = 0; i < elements_.size(); ++i)
+    if (elements_[i].vertex.front() > x && elements_[i].vertex.back() < x) { k = i; break; }
+  const Element& e = elements_[k];
+  size_t first = 0;                                  // the element's first degree of freedom (shared ends)
+  for (size_t i = 0; i < k; ++i) first += elements_[i].value.size() - 1;
+  const double r = 2.0 * (x - e.vertex.front()) / (e.vertex.back() - e.vertex.front()) - 1.0;
+  for (size_t i = 0; i < e.ref.size(); ++i) {
+    double l = 1;
+    for (size_t j = 0; j < e.ref.size(); ++j)
+      if (i != j) l *= (r - e.ref[j]) / (e.ref[i] - e.ref[j]);
+    b.index.push_back(first + i);
+    b.weight.push_back(l);
+  }
+  return b;
+}
+
 // ---------------------------------------------------------------- atmosphere
 Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& temperature_desc,
                        const std::string& chemistry_desc)
@@ -87,6 +113,7 @@ Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& t
     if (!(t >> nb_el >> order)) raise(kInvalidInput, "the poly temperature profile needs two parameters");
     poly_.reset(new PiecewisePolynomial(nb_el, order, bottom, top));
     nb_temperature_param_ = nb_el * order + 1;
+    for (size_t l = 0; l < L; ++l) temperature_basis_.push_back(poly_->basis(std::log10(pressure[l])));
     if (poly_->nbDof() != nb_temperature_param_)
       raise(kInvalidInput, "inconsistent piecewise-polynomial degrees of freedom");
   } else if (kind == "const") {
@@ -116,6 +143,14 @@ Atmosphere::Atmosphere(size_t L, double bottom, double top, const std::string& t
     }
     chemistry_.push_back(m);
   }
+  std::vector<char> can_be_set(n_ids_, 0);
+  for (const ChemistryModule& m : chemistry_) {
+    for (int s : m.species) can_be_set[size_t(s)] = 1;
+    if (!m.background && !m.sodium_free) can_be_set[sid::Na] = 1;
+    if (m.background) can_be_set[sid::H2] = can_be_set[sid::He] = 1;
+  }
+  for (size_t j = 1; j < n_ids_; ++j)
+    if (can_be_set[j]) active_ids_.push_back(int(j));
   temperature.assign(L, 0.0);
   altitude.assign(L, 0.0);
   mean_molecular_weight.assign(L, 0.0);
@@ -134,11 +169,12 @@ Atmosphere::Atmosphere(const Atmosphere& o)
       isothermal_(o.isothermal_),
       nb_temperature_param_(o.nb_temperature_param_),
       nb_chemistry_param_(o.nb_chemistry_param_),
-      chemistry_(o.chemistry_) {}
+      chemistry_(o.chemistry_),
+      active_ids_(o.active_ids_),
+      temperature_basis_(o.temperature_basis_) {}
 
 bool Atmosphere::compute(double g, const double* tp, const double* cp, const AltitudeOptions& opt) {
   const size_t L = nb_levels;
-  const int S = int(n_ids_);
   bool neglect = false;
   // temperature (piecewise_poly_temperature.cpp:49-84; constant_temperature.h:34-46)
   if (isothermal_) {
@@ -147,8 +183,12 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
     std::vector<double> dof(nb_temperature_param_);
     dof[0] = tp[0];
     for (size_t i = 1; i < dof.size(); ++i) dof[i] = dof[i - 1] * tp[i];
-    poly_->setValues(dof);
-    for (size_t i = 0; i < L; ++i) temperature[i] = poly_->value(std::log10(pressure[i]));
+    for (size_t i = 0; i < L; ++i) {
+      const PiecewisePolynomial::Basis& b = temperature_basis_[i];
+      double y = 0;                                  // as PiecewisePolynomial::value, weights precomputed
+      for (size_t k = 0; k < b.index.size(); ++k) y += dof[b.index[k]] * b.weight[k];
+      temperature[i] = y;
+    }
   }
   // the polynomial profile floors at 50 K and flags the model
   // (piecewise_poly_temperature.cpp:77-81); the isothermal one does not
@@ -162,10 +202,13 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
   // background_chemistry.cpp:72-137, chemistry.h:47-76)
   std::fill(number_density.begin(), number_density.end(), 0.0);
   auto n = [&](size_t l, int id) -> double& { return number_density[l * n_ids_ + id]; };
+  // the species sums run over active_ids_ (and the total at j = 0 for the
+  // mean weight): every other term is an exact zero
+  const double w_total = molecularWeight(sid::total);
   auto meanWeight = [&]() {
     for (size_t l = 0; l < L; ++l) {
-      double mu = 0;
-      for (int j = 0; j < S; ++j) mu += n(l, j) / n(l, sid::total) * molecularWeight(j);
+      double mu = n(l, sid::total) / n(l, sid::total) * w_total;
+      for (int j : active_ids_) mu += n(l, j) / n(l, sid::total) * molecularWeight(j);
       mean_molecular_weight[l] = mu;
     }
   };
@@ -181,7 +224,7 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
       meanWeight();
       for (size_t l = 0; l < L; ++l) {
         double sum = 0;
-        for (int j = 1; j < S; ++j) sum += n(l, j) / n(l, sid::total);
+        for (int j : active_ids_) sum += n(l, j) / n(l, sid::total);
         if (sum > 1 + 1e-10) { neglect = true; break; }
       }
     } else {
@@ -189,7 +232,7 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
       for (size_t l = 0; l < L; ++l) n(l, sid::total) = pressure[l] * 1.e6 / phys::boltzmann_k / temperature[l];
       for (size_t l = 0; l < L; ++l) {
         double bg = 1.0;
-        for (int j = 1; j < S; ++j)
+        for (int j : active_ids_)
           if (j != sid::H2 && j != sid::He) bg -= n(l, j) / n(l, sid::total);
         if (bg < 0) { neglect = true; bg = 0; bg_clamped_ = true; }
         n(l, sid::H2) = n(l, sid::total) * (bg * h2 / h2he);
