@@ -142,6 +142,11 @@ class OpacitySpecies {
 
   bool hasTable() const { return !by_temperature.empty(); }
   Stencil bracket(double pressure, double temperature) const;
+  // the pressure half of bracket() done once for the atmosphere's fixed
+  // pressures (per level and temperature row), then bracket(pressures[level],
+  // temperature) by bracketLevel with the same results
+  void preparePressures(const std::vector<double>& pressures);
+  Stencil bracketLevel(size_t level, double temperature) const;
   // the rows of this species at a stencil bracketed on a species with the same grid
   void takeRows(Stencil& st) const {
     for (int k = 0; k < 4; ++k) st.row[k] = by_temperature[st.t_pos[k / 2]][st.p_pos[k]].id;
@@ -149,6 +154,11 @@ class OpacitySpecies {
   // same temperatures and, per temperature, the same pressures: bracket()
   // then gives the same fractions for every (p, T)
   bool sameGrid(const OpacitySpecies& other) const;
+
+ private:
+  struct PressurePick { uint16_t lower, upper; double fraction; };
+  std::vector<PressurePick> pressure_picks_;          // [level][temperature row]
+  void bracketTemperature(double t, size_t& lo_i, size_t& hi_i) const;
 };
 
 // All species' sampled rows, log10 cross-sections in FP64, [rows][N].

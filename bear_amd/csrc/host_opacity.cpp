@@ -193,8 +193,8 @@ bool OpacitySpecies::sameGrid(const OpacitySpecies& o) const {
   return true;
 }
 
-Stencil OpacitySpecies::bracket(double p, double t) const {
-  size_t lo_i = 0, hi_i = 0;
+void OpacitySpecies::bracketTemperature(double t, size_t& lo_i, size_t& hi_i) const {
+  lo_i = hi_i = 0;
   if (t < by_temperature.front().front().temperature) {
   } else if (t > by_temperature.back().front().temperature) {
     lo_i = hi_i = by_temperature.size() - 1;
@@ -209,22 +209,35 @@ Stencil OpacitySpecies::bracket(double p, double t) const {
       }
     }
   }
-  auto pick = [p](const std::vector<TableRow>& rows, size_t& a, size_t& b) {
-    if (p < rows.front().pressure) { a = b = 0; return; }
-    if (p > rows.back().pressure) { a = b = rows.size() - 1; return; }
-    for (size_t i = 0; i < rows.size(); ++i) {
-      if (rows[i].pressure == p) { a = b = i; return; }
-      if (rows[i].pressure < p && rows[i + 1].pressure > p) { a = i; b = i + 1; return; }
-    }
+}
+
+namespace {
+// the two rows of one temperature bracketing p, and the log-pressure fraction
+void pickPressure(const std::vector<TableRow>& rows, double p, double lp, size_t& a, size_t& b, double& f) {
+  if (p < rows.front().pressure) a = b = 0;
+  else if (p > rows.back().pressure) a = b = rows.size() - 1;
+  else {
     a = b = 0;
-  };
+    for (size_t i = 0; i < rows.size(); ++i) {
+      if (rows[i].pressure == p) { a = b = i; break; }
+      if (rows[i].pressure < p && rows[i + 1].pressure > p) { a = i; b = i + 1; break; }
+    }
+  }
+  f = a != b ? (lp - rows[a].log_pressure) / (rows[b].log_pressure - rows[a].log_pressure) : 0.0;
+}
+}  // namespace
+
+Stencil OpacitySpecies::bracket(double p, double t) const {
+  size_t lo_i, hi_i;
+  bracketTemperature(t, lo_i, hi_i);
+  const double lp = std::log10(p);
+  size_t p0, p1, p2, p3;
+  Stencil s;
+  pickPressure(by_temperature[lo_i], p, lp, p0, p1, s.f_p_lower);
+  pickPressure(by_temperature[hi_i], p, lp, p2, p3, s.f_p_upper);
   const std::vector<TableRow>& lo = by_temperature[lo_i];
   const std::vector<TableRow>& hi = by_temperature[hi_i];
-  size_t p0, p1, p2, p3;
-  pick(lo, p0, p1);
-  pick(hi, p2, p3);
-  const TableRow *r0 = &lo[p0], *r1 = &lo[p1], *r2 = &hi[p2], *r3 = &hi[p3];
-  Stencil s;
+  const TableRow *r0 = &lo[p0], *r2 = &hi[p2];
   s.t_pos[0] = uint16_t(lo_i);
   s.t_pos[1] = uint16_t(hi_i);
   s.p_pos[0] = uint16_t(p0);
@@ -232,12 +245,50 @@ Stencil OpacitySpecies::bracket(double p, double t) const {
   s.p_pos[2] = uint16_t(p2);
   s.p_pos[3] = uint16_t(p3);
   s.row[0] = r0->id;
-  s.row[1] = r1->id;
+  s.row[1] = lo[p1].id;
   s.row[2] = r2->id;
-  s.row[3] = r3->id;
-  const double lp = std::log10(p);
-  s.f_p_lower = r0 != r1 ? (lp - r0->log_pressure) / (r1->log_pressure - r0->log_pressure) : 0.0;
-  s.f_p_upper = r2 != r3 ? (lp - r2->log_pressure) / (r3->log_pressure - r2->log_pressure) : 0.0;
+  s.row[3] = hi[p3].id;
+  s.f_t = r0 != r2 ? (t - r0->temperature) / (r2->temperature - r0->temperature) : 0.0;
+  s.dft = r0 != r2 ? 1.0 / (r2->temperature - r0->temperature) : 0.0;
+  return s;
+}
+
+void OpacitySpecies::preparePressures(const std::vector<double>& pressures) {
+  const size_t nt = by_temperature.size();
+  pressure_picks_.assign(pressures.size() * nt, PressurePick{0, 0, 0.0});
+  for (size_t l = 0; l < pressures.size(); ++l) {
+    const double p = pressures[l], lp = std::log10(p);
+    for (size_t i = 0; i < nt; ++i) {
+      size_t a, b;
+      double f;
+      pickPressure(by_temperature[i], p, lp, a, b, f);
+      pressure_picks_[l * nt + i] = PressurePick{uint16_t(a), uint16_t(b), f};
+    }
+  }
+}
+
+Stencil OpacitySpecies::bracketLevel(size_t level, double t) const {
+  size_t lo_i, hi_i;
+  bracketTemperature(t, lo_i, hi_i);
+  const size_t nt = by_temperature.size();
+  const PressurePick& pl = pressure_picks_[level * nt + lo_i];
+  const PressurePick& ph = pressure_picks_[level * nt + hi_i];
+  const std::vector<TableRow>& lo = by_temperature[lo_i];
+  const std::vector<TableRow>& hi = by_temperature[hi_i];
+  const TableRow *r0 = &lo[pl.lower], *r2 = &hi[ph.lower];
+  Stencil s;
+  s.t_pos[0] = uint16_t(lo_i);
+  s.t_pos[1] = uint16_t(hi_i);
+  s.p_pos[0] = pl.lower;
+  s.p_pos[1] = pl.upper;
+  s.p_pos[2] = ph.lower;
+  s.p_pos[3] = ph.upper;
+  s.row[0] = r0->id;
+  s.row[1] = lo[pl.upper].id;
+  s.row[2] = r2->id;
+  s.row[3] = hi[ph.upper].id;
+  s.f_p_lower = pl.fraction;
+  s.f_p_upper = ph.fraction;
   s.f_t = r0 != r2 ? (t - r0->temperature) / (r2->temperature - r0->temperature) : 0.0;
   s.dft = r0 != r2 ? 1.0 / (r2->temperature - r0->temperature) : 0.0;
   return s;

@@ -36,6 +36,7 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
   }
   engine_.uploadTables(store);
   engine_.uploadRayleigh(rayleigh);
+  for (OpacitySpecies& s : species_) s.preparePressures(atmosphere_.pressure);   // fixed pressure grid
   engine_.configure(m.nb_levels, species_.size(), transmission);
   // stencil groups: the species' table grids, at most two distinct ones
   StencilGroups groups;
@@ -74,7 +75,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     lev[l].rayleigh_density = n_r;
     for (int g = 0; g < 2; ++g) {
       if (group_rep_[g] < 0) continue;
-      const Stencil st = species_[size_t(group_rep_[g])].bracket(atm.pressure[l], t);
+      const Stencil st = species_[size_t(group_rep_[g])].bracketLevel(l, t);
       group_st[g] = st;
       const double ft = st.f_t, fl = st.f_p_lower, fu = st.f_p_upper;
       lev[l].group_w[g][0] = (1.0 - ft) * (1.0 - fl) * log2_ten;
@@ -101,7 +102,7 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
           st = group_st[(groups.group1 >> s) & 1u];
           sp.takeRows(st);
         } else {
-          st = sp.bracket(atm.pressure[l], t);                       // rows stay valid when n == 0
+          st = sp.bracketLevel(l, t);                                // rows stay valid when n == 0
         }
       }
       uint32_t block[4];
