@@ -6,7 +6,7 @@
 // Reverse order of the forward chain:
 //   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
 //     eclipse: also d logL / d P_j for the planet's binned flux and the
-//     star's terms                                              (eclipseGbarKernel)
+//     star's terms                          (eclipseSlopeKernel, eclipseGbarKernel)
 //   d logL / d F(nu) = sum_j d logL / d P_j  W_j(nu)            (fluxAdjointKernel, the
 //                      folded operator transposed, per wavenumber)
 //   short characteristics and Planck function                   (rtAdjointKernel)
@@ -86,54 +86,65 @@ __global__ __launch_bounds__(kBlock) void gbarKernel(const double* __restrict__ 
   gbar[size_t(m) * nb_bins + j] = d.weight * (d.data - bins[size_t(m) * nb_bins + j]) / e2;
 }
 
-// Secondary eclipse (binBody mode 1): depth_j = P_j / S_j (Rp/Rs)^2 1e6 (+
-// shift) with P_j = sum W_j F, S_j = sum W_j pi B(T_*) 1e-3.  One workgroup
-// per (bin, point) recomputes P_j, S_j and dS_j/dT_* and writes
-//   gbar[j]    d logL / d depth_j
-//   fscale[j]  d logL / d P_j = gbar_j (Rp/Rs)^2 1e6 / S_j
-//   terms[j]   {gbar_j (depth_j - shift), -gbar_j (depth_j - shift) dS_j/dT_* / S_j}
-// (secondary_eclipse.cpp:231-246, 262-355).
-__global__ __launch_bounds__(kBlockBins) void eclipseGbarKernel(
-    const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
-    const double* __restrict__ weights, const BinDesc* __restrict__ desc, int nb_bins, Records rec,
-    const double* __restrict__ bins, double* __restrict__ gbar, double* __restrict__ fscale,
-    double* __restrict__ terms) {
-  __shared__ double partial[3][kBlockBins / 64];
-  const int j = blockIdx.x, m = blockIdx.y;
+// Secondary eclipse (secondary_eclipse.cpp:231-246, 262-355): depth_j =
+// P_j / S_j (Rp/Rs)^2 1e6 (+ shift) with P_j = sum W_j F and S_j = sum W_j
+// pi B(T_*) 1e-3, both summed per piece of the bin's folded segment by the
+// forward bin kernel (chunk_sums[point][piece][2]).  Two steps:
+//  1. eclipseSlopeKernel, one workgroup per (piece, point): the piece's sum
+//     of W_j dB(T_*)/dT_* pi 1e-3 (so that a wide photometric band is spread
+//     over as many workgroups as it has pieces);
+//  2. eclipseGbarKernel, one thread per (bin, point): the pieces' sums in
+//     order, then
+//       gbar[j]    d logL / d depth_j
+//       fscale[j]  d logL / d P_j = gbar_j (Rp/Rs)^2 1e6 / S_j
+//       terms[j]   {gbar_j (depth_j - shift), -gbar_j (depth_j - shift) dS_j/dT_* / S_j}
+__global__ __launch_bounds__(kBlockBins) void eclipseSlopeKernel(const double* __restrict__ wavenumber,
+                                                                 const double* __restrict__ weights,
+                                                                 const BinChunk* __restrict__ chunks, int nb_chunks,
+                                                                 Records rec, double* __restrict__ slope) {
+  __shared__ double partial[kBlockBins / 64];
+  const int m = blockIdx.y;
+  const BatchModel* bm = reinterpret_cast<const BatchModel*>(rec.at(m));
+  const BinChunk c = chunks[blockIdx.x];
+  const double* w = weights + c.weight_offset;
+  double acc = 0.0;
+  for (uint32_t k = threadIdx.x; k < c.length; k += kBlockBins) {
+    double s, ds;
+    starFluxWithDerivative(wavenumber[c.start + k], bm->star_inv_temperature, s, ds);
+    acc = fma(w[k], ds, acc);
+  }
+  acc = waveSum(acc);
+  if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    slope[size_t(m) * nb_chunks + blockIdx.x] = (partial[0] + partial[1]) + (partial[2] + partial[3]);
+}
+
+__global__ __launch_bounds__(kBlock) void eclipseGbarKernel(
+    const double* __restrict__ chunk_sums, const double* __restrict__ slope, int nb_chunks,
+    const BinDesc* __restrict__ desc, int nb_bins, Records rec, const double* __restrict__ bins,
+    double* __restrict__ gbar, double* __restrict__ fscale, double* __restrict__ terms) {
+  const int j = blockIdx.x * kBlock + threadIdx.x, m = blockIdx.y;
+  if (j >= nb_bins) return;
   const BatchModel* bm = reinterpret_cast<const BatchModel*>(rec.at(m));
   const BinDesc d = desc[j];
-  const double* w = weights + d.weight_offset;
-  const double* f = flux + size_t(m) * stride + d.start;
-  double p = 0.0, st = 0.0, dst = 0.0;
-  for (uint32_t k = threadIdx.x; k < d.length; k += kBlockBins) {
-    double s, ds;
-    starFluxWithDerivative(wavenumber[d.start + k], bm->star_inv_temperature, s, ds);
-    p = fma(w[k], f[k], p);
-    st = fma(w[k], s, st);
-    dst = fma(w[k], ds, dst);
+  const uint32_t pieces = (d.length + kBinChunk - 1) / kBinChunk;
+  const double* cs = chunk_sums + (size_t(m) * nb_chunks + d.first_chunk) * 2;
+  const double* sl = slope + size_t(m) * nb_chunks + d.first_chunk;
+  double pj = 0.0, sj = 0.0, dsj = 0.0;
+  for (uint32_t k = 0; k < pieces; ++k) {
+    pj += cs[2 * k];
+    sj += cs[2 * k + 1];
+    dsj += sl[k];
   }
-  p = waveSum(p);
-  st = waveSum(st);
-  dst = waveSum(dst);
-  if ((threadIdx.x & 63) == 0) {
-    partial[0][threadIdx.x >> 6] = p;
-    partial[1][threadIdx.x >> 6] = st;
-    partial[2][threadIdx.x >> 6] = dst;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double pj = (partial[0][0] + partial[0][1]) + (partial[0][2] + partial[0][3]);
-    const double sj = (partial[1][0] + partial[1][1]) + (partial[1][2] + partial[1][3]);
-    const double dsj = (partial[2][0] + partial[2][1]) + (partial[2][2] + partial[2][3]);
-    const size_t o = size_t(m) * nb_bins + j;
-    const double e2 = d.error * d.error + bm->inflation;
-    const double g = d.weight * (d.data - bins[o]) / e2;
-    const double depth = pj / sj * bm->depth_scale;
-    gbar[o] = g;
-    fscale[o] = g * bm->depth_scale / sj;
-    terms[2 * o] = g * depth;
-    terms[2 * o + 1] = -g * depth * dsj / sj;
-  }
+  const size_t o = size_t(m) * nb_bins + j;
+  const double e2 = d.error * d.error + bm->inflation;
+  const double g = d.weight * (d.data - bins[o]) / e2;
+  const double depth = pj / sj * bm->depth_scale;
+  gbar[o] = g;
+  fscale[o] = g * bm->depth_scale / sj;
+  terms[2 * o] = g * depth;
+  terms[2 * o + 1] = -g * depth * dsj / sj;
 }
 
 // d logL / d F(nu) through the transposed operator
@@ -589,10 +600,10 @@ size_t align8(size_t x) { return (x + 7) / 8 * 8; }
 }  // namespace
 
 void DeviceEngine::releaseAdjoint() {
-  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_fscale_, (void*)adj_terms_, (void*)adj_intensity_,
-                  (void*)adj_partial_, (void*)adj_out_})
+  for (void* p : {(void*)adj_fbar_, (void*)adj_gbar_, (void*)adj_fscale_, (void*)adj_terms_, (void*)adj_slope_,
+                  (void*)adj_intensity_, (void*)adj_partial_, (void*)adj_out_})
     deviceFree(p);
-  adj_fbar_ = adj_gbar_ = adj_fscale_ = adj_terms_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
+  adj_fbar_ = adj_gbar_ = adj_fscale_ = adj_terms_ = adj_slope_ = adj_intensity_ = adj_partial_ = adj_out_ = nullptr;
   adj_capacity_ = adj_grid_ = adj_partial_size_ = adj_intensity_size_ = 0;
 }
 
@@ -648,6 +659,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     adj_gbar_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint bins");
     adj_fscale_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_), "adjoint planet bins");
     adj_terms_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_bins_) * 2, "adjoint eclipse terms");
+    adj_slope_ = deviceAlloc<double>(adj_capacity_ * std::max<size_t>(1, n_chunks_), "adjoint eclipse star slopes");
     adj_out_ = deviceAlloc<double>(adj_capacity_ * L * K, "adjoint level gradients");
     adj_fbar_ = deviceAlloc<double>(adj_capacity_ * stride_, "adjoint flux");
     adj_intensity_ = deviceAlloc<double>(buffer_size, "adjoint forward quantities");
@@ -666,8 +678,12 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     double* gbar = adj_gbar_ + first * n_bins_;
     double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;
     if (nb > 0 && eclipse) {
-      hipLaunchKernelGGL(eclipseGbarKernel, dim3(unsigned(nb), unsigned(cb)), dim3(kBlockBins), 0, s,
-                         batch_flux_ + first * stride_, stride_, wavenumber_, weights_, bins_desc_, nb, rec,
+      const int pieces = int(n_chunks_);
+      hipLaunchKernelGGL(eclipseSlopeKernel, dim3(unsigned(pieces), unsigned(cb)), dim3(kBlockBins), 0, s,
+                         wavenumber_, weights_, bin_chunks_, pieces, rec, adj_slope_);
+      check(hipGetLastError(), "adjoint eclipse star kernel");
+      hipLaunchKernelGGL(eclipseGbarKernel, dim3(unsigned((nb + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock),
+                         0, s, batch_chunk_sums_ + first * n_chunks_ * 2, adj_slope_, pieces, bins_desc_, nb, rec,
                          batch_bins_ + first * n_bins_, gbar, fscale, adj_terms_ + first * n_bins_ * 2);
       check(hipGetLastError(), "adjoint eclipse bin kernel");
     } else if (nb > 0) {

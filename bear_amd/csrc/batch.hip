@@ -373,9 +373,8 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
 }
 
 // ---------------------------------------------------------------- bins and likelihood
-// One workgroup per (bin, point); mode 1 (eclipse) divides by the folded
-// blackbody star of that point.
-// One workgroup per (piece of a bin's segment, point) (binChunkBody)
+// One workgroup per (piece of a bin's segment, point) (binChunkBody); mode 1
+// (eclipse) also sums the folded blackbody star of that point.
 __global__ __launch_bounds__(kBlockBins) void binBatchKernel(
     const double* __restrict__ flux, size_t stride, const double* __restrict__ wavenumber,
     const double* __restrict__ weights, const BinChunk* __restrict__ chunks, int nb_chunks, Records rec, int mode,

@@ -344,6 +344,7 @@ class DeviceEngine {
   double* adj_gbar_ = nullptr;                   // [capacity][bins]
   double* adj_fscale_ = nullptr;                 // eclipse: dL/d(planet bin) [capacity][bins]
   double* adj_terms_ = nullptr;                  // eclipse: [capacity][bins][2]
+  double* adj_slope_ = nullptr;                  // eclipse: [capacity][pieces]
   double* adj_intensity_ = nullptr;              // kept: FP32 [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]

@@ -77,7 +77,7 @@ def check(grad, ref, label, fd_error):
     assert not bad.any(), label
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small", "eclipse_wide"])
 def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -92,7 +92,7 @@ def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, nam
         check(grad[k], fd2, f"{name} point {k} vs GPU central differences", np.abs(fd - fd2))
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small", "eclipse_wide"])
 def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
