@@ -315,8 +315,8 @@ class Retrieval:
 
     def gradientBatch(self, physical_parameters):
         """(log-likelihoods [batch], gradients [batch, nb_parameters]) from the
-        hand-written adjoint (emission and secondary-eclipse models, FP64
-        tables)."""
+        hand-written adjoint (emission, secondary-eclipse and transmission
+        models, FP64 tables)."""
         p = np.ascontiguousarray(np.atleast_2d(physical_parameters), dtype=np.float64)
         ll = np.empty(p.shape[0])
         grad = np.empty_like(p)

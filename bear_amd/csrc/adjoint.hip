@@ -595,6 +595,229 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
   }
 }
 
+// ---------------------------------------------------------------- transmission
+// Adjoint of the transit depth (transmission_spectrum.cpp:32-154) over the
+// kept forward opacity, keep = [point][level][3][stride] FP32 with slot 0 the
+// extinction k_l of the forward batch kernel.  With e = d logL / d depth
+// 1e6 / R_*^2 (= d logL / dH, depth = (H + R^2) 1e6 / R_*^2) and
+//   H = sum_t c_t (1 - e^-tau_t),  c_t = (R + z_t)(z_{t+1} - z_{t-1}),
+//   tau_t = sum_{q = t}^{last_t} chord[q][t] S_q,  S_q = k_q + k_{q+1},
+// (last_t: the segment at which the sum first passes 10, else L - 2):
+//  1. tangentAdjointKernel: tau_t again, then tau-bar_t = e c_t e^-tau_t into
+//     slot 1 and last_t into slot 2 of level t; sums over wavenumbers of
+//     e (1 - e^-tau_t) (d logL / d c_t) in slot NS+1 of level t, of e (the
+//     R^2 term) in slot NS+1 of level L-1 and of d logL / d depth x depth
+//     (the R_* term) in slot NS+2 of level L-1;
+//  2. chordAdjointKernel + chordReduceKernel: d logL / d chord[q][t] =
+//     sum over wavenumbers of tau-bar_t S_q where q <= last_t;
+//  3. segmentAdjointKernel: S-bar_q = sum_{t <= q, q <= last_t} tau-bar_t
+//     chord[q][t] and d logL / d k_l = S-bar_{l-1} + S-bar_l over slot 2 (in
+//     decreasing q: slot 2 of level q+1 is read for the last time by
+//     S-bar_q), and the Rayleigh sum sigma_R d logL / d k_l in slot NS+3;
+//  4. opacityAdjointKernel as for emission (slot 2 = d logL / d k).
+// The host carries d logL / d chord and d logL / d c_t to the altitudes and
+// the radius.
+__global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, const double* __restrict__ chord,
+                                                              int nb_levels, int n, size_t stride, int tiles,
+                                                              int batch, int ns, const double* __restrict__ fbar,
+                                                              const double* __restrict__ depth,
+                                                              float* __restrict__ keep, double* __restrict__ partial) {
+  const int L = nb_levels, K = ns + 4;
+  const int segments = chordSegments(L), tangents = chordTangents(L);
+  const int lane = int(threadIdx.x);
+  const size_t level_step = 3 * stride;
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    const int i = tile * kWave + lane;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;                                // idle lanes: zero adjoints, no stores
+    const unsigned char* r = rec.at(m);
+    const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
+    const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
+    float* kp = keep + size_t(m) * L * level_step + ic;
+    const double* c = chord + size_t(m) * segments * tangents;
+    double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
+    const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
+    const double e = fb * 1e6 / (bm->star_radius * bm->star_radius);
+    const double r0 = bm->bottom_radius;
+    for (int t = 0; t < L - 1; ++t) {
+      double x = 0.0;
+      int last = L - 2;
+      bool done = false;
+      for (int q = t; q < L - 1; ++q) {
+        const double sq = double(kp[size_t(q) * level_step]) + double(kp[size_t(q + 1) * level_step]);
+        if (!done) {
+          x = fma(c[size_t(q) * tangents + t], sq, x);
+          if (x > 10.0) {
+            done = true;
+            last = q;
+          }
+        }
+        if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+      }
+      const double tr = pow2(-x * kLog2E);
+      const double zl = levels[t > 0 ? t - 1 : 0].altitude, zu = levels[t + 1].altitude;
+      const double ct = (r0 + levels[t].altitude) * (zu - zl);
+      if (valid) {
+        kp[size_t(t) * level_step + stride] = float(e * ct * tr);
+        kp[size_t(t) * level_step + 2 * stride] = float(last);
+      }
+      double x4[4] = {e * (1.0 - tr), 0.0, 0.0, 0.0};
+      emitSums<4>(out + size_t(t) * tiles * K, lane, x4, ns + 1, 2);
+    }
+    const double d = valid ? depth[size_t(m) * stride + i] : 0.0;
+    double x4[4] = {e, fb * d, 0.0, 0.0};
+    emitSums<4>(out + size_t(L - 1) * tiles * K, lane, x4, ns + 1, 2);
+  }
+}
+
+// Step 2: one workgroup of 256 threads per (64 x 64 block of (q, t) with
+// t-block <= q-block, point, range of wavenumbers), 4 x 4 pairs per thread,
+// the wavenumbers in steps of 32 through LDS;
+// partial[point][range][q][t] for P = L - 1 segments and tangent paths.
+constexpr int kChordBlock = 64, kChordNu = 32, kChordRanges = 16;
+
+__global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restrict__ keep, int nb_levels, int n,
+                                                          size_t stride, int ranges, int nu_per_range,
+                                                          double* __restrict__ partial) {
+  __shared__ double s_sh[kChordBlock][kChordNu + 1];
+  __shared__ double a_sh[kChordBlock][kChordNu + 1];
+  __shared__ float l_sh[kChordBlock][kChordNu + 1];
+  const int L = nb_levels, P = L - 1;
+  int bq = 0, bt = int(blockIdx.x);                                 // triangular block index
+  while (bt > bq) {
+    bt -= bq + 1;
+    ++bq;
+  }
+  const int m = int(blockIdx.y), range = int(blockIdx.z);
+  const int q0 = bq * kChordBlock, t0 = bt * kChordBlock;
+  const int tid = int(threadIdx.x), tq = tid >> 4, tt = tid & 15;
+  const size_t level_step = 3 * stride;
+  const float* base = keep + size_t(m) * L * level_step;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  const int nu0 = range * nu_per_range, nu1 = min(n, nu0 + nu_per_range);
+  for (int v0 = nu0; v0 < nu1; v0 += kChordNu) {
+    for (int e = tid; e < kChordBlock * kChordNu; e += 256) {
+      const int row = e / kChordNu, col = e % kChordNu;
+      const int v = v0 + col, q = q0 + row, t = t0 + row;
+      double sv = 0.0, av = 0.0;
+      float lv = -1.0f;
+      if (v < nu1) {
+        if (q < P) sv = double(base[size_t(q) * level_step + v]) + double(base[size_t(q + 1) * level_step + v]);
+        if (t < P) {
+          av = base[size_t(t) * level_step + stride + v];
+          lv = base[size_t(t) * level_step + 2 * stride + v];
+        }
+      }
+      s_sh[row][col] = sv;
+      a_sh[row][col] = av;
+      l_sh[row][col] = lv;
+    }
+    __syncthreads();
+    for (int v = 0; v < kChordNu; ++v) {
+      double sv[4], av[4];
+      float lv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) sv[a] = s_sh[tq + 16 * a][v];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        av[b] = a_sh[tt + 16 * b][v];
+        lv[b] = l_sh[tt + 16 * b][v];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const float q = float(q0 + tq + 16 * a);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(q <= lv[b] ? av[b] : 0.0, sv[a], acc[a][b]);
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + (size_t(m) * ranges + range) * size_t(P) * P;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int q = q0 + tq + 16 * a;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int t = t0 + tt + 16 * b;
+      if (q < P && t < P) out[size_t(q) * P + t] = t <= q ? acc[a][b] : 0.0;
+    }
+  }
+}
+
+// out[point][q][t] = the ranges' partial sums in order (q >= t; else 0)
+__global__ __launch_bounds__(kBlock) void chordReduceKernel(const double* __restrict__ partial, int P, int ranges,
+                                                            double* __restrict__ out) {
+  const int e = int(blockIdx.x) * kBlock + int(threadIdx.x), m = int(blockIdx.y);
+  if (e >= P * P) return;
+  const int q = e / P, t = e % P;
+  double sum = 0.0;
+  if (t <= q)
+    for (int k = 0; k < ranges; ++k) sum += partial[(size_t(m) * ranges + k) * size_t(P) * P + e];
+  out[size_t(m) * P * P + e] = sum;
+}
+
+__global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const double* __restrict__ chord,
+                                                              const double* __restrict__ rayleigh, int nb_levels,
+                                                              int n, size_t stride, int tiles, int batch, int ns,
+                                                              float* __restrict__ keep, double* __restrict__ partial) {
+  const int L = nb_levels, K = ns + 4;
+  const int segments = chordSegments(L), tangents = chordTangents(L);
+  const int lane = int(threadIdx.x);
+  const size_t level_step = 3 * stride;
+  const size_t total = nbItems(tiles, batch);
+  for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int m, tile;
+    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    const int i = tile * kWave + lane;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;
+    float* kp = keep + size_t(m) * L * level_step + ic;
+    const double* c = chord + size_t(m) * segments * tangents;
+    double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
+    const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
+    double s_hi = 0.0;                                               // S-bar_{q+1}
+    for (int q = L - 2; q >= -1; --q) {
+      double sb = 0.0;
+      for (int t = 0; t <= q; ++t) {
+        const float* col = kp + size_t(t) * level_step;
+        const double tb = col[stride];
+        const float last = col[2 * stride];
+        sb = fma(float(q) <= last ? tb : 0.0, c[size_t(q) * tangents + t], sb);
+      }
+      const double kb = sb + s_hi;                                   // d logL / d k_{q+1}
+      if (valid) kp[size_t(q + 1) * level_step + 2 * stride] = float(kb);
+      double x4[4] = {kb * sr, 0.0, 0.0, 0.0};
+      emitSums<4>(out + size_t(q + 1) * tiles * K, lane, x4, ns + 3, 1);
+      s_hi = sb;
+    }
+  }
+}
+
+template <int NS>
+void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const double* tables, size_t stride,
+                               const double* rayleigh, Records rec, const double* chord, int L, int n, int tiles,
+                               int batch, const double* fbar, const double* depth, float* keep, double* partial,
+                               int ranges, int nu_per_range, double* chord_partial, double* chord_out) {
+  hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord, L, n, stride, tiles, batch, NS,
+                     fbar, depth, keep, partial);
+  const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
+  hipLaunchKernelGGL(chordAdjointKernel, dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(ranges)),
+                     dim3(256), 0, s, keep, L, n, stride, ranges, nu_per_range, chord_partial);
+  hipLaunchKernelGGL(chordReduceKernel, dim3(unsigned((P * P + kBlock - 1) / kBlock), unsigned(batch)), dim3(kBlock),
+                     0, s, chord_partial, P, ranges, chord_out);
+  hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord, rayleigh, L, n, stride, tiles,
+                     batch, NS, keep, partial);
+  hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch, keep,
+                     partial);
+}
+
 size_t align8(size_t x) { return (x + 7) / 8 * 8; }
 
 }  // namespace
@@ -610,8 +833,8 @@ void DeviceEngine::releaseAdjoint() {
 void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
                                         std::vector<double>& bin_grad, std::vector<double>& bins,
                                         std::vector<double>& bin_terms) {
-  if (transmission_ || kind == ModelKind::transmission)
-    raise(kInvalidInput, "the adjoint covers the emission and secondary-eclipse models");
+  const bool transmission = kind == ModelKind::transmission;
+  if (transmission != transmission_) raise(kInternal, "adjoint model kind differs from the configured model");
   const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
@@ -626,9 +849,13 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
 
   // points per chunk: kept forward quantities within the budget (and 80 % of
   // the free memory, checked when the buffers grow), else recomputed
+  // (emission and eclipse only); transmission also holds per point the
+  // chord gradients and their per-range partial sums
+  const int P = L - 1;
+  const size_t chord_point = transmission ? size_t(P) * P * (kChordRanges + 1) : 0;   // doubles per point
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
   const size_t kept_values = size_t(L) * 3 * stride_;                          // FP32 values per point
-  const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_;  // doubles per point
+  const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_ + chord_point;  // doubles per point
   size_t budget = adjoint_budget_;
   size_t chunk = 0, buffer_size = 0, partial_size = 0;
   bool kept = false;
@@ -640,9 +867,13 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     const size_t chunks = (batch + chunk - 1) / chunk;                      // equal chunks, no small tail
     chunk = (batch + chunks - 1) / chunks;
     buffer_size = kept ? (chunk * kept_values + 1) / 2 : size_t(grid) * L * 2 * kWave;   // doubles
-    partial_size = chunk * partial_point;
+    partial_size = chunk * (partial_point + chord_point);
   };
   plan();
+  if (transmission && !kept)
+    raise(kInvalidInput, "the transmission adjoint keeps each point's opacity and path adjoints (" +
+                             std::to_string(kept_point * sizeof(double)) +
+                             " bytes per point): raise the adjoint memory");
   const bool grow = batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ ||
                     partial_size > adj_partial_size_ || buffer_size > adj_intensity_size_;
   if (grow) {
@@ -651,6 +882,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     const size_t held = (adj_intensity_size_ + adj_partial_size_) * sizeof(double);
     budget = std::min(adjoint_budget_, (free_bytes + held) / 10 * 8);
     plan();
+    if (transmission && !kept) raise(kOutOfMemory, "not enough free device memory for the transmission adjoint");
     releaseAdjoint();
     adj_capacity_ = batch_capacity_;
     adj_grid_ = size_t(grid);
@@ -665,15 +897,16 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     adj_intensity_ = deviceAlloc<double>(buffer_size, "adjoint forward quantities");
     adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
   }
-  const BatchLaunch launch = tuned(ModelKind::emission, chunk);
+  const BatchLaunch launch = tuned(transmission ? ModelKind::transmission : ModelKind::emission, chunk);
+  bin_terms.assign(transmission ? batch * P * P : eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
   check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
   for (size_t first = 0; first < batch; first += chunk) {
     const int cb = int(std::min(chunk, batch - first));
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                       rec_shifts_, groups_};
-    launchBatchSpectrum(ModelKind::emission, size_t(cb), launch, first,
-                        kept ? reinterpret_cast<float*>(adj_intensity_) : nullptr);
+    float* keep = kept ? reinterpret_cast<float*>(adj_intensity_) : nullptr;
+    launchBatchSpectrum(transmission ? ModelKind::transmission : ModelKind::emission, size_t(cb), launch, first, keep);
     launchBatchRest(kind, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
     double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;
@@ -694,6 +927,35 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     hipLaunchKernelGGL(fluxAdjointKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(cb)), dim3(kBlock), 0,
                        s, adj_ptr_, adj_bin_, adj_weight_, fscale, nb, n, stride_, adj_fbar_);
     check(hipGetLastError(), "adjoint flux kernel");
+    if (transmission) {
+      // wavenumber ranges of the chord products: enough workgroups to fill the device
+      const int blocks = (P + kChordBlock - 1) / kChordBlock;
+      const int pairs = blocks * (blocks + 1) / 2;
+      int ranges = int(std::min<size_t>(kChordRanges, (size_t(nb_cus_) * 8 + size_t(pairs) * cb - 1) /
+                                                          (size_t(pairs) * cb)));
+      ranges = std::max(1, std::min(ranges, (n + 511) / 512));
+      const int per_range = (n + ranges - 1) / ranges;
+      const int nu_per_range = (per_range + kChordNu - 1) / kChordNu * kChordNu;
+      double* chord_partial = adj_partial_ + size_t(cb) * partial_point;
+      double* chord_out = chord_partial + size_t(cb) * P * P * kChordRanges;
+      switch (NS) {
+#define BEAR_TADJ(NSV)                                                                                             \
+  case NSV:                                                                                                        \
+    launchTransmissionAdjoint<NSV>(dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, tables_, stride_, rayleigh_,  \
+                                   rec, batch_chord_, L, n, tiles, cb, adj_fbar_, batch_flux_ + first * stride_,  \
+                                   keep, adj_partial_, ranges, nu_per_range, chord_partial, chord_out);           \
+    break;
+        BEAR_TADJ(4)
+        BEAR_TADJ(8)
+        BEAR_TADJ(12)
+        BEAR_TADJ(16)
+#undef BEAR_TADJ
+      }
+      check(hipGetLastError(), "transmission adjoint kernels");
+      check(hipMemcpyAsync(bin_terms.data() + first * P * P, chord_out, size_t(cb) * P * P * sizeof(double),
+                           hipMemcpyDeviceToHost, s),
+            "read chord gradients");
+    } else {
     switch (NS) {
 #define BEAR_ADJ(NSV)                                                                                              \
   case NSV:                                                                                                        \
@@ -707,6 +969,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
 #undef BEAR_ADJ
     }
     check(hipGetLastError(), "adjoint kernels");
+    }
     double* level_out = adj_out_ + first * L * K;
     switch (K) {
 #define BEAR_RED(KV)                                                                                               \
@@ -726,7 +989,6 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   level_grad.resize(batch * L * K);
   bin_grad.resize(batch * n_bins_);
   bins.resize(batch * n_bins_);
-  bin_terms.assign(eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   check(hipMemcpyAsync(level_grad.data(), adj_out_, level_grad.size() * sizeof(double), hipMemcpyDeviceToHost, s),
         "read level gradients");
   if (eclipse && nb > 0)

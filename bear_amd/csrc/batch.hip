@@ -293,7 +293,7 @@ template <class Op>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
-    int tiles, int tile, int batch, int point_group, double* __restrict__ depth) {
+    int tiles, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep) {
   using K = typename Op::K;
   using Species = typename Op::Species;
   extern __shared__ double lds[];
@@ -325,6 +325,7 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
         const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
                                           l == l0 || levels[l].reload != 0));
         const K k = K(kd + sigma_r * levels[l].rayleigh_density);
+        if (keep) keep[(size_t(m) * L + l) * 3 * stride + i] = float(k);
         if (l == l0) part[g * tile + j] = double(k);
         else S[(l - 1) * tile + j] = last + k;
         last = k;
@@ -446,24 +447,24 @@ template <int NS>
 void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
                         const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
                         const double* chord, const float* chord32, int L, int n, int tiles, int tile, int batch,
-                        int group, double* depth) {
+                        int group, double* depth, float* keep) {
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                         rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
+                         rec, chord, chord32, L, n, tiles, tile, batch, group, depth, keep);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth);
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
       break;
   }
 }
@@ -602,7 +603,8 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_T(NS)                                                                                                  \
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
-                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, lc.group, flux);  \
+                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, lc.group, flux, \
+                           keep);                                                                              \
     break;
       BEAR_T(4)
       BEAR_T(8)

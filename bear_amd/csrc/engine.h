@@ -231,7 +231,13 @@ class DeviceEngine {
   // instead.  kind: emission, or eclipse (planet flux at scale 1 against the
   // blackbody star; bin_terms then gets per (point, bin) dL/dbin times the
   // depth without shift, and dL/dbin times d depth / dT_star,
-  // [batch][bins][2]).
+  // [batch][bins][2]), or transmission (always kept; the level slots are
+  // n dL/dn, the temperature through the tables, dL/dc_t for the trapezoid
+  // coefficient c_t = (R + z_t)(z_{t+1} - z_{t-1}) of tangent path t (level
+  // L-1: dL/dH for the R^2 term), at level L-1 only the sum of dL/ddepth x
+  // depth, and dL/dn_R of the Rayleigh scatterer; bin_terms gets dL/dchord
+  // [batch][L-1 segments][L-1 tangents], zero where the segment lies below
+  // the tangent level).
   void evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
                             std::vector<double>& bin_grad, std::vector<double>& bins, std::vector<double>& bin_terms);
   // HBM budget (bytes) of the kept forward quantities; 0: always recompute
@@ -303,7 +309,8 @@ class DeviceEngine {
   unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
   // points [first, first + batch) of the staged records; keep: also store
   // every level's I - B of both angles and opacity, FP32
-  // [batch][levels][3][stride] (emission)
+  // [batch][levels][3][stride] (emission; transmission: the extinction in
+  // the first of the three values)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
                            float* keep = nullptr);
   void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);

@@ -216,13 +216,16 @@ class Atmosphere {
   bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param,
                const AltitudeOptions& options = AltitudeOptions());
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
-  // Reverse-mode derivative of the last compute() (constant gravity, the
-  // chemistry's own mean molecular weight): from dL/dT[l] (direct effects),
-  // dL/dn[l][id] and dL/dz[l] (consumed) to dL/dg and dL/d(parameters).
+  // Reverse-mode derivative of the last compute(), made with `options`:
+  // from dL/dT[l] (direct effects), dL/dn[l][id] and dL/dz[l] (consumed) to
+  // dL/dg and dL/d(parameters), adding to *radius_bar (variable gravity: the
+  // bottom radius) and *option_bar (the fixed mean molecular weight, or the
+  // scale height in km) when given.
   void adjoint(double surface_gravity, const double* temperature_param, const double* chemistry_param,
                std::vector<double>& temperature_bar, std::vector<double>& number_density_bar,
                std::vector<double>& altitude_bar, double& gravity_bar, double* temperature_param_bar,
-               double* chemistry_param_bar) const;
+               double* chemistry_param_bar, const AltitudeOptions& options = AltitudeOptions(),
+               double* radius_bar = nullptr, double* option_bar = nullptr) const;
   // copies are independent (each owns its temperature profile's state)
   Atmosphere(const Atmosphere& other);
  private:

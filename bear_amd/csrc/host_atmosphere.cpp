@@ -267,14 +267,16 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
 }
 
 // Reverse sweep through compute(), last operation first:
-//   z_l = z_{l-1} + (u_l + u_{l-1}) / (2 g) (p_{l-1} - p_l) 1e6, u = 1/rho = R T / (mu p 1e6)
+//   z_l = z_{l-1} + (u_l + u_{l-1}) / (2 g_l) (p_{l-1} - p_l) 1e6, u = 1/rho = R T / (mu p 1e6),
+//     g_l = g, or g (R / (R + z_{l-1}))^2 with variable gravity; mu the
+//     chemistry's or the fixed one; or z_l = -H ln(p_l / p_0)
 //   mu_l = sum_j n_j m_j / n_tot
 //   background: n_H2,He = n_tot bg r_H2,He, bg = 1 - sum_{j != H2,He} n_j / n_tot
 //   isoprofile: n_j = n_tot x_j, then Na = 16.2181 K unless Na is free
 //   n_tot = p 1e6 / (k T);  T_l = poly(log10 p_l; dof), dof_i = dof_{i-1} theta_i
 void Atmosphere::adjoint(double g, const double* tp, const double* cp, std::vector<double>& t_bar,
                          std::vector<double>& n_bar, std::vector<double>& z_bar, double& g_bar, double* tp_bar,
-                         double* cp_bar) const {
+                         double* cp_bar, const AltitudeOptions& opt, double* radius_bar, double* option_bar) const {
   const size_t L = nb_levels;
   const int S = int(n_ids_);
   auto n = [&](size_t l, int id) { return number_density[l * n_ids_ + id]; };
@@ -282,24 +284,49 @@ void Atmosphere::adjoint(double g, const double* tp, const double* cp, std::vect
   std::vector<double> mu_bar(L, 0.0), ntot_bar(L, 0.0);
   for (size_t l = 0; l < L; ++l) ntot_bar[l] = nb(l, sid::total);
   g_bar = 0;
-  // altitude: Z_k = sum_{l >= k} dL/dz_l is the adjoint of layer k's increment h_k
-  double suffix = 0;
   std::vector<double> u_bar(L, 0.0);
-  for (size_t k = L - 1; k >= 1; --k) {
-    suffix += z_bar[k];
-    const double rho_k = mean_molecular_weight[k] * pressure[k] * 1e6 / (phys::gas_constant * temperature[k]);
-    const double rho_p = mean_molecular_weight[k - 1] * pressure[k - 1] * 1e6 /
-                         (phys::gas_constant * temperature[k - 1]);
-    const double dp = 0.5 * (pressure[k - 1] * 1e6 - pressure[k] * 1e6);
-    const double h = (1.0 / (rho_k * g) + 1.0 / (rho_p * g)) * dp;
-    u_bar[k] += suffix * dp / g;
-    u_bar[k - 1] += suffix * dp / g;
-    g_bar -= suffix * h / g;
+  if (opt.scale_height_km > 0) {
+    double h_bar = 0;
+    for (size_t l = 1; l < L; ++l) h_bar -= z_bar[l] * std::log(pressure[l] / pressure[0]);
+    if (option_bar) *option_bar += h_bar * 100000.0;
+  } else {
+    // altitude: zb[k] = dL/dz_k with the layers above carried down
+    std::vector<double> zb(z_bar.begin(), z_bar.begin() + L);
+    for (size_t k = L - 1; k >= 1; --k) {
+      const double rho_k = mean_molecular_weight[k] * pressure[k] * 1e6 / (phys::gas_constant * temperature[k]);
+      const double rho_p = mean_molecular_weight[k - 1] * pressure[k - 1] * 1e6 /
+                           (phys::gas_constant * temperature[k - 1]);
+      const double dp = 0.5 * (pressure[k - 1] * 1e6 - pressure[k] * 1e6);
+      double gk = g, radius = 0;
+      if (opt.variable_gravity) {
+        radius = opt.bottom_radius + altitude[k - 1];
+        gk = g * std::pow(opt.bottom_radius / radius, 2.0);
+      }
+      const double h = (1.0 / (rho_k * gk) + 1.0 / (rho_p * gk)) * dp;
+      u_bar[k] += zb[k] * dp / gk;
+      u_bar[k - 1] += zb[k] * dp / gk;
+      const double gk_bar = -zb[k] * h / gk;
+      zb[k - 1] += zb[k];
+      if (opt.variable_gravity) {
+        g_bar += gk_bar * gk / g;
+        if (radius_bar) *radius_bar += gk_bar * gk * (2.0 / opt.bottom_radius - 2.0 / radius);
+        zb[k - 1] -= gk_bar * 2.0 * gk / radius;
+      } else {
+        g_bar += gk_bar;
+      }
+    }
   }
   for (size_t l = 0; l < L; ++l) {
     const double u = phys::gas_constant * temperature[l] / (mean_molecular_weight[l] * pressure[l] * 1e6);
     t_bar[l] += u_bar[l] * u / temperature[l];
     mu_bar[l] -= u_bar[l] * u / mean_molecular_weight[l];
+  }
+  if (opt.scale_height_km <= 0 && opt.fixed_mean_molecular_weight > 0) {
+    // the fixed weight replaced the chemistry's in the altitude
+    double m_bar = 0;
+    for (size_t l = 0; l < L; ++l) m_bar += mu_bar[l];
+    if (option_bar) *option_bar += m_bar;
+    mu_bar.assign(L, 0.0);
   }
   // the last mean-weight evaluation
   for (size_t l = 0; l < L; ++l) {

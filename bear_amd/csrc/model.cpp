@@ -245,7 +245,7 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
 void TabulatedModel::chainGradient(const double* params, size_t batch, size_t stride,
                                    const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
                                    const std::vector<double>& bin_terms, const char* neglect, double* gradient) {
-  const size_t L = atmosphere_.nb_levels, S = species_.size(), NS = engine_.speciesStride(), K = NS + 4;
+  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = NS + 4;
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
@@ -267,36 +267,14 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
       for (size_t l = 0; l < L; ++l) {
         t_bar[l] = G[l * K + NS] + G[l * K + NS + 3];
         z_bar[l] = G[l * K + NS + 1];
-        for (size_t s = 0; s < S; ++s) {
-          const OpacitySpecies& sp = species_[s];
-          // the kernels give n dL/dn
-          double n_eff = atm.nd(l, sp.species_id);
-          for (int q : sp.cia_partners) n_eff *= atm.nd(l, q);
-          const double gs = n_eff > 0 ? G[l * K + s] / n_eff : 0.0;
-          double others = 1.0;
-          for (int q : sp.cia_partners) others *= atm.nd(l, q);
-          n_bar[l * ids + sp.species_id] += gs * others;
-          for (size_t a = 0; a < sp.cia_partners.size(); ++a) {
-            double rest = atm.nd(l, sp.species_id);
-            for (size_t c = 0; c < sp.cia_partners.size(); ++c)
-              if (c != a) rest *= atm.nd(l, sp.cia_partners[c]);
-            n_bar[l * ids + sp.cia_partners[a]] += gs * rest;
-          }
-        }
       }
+      densityGradient(atm, G, n_bar);
       double* out = gradient + b * stride;
       double g_bar = 0;
       atm.adjoint(g, temp, chem, t_bar, n_bar, z_bar, g_bar, out + 3 + nb_chem_, out + 3);
       out[0] = g_bar * g * std::log(10.0);
       generalGradient(p, G[NS + 2], bin_terms.empty() ? nullptr : bin_terms.data() + b * nb * 2, out);
-      // shift_const: one parameter per observation, added to its bins
-      size_t k = 3 + nb_chem_ + nb_temp_, j = 0;
-      for (const auto& o : observations_) {
-        double sum = 0;
-        for (size_t q = 0; q < o.nbPoints(); ++q) sum += bin_grad[b * nb + j + q];
-        if (o.modifier_parameters) out[k++] = sum;
-        j += o.nbPoints();
-      }
+      shiftGradient(bin_grad.data() + b * nb, out);
     } catch (const Error& e) {
 #pragma omp critical(bear_gradient_failure)
       if (failure.empty()) { failure = e.what(); failure_code = e.code; }
@@ -306,6 +284,39 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
     }
   }
   if (!failure.empty()) raise(failure_code, failure);
+}
+
+void TabulatedModel::densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar) const {
+  const size_t L = atm.nb_levels, K = engine_.speciesStride() + 4, ids = size_t(nbSpeciesIds());
+  for (size_t l = 0; l < L; ++l)
+    for (size_t s = 0; s < species_.size(); ++s) {
+      const OpacitySpecies& sp = species_[s];
+      // the kernels give n dL/dn
+      double n_eff = atm.nd(l, sp.species_id);
+      for (int q : sp.cia_partners) n_eff *= atm.nd(l, q);
+      const double gs = n_eff > 0 ? G[l * K + s] / n_eff : 0.0;
+      double others = 1.0;
+      for (int q : sp.cia_partners) others *= atm.nd(l, q);
+      n_bar[l * ids + sp.species_id] += gs * others;
+      for (size_t a = 0; a < sp.cia_partners.size(); ++a) {
+        double rest = atm.nd(l, sp.species_id);
+        for (size_t c = 0; c < sp.cia_partners.size(); ++c)
+          if (c != a) rest *= atm.nd(l, sp.cia_partners[c]);
+        n_bar[l * ids + sp.cia_partners[a]] += gs * rest;
+      }
+    }
+}
+
+// shift_const: one parameter per observation, added to its bins
+// (forward_model.cpp:161-180)
+void TabulatedModel::shiftGradient(const double* bin_grad, double* out) const {
+  size_t k = nb_general_ + nb_chem_ + nb_temp_, j = 0;
+  for (const auto& o : observations_) {
+    double sum = 0;
+    for (size_t q = 0; q < o.nbPoints(); ++q) sum += bin_grad[j + q];
+    if (o.modifier_parameters) out[k++] = sum;
+    j += o.nbPoints();
+  }
 }
 
 // emission: scale = (R_J / d)^2 f (emission.cpp:123-153)
@@ -371,18 +382,101 @@ TransmissionModel::TransmissionModel(const GlobalSettings& global, const ModelSe
   nb_general_ = 3 + ((fit_mmw_ || fit_scale_height_) ? 1 : 0);    // transmission.cpp:69-72
 }
 
-// parameters and atmosphere variants of calcAtmosphereStructure (transmission.cpp:150-195)
-bool TransmissionModel::setup(const double* p, Atmosphere& atm, BatchModel& scalars) const {
-  const double* chem = p + nb_general_;
-  const double* temp = chem + nb_chem_;
+// altitude variants of calcAtmosphereStructure (transmission.cpp:150-195)
+AltitudeOptions TransmissionModel::altitudeOptions(const double* p) const {
   AltitudeOptions opt;
   opt.bottom_radius = p[1];
   opt.variable_gravity = variable_gravity_;
   if (fit_mmw_) opt.fixed_mean_molecular_weight = p[3];
   if (fit_scale_height_) opt.scale_height_km = p[3];
+  return opt;
+}
+
+bool TransmissionModel::setup(const double* p, Atmosphere& atm, BatchModel& scalars) const {
+  const double* chem = p + nb_general_;
+  const double* temp = chem + nb_chem_;
   scalars.bottom_radius = p[1];
   scalars.star_radius = p[2];
-  return atm.compute(std::pow(10, p[0]), temp, chem, opt);
+  return atm.compute(std::pow(10, p[0]), temp, chem, altitudeOptions(p));
+}
+
+// Chain rule of the transmission model, per point, from the device's level
+// gradients and dL/dchord (transmission_spectrum.cpp:32-154):
+//   depth = (H + R^2) 1e6 / R_*^2      -> R_*: -2 / R_* sum dL/ddepth depth
+//   H = sum_t c_t (1 - T_t), c_t = (R + z_t)(z_{t+1} - z_{t-1}), z_{-1} = z_0
+//   chord[q][t] = D(q+1, t) - D(q, t), D(a, t) = sqrt((R + z_a)^2 - (R + z_t)^2)
+//     (D(t, t) = 0 whatever the altitudes)
+// then the atmosphere's reverse sweep with the model's altitude variant
+// (radius through variable gravity, the fixed weight or scale height).
+void TransmissionModel::gradientBatch(const double* params, size_t batch, size_t stride,
+                                      const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
+                                      const std::vector<double>& chord_grad, const char* neglect, double* gradient) {
+  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = NS + 4, P = L - 1;
+  const size_t ids = size_t(nbSpeciesIds());
+  const size_t nb = engine_.nbBins();
+  const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
+  while (atmosphere_pool_.size() < size_t(threads)) atmosphere_pool_.emplace_back(new Atmosphere(atmosphere_));
+  std::string failure;
+  int failure_code = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (size_t b = 0; b < batch; ++b) {
+    if (neglect[b]) continue;
+    try {
+      Atmosphere& atm = *atmosphere_pool_[omp_get_thread_num()];
+      const double* p = params + b * stride;
+      const double* chem = p + nb_general_;
+      const double* temp = chem + nb_chem_;
+      const double g = std::pow(10, p[0]);
+      const AltitudeOptions opt = altitudeOptions(p);
+      atm.compute(g, temp, chem, opt);
+      const double* G = level_grad.data() + b * L * K;
+      std::vector<double> t_bar(L), z_bar(L, 0.0), n_bar(L * ids, 0.0);
+      for (size_t l = 0; l < L; ++l) t_bar[l] = G[l * K + NS];
+      densityGradient(atm, G, n_bar);
+      if (rayleigh_species_ >= 0)
+        for (size_t l = 0; l < L; ++l) n_bar[l * ids + size_t(rayleigh_species_)] += G[l * K + NS + 3];
+      const std::vector<double>& z = atm.altitude;
+      const double r0 = p[1];
+      double r_bar = 2.0 * r0 * G[(L - 1) * K + NS + 1];
+      for (size_t t = 0; t < P; ++t) {
+        const size_t lo = t > 0 ? t - 1 : 0;
+        const double e1 = G[t * K + NS + 1], w = z[t + 1] - z[lo], rt = r0 + z[t];
+        r_bar += e1 * w;
+        z_bar[t] += e1 * w;
+        z_bar[t + 1] += e1 * rt;
+        z_bar[lo] -= e1 * rt;
+      }
+      const double* C = chord_grad.data() + b * P * P;
+      for (size_t t = 0; t < P; ++t) {
+        const double bt = r0 + z[t];
+        for (size_t a = t + 1; a < L; ++a) {
+          const double ar = r0 + z[a];
+          const double d = std::sqrt(ar * ar - bt * bt);
+          const double d_bar = C[(a - 1) * P + t] - (a < P ? C[a * P + t] : 0.0);
+          if (!(d > 0)) continue;
+          z_bar[a] += d_bar * ar / d;
+          z_bar[t] -= d_bar * bt / d;
+          r_bar += d_bar * (ar - bt) / d;
+        }
+      }
+      double* out = gradient + b * stride;
+      double g_bar = 0, option_bar = 0;
+      atm.adjoint(g, temp, chem, t_bar, n_bar, z_bar, g_bar, out + nb_general_ + nb_chem_, out + nb_general_, opt,
+                  &r_bar, &option_bar);
+      out[0] = g_bar * g * std::log(10.0);
+      out[1] = r_bar;
+      out[2] = -2.0 / p[2] * G[(L - 1) * K + NS + 2];
+      if (nb_general_ > 3) out[3] = option_bar;
+      shiftGradient(bin_grad.data() + b * nb, out);
+    } catch (const Error& e) {
+#pragma omp critical(bear_gradient_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = e.code; }
+    } catch (const std::exception& e) {
+#pragma omp critical(bear_gradient_failure)
+      if (failure.empty()) { failure = e.what(); failure_code = kInternal; }
+    }
+  }
+  if (!failure.empty()) raise(failure_code, failure);
 }
 
 // TransmissionModel::calcModelGPU (transmission.cpp:257-321)
@@ -522,8 +616,6 @@ void Retrieval::setTablePrecision(TablePrecision p) { engine_->setTablePrecision
 // with s_j^2 = sigma_j^2 + 10^e.
 void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, double* ll, double* gradient) {
   if (batch == 0) return;
-  if (model_->kind() == ModelKind::transmission)
-    raise(kInvalidInput, "the gradient is available for the emission and secondary-eclipse models");
   if (engine_->tablePrecision() != TablePrecision::fp64)
     raise(kInvalidInput, "the gradient needs the FP64 tables (table precision fp64)");
   const std::vector<char> neglect = stageBatch(params, batch, n);

@@ -52,14 +52,15 @@ class ForwardModel {
   virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
   // gradient rows [batch][stride] of the model's parameters (and observation
   // modifiers) from the engine's adjoint: level gradients [batch][levels][slots],
-  // bin gradients [batch][bins] and (eclipse) the bin terms [batch][bins][2];
+  // bin gradients [batch][bins] and the model's terms (eclipse: per bin
+  // [batch][bins][2]; transmission: per chord [batch][segments][tangents]);
   // rows of neglected points are left alone
   virtual void gradientBatch(const double* parameters, size_t batch, size_t stride,
                              const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
                              const std::vector<double>& bin_terms, const char* neglect, double* gradient) {
     (void)parameters, (void)batch, (void)stride, (void)level_grad, (void)bin_grad, (void)bin_terms, (void)neglect,
         (void)gradient;
-    raise(kInvalidInput, "the gradient is available for the emission and secondary-eclipse models");
+    raise(kInvalidInput, "this model has no gradient");
   }
 };
 
@@ -93,6 +94,11 @@ class TabulatedModel : public ForwardModel {
   virtual void generalGradient(const double* p, double scale_bar, const double* bin_terms, double* out) const {
     (void)p, (void)scale_bar, (void)bin_terms, (void)out;
   }
+  // dL/dn[level][species id] from the level gradients G [levels][slots]
+  // (n_eff dL/dn_eff per stencil slot) through the CIA products
+  void densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar) const;
+  // the observations' shift parameters: per observation the sum of its bins' dL/dbin
+  void shiftGradient(const double* bin_grad, double* out) const;
   const std::vector<ObservationSetup>& observations_;
   DeviceEngine& engine_;
   Atmosphere atmosphere_;
@@ -160,11 +166,15 @@ class TransmissionModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return false; }
   ModelKind kind() const override { return ModelKind::transmission; }
+  void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
+                     const std::vector<double>& bin_grad, const std::vector<double>& chord_grad, const char* neglect,
+                     double* gradient) override;
 
  protected:
   bool setup(const double* parameters, Atmosphere& atmosphere, BatchModel& scalars) const override;
 
  private:
+  AltitudeOptions altitudeOptions(const double* p) const;
   bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;
 };
 
@@ -186,7 +196,7 @@ class Retrieval {
   void computeModelBatch(const double* parameters, size_t batch, size_t n, double* spectrum, double* spectrum_obs,
                          int* neglect);
   void setTablePrecision(TablePrecision p);
-  // log-likelihoods [batch] and their gradients [batch][n] (emission, FP64 tables)
+  // log-likelihoods [batch] and their gradients [batch][n] (FP64 tables)
   void gradientBatch(const double* parameters, size_t batch, size_t n, double* log_likelihood, double* gradient);
   DeviceEngine& engine() const { return *engine_; }
 

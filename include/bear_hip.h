@@ -209,12 +209,13 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
                                        size_t nb_parameters, double* spectrum, double* spectrum_obs,
                                        int* neglect_model);
 
-/* Value and gradient of the log-likelihood for a batch (emission and
- * secondary-eclipse models, FP64 tables; transmission returns
- * BEAR_ERROR_INVALID_INPUT).  No counterpart in the reference, which has no
- * derivatives: a hand-written adjoint of the whole chain (parameters ->
- * temperature profile, chemistry, altitudes -> opacity interpolation ->
- * short characteristics -> folded observation operator, for the eclipse
+/* Value and gradient of the log-likelihood for a batch (emission,
+ * secondary-eclipse and transmission models, FP64 tables).  No counterpart
+ * in the reference, which has no derivatives: a hand-written adjoint of the
+ * whole chain (parameters -> temperature profile, chemistry, altitudes ->
+ * opacity interpolation (transmission: with Rayleigh scattering) -> short
+ * characteristics, or the tangent paths' optical depths, chord lengths and
+ * transit-depth integral -> folded observation operator, for the eclipse
  * against the blackbody star -> Gaussian likelihood).  log_likelihood
  * [batch] as bear_retrieval_compute_likelihood_batch; gradient
  * [batch][nb_parameters] (rows of neglected points are zero). */
@@ -224,10 +225,14 @@ int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, s
 /* Device memory (bytes) the gradient may hold for the forward quantities of
  * every level that its backward pass reads back (4 bytes x (2 angles'
  * intensity less the Planck function + opacity) x levels x spectral points
- * per live point, plus per-point partial sums).
+ * per live point, plus per-point partial sums; transmission: the same 12
+ * bytes per level and spectral point for the extinction and the tangent
+ * paths' adjoints, plus 17 x 8 bytes x (levels - 1)^2 per point).
  * Batches are processed in chunks of as many points as fit; with 0, or when
  * a single point does not fit, the backward kernel recomputes the forward
- * pass instead.  Default 96 GiB, never more than 80 % of the free memory.
+ * pass instead (emission and eclipse; the transmission gradient then
+ * returns BEAR_ERROR_INVALID_INPUT).  Default 96 GiB, never more than 80 %
+ * of the free memory.
  * No counterpart in the reference. */
 int bear_retrieval_set_adjoint_memory(bear_retrieval* r, size_t bytes);
 

@@ -1,5 +1,5 @@
-"""The hand-written adjoint of the emission and secondary-eclipse
-likelihoods, through the C ABI and the torch.autograd.Function.
+"""The hand-written adjoint of the emission, secondary-eclipse and
+transmission likelihoods, through the C ABI and the torch.autograd.Function.
 
 Bar: every gradient component within 1e-4 relative of central finite
 differences (step 1e-4 relative) of the GPU FP64 likelihood, and of the FP64
@@ -55,7 +55,8 @@ def gradient_case(case_factory, tmp_path_factory, name):
 
 
 def steps(p):
-    return STEP * np.maximum(np.abs(p), 1e-12)
+    # relative steps; an absolute one for parameters at 0 (observation shifts)
+    return STEP * np.where(p == 0, 1.0, np.abs(p))
 
 
 def central(f, p, h):
@@ -77,7 +78,12 @@ def check(grad, ref, label, fd_error):
     assert not bad.any(), label
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small", "eclipse_wide"])
+TRANSMISSION = ["transmission_plain", "transmission_variable_gravity", "transmission_fit_mmw",
+                "transmission_fit_scale_height"]
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small", "eclipse_wide"] +
+                         TRANSMISSION)
 def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -92,7 +98,8 @@ def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, nam
         check(grad[k], fd2, f"{name} point {k} vs GPU central differences", np.abs(fd - fd2))
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small", "eclipse_wide"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small", "eclipse_wide", "transmission_plain",
+                                  "transmission_fit_scale_height"])
 def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -172,12 +179,37 @@ def test_gradient_refuses_other_models_and_precisions(case_factory, tmp_path_fac
         r.gradientBatch(np.array(params))
     assert e.value.code == -1
     r.set_table_precision("fp64")
-    folder_t, params_t = case_factory("transmission_plain")
-    rt = synth.open_retrieval(folder_t)
-    with pytest.raises(api.BearError) as e:
-        rt.gradientBatch(np.array(params_t[:1]))
-    assert e.value.code == -1
-    rt.close()
+    # the transmission adjoint has no recomputing variant
+    folder_t, params_t, rt = gradient_case(case_factory, tmp_path_factory, "transmission_plain")
+    rt.set_adjoint_memory(0)
+    try:
+        with pytest.raises(api.BearError) as e:
+            rt.gradientBatch(np.array(params_t[:1]))
+        assert e.value.code == -1
+    finally:
+        rt.set_adjoint_memory(96 << 30)
+
+
+def test_transmission_whole_and_chunked_batches_agree(case_factory, tmp_path_factory):
+    """One chunk for the batch and one point per chunk: the same gradients."""
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, "transmission_plain")
+    r.set_table_precision("fp64")
+    pts = np.array(params)
+    st = r.stats()
+    n, L = st["nb_spectral_points"], st["nb_levels"]
+    stride = (n + 63) // 64 * 64
+    slots = (st["nb_species"] + 3) // 4 * 4 + 4
+    per_point = 4 * L * 3 * stride + 8 * (L * slots * ((n + 63) // 64) + stride + 17 * (L - 1) ** 2)
+    try:
+        ll_s, g_s = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == len(pts)
+        r.set_adjoint_memory(per_point * 3 // 2)
+        ll_c, g_c = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == 1
+    finally:
+        r.set_adjoint_memory(96 << 30)
+    np.testing.assert_array_equal(ll_c, ll_s)
+    np.testing.assert_array_equal(g_c, g_s)
 
 
 def test_neglected_point_has_zero_gradient(case_factory, tmp_path_factory):
