@@ -597,14 +597,15 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 
 // ---------------------------------------------------------------- transmission
 // Adjoint of the transit depth (transmission_spectrum.cpp:32-154) over the
-// kept forward opacity, keep = [point][level][3][stride] FP32 with slot 0 the
-// extinction k_l of the forward batch kernel.  With e = d logL / d depth
+// kept forward quantities, keep = [point][level][3][stride] FP32 with slot 0
+// the extinction k_l of the forward batch kernel.  With e = d logL / d depth
 // 1e6 / R_*^2 (= d logL / dH, depth = (H + R^2) 1e6 / R_*^2) and
 //   H = sum_t c_t (1 - e^-tau_t),  c_t = (R + z_t)(z_{t+1} - z_{t-1}),
 //   tau_t = sum_{q = t}^{last_t} chord[q][t] S_q,  S_q = k_q + k_{q+1},
 // (last_t: the segment at which the sum first passes 10, else L - 2):
-//  1. tangentAdjointKernel: tau_t again, then tau-bar_t = e c_t e^-tau_t into
-//     slot 1 and last_t into slot 2 of level t; sums over wavenumbers of
+//  0. the kept forward (transmissionBatchKernel) stores tau_t in slot 1 and
+//     last_t in slot 2 of level t;
+//  1. tangentAdjointKernel: tau-bar_t = e c_t e^-tau_t over slot 1; sums over wavenumbers of
 //     e (1 - e^-tau_t) (d logL / d c_t) in slot NS+1 of level t, of e (the
 //     R^2 term) in slot NS+1 of level L-1 and of d logL / d depth x depth
 //     (the R_* term) in slot NS+2 of level L-1;
@@ -617,13 +618,13 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 //  4. opacityAdjointKernel as for emission (slot 2 = d logL / d k).
 // The host carries d logL / d chord and d logL / d c_t to the altitudes and
 // the radius.
-__global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, const double* __restrict__ chord,
-                                                              int nb_levels, int n, size_t stride, int tiles,
+constexpr int kAdjointSegments = 16;
+
+__global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int nb_levels, int n, size_t stride, int tiles,
                                                               int batch, int ns, const double* __restrict__ fbar,
                                                               const double* __restrict__ depth,
                                                               float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels, K = ns + 4;
-  const int segments = chordSegments(L), tangents = chordTangents(L);
   const int lane = int(threadIdx.x);
   const size_t level_step = 3 * stride;
   const size_t total = nbItems(tiles, batch);
@@ -637,33 +638,16 @@ __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, const
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     float* kp = keep + size_t(m) * L * level_step + ic;
-    const double* c = chord + size_t(m) * segments * tangents;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double e = fb * 1e6 / (bm->star_radius * bm->star_radius);
     const double r0 = bm->bottom_radius;
     for (int t = 0; t < L - 1; ++t) {
-      double x = 0.0;
-      int last = L - 2;
-      bool done = false;
-      for (int q = t; q < L - 1; ++q) {
-        const double sq = double(kp[size_t(q) * level_step]) + double(kp[size_t(q + 1) * level_step]);
-        if (!done) {
-          x = fma(c[size_t(q) * tangents + t], sq, x);
-          if (x > 10.0) {
-            done = true;
-            last = q;
-          }
-        }
-        if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
-      }
-      const double tr = pow2(-x * kLog2E);
+      float* kt = kp + size_t(t) * level_step + stride;
+      const double tr = pow2(-double(*kt) * kLog2E);
       const double zl = levels[t > 0 ? t - 1 : 0].altitude, zu = levels[t + 1].altitude;
       const double ct = (r0 + levels[t].altitude) * (zu - zl);
-      if (valid) {
-        kp[size_t(t) * level_step + stride] = float(e * ct * tr);
-        kp[size_t(t) * level_step + 2 * stride] = float(last);
-      }
+      if (valid) *kt = float(e * ct * tr);
       double x4[4] = {e * (1.0 - tr), 0.0, 0.0, 0.0};
       emitSums<4>(out + size_t(t) * tiles * K, lane, x4, ns + 1, 2);
     }
@@ -703,6 +687,7 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
   const int nu0 = range * nu_per_range, nu1 = min(n, nu0 + nu_per_range);
   for (int v0 = nu0; v0 < nu1; v0 += kChordNu) {
+    bool reach = false;                              // some path of the block sums a segment >= q0
     for (int e = tid; e < kChordBlock * kChordNu; e += 256) {
       const int row = e / kChordNu, col = e % kChordNu;
       const int v = v0 + col, q = q0 + row, t = t0 + row;
@@ -718,8 +703,9 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
       s_sh[row][col] = sv;
       a_sh[row][col] = av;
       l_sh[row][col] = lv;
+      reach = reach || lv >= float(q0);
     }
-    __syncthreads();
+    if (!__syncthreads_or(reach)) continue;          // every product of this step is masked
     for (int v = 0; v < kChordNu; ++v) {
       double sv[4], av[4];
       float lv[4];
@@ -782,20 +768,50 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     const double* c = chord + size_t(m) * segments * tangents;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
     const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
-    double s_hi = 0.0;                                               // S-bar_{q+1}
-    for (int q = L - 2; q >= -1; --q) {
-      double sb = 0.0;
-      for (int t = 0; t <= q; ++t) {
-        const float* col = kp + size_t(t) * level_step;
-        const double tb = col[stride];
-        const float last = col[2 * stride];
-        sb = fma(float(q) <= last ? tb : 0.0, c[size_t(q) * tangents + t], sb);
+    double s_hi = 0.0;                                               // S-bar of the segment above
+    // kAdjointSegments segments per pass over the paths (one load of
+    // tau-bar_t and last_t for all of them); chord[q][t] = 0 for t > q, and
+    // rows past the last segment are not read
+    for (int qb = (L - 2) / kAdjointSegments * kAdjointSegments; qb >= 0; qb -= kAdjointSegments) {
+      double sb[kAdjointSegments];
+#pragma unroll
+      for (int u = 0; u < kAdjointSegments; ++u) sb[u] = 0.0;
+      const int t_end = min(qb + kAdjointSegments, L - 1);
+      for (int t0 = 0; t0 < t_end; t0 += 4) {          // four paths' values loaded together
+        double tb[4];
+        float last[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float* col = kp + size_t(min(t0 + v, t_end - 1)) * level_step;
+          tb[v] = t0 + v < t_end ? double(col[stride]) : 0.0;
+          last[v] = col[2 * stride];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int t = t0 + v;
+          if (t >= t_end) break;                       // uniform
+          const double* ct = c + size_t(qb) * tangents + t;
+#pragma unroll
+          for (int u = 0; u < kAdjointSegments; ++u)
+            if (qb + u < L - 1) sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[size_t(u) * tangents], sb[u]);
+        }
       }
-      const double kb = sb + s_hi;                                   // d logL / d k_{q+1}
-      if (valid) kp[size_t(q + 1) * level_step + 2 * stride] = float(kb);
+#pragma unroll
+      for (int u = kAdjointSegments - 1; u >= 0; --u) {
+        const int q = qb + u;
+        if (q > L - 2) continue;
+        const double kb = sb[u] + s_hi;                              // d logL / d k_{q+1}
+        if (valid) kp[size_t(q + 1) * level_step + 2 * stride] = float(kb);
+        double x4[4] = {kb * sr, 0.0, 0.0, 0.0};
+        emitSums<4>(out + size_t(q + 1) * tiles * K, lane, x4, ns + 3, 1);
+        s_hi = sb[u];
+      }
+    }
+    {
+      const double kb = s_hi;                                        // d logL / d k_0
+      if (valid) kp[2 * stride] = float(kb);
       double x4[4] = {kb * sr, 0.0, 0.0, 0.0};
-      emitSums<4>(out + size_t(q + 1) * tiles * K, lane, x4, ns + 3, 1);
-      s_hi = sb;
+      emitSums<4>(out, lane, x4, ns + 3, 1);
     }
   }
 }
@@ -805,8 +821,8 @@ void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const dou
                                const double* rayleigh, Records rec, const double* chord, int L, int n, int tiles,
                                int batch, const double* fbar, const double* depth, float* keep, double* partial,
                                int ranges, int nu_per_range, double* chord_partial, double* chord_out) {
-  hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord, L, n, stride, tiles, batch, NS,
-                     fbar, depth, keep, partial);
+  hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, batch, NS, fbar,
+                     depth, keep, partial);
   const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
   hipLaunchKernelGGL(chordAdjointKernel, dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(ranges)),
                      dim3(256), 0, s, keep, L, n, stride, ranges, nu_per_range, chord_partial);

@@ -289,7 +289,9 @@ __device__ __forceinline__ double transmittance(float tau) {
 //     the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R
 //     and depth (R + h)^2 / R_*^2 1e6 in FP64.
-template <class Op>
+// kKeep (FP64 tables, for the adjoint): also keep[point][level][3][i] =
+// {k_l, tau_t, last segment summed by path t} in FP32.
+template <class Op, bool kKeep = false>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
         const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
                                           l == l0 || levels[l].reload != 0));
         const K k = K(kd + sigma_r * levels[l].rayleigh_density);
-        if (keep) keep[(size_t(m) * L + l) * 3 * stride + i] = float(k);
+        if constexpr (kKeep) keep[(size_t(m) * L + l) * 3 * stride + i] = float(k);
         if (l == l0) part[g * tile + j] = double(k);
         else S[(l - 1) * tile + j] = last + k;
         last = k;
@@ -349,7 +351,22 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     double h = 0.0;
     for (int t0 = kTangentBlock * g; t0 < tangents; t0 += kTangentBlock * groups) {
       K tau[kTangentBlock];
-      tangentBlock<K>(S + j, tile, chords, tangents, t0, L - 1, tau);
+      if constexpr (kKeep) {
+        int stop[kTangentBlock];
+        tangentBlock<K, true>(S + j, tile, chords, tangents, t0, L - 1, tau, stop);
+        if (i < n)
+#pragma unroll
+          for (int k = 0; k < kTangentBlock; ++k) {
+            const int t = t0 + k;
+            if (t < L - 1) {
+              float* kt = keep + (size_t(m) * L + t) * 3 * stride + i;
+              kt[stride] = float(tau[k]);
+              kt[2 * stride] = float(stop[k]);
+            }
+          }
+      } else {
+        tangentBlock<K>(S + j, tile, chords, tangents, t0, L - 1, tau);
+      }
 #pragma unroll
       for (int k = 0; k < kTangentBlock; ++k) {
         const int t = t0 + k;
@@ -451,8 +468,12 @@ void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, 
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
-      hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                         rec, chord, chord32, L, n, tiles, tile, batch, group, depth, keep);
+      if (keep)
+        hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true>), grid, block, lds, s, tab64, base, stride,
+                           rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, keep);
+      else
+        hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
+                           rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,

@@ -278,16 +278,20 @@ __device__ __forceinline__ int rowSlot(int row) { return row == 1 ? 2 : row == 2
 // is summed again one segment at a time from its value before the step (the
 // same fused operations in the same order), so it keeps the partial sum at
 // which the reference stops.  Paths already past 10 keep accumulating, which
-// leaves their stored value untouched.
-template <class K>
+// leaves their stored value untouched.  kStop: also the last segment each
+// path summed (nb_segments - 1 when it never passed 10) to stop[].
+template <class K, bool kStop = false>
 __device__ __forceinline__ void tangentBlock(const K* __restrict__ seg, int tile, const K* __restrict__ chord,
-                                             int tangents, int t0, int nb_segments, K (&out)[kTangentBlock]) {
+                                             int tangents, int t0, int nb_segments, K (&out)[kTangentBlock],
+                                             int* stop = nullptr) {
   // fin[k]: the stopping sum once path k has passed 10 (always above 10), 0 before
   K tau[kTangentBlock], fin[kTangentBlock];
+  int last[kTangentBlock];
 #pragma unroll
   for (int k = 0; k < kTangentBlock; ++k) {
     tau[k] = K(0);
     fin[k] = K(0);
+    last[k] = nb_segments - 1;
   }
   for (int q0 = t0; q0 < nb_segments; q0 += kSegmentStep) {
     K sq[kSegmentStep];
@@ -308,15 +312,18 @@ __device__ __forceinline__ void tangentBlock(const K* __restrict__ seg, int tile
       const bool first = past && fin[k] == K(0);
       if (__builtin_amdgcn_ballot_w64(first) != 0) {       // uniform branch: exact re-sum of path k
         K x = prev[k], hit = K(0);
+        int hit_q = 0;
         bool found = false;
 #pragma unroll
         for (int u = 0; u < kSegmentStep; ++u) {
           x = fma(c[u * tangents + k], sq[u], x);
           const bool now = !found && x > K(10);
           hit = now ? x : hit;
+          if constexpr (kStop) hit_q = now ? q0 + u : hit_q;
           found = found || now;
         }
         fin[k] = first ? hit : fin[k];
+        if constexpr (kStop) last[k] = first ? hit_q : last[k];
       }
       all = all && past;
     }
@@ -324,6 +331,9 @@ __device__ __forceinline__ void tangentBlock(const K* __restrict__ seg, int tile
   }
 #pragma unroll
   for (int k = 0; k < kTangentBlock; ++k) out[k] = fin[k] != K(0) ? fin[k] : tau[k];
+  if constexpr (kStop)
+#pragma unroll
+    for (int k = 0; k < kTangentBlock; ++k) stop[k] = last[k];
 }
 
 // Bins (secondary_eclipse.cpp:231-246, forward_model.cpp:161-180,

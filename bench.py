@@ -215,7 +215,8 @@ def config_workload(name, batch, steps):
     single-point rate, each table precision's batched rate and its error
     against the FP64 single-point path on the first four points (bins as
     absolute differences in the result's unit, high-resolution spectrum
-    relative, log-likelihood absolute), and for emission_c4 value + gradient."""
+    relative, log-likelihood absolute), and the FP64 value + gradient rate
+    (hand-written adjoint) with its central-difference check on one point."""
     import torch
     from bear_amd import synth
     builder, default_batch, precisions, unit = CONFIG_WORKLOADS[name]
@@ -272,7 +273,7 @@ def config_workload(name, batch, steps):
                                "launch": {k: bst[k] for k in ("block", "tile", "grid", "point_group", "used_graph")}}
             log(f"{name} batch {precision}: {rate:.1f} evals/s, bins err {err_bins:.3e} {unit}, |dlogL| {err_ll:.2e}")
         out["batched"] = legs
-        if name == "emission_c4":
+        if "fp64" in precisions:
             r.set_table_precision("fp64")
             r.gradientBatch(points)
             r.set_profiling(True)
@@ -288,7 +289,7 @@ def config_workload(name, batch, steps):
             p0 = points[0]
             fd_rel = {}
             for k in range(p0.size):
-                h = 1e-4 * abs(p0[k])
+                h = 1e-4 * (abs(p0[k]) if p0[k] != 0 else 1.0)
                 e = np.zeros_like(p0)
                 e[k] = h
                 fd = (r.computeLikelihood(p0 + e) - r.computeLikelihood(p0 - e)) / (2 * h)
