@@ -618,8 +618,6 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 //  4. opacityAdjointKernel as for emission (slot 2 = d logL / d k).
 // The host carries d logL / d chord and d logL / d c_t to the altitudes and
 // the radius.
-constexpr int kAdjointSegments = 16;
-
 __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int nb_levels, int n, size_t stride, int tiles,
                                                               int batch, int ns, const double* __restrict__ fbar,
                                                               const double* __restrict__ depth,
@@ -749,12 +747,30 @@ __global__ __launch_bounds__(kBlock) void chordReduceKernel(const double* __rest
   out[size_t(m) * P * P + e] = sum;
 }
 
-__global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const double* __restrict__ chord,
+// chord_t[point][t][q] = chord[point][q][t] for t < L - 1, q < segmentRows(L)
+// (zero past the last segment), so that step 3 reads a path's chords over
+// consecutive segments with one scalar load
+constexpr int kAdjointSegments = 16;
+__host__ __device__ constexpr int segmentRows(int nb_levels) {
+  return (nb_levels - 1 + kAdjointSegments - 1) / kAdjointSegments * kAdjointSegments;
+}
+
+__global__ __launch_bounds__(kBlock) void chordTransposeKernel(const double* __restrict__ chord, int nb_levels,
+                                                               double* __restrict__ chord_t) {
+  const int L = nb_levels, P = L - 1, rows = segmentRows(L);
+  const int segments = chordSegments(L), tangents = chordTangents(L);
+  const int e = int(blockIdx.x) * kBlock + int(threadIdx.x), m = int(blockIdx.y);
+  if (e >= P * rows) return;
+  const int t = e / rows, q = e % rows;
+  chord_t[size_t(m) * P * rows + e] = q < P ? chord[(size_t(m) * segments + q) * tangents + t] : 0.0;
+}
+
+__global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const double* __restrict__ chord_t,
                                                               const double* __restrict__ rayleigh, int nb_levels,
                                                               int n, size_t stride, int tiles, int batch, int ns,
                                                               float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels, K = ns + 4;
-  const int segments = chordSegments(L), tangents = chordTangents(L);
+  const int rows = segmentRows(L);
   const int lane = int(threadIdx.x);
   const size_t level_step = 3 * stride;
   const size_t total = nbItems(tiles, batch);
@@ -765,13 +781,13 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
     float* kp = keep + size_t(m) * L * level_step + ic;
-    const double* c = chord + size_t(m) * segments * tangents;
+    const double* c = chord_t + size_t(m) * (L - 1) * rows;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
     const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
     double s_hi = 0.0;                                               // S-bar of the segment above
     // kAdjointSegments segments per pass over the paths (one load of
-    // tau-bar_t and last_t for all of them); chord[q][t] = 0 for t > q, and
-    // rows past the last segment are not read
+    // tau-bar_t and last_t for all of them); chord[q][t] = 0 for t > q and
+    // past the last segment
     for (int qb = (L - 2) / kAdjointSegments * kAdjointSegments; qb >= 0; qb -= kAdjointSegments) {
       double sb[kAdjointSegments];
 #pragma unroll
@@ -790,10 +806,9 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
         for (int v = 0; v < 4; ++v) {
           const int t = t0 + v;
           if (t >= t_end) break;                       // uniform
-          const double* ct = c + size_t(qb) * tangents + t;
+          const double* ct = c + size_t(t) * rows + qb;
 #pragma unroll
-          for (int u = 0; u < kAdjointSegments; ++u)
-            if (qb + u < L - 1) sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[size_t(u) * tangents], sb[u]);
+          for (int u = 0; u < kAdjointSegments; ++u) sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[u], sb[u]);
         }
       }
 #pragma unroll
@@ -820,7 +835,8 @@ template <int NS>
 void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const double* tables, size_t stride,
                                const double* rayleigh, Records rec, const double* chord, int L, int n, int tiles,
                                int batch, const double* fbar, const double* depth, float* keep, double* partial,
-                               int ranges, int nu_per_range, double* chord_partial, double* chord_out) {
+                               int ranges, int nu_per_range, double* chord_partial, double* chord_out,
+                               double* chord_t) {
   hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, batch, NS, fbar,
                      depth, keep, partial);
   const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
@@ -828,7 +844,10 @@ void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const dou
                      dim3(256), 0, s, keep, L, n, stride, ranges, nu_per_range, chord_partial);
   hipLaunchKernelGGL(chordReduceKernel, dim3(unsigned((P * P + kBlock - 1) / kBlock), unsigned(batch)), dim3(kBlock),
                      0, s, chord_partial, P, ranges, chord_out);
-  hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord, rayleigh, L, n, stride, tiles,
+  const int rows = segmentRows(L);
+  hipLaunchKernelGGL(chordTransposeKernel, dim3(unsigned((P * rows + kBlock - 1) / kBlock), unsigned(batch)),
+                     dim3(kBlock), 0, s, chord, L, chord_t);
+  hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord_t, rayleigh, L, n, stride, tiles,
                      batch, NS, keep, partial);
   hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch, keep,
                      partial);
@@ -868,7 +887,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   // (emission and eclipse only); transmission also holds per point the
   // chord gradients and their per-range partial sums
   const int P = L - 1;
-  const size_t chord_point = transmission ? size_t(P) * P * (kChordRanges + 1) : 0;   // doubles per point
+  const size_t chord_point =
+      transmission ? size_t(P) * P * (kChordRanges + 1) + size_t(P) * segmentRows(L) : 0;   // doubles per point
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
   const size_t kept_values = size_t(L) * 3 * stride_;                          // FP32 values per point
   const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_ + chord_point;  // doubles per point
@@ -954,12 +974,13 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
       const int nu_per_range = (per_range + kChordNu - 1) / kChordNu * kChordNu;
       double* chord_partial = adj_partial_ + size_t(cb) * partial_point;
       double* chord_out = chord_partial + size_t(cb) * P * P * kChordRanges;
+      double* chord_t = chord_out + size_t(cb) * P * P;
       switch (NS) {
 #define BEAR_TADJ(NSV)                                                                                             \
   case NSV:                                                                                                        \
     launchTransmissionAdjoint<NSV>(dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, tables_, stride_, rayleigh_,  \
                                    rec, batch_chord_, L, n, tiles, cb, adj_fbar_, batch_flux_ + first * stride_,  \
-                                   keep, adj_partial_, ranges, nu_per_range, chord_partial, chord_out);           \
+                                   keep, adj_partial_, ranges, nu_per_range, chord_partial, chord_out, chord_t);  \
     break;
         BEAR_TADJ(4)
         BEAR_TADJ(8)

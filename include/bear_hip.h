@@ -227,7 +227,8 @@ int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, s
  * intensity less the Planck function + opacity) x levels x spectral points
  * per live point, plus per-point partial sums; transmission: the same 12
  * bytes per level and spectral point for the extinction and the tangent
- * paths' adjoints, plus 17 x 8 bytes x (levels - 1)^2 per point).
+ * paths' adjoints, plus 8 bytes x (levels - 1) x (17 (levels - 1) + the
+ * levels - 1 rounded up to 16) per point).
  * Batches are processed in chunks of as many points as fit; with 0, or when
  * a single point does not fit, the backward kernel recomputes the forward
  * pass instead (emission and eclipse; the transmission gradient then
