@@ -1,9 +1,12 @@
-// Adjoint of the batched emission and secondary-eclipse likelihoods on
-// gfx950: the gradient of log L with respect to every level's temperature,
-// altitude and effective species densities (and the flux scaling), for every
-// live point of a batch.  The host carries it on to the retrieval parameters.
+// Adjoint of the batched emission, secondary-eclipse and transmission
+// likelihoods on gfx950: the gradient of log L with respect to every level's
+// temperature, altitude and effective species densities (and the flux
+// scaling; transmission: the chord lengths and trapezoid coefficients of the
+// tangent paths), for every live point of a batch.  The host carries it on
+// to the retrieval parameters.  The transmission kernels are described at
+// tangentAdjointKernel.
 //
-// Reverse order of the forward chain:
+// Reverse order of the forward chain (emission and eclipse):
 //   d logL / d bin_j = w_j (d_j - m_j) / s_j^2                  (gbarKernel)
 //     eclipse: also d logL / d P_j for the planet's binned flux and the
 //     star's terms                          (eclipseSlopeKernel, eclipseGbarKernel)
