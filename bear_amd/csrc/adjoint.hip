@@ -970,7 +970,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
       // wavenumber ranges of the chord products: enough workgroups to fill the device
       const int blocks = (P + kChordBlock - 1) / kChordBlock;
       const int pairs = blocks * (blocks + 1) / 2;
-      int ranges = int(std::min<size_t>(kChordRanges, (size_t(nb_cus_) * 8 + size_t(pairs) * cb - 1) /
+      int ranges = int(std::min<size_t>(kChordRanges, (size_t(nb_cus_) * 32 + size_t(pairs) * cb - 1) /
                                                           (size_t(pairs) * cb)));
       ranges = std::max(1, std::min(ranges, (n + 511) / 512));
       const int per_range = (n + ranges - 1) / ranges;
