@@ -17,7 +17,7 @@
 //   sums over wavenumbers                                       (adjointReduceKernel)
 //
 // The forward batch kernel keeps both angles' intensity less the Planck
-// function and the opacity at every level ([point][level][3][wavenumber],
+// function and the opacity at every level (keptColumn layout,
 // FP32) for as many points at a time as the budget holds.  rtAdjointKernel
 // runs the short-characteristics recurrence backwards over them (no tables,
 // few registers, many waves) and
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 }
 
 // Backward short characteristics over the kept forward quantities of a
-// chunk, keep = [point][level][3][stride] (I_1, I_2, kappa).  Emits per level
+// chunk (keptColumn layout: I_1 - B, I_2 - B, kappa per level).  Emits per level
 // the altitude, the flux scaling (level 0) and the temperature through the
 // Planck function, and overwrites each level's kappa with d logL / d kappa
 // for opacityAdjointKernel.  ns: stencil slots per level.
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
   const int L = nb_levels;
   const int K = ns + 4;
   const int lane = int(threadIdx.x);
-  const size_t level_step = 3 * stride;
+  const size_t level_step = kKeptLevel;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
-    float* kp = keep + size_t(m) * L * level_step + ic;
+    float* kp = keep + keptColumn(m, stride, L, ic);
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double nu = wavenumber[ic];
     const double b_num = kPlanckC1 * nu * nu * nu;
@@ -446,21 +446,21 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     const float* top = kp + size_t(L - 1) * level_step;
     double b_hi, db_hi;
     planckAdjoint(b_num, x_nu, levels[L - 1].inv_temperature, b_hi, db_hi);
-    const double s_bar = fb * unit * ((double(top[0]) + b_hi) * (kMu1 * kW) + (double(top[stride]) + b_hi) * (kMu2 * kW));
+    const double s_bar = fb * unit * ((double(top[0]) + b_hi) * (kMu1 * kW) + (double(top[kKeptValue]) + b_hi) * (kMu2 * kW));
     double ib1 = fb * unit * bm->scaling * (kMu1 * kW);
     double ib2 = fb * unit * bm->scaling * (kMu2 * kW);
-    double k_hi = top[2 * stride];
+    double k_hi = top[2 * kKeptValue];
     double z_hi = levels[L - 1].altitude;
     double kbar_hi = 0.0, zbar_hi = 0.0, bbar_hi = 0.0;
     const float* below = kp + size_t(L - 2) * level_step;          // the level below, read one step ahead
-    double d1n = below[0], d2n = below[stride], kn = below[2 * stride];
+    double d1n = below[0], d2n = below[kKeptValue], kn = below[2 * kKeptValue];
     for (int l = L - 1; l >= 1; --l) {
       const double d1p = d1n, d2p = d2n, k_lo = kn;
       if (l >= 2) {
         below = kp + size_t(l - 2) * level_step;
         d1n = below[0];
-        d2n = below[stride];
-        kn = below[2 * stride];
+        d2n = below[kKeptValue];
+        kn = below[2 * kKeptValue];
       }
       double b_lo, db_lo;
       planckAdjoint(b_num, x_nu, levels[l - 1].inv_temperature, b_lo, db_lo);
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       kbar_hi += kbar_lo;
       zbar_hi = fma(tbar, k_mean, zbar_hi);
       // every contribution to level l is in
-      if (valid) kp[size_t(l) * level_step + 2 * stride] = float(kbar_hi);
+      if (valid) kp[size_t(l) * level_step + 2 * kKeptValue] = float(kbar_hi);
       double x[4] = {zbar_hi, 0.0, bbar_hi * db_hi, 0.0};
       emitSums<4>(out + size_t(l) * tiles * K, lane, x, ns + 1, 3);
       k_hi = k_lo;
@@ -490,13 +490,13 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       bbar_hi = bbar_lo;
     }
     bbar_hi += ib1 + ib2;                                            // I_0 = B_0
-    if (valid) kp[2 * stride] = float(kbar_hi);
+    if (valid) kp[2 * kKeptValue] = float(kbar_hi);
     double x[4] = {zbar_hi, s_bar, bbar_hi * db_hi, 0.0};
     emitSums<4>(out, lane, x, ns + 1, 3);
   }
 }
 
-// Per level, from d logL / d kappa (keep[point][level][2][i], written by
+// Per level, from d logL / d kappa (value 2 of each level's kept triple, written by
 // rtAdjointKernel): the opacity terms recomputed from the tables as the
 // forward kernels do (register-cached rows, reloaded when they change),
 // n_s d logL / d n_s = sum over wavenumbers of kappa-bar t_s and the
@@ -520,12 +520,12 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
     const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
-    const float* kb = keep + size_t(m) * L * 3 * stride + 2 * stride + ic;
+    const float* kb = keep + keptColumn(m, stride, L, ic) + 2 * kKeptValue;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double* tab = tables + ic;
     double v[NS][4];
     // d logL / d kappa streams from HBM: read two levels ahead
-    double kb1 = kb[0], kb2 = L > 1 ? kb[3 * stride] : 0.0;
+    double kb1 = kb[0], kb2 = L > 1 ? kb[kKeptLevel] : 0.0;
     for (int l = 0; l < L; ++l) {
       const double kbar = valid ? kb1 : 0.0;
       const LevelSpecies* d = species + size_t(l) * NS;
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
           for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
       }
       kb1 = kb2;
-      if (l + 2 < L) kb2 = kb[size_t(l + 2) * 3 * stride];
+      if (l + 2 < L) kb2 = kb[size_t(l + 2) * kKeptLevel];
       double x[K];
       double dk = 0.0;
       // the terms to 1.2e-7 (pow2Gradient): they only weight d logL / d kappa
@@ -600,8 +600,8 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 
 // ---------------------------------------------------------------- transmission
 // Adjoint of the transit depth (transmission_spectrum.cpp:32-154) over the
-// kept forward quantities, keep = [point][level][3][stride] FP32 with slot 0
-// the extinction k_l of the forward batch kernel.  With e = d logL / d depth
+// kept forward quantities (keptColumn layout, FP32) with value 0 the
+// extinction k_l of the forward batch kernel.  With e = d logL / d depth
 // 1e6 / R_*^2 (= d logL / dH, depth = (H + R^2) 1e6 / R_*^2) and
 //   H = sum_t c_t (1 - e^-tau_t),  c_t = (R + z_t)(z_{t+1} - z_{t-1}),
 //   tau_t = sum_{q = t}^{last_t} chord[q][t] S_q,  S_q = k_q + k_{q+1},
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int n
                                                               float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels, K = ns + 4;
   const int lane = int(threadIdx.x);
-  const size_t level_step = 3 * stride;
+  const size_t level_step = kKeptLevel;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
@@ -638,13 +638,13 @@ __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int n
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
-    float* kp = keep + size_t(m) * L * level_step + ic;
+    float* kp = keep + keptColumn(m, stride, L, ic);
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double e = fb * 1e6 / (bm->star_radius * bm->star_radius);
     const double r0 = bm->bottom_radius;
     for (int t = 0; t < L - 1; ++t) {
-      float* kt = kp + size_t(t) * level_step + stride;
+      float* kt = kp + size_t(t) * level_step + kKeptValue;
       const double tr = pow2(-double(*kt) * kLog2E);
       const double zl = levels[t > 0 ? t - 1 : 0].altitude, zu = levels[t + 1].altitude;
       const double ct = (r0 + levels[t].altitude) * (zu - zl);
@@ -679,8 +679,6 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
   const int m = int(blockIdx.y), range = int(blockIdx.z);
   const int q0 = bq * kChordBlock, t0 = bt * kChordBlock;
   const int tid = int(threadIdx.x), tq = tid >> 4, tt = tid & 15;
-  const size_t level_step = 3 * stride;
-  const float* base = keep + size_t(m) * L * level_step;
   double acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -695,10 +693,11 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
       double sv = 0.0, av = 0.0;
       float lv = -1.0f;
       if (v < nu1) {
-        if (q < P) sv = double(base[size_t(q) * level_step + v]) + double(base[size_t(q + 1) * level_step + v]);
+        const float* col = keep + keptColumn(m, stride, L, v);
+        if (q < P) sv = double(col[size_t(q) * kKeptLevel]) + double(col[size_t(q + 1) * kKeptLevel]);
         if (t < P) {
-          av = base[size_t(t) * level_step + stride + v];
-          lv = base[size_t(t) * level_step + 2 * stride + v];
+          av = col[size_t(t) * kKeptLevel + kKeptValue];
+          lv = col[size_t(t) * kKeptLevel + 2 * kKeptValue];
         }
       }
       s_sh[row][col] = sv;
@@ -775,7 +774,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
   const int L = nb_levels, K = ns + 4;
   const int rows = segmentRows(L);
   const int lane = int(threadIdx.x);
-  const size_t level_step = 3 * stride;
+  const size_t level_step = kKeptLevel;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
@@ -783,7 +782,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     const int i = tile * kWave + lane;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
-    float* kp = keep + size_t(m) * L * level_step + ic;
+    float* kp = keep + keptColumn(m, stride, L, ic);
     const double* c = chord_t + size_t(m) * (L - 1) * rows;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
     const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
@@ -802,8 +801,8 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float* col = kp + size_t(min(t0 + v, t_end - 1)) * level_step;
-          tb[v] = t0 + v < t_end ? double(col[stride]) : 0.0;
-          last[v] = col[2 * stride];
+          tb[v] = t0 + v < t_end ? double(col[kKeptValue]) : 0.0;
+          last[v] = col[2 * kKeptValue];
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -819,7 +818,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
         const int q = qb + u;
         if (q > L - 2) continue;
         const double kb = sb[u] + s_hi;                              // d logL / d k_{q+1}
-        if (valid) kp[size_t(q + 1) * level_step + 2 * stride] = float(kb);
+        if (valid) kp[size_t(q + 1) * level_step + 2 * kKeptValue] = float(kb);
         double x4[4] = {kb * sr, 0.0, 0.0, 0.0};
         emitSums<4>(out + size_t(q + 1) * tiles * K, lane, x4, ns + 3, 1);
         s_hi = sb[u];
@@ -827,7 +826,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     }
     {
       const double kb = s_hi;                                        // d logL / d k_0
-      if (valid) kp[2 * stride] = float(kb);
+      if (valid) kp[2 * kKeptValue] = float(kb);
       double x4[4] = {kb * sr, 0.0, 0.0, 0.0};
       emitSums<4>(out, lane, x4, ns + 3, 1);
     }

@@ -189,7 +189,7 @@ struct Opacity32 {
 // point and the flux F = 2 pi sum mu w I 1e-3 scaling goes to flux[point][i].
 // With `keep`, both angles' intensity less the level's Planck function,
 // I - B, and the opacity at every level also go to
-// keep[point][level][0, 1, 2][i] in FP32, for the adjoint's backward sweep
+// keep (keptColumn layout) in FP32, for the adjoint's backward sweep
 // (which recomputes B in FP64: the difference it needs keeps its own
 // relative precision, and the gradient's bar is 1e-4).
 template <class Op>
@@ -217,11 +217,11 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
-    float* kept = keep ? keep + size_t(m) * nb_levels * 3 * stride + i : nullptr;
+    float* kept = keep ? keep + keptColumn(m, stride, nb_levels, i) : nullptr;
     if (kept) {
       kept[0] = 0.0f;                                      // I = B at the bottom
-      kept[stride] = 0.0f;
-      kept[2 * stride] = float(k_prev);
+      kept[kKeptValue] = 0.0f;
+      kept[2 * kKeptValue] = float(k_prev);
     }
     for (int l = 1; l < nb_levels; ++l) {
       const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups, levels[l].reload != 0));
@@ -229,10 +229,10 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
       const double z_next = levels[l].altitude;
       shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
       if (kept) {
-        float* at = kept + size_t(l) * 3 * stride;
+        float* at = kept + size_t(l) * kKeptLevel;
         at[0] = float(i1 - b_next);
-        at[stride] = float(i2 - b_next);
-        at[2 * stride] = float(k_next);
+        at[kKeptValue] = float(i2 - b_next);
+        at[2 * kKeptValue] = float(k_next);
       }
       k_prev = k_next;
       b_prev = b_next;
@@ -289,8 +289,8 @@ __device__ __forceinline__ double transmittance(float tau) {
 //     the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R
 //     and depth (R + h)^2 / R_*^2 1e6 in FP64.
-// kKeep (FP64 tables, for the adjoint): also keep[point][level][3][i] =
-// {k_l, tau_t, last segment summed by path t} in FP32.
+// kKeep (FP64 tables, for the adjoint): also, in the keptColumn layout,
+// {k_l, tau_t, last segment summed by path t} per level in FP32.
 template <class Op, bool kKeep = false>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
         const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
                                           l == l0 || levels[l].reload != 0));
         const K k = K(kd + sigma_r * levels[l].rayleigh_density);
-        if constexpr (kKeep) keep[(size_t(m) * L + l) * 3 * stride + i] = float(k);
+        if constexpr (kKeep) keep[keptColumn(m, stride, L, i) + size_t(l) * kKeptLevel] = float(k);
         if (l == l0) part[g * tile + j] = double(k);
         else S[(l - 1) * tile + j] = last + k;
         last = k;
@@ -359,9 +359,9 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
           for (int k = 0; k < kTangentBlock; ++k) {
             const int t = t0 + k;
             if (t < L - 1) {
-              float* kt = keep + (size_t(m) * L + t) * 3 * stride + i;
-              kt[stride] = float(tau[k]);
-              kt[2 * stride] = float(stop[k]);
+              float* kt = keep + keptColumn(m, stride, L, i) + size_t(t) * kKeptLevel;
+              kt[kKeptValue] = float(tau[k]);
+              kt[2 * kKeptValue] = float(stop[k]);
             }
           }
       } else {

@@ -416,6 +416,18 @@ __device__ __forceinline__ void binFinalizeBody(const double* __restrict__ chunk
   }
 }
 
+// The adjoint's kept forward quantities, FP32 [point][64-wavenumber tile]
+// [level][3][64] (per level three values: emission I_1 - B, I_2 - B, kappa;
+// transmission k, tau, last segment), so that a wavefront's store or load
+// of one value of one level is one contiguous 256-byte run next to the
+// level's other values.  keptColumn: offset of (point m, level 0, value 0,
+// wavenumber i); level l at + kKeptLevel l, value v at + kKeptValue v.
+constexpr int kKeptValue = 64, kKeptLevel = 3 * kKeptValue;
+__device__ __forceinline__ size_t keptColumn(int m, size_t stride, int levels, int i) {
+  return (size_t(m) * (stride / kKeptValue) + size_t(i / kKeptValue)) * size_t(levels) * kKeptLevel +
+         size_t(i % kKeptValue);
+}
+
 // ---------------------------------------------------------------- batch work items
 // Per-point records of a batch: [BatchModel][LevelState L][stencils L x NS]
 // [adjoint stencils L x NS (FP64 tables)][shifts]

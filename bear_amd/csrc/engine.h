@@ -309,7 +309,7 @@ class DeviceEngine {
   unsigned char* record(size_t b) { return batch_host_ + b * rec_bytes_; }
   // points [first, first + batch) of the staged records; keep: also store
   // every level's I - B of both angles and opacity, FP32
-  // [batch][levels][3][stride] (emission; transmission: the extinction in
+  // [batch][64-wavenumber tile][levels][3][64] (emission; transmission: the extinction in
   // the first of the three values)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
                            float* keep = nullptr);
@@ -352,7 +352,7 @@ class DeviceEngine {
   double* adj_fscale_ = nullptr;                 // eclipse: dL/d(planet bin) [capacity][bins]
   double* adj_terms_ = nullptr;                  // eclipse: [capacity][bins][2]
   double* adj_slope_ = nullptr;                  // eclipse: [capacity][pieces]
-  double* adj_intensity_ = nullptr;              // kept: FP32 [chunk][levels][3][stride]; recomputing: [grid][levels][2][64]
+  double* adj_intensity_ = nullptr;              // kept: FP32 [chunk][tile][levels][3][64]; recomputing: [grid][levels][2][64]
   double* adj_partial_ = nullptr;                // [chunk][levels][slots][tiles]
   double* adj_out_ = nullptr;                    // [capacity][levels][slots]
   size_t adj_capacity_ = 0, adj_grid_ = 0, adj_partial_size_ = 0, adj_intensity_size_ = 0;
