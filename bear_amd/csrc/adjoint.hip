@@ -667,8 +667,8 @@ constexpr int kChordBlock = 64, kChordNu = 32, kChordRanges = 16;
 __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restrict__ keep, int nb_levels, int n,
                                                           size_t stride, int ranges, int nu_per_range,
                                                           double* __restrict__ partial) {
-  __shared__ double s_sh[kChordBlock][kChordNu + 1];
-  __shared__ double a_sh[kChordBlock][kChordNu + 1];
+  __shared__ float s_sh[kChordBlock][kChordNu + 1];           // S_q in FP32 (the gradient's inputs are FP32)
+  __shared__ float a_sh[kChordBlock][kChordNu + 1];
   __shared__ float l_sh[kChordBlock][kChordNu + 1];
   const int L = nb_levels, P = L - 1;
   int bq = 0, bt = int(blockIdx.x);                                 // triangular block index
@@ -690,11 +690,10 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
     for (int e = tid; e < kChordBlock * kChordNu; e += 256) {
       const int row = e / kChordNu, col = e % kChordNu;
       const int v = v0 + col, q = q0 + row, t = t0 + row;
-      double sv = 0.0, av = 0.0;
-      float lv = -1.0f;
+      float sv = 0.0f, av = 0.0f, lv = -1.0f;
       if (v < nu1) {
         const float* col = keep + keptColumn(m, stride, L, v);
-        if (q < P) sv = double(col[size_t(q) * kKeptLevel]) + double(col[size_t(q + 1) * kKeptLevel]);
+        if (q < P) sv = float(double(col[size_t(q) * kKeptLevel]) + double(col[size_t(q + 1) * kKeptLevel]));
         if (t < P) {
           av = col[size_t(t) * kKeptLevel + kKeptValue];
           lv = col[size_t(t) * kKeptLevel + 2 * kKeptValue];
@@ -710,10 +709,10 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
       double sv[4], av[4];
       float lv[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) sv[a] = s_sh[tq + 16 * a][v];
+      for (int a = 0; a < 4; ++a) sv[a] = double(s_sh[tq + 16 * a][v]);
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        av[b] = a_sh[tt + 16 * b][v];
+        av[b] = double(a_sh[tt + 16 * b][v]);
         lv[b] = l_sh[tt + 16 * b][v];
       }
 #pragma unroll
