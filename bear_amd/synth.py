@@ -95,6 +95,9 @@ class Case:
     use_error_inflation: bool = False
     model: str = "emission"
     extra: dict = field(default_factory=dict)
+    # GlobalConfig::spectral_disecretisation: 0 const_wavenumber (spectral
+    # resolution in cm^-1), 1 const_wavelength (micron), 2 const_resolution (R)
+    discretisation: int = 0
 
     @property
     def opacity_entries(self):
@@ -518,7 +521,19 @@ def wide_bands(case: Case, seed, bands):
     return case
 
 
+def resampled(case: Case, discretisation, resolution, name):
+    """The case on a grid sampled at constant wavelength steps (micron,
+    discretisation 1) or constant resolving power (discretisation 2)."""
+    case.discretisation = discretisation
+    case.spectral_resolution = resolution
+    case.name = name
+    return case
+
+
 CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2,
+         # tiny at constant wavelength steps of 2 nm and at R = 1500
+         "tiny_const_wavelength": lambda: resampled(tiny_case(), 1, 2e-3, "tiny_const_wavelength"),
+         "tiny_const_resolution": lambda: resampled(tiny_case(), 2, 1500.0, "tiny_const_resolution"),
          # emission_c1 grid: 1000-10000 cm^-1 at 0.45; bands of 14,800 and 2,560 points
          "emission_wide": lambda: wide_bands(emission_c1(), 71, [(1.25, 7.5, "Photon"), (2.0, 2.6, "Energy")]),
          # eclipse_small: 2500-7000 cm^-1 sampled every 2 cm^-1; a band of 2,180 points
@@ -629,6 +644,7 @@ def write_case(case: Case, folder: str) -> str:
         "parameters": [float(v) for v in case.parameters],
         "use_error_inflation": case.use_error_inflation,
         "extra": case.extra,
+        "discretisation": case.discretisation,
     }
     with open(folder + "case.json", "w") as f:
         json.dump(meta, f, indent=1)
@@ -636,6 +652,7 @@ def write_case(case: Case, folder: str) -> str:
     with open(folder + "case.txt", "w") as f:
         f.write(f"model {case.model}\n")
         f.write(f"spectral_resolution {_fmt(case.spectral_resolution)}\n")
+        f.write(f"spectral_discretisation {case.discretisation}\n")
         f.write(f"nb_levels {case.nb_levels}\n")
         f.write(f"p_bottom {_fmt(case.p_bottom)}\n")
         f.write(f"p_top {_fmt(case.p_top)}\n")
@@ -678,15 +695,20 @@ def open_retrieval(folder, device=0):
     meta = json.load(open(folder + "case.json"))
     cfg = api.GlobalConfig(retrieval_folder_path=folder, cross_section_file_path=folder + "opacity/",
                            wavenumber_file_path=folder + "wavenumber_full.dat",
+                           spectral_disecretisation=DISCRETISATIONS[meta.get("discretisation", 0)],
                            spectral_resolution=meta["spectral_resolution"],
                            use_error_inflation=meta["use_error_inflation"], device=device)
     return api.Retrieval(cfg, _model_config(meta), observation_files=meta["observations"],
                          observation_modifiers=meta["modifiers"], forward_model_type=meta["model"])
 
 
+DISCRETISATIONS = ("const_wavenumber", "const_wavelength", "const_resolution")
+
+
 def case_meta(case: Case):
     return {
         "spectral_resolution": case.spectral_resolution, "nb_levels": case.nb_levels,
+        "discretisation": case.discretisation,
         "p_bottom": case.p_bottom, "p_top": case.p_top,
         "temperature_model": list(case.temperature_model),
         "chemistry": [[c[0], list(c[1])] for c in case.chemistry],
@@ -734,6 +756,7 @@ def open_retrieval_in_memory(case: Case, scratch_folder, device=0):
     meta = case_meta(case)
     cfg = api.GlobalConfig(retrieval_folder_path=scratch_folder, cross_section_file_path="",
                            wavenumber_file_path=wn, spectral_resolution=meta["spectral_resolution"],
+                           spectral_disecretisation=DISCRETISATIONS[case.discretisation],
                            use_error_inflation=meta["use_error_inflation"], device=device)
     r = api.Retrieval(cfg, _model_config(meta), observations=observation_inputs(case), forward_model_type=case.model)
     api.clear_opacity_data()

@@ -53,12 +53,43 @@ def lib():
         L.oracle_coefficients.argtypes = [vp, dp, dp]
         L.oracle_planck.restype = C.c_double
         L.oracle_planck.argtypes = [C.c_double, C.c_double]
+        i32 = C.POINTER(C.c_int32)
+        L.oracle_register_table.argtypes = [C.c_char_p, C.c_double, C.c_int, sz, dp, dp, sz, C.POINTER(C.c_float)]
+        L.oracle_transmission_record_stops.argtypes = [vp, C.c_int]
+        L.oracle_transmission_stops.argtypes = [vp, i32]
+        L.oracle_transmission_freeze_stops.argtypes = [vp, i32]
+        L.oracle_transmission_paths.argtypes = [vp, sz, dp, i32]
+        L.oracle_tangent_sum.restype = C.c_double
+        L.oracle_tangent_sum.argtypes = [dp, dp, sz, C.c_int32, i32]
         _lib = L
     return _lib
 
 
 def _p(a, t=C.c_double):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+# arrays of the in-memory tables, kept alive while registered
+_REGISTERED = {}
+
+
+def register_table(folder, temperatures, pressures, rows, species_mass=0.0, log_storage=False):
+    """Hold one species' table in memory under ``folder`` (in place of
+    opacity/<folder>/filelist.dat and its row files): rows [nb_rows,
+    nb_points] float32 over the full wavenumber list with the (T, p) of each
+    row.  The oracle references the arrays; they stay alive here until
+    clear_tables()."""
+    rows = np.ascontiguousarray(rows, dtype=np.float32).reshape(len(temperatures), -1)
+    t = np.ascontiguousarray(temperatures, dtype=np.float64)
+    p = np.ascontiguousarray(pressures, dtype=np.float64)
+    _REGISTERED[folder] = (rows, t, p)
+    lib().oracle_register_table(folder.encode(), float(species_mass), int(log_storage), t.size, _p(t), _p(p),
+                                rows.shape[1], _p(rows, C.c_float))
+
+
+def clear_tables():
+    lib().oracle_clear_tables()
+    _REGISTERED.clear()
 
 
 class EmissionOracle:
@@ -119,6 +150,37 @@ class EmissionOracle:
             k += nb
         return dict(spectrum_flux=flux, spectrum=spec, bins=out, loglike=ll.value, neglect=bool(neg.value))
 
+    # ---- transmission: stopping segments of the tangent paths
+    def record_stops(self, enable=True):
+        self._L.oracle_transmission_record_stops(self._h, int(enable))
+
+    def stops(self):
+        """Stopping segment of every (wavenumber, tangent path) of the last
+        evaluation, [N, L-1] (recording on)."""
+        out = np.empty((self.nb_points, self.nb_levels - 1), np.int32)
+        if self._L.oracle_transmission_stops(self._h, _p(out, C.c_int32)) != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+        return out
+
+    def freeze_stops(self, stops):
+        """Hold every path's stopping segment fixed in the following
+        evaluations (None: back to the tau > 10 rule)."""
+        if stops is None:
+            self._L.oracle_transmission_freeze_stops(self._h, None)
+            return
+        s = np.ascontiguousarray(stops, dtype=np.int32)
+        assert s.shape == (self.nb_points, self.nb_levels - 1)
+        self._L.oracle_transmission_freeze_stops(self._h, _p(s, C.c_int32))
+
+    def paths(self, w):
+        """(tau [L-1], stop [L-1]) of the tangent paths at wavenumber index w
+        in the last evaluation."""
+        tau = np.empty(self.nb_levels - 1)
+        stop = np.empty(self.nb_levels - 1, np.int32)
+        if self._L.oracle_transmission_paths(self._h, int(w), _p(tau), _p(stop, C.c_int32)) != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+        return tau, stop
+
     def atmosphere(self):
         L = self.nb_levels
         t, z, p, mmw = np.empty(L), np.empty(L), np.empty(L), np.empty(L)
@@ -131,6 +193,17 @@ class EmissionOracle:
         s = np.empty_like(k)
         self._L.oracle_coefficients(self._h, _p(k), _p(s))
         return k, s
+
+
+def tangent_sum(path, extinction, frozen=-1):
+    """One tangent path's optical depth on given segment lengths [n] and
+    level extinctions [n + 1]: (tau, last segment summed)."""
+    path = np.ascontiguousarray(path, dtype=np.float64)
+    ext = np.ascontiguousarray(extinction, dtype=np.float64)
+    assert ext.size == path.size + 1
+    last = C.c_int32(0)
+    tau = lib().oracle_tangent_sum(_p(path), _p(ext), path.size, int(frozen), C.byref(last))
+    return tau, last.value
 
 
 def planck(t, nu):

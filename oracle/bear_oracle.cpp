@@ -514,8 +514,32 @@ std::vector<double> processSpectrum(const Grid& g, const Observation& o, const s
 }
 
 // ---------------------------------------------------------------- opacity
-// one tabulated row after sampling (sampled_data.cpp:48-87, cross_section_file.cpp:33-68)
-struct Row { double temperature, pressure, log_pressure; std::vector<double> logsigma; };
+// Tables held in memory instead of a filelist.dat folder (the oracle's
+// counterpart of the engine's bear_opacity_data_register): the row values are
+// what the row files would hold, float32 over the full wavenumber list.  The
+// caller keeps the arrays alive while oracles built from them exist.
+struct MemoryTable {
+  double mass = 0;
+  bool log_storage = false;
+  std::vector<double> temperature, pressure;
+  size_t nb_points = 0;
+  const float* rows = nullptr;             // [rows][nb_points]
+};
+std::map<std::string, MemoryTable>& memoryTables() {
+  static std::map<std::string, MemoryTable> tables;
+  return tables;
+}
+
+// one tabulated row after sampling (sampled_data.cpp:48-87, cross_section_file.cpp:33-68);
+// logsigma is filled on first use, as the reference loads a row's data the
+// first time an evaluation touches it (opacity_species.cpp:349-365)
+struct Row {
+  double temperature, pressure, log_pressure;
+  std::vector<double> logsigma;
+  std::string file;                        // row file, or empty
+  const float* memory = nullptr;           // in-memory row over the full list
+  size_t memory_points = 0;
+};
 
 struct OpacitySpecies {
   std::string symbol;
@@ -523,48 +547,38 @@ struct OpacitySpecies {
   std::vector<int> partners;
   bool rayleigh = false;
   double mass = 0;
+  bool log_storage = false;
   std::vector<Row> rows;
   std::vector<std::vector<Row*>> ordered;        // by temperature, then pressure
   std::vector<double> rayleigh_sigma;
 
-  void loadTables(const std::string& dir, const std::vector<size_t>& index) {  // opacity_species.cpp:66-157
-    std::ifstream f(dir + "/filelist.dat");
-    if (!f) throw std::runtime_error("no opacity data for " + symbol);
-    std::string line, storage;
-    std::getline(f, line);
-    std::istringstream head(line);
-    head >> mass >> storage;
-    const bool log_storage = storage == "log";
-    std::vector<std::string> names;
-    while (std::getline(f, line)) {
-      std::istringstream in(line);
-      double t, p;
-      std::string name;
-      in >> p >> t >> name;
-      if (name.empty()) continue;
-      rows.push_back(Row{t, p, std::log10(p), {}});
-      names.push_back(name);
-    }
-    for (size_t r = 0; r < rows.size(); ++r) {
-      std::ifstream bf(dir + "/" + names[r], std::ios::binary | std::ios::ate);
-      if (!bf) throw std::runtime_error("missing row file " + names[r]);
-      const size_t n = size_t(bf.tellg()) / sizeof(float);
-      bf.seekg(0);
-      std::vector<float> raw(n);
-      bf.read(reinterpret_cast<char*>(raw.data()), n * sizeof(float));
-      std::vector<double>& cs = rows[r].logsigma;
-      cs.assign(index.size(), 0.0);
-      for (size_t i = 0; i < index.size(); ++i) {
-        if (index[i] > n - 1) break;
-        double v = raw[index[i]];
-        if (log_storage) v = std::pow(10.0, v);
-        cs[i] = v;
+  void loadTables(const std::string& dir, const std::string& folder_key) {  // opacity_species.cpp:66-157
+    auto mem = memoryTables().find(folder_key);
+    if (mem != memoryTables().end()) {
+      const MemoryTable& m = mem->second;
+      mass = m.mass;
+      log_storage = m.log_storage;
+      for (size_t r = 0; r < m.temperature.size(); ++r) {
+        rows.push_back(Row{m.temperature[r], m.pressure[r], std::log10(m.pressure[r]), {}, "", nullptr, 0});
+        rows.back().memory = m.rows + r * m.nb_points;
+        rows.back().memory_points = m.nb_points;
       }
-      for (auto& v : cs) {
-        if (mass > 0) v *= mass / kAvogadro;
-        if (v < 1e-200) v = 1e-200;
+    } else {
+      std::ifstream f(dir + "/filelist.dat");
+      if (!f) throw std::runtime_error("no opacity data for " + symbol);
+      std::string line, storage;
+      std::getline(f, line);
+      std::istringstream head(line);
+      head >> mass >> storage;
+      log_storage = storage == "log";
+      while (std::getline(f, line)) {
+        std::istringstream in(line);
+        double t, p;
+        std::string name;
+        in >> p >> t >> name;
+        if (name.empty()) continue;
+        rows.push_back(Row{t, p, std::log10(p), {}, dir + "/" + name, nullptr, 0});
       }
-      for (auto& v : cs) v = std::log10(v);
     }
     for (auto& r : rows) {
       auto it = std::find_if(ordered.begin(), ordered.end(),
@@ -576,6 +590,41 @@ struct OpacitySpecies {
               [](const std::vector<Row*>& a, const std::vector<Row*>& b) { return a[0]->temperature < b[0]->temperature; });
     for (auto& t : ordered)
       std::sort(t.begin(), t.end(), [](Row* a, Row* b) { return a->pressure < b->pressure; });
+  }
+
+  // sample one row at the grid's indices (sampled_data.cpp:48-87): float32
+  // value -> double, 10^x for log storage, x mass / N_A, 1e-200 floor, log10
+  void ensureRow(Row& row, const std::vector<size_t>& index) const {
+    if (!row.logsigma.empty() || index.empty()) return;
+    std::vector<float> raw;
+    const float* data = row.memory;
+    size_t n = row.memory_points;
+    if (!data) {
+      std::ifstream bf(row.file, std::ios::binary | std::ios::ate);
+      if (!bf) throw std::runtime_error("missing row file " + row.file);
+      n = size_t(bf.tellg()) / sizeof(float);
+      bf.seekg(0);
+      raw.resize(n);
+      bf.read(reinterpret_cast<char*>(raw.data()), n * sizeof(float));
+      data = raw.data();
+    }
+    std::vector<double>& cs = row.logsigma;
+    cs.assign(index.size(), 0.0);
+    const double factor = mass / kAvogadro;
+    #pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < index.size(); ++i) {
+      if (index[i] > n - 1) continue;
+      double v = data[index[i]];
+      if (log_storage) v = std::pow(10.0, v);
+      cs[i] = v;
+    }
+    #pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < cs.size(); ++i) {
+      double v = cs[i];
+      if (mass > 0) v *= factor;
+      if (v < 1e-200) v = 1e-200;
+      cs[i] = std::log10(v);
+    }
   }
 
   // opacity_species.cpp:160-273
@@ -636,6 +685,20 @@ struct OpacitySpecies {
 double rayleighGeneral(double ref_density, double n_refr, double king, double nu) {
   return 24. * std::pow(kPi, 3) * std::pow(nu, 4) / (ref_density * ref_density)
          * std::pow((n_refr * n_refr - 1) / (n_refr * n_refr + 2), 2) * king;
+}
+
+// One tangent path's optical depth (transmission_spectrum.cpp:110-140):
+// segment k (level k to k+1 of the path, n segments) adds path[k] (ext[k] +
+// ext[k+1]); the sum stops after the first partial sum above 10, or after
+// segment `frozen` when that is >= 0.  *last: the last segment summed.
+double tangentSum(const double* path, const double* ext, size_t n, int32_t frozen, int32_t* last) {
+  double sum = 0;
+  *last = int32_t(n) - 1;
+  for (size_t k = 0; k < n; ++k) {
+    sum += path[k] * (ext[k] + ext[k + 1]);
+    if (frozen >= 0 ? int32_t(k) == frozen : sum > 10) { *last = int32_t(k); break; }
+  }
+  return sum;
 }
 
 // ---------------------------------------------------------------- atmosphere
@@ -728,6 +791,11 @@ struct EmissionOracle {
   std::vector<std::vector<double>> bins;
   bool neglect = false;
   double log_like = 0;
+  // transmission: stopping segments of the last evaluation (when recorded) and
+  // stopping segments imposed on the next ones (when set), [N][L-1]
+  bool record_stops = false;
+  std::vector<int32_t> recorded_stops, frozen_stops;
+  double last_bottom_radius = 0;
 
   void load(const std::string& folder_in) {
     std::string folder = folder_in;
@@ -753,6 +821,7 @@ struct EmissionOracle {
       if (key == "fit_scale_height") fit_scale_height = r[0] == "1";
       if (key == "stellar_model" && r[0] != "blackbody") throw std::runtime_error("stellar model not restated: " + r[0]);
       if (key == "spectral_resolution") grid.resolution = std::stod(r[0]);
+      if (key == "spectral_discretisation") grid.discretisation = std::stoi(r[0]);   // global_config.cpp:147-168
       if (key == "nb_levels") nb_levels = std::stoul(r[0]);
       if (key == "p_bottom") p_bottom = std::stod(r[0]);
       if (key == "p_top") p_top = std::stod(r[0]);
@@ -773,7 +842,7 @@ struct EmissionOracle {
       if (key == "parameters")
         for (size_t i = 0; i < std::stoul(r[0]); ++i) default_parameters.push_back(std::stod(r[1 + i]));
     }
-    grid.discretisation = 0;
+    if (grid.discretisation < 0 || grid.discretisation > 2) throw std::runtime_error("unknown spectral discretisation");
     grid.load(folder + "wavenumber_full.dat");
 
     std::ifstream lf(folder + "observations.list");
@@ -822,7 +891,7 @@ struct EmissionOracle {
       } else if (e.second == "Rayleigh") {
         throw std::runtime_error("Rayleigh species not restated: " + e.first);
       } else {
-        s.loadTables(folder + "opacity/" + e.second, grid.index);
+        s.loadTables(folder + "opacity/" + e.second, e.second);
       }
       species.push_back(std::move(s));
     }
@@ -927,6 +996,14 @@ struct EmissionOracle {
   // (opacity_calc.h:154-182, transport_coeff.cpp:242-260, opacity_species.cpp:400-455)
   void opacity() {
     const size_t N = grid.nu.size(), L = nb_levels;
+    // rows the evaluation touches, sampled on first use
+    for (size_t l = 0; l < L; ++l)
+      for (auto& s : species) {
+        if (s.rows.empty()) continue;
+        Row* d[4] = {nullptr, nullptr, nullptr, nullptr};
+        s.bracket(pressure[l], temperature[l], d);
+        for (Row* r : d) s.ensureRow(*r, grid.index);
+      }
     kappa.assign(L, std::vector<double>(N, 0.0));
     scattering.assign(L, std::vector<double>(N, 0.0));
     #pragma omp parallel for schedule(dynamic, 1)
@@ -978,29 +1055,41 @@ struct EmissionOracle {
   // (transmission_spectrum.cpp:32-154): per wavenumber, the tangent optical
   // depth of every level (trapezoid along the chord, both hemispheres, stop
   // once tau > 10), the effective tangent height and the transit depth in ppm
-  void transmissionSpectrum(double bottom_radius, double star_radius) {
-    const size_t N = grid.nu.size(), L = nb_levels;
-    spectrum_flux.assign(N, 0.0);
+  // Tangent optical depths at wavenumber w (transmission_spectrum.cpp:84-140):
+  // tau[t] summed outward from level t, segment by segment, stopping after the
+  // first partial sum above 10; stop[t] is the last segment summed (L - 2 when
+  // the sum never passes 10).  With frozen stops ([N][L-1], test
+  // infrastructure for derivatives with the stopping segments held fixed) each
+  // path stops at its given segment instead.
+  void tangentDepths(size_t w, double bottom_radius, double* tau, int32_t* stop) const {
+    const size_t L = nb_levels;
     auto dist = [&](size_t tangent, size_t level) {
       const double a = bottom_radius + altitude[level];
       const double b = bottom_radius + altitude[tangent];
       return std::sqrt(a * a - b * b);
     };
+    const int32_t* frozen = frozen_stops.empty() ? nullptr : frozen_stops.data() + w * (L - 1);
+    std::vector<double> path(L - 1), ext(L);
+    for (size_t i = 0; i < L; ++i) ext[i] = kappa[i][w] + scattering[i][w];
+    for (int t = int(L) - 2; t > -1; --t) {
+      const size_t n = L - 1 - size_t(t);
+      for (size_t k = 0; k < n; ++k) path[k] = dist(size_t(t), size_t(t) + k + 1) - dist(size_t(t), size_t(t) + k);
+      int32_t last = 0;
+      tau[t] = tangentSum(path.data(), ext.data() + t, n, frozen ? frozen[t] - t : -1, &last);
+      if (stop) stop[t] = last + t;
+    }
+  }
+
+  void transmissionSpectrum(double bottom_radius, double star_radius) {
+    const size_t N = grid.nu.size(), L = nb_levels;
+    spectrum_flux.assign(N, 0.0);
+    if (record_stops) recorded_stops.assign(N * (L - 1), 0);
     #pragma omp parallel for schedule(dynamic, 256)
     for (size_t w = 0; w < N; ++w) {
-      std::vector<double> trans(L, 0.0);
+      std::vector<double> trans(L, 0.0), tau(L - 1, 0.0);
+      tangentDepths(w, bottom_radius, tau.data(), record_stops ? recorded_stops.data() + w * (L - 1) : nullptr);
       trans.back() = 1;
-      for (int t = int(L) - 2; t > -1; --t) {
-        double tau = 0;
-        for (size_t i = size_t(t); i < L - 1; ++i) {
-          const double path = dist(size_t(t), i + 1) - dist(size_t(t), i);
-          const double e1 = kappa[i][w] + scattering[i][w];
-          const double e2 = kappa[i + 1][w] + scattering[i + 1][w];
-          tau += path * (e1 + e2);
-          if (tau > 10) break;
-        }
-        trans[size_t(t)] = std::exp(-tau);
-      }
+      for (size_t t = 0; t + 1 < L; ++t) trans[t] = std::exp(-tau[t]);
       double h = 0;
       for (size_t i = 0; i < L - 1; i++)
         h += 2. * ((bottom_radius + altitude[i]) * (1. - trans[i]) + (bottom_radius + altitude[i + 1]) * (1. - trans[i + 1]))
@@ -1035,6 +1124,7 @@ struct EmissionOracle {
       neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp, p[1], fit_mmw ? fit : 0.0,
                                     fit_scale_height ? fit : 0.0);
       opacity();
+      last_bottom_radius = p[1];
       transmissionSpectrum(p[1], p[2]);
       for (size_t i = 0; i < obs.size(); ++i) bins[i] = processSpectrum(grid, obs[i], spectrum_flux, false);
       applyModifiers(p, mods);
@@ -1178,6 +1268,62 @@ void oracle_coefficients(void* h, double* kappa, double* scattering) {
 }
 
 int oracle_nb_species_ids() { return kNbSpecies; }
+
+// one tangent path's sum (tangentSum) on given segment lengths [n] and extinctions [n + 1]
+double oracle_tangent_sum(const double* path, const double* ext, size_t n, int32_t frozen, int32_t* last) {
+  return tangentSum(path, ext, n, frozen, last);
+}
+
+// tables in memory (see MemoryTable); the arrays are referenced, not copied
+int oracle_register_table(const char* folder, double mass, int log_storage, size_t nb_rows, const double* temperatures,
+                          const double* pressures, size_t nb_points, const float* rows) {
+  MemoryTable t;
+  t.mass = mass;
+  t.log_storage = log_storage != 0;
+  t.temperature.assign(temperatures, temperatures + nb_rows);
+  t.pressure.assign(pressures, pressures + nb_rows);
+  t.nb_points = nb_points;
+  t.rows = rows;
+  memoryTables()[folder] = t;
+  return 0;
+}
+
+void oracle_clear_tables() { memoryTables().clear(); }
+
+// transmission: record the stopping segments of the following evaluations
+void oracle_transmission_record_stops(void* h, int enable) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  m->record_stops = enable != 0;
+  if (!enable) m->recorded_stops.clear();
+}
+
+// the last evaluation's stopping segments [N][L-1] (recording on)
+int oracle_transmission_stops(void* h, int32_t* out) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  if (m->recorded_stops.empty()) { g_error = "no recorded stopping segments"; return -1; }
+  std::copy(m->recorded_stops.begin(), m->recorded_stops.end(), out);
+  return 0;
+}
+
+// impose stopping segments [N][L-1] on the following evaluations; NULL: the reference's rule
+void oracle_transmission_freeze_stops(void* h, const int32_t* stops) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  if (!stops) { m->frozen_stops.clear(); return; }
+  const size_t n = m->grid.nu.size() * (m->nb_levels - 1);
+  m->frozen_stops.assign(stops, stops + n);
+}
+
+// tangent optical depths tau[L-1] and stopping segments stop[L-1] at
+// wavenumber index w, from the last evaluation's extinction and altitudes
+int oracle_transmission_paths(void* h, size_t w, double* tau, int32_t* stop) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  if (m->model != "transmission" || m->kappa.empty() || w >= m->grid.nu.size()) {
+    g_error = "no transmission evaluation at this wavenumber";
+    return -1;
+  }
+  m->tangentDepths(w, m->last_bottom_radius, tau, stop);
+  return 0;
+}
 
 // closed-form helpers exposed for the known-answer tests
 double oracle_planck(double t, double nu) { return planck(t, nu); }

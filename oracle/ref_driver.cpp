@@ -69,6 +69,7 @@ struct CaseText {
   bool variable_gravity = false, fit_mmw = false, fit_scale_height = false;
   size_t nb_levels = 0;
   double resolution = 0, p_bottom = 0, p_top = 0;
+  int discretisation = 0;
   std::vector<std::string> temperature;
   std::vector<std::vector<std::string>> chemistry;
   std::vector<std::string> opac_symbol, opac_folder;
@@ -92,6 +93,7 @@ CaseText readCase(const std::string& folder) {
     else if (key == "fit_mean_molecular_weight") c.fit_mmw = rest[0] == "1";
     else if (key == "fit_scale_height") c.fit_scale_height = rest[0] == "1";
     else if (key == "spectral_resolution") c.resolution = std::stod(rest[0]);
+    else if (key == "spectral_discretisation") c.discretisation = std::stoi(rest[0]);
     else if (key == "nb_levels") c.nb_levels = std::stoul(rest[0]);
     else if (key == "p_bottom") c.p_bottom = std::stod(rest[0]);
     else if (key == "p_top") c.p_top = std::stod(rest[0]);
@@ -125,7 +127,7 @@ int main(int argc, char** argv) {
   config.retrieval_folder_path = folder;
   config.wavenumber_file_path = folder + "wavenumber_full.dat";
   config.cross_section_file_path = folder + "opacity/";
-  config.spectral_disecretisation = 0;
+  config.spectral_disecretisation = cs.discretisation;
   config.spectral_resolution = cs.resolution;
   config.use_error_inflation = cs.error_inflation;
 

@@ -8,7 +8,9 @@ inputs that matter (parameter vectors, injected data, a digest of the table
 files) and the reference's outputs: sampled grid, band-edge and convolution
 indices, atmosphere, spectra, bins and log-likelihoods.
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [case ...]
+
+(all cases but the wide-band ones by default)
 """
 
 import hashlib
@@ -58,7 +60,8 @@ def main():
     import json
     tmp = tempfile.mkdtemp()
     try:
-        for name in cases.CASES:
+        names = sys.argv[1:] or [n for n in cases.CASES if not n.endswith("_wide")]
+        for name in names:
             folder = os.path.join(tmp, name)
             work = os.path.join(tmp, name + "_out")
             transmission = name.startswith("transmission")
@@ -102,8 +105,8 @@ def main():
                     if f"obs{i}_{key}" in out:
                         fixture[f"obs{i}_{key}"] = out[f"obs{i}_{key}"]
             path = os.path.join(ROOT, "tests", "golden", f"{name if not name.startswith('emission_') else name}.npz")
-            if name == "tiny":
-                path = os.path.join(ROOT, "tests", "golden", "emission_tiny.npz")
+            if name.startswith("tiny"):
+                path = os.path.join(ROOT, "tests", "golden", "emission_" + name + ".npz")
             np.savez_compressed(path, **fixture)
             print(path, os.path.getsize(path), "bytes")
     finally:

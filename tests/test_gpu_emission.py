@@ -30,7 +30,7 @@ def retrieval(case_factory, name):
     return _RETRIEVALS[name]
 
 
-@pytest.mark.parametrize("name", ["tiny", "emission_c1"])
+@pytest.mark.parametrize("name", ["tiny", "emission_c1", "tiny_const_wavelength", "tiny_const_resolution"])
 def test_grid_matches_oracle(case_factory, name):
     r, o, _, _ = retrieval(case_factory, name)
     nu, lam = r.spectral_grid()
@@ -41,7 +41,8 @@ def test_grid_matches_oracle(case_factory, name):
     assert r.nb_observation_points == o.nb_bins
 
 
-@pytest.mark.parametrize("name", ["tiny", "emission_c1", "emission_wide"])
+@pytest.mark.parametrize("name", ["tiny", "emission_c1", "emission_wide", "tiny_const_wavelength",
+                                  "tiny_const_resolution"])
 def test_emission_parity(case_factory, name):
     r, o, params, _ = retrieval(case_factory, name)
     for k, p in enumerate(params):

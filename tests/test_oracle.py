@@ -16,12 +16,13 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def _golden(name):
-    fname = "emission_tiny.npz" if name == "tiny" else f"{name}.npz"
+    fname = f"emission_{name}.npz" if name.startswith("tiny") else f"{name}.npz"
     return np.load(os.path.join(GOLDEN, fname))
 
 
 ALL_CASES = ["tiny", "emission_c1", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
-             "transmission_fit_mmw", "transmission_fit_scale_height"]
+             "transmission_fit_mmw", "transmission_fit_scale_height", "tiny_const_wavelength",
+             "tiny_const_resolution"]
 
 
 @pytest.mark.parametrize("name", ALL_CASES)
@@ -71,7 +72,8 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
 
 
 @pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
-@pytest.mark.parametrize("name", ["tiny", "eclipse_small", "transmission_plain", "transmission_variable_gravity"])
+@pytest.mark.parametrize("name", ["tiny", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
+                                  "tiny_const_wavelength", "tiny_const_resolution"])
 def test_oracle_intermediates_match_live_reference(case_factory, tmp_path, name):
     folder, params = case_factory(name)
     ref = api.run_reference(folder, str(tmp_path / "ref"))
@@ -246,3 +248,131 @@ def test_constant_extinction_telescopes(tmp_path):
     H = np.sum(2 * ((R + z[:-1]) * (1 - trans[:-1]) + (R + z[1:]) * (1 - trans[1:])) * np.diff(z) * 0.5)
     depth = (np.sqrt(H + R * R)) ** 2 / p[2] ** 2 * 1e6
     np.testing.assert_allclose(r["spectrum"], depth, rtol=1e-6)
+
+
+# ---------------------------------------------------------------- the tau > 10 cut-off
+def test_tangent_sum_stops_after_the_first_sum_above_ten():
+    """transmission_spectrum.cpp:123-137 adds a segment, then stops once the
+    sum exceeds 10.  Segments of exactly 5 put a partial sum at 10.0: the
+    path goes on (10 is not above 10) and stops after the third segment at
+    15; a stop tested before the add, or at >= 10, stops at 10."""
+    tau, last = api.tangent_sum([1.0, 1.0, 1.0, 1.0], [2.5, 2.5, 2.5, 2.5, 2.5])
+    assert (tau, last) == (15.0, 2)
+    tau, last = api.tangent_sum([1.0, 1.0, 1.0], [1.0, 1.0, 1.0, 1.0])
+    assert (tau, last) == (6.0, 2)                     # never above 10: every segment
+    tau, last = api.tangent_sum([4.0, 1.0, 1.0], [0.25, 1.0, 1.0, 1.0])
+    assert (tau, last) == (9.0, 2)
+    tau, last = api.tangent_sum([4.0, 1.0, 1.0], [2.0, 1.0, 1.0, 1.0])
+    assert (tau, last) == (12.0, 0)                    # the first segment alone passes 10
+    tau, last = api.tangent_sum([1.0, 1.0, 1.0, 1.0], [2.5] * 5, frozen=1)
+    assert (tau, last) == (10.0, 1)                    # a frozen stop overrides the rule
+
+
+def test_constant_extinction_cutoff_closed_form(tmp_path):
+    """log10 sigma = a - log10 p on exact float32 nodes and an isothermal
+    atmosphere give the same extinction k = 2 x 1e6 10^a / (k_B T) at every
+    level (two absorbers), so a path's partial sums telescope: after segment
+    q, tau_t = 2 k sqrt((R + z_{q+1})^2 - (R + z_t)^2).  The path stops at
+    the first q where that exceeds 10 and keeps that sum; the transit depth
+    follows from the trapezoid of 2 r (1 - e^-tau) dr.  The case is chosen so
+    that some paths stop early and others run to the top."""
+    a, x = -24.0, 4e-4
+    folder, p = _affine_transmission_case(tmp_path, lambda t, pr: a - np.log10(pr) + 0 * t, mixing=x, tag="cut")
+    o = api.EmissionOracle(folder)
+    r = o.evaluate(p)
+    atm = o.atmosphere()
+    T = atm["temperature"][0]
+    k = 2 * x * 1e6 * 10 ** a / (synth.BOLTZMANN_K * T)
+    kap, _ = o.coefficients()
+    np.testing.assert_allclose(kap, k, rtol=1e-12)
+    z, R = atm["altitude"], p[1]
+    L = z.size
+    tau, stop = np.empty(L - 1), np.empty(L - 1, int)
+    margin = np.inf
+    for t in range(L - 1):
+        partial = 2 * k * np.sqrt((R + z[t + 1:]) ** 2 - (R + z[t]) ** 2)     # after segments t .. L-2
+        above = np.nonzero(partial > 10)[0]
+        q = above[0] if above.size else L - 2 - t
+        tau[t], stop[t] = partial[q], t + q
+        margin = min(margin, np.min(np.abs(partial / 10 - 1)))
+    assert margin > 1e-6                                 # no partial sum within rounding of 10
+    early = stop < L - 2
+    assert 3 <= early.sum() <= L - 4, early.sum()        # both kinds of path
+    for w in (0, o.nb_points // 2, o.nb_points - 1):
+        t_or, s_or = o.paths(w)
+        np.testing.assert_array_equal(s_or, stop)
+        np.testing.assert_allclose(t_or, tau, rtol=1e-9)
+    trans = np.append(np.exp(-tau), 1.0)
+    H = np.sum(((R + z[:-1]) * (1 - trans[:-1]) + (R + z[1:]) * (1 - trans[1:])) * np.diff(z))
+    np.testing.assert_allclose(r["spectrum"], (H + R * R) / p[2] ** 2 * 1e6, rtol=1e-10)
+
+
+# ---------------------------------------------------------------- eclipse and likelihood closed forms
+def test_isothermal_eclipse_at_the_star_temperature(case_factory):
+    """A planet isothermal at the star's temperature emits pi B(T_*) 1e-3, the
+    blackbody star's flux (star_blackbody.cpp:40-54), so every high-resolution
+    point and every bin, whatever its filter, line-spread function or band,
+    is (Rp/Rs)^2 1e6 (secondary_eclipse.cpp:231-251)."""
+    folder, params = case_factory("eclipse_small")
+    o = api.EmissionOracle(folder)
+    p = params[0].copy()
+    nb_mix = 2
+    p[3 + nb_mix] = p[2]                  # T_0 = T_star
+    p[3 + nb_mix + 1:3 + nb_mix + 7] = 1.0
+    r = o.evaluate(p)
+    np.testing.assert_allclose(o.atmosphere()["temperature"], p[2], rtol=1e-14)
+    expected = p[1] ** 2 * 1e6
+    np.testing.assert_allclose(r["spectrum"], expected, rtol=1e-10)
+    for b in r["bins"]:
+        np.testing.assert_allclose(b, expected, rtol=1e-10)
+    # and half the star's radius ratio: a quarter of the depth
+    q = p.copy()
+    q[1] *= 0.5
+    np.testing.assert_allclose(o.evaluate(q)["bins"][0], expected / 4, rtol=1e-10)
+
+
+def test_gaussian_log_likelihood_with_error_inflation(case_factory, tmp_path):
+    """With error inflation the last parameter e adds 10^e to every variance
+    (retrieval.cpp:330-358): log L = sum_j w_j (-ln(2 pi s_j^2) / 2 -
+    (d_j - m_j)^2 / (2 s_j^2)), s_j^2 = sigma_j^2 + 10^e."""
+    import shutil
+    src, params = case_factory("tiny")
+    folder = str(tmp_path / "inflated") + "/"
+    shutil.copytree(src, folder)
+    lines = open(folder + "case.txt").read().splitlines()
+    lines = [l if not l.startswith("error_inflation") else "error_inflation 1" for l in lines]
+    open(folder + "case.txt", "w").write("\n".join(lines) + "\n")
+    plain = api.EmissionOracle(src)
+    o = api.EmissionOracle(folder)
+    assert o.nb_parameters == plain.nb_parameters + 1
+    for e in (-30.0, -20.0, -19.0):
+        p = np.append(params[1], e)
+        r = o.evaluate(p)
+        ll = 0.0
+        for i in range(o.nb_observations):
+            ob = o.observation(i)
+            m = r["bins"][i]
+            s2 = ob["error"] ** 2 + 10.0 ** e
+            ll += np.sum(ob["weight"] * (-0.5 * np.log(2 * np.pi * s2) - 0.5 * (ob["data"] - m) ** 2 / s2))
+        assert abs(ll - r["loglike"]) < 1e-9 * abs(ll), (e, ll, r["loglike"])
+    # a negligible inflation leaves the plain likelihood
+    assert abs(o.evaluate(np.append(params[1], -60.0))["loglike"] - plain.evaluate(params[1])["loglike"]) < 1e-9
+
+
+def test_grid_discretisations_differ(case_factory):
+    """The three samplings of the same observations (spectral_grid.cpp:158-236):
+    constant wavenumber steps of 1.5 cm^-1, constant wavelength steps of 2 nm,
+    constant resolving power 1500; each keeps the band edges."""
+    grids = {}
+    for name in ("tiny", "tiny_const_wavelength", "tiny_const_resolution"):
+        folder, _ = case_factory(name)
+        grids[name] = api.EmissionOracle(folder).grid()[0]
+    dnu = np.diff(grids["tiny"])
+    dlam = -np.diff(1e4 / grids["tiny_const_wavelength"])
+    res = (1e4 / grids["tiny_const_resolution"])[:-1] / -np.diff(1e4 / grids["tiny_const_resolution"])
+    # each step takes the last full-grid point within the resolution (the
+    # forced band edges add a few shorter steps)
+    assert np.median(dnu) == pytest.approx(1.5, rel=0.02)
+    assert np.percentile(dlam, 95) <= 2e-3 and np.median(dlam) > 1e-3
+    assert np.percentile(res, 5) >= 1500 * (1 - 1e-9) and np.median(res) < 2 * 1500
+    assert len({g.size for g in grids.values()}) == 3
