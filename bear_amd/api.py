@@ -107,6 +107,18 @@ def lib():
         L.bear_retrieval_batch_stats.argtypes = [vp, C.POINTER(BatchStats)]
         L.bear_retrieval_gradient_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp]
         L.bear_retrieval_set_adjoint_memory.argtypes = [vp, sz]
+        cpp = C.POINTER(C.c_char_p)
+        L.bear_spectrum_model_create.argtypes = [C.POINTER(_GlobalConfig), C.c_char_p, C.c_double, C.c_double, sz,
+                                                 C.c_char_p, _dp, _dp, sz, C.POINTER(vp)]
+        L.bear_spectrum_model_destroy.argtypes = [vp]
+        L.bear_spectrum_model_nb_spectral_points.restype = sz
+        L.bear_spectrum_model_nb_spectral_points.argtypes = [vp]
+        L.bear_spectrum_model_spectral_grid.argtypes = [vp, _dp, _dp]
+        L.bear_emission_calc_spectrum.argtypes = [vp, C.c_double, C.c_double, C.c_double, _dp, _dp, sz, cpp, sz, _dp,
+                                                  _dp, C.c_int, _dp]
+        L.bear_transmission_calc_spectrum.argtypes = [vp, C.c_double, C.c_double, C.c_double, _dp, _dp, sz, cpp, sz,
+                                                      _dp, _dp, C.c_int, _dp]
+        L.bear_occultation_calc_spectrum.argtypes = [vp, C.c_double, C.c_double, _dp, _dp, sz, cpp, sz, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -351,6 +363,136 @@ class Retrieval:
                                                          C.c_void_p(int(spectrum_ptr)) if spectrum_ptr else None,
                                                          ptrs, C.byref(neg)))
         return bool(neg.value)
+
+
+# ---------------------------------------------------------------- direct spectra
+class SpectralGrid:
+    """pyBeAR's SpectralGrid(config, wavelength_min, wavelength_max): the
+    wavenumber list of the configuration sampled between two wavelengths
+    (micron) for the direct models below (spectral_grid.cpp:52-72).  The
+    sampled lists are those of the model built on it (wavenumber_list,
+    wavelength_list)."""
+
+    def __init__(self, config: GlobalConfig, wavelength_min: float, wavelength_max: float):
+        self.config = config
+        self.wavelength_min = float(wavelength_min)
+        self.wavelength_max = float(wavelength_max)
+
+
+class _DirectModel:
+    """A forward model whose calcSpectrum takes the atmosphere's structure
+    from the caller (bear_spectrum_model in include/bear_hip.h)."""
+
+    def __init__(self, kind, spectral_grid, nb_grid_points, opacity_species_symbol, opacity_species_folder,
+                 stellar=None):
+        L = lib()
+        if len(opacity_species_symbol) != len(opacity_species_folder):
+            raise BearError(-1, "one opacity folder per opacity species")
+        gc = spectral_grid.config._c()
+        opac = "\n".join(f"{s} {f}" for s, f in zip(opacity_species_symbol, opacity_species_folder))
+        sw = sf = None
+        if stellar is not None:
+            sw = np.ascontiguousarray(stellar[0], dtype=np.float64)
+            sf = np.ascontiguousarray(stellar[1], dtype=np.float64)
+        h = C.c_void_p()
+        _check(L.bear_spectrum_model_create(C.byref(gc), _b(kind), spectral_grid.wavelength_min,
+                                            spectral_grid.wavelength_max, int(nb_grid_points), _b(opac), _ptr(sw),
+                                            _ptr(sf), 0 if sw is None else sw.size, C.byref(h)))
+        self._L, self._h = L, h
+        self.nb_grid_points = int(nb_grid_points)
+        self.nb_spectral_points = L.bear_spectrum_model_nb_spectral_points(h)
+        nu, lam = np.empty(self.nb_spectral_points), np.empty(self.nb_spectral_points)
+        _check(L.bear_spectrum_model_spectral_grid(h, _ptr(nu), _ptr(lam)))
+        self.wavenumber_list, self.wavelength_list = nu, lam
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bear_spectrum_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _structure(self, pressure, temperature, species_symbol, mixing_ratios, cloud_optical_depth):
+        p = np.ascontiguousarray(pressure, dtype=np.float64)
+        t = np.ascontiguousarray(temperature, dtype=np.float64)
+        x = np.ascontiguousarray(mixing_ratios, dtype=np.float64).reshape(p.size, len(species_symbol))
+        names = (C.c_char_p * max(len(species_symbol), 1))(*[_b(s) for s in species_symbol])
+        cloud = None
+        if cloud_optical_depth is not None and len(cloud_optical_depth) > 0:
+            cloud = np.ascontiguousarray(cloud_optical_depth, dtype=np.float64)
+            if cloud.size != (p.size - 1) * self.nb_spectral_points:
+                raise BearError(-1, "cloud optical depths: one per layer and spectral point")
+        keep = (p, t, x, names, cloud)
+        return keep, np.empty(self.nb_spectral_points)
+
+
+class EmissionModel(_DirectModel):
+    """pyBeAR EmissionModel(config, spectral_grid, nb_grid_points,
+    radiative_transfer_desc, radiative_transfer_param, opacity_species_symbol,
+    opacity_species_folder) (emission.h:134-166)."""
+
+    def __init__(self, config: GlobalConfig, spectral_grid: SpectralGrid, nb_grid_points, radiative_transfer_desc="scm",
+                 radiative_transfer_param=(), opacity_species_symbol=(), opacity_species_folder=()):
+        if radiative_transfer_desc != "scm":
+            raise BearError(-1, f"radiative transfer {radiative_transfer_desc} is not available (only scm)")
+        spectral_grid.config = config
+        super().__init__("emission", spectral_grid, nb_grid_points, opacity_species_symbol, opacity_species_folder)
+
+    def calcSpectrum(self, surface_gravity, radius, distance, pressure, temperature, species_symbol, mixing_ratios,
+                     cloud_optical_depth=(), use_variable_gravity=False):
+        """The flux in W m^-2 cm times (radius / distance)^2 per wavenumber
+        (emission.cpp:268-340); mixing_ratios [levels][species]."""
+        (p, t, x, names, cloud), out = self._structure(pressure, temperature, species_symbol, mixing_ratios,
+                                                       cloud_optical_depth)
+        _check(self._L.bear_emission_calc_spectrum(self._h, float(surface_gravity), float(radius), float(distance),
+                                                   _ptr(p), _ptr(t), p.size, names, len(species_symbol), _ptr(x),
+                                                   _ptr(cloud), int(use_variable_gravity), _ptr(out)))
+        return out
+
+
+class TransmissionModel(_DirectModel):
+    """pyBeAR TransmissionModel(config, spectral_grid, nb_grid_points,
+    opacity_species_symbol, opacity_species_folder) (transmission.h:136-190)."""
+
+    def __init__(self, config: GlobalConfig, spectral_grid: SpectralGrid, nb_grid_points, opacity_species_symbol=(),
+                 opacity_species_folder=()):
+        spectral_grid.config = config
+        super().__init__("transmission", spectral_grid, nb_grid_points, opacity_species_symbol, opacity_species_folder)
+
+    def calcSpectrum(self, surface_gravity, planet_radius, radius_ratio, pressure, temperature, species_symbol,
+                     mixing_ratios, cloud_optical_depth=(), use_variable_gravity=False):
+        """The transit depth in ppm per wavenumber (transmission.cpp:371-465)."""
+        (p, t, x, names, cloud), out = self._structure(pressure, temperature, species_symbol, mixing_ratios,
+                                                       cloud_optical_depth)
+        _check(self._L.bear_transmission_calc_spectrum(self._h, float(surface_gravity), float(planet_radius),
+                                                       float(radius_ratio), _ptr(p), _ptr(t), p.size, names,
+                                                       len(species_symbol), _ptr(x), _ptr(cloud),
+                                                       int(use_variable_gravity), _ptr(out)))
+        return out
+
+
+class OccultationModel(_DirectModel):
+    """pyBeAR OccultationModel(config, spectral_grid, nb_grid_points,
+    stellar_spectrum_wavelengths, stellar_spectrum_flux, opacity_species_symbol,
+    opacity_species_folder) (secondary_eclipse.cpp:79-114): the star's flux in
+    W m^-2 micron^-1 at the given wavelengths (micron)."""
+
+    def __init__(self, config: GlobalConfig, spectral_grid: SpectralGrid, nb_grid_points, stellar_spectrum_wavelengths,
+                 stellar_spectrum_flux, opacity_species_symbol=(), opacity_species_folder=()):
+        spectral_grid.config = config
+        super().__init__("secondary_eclipse", spectral_grid, nb_grid_points, opacity_species_symbol,
+                         opacity_species_folder, stellar=(stellar_spectrum_wavelengths, stellar_spectrum_flux))
+
+    def calcSpectrum(self, surface_gravity, radius_ratio, pressure, temperature, species_symbol, mixing_ratios,
+                     cloud_optical_depth=()):
+        """The eclipse depth in ppm per wavenumber (secondary_eclipse.cpp:395-482)."""
+        (p, t, x, names, cloud), out = self._structure(pressure, temperature, species_symbol, mixing_ratios,
+                                                       cloud_optical_depth)
+        _check(self._L.bear_occultation_calc_spectrum(self._h, float(surface_gravity), float(radius_ratio), _ptr(p),
+                                                      _ptr(t), p.size, names, len(species_symbol), _ptr(x),
+                                                      _ptr(cloud), _ptr(out)))
+        return out
 
 
 def version():

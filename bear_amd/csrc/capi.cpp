@@ -12,6 +12,10 @@ struct bear_retrieval {
   std::unique_ptr<bear_amd::Retrieval> impl;
 };
 
+struct bear_spectrum_model {
+  std::unique_ptr<bear_amd::DirectModel> impl;
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -37,6 +41,52 @@ int guarded(F&& f) {
 std::string str(const char* s) { return s ? std::string(s) : std::string(); }
 
 std::vector<double> vec(const double* p, size_t n) { return p ? std::vector<double>(p, p + n) : std::vector<double>(); }
+
+bear_amd::GlobalSettings settings(const bear_global_config* config) {
+  bear_amd::GlobalSettings g;
+  g.device = config->device;
+  g.retrieval_folder = str(config->retrieval_folder_path);
+  g.cross_section_path = str(config->cross_section_file_path);
+  g.wavenumber_file = str(config->wavenumber_file_path);
+  g.discretisation = config->spectral_discretisation;
+  g.resolution = config->spectral_resolution;
+  g.error_inflation = config->use_error_inflation != 0;
+  return g;
+}
+
+std::vector<std::pair<std::string, std::string>> opacityEntries(const char* text) {
+  std::vector<std::pair<std::string, std::string>> out;
+  std::istringstream op(str(text));
+  std::string line;
+  while (std::getline(op, line)) {
+    std::istringstream in(line);
+    std::string sym, folder;
+    if (!(in >> sym)) continue;
+    if (!(in >> folder)) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "opacity species " + sym + " without a folder");
+    out.emplace_back(sym, folder);
+  }
+  return out;
+}
+
+// the arguments shared by the three calcSpectrum entries, checked; the
+// species ids of FixedChemistry
+std::vector<int> structureSpecies(const bear_spectrum_model* m, const double* pressure, const double* temperature,
+                                  size_t nb_levels, const char* const* species, size_t nb_species,
+                                  const double* mixing_ratios, const double* cloud, double* spectrum) {
+  if (!m || !pressure || !temperature || !spectrum || (nb_species && (!species || !mixing_ratios)))
+    bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+  if (nb_levels != m->impl->nbLevels())
+    bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "expected " + std::to_string(m->impl->nbLevels()) + " levels, got " +
+                                                  std::to_string(nb_levels));
+  if (cloud) {                                     // emission.cpp:344-360: an all-zero cloud is no cloud
+    const size_t n = (nb_levels - 1) * m->impl->grid().size();
+    for (size_t i = 0; i < n; ++i)
+      if (cloud[i] > 0) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "cloud optical depths are not available");
+  }
+  std::vector<int> ids;
+  for (size_t j = 0; j < nb_species; ++j) ids.push_back(bear_amd::speciesIdOrFastChem(str(species[j])));
+  return ids;
+}
 
 }  // namespace
 
@@ -69,14 +119,7 @@ int bear_retrieval_create(const bear_global_config* config, const char* forward_
     if (!config || !model_config || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
     *out = nullptr;
     if (!config->use_gpu) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "this engine runs on the GPU only (use_gpu = 1)");
-    bear_amd::GlobalSettings g;
-    g.device = config->device;
-    g.retrieval_folder = str(config->retrieval_folder_path);
-    g.cross_section_path = str(config->cross_section_file_path);
-    g.wavenumber_file = str(config->wavenumber_file_path);
-    g.discretisation = config->spectral_discretisation;
-    g.resolution = config->spectral_resolution;
-    g.error_inflation = config->use_error_inflation != 0;
+    const bear_amd::GlobalSettings g = settings(config);
 
     bear_amd::ModelSettings m;
     m.nb_levels = model_config->nb_grid_points;
@@ -89,15 +132,7 @@ int bear_retrieval_create(const bear_global_config* config, const char* forward_
     m.fit_scale_height = model_config->fit_scale_height != 0;
     m.use_variable_gravity = model_config->use_variable_gravity != 0;
     if (model_config->stellar_spectrum_model) m.stellar_model = model_config->stellar_spectrum_model;
-    std::istringstream op(str(model_config->opacity_species));
-    std::string line;
-    while (std::getline(op, line)) {
-      std::istringstream in(line);
-      std::string sym, folder;
-      if (!(in >> sym)) continue;
-      if (!(in >> folder)) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "opacity species " + sym + " without a folder");
-      m.opacity.emplace_back(sym, folder);
-    }
+    m.opacity = opacityEntries(model_config->opacity_species);
 
     std::vector<bear_amd::ObservationInputView> inputs;
     for (size_t i = 0; i < nb_observations; ++i) {
@@ -248,6 +283,84 @@ int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out) {
     out->gpu_ms_total = e.batchMilliseconds();
     out->adjoint_chunk = e.adjointChunk();
     out->point_group = lc.group;
+  });
+}
+
+int bear_spectrum_model_create(const bear_global_config* config, const char* forward_model_type,
+                               double wavelength_min, double wavelength_max, size_t nb_grid_points,
+                               const char* opacity_species, const double* stellar_wavelengths,
+                               const double* stellar_flux, size_t nb_stellar_points, bear_spectrum_model** out) {
+  return guarded([&] {
+    if (!config || !out) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    if (!config->use_gpu) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "this engine runs on the GPU only (use_gpu = 1)");
+    if (nb_grid_points < 2) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "the atmosphere needs at least two levels");
+    const std::string type = str(forward_model_type);
+    if ((type == "secondary_eclipse" || type == "se") && (!stellar_wavelengths || !stellar_flux))
+      bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "the secondary-eclipse model needs a stellar spectrum");
+    auto* m = new bear_spectrum_model;
+    try {
+      m->impl.reset(new bear_amd::DirectModel(settings(config), type, wavelength_min, wavelength_max, nb_grid_points,
+                                              opacityEntries(opacity_species), vec(stellar_wavelengths, nb_stellar_points),
+                                              vec(stellar_flux, nb_stellar_points)));
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+void bear_spectrum_model_destroy(bear_spectrum_model* m) { delete m; }
+
+size_t bear_spectrum_model_nb_spectral_points(const bear_spectrum_model* m) { return m ? m->impl->grid().size() : 0; }
+
+int bear_spectrum_model_spectral_grid(const bear_spectrum_model* m, double* wavenumbers, double* wavelengths) {
+  return guarded([&] {
+    if (!m) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null model");
+    const auto& g = m->impl->grid();
+    if (wavenumbers) std::memcpy(wavenumbers, g.wavenumber.data(), g.size() * sizeof(double));
+    if (wavelengths) std::memcpy(wavelengths, g.wavelength.data(), g.size() * sizeof(double));
+  });
+}
+
+int bear_emission_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double radius, double distance,
+                                const double* pressure, const double* temperature, size_t nb_levels,
+                                const char* const* species, size_t nb_species, const double* mixing_ratios,
+                                const double* cloud_optical_depth, int use_variable_gravity, double* spectrum) {
+  return guarded([&] {
+    const std::vector<int> ids = structureSpecies(m, pressure, temperature, nb_levels, species, nb_species,
+                                                  mixing_ratios, cloud_optical_depth, spectrum);
+    if (!m->impl->emission()) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "not an emission model");
+    m->impl->emission()->calcSpectrum(surface_gravity, radius, distance, pressure, temperature, ids, mixing_ratios,
+                                      use_variable_gravity != 0, spectrum);
+  });
+}
+
+int bear_transmission_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double planet_radius,
+                                    double radius_ratio, const double* pressure, const double* temperature,
+                                    size_t nb_levels, const char* const* species, size_t nb_species,
+                                    const double* mixing_ratios, const double* cloud_optical_depth,
+                                    int use_variable_gravity, double* spectrum) {
+  return guarded([&] {
+    const std::vector<int> ids = structureSpecies(m, pressure, temperature, nb_levels, species, nb_species,
+                                                  mixing_ratios, cloud_optical_depth, spectrum);
+    if (!m->impl->transmission()) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "not a transmission model");
+    m->impl->transmission()->calcSpectrum(surface_gravity, planet_radius, radius_ratio, pressure, temperature, ids,
+                                          mixing_ratios, use_variable_gravity != 0, spectrum);
+  });
+}
+
+int bear_occultation_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double radius_ratio,
+                                   const double* pressure, const double* temperature, size_t nb_levels,
+                                   const char* const* species, size_t nb_species, const double* mixing_ratios,
+                                   const double* cloud_optical_depth, double* spectrum) {
+  return guarded([&] {
+    const std::vector<int> ids = structureSpecies(m, pressure, temperature, nb_levels, species, nb_species,
+                                                  mixing_ratios, cloud_optical_depth, spectrum);
+    if (!m->impl->occultation()) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "not a secondary-eclipse model");
+    m->impl->occultation()->calcSpectrum(surface_gravity, radius_ratio, pressure, temperature, ids, mixing_ratios,
+                                         spectrum);
   });
 }
 

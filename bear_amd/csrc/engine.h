@@ -166,7 +166,9 @@ class DeviceEngine {
   DeviceEngine& operator=(const DeviceEngine&) = delete;
 
   void uploadGrid(const std::vector<double>& wavenumber, const std::vector<double>& wavelength);
-  void uploadTables(const TableStore& store);
+  // device tables [rows][stride] FP64; rows [first, first + count) from host [count][N]
+  void allocateTables(size_t nb_rows);
+  void uploadTableRows(size_t first, size_t count, const double* rows);
   void uploadRayleigh(const std::vector<double>& cross_section);   // [N] cm^2, empty: no scatterer
   void uploadObservations(const std::vector<ObservationSetup>& observations);
   void configure(size_t nb_levels, size_t nb_species, bool transmission);
@@ -193,6 +195,11 @@ class DeviceEngine {
                        double* const* bins_dev);
   void evaluateTransmission(double bottom_radius, double star_radius, double error_inflation, double* spectrum_dev,
                             double* const* bins_dev);
+  // eclipse depth against the tabulated stellar flux of uploadStellarSpectrum
+  // (no bins): spectrum_dev = F / F_star (Rp/Rs)^2 1e6
+  void evaluateEclipseTabulatedStar(double radius_ratio, double* spectrum_dev);
+  void uploadStellarSpectrum(const std::vector<double>& flux);   // [N], W m^-2 cm
+  void copyFlux(double* host);                   // the last evaluation's flux [N] (emission, eclipse)
   double logLikelihood();                        // waits for the evaluation
   void copyBins(std::vector<double>& out);       // internal (descending) order, all observations
   void copySpectrum(double* host, const double* spectrum_dev);
@@ -260,6 +267,15 @@ class DeviceEngine {
   int device() const { return device_; }
 
  private:
+  // per-evaluation device buffers of the single-model path
+  struct EvalBuffers {
+    unsigned char* staging;
+    double* flux;
+    double* chunk_sums;
+    double* bins;
+    double* ll;
+  };
+  EvalBuffers buffers() const;
   void upload();                                 // staging -> device, on the stream
   void beginTimed();
   void endTimed();
@@ -277,6 +293,7 @@ class DeviceEngine {
   double* rayleigh_ = nullptr;
   double* flux_ = nullptr;
   double* spectrum_ = nullptr;
+  double* star_ = nullptr;                       // tabulated stellar flux [N], or none
   double* weights_ = nullptr;
   BinDesc* bins_desc_ = nullptr;
   double* bins_ = nullptr;

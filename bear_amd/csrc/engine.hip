@@ -48,10 +48,13 @@ size_t transmissionLds(size_t nb_levels, size_t tile) {
 // high-resolution output is F per micron (radiative_transfer.h:58-60) for
 // emission (mode 0) or the eclipse depth F / F_star (Rp/Rs)^2 1e6
 // (secondary_eclipse.cpp:249-251, mode 1).
+// Mode 2: the eclipse depth against a tabulated stellar flux on the grid,
+// F / F_star[i] (Rp/Rs)^2 1e6 (secondary_eclipse.cpp:462-464).
 struct SpectrumOut {
   int mode;
   double star_inv_temperature;
   double depth_scale;
+  const double* star;
 };
 
 __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2, double scaling,
@@ -62,6 +65,8 @@ __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2
   if (spectrum) {
     if (out.mode == 1) {
       spectrum[i] = f * recip(starFlux(nu, out.star_inv_temperature)) * out.depth_scale;
+    } else if (out.mode == 2) {
+      spectrum[i] = f / out.star[i] * out.depth_scale;
     } else {
       const double lam = wavelength[i];
       spectrum[i] = f / lam / lam * 10000.0;
@@ -273,6 +278,10 @@ DeviceEngine::DeviceEngine(int device) : device_(device) {
   lds_limit_ = prop.sharedMemPerBlock;
 }
 
+DeviceEngine::EvalBuffers DeviceEngine::buffers() const {
+  return EvalBuffers{staging_dev_, flux_, chunk_sums_, bins_, ll_};
+}
+
 DeviceEngine::~DeviceEngine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
@@ -280,7 +289,8 @@ DeviceEngine::~DeviceEngine() {
   releaseAdjoint();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
                   (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_, (void*)chunk_sums_, (void*)ll_,
-                  (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_})
+                  (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,
+                  (void*)star_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
   if (event_start_) (void)hipEventDestroy(static_cast<hipEvent_t>(event_start_));
@@ -365,20 +375,23 @@ void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<d
   check(hipStreamSynchronize(s), "copy grid");
 }
 
-void DeviceEngine::uploadTables(const TableStore& store) {
+void DeviceEngine::allocateTables(size_t nb_rows) {
   check(hipSetDevice(device_), "hipSetDevice");
-  if (store.nb_rows > 0 && store.nb_points != n_points_) raise(kInvalidInput, "table rows do not match the grid");
-  if (size_t(store.nb_rows) * (stride_ / 64) > size_t(UINT32_MAX))   // stencils hold 32-bit block offsets
+  if (nb_rows * (stride_ / 64) > size_t(UINT32_MAX))   // stencils hold 32-bit block offsets
     raise(kInvalidInput, "tables too large: rows x wavenumbers exceeds 2^38 values");
   deviceFree(tables_);
-  tables_ = deviceAlloc<double>(size_t(store.nb_rows) * stride_, "cross-section tables");
-  n_rows_ = store.nb_rows;
+  tables_ = nullptr;
+  tables_ = deviceAlloc<double>(nb_rows * stride_, "cross-section tables");
+  n_rows_ = nb_rows;
+}
+
+void DeviceEngine::uploadTableRows(size_t first, size_t count, const double* rows) {
+  if (first + count > n_rows_) raise(kInternal, "table rows beyond the allocated store");
+  if (count == 0) return;
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  if (store.nb_rows > 0)
-    check(hipMemcpy2DAsync(tables_, stride_ * sizeof(double), store.log_cross_sections.data(),
-                           n_points_ * sizeof(double), n_points_ * sizeof(double), store.nb_rows,
-                           hipMemcpyHostToDevice, s),
-          "copy cross-section tables");
+  check(hipMemcpy2DAsync(tables_ + first * stride_, stride_ * sizeof(double), rows, n_points_ * sizeof(double),
+                         n_points_ * sizeof(double), count, hipMemcpyHostToDevice, s),
+        "copy cross-section tables");
   check(hipStreamSynchronize(s), "copy cross-section tables");
 }
 
@@ -437,10 +450,12 @@ void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) 
   bins_ = deviceAlloc<double>(n_bins_, "bins");
   chunk_sums_ = deviceAlloc<double>(n_chunks_ * 2, "bin piece sums");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  check(hipMemcpyAsync(weights_, weights.data(), weights.size() * sizeof(double), hipMemcpyHostToDevice, s),
-        "copy weights");
-  check(hipMemcpyAsync(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, s),
-        "copy bins");
+  if (!weights.empty())
+    check(hipMemcpyAsync(weights_, weights.data(), weights.size() * sizeof(double), hipMemcpyHostToDevice, s),
+          "copy weights");
+  if (!desc.empty())
+    check(hipMemcpyAsync(bins_desc_, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, s),
+          "copy bins");
   if (!chunks.empty())
     check(hipMemcpyAsync(bin_chunks_, chunks.data(), chunks.size() * sizeof(BinChunk), hipMemcpyHostToDevice, s),
           "copy bin pieces");
@@ -516,7 +531,7 @@ void DeviceEngine::configure(size_t nb_levels, size_t nb_species, bool transmiss
 
 void DeviceEngine::upload() {
   check(hipSetDevice(device_), "hipSetDevice");
-  check(hipMemcpyAsync(staging_dev_, staging_host_, staging_bytes_, hipMemcpyHostToDevice,
+  check(hipMemcpyAsync(buffers().staging, staging_host_, staging_bytes_, hipMemcpyHostToDevice,
                        static_cast<hipStream_t>(stream_)),
         "upload inputs");
 }
@@ -524,20 +539,21 @@ void DeviceEngine::upload() {
 void DeviceEngine::launchBins(int mode, double star_temperature, double depth_scale, double inflation,
                               double* const* bins_dev) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const double* shifts = reinterpret_cast<const double*>(staging_dev_ + off_shift_);
+  const EvalBuffers buf = buffers();
+  const double* shifts = reinterpret_cast<const double*>(buf.staging + off_shift_);
   if (n_bins_ > 0) {
-    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_chunks_)), dim3(kBlockBins), 0, s, flux_, wavenumber_, weights_,
-                       bin_chunks_, mode, mode == 1 ? 1.0 / star_temperature : 0.0, chunk_sums_);
+    hipLaunchKernelGGL(binKernel, dim3(unsigned(n_chunks_)), dim3(kBlockBins), 0, s, buf.flux, wavenumber_, weights_,
+                       bin_chunks_, mode, mode == 1 ? 1.0 / star_temperature : 0.0, buf.chunk_sums);
     check(hipGetLastError(), "bin kernel");
-    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, chunk_sums_, bins_desc_, int(n_bins_), mode,
-                       depth_scale, shifts, inflation, bins_, ll_, ll_host_dev_);
+    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, buf.chunk_sums, bins_desc_, int(n_bins_), mode,
+                       depth_scale, shifts, inflation, buf.bins, buf.ll, ll_host_dev_);
     check(hipGetLastError(), "likelihood kernel");
   }
   if (bins_dev)
     for (size_t o = 0; o < n_obs_; ++o) {
       const size_t a = obs_bin_offset_[o], b = obs_bin_offset_[o + 1];
       if (b > a && bins_dev[o])
-        check(hipMemcpyAsync(bins_dev[o], bins_ + a, (b - a) * sizeof(double), hipMemcpyDeviceToDevice, s),
+        check(hipMemcpyAsync(bins_dev[o], buf.bins + a, (b - a) * sizeof(double), hipMemcpyDeviceToDevice, s),
               "copy bins");
     }
 }
@@ -568,13 +584,14 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
 void DeviceEngine::evaluateEmission(double scaling, double inflation, double* spectrum_dev, double* const* bins_dev) {
   if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
   upload();
+  const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
-  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(buf.staging);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
   const int n = int(n_points_);
   beginTimed();
   launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0}, flux_, spectrum_dev);
+                 groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(0, 0.0, 0.0, inflation, bins_dev);
@@ -584,27 +601,65 @@ void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio,
                                    double* spectrum_dev, double* const* bins_dev) {
   if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
   upload();
+  const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
-  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(buf.staging);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
   const int n = int(n_points_);
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
   launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale}, flux_, spectrum_dev);
+                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(1, star_temperature, depth_scale, inflation, bins_dev);
+}
+
+void DeviceEngine::evaluateEclipseTabulatedStar(double radius_ratio, double* spectrum_dev) {
+  if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
+  if (!star_) raise(kInvalidInput, "no stellar spectrum on the grid");
+  upload();
+  const EvalBuffers buf = buffers();
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(buf.staging);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
+  const int n = int(n_points_);
+  const double depth_scale = radius_ratio * radius_ratio * 1e6;
+  beginTimed();
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
+                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{2, 0.0, depth_scale, star_}, buf.flux, spectrum_dev);
+  check(hipGetLastError(), "emission kernel");
+  endTimed();
+}
+
+void DeviceEngine::uploadStellarSpectrum(const std::vector<double>& flux) {
+  check(hipSetDevice(device_), "hipSetDevice");
+  if (flux.size() != n_points_) raise(kInvalidInput, "stellar spectrum does not match the grid");
+  deviceFree(star_);
+  star_ = nullptr;
+  star_ = deviceAlloc<double>(n_points_, "stellar spectrum");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(star_, flux.data(), n_points_ * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy stellar spectrum");
+  check(hipStreamSynchronize(s), "copy stellar spectrum");
+}
+
+void DeviceEngine::copyFlux(double* host) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(host, buffers().flux, n_points_ * sizeof(double), hipMemcpyDeviceToHost, s), "read flux");
+  check(hipStreamSynchronize(s), "evaluation");
+  collectTiming();
 }
 
 void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius, double inflation,
                                         double* spectrum_dev, double* const* bins_dev) {
   if (!transmission_) raise(kInvalidInput, "the engine is not configured for transmission");
   upload();
+  const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const LevelState* lev = reinterpret_cast<const LevelState*>(staging_dev_);
-  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(staging_dev_ + off_species_);
-  const double* path = reinterpret_cast<const double*>(staging_dev_ + off_path_);
+  const LevelState* lev = reinterpret_cast<const LevelState*>(buf.staging);
+  const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
+  const double* path = reinterpret_cast<const double*>(buf.staging + off_path_);
   const int n = int(n_points_), L = int(nb_levels_);
   // tiles of 128 or 64 wavenumbers, the wider one when its LDS (the segment
   // sums and the partials) fits the workgroup
@@ -616,7 +671,7 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
 #define BEAR_TRANSMISSION(NS)                                                                                       \
   case NS:                                                                                                          \
     hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_, stride_,    \
-                       rayleigh_, lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, flux_,       \
+                       rayleigh_, lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, buf.flux,       \
                        spectrum_dev);                                                                               \
     break;
     BEAR_TRANSMISSION(4)

@@ -38,6 +38,7 @@ constexpr double avogadro = 6.022140857e23;
 // Chemical species ids: indices into a level's number densities, in the
 // order of chemistry/chem_species.h:30-90 (with its molecular weights).
 int speciesId(const std::string& symbol);           // throws InvalidInput
+int speciesIdOrFastChem(const std::string& symbol); // also FastChem's symbols (H2O1, C1O2, ...)
 int nbSpeciesIds();
 double molecularWeight(int id);
 namespace sid { constexpr int total = 0, H = 1, He = 2, H2 = 10, Na = 19, K = 20; }
@@ -57,10 +58,22 @@ class SpectralGrid {
 
   void loadFullList(const std::string& path);
   void sample(std::vector<ObservationSetup>& observations);
+  // SpectralGrid(config, wavelength_min, wavelength_max) (spectral_grid.cpp:52-72):
+  // the list sampled between two wavelengths (micron), no observations
+  void sampleRange(double wavelength_min, double wavelength_max);
   size_t size() const { return wavenumber.size(); }
   // nearest element of a monotone list, searching from position start
   static size_t nearest(double x, const std::vector<double>& list, size_t start);
+
+ private:
+  // createHighResGrid* over wavenumber ranges (findBinEdges + the sampling rule)
+  std::vector<char> markRanges(const std::vector<std::array<double, 2>>& ranges) const;
+  void takePoints(const std::vector<char>& keep);
 };
+
+// linear interpolation of (x, y) onto the descending new_x, zero outside x's
+// range (spectral_grid_interpolate.cpp:60-106)
+std::vector<double> interpolateDescending(std::vector<double> x, std::vector<double> y, const std::vector<double>& nx);
 
 // ------------------------------------------------------------------ observations
 enum class BandType { spectroscopy, band_spectroscopy, photometry };
@@ -161,15 +174,26 @@ class OpacitySpecies {
   void bracketTemperature(double t, size_t& lo_i, size_t& hi_i) const;
 };
 
-// All species' sampled rows, log10 cross-sections in FP64, [rows][N].
-struct TableStore {
-  size_t nb_points = 0;
-  std::vector<double> log_cross_sections;
-  uint32_t nb_rows = 0;
+// One species' table before sampling: where its rows come from (row files
+// of a filelist.dat folder, or a registered in-memory table), in listed
+// order, and the first id they take in the device store.
+struct TableSource {
+  std::string symbol, dir;
+  double mass = 0;
+  bool log_storage = false;
+  std::vector<std::string> files;          // per row; empty for registered data
+  const float* registered = nullptr;       // [rows][registered_points]
+  size_t registered_points = 0;
+  uint32_t first_row = 0;
+  size_t nb_rows = 0;
 };
 
-void loadSpeciesTable(OpacitySpecies& s, const std::string& cross_section_path, const SpectralGrid& grid,
-                      TableStore& store);
+// filelist.dat or the registered table of s: fills s.by_temperature with
+// row ids from first_row on (opacity_species.cpp:66-157)
+TableSource scanSpeciesTable(OpacitySpecies& s, const std::string& cross_section_path, uint32_t first_row);
+// the rows sampled at the grid's index list as log10 FP64, [rows][N]
+// (cross_section_file.cpp:33-62, sampled_data.cpp:48-87)
+void sampleSpeciesTable(const TableSource& src, const SpectralGrid& grid, std::vector<double>& out);
 OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& folder, const SpectralGrid& grid);
 
 // ------------------------------------------------------------------ atmosphere
@@ -215,6 +239,15 @@ class Atmosphere {
   // Atmosphere::calcAtmosphereStructure (atmosphere.cpp:63-122); returns "neglect"
   bool compute(double surface_gravity, const double* temperature_param, const double* chemistry_param,
                const AltitudeOptions& options = AltitudeOptions());
+  // Atmosphere::setAtmosphericStructure (atmosphere.cpp:245-283) with
+  // FixedChemistry (fixed_chemistry.cpp:71-89): the caller's pressure (bar)
+  // and temperature per level, n_tot = p 1e6 / (k_B T), n_s = n_tot x_s for
+  // the given species ids (mixing_ratios [levels][species]), the mean
+  // molecular weight over all species, and the hydrostatic altitude (with
+  // gravity falling off as 1/r^2 from bottom_radius when asked).  No
+  // temperature floor, no background gas, no neglect.
+  void setStructure(double surface_gravity, double bottom_radius, bool variable_gravity, const double* pressure,
+                    const double* temperature, const std::vector<int>& species, const double* mixing_ratios);
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
   // Reverse-mode derivative of the last compute(), made with `options`:
   // from dL/dT[l] (direct effects), dL/dn[l][id] and dL/dz[l] (consumed) to
@@ -240,6 +273,8 @@ class Atmosphere {
   std::vector<PiecewisePolynomial::Basis> temperature_basis_;   // per level, for the fixed pressures
   std::vector<char> clamped_;               // levels floored at 50 K by the last compute()
   bool bg_clamped_ = false;
+  // z from the mean molecular weight, temperature and pressure (atmosphere.cpp:288-350)
+  void altitudes(double surface_gravity, const AltitudeOptions& options);
 };
 
 }  // namespace bear_amd

@@ -241,11 +241,17 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
       meanWeight();
     }
   }
+  altitudes(g, opt);
+  return neglect;
+}
+
+void Atmosphere::altitudes(double g, const AltitudeOptions& opt) {
+  const size_t L = nb_levels;
   altitude[0] = 0.0;
   if (opt.scale_height_km > 0) {                   // constant scale height (atmosphere.cpp:338-350)
     const double h = opt.scale_height_km * 100000.0;
     for (size_t l = 1; l < L; ++l) altitude[l] = -h * std::log(pressure[l] / pressure[0]);
-    return neglect;
+    return;
   }
   if (opt.fixed_mean_molecular_weight > 0)          // atmosphere.cpp:168
     std::fill(mean_molecular_weight.begin(), mean_molecular_weight.end(), opt.fixed_mean_molecular_weight);
@@ -263,7 +269,29 @@ bool Atmosphere::compute(double g, const double* tp, const double* cp, const Alt
     altitude[l] = altitude[l - 1] +
                   (1.0 / (rho[l] * gl) + 1.0 / (rho[l - 1] * gl)) * 0.5 * (pressure[l - 1] * 1e6 - pressure[l] * 1e6);
   }
-  return neglect;
+}
+
+void Atmosphere::setStructure(double g, double bottom_radius, bool variable_gravity, const double* p,
+                              const double* t, const std::vector<int>& species, const double* x) {
+  const size_t L = nb_levels, S = species.size();
+  pressure.assign(p, p + L);
+  temperature.assign(t, t + L);
+  std::fill(number_density.begin(), number_density.end(), 0.0);
+  auto n = [&](size_t l, int id) -> double& { return number_density[l * n_ids_ + id]; };
+  for (size_t l = 0; l < L; ++l) {
+    n(l, sid::total) = pressure[l] * 1.e6 / phys::boltzmann_k / temperature[l];
+    for (size_t j = 0; j < S; ++j) n(l, species[j]) = n(l, sid::total) * x[l * S + j];
+  }
+  // chemistry.h:47-58: every species in id order (the absent ones add zeros)
+  for (size_t l = 0; l < L; ++l) {
+    double mu = 0;
+    for (size_t j = 0; j < n_ids_; ++j) mu += n(l, int(j)) / n(l, sid::total) * molecularWeight(int(j));
+    mean_molecular_weight[l] = mu;
+  }
+  AltitudeOptions opt;
+  opt.bottom_radius = bottom_radius;
+  opt.variable_gravity = variable_gravity;
+  altitudes(g, opt);
 }
 
 // Reverse sweep through compute(), last operation first:

@@ -1,6 +1,7 @@
 // Observations of the host side: input, file formats, set-up on the sampled
 // grid, and the folded per-bin operator the device applies.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -210,8 +211,9 @@ ObservationSetup observationFromFile(const std::string& file, const std::string&
 // profile, convolution windows and bandpass on the sampled grid
 // (spectral_band.cpp:127-154, spectral_band_convolve.cpp:44-95,
 //  spectral_grid_interpolate.cpp:123-172, observations_filter_response.cpp:122-160)
-namespace {
 // linear interpolation of (x, y) onto descending new_x; zero outside [x_min, x_max]
+// (the first x at or below each new point, found by bisection on the
+// descending x: the element the reference's linear search finds)
 std::vector<double> interpolateDescending(std::vector<double> x, std::vector<double> y, const std::vector<double>& nx) {
   if (x[0] < x[1]) {
     std::reverse(x.begin(), x.end());
@@ -221,12 +223,11 @@ std::vector<double> interpolateDescending(std::vector<double> x, std::vector<dou
   for (size_t i = 0; i < nx.size(); ++i) {
     if (nx[i] > x.front()) continue;
     if (nx[i] < x.back()) break;
-    const size_t k = size_t(std::find_if(x.begin(), x.end(), [&](double v) { return v <= nx[i]; }) - x.begin());
+    const size_t k = size_t(std::lower_bound(x.begin(), x.end(), nx[i], std::greater<double>()) - x.begin());
     out[i] = x[k] == nx[i] ? y[k] : y[k - 1] + (y[k] - y[k - 1]) * (nx[i] - x[k - 1]) / (x[k] - x[k - 1]);
   }
   return out;
 }
-}  // namespace
 
 void ObservationSetup::setupOnGrid(const SpectralGrid& grid) {
   const size_t N = grid.size();

@@ -80,87 +80,50 @@ OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& 
   return s;
 }
 
-// filelist.dat + float32 row files (opacity_species.cpp:66-157,
-// cross_section_file.cpp:33-62, sampled_data.cpp:48-87): every row is sampled
-// at the grid's index list, scaled by mass/N_A when a mass is given, floored
-// at 1e-200 and stored as log10 in FP64.
-void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const SpectralGrid& grid, TableStore& store) {
-  if (s.rayleigh) return;
-  const std::string dir = path + s.folder;
-  struct Entry { double t, p; std::string file; };
+// filelist.dat (opacity_species.cpp:66-157) or a registered table: the rows'
+// (T, p) and sources; rows ordered by temperature, then pressure
+// (opacity_species.cpp:129-157), take ids first_row, first_row + 1, ... in
+// listed order.
+TableSource scanSpeciesTable(OpacitySpecies& s, const std::string& path, uint32_t first_row) {
+  TableSource src;
+  src.symbol = s.symbol;
+  src.dir = path + s.folder;
+  src.first_row = first_row;
+  struct Entry { double t, p; };
   std::vector<Entry> entries;
-  double mass = 0;
-  bool log_storage = false;
   const RegisteredTable* reg = nullptr;
   {
     std::lock_guard<std::mutex> lock(g_registry_mutex);
     auto it = registry().find(normalise(s.folder));
-    if (it == registry().end()) it = registry().find(normalise(dir));
+    if (it == registry().end()) it = registry().find(normalise(src.dir));
     if (it != registry().end()) reg = &it->second;
   }
   if (reg) {
-    mass = reg->mass;
-    log_storage = reg->log_storage;
-    for (size_t r = 0; r < reg->temperature.size(); ++r) entries.push_back({reg->temperature[r], reg->pressure[r], ""});
+    src.mass = reg->mass;
+    src.log_storage = reg->log_storage;
+    src.registered = reg->rows.data();
+    src.registered_points = reg->nb_wavenumbers;
+    for (size_t r = 0; r < reg->temperature.size(); ++r) entries.push_back({reg->temperature[r], reg->pressure[r]});
   } else {
-    std::ifstream f(dir + "/filelist.dat");
-    if (!f) raise(kFileNotFound, "no cross-section data for " + s.symbol + " (" + dir + "/filelist.dat)");
+    std::ifstream f(src.dir + "/filelist.dat");
+    if (!f) raise(kFileNotFound, "no cross-section data for " + s.symbol + " (" + src.dir + "/filelist.dat)");
     std::string line, storage;
     std::getline(f, line);
     std::istringstream head(line);
-    head >> mass >> storage;
-    log_storage = storage == "log";
+    head >> src.mass >> storage;
+    src.log_storage = storage == "log";
     while (std::getline(f, line)) {
       std::istringstream in(line);
-      Entry e{0, 0, ""};
-      in >> e.p >> e.t >> e.file;
-      if (!e.file.empty()) entries.push_back(e);
+      Entry e{0, 0};
+      std::string file;
+      in >> e.p >> e.t >> file;
+      if (file.empty()) continue;
+      entries.push_back(e);
+      src.files.push_back(file);
     }
   }
   if (entries.empty()) raise(kInvalidInput, "empty cross-section table for " + s.symbol);
-
-  const size_t N = grid.size();
-  const uint32_t first_row = store.nb_rows;
-  store.log_cross_sections.resize((size_t(first_row) + entries.size()) * N);
-  // rows are independent: sample, scale, floor and take log10 in parallel
-  bool missing = false;
-  std::string missing_name;
-  #pragma omp parallel for schedule(dynamic, 1)
-  for (size_t r = 0; r < entries.size(); ++r) {
-    std::vector<float> raw;
-    const float* src;
-    size_t n_full;
-    if (reg) {
-      src = reg->rows.data() + r * reg->nb_wavenumbers;
-      n_full = reg->nb_wavenumbers;
-    } else {
-      std::ifstream bf(dir + "/" + entries[r].file, std::ios::binary | std::ios::ate);
-      if (!bf) {
-        #pragma omp critical
-        { missing = true; missing_name = entries[r].file; }
-        continue;
-      }
-      n_full = size_t(bf.tellg()) / sizeof(float);
-      raw.resize(n_full);
-      bf.seekg(0);
-      bf.read(reinterpret_cast<char*>(raw.data()), std::streamsize(n_full * sizeof(float)));
-      src = raw.data();
-    }
-    double* dst = store.log_cross_sections.data() + (size_t(first_row) + r) * N;
-    const double factor = mass / phys::avogadro;
-    for (size_t i = 0; i < N; ++i) {
-      double v = 0.0;
-      if (grid.index[i] < n_full) {
-        v = src[grid.index[i]];
-        if (log_storage) v = std::pow(10.0, v);
-      }
-      if (mass > 0) v *= factor;
-      if (v < 1e-200) v = 1e-200;
-      dst[i] = std::log10(v);
-    }
-  }
-  if (missing) raise(kFileNotFound, "cross-section file not found: " + dir + "/" + missing_name);
-  // order rows by temperature, then pressure (opacity_species.cpp:129-157)
+  src.nb_rows = entries.size();
   std::map<double, std::vector<TableRow>> by_t;
   for (size_t r = 0; r < entries.size(); ++r)
     by_t[entries[r].t].push_back(TableRow{entries[r].t, entries[r].p, std::log10(entries[r].p), uint32_t(first_row + r)});
@@ -170,8 +133,51 @@ void loadSpeciesTable(OpacitySpecies& s, const std::string& path, const Spectral
     std::sort(rows.begin(), rows.end(), [](const TableRow& a, const TableRow& b) { return a.pressure < b.pressure; });
     s.by_temperature.push_back(rows);
   }
-  store.nb_rows = uint32_t(first_row + entries.size());
-  store.nb_points = N;
+  return src;
+}
+
+// every row sampled at the grid's index list, scaled by mass/N_A when a mass
+// is given, floored at 1e-200 and stored as log10 in FP64; rows in parallel
+void sampleSpeciesTable(const TableSource& src, const SpectralGrid& grid, std::vector<double>& out) {
+  const size_t N = grid.size();
+  out.resize(src.nb_rows * N);
+  bool missing = false;
+  std::string missing_name;
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (size_t r = 0; r < src.nb_rows; ++r) {
+    std::vector<float> raw;
+    const float* data;
+    size_t n_full;
+    if (src.registered) {
+      data = src.registered + r * src.registered_points;
+      n_full = src.registered_points;
+    } else {
+      std::ifstream bf(src.dir + "/" + src.files[r], std::ios::binary | std::ios::ate);
+      if (!bf) {
+        #pragma omp critical
+        { missing = true; missing_name = src.files[r]; }
+        continue;
+      }
+      n_full = size_t(bf.tellg()) / sizeof(float);
+      raw.resize(n_full);
+      bf.seekg(0);
+      bf.read(reinterpret_cast<char*>(raw.data()), std::streamsize(n_full * sizeof(float)));
+      data = raw.data();
+    }
+    double* dst = out.data() + r * N;
+    const double factor = src.mass / phys::avogadro;
+    for (size_t i = 0; i < N; ++i) {
+      double v = 0.0;
+      if (grid.index[i] < n_full) {
+        v = data[grid.index[i]];
+        if (src.log_storage) v = std::pow(10.0, v);
+      }
+      if (src.mass > 0) v *= factor;
+      if (v < 1e-200) v = 1e-200;
+      dst[i] = std::log10(v);
+    }
+  }
+  if (missing) raise(kFileNotFound, "cross-section file not found: " + src.dir + "/" + missing_name);
 }
 
 // OpacitySpecies::findClosestDataPoints (opacity_species.cpp:160-273) and

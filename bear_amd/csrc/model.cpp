@@ -16,9 +16,9 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
     : observations_(observations),
       engine_(engine),
       atmosphere_(m.nb_levels, m.bottom_pressure, m.top_pressure, m.temperature, m.chemistry) {
-  TableStore store;
-  store.nb_points = grid.size();
   std::vector<double> rayleigh;
+  std::vector<TableSource> sources;
+  uint32_t nb_rows = 0;
   for (auto& e : m.opacity) {
     OpacitySpecies s = makeOpacitySpecies(e.first, e.second, grid);
     if (s.rayleigh) {
@@ -31,10 +31,19 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
       rayleigh = s.rayleigh_cross_section;
       continue;
     }
-    loadSpeciesTable(s, global.cross_section_path, grid, store);
+    sources.push_back(scanSpeciesTable(s, global.cross_section_path, nb_rows));
+    nb_rows += uint32_t(sources.back().nb_rows);
     species_.push_back(std::move(s));
   }
-  engine_.uploadTables(store);
+  // one species at a time: host memory holds one species' FP64 rows
+  engine_.allocateTables(nb_rows);
+  std::vector<double> rows;
+  for (const TableSource& src : sources) {
+    sampleSpeciesTable(src, grid, rows);
+    engine_.uploadTableRows(src.first_row, src.nb_rows, rows.data());
+  }
+  rows.clear();
+  rows.shrink_to_fit();
   engine_.uploadRayleigh(rayleigh);
   for (OpacitySpecies& s : species_) s.preparePressures(atmosphere_.pressure);   // fixed pressure grid
   engine_.configure(m.nb_levels, species_.size(), transmission);
@@ -158,6 +167,13 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
   }
 }
 
+void TabulatedModel::stageStructure(double g, double radius, bool variable_gravity, const double* p, const double* t,
+                                    const std::vector<int>& species, const double* x) {
+  atmosphere_.setStructure(g, radius, variable_gravity, p, t, species, x);
+  for (OpacitySpecies& s : species_) s.preparePressures(atmosphere_.pressure);   // this call's pressures
+  stageLevels(atmosphere_, nullptr, engine_.levels(), engine_.levelSpecies(), nullptr, nullptr, engine_.shifts());
+}
+
 bool TabulatedModel::stage(const double* p, BatchModel& scalars) {
   scalars = BatchModel{};
   const bool neglect = setup(p, atmosphere_, scalars);
@@ -235,6 +251,14 @@ bool EmissionModel::calcModelGPU(const double* p, double inflation, double* spec
   const bool neglect = stage(p, scalars);
   engine_.evaluateEmission(scalars.scaling, inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
+}
+
+void EmissionModel::calcSpectrum(double g, double radius, double distance, const double* p, const double* t,
+                                 const std::vector<int>& species, const double* x, bool variable_gravity,
+                                 double* spectrum) {
+  stageStructure(g, radius, variable_gravity, p, t, species, x);
+  engine_.evaluateEmission(radius * radius / distance / distance, 0.0, nullptr, nullptr);   // emission.cpp:335-336
+  engine_.copyFlux(spectrum);
 }
 
 // Chain rule from the device's level gradients to the parameters
@@ -360,6 +384,23 @@ void OccultationModel::generalGradient(const double* p, double, const double* bi
   out[2] = t_bar;
 }
 
+void OccultationModel::setStellarSpectrum(const std::vector<double>& wavelength, const std::vector<double>& flux,
+                                          const SpectralGrid& grid) {
+  if (wavelength.size() < 2 || wavelength.size() != flux.size())
+    raise(kInvalidInput, "the stellar spectrum needs matching wavelengths and fluxes (two at least)");
+  std::vector<double> f = flux;
+  for (size_t i = 0; i < f.size(); ++i) f[i] = f[i] * wavelength[i] * wavelength[i] / 10000.;   // per cm
+  engine_.uploadStellarSpectrum(interpolateDescending(wavelength, f, grid.wavelength));
+}
+
+void OccultationModel::calcSpectrum(double g, double radius_ratio, const double* p, const double* t,
+                                    const std::vector<int>& species, const double* x, double* spectrum) {
+  stageStructure(g, 1.0, false, p, t, species, x);
+  double* dev = engine_.spectrumBuffer();
+  engine_.evaluateEclipseTabulatedStar(radius_ratio, dev);
+  engine_.copySpectrum(spectrum, dev);
+}
+
 // OccultationModel::calcModelGPU (secondary_eclipse.cpp:262-355): planet
 // emission at scale 1, blackbody star, depth = F_p / F_* (Rp/Rs)^2 1e6 per bin
 bool OccultationModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
@@ -479,18 +520,13 @@ void TransmissionModel::gradientBatch(const double* params, size_t batch, size_t
   if (!failure.empty()) raise(failure_code, failure);
 }
 
-// TransmissionModel::calcModelGPU (transmission.cpp:257-321)
-bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
-                                     double* const* spectrum_obs_dev) {
-  BatchModel scalars;
-  const bool neglect = stage(p, scalars);
-  // chord lengths between successive levels for every tangent level
-  // (transmission_spectrum.cpp:117-118, 145-154), transposed: chord[i][t] =
-  // sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2), i >= t
-  // (the entries with i < t and the padding stay zero)
+// chord lengths between successive levels for every tangent level
+// (transmission_spectrum.cpp:117-118, 145-154), transposed: chord[i][t] =
+// sqrt((R+z_{i+1})^2 - (R+z_t)^2) - sqrt((R+z_i)^2 - (R+z_t)^2), i >= t
+// (the entries with i < t and the padding stay zero)
+void TransmissionModel::stageChords(double r0) {
   const size_t L = atmosphere_.nb_levels;
   const size_t tangents = size_t(chordTangents(int(L)));
-  const double r0 = p[1];
   const std::vector<double>& z = atmosphere_.altitude;
   double* chord = engine_.pathLengths();
   for (size_t t = 0; t + 1 < L; ++t) {
@@ -503,8 +539,26 @@ bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* 
       previous = next;
     }
   }
+}
+
+// TransmissionModel::calcModelGPU (transmission.cpp:257-321)
+bool TransmissionModel::calcModelGPU(const double* p, double inflation, double* spectrum_dev,
+                                     double* const* spectrum_obs_dev) {
+  BatchModel scalars;
+  const bool neglect = stage(p, scalars);
+  stageChords(p[1]);
   engine_.evaluateTransmission(p[1], p[2], inflation, spectrum_dev, spectrum_obs_dev);
   return neglect;
+}
+
+void TransmissionModel::calcSpectrum(double g, double planet_radius, double radius_ratio, const double* p,
+                                     const double* t, const std::vector<int>& species, const double* x,
+                                     bool variable_gravity, double* spectrum) {
+  stageStructure(g, planet_radius, variable_gravity, p, t, species, x);
+  stageChords(planet_radius);
+  double* dev = engine_.spectrumBuffer();
+  engine_.evaluateTransmission(planet_radius, planet_radius / radius_ratio, 0.0, dev, nullptr);   // transmission.cpp:391-392
+  engine_.copySpectrum(spectrum, dev);
 }
 
 // ---------------------------------------------------------------- retrieval
@@ -658,6 +712,47 @@ bool Retrieval::computeModel(const double* p, size_t n, bool high_res, double* s
     k += nb;
   }
   return neglect;
+}
+
+}  // namespace bear_amd
+
+namespace bear_amd {
+
+DirectModel::DirectModel(const GlobalSettings& global, const std::string& model_type, double wavelength_min,
+                         double wavelength_max, size_t nb_levels,
+                         const std::vector<std::pair<std::string, std::string>>& opacity,
+                         const std::vector<double>& stellar_wavelength, const std::vector<double>& stellar_flux)
+    : global_(global), type_(model_type), nb_levels_(nb_levels) {
+  const bool emission = model_type == "emission";
+  const bool eclipse = model_type == "secondary_eclipse" || model_type == "se";
+  const bool transmission = model_type == "transmission";
+  if (!emission && !eclipse && !transmission) raise(kInvalidInput, "forward model " + model_type + " is not available");
+  if (global_.discretisation < 0 || global_.discretisation > 2) raise(kInvalidInput, "unknown spectral discretisation");
+  grid_.discretisation = global_.discretisation;
+  grid_.resolution = global_.resolution;
+  grid_.loadFullList(global_.wavenumber_file);
+  grid_.sampleRange(wavelength_min, wavelength_max);
+  ModelSettings m;
+  m.nb_levels = nb_levels;
+  m.bottom_pressure = 10.0;
+  m.top_pressure = 1e-5;
+  m.temperature = "const";
+  m.radiative_transfer = "scm";
+  m.opacity = opacity;
+  engine_.reset(new DeviceEngine(global_.device));
+  engine_->uploadGrid(grid_.wavenumber, grid_.wavelength);
+  engine_->uploadObservations(observations_);
+  if (emission) {
+    emission_ = new EmissionModel(global_, m, grid_, observations_, *engine_);
+    model_.reset(emission_);
+  } else if (transmission) {
+    transmission_ = new TransmissionModel(global_, m, grid_, observations_, *engine_);
+    model_.reset(transmission_);
+  } else {
+    occultation_ = new OccultationModel(global_, m, grid_, observations_, *engine_);
+    model_.reset(occultation_);
+    occultation_->setStellarSpectrum(stellar_wavelength, stellar_flux, grid_);
+  }
 }
 
 }  // namespace bear_amd

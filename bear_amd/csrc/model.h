@@ -71,6 +71,11 @@ class TabulatedModel : public ForwardModel {
   void stageBatch(const double* parameters, size_t batch, size_t stride, const double* error_inflation,
                   char* neglect) override;
   std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const override;
+  // pyBeAR's direct spectra: the atmosphere set from the caller's structure
+  // (Atmosphere::setStructure: pressure, temperature, mixing ratios [levels]
+  // [species] of the given species ids) and staged for one evaluation
+  void stageStructure(double surface_gravity, double bottom_radius, bool variable_gravity, const double* pressure,
+                      const double* temperature, const std::vector<int>& species, const double* mixing_ratios);
 
  protected:
   TabulatedModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
@@ -120,6 +125,11 @@ class EmissionModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
   ModelKind kind() const override { return ModelKind::emission; }
+  // EmissionModel::calcSpectrum (emission.cpp:268-340): the flux [N] in
+  // W m^-2 cm times radius^2 / distance^2
+  void calcSpectrum(double surface_gravity, double radius, double distance, const double* pressure,
+                    const double* temperature, const std::vector<int>& species, const double* mixing_ratios,
+                    bool variable_gravity, double* spectrum);
   void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
                      const std::vector<double>& bin_grad, const std::vector<double>& bin_terms, const char* neglect,
                      double* gradient) override {
@@ -143,6 +153,15 @@ class OccultationModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
   ModelKind kind() const override { return ModelKind::eclipse; }
+  // StarSpectrumFile(wavelengths, flux, grid) (star_file_spectrum.cpp:57-72):
+  // a stellar flux in W m^-2 micron^-1 at the given wavelengths (micron),
+  // converted to W m^-2 cm and interpolated onto the grid
+  void setStellarSpectrum(const std::vector<double>& wavelength, const std::vector<double>& flux,
+                          const SpectralGrid& grid);
+  // OccultationModel::calcSpectrum (secondary_eclipse.cpp:395-482): the
+  // eclipse depth [N] in ppm against that stellar flux
+  void calcSpectrum(double surface_gravity, double radius_ratio, const double* pressure, const double* temperature,
+                    const std::vector<int>& species, const double* mixing_ratios, double* spectrum);
   void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
                      const std::vector<double>& bin_grad, const std::vector<double>& bin_terms, const char* neglect,
                      double* gradient) override {
@@ -166,6 +185,11 @@ class TransmissionModel : public TabulatedModel {
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return false; }
   ModelKind kind() const override { return ModelKind::transmission; }
+  // TransmissionModel::calcSpectrum (transmission.cpp:371-465): the transit
+  // depth [N] in ppm, R_* = planet_radius / radius_ratio
+  void calcSpectrum(double surface_gravity, double planet_radius, double radius_ratio, const double* pressure,
+                    const double* temperature, const std::vector<int>& species, const double* mixing_ratios,
+                    bool variable_gravity, double* spectrum);
   void gradientBatch(const double* parameters, size_t batch, size_t stride, const std::vector<double>& level_grad,
                      const std::vector<double>& bin_grad, const std::vector<double>& chord_grad, const char* neglect,
                      double* gradient) override;
@@ -175,6 +199,8 @@ class TransmissionModel : public TabulatedModel {
 
  private:
   AltitudeOptions altitudeOptions(const double* p) const;
+  // the chord lengths of the staged atmosphere into the engine's staging
+  void stageChords(double bottom_radius);
   bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;
 };
 
@@ -209,6 +235,38 @@ class Retrieval {
   std::vector<ObservationSetup> observations_;
   std::unique_ptr<DeviceEngine> engine_;
   std::unique_ptr<ForwardModel> model_;
+};
+
+// pyBeAR's direct forward models (python/src/bear_python_wrapper.cpp:343-372):
+// a model on a grid sampled between two wavelengths (micron), without
+// observations, whose calcSpectrum takes the atmosphere's structure from
+// the caller.  The pressure boundaries of the set-up are the reference's
+// placeholders (10 and 1e-5 bar, emission.cpp:86-90); every call brings its
+// own pressures.
+class DirectModel {
+ public:
+  DirectModel(const GlobalSettings& global, const std::string& model_type, double wavelength_min,
+              double wavelength_max, size_t nb_levels, const std::vector<std::pair<std::string, std::string>>& opacity,
+              const std::vector<double>& stellar_wavelength, const std::vector<double>& stellar_flux);
+  const SpectralGrid& grid() const { return grid_; }
+  size_t nbLevels() const { return nb_levels_; }
+  const std::string& type() const { return type_; }
+  EmissionModel* emission() const { return emission_; }
+  TransmissionModel* transmission() const { return transmission_; }
+  OccultationModel* occultation() const { return occultation_; }
+  DeviceEngine& engine() const { return *engine_; }
+
+ private:
+  GlobalSettings global_;
+  std::string type_;
+  size_t nb_levels_ = 0;
+  SpectralGrid grid_;
+  std::vector<ObservationSetup> observations_;   // none
+  std::unique_ptr<DeviceEngine> engine_;
+  std::unique_ptr<ForwardModel> model_;
+  EmissionModel* emission_ = nullptr;
+  TransmissionModel* transmission_ = nullptr;
+  OccultationModel* occultation_ = nullptr;
 };
 
 void registerTable(const std::string& folder, double mass, bool log_storage, size_t nb_rows,

@@ -121,7 +121,7 @@ def _line_list(rng, n_lines, nu_lo, nu_hi):
     return centres, strength, lower_energy
 
 
-def _lorentz_rows(nu, centres, weights_per_row, widths, backend="numpy"):
+def _lorentz_rows(nu, centres, weights_per_row, widths):
     """Sum of Lorentzians of one width per row, by FFT on a padded grid.
 
     ``weights_per_row`` is [R, n_lines] (line strengths at each row's
@@ -135,22 +135,47 @@ def _lorentz_rows(nu, centres, weights_per_row, widths, backend="numpy"):
     idx = np.clip(np.rint((centres - nu[0]) / step).astype(np.int64), 0, n - 1)
     freqs = np.fft.rfftfreq(m, d=step)
     out = np.empty((weights_per_row.shape[0], n), dtype=np.float64)
-    if backend == "torch":
-        import torch
-        dev = torch.device("cuda")
-        f_t = torch.as_tensor(freqs, device=dev)
-        for r in range(weights_per_row.shape[0]):
-            # histogram on the CPU (deterministic), transforms on the GPU
-            hist = torch.as_tensor(np.bincount(idx, weights=weights_per_row[r], minlength=m), device=dev)
-            spec = torch.fft.rfft(hist)
-            spec *= torch.exp(-2.0 * math.pi * widths[r] * f_t)
-            out[r] = (torch.fft.irfft(spec, n=m)[:n] / step).cpu().numpy()
-        return out
     for r in range(weights_per_row.shape[0]):
         hist = np.bincount(idx, weights=weights_per_row[r], minlength=m)
         spec = np.fft.rfft(hist)
         spec *= np.exp(-2.0 * math.pi * widths[r] * freqs)
         out[r] = np.fft.irfft(spec, n=m)[:n] / step
+    return out
+
+
+def _line_rows_gpu(nu, centres, weights_per_row, widths, mass, log_storage, batch=32):
+    """The rows of line_table computed entirely on the GPU (torch), in
+    batches of rows: the line histogram, the Lorentzian transform, the
+    1e-30 floor, the mass scaling and log10; returns float32 [R, n].  For
+    the full-size workloads (millions of values per row), where the numpy
+    path takes minutes; the histogram is summed with atomics, so the values
+    are reproducible to the last bit of float64 only, not across backends."""
+    import torch
+    dev = torch.device("cuda")
+    n = nu.size
+    step = (nu[-1] - nu[0]) / (n - 1)
+    m = 1 << int(math.ceil(math.log2(2 * n)))
+    idx = torch.as_tensor(np.clip(np.rint((centres - nu[0]) / step).astype(np.int64), 0, n - 1), device=dev)
+    freqs = torch.as_tensor(np.fft.rfftfreq(m, d=step), device=dev)
+    w = torch.as_tensor(weights_per_row, device=dev)
+    g = torch.as_tensor(widths, device=dev)
+    R = weights_per_row.shape[0]
+    out = np.empty((R, n), dtype=np.float32)
+    for r0 in range(0, R, batch):
+        r1 = min(R, r0 + batch)
+        hist = torch.zeros((r1 - r0, m), dtype=torch.float64, device=dev)
+        hist.index_add_(1, idx, w[r0:r1])
+        spec = torch.fft.rfft(hist, dim=1)
+        del hist
+        spec *= torch.exp(-2.0 * math.pi * g[r0:r1, None] * freqs[None, :])
+        rows = torch.fft.irfft(spec, n=m, dim=1)[:, :n] / step
+        del spec
+        sigma = 1e-30 + torch.clamp(rows, min=0.0)
+        if mass > 0:
+            sigma = sigma / (mass / AVOGADRO)
+        if log_storage:
+            sigma = torch.log10(sigma)
+        out[r0:r1] = sigma.to(torch.float32).cpu().numpy()
     return out
 
 
@@ -165,7 +190,12 @@ def line_table(rng, symbol, folder, nu, temperatures, pressures, n_lines=3000,
     w = s0[None, :] * (1000.0 / t_rows[:, None]) ** 1.5 \
         * np.exp(-c2 * e_low[None, :] * (1.0 / t_rows[:, None] - 1.0 / 1000.0))
     gamma = 0.01 * np.sqrt(t_rows / 1000.0) + 0.07 * p_rows * (296.0 / t_rows) ** 0.7
-    sigma = 1e-30 + np.maximum(_lorentz_rows(nu, centres, w, gamma, backend), 0.0)
+    if backend == "torch":
+        values = _line_rows_gpu(nu, centres, w, gamma, mass, log_storage)
+        return SpeciesTable(symbol, folder, np.asarray(temperatures, float), np.asarray(pressures, float),
+                            values.reshape(temperatures.size, pressures.size, nu.size),
+                            mass=mass, log_storage=log_storage, kind="line")
+    sigma = 1e-30 + np.maximum(_lorentz_rows(nu, centres, w, gamma), 0.0)
     sigma = sigma.reshape(temperatures.size, pressures.size, nu.size)
     if mass > 0:
         sigma = sigma / (mass / AVOGADRO)            # store as cm^2/g
@@ -327,7 +357,8 @@ def emission_c1(seed=11):
 def emission_c2(seed=23, backend="numpy", n_wavenumbers=400_000, nb_bins=300):
     """Plain GPU drop-in emission model: 100 levels, 6 absorbers + H2-H2/H2-He
     CIA + H2 Rayleigh, 400,000 wavenumbers at 0.025 cm^-1, 30 T x 9 p tables,
-    300 photometric bins each with a seeded 50-point bandpass."""
+    300 photometric bins each with a seeded 50-point bandpass; the bins tile
+    the wavenumber list, so all 400,000 wavenumbers are sampled."""
     rng = np.random.default_rng(seed)
     step = 0.025
     nu = 1000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
@@ -338,7 +369,8 @@ def emission_c2(seed=23, backend="numpy", n_wavenumbers=400_000, nb_bins=300):
     species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
                 cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
     lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
-    edges = np.linspace(lam_lo * 1.0005, lam_hi * 0.9995, nb_bins + 1)
+    # the bands tile the whole list: every wavenumber is sampled
+    edges = np.linspace(lam_lo * (1 + 1e-7), lam_hi * (1 - 1e-7), nb_bins + 1)
     scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
     obs = []
     for b in range(nb_bins):
@@ -359,6 +391,25 @@ def transmission_parameters(rng, nb_absorbers, isothermal=True):
     mixing = 10.0 ** rng.uniform(-8.0, -3.0, nb_absorbers)
     t = [rng.uniform(500.0, 2500.0)] if isothermal else list(poly_temperature_params(rng, 7))
     return np.concatenate([[log_g, radius, star], mixing, t])
+
+
+def lsf_edges(lam_min, lam_max, nb, r, low_end=True, high_end=True, margin=1e-6):
+    """Band edges (micron, descending) of nb equal bins over [lo, hi] whose
+    observation range, widened by 5 sigma of a Gaussian line-spread function
+    of FWHM = centre / r at the outermost bins (observations.cpp:236-251),
+    reaches lam_min (1 + margin) and lam_max (1 - margin) at the ends
+    flagged, so that the sampled grid runs to the ends of the wavenumber
+    list; an unflagged end is that bound itself."""
+    a = 5.0 / (r * 2.355)
+    lo, hi = lam_min, lam_max
+    for _ in range(50):            # the outer bins' centres depend on both ends
+        w = (hi - lo) / nb
+        if low_end:
+            lo = (lam_min * (1 + margin) + a * w / 2) / (1 - a)
+        if high_end:
+            hi = (lam_max * (1 - margin) + a * w / 2) / (1 + a)
+    edges = np.linspace(hi, lo, nb + 1)
+    return np.stack([edges[:-1], edges[1:]], axis=1)
 
 
 def _transit_data(lo, hi, nb, r, rng):
@@ -439,7 +490,8 @@ def transmission_c3(seed=53, backend="numpy", n_wavenumbers=300_000, nb_levels=2
     tangent paths), isothermal, 8 absorbers + H2-H2/H2-He CIA + H2 Rayleigh
     on 30 T x 9 p tables, 300,000 wavenumbers at 0.025 cm^-1 from 2000 cm^-1,
     three instruments of uneven size: 20, 147 and 400 bins with Gaussian
-    line-spread functions at R = 70, 130 and 1000."""
+    line-spread functions at R = 70, 130 and 1000, side by side over the
+    whole list (all 300,000 wavenumbers are sampled)."""
     rng = np.random.default_rng(seed)
     step = 0.025
     nu = 2000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
@@ -450,10 +502,15 @@ def transmission_c3(seed=53, backend="numpy", n_wavenumbers=300_000, nb_levels=2
     species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
                 cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
     obs = []
-    for k, (lo, hi, nb, r) in enumerate([(1.10, 1.70, 20, 70.0), (1.75, 3.30, 147, 130.0),
-                                         (3.35, 4.85, 400, 1000.0)]):
-        be, d, e, fw = _transit_data(lo, hi, nb, r, rng)
-        obs.append(ObservationSpec(f"instrument{k}", "band-spectroscopy", bin_edges=be, data=d, error=e, fwhm=fw))
+    lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
+    # three instruments side by side over the whole list (the outer ones'
+    # line-spread margins reach its ends): every wavenumber is sampled
+    for k, (lo, hi, nb, r) in enumerate([(lam_lo, 1.70, 20, 70.0), (1.70, 3.30, 147, 130.0),
+                                         (3.30, lam_hi, 400, 1000.0)]):
+        be = lsf_edges(lo, hi, nb, r, low_end=k == 0, high_end=k == 2)
+        centres = be.mean(axis=1)
+        obs.append(ObservationSpec(f"instrument{k}", "band-spectroscopy", bin_edges=be, data=1.2e4 * np.ones(nb),
+                                   error=50.0 * np.ones(nb), fwhm=centres / r))
     params = transmission_parameters(rng, 8)
     return Case("transmission_c3", nu, step, nb_levels, 100.0, 1e-6, ("const",),
                 [("iso", names), ("bg", ["H2He"])], species, ["H2"], obs, params, model="transmission",
@@ -474,8 +531,9 @@ def eclipse_c5(seed=61, backend="numpy", n_wavenumbers=1_000_000):
     """Reduced-precision batched secondary-eclipse configuration: 100 levels,
     poly 6 1, 12 absorbers + H2-H2/H2-He CIA on 30 T x 12 p tables over
     1,000,000 wavenumbers at 0.025 cm^-1 from 1000 cm^-1, a blackbody star,
-    a 9-bandpass photometric set and a 400-bin band spectrum with a Gaussian
-    line-spread function at R = 400."""
+    a 9-bandpass photometric set tiling the whole list (all 1,000,000
+    wavenumbers are sampled) and a 400-bin band spectrum over 1.2-5 micron
+    with a Gaussian line-spread function at R = 400."""
     rng = np.random.default_rng(seed)
     step = 0.025
     nu = 1000.0 + step * np.arange(n_wavenumbers, dtype=np.float64)
@@ -487,9 +545,10 @@ def eclipse_c5(seed=61, backend="numpy", n_wavenumbers=1_000_000):
                 cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
     lam_lo, lam_hi = 1e4 / nu[-1], 1e4 / nu[0]
     obs = []
-    centres = np.geomspace(lam_lo * 1.3, lam_hi * 0.8, 9)
-    for k, c in enumerate(centres):
-        lo, hi = c * 0.95, c * 1.05
+    # nine bandpasses tiling the list in log wavelength (every wavenumber is sampled)
+    bounds = np.geomspace(lam_lo * (1 + 1e-7), lam_hi * (1 - 1e-7), 10)
+    for k in range(9):
+        lo, hi = bounds[k], bounds[k + 1]
         w, r = bandpass(rng, lo, hi, 50)
         obs.append(ObservationSpec(f"phot{k}", "photometry", bin_edges=np.array([[lo, hi]]),
                                    data=np.array([1500.0]), error=np.array([80.0]),
@@ -553,12 +612,14 @@ def _fmt(x):
     return repr(float(x))
 
 
-def write_case(case: Case, folder: str) -> str:
+def write_case(case: Case, folder: str, tables: bool = True) -> str:
     """Write ``case`` as a retrieval folder in the reference's formats.
 
     Returns the folder path (with a trailing slash).  Besides the reference
     files, ``case.json`` records the configuration a driver needs
-    (forward-model settings, opacity entries, parameter vector)."""
+    (forward-model settings, opacity entries, parameter vector).  With
+    tables=False the opacity folders are left out (the tables are then
+    registered in memory with the engine and the oracle)."""
     folder = os.path.join(folder, "")
     os.makedirs(folder, exist_ok=True)
     opac = os.path.join(folder, "opacity", "")
@@ -566,7 +627,7 @@ def write_case(case: Case, folder: str) -> str:
         f.write(f"{case.wavenumber_full.size}\n")
         f.write("\n".join(_fmt(v) for v in case.wavenumber_full))
         f.write("\n")
-    for s in case.species:
+    for s in (case.species if tables else []):
         d = os.path.join(opac, s.folder)
         os.makedirs(d, exist_ok=True)
         lines = [f"{_fmt(s.mass)} {'log' if s.log_storage else 'lin'}"]

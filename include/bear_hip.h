@@ -251,6 +251,61 @@ typedef struct bear_batch_stats {
 
 int bear_retrieval_batch_stats(const bear_retrieval* r, bear_batch_stats* out);
 
+/* ---------------------------------------------------------------- direct spectra
+ * pyBeAR's direct forward models (python/src/bear_python_wrapper.cpp:343-372):
+ * EmissionModel, TransmissionModel and OccultationModel built on a spectral
+ * grid sampled between two wavelengths (SpectralGrid(config, wavelength_min,
+ * wavelength_max), spectral_grid.cpp:52-72), without observations, whose
+ * calcSpectrum takes the atmosphere's structure from the caller
+ * (Atmosphere::setAtmosphericStructure with FixedChemistry,
+ * atmosphere.cpp:245-283, fixed_chemistry.cpp:33-89): per level the
+ * pressure (bar, bottom first), the temperature and the mixing ratios of the
+ * named species (chemistry symbols or FastChem symbols); n = p 1e6 / (k_B T)
+ * per species, the mean molecular weight over all species, the hydrostatic
+ * altitude.  No background gas is added: CIA partners (H2, He) must be
+ * among the species.
+ * forward_model_type: "emission", "transmission" or "secondary_eclipse";
+ * opacity_species as in bear_model_config; the eclipse takes the star's
+ * flux (W m^-2 micron^-1) at the given wavelengths (micron) (the reference's
+ * StarSpectrumFile, star_file_spectrum.cpp:57-72), the other models NULL. */
+typedef struct bear_spectrum_model bear_spectrum_model;
+
+int bear_spectrum_model_create(const bear_global_config* config, const char* forward_model_type,
+                               double wavelength_min, double wavelength_max, size_t nb_grid_points,
+                               const char* opacity_species, const double* stellar_wavelengths,
+                               const double* stellar_flux, size_t nb_stellar_points, bear_spectrum_model** out);
+void bear_spectrum_model_destroy(bear_spectrum_model* m);
+size_t bear_spectrum_model_nb_spectral_points(const bear_spectrum_model* m);
+int bear_spectrum_model_spectral_grid(const bear_spectrum_model* m, double* wavenumbers, double* wavelengths);
+
+/* EmissionModel::calcSpectrum (emission.cpp:268-340): spectrum [points] is
+ * the flux in W m^-2 cm times radius^2 / distance^2 (same units for both).
+ * surface_gravity in cm s^-2; radius is also the bottom radius of the
+ * variable gravity.  mixing_ratios [nb_levels][nb_species].
+ * cloud_optical_depth [nb_levels - 1][points] may be NULL; a cloud with
+ * non-zero optical depth is refused (the reference's FixedCloudModel is not
+ * part of this engine). */
+int bear_emission_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double radius, double distance,
+                                const double* pressure, const double* temperature, size_t nb_levels,
+                                const char* const* species, size_t nb_species, const double* mixing_ratios,
+                                const double* cloud_optical_depth, int use_variable_gravity, double* spectrum);
+
+/* TransmissionModel::calcSpectrum (transmission.cpp:371-465): spectrum
+ * [points] is the transit depth in ppm; the star's radius is
+ * planet_radius / radius_ratio. */
+int bear_transmission_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double planet_radius,
+                                    double radius_ratio, const double* pressure, const double* temperature,
+                                    size_t nb_levels, const char* const* species, size_t nb_species,
+                                    const double* mixing_ratios, const double* cloud_optical_depth,
+                                    int use_variable_gravity, double* spectrum);
+
+/* OccultationModel::calcSpectrum (secondary_eclipse.cpp:395-482): spectrum
+ * [points] is the eclipse depth in ppm against the model's stellar flux. */
+int bear_occultation_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double radius_ratio,
+                                   const double* pressure, const double* temperature, size_t nb_levels,
+                                   const char* const* species, size_t nb_species, const double* mixing_ratios,
+                                   const double* cloud_optical_depth, double* spectrum);
+
 /* ---------------------------------------------------------------- measurement
  * No counterpart in the reference (it has no timing code).  With profiling
  * on, HIP events on the retrieval's stream time every launch of the fused

@@ -59,6 +59,17 @@ int speciesId(const std::string& s) {
   throw std::runtime_error("unknown species " + s);
 }
 
+// FixedChemistry::findSpecies (fixed_chemistry.cpp:42-63): the symbol or FastChem's symbol
+const char* const kFastChemSymbol[] = {
+  "Total", "H", "He", "C", "O", "Fe", "Fe+", "Ca", "Ti", "Ti+", "H2", "H2O1", "C1O2", "C1O1", "C1H4", "C1H1N1_1",
+  "H3N1", "C2H2", "N2", "Na", "K", "H2S1", "H1-", "O1Ti1", "O1V1", "H1Fe1", "H1S1", "Mg1O1", "Al1O1", "Ca1O1",
+  "Cr1H1", "H1Mg1", "Ca1H1", "H1Ti1", "H1O1", "e-", "V", "V1+", "Mn", "Si", "Cr", "Cr1+", "O1Si1", "O2Si1", "O2S1",
+  "C1S2"};
+int fixedChemistryId(const std::string& s) {
+  for (int i = 0; i < kNbSpecies; ++i) if (s == kSpecies[i].symbol || s == kFastChemSymbol[i]) return i;
+  throw std::runtime_error("Chemical species " + s + " not found in the list of species");
+}
+
 double lerp(double x1, double x2, double y1, double y2, double x) {
   return y1 + (y2 - y1) * (x - x1) / (x2 - x1);
 }
@@ -350,6 +361,9 @@ Observation loadObservation(const std::string& file, const std::string& modifier
   return o;
 }
 
+std::vector<int> markRanges(const Grid& g, const std::vector<std::vector<size_t>>& edge_indices);
+void takeIndices(Grid& g, const std::vector<int>& inc);
+
 // SpectralGrid::sampleSpectralGrid + createHighResGrid* (spectral_grid.cpp:123-304)
 void sampleGrid(Grid& g, std::vector<Observation>& obs) {
   std::vector<std::vector<double>> oe;
@@ -361,7 +375,50 @@ void sampleGrid(Grid& g, std::vector<Observation>& obs) {
     if (oe[i][0] < we.back()[1] && oe[i][1] > we.back()[1]) we.back()[1] = oe[i][1];
     else we.push_back({oe[i][0], oe[i][1]});
   }
-  auto edge_indices = g.binEdges(we);
+  std::vector<int> inc = markRanges(g, g.binEdges(we));
+  for (auto& o : obs) {                                 // band edges are always included
+    size_t start = 0;
+    double last_nu = 0;
+    for (auto& b : o.edges_nu) {
+      if (last_nu > b[0]) start = 0;
+      const size_t i1 = Grid::closest(b[0], g.nu_full, start);
+      const size_t i2 = Grid::closest(b[1], g.nu_full, start);
+      inc[i1] = 1;
+      inc[i2] = 1;
+      last_nu = b[1];
+      start = i2;
+    }
+  }
+  takeIndices(g, inc);
+
+  // SpectralBands::setBandEdgeIndices + init (spectral_band.cpp:66-121)
+  for (auto& o : obs) {
+    o.edge_idx.clear();
+    size_t start = 0;
+    double last_nu = 0;
+    for (auto& b : o.edges_nu) {
+      if (last_nu > b[0]) start = 0;
+      const size_t i1 = Grid::closest(b[0], g.nu, start);
+      const size_t i2 = Grid::closest(b[1], g.nu, start);
+      if (i1 > g.nu.size() - 1 || i2 > g.nu.size() - 1) throw std::runtime_error("edge index not found");
+      if (i1 == i2) throw std::runtime_error("spectral grid too coarse to resolve a bin");
+      o.edge_idx.push_back({i1, i2});
+      last_nu = b[1];
+      start = i2;
+    }
+    o.idx_first = Grid::closest(1.0 / o.lam_range[0] * 1e4, g.nu, 0);
+    o.idx_second = Grid::closest(1.0 / o.lam_range[1] * 1e4, g.nu, 0);
+  }
+}
+
+// SpectralGrid(config, wavelength_min, wavelength_max) (spectral_grid.cpp:52-72)
+void sampleRangeGrid(Grid& g, double wavelength_min, double wavelength_max) {
+  const std::vector<std::vector<double>> edges{{1.0 / wavelength_max * 1e4, 1.0 / wavelength_min * 1e4}};
+  takeIndices(g, markRanges(g, g.binEdges(edges)));
+}
+
+// createHighResGridConst* (spectral_grid.cpp:158-236) over bin-edge index pairs
+std::vector<int> markRanges(const Grid& g, const std::vector<std::vector<size_t>>& edge_indices) {
   std::vector<int> inc(g.nu_full.size(), 0);
   for (auto& e : edge_indices) {
     inc[e[0]] = 1;
@@ -382,43 +439,15 @@ void sampleGrid(Grid& g, std::vector<Observation>& obs) {
       inc[e[1]] = 1;
     }
   }
-  for (auto& o : obs) {                                 // band edges are always included
-    size_t start = 0;
-    double last_nu = 0;
-    for (auto& b : o.edges_nu) {
-      if (last_nu > b[0]) start = 0;
-      const size_t i1 = Grid::closest(b[0], g.nu_full, start);
-      const size_t i2 = Grid::closest(b[1], g.nu_full, start);
-      inc[i1] = 1;
-      inc[i2] = 1;
-      last_nu = b[1];
-      start = i2;
-    }
-  }
+  return inc;
+}
+
+void takeIndices(Grid& g, const std::vector<int>& inc) {
   g.index.clear();
   for (size_t i = 0; i < inc.size(); ++i) if (inc[i] == 1) g.index.push_back(i);
   g.nu.resize(g.index.size());
   g.lam.resize(g.index.size());
   for (size_t i = 0; i < g.index.size(); ++i) { g.nu[i] = g.nu_full[g.index[i]]; g.lam[i] = g.lam_full[g.index[i]]; }
-
-  // SpectralBands::setBandEdgeIndices + init (spectral_band.cpp:66-121)
-  for (auto& o : obs) {
-    o.edge_idx.clear();
-    size_t start = 0;
-    double last_nu = 0;
-    for (auto& b : o.edges_nu) {
-      if (last_nu > b[0]) start = 0;
-      const size_t i1 = Grid::closest(b[0], g.nu, start);
-      const size_t i2 = Grid::closest(b[1], g.nu, start);
-      if (i1 > g.nu.size() - 1 || i2 > g.nu.size() - 1) throw std::runtime_error("edge index not found");
-      if (i1 == i2) throw std::runtime_error("spectral grid too coarse to resolve a bin");
-      o.edge_idx.push_back({i1, i2});
-      last_nu = b[1];
-      start = i2;
-    }
-    o.idx_first = Grid::closest(1.0 / o.lam_range[0] * 1e4, g.nu, 0);
-    o.idx_second = Grid::closest(1.0 / o.lam_range[1] * 1e4, g.nu, 0);
-  }
 }
 
 // SpectralGrid::interpolateToWavelengthGrid (spectral_grid_interpolate.cpp:60-172)
@@ -781,6 +810,11 @@ struct EmissionOracle {
   std::vector<OpacitySpecies> species;
   bool error_inflation = false;
   std::vector<double> default_parameters;
+  // pyBeAR's direct models: a grid sampled over [direct_range[0], direct_range[1]]
+  // micron, no observations; the eclipse's tabulated stellar flux on the grid
+  bool direct = false;
+  double direct_range[2] = {0, 0};
+  std::vector<double> stellar_flux;
 
   // per-evaluation state
   std::vector<double> temperature, altitude, mmw;
@@ -797,9 +831,22 @@ struct EmissionOracle {
   std::vector<int32_t> recorded_stops, frozen_stops;
   double last_bottom_radius = 0;
 
+  // StarSpectrumFile(wavelengths, flux, grid) (star_file_spectrum.cpp:57-72):
+  // two columns, micron and W m^-2 micron^-1, to W m^-2 cm on the grid
+  void readStellarSpectrum(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("stellar spectrum not found: " + path);
+    std::vector<double> lam, flux;
+    double a, b;
+    while (f >> a >> b) { lam.push_back(a); flux.push_back(b); }
+    for (size_t i = 0; i < flux.size(); ++i) flux[i] = flux[i] * lam[i] * lam[i] / 10000.;
+    stellar_flux = interpolateTo(lam, flux, grid.lam);
+  }
+
   void load(const std::string& folder_in) {
     std::string folder = folder_in;
     if (folder.back() != '/') folder += '/';
+    std::string stellar_file;
     std::ifstream cf(folder + "case.txt");
     if (!cf) throw std::runtime_error("case.txt not found in " + folder);
     std::string line;
@@ -841,21 +888,32 @@ struct EmissionOracle {
       if (key == "error_inflation") error_inflation = r[0] == "1";
       if (key == "parameters")
         for (size_t i = 0; i < std::stoul(r[0]); ++i) default_parameters.push_back(std::stod(r[1 + i]));
+      if (key == "direct_range") {
+        direct = true;
+        direct_range[0] = std::stod(r[0]);
+        direct_range[1] = std::stod(r[1]);
+      }
+      if (key == "stellar_spectrum") stellar_file = folder + r[0];
     }
     if (grid.discretisation < 0 || grid.discretisation > 2) throw std::runtime_error("unknown spectral discretisation");
     grid.load(folder + "wavenumber_full.dat");
 
-    std::ifstream lf(folder + "observations.list");
-    while (std::getline(lf, line)) {
-      std::istringstream in(line);
-      std::string a, b;
-      in >> a >> b;
-      if (a.empty()) continue;
-      obs.push_back(loadObservation(folder + a, b, folder));
-      nb_modifier += obs.back().nb_modifier;
+    if (direct) {
+      sampleRangeGrid(grid, direct_range[0], direct_range[1]);
+      if (!stellar_file.empty()) readStellarSpectrum(stellar_file);
+    } else {
+      std::ifstream lf(folder + "observations.list");
+      while (std::getline(lf, line)) {
+        std::istringstream in(line);
+        std::string a, b;
+        in >> a >> b;
+        if (a.empty()) continue;
+        obs.push_back(loadObservation(folder + a, b, folder));
+        nb_modifier += obs.back().nb_modifier;
+      }
+      sampleGrid(grid, obs);
+      for (auto& o : obs) setupObservation(grid, o);
     }
-    sampleGrid(grid, obs);
-    for (auto& o : obs) setupObservation(grid, o);
 
     // atmosphere (atmosphere.cpp:365-383)
     pressure.assign(nb_levels, 0.0);
@@ -977,19 +1035,73 @@ struct EmissionOracle {
       return neg;
     }
     if (fixed_mmw > 0) mmw.assign(L, fixed_mmw);               // atmosphere.cpp:168
-    // altitude (atmosphere.cpp:288-334)
+    hydrostaticAltitude(g, bottom_radius, variable_gravity);
+    return neg;
+  }
+
+  // Atmosphere::setAtmosphericStructure with FixedChemistry (atmosphere.cpp:245-283,
+  // fixed_chemistry.cpp:71-89, chemistry.h:47-58)
+  void setStructure(double g, double bottom_radius, bool var_gravity, const std::vector<double>& p,
+                    const std::vector<double>& t, const std::vector<int>& ids, const std::vector<double>& x) {
+    const size_t L = nb_levels, S = ids.size();
+    pressure = p;
+    temperature = t;
+    nd.assign(L, std::vector<double>(kNbSpecies, 0.0));
+    for (size_t i = 0; i < L; ++i) {
+      nd[i][kTotal] = pressure[i] * 1.e6 / kBoltzmannK / temperature[i];
+      for (size_t j = 0; j < S; ++j) nd[i][ids[j]] = nd[i][kTotal] * x[i * S + j];
+    }
+    mmw.assign(L, 0.0);
+    for (size_t i = 0; i < L; ++i) {
+      double mu = 0;
+      for (int j = 0; j < kNbSpecies; ++j) mu += nd[i][j] / nd[i][kTotal] * kSpecies[j].weight;
+      mmw[i] = mu;
+    }
+    altitude.assign(L, 0.0);
+    hydrostaticAltitude(g, bottom_radius, var_gravity);
+  }
+
+  // hydrostatic altitude from mmw, T and p (atmosphere.cpp:288-334)
+  void hydrostaticAltitude(double g, double bottom_radius, bool var_gravity) {
+    const size_t L = nb_levels;
     std::vector<double> rho(L);
     for (size_t i = 0; i < L; ++i) rho[i] = mmw[i] * pressure[i] * 1e6 / (kGasConstant * temperature[i]);
     for (size_t i = 1; i < L; ++i) {
       double gl = g;
-      if (variable_gravity) {
+      if (var_gravity) {
         const double local_radius = bottom_radius + altitude[i - 1];
         gl = g * std::pow(bottom_radius / local_radius, 2.0);
       }
       const double dz = (1.0 / (rho[i] * gl) + 1.0 / (rho[i - 1] * gl)) * 0.5 * (pressure[i - 1] * 1e6 - pressure[i] * 1e6);
       altitude[i] = altitude[i - 1] + dz;
     }
-    return neg;
+  }
+
+  // the direct models' calcSpectrum: emission (emission.cpp:268-340, a =
+  // radius, b = distance), transmission (transmission.cpp:371-465, a = planet
+  // radius, b = radius ratio), eclipse (secondary_eclipse.cpp:395-482, a =
+  // radius ratio)
+  void directSpectrum(double g, double a, double b, const std::vector<double>& p, const std::vector<double>& t,
+                      const std::vector<int>& ids, const std::vector<double>& x, bool var_gravity) {
+    if (model == "transmission") {
+      setStructure(g, a, var_gravity, p, t, ids, x);
+      opacity();
+      last_bottom_radius = a;
+      transmissionSpectrum(a, a / b);
+      spectrum = spectrum_flux;
+    } else if (model == "secondary_eclipse") {
+      setStructure(g, 1.0, false, p, t, ids, x);
+      opacity();
+      radiativeTransfer(1.0);
+      spectrum.resize(spectrum_flux.size());
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] = spectrum_flux[i] / stellar_flux[i] * a * a * 1e6;
+    } else {
+      setStructure(g, a, var_gravity, p, t, ids, x);
+      opacity();
+      radiativeTransfer(1.0);
+      spectrum = spectrum_flux;
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] *= a * a / b / b;
+    }
   }
 
   // OpacityCalculation::calculate + TransportCoefficients::calculate
@@ -1061,21 +1173,33 @@ struct EmissionOracle {
   // the sum never passes 10).  With frozen stops ([N][L-1], test
   // infrastructure for derivatives with the stopping segments held fixed) each
   // path stops at its given segment instead.
-  void tangentDepths(size_t w, double bottom_radius, double* tau, int32_t* stop) const {
+  // Segment lengths of every tangent path, path[t][k] = d(t, t+k+1) - d(t, t+k)
+  // with d(t, l) = sqrt((R + z_l)^2 - (R + z_t)^2) (transmission_spectrum.cpp:117-118,
+  // 145-154); the same for every wavenumber, so computed once per evaluation.
+  std::vector<std::vector<double>> pathLengths(double bottom_radius) const {
     const size_t L = nb_levels;
     auto dist = [&](size_t tangent, size_t level) {
       const double a = bottom_radius + altitude[level];
       const double b = bottom_radius + altitude[tangent];
       return std::sqrt(a * a - b * b);
     };
+    std::vector<std::vector<double>> path(L - 1);
+    for (size_t t = 0; t + 1 < L; ++t) {
+      path[t].resize(L - 1 - t);
+      for (size_t k = 0; k < path[t].size(); ++k) path[t][k] = dist(t, t + k + 1) - dist(t, t + k);
+    }
+    return path;
+  }
+
+  void tangentDepths(size_t w, const std::vector<std::vector<double>>& path, double* tau, int32_t* stop) const {
+    const size_t L = nb_levels;
     const int32_t* frozen = frozen_stops.empty() ? nullptr : frozen_stops.data() + w * (L - 1);
-    std::vector<double> path(L - 1), ext(L);
+    std::vector<double> ext(L);
     for (size_t i = 0; i < L; ++i) ext[i] = kappa[i][w] + scattering[i][w];
     for (int t = int(L) - 2; t > -1; --t) {
-      const size_t n = L - 1 - size_t(t);
-      for (size_t k = 0; k < n; ++k) path[k] = dist(size_t(t), size_t(t) + k + 1) - dist(size_t(t), size_t(t) + k);
       int32_t last = 0;
-      tau[t] = tangentSum(path.data(), ext.data() + t, n, frozen ? frozen[t] - t : -1, &last);
+      tau[t] = tangentSum(path[size_t(t)].data(), ext.data() + t, path[size_t(t)].size(), frozen ? frozen[t] - t : -1,
+                          &last);
       if (stop) stop[t] = last + t;
     }
   }
@@ -1084,10 +1208,11 @@ struct EmissionOracle {
     const size_t N = grid.nu.size(), L = nb_levels;
     spectrum_flux.assign(N, 0.0);
     if (record_stops) recorded_stops.assign(N * (L - 1), 0);
+    const std::vector<std::vector<double>> path = pathLengths(bottom_radius);
     #pragma omp parallel for schedule(dynamic, 256)
     for (size_t w = 0; w < N; ++w) {
       std::vector<double> trans(L, 0.0), tau(L - 1, 0.0);
-      tangentDepths(w, bottom_radius, tau.data(), record_stops ? recorded_stops.data() + w * (L - 1) : nullptr);
+      tangentDepths(w, path, tau.data(), record_stops ? recorded_stops.data() + w * (L - 1) : nullptr);
       trans.back() = 1;
       for (size_t t = 0; t + 1 < L; ++t) trans[t] = std::exp(-tau[t]);
       double h = 0;
@@ -1269,6 +1394,28 @@ void oracle_coefficients(void* h, double* kappa, double* scattering) {
 
 int oracle_nb_species_ids() { return kNbSpecies; }
 
+// the direct models' calcSpectrum (EmissionOracle::directSpectrum): species
+// as newline-separated symbols, mixing ratios [levels][species], spectrum [N]
+int oracle_direct_spectrum(void* h, double g, double a, double b, const double* p, const double* t,
+                           const char* species, const double* x, int variable_gravity, double* spectrum) {
+  try {
+    auto* m = static_cast<EmissionOracle*>(h);
+    if (!m->direct) throw std::runtime_error("not a direct model (no direct_range)");
+    std::vector<int> ids;
+    std::istringstream in(species ? species : "");
+    std::string sym;
+    while (in >> sym) ids.push_back(fixedChemistryId(sym));
+    const size_t L = m->nb_levels;
+    m->directSpectrum(g, a, b, std::vector<double>(p, p + L), std::vector<double>(t, t + L), ids,
+                      std::vector<double>(x, x + L * ids.size()), variable_gravity != 0);
+    std::copy(m->spectrum.begin(), m->spectrum.end(), spectrum);
+    return 0;
+  } catch (std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
 // one tangent path's sum (tangentSum) on given segment lengths [n] and extinctions [n + 1]
 double oracle_tangent_sum(const double* path, const double* ext, size_t n, int32_t frozen, int32_t* last) {
   return tangentSum(path, ext, n, frozen, last);
@@ -1321,7 +1468,7 @@ int oracle_transmission_paths(void* h, size_t w, double* tau, int32_t* stop) {
     g_error = "no transmission evaluation at this wavenumber";
     return -1;
   }
-  m->tangentDepths(w, m->last_bottom_radius, tau, stop);
+  m->tangentDepths(w, m->pathLengths(m->last_bottom_radius), tau, stop);
   return 0;
 }
 

@@ -12,6 +12,14 @@ for p in (ROOT, os.path.join(ROOT, "tests")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (MI355X) and the built HIP library")
+    # torch carries its own HIP runtime beside the engine's: let torch
+    # enumerate the devices before the engine's runtime starts, or torch
+    # finds no device afterwards (counting devices does not initialise one)
+    try:
+        import torch
+        torch.cuda.device_count()
+    except Exception:
+        pass
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-s", "-f", "oracle/Makefile"], cwd=ROOT, check=True)
 
