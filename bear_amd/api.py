@@ -100,6 +100,8 @@ def lib():
         L.bear_retrieval_set_profiling.argtypes = [vp, C.c_int]
         L.bear_retrieval_stats.argtypes = [vp, C.POINTER(EvalStats)]
         L.bear_forward_model_calc_model_gpu.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
+        L.bear_forward_model_calc_model_gpu_noalloc.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p),
+                                                                C.POINTER(C.c_int)]
         L.bear_retrieval_set_table_precision.argtypes = [vp, C.c_int]
         L.bear_retrieval_set_batch_options.argtypes = [vp, C.c_int, C.c_int]
         L.bear_retrieval_compute_likelihood_batch.argtypes = [vp, _dp, sz, sz, _dp]
@@ -353,15 +355,17 @@ class Retrieval:
         _check(self._L.bear_retrieval_stats(self._h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in EvalStats._fields_}
 
-    def calcModelGPU(self, physical_parameters, spectrum_ptr, spectrum_obs_ptrs):
+    def calcModelGPU(self, physical_parameters, spectrum_ptr, spectrum_obs_ptrs, no_alloc=False):
         """ForwardModel::calcModelGPU on caller-owned device buffers (raw
-        addresses, e.g. torch tensors' data_ptr()); returns the neglect flag."""
+        addresses, e.g. torch tensors' data_ptr()); returns the neglect flag.
+        no_alloc: the variant that keeps its per-evaluation device buffers in
+        the library's static scratch array and allocates nothing."""
         p = np.ascontiguousarray(physical_parameters, dtype=np.float64)
         ptrs = (C.c_void_p * max(len(spectrum_obs_ptrs), 1))(*[C.c_void_p(int(x)) for x in spectrum_obs_ptrs])
         neg = C.c_int(0)
-        _check(self._L.bear_forward_model_calc_model_gpu(self._h, _ptr(p), p.size,
-                                                         C.c_void_p(int(spectrum_ptr)) if spectrum_ptr else None,
-                                                         ptrs, C.byref(neg)))
+        f = self._L.bear_forward_model_calc_model_gpu_noalloc if no_alloc else self._L.bear_forward_model_calc_model_gpu
+        _check(f(self._h, _ptr(p), p.size, C.c_void_p(int(spectrum_ptr)) if spectrum_ptr else None, ptrs,
+                 C.byref(neg)))
         return bool(neg.value)
 
 

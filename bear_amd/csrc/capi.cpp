@@ -220,6 +220,16 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
   });
 }
 
+int bear_forward_model_calc_model_gpu_noalloc(bear_retrieval* r, const double* parameters, size_t n,
+                                              double* spectrum_dev, double* const* spectrum_obs_dev,
+                                              int* neglect_model) {
+  return guarded([&] {
+    if (!r || !parameters) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null argument");
+    const bool neglect = r->impl->calcModelGPUNoAlloc(parameters, n, spectrum_dev, spectrum_obs_dev);
+    if (neglect_model) *neglect_model = neglect;
+  });
+}
+
 int bear_retrieval_set_table_precision(bear_retrieval* r, int precision) {
   return guarded([&] {
     if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");

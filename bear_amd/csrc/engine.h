@@ -206,6 +206,16 @@ class DeviceEngine {
   void synchronize();
   double* spectrumBuffer();                      // engine-owned [N], allocated on first use
 
+  // No-allocation mode of the single-model path: while on, the evaluate*
+  // calls keep their per-evaluation device buffers (staged inputs, flux, bin
+  // piece sums, bins, log-likelihood) in the library's static scratch array
+  // on this device instead of the engine's own, and allocate nothing.  On
+  // raises InvalidInput when this model's buffers exceed the array
+  // (scratchBytes()).  Calls that overlap on one GPU share the array and race.
+  void useScratch(bool on);
+  size_t scratchRequired() const;
+  static size_t scratchBytes();
+
   // ---- batched evaluation: many live points per launch
   // Table storage of the batched kernels; species_rows[s] = [first, end) of
   // species s's rows in the table store.
@@ -276,6 +286,8 @@ class DeviceEngine {
     double* ll;
   };
   EvalBuffers buffers() const;
+  bool use_scratch_ = false;
+  unsigned char* scratch_ = nullptr;             // this device's static scratch array
   void upload();                                 // staging -> device, on the stream
   void beginTimed();
   void endTimed();

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "device_math.h"
@@ -24,6 +25,14 @@
 namespace bear_amd {
 
 using namespace dev;
+
+// The single-model path's per-evaluation scratch for callers that cannot
+// allocate device memory (DeviceEngine::useScratch): one array per GPU, held
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel: its device address on each GPU comes from hipGetSymbolAddress at
+// set-up (scratchAddress).
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -261,6 +270,25 @@ void deviceFree(void* p) {
 
 constexpr size_t kMaxSpecies = 16;                    // register-cached kernels up to 16 species
 
+// scratch's address on `device` (current), resolved once per device
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_address[kMaxDevices] = {};
+
+unsigned char* scratchAddress(int device) {
+  if (device < 0 || device >= kMaxDevices) raise(kDevice, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_address[device]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) raise(kDevice, "no device address for the scratch array");
+    g_scratch_address[device] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_address[device];
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
 }  // namespace
 
 DeviceEngine::DeviceEngine(int device) : device_(device) {
@@ -276,10 +304,39 @@ DeviceEngine::DeviceEngine(int device) : device_(device) {
   check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   nb_cus_ = prop.multiProcessorCount;
   lds_limit_ = prop.sharedMemPerBlock;
+  scratch_ = scratchAddress(device);
+}
+
+size_t DeviceEngine::scratchBytes() { return kScratchBytes; }
+
+// staged inputs, flux [stride], piece sums [pieces][2], bins, log-likelihood
+size_t DeviceEngine::scratchRequired() const {
+  return align256(staging_bytes_) + align256(stride_ * sizeof(double)) + align256(n_chunks_ * 2 * sizeof(double)) +
+         align256(n_bins_ * sizeof(double)) + align256(sizeof(double));
+}
+
+void DeviceEngine::useScratch(bool on) {
+  if (on && scratchRequired() > kScratchBytes)
+    raise(kInvalidInput, "the no-allocation path holds at most " + std::to_string(kScratchBytes >> 20) +
+                             " MiB of per-evaluation buffers; this model needs " + std::to_string(scratchRequired()) +
+                             " bytes");
+  use_scratch_ = on;
 }
 
 DeviceEngine::EvalBuffers DeviceEngine::buffers() const {
-  return EvalBuffers{staging_dev_, flux_, chunk_sums_, bins_, ll_};
+  if (!use_scratch_) return EvalBuffers{staging_dev_, flux_, chunk_sums_, bins_, ll_};
+  unsigned char* p = scratch_;
+  EvalBuffers b;
+  b.staging = p;
+  p += align256(staging_bytes_);
+  b.flux = reinterpret_cast<double*>(p);
+  p += align256(stride_ * sizeof(double));
+  b.chunk_sums = reinterpret_cast<double*>(p);
+  p += align256(n_chunks_ * 2 * sizeof(double));
+  b.bins = reinterpret_cast<double*>(p);
+  p += align256(n_bins_ * sizeof(double));
+  b.ll = reinterpret_cast<double*>(p);
+  return b;
 }
 
 DeviceEngine::~DeviceEngine() {

@@ -621,6 +621,21 @@ bool Retrieval::calcModelGPU(const double* p, size_t n, double* spectrum_dev, do
   return neglect;
 }
 
+bool Retrieval::calcModelGPUNoAlloc(const double* p, size_t n, double* spectrum_dev, double* const* obs_dev) {
+  checkSize(n);
+  engine_->useScratch(true);
+  bool neglect = false;
+  try {
+    neglect = model_->calcModelGPU(p, errorInflation(p, n), spectrum_dev, obs_dev);
+    engine_->synchronize();
+  } catch (...) {
+    engine_->useScratch(false);
+    throw;
+  }
+  engine_->useScratch(false);
+  return neglect;
+}
+
 std::vector<char> Retrieval::stageBatch(const double* params, size_t batch, size_t n) {
   checkSize(n);
   engine_->reserveBatch(batch);

@@ -217,6 +217,9 @@ class Retrieval {
   // ForwardModel::calcModel (forward_model.cpp:60-127) with host outputs
   bool computeModel(const double* parameters, size_t n, bool high_res, double* spectrum, double* spectrum_obs);
   bool calcModelGPU(const double* parameters, size_t n, double* spectrum_dev, double* const* spectrum_obs_dev);
+  // the same with the per-evaluation device buffers in the static scratch
+  // array (DeviceEngine::useScratch): no device memory is allocated
+  bool calcModelGPUNoAlloc(const double* parameters, size_t n, double* spectrum_dev, double* const* spectrum_obs_dev);
   // batches: parameters [batch][n] row-major
   void computeLikelihoodBatch(const double* parameters, size_t batch, size_t n, double* log_likelihood);
   void computeModelBatch(const double* parameters, size_t batch, size_t n, double* spectrum, double* spectrum_obs,

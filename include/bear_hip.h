@@ -167,6 +167,22 @@ int bear_forward_model_calc_model_gpu(bear_retrieval* r, const double* parameter
                                       double* spectrum_dev, double* const* spectrum_obs_dev,
                                       int* neglect_model);
 
+/* bear_forward_model_calc_model_gpu for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results; the per-evaluation device buffers (staged level and stencil
+ * records, the flux, bin piece sums, bins, the log-likelihood) live in a
+ * static device array of 16 MiB per GPU, held until the process ends, so
+ * the call allocates nothing (the caller's output buffers and the
+ * retrieval's tables are set up beforehand as usual).  A model whose buffers
+ * exceed 16 MiB (more than about 1.9 million spectral points) is refused
+ * with BEAR_ERROR_INVALID_INPUT.  Every call on one GPU uses that GPU's
+ * array: calls that overlap on one GPU (from several threads or retrievals)
+ * race. */
+int bear_forward_model_calc_model_gpu_noalloc(bear_retrieval* r, const double* parameters, size_t nb_parameters,
+                                              double* spectrum_dev, double* const* spectrum_obs_dev,
+                                              int* neglect_model);
+
 /* ---------------------------------------------------------------- batches
  * Many live points per launch sequence.  The reference evaluates one point
  * per call (the sampler callback, retrieval.h:118-129 and
