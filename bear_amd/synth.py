@@ -733,6 +733,44 @@ def write_case(case: Case, folder: str, tables: bool = True) -> str:
     return folder
 
 
+def write_direct_case(case: Case, folder: str, wavelength_range, stellar=None) -> str:
+    """The case's tables and wavenumber list for pyBeAR's direct models: a
+    grid sampled over wavelength_range (micron) without observations, and
+    for the eclipse a tabulated stellar flux (wavelengths in micron, flux in
+    W m^-2 micron^-1) written to stellar_spectrum.dat."""
+    folder = write_case(case, folder)
+    with open(folder + "case.txt", "a") as f:
+        f.write(f"direct_range {_fmt(wavelength_range[0])} {_fmt(wavelength_range[1])}\n")
+        if stellar is not None:
+            f.write("stellar_spectrum stellar_spectrum.dat\n")
+    if stellar is not None:
+        with open(folder + "stellar_spectrum.dat", "w") as f:
+            for a, b in zip(*stellar):
+                f.write(f"{_fmt(a)} {_fmt(b)}\n")
+    return folder
+
+
+def direct_structure(rng, nb_levels, species, p_bottom=100.0, p_top=1e-6):
+    """A caller-made atmosphere for the direct models: pressures log-spaced
+    from the bottom (bar), a smooth temperature profile, per-level mixing
+    ratios of the given species (H2 and He fill most of the rest)."""
+    p = np.logspace(np.log10(p_bottom), np.log10(p_top), nb_levels)
+    x = np.linspace(0.0, 1.0, nb_levels)
+    t = 1800.0 - 900.0 * x + 150.0 * np.sin(3.0 * x + rng.uniform(0, 1))
+    trace = [s for s in species if s not in ("H2", "He")]
+    mix = np.zeros((nb_levels, len(species)))
+    for j, s in enumerate(species):
+        if s in trace:
+            mix[:, j] = 10.0 ** (rng.uniform(-6.0, -3.5) + 0.5 * np.sin(2.0 * x + j))
+    rest = 1.0 - mix.sum(axis=1)
+    for j, s in enumerate(species):
+        if s == "H2":
+            mix[:, j] = rest * 0.85
+        if s == "He":
+            mix[:, j] = rest * 0.15
+    return p, t, mix
+
+
 # ---------------------------------------------------------------------------
 # engine construction from a case
 # ---------------------------------------------------------------------------

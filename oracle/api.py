@@ -59,6 +59,9 @@ def lib():
         L.oracle_transmission_stops.argtypes = [vp, i32]
         L.oracle_transmission_freeze_stops.argtypes = [vp, i32]
         L.oracle_transmission_paths.argtypes = [vp, sz, dp, i32]
+        L.oracle_direct_spectrum.restype = C.c_int
+        L.oracle_direct_spectrum.argtypes = [vp, C.c_double, C.c_double, C.c_double, dp, dp, C.c_char_p, dp, C.c_int,
+                                             dp]
         L.oracle_tangent_sum.restype = C.c_double
         L.oracle_tangent_sum.argtypes = [dp, dp, sz, C.c_int32, i32]
         _lib = L
@@ -149,6 +152,22 @@ class EmissionOracle:
             out.append(bins[k:k + nb].copy())
             k += nb
         return dict(spectrum_flux=flux, spectrum=spec, bins=out, loglike=ll.value, neglect=bool(neg.value))
+
+    def direct_spectrum(self, surface_gravity, a, b, pressure, temperature, species, mixing_ratios,
+                        use_variable_gravity=False):
+        """The direct models' calcSpectrum (a case written with a direct
+        range): emission a = radius, b = distance; transmission a = planet
+        radius, b = radius ratio; eclipse a = radius ratio (b unused).
+        mixing_ratios [levels][species]."""
+        p = np.ascontiguousarray(pressure, dtype=np.float64)
+        t = np.ascontiguousarray(temperature, dtype=np.float64)
+        x = np.ascontiguousarray(mixing_ratios, dtype=np.float64)
+        out = np.empty(self.nb_points)
+        rc = self._L.oracle_direct_spectrum(self._h, float(surface_gravity), float(a), float(b), _p(p), _p(t),
+                                            "\n".join(species).encode(), _p(x), int(use_variable_gravity), _p(out))
+        if rc != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+        return out
 
     # ---- transmission: stopping segments of the tangent paths
     def record_stops(self, enable=True):

@@ -32,7 +32,8 @@ SRCS := config/global_config.cpp \
         chemistry/isoprofile_chemistry.cpp chemistry/background_chemistry.cpp \
         chemistry/fixed_chemistry.cpp \
         radiative_transfer/short_characteristics.cpp \
-        forward_model/stellar_spectrum/star_blackbody.cpp
+        forward_model/stellar_spectrum/star_blackbody.cpp \
+        forward_model/stellar_spectrum/star_file_spectrum.cpp
 
 OBJS := $(addprefix $(OBJ)/,$(subst /,_,$(SRCS:.cpp=.o)))
 

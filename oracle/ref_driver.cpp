@@ -42,6 +42,7 @@
 #include "chemistry/background_chemistry.h"
 #include "radiative_transfer/short_characteristics.h"
 #include "forward_model/stellar_spectrum/star_blackbody.h"
+#include "forward_model/stellar_spectrum/star_file_spectrum.h"
 #include "additional/physical_const.h"
 
 namespace {
@@ -75,7 +76,46 @@ struct CaseText {
   std::vector<std::string> opac_symbol, opac_folder;
   bool error_inflation = false;
   std::vector<double> parameters;
+  bool direct = false;                     // pyBeAR's direct models over direct_range (micron)
+  double direct_range[2] = {0, 0};
+  std::string stellar_spectrum;
 };
+
+// The direct models' inputs (direct_input.txt): gravity, a, b,
+// variable_gravity, species symbols, then one "p T x_1 .. x_S" line per level.
+struct DirectInput {
+  double g = 0, a = 0, b = 0;
+  bool variable_gravity = false;
+  std::vector<std::string> species;
+  std::vector<double> pressure, temperature;
+  std::vector<std::vector<double>> mixing_ratios;
+};
+
+DirectInput readDirectInput(const std::string& path) {
+  DirectInput d;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    std::string key;
+    in >> key;
+    if (key == "gravity") in >> d.g;
+    else if (key == "a") in >> d.a;
+    else if (key == "b") in >> d.b;
+    else if (key == "variable_gravity") { int v = 0; in >> v; d.variable_gravity = v != 0; }
+    else if (key == "species") { std::string s; while (in >> s) d.species.push_back(s); }
+    else if (key == "level") {
+      double p, t;
+      in >> p >> t;
+      std::vector<double> x(d.species.size());
+      for (auto& v : x) in >> v;
+      d.pressure.push_back(p);
+      d.temperature.push_back(t);
+      d.mixing_ratios.push_back(x);
+    }
+  }
+  return d;
+}
 
 CaseText readCase(const std::string& folder) {
   CaseText c;
@@ -101,6 +141,11 @@ CaseText readCase(const std::string& folder) {
     else if (key == "chemistry") c.chemistry.push_back(rest);
     else if (key == "opacity") { c.opac_symbol.push_back(rest[0]); c.opac_folder.push_back(rest[1]); }
     else if (key == "error_inflation") c.error_inflation = rest[0] == "1";
+    else if (key == "direct_range") {
+      c.direct = true;
+      c.direct_range[0] = std::stod(rest[0]);
+      c.direct_range[1] = std::stod(rest[1]);
+    } else if (key == "stellar_spectrum") c.stellar_spectrum = rest[0];
     else if (key == "parameters") {
       size_t n = std::stoul(rest[0]);
       for (size_t i = 0; i < n; ++i) c.parameters.push_back(std::stod(rest[1 + i]));
@@ -110,6 +155,60 @@ CaseText readCase(const std::string& folder) {
 }
 
 }  // namespace
+
+// pyBeAR's direct models, composed as EmissionModel::calcSpectrum
+// (emission.cpp:268-340) and OccultationModel::calcSpectrum
+// (secondary_eclipse.cpp:395-482) do on the CPU; for transmission the
+// atmosphere and the coefficients (the tangent-path integral is a
+// TransmissionModel member, whose class cannot be built here).
+int runDirect(const CaseText& cs, bear::GlobalConfig& config, const std::string& folder, const std::string& out) {
+  using namespace bear;
+  SpectralGrid grid(&config, cs.direct_range[0], cs.direct_range[1]);
+  const std::vector<double> boundaries = {10, 1e-5};
+  Atmosphere atmosphere(cs.nb_levels, boundaries, false);
+  OpacityCalculation opacity(&config, &grid, &atmosphere, cs.opac_symbol, cs.opac_folder, false, true);
+  ShortCharacteristics rt(&grid);
+  const DirectInput d = readDirectInput(folder + "direct_input.txt");
+  const bool transmission = cs.model == "transmission", eclipse = cs.model == "secondary_eclipse";
+  const double radius = transmission || !eclipse ? d.a : 1.0;
+  atmosphere.setAtmosphericStructure(d.g, radius, eclipse ? false : d.variable_gravity, d.pressure, d.temperature,
+                                     d.species, d.mixing_ratios);
+  std::vector<CloudModel*> no_clouds;
+  opacity.calculate(no_clouds, std::vector<double>{});
+  std::vector<double> spectrum;
+  if (!transmission) {
+    spectrum.assign(grid.nbSpectralPoints(), 0.0);
+    rt.calcSpectrum(atmosphere, opacity.absorption_coeff, opacity.absorption_coeff, opacity.cloud_optical_depths,
+                    opacity.cloud_single_scattering, opacity.cloud_asym_param, 1.0, spectrum);
+    if (eclipse) {
+      std::vector<double> lam, flux;
+      std::ifstream f(folder + cs.stellar_spectrum);
+      double a, b;
+      while (f >> a >> b) { lam.push_back(a); flux.push_back(b); }
+      StarSpectrumFile star(lam, flux, &grid);
+      const std::vector<double> stellar = star.calcFlux(std::vector<double>{});
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] = spectrum[i] / stellar[i] * d.a * d.a * 1e6;
+    } else {
+      for (size_t i = 0; i < spectrum.size(); ++i) spectrum[i] *= d.a * d.a / d.b / d.b;
+    }
+  }
+  writeNpy(out + "wavenumber.npy", grid.wavenumber_list);
+  writeNpy(out + "wavelength.npy", grid.wavelength_list);
+  writeNpy(out + "temperature.npy", atmosphere.temperature);
+  writeNpy(out + "pressure.npy", atmosphere.pressure);
+  writeNpy(out + "altitude.npy", atmosphere.altitude);
+  std::vector<double> nd;
+  for (auto& l : atmosphere.number_densities) nd.insert(nd.end(), l.begin(), l.end());
+  writeNpy(out + "number_densities.npy", nd, atmosphere.number_densities.size());
+  std::vector<double> k, sc;
+  for (auto& row : opacity.absorption_coeff) k.insert(k.end(), row.begin(), row.end());
+  for (auto& row : opacity.scattering_coeff) sc.insert(sc.end(), row.begin(), row.end());
+  writeNpy(out + "absorption_coeff.npy", k, opacity.absorption_coeff.size());
+  writeNpy(out + "scattering_coeff.npy", sc, opacity.scattering_coeff.size());
+  if (!spectrum.empty()) writeNpy(out + "spectrum.npy", spectrum);
+  std::printf("direct %s N=%zu\n", cs.model.c_str(), grid.nbSpectralPoints());
+  return 0;
+}
 
 int main(int argc, char** argv) {
   using namespace bear;
@@ -130,6 +229,8 @@ int main(int argc, char** argv) {
   config.spectral_disecretisation = cs.discretisation;
   config.spectral_resolution = cs.resolution;
   config.use_error_inflation = cs.error_inflation;
+
+  if (cs.direct) return runDirect(cs, config, folder, out);
 
   SpectralGrid grid(&config);
 

@@ -8,9 +8,9 @@ inputs that matter (parameter vectors, injected data, a digest of the table
 files) and the reference's outputs: sampled grid, band-edge and convolution
 indices, atmosphere, spectra, bins and log-likelihoods.
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py [case ...]
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [case ... | direct]
 
-(all cases but the wide-band ones by default)
+(all cases but the wide-band ones by default; "direct": the direct models)
 """
 
 import hashlib
@@ -113,5 +113,40 @@ def main():
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def make_direct():
+    """Fixtures of pyBeAR's direct models: per model, the calcSpectrum
+    arguments of cases.direct_case and the reference build's atmosphere,
+    coefficients (every 7th wavenumber) and, for emission and eclipse, the
+    spectrum."""
+    tmp = tempfile.mkdtemp()
+    try:
+        for model in cases.DIRECT_MODELS:
+            folder, inputs = cases.direct_case(model, os.path.join(tmp, model))
+            fixture = {"table_digest": np.array(table_digest(folder))}
+            for k, d in enumerate(inputs):
+                cases.write_direct_input(folder, d)
+                out = api.run_reference(folder, os.path.join(tmp, f"{model}_{k}"))
+                pre = f"d{k}_"
+                for key in ("g", "a", "b", "variable_gravity"):
+                    fixture[pre + key] = np.array(d[key])
+                fixture[pre + "pressure"], fixture[pre + "temperature"] = d["p"], d["t"]
+                fixture[pre + "mixing_ratios"] = d["x"]
+                for key in ("altitude", "number_densities"):
+                    fixture[pre + key] = out[key]
+                fixture[pre + "absorption_coeff"] = out["absorption_coeff"][::7]
+                fixture[pre + "scattering_coeff"] = out["scattering_coeff"][::7]
+                if "spectrum" in out:
+                    fixture[pre + "spectrum"] = out["spectrum"]
+            fixture["wavenumber"] = out["wavenumber"]
+            path = os.path.join(ROOT, "tests", "golden", f"direct_{model}.npz")
+            np.savez_compressed(path, **fixture)
+            print(path, os.path.getsize(path), "bytes")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["direct"]:
+        make_direct()
+    else:
+        main()

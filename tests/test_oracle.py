@@ -376,3 +376,75 @@ def test_grid_discretisations_differ(case_factory):
     assert np.percentile(dlam, 95) <= 2e-3 and np.median(dlam) > 1e-3
     assert np.percentile(res, 5) >= 1500 * (1 - 1e-9) and np.median(res) < 2 * 1500
     assert len({g.size for g in grids.values()}) == 3
+
+
+# ---------------------------------------------------------------- direct models
+def test_direct_models_closed_forms(tmp_path):
+    """pyBeAR's direct calcSpectrum in the oracle: an isothermal atmosphere
+    emits pi B(T) 1e-3 (R/d)^2; in pure N2 (no absorber) the transit depth
+    is the bare disc; the mean molecular weight is the mixing-ratio average of the
+    given species (FixedChemistry, fixed_chemistry.cpp:71-89)."""
+    case = synth.tiny_case()
+    folder = synth.write_direct_case(case, str(tmp_path / "direct"), (2.0, 4.5))
+    o = api.EmissionOracle(folder)
+    L = o.nb_levels
+    rng = np.random.default_rng(2)
+    species = ["H2O", "CH4", "H2", "He"]
+    p, _, x = synth.direct_structure(rng, L, species)
+    r, d = synth.RADIUS_JUPITER, 10 * synth.PARSEC
+    spec = o.direct_spectrum(2e3, r, d, p, np.full(L, 1400.0), species, x)
+    nu = o.grid()[0]
+    assert nu.min() >= 1e4 / 4.5 * 0.999 and nu.max() <= 1e4 / 2.0 * 1.001
+    expected = np.pi * _planck(1400.0, nu) * 1e-3 * r * r / d / d
+    np.testing.assert_allclose(spec, expected, rtol=1e-11)
+    weights = np.array([18.01528, 16.04246, 2.01588, 4.002602])
+    np.testing.assert_allclose(o.atmosphere()["mean_molecular_weight"], x @ weights, rtol=1e-13)
+    np.testing.assert_allclose(o.atmosphere()["pressure"], p, rtol=0)
+    case.model = "transmission"
+    folder = synth.write_direct_case(case, str(tmp_path / "direct_t"), (2.0, 4.5))
+    o = api.EmissionOracle(folder)
+    ratio = 0.1
+    spec = o.direct_spectrum(2e3, r, ratio, p, np.full(L, 900.0), ["N2"], np.ones((L, 1)))   # no opacity
+    np.testing.assert_allclose(spec, ratio ** 2 * 1e6, rtol=1e-13)
+
+
+@pytest.mark.parametrize("model", ["emission", "transmission", "secondary_eclipse"])
+def test_oracle_direct_matches_reference_fixtures(tmp_path, model):
+    """The oracle's direct calcSpectrum against the reference build's
+    composition of the same classes (setAtmosphericStructure with
+    FixedChemistry, OpacityCalculation, ShortCharacteristics, the star from
+    a tabulated spectrum): altitudes, number densities, coefficients and
+    spectra."""
+    import cases
+    from golden.make_golden import table_digest
+    g = np.load(os.path.join(GOLDEN, f"direct_{model}.npz"))
+    folder, inputs = cases.direct_case(model, str(tmp_path / model))
+    assert str(g["table_digest"]) == table_digest(folder)
+    o = api.EmissionOracle(folder)
+    np.testing.assert_array_equal(o.grid()[0], g["wavenumber"])
+    for k, d in enumerate(inputs):
+        pre = f"d{k}_"
+        np.testing.assert_array_equal(d["x"], g[pre + "mixing_ratios"])
+        spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"])
+        atm = o.atmosphere()
+        np.testing.assert_allclose(atm["altitude"], g[pre + "altitude"], rtol=1e-13)
+        np.testing.assert_allclose(atm["number_densities"], g[pre + "number_densities"], rtol=1e-14)
+        kap, sca = o.coefficients()
+        np.testing.assert_allclose(kap.T[::7], g[pre + "absorption_coeff"], rtol=1e-13)
+        np.testing.assert_allclose(sca.T[::7], g[pre + "scattering_coeff"], rtol=1e-13)
+        if pre + "spectrum" in g:
+            np.testing.assert_allclose(spec, g[pre + "spectrum"], rtol=1e-12)
+
+
+@pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
+@pytest.mark.parametrize("model", ["emission", "secondary_eclipse"])
+def test_oracle_direct_matches_live_reference(tmp_path, model):
+    import cases
+    folder, inputs = cases.direct_case(model, str(tmp_path / model))
+    o = api.EmissionOracle(folder)
+    d = inputs[1]
+    cases.write_direct_input(folder, d)
+    ref = api.run_reference(folder, str(tmp_path / "ref"))
+    spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"])
+    np.testing.assert_array_equal(spec, ref["spectrum"])
+    np.testing.assert_array_equal(o.atmosphere()["altitude"], ref["altitude"])
