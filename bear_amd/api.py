@@ -33,7 +33,7 @@ class _GlobalConfig(C.Structure):
     _fields_ = [("use_gpu", C.c_int), ("device", C.c_int), ("retrieval_folder_path", C.c_char_p),
                 ("cross_section_file_path", C.c_char_p), ("wavenumber_file_path", C.c_char_p),
                 ("spectral_discretisation", C.c_int), ("spectral_resolution", C.c_double),
-                ("use_error_inflation", C.c_int)]
+                ("use_error_inflation", C.c_int), ("table_storage", C.c_int)]
 
 
 class _ModelConfig(C.Structure):
@@ -58,7 +58,8 @@ class _ObservationInput(C.Structure):
 class EvalStats(C.Structure):
     _fields_ = [("nb_spectral_points", C.c_size_t), ("nb_levels", C.c_size_t), ("nb_species", C.c_size_t),
                 ("nb_bins", C.c_size_t), ("distinct_table_rows", C.c_size_t), ("table_value_bytes", C.c_size_t),
-                ("kernel_launches", C.c_size_t), ("kernel_ms_total", C.c_double)]
+                ("kernel_launches", C.c_size_t), ("kernel_ms_total", C.c_double),
+                ("table_device_bytes", C.c_size_t)]
 
 
 class BatchStats(C.Structure):
@@ -67,7 +68,7 @@ class BatchStats(C.Structure):
                 ("adjoint_chunk", C.c_size_t), ("point_group", C.c_int)]
 
 
-TABLE_PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+TABLE_PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "int16": 4}
 
 _lib = None
 
@@ -149,14 +150,21 @@ class GlobalConfig:
     use_error_inflation: bool = False
     use_gpu: bool = True
     device: int = 0
+    # device storage of the cross-section tables: "fp64" (every call), or
+    # "fp32" / "fp16" / "bf16" / "int16" only (the batched forward calls at that
+    # precision; single-point likelihoods and models as batches of one)
+    table_storage: str = "fp64"
 
     def _c(self):
         kinds = {"const_wavenumber": 0, "const_wavelength": 1, "const_resolution": 2}
         if self.spectral_disecretisation not in kinds:
             raise BearError(-1, f"unknown spectral discretisation {self.spectral_disecretisation}")
         self._keep = [_b(self.retrieval_folder_path), _b(self.cross_section_file_path), _b(self.wavenumber_file_path)]
+        if self.table_storage not in TABLE_PRECISIONS:
+            raise BearError(-1, f"unknown table storage {self.table_storage}")
         return _GlobalConfig(int(self.use_gpu), self.device, *self._keep, kinds[self.spectral_disecretisation],
-                             float(self.spectral_resolution), int(self.use_error_inflation))
+                             float(self.spectral_resolution), int(self.use_error_inflation),
+                             TABLE_PRECISIONS[self.table_storage])
 
 
 @dataclass
@@ -296,7 +304,8 @@ class Retrieval:
 
     # ---- batches: parameters [batch, nb_parameters]
     def set_table_precision(self, precision="fp64"):
-        """Table storage of the batched path: "fp64", "fp32", "fp16" or "bf16"."""
+        """Table storage of the batched path: "fp64", "fp32", "fp16", "bf16" or
+        "int16" (16-bit integers scaled per species and wavenumber)."""
         if precision not in TABLE_PRECISIONS:
             raise BearError(-1, f"unknown table precision {precision}")
         _check(self._L.bear_retrieval_set_table_precision(self._h, TABLE_PRECISIONS[precision]))

@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void chordReduceKernel(const double* __rest
 // chord_t[point][t][q] = chord[point][q][t] for t < L - 1, q < segmentRows(L)
 // (zero past the last segment), so that step 3 reads a path's chords over
 // consecutive segments with one scalar load
-constexpr int kAdjointSegments = 16;
+constexpr int kAdjointSegments = 32;
 __host__ __device__ constexpr int segmentRows(int nb_levels) {
   return (nb_levels - 1 + kAdjointSegments - 1) / kAdjointSegments * kAdjointSegments;
 }
@@ -788,12 +788,15 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     double s_hi = 0.0;                                               // S-bar of the segment above
     // kAdjointSegments segments per pass over the paths (one load of
     // tau-bar_t and last_t for all of them); chord[q][t] = 0 for t > q and
-    // past the last segment
+    // past the last segment.  Per path, tests uniform over the wavefront:
+    // every lane's path ends below the pass (skipped), or reaches past it
+    // (no masks); otherwise the products are masked lane by lane.
     for (int qb = (L - 2) / kAdjointSegments * kAdjointSegments; qb >= 0; qb -= kAdjointSegments) {
       double sb[kAdjointSegments];
 #pragma unroll
       for (int u = 0; u < kAdjointSegments; ++u) sb[u] = 0.0;
       const int t_end = min(qb + kAdjointSegments, L - 1);
+      const float q_first = float(qb), q_last = float(qb + kAdjointSegments - 1);
       for (int t0 = 0; t0 < t_end; t0 += 4) {          // four paths' values loaded together
         double tb[4];
         float last[4];
@@ -807,9 +810,16 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
         for (int v = 0; v < 4; ++v) {
           const int t = t0 + v;
           if (t >= t_end) break;                       // uniform
+          if (!__any(last[v] >= q_first)) continue;    // uniform: the path ends below every segment of the pass
           const double* ct = c + size_t(t) * rows + qb;
+          if (__all(last[v] >= q_last)) {
 #pragma unroll
-          for (int u = 0; u < kAdjointSegments; ++u) sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[u], sb[u]);
+            for (int u = 0; u < kAdjointSegments; ++u) sb[u] = fma(tb[v], ct[u], sb[u]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < kAdjointSegments; ++u)
+              sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[u], sb[u]);
+          }
         }
       }
 #pragma unroll
@@ -965,12 +975,10 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
                        s, adj_ptr_, adj_bin_, adj_weight_, fscale, nb, n, stride_, adj_fbar_);
     check(hipGetLastError(), "adjoint flux kernel");
     if (transmission) {
-      // wavenumber ranges of the chord products: enough workgroups to fill the device
-      const int blocks = (P + kChordBlock - 1) / kChordBlock;
-      const int pairs = blocks * (blocks + 1) / 2;
-      int ranges = int(std::min<size_t>(kChordRanges, (size_t(nb_cus_) * 32 + size_t(pairs) * cb - 1) /
-                                                          (size_t(pairs) * cb)));
-      ranges = std::max(1, std::min(ranges, (n + 511) / 512));
+      // wavenumber ranges of the chord products, from the grid alone (not
+      // the chunk's points), so that a point's summation order and hence
+      // its gradient do not depend on the batch it came in
+      const int ranges = std::max(1, std::min(kChordRanges, (n + 511) / 512));
       const int per_range = (n + ranges - 1) / ranges;
       const int nu_per_range = (per_range + kChordNu - 1) / kChordNu * kChordNu;
       double* chord_partial = adj_partial_ + size_t(cb) * partial_point;

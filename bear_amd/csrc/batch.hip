@@ -60,6 +60,7 @@ struct TabF32 {
   static __device__ __forceinline__ R raw(const T* p) { return *p; }
   static __device__ __forceinline__ float value(R r) { return r; }
   static __device__ __forceinline__ T store(double v) { return float(v); }
+  static constexpr bool kScaled = false;
 };
 struct TabF16 {
   using T = _Float16;
@@ -67,6 +68,7 @@ struct TabF16 {
   static __device__ __forceinline__ R raw(const T* p) { return *p; }
   static __device__ __forceinline__ float value(R r) { return float(r); }
   static __device__ __forceinline__ T store(double v) { return _Float16(float(v)); }
+  static constexpr bool kScaled = false;
 };
 struct TabBF16 {
   using T = uint16_t;
@@ -76,30 +78,57 @@ struct TabBF16 {
   static __device__ __forceinline__ T store(double v) {   // round to nearest even
     const uint32_t u = __float_as_uint(float(v));
     return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-  }
+  }  static constexpr bool kScaled = false;
+};
+// 16-bit integers q in [-32767, 32767] with value = base + scale q, base and
+// scale per (species, wavenumber): the 16 bits cover each wavenumber's own
+// range uniformly (an absolute error of at most range / 131068), where FP16
+// residuals spend bits on exponents.  The base array then holds per species
+// slot the base row and the scale row.
+struct TabS16 {
+  using T = int16_t;
+  using R = int32_t;
+  static __device__ __forceinline__ R raw(const T* p) { return *p; }
+  static __device__ __forceinline__ float value(R r) { return float(r); }
+  static constexpr bool kScaled = true;
+  static constexpr int kMax = 32767;
 };
 
-// Species s's rows [r0, r1) at wavenumber i: base = FP32 midrange of the
-// values, residual = value - base.  Nothing is floored: log-linear
-// interpolation between a row at the reference's 1e-200 floor and a row far
-// above it yields values in between that can dominate a level's opacity.
+// One species' FP64 rows [0, count) (row-major, `stride` apart) at
+// wavenumber i: base = FP32 midrange of the values, residual = value - base
+// into the species' rows of `low` (same layout).  Nothing is floored:
+// log-linear interpolation between a row at the reference's 1e-200 floor and
+// a row far above it yields values in between that can dominate a level's
+// opacity.
 template <class Tab>
-__global__ __launch_bounds__(kBatchBlock) void splitTableKernel(const double* __restrict__ tables, size_t stride, int n,
-                                                                uint32_t r0, uint32_t r1, int slot,
-                                                                float* __restrict__ base,
+__global__ __launch_bounds__(kBatchBlock) void splitTableKernel(const double* __restrict__ rows, size_t stride, int n,
+                                                                uint32_t count, int slot, float* __restrict__ base,
                                                                 typename Tab::T* __restrict__ low) {
   const int i = blockIdx.x * kBatchBlock + threadIdx.x;
   if (i >= n) return;
   double hi = -INFINITY, lo = INFINITY;
-  for (uint32_t r = r0; r < r1; ++r) {
-    const double v = tables[size_t(r) * stride + i];
+  for (uint32_t r = 0; r < count; ++r) {
+    const double v = rows[size_t(r) * stride + i];
     hi = fmax(hi, v);
     lo = fmin(lo, v);
   }
   const float b = float(0.5 * (hi + lo));
-  base[size_t(slot) * stride + i] = b;
-  for (uint32_t r = r0; r < r1; ++r)
-    low[size_t(r) * stride + i] = Tab::store(tables[size_t(r) * stride + i] - double(b));
+  if constexpr (Tab::kScaled) {
+    // scale: the smallest FP32 value with kMax scale >= the largest residual
+    const double reach = fmax(hi - double(b), double(b) - lo);
+    float sc = reach > 0.0 ? float(reach / Tab::kMax) : 1.0f;
+    if (double(sc) * Tab::kMax < reach) sc = __uint_as_float(__float_as_uint(sc) + 1u);   // next FP32 up
+    base[size_t(2 * slot) * stride + i] = b;
+    base[size_t(2 * slot + 1) * stride + i] = sc;
+    for (uint32_t r = 0; r < count; ++r) {
+      const double q = rint((rows[size_t(r) * stride + i] - double(b)) / double(sc));
+      low[size_t(r) * stride + i] = typename Tab::T(fmin(fmax(q, -double(Tab::kMax)), double(Tab::kMax)));
+    }
+  } else {
+    base[size_t(slot) * stride + i] = b;
+    for (uint32_t r = 0; r < count; ++r)
+      low[size_t(r) * stride + i] = Tab::store(rows[size_t(r) * stride + i] - double(b));
+  }
 }
 
 // ---------------------------------------------------------------- opacity per level
@@ -132,13 +161,36 @@ struct Opacity32 {
   const typename Tab::T* tab;
   size_t stride;
   float bz[NS];
+  float sc[Tab::kScaled ? NS : 1];                 // scales of TabS16
   typename Tab::R v[NS][4];       // as stored; converted where used (FP16 folds into v_fma_mix_f32)
   // slot0: the first stencil slot this thread handles (its base row)
   __device__ __forceinline__ void init(const void* tables, const float* base, size_t s, int i, int slot0) {
     tab = static_cast<const typename Tab::T*>(tables) + i;
     stride = s;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
+    for (int q = 0; q < NS; ++q) {
+      if constexpr (Tab::kScaled) {
+        bz[q] = base[size_t(2 * (slot0 + q)) * s + i] * kLog2TenF;
+        sc[q] = base[size_t(2 * (slot0 + q) + 1) * s + i];
+      } else {
+        bz[q] = base[size_t(slot0 + q) * s + i] * kLog2TenF;
+      }
+    }
+  }
+  // x = bz + log2 n + scale sum_k w_k v_k (scale 1 unless TabS16)
+  __device__ __forceinline__ float exponent(int s, float start, const float* w) const {
+    if constexpr (Tab::kScaled) {
+      float a = w[0] * Tab::value(v[s][0]);
+      a = fmaf(w[1], Tab::value(v[s][1]), a);
+      a = fmaf(w[2], Tab::value(v[s][2]), a);
+      a = fmaf(w[3], Tab::value(v[s][3]), a);
+      return fmaf(sc[s], a, start);
+    } else {
+      float x = fmaf(w[0], Tab::value(v[s][0]), start);
+      x = fmaf(w[1], Tab::value(v[s][1]), x);
+      x = fmaf(w[2], Tab::value(v[s][2]), x);
+      return fmaf(w[3], Tab::value(v[s][3]), x);
+    }
   }
   // grouped stencils: the level's shared weights and contiguous densities
   // (FP32 copies), as kappaTerms does for the FP64 tables
@@ -161,23 +213,12 @@ struct Opacity32 {
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const bool one = (groups.group1 >> s) & 1u;
-        float x = bz[s] + lev.log2_n32[s];
-        x = fmaf(one ? b[0] : a[0], Tab::value(v[s][0]), x);
-        x = fmaf(one ? b[1] : a[1], Tab::value(v[s][1]), x);
-        x = fmaf(one ? b[2] : a[2], Tab::value(v[s][2]), x);
-        x = fmaf(one ? b[3] : a[3], Tab::value(v[s][3]), x);
-        k += __builtin_amdgcn_exp2f(x);
+        const float w[4] = {one ? b[0] : a[0], one ? b[1] : a[1], one ? b[2] : a[2], one ? b[3] : a[3]};
+        k += __builtin_amdgcn_exp2f(exponent(s, bz[s] + lev.log2_n32[s], w));
       }
     } else {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        float x = bz[s] + d[s].log2_n;
-        x = fmaf(d[s].w[0], Tab::value(v[s][0]), x);
-        x = fmaf(d[s].w[1], Tab::value(v[s][1]), x);
-        x = fmaf(d[s].w[2], Tab::value(v[s][2]), x);
-        x = fmaf(d[s].w[3], Tab::value(v[s][3]), x);
-        k += __builtin_amdgcn_exp2f(x);
-      }
+      for (int s = 0; s < NS; ++s) k += __builtin_amdgcn_exp2f(exponent(s, bz[s] + d[s].log2_n, d[s].w));
     }
     return k;
   }
@@ -457,6 +498,10 @@ void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, cons
       hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, tab_low, base, stride,
                          rec, L, n, tiles, batch, flux, keep);
       break;
+    case TablePrecision::int16:
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, 0, s, nu, tab_low, base, stride,
+                         rec, L, n, tiles, batch, flux, keep);
+      break;
   }
 }
 
@@ -487,6 +532,10 @@ void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, 
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
                          stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
       break;
+    case TablePrecision::int16:
+      hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, lds, s, tab_low, base,
+                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+      break;
   }
 }
 
@@ -500,9 +549,56 @@ size_t transmissionLds(int L, int tile, bool wide) {
 }  // namespace
 
 // ---------------------------------------------------------------- precision
+// residual tables [rows][stride] and the base [slots][stride], zero base in
+// the padding slots (their density is zero)
+void DeviceEngine::allocateReduced(TablePrecision p, size_t nb_rows, size_t slots) {
+  void* low = nullptr;
+  check(hipMalloc(&low, std::max<size_t>(1, nb_rows) * stride_ * tableElementBytes(p)), "reduced-precision tables");
+  tables_low_ = low;
+  base_rows_ = slots * (p == TablePrecision::int16 ? 2 : 1);          // int16: base and scale per slot
+  table_base_ = deviceAlloc<float>(base_rows_ * stride_, "table base");
+  check(hipMemsetAsync(table_base_, 0, base_rows_ * stride_ * sizeof(float), static_cast<hipStream_t>(stream_)),
+        "clear table base");
+}
+
+// species `slot`'s FP64 rows (device, [count][stride]) split into the base and
+// the residual rows [first, first + count)
+void DeviceEngine::splitSpecies(const double* rows, size_t count, int slot, size_t first) {
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  const int n = int(n_points_);
+  const dim3 grid((n + kBatchBlock - 1) / kBatchBlock), block(kBatchBlock);
+  const uint32_t c = uint32_t(count);
+  switch (precision_) {
+    case TablePrecision::fp32:
+      hipLaunchKernelGGL(splitTableKernel<TabF32>, grid, block, 0, s, rows, stride_, n, c, slot, table_base_,
+                         static_cast<float*>(tables_low_) + first * stride_);
+      break;
+    case TablePrecision::fp16:
+      hipLaunchKernelGGL(splitTableKernel<TabF16>, grid, block, 0, s, rows, stride_, n, c, slot, table_base_,
+                         static_cast<_Float16*>(tables_low_) + first * stride_);
+      break;
+    case TablePrecision::bf16:
+      hipLaunchKernelGGL(splitTableKernel<TabBF16>, grid, block, 0, s, rows, stride_, n, c, slot, table_base_,
+                         static_cast<uint16_t*>(tables_low_) + first * stride_);
+      break;
+    case TablePrecision::int16:
+      hipLaunchKernelGGL(splitTableKernel<TabS16>, grid, block, 0, s, rows, stride_, n, c, slot, table_base_,
+                         static_cast<int16_t*>(tables_low_) + first * stride_);
+      break;
+    default:
+      raise(kInternal, "no split for FP64 tables");
+  }
+  check(hipGetLastError(), "table split kernel");
+}
+
 void DeviceEngine::setTablePrecision(TablePrecision p,
                                      const std::vector<std::pair<uint32_t, uint32_t>>& species_rows) {
   check(hipSetDevice(device_), "hipSetDevice");
+  if (storage_ != TablePrecision::fp64) {
+    if (p != storage_)
+      raise(kInvalidInput, "the tables were loaded at reduced precision only; no other precision is available");
+    return;
+  }
   releaseGraph();
   deviceFree(tables_low_);
   deviceFree(table_base_);
@@ -513,35 +609,13 @@ void DeviceEngine::setTablePrecision(TablePrecision p,
   tuned_.clear();
   if (p == TablePrecision::fp64) return;
   if (species_rows.size() > species_stride_) raise(kInvalidInput, "more species than stencil slots");
-  const size_t elem = p == TablePrecision::fp32 ? 4 : 2;
-  void* low = nullptr;
-  check(hipMalloc(&low, std::max<size_t>(1, n_rows_) * stride_ * elem), "reduced-precision tables");
-  tables_low_ = low;
-  table_base_ = deviceAlloc<float>(species_stride_ * stride_, "table base");
-  hipStream_t s = static_cast<hipStream_t>(stream_);
-  // padding slots keep a zero base (their density is zero)
-  check(hipMemsetAsync(table_base_, 0, species_stride_ * stride_ * sizeof(float), s), "clear table base");
-  const int n = int(n_points_);
-  const dim3 grid((n + kBatchBlock - 1) / kBatchBlock), block(kBatchBlock);
+  allocateReduced(p, n_rows_, species_stride_);
   for (size_t q = 0; q < species_rows.size(); ++q) {
     const uint32_t r0 = species_rows[q].first, r1 = species_rows[q].second;
     if (r1 <= r0 || r1 > n_rows_) raise(kInternal, "species rows out of range");
-    switch (p) {
-      case TablePrecision::fp32:
-        hipLaunchKernelGGL(splitTableKernel<TabF32>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
-                           table_base_, static_cast<float*>(low));
-        break;
-      case TablePrecision::fp16:
-        hipLaunchKernelGGL(splitTableKernel<TabF16>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
-                           table_base_, static_cast<_Float16*>(low));
-        break;
-      default:
-        hipLaunchKernelGGL(splitTableKernel<TabBF16>, grid, block, 0, s, tables_, stride_, n, r0, r1, int(q),
-                           table_base_, static_cast<uint16_t*>(low));
-    }
-    check(hipGetLastError(), "table split kernel");
+    splitSpecies(tables_ + size_t(r0) * stride_, r1 - r0, int(q), r0);
   }
-  check(hipStreamSynchronize(s), "table split");
+  check(hipStreamSynchronize(static_cast<hipStream_t>(stream_)), "table split");
 }
 
 // ---------------------------------------------------------------- staging

@@ -51,6 +51,9 @@ bear_amd::GlobalSettings settings(const bear_global_config* config) {
   g.discretisation = config->spectral_discretisation;
   g.resolution = config->spectral_resolution;
   g.error_inflation = config->use_error_inflation != 0;
+  if (config->table_storage < BEAR_TABLE_FP64 || config->table_storage > BEAR_TABLE_INT16)
+    bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "unknown table storage " + std::to_string(config->table_storage));
+  g.table_storage = static_cast<bear_amd::TablePrecision>(config->table_storage);
   return g;
 }
 
@@ -233,7 +236,7 @@ int bear_forward_model_calc_model_gpu_noalloc(bear_retrieval* r, const double* p
 int bear_retrieval_set_table_precision(bear_retrieval* r, int precision) {
   return guarded([&] {
     if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
-    if (precision < BEAR_TABLE_FP64 || precision > BEAR_TABLE_BF16)
+    if (precision < BEAR_TABLE_FP64 || precision > BEAR_TABLE_INT16)
       bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "unknown table precision " + std::to_string(precision));
     r->impl->setTablePrecision(static_cast<bear_amd::TablePrecision>(precision));
   });
@@ -393,6 +396,7 @@ int bear_retrieval_stats(const bear_retrieval* r, bear_eval_stats* out) {
     out->table_value_bytes = sizeof(double);
     out->kernel_launches = e.timedLaunches();
     out->kernel_ms_total = e.timedMilliseconds();
+    out->table_device_bytes = e.tableBytes();
   });
 }
 

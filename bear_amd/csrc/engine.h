@@ -114,7 +114,13 @@ struct alignas(16) LevelSpecies32 {
 // base, the midrange of that species' rows there, and per row the residual
 // log10 sigma - base in the named type; interpolation and exponentials in
 // FP32, radiative transfer, bins and likelihood in FP64.
-enum class TablePrecision : int { fp64 = 0, fp32 = 1, fp16 = 2, bf16 = 3 };
+// fp32 / fp16 / bf16: residuals against a per-(species, wavenumber) FP32
+// base; int16: 16-bit integers scaled per (species, wavenumber), value =
+// base + scale q
+enum class TablePrecision : int { fp64 = 0, fp32 = 1, fp16 = 2, bf16 = 3, int16 = 4 };
+inline size_t tableElementBytes(TablePrecision p) {
+  return p == TablePrecision::fp64 ? 8 : p == TablePrecision::fp32 ? 4 : 2;
+}
 
 enum class ModelKind : int { emission = 0, eclipse = 1, transmission = 2 };
 
@@ -166,9 +172,19 @@ class DeviceEngine {
   DeviceEngine& operator=(const DeviceEngine&) = delete;
 
   void uploadGrid(const std::vector<double>& wavenumber, const std::vector<double>& wavelength);
-  // device tables [rows][stride] FP64; rows [first, first + count) from host [count][N]
-  void allocateTables(size_t nb_rows);
-  void uploadTableRows(size_t first, size_t count, const double* rows);
+  // Storage of the cross-section tables, set before they are loaded: FP64
+  // (every path; reduced precisions split from it on the device by
+  // setTablePrecision), or FP32/FP16/BF16 only (the batched forward kernels
+  // at that precision; no FP64 rows are kept, so the single-point and
+  // adjoint kernels are unavailable).
+  void setTableStorage(TablePrecision p);
+  TablePrecision tableStorage() const { return storage_; }
+  // device tables [rows][stride]; species `slot`'s rows [first, first + count)
+  // from host FP64 [count][N], split into base and residuals on the device
+  // when the storage is reduced
+  void allocateTables(size_t nb_rows, size_t nb_species);
+  void uploadSpeciesRows(int slot, size_t first, size_t count, const double* rows);
+  size_t tableBytes() const;                     // device bytes of the resident tables
   void uploadRayleigh(const std::vector<double>& cross_section);   // [N] cm^2, empty: no scatterer
   void uploadObservations(const std::vector<ObservationSetup>& observations);
   void configure(size_t nb_levels, size_t nb_species, bool transmission);
@@ -350,6 +366,11 @@ class DeviceEngine {
   int nb_cus_ = 0;
   size_t lds_limit_ = 64 * 1024;                 // LDS bytes per workgroup (device attribute)
   TablePrecision precision_ = TablePrecision::fp64;
+  TablePrecision storage_ = TablePrecision::fp64;
+  size_t base_rows_ = 0;                         // rows of table_base_
+  void allocateReduced(TablePrecision p, size_t nb_rows, size_t slots);
+  void splitSpecies(const double* rows, size_t count, int slot, size_t first);
+  void requireFP64Tables(const char* what) const;
   void* tables_low_ = nullptr;                   // residuals [rows][stride] (fp32 / fp16 / bf16)
   float* table_base_ = nullptr;                  // [species stride][stride]
   bool use_graph_ = true, autotune_ = true;

@@ -432,24 +432,60 @@ void DeviceEngine::uploadGrid(const std::vector<double>& nu, const std::vector<d
   check(hipStreamSynchronize(s), "copy grid");
 }
 
-void DeviceEngine::allocateTables(size_t nb_rows) {
+void DeviceEngine::requireFP64Tables(const char* what) const {
+  if (storage_ != TablePrecision::fp64)
+    raise(kInvalidInput, std::string(what) +
+                             " needs the FP64 tables; these were loaded at reduced precision only (use the batched "
+                             "forward calls)");
+}
+
+void DeviceEngine::setTableStorage(TablePrecision p) {
+  if (tables_ || tables_low_) raise(kInternal, "table storage set after the tables were loaded");
+  storage_ = p;
+  precision_ = p;
+}
+
+void DeviceEngine::allocateTables(size_t nb_rows, size_t nb_species) {
   check(hipSetDevice(device_), "hipSetDevice");
   if (nb_rows * (stride_ / 64) > size_t(UINT32_MAX))   // stencils hold 32-bit block offsets
     raise(kInvalidInput, "tables too large: rows x wavenumbers exceeds 2^38 values");
   deviceFree(tables_);
   tables_ = nullptr;
-  tables_ = deviceAlloc<double>(nb_rows * stride_, "cross-section tables");
   n_rows_ = nb_rows;
+  if (storage_ == TablePrecision::fp64) {
+    tables_ = deviceAlloc<double>(nb_rows * stride_, "cross-section tables");
+    return;
+  }
+  // residual rows and the per-species base only; every species passes
+  // through an FP64 staging buffer of its own rows (uploadSpeciesRows)
+  const size_t slots = std::max<size_t>(kChunk, (nb_species + kChunk - 1) / kChunk * kChunk);
+  allocateReduced(storage_, nb_rows, slots);
 }
 
-void DeviceEngine::uploadTableRows(size_t first, size_t count, const double* rows) {
+void DeviceEngine::uploadSpeciesRows(int slot, size_t first, size_t count, const double* rows) {
   if (first + count > n_rows_) raise(kInternal, "table rows beyond the allocated store");
   if (count == 0) return;
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  check(hipMemcpy2DAsync(tables_ + first * stride_, stride_ * sizeof(double), rows, n_points_ * sizeof(double),
-                         n_points_ * sizeof(double), count, hipMemcpyHostToDevice, s),
-        "copy cross-section tables");
-  check(hipStreamSynchronize(s), "copy cross-section tables");
+  double* dst = tables_;
+  if (storage_ == TablePrecision::fp64) dst += first * stride_;
+  else dst = deviceAlloc<double>(count * stride_, "table staging");
+  try {
+    check(hipMemcpy2DAsync(dst, stride_ * sizeof(double), rows, n_points_ * sizeof(double), n_points_ * sizeof(double),
+                           count, hipMemcpyHostToDevice, s),
+          "copy cross-section tables");
+    if (storage_ != TablePrecision::fp64) splitSpecies(dst, count, slot, first);
+    check(hipStreamSynchronize(s), "copy cross-section tables");
+  } catch (...) {
+    if (storage_ != TablePrecision::fp64) deviceFree(dst);
+    throw;
+  }
+  if (storage_ != TablePrecision::fp64) deviceFree(dst);
+}
+
+size_t DeviceEngine::tableBytes() const {
+  size_t bytes = tables_ ? n_rows_ * stride_ * sizeof(double) : 0;
+  if (tables_low_) bytes += n_rows_ * stride_ * tableElementBytes(precision_) + base_rows_ * stride_ * sizeof(float);
+  return bytes;
 }
 
 void DeviceEngine::uploadRayleigh(const std::vector<double>& sigma) {
@@ -640,6 +676,7 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
 
 void DeviceEngine::evaluateEmission(double scaling, double inflation, double* spectrum_dev, double* const* bins_dev) {
   if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
+  requireFP64Tables("the single-model path");
   upload();
   const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
@@ -657,6 +694,7 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
 void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio, double inflation,
                                    double* spectrum_dev, double* const* bins_dev) {
   if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
+  requireFP64Tables("the single-model path");
   upload();
   const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
@@ -675,6 +713,7 @@ void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio,
 void DeviceEngine::evaluateEclipseTabulatedStar(double radius_ratio, double* spectrum_dev) {
   if (transmission_) raise(kInvalidInput, "the engine is configured for transmission");
   if (!star_) raise(kInvalidInput, "no stellar spectrum on the grid");
+  requireFP64Tables("the single-model path");
   upload();
   const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);
@@ -711,6 +750,7 @@ void DeviceEngine::copyFlux(double* host) {
 void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius, double inflation,
                                         double* spectrum_dev, double* const* bins_dev) {
   if (!transmission_) raise(kInvalidInput, "the engine is not configured for transmission");
+  requireFP64Tables("the single-model path");
   upload();
   const EvalBuffers buf = buffers();
   hipStream_t s = static_cast<hipStream_t>(stream_);

@@ -35,12 +35,13 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
     nb_rows += uint32_t(sources.back().nb_rows);
     species_.push_back(std::move(s));
   }
-  // one species at a time: host memory holds one species' FP64 rows
-  engine_.allocateTables(nb_rows);
+  // one species at a time: host memory holds one species' FP64 rows (and,
+  // with reduced-precision storage, the device one species' FP64 staging)
+  engine_.allocateTables(nb_rows, sources.size());
   std::vector<double> rows;
-  for (const TableSource& src : sources) {
-    sampleSpeciesTable(src, grid, rows);
-    engine_.uploadTableRows(src.first_row, src.nb_rows, rows.data());
+  for (size_t q = 0; q < sources.size(); ++q) {
+    sampleSpeciesTable(sources[q], grid, rows);
+    engine_.uploadSpeciesRows(int(q), sources[q].first_row, sources[q].nb_rows, rows.data());
   }
   rows.clear();
   rows.shrink_to_fit();
@@ -590,6 +591,7 @@ Retrieval::Retrieval(const GlobalSettings& global, const std::string& model_type
     o.foldOperator(grid_, !transmission);                         // transmission bins are not fluxes
   }
   engine_.reset(new DeviceEngine(global_.device));
+  engine_->setTableStorage(global_.table_storage);
   engine_->uploadGrid(grid_.wavenumber, grid_.wavelength);
   engine_->uploadObservations(observations_);
   if (emission) model_.reset(new EmissionModel(global_, model, grid_, observations_, *engine_));
@@ -609,6 +611,11 @@ double Retrieval::errorInflation(const double* p, size_t n) const {
 // Retrieval::logLikelihoodGPU (retrieval.cpp:374-420)
 double Retrieval::computeLikelihood(const double* p, size_t n) {
   checkSize(n);
+  if (engine_->tableStorage() != TablePrecision::fp64) {   // reduced-precision tables: a batch of one
+    double ll = 0;
+    computeLikelihoodBatch(p, 1, n, &ll);
+    return ll;
+  }
   const bool neglect = model_->calcModelGPU(p, errorInflation(p, n), nullptr, nullptr);
   const double ll = engine_->logLikelihood();
   return neglect ? -1e30 : ll;
@@ -713,6 +720,11 @@ void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, doub
 
 bool Retrieval::computeModel(const double* p, size_t n, bool high_res, double* spectrum, double* spectrum_obs) {
   checkSize(n);
+  if (engine_->tableStorage() != TablePrecision::fp64) {   // reduced-precision tables: a batch of one
+    int neglect = 0;
+    computeModelBatch(p, 1, n, high_res ? spectrum : nullptr, spectrum_obs, &neglect);
+    return neglect != 0;
+  }
   double* spec_dev = high_res ? engine_->spectrumBuffer() : nullptr;
   const bool neglect = model_->calcModelGPU(p, errorInflation(p, n), spec_dev, nullptr);
   std::vector<double> bins;
@@ -743,6 +755,8 @@ DirectModel::DirectModel(const GlobalSettings& global, const std::string& model_
   const bool transmission = model_type == "transmission";
   if (!emission && !eclipse && !transmission) raise(kInvalidInput, "forward model " + model_type + " is not available");
   if (global_.discretisation < 0 || global_.discretisation > 2) raise(kInvalidInput, "unknown spectral discretisation");
+  if (global_.table_storage != TablePrecision::fp64)
+    raise(kInvalidInput, "the direct models keep FP64 tables (table_storage fp64)");
   grid_.discretisation = global_.discretisation;
   grid_.resolution = global_.resolution;
   grid_.loadFullList(global_.wavenumber_file);

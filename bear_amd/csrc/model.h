@@ -18,6 +18,7 @@ struct GlobalSettings {                 // GlobalConfig (config/global_config.h:
   int discretisation = 0;
   double resolution = 0;
   bool error_inflation = false;
+  TablePrecision table_storage = TablePrecision::fp64;   // see DeviceEngine::setTableStorage
 };
 
 // EmissionModelConfig (emission.h:51-102), TransmissionModelConfig

@@ -787,7 +787,7 @@ def _model_config(meta):
                            use_variable_gravity=variant == "variable_gravity")
 
 
-def open_retrieval(folder, device=0):
+def open_retrieval(folder, device=0, table_storage="fp64"):
     """Engine Retrieval over a case folder written by write_case (files only)."""
     from . import api
     folder = os.path.join(folder, "")
@@ -796,7 +796,8 @@ def open_retrieval(folder, device=0):
                            wavenumber_file_path=folder + "wavenumber_full.dat",
                            spectral_disecretisation=DISCRETISATIONS[meta.get("discretisation", 0)],
                            spectral_resolution=meta["spectral_resolution"],
-                           use_error_inflation=meta["use_error_inflation"], device=device)
+                           use_error_inflation=meta["use_error_inflation"], device=device,
+                           table_storage=table_storage)
     return api.Retrieval(cfg, _model_config(meta), observation_files=meta["observations"],
                          observation_modifiers=meta["modifiers"], forward_model_type=meta["model"])
 

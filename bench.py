@@ -14,9 +14,9 @@ stream) and a CPU baseline timed on this host's cores.
 """
 
 import argparse
+import hashlib
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -29,7 +29,12 @@ sys.path.insert(0, ROOT)
 METRIC = "likelihood evals/sec (batched+adjoint vs plain GPU) at ≤10 ppm spectral error"
 HBM_PEAK_GBS = 8000.0
 FULL_N = 400_000
-SAMPLE_N = 20_000
+CPU_THREADS = 16            # the GPU box's CPU share per GPU
+CPU_EVALS = 20
+SPECTRUM_BAR = 1e-5         # relative, emission spectra and bins
+LOGLIKE_BAR = 1e-6          # absolute
+# the sources of the single-point kernel: a counter profile is valid for these only
+KERNEL_SOURCES = ("bear_amd/csrc/engine.hip", "bear_amd/csrc/device_math.h", "bear_amd/csrc/engine.h")
 
 
 def log(msg):
@@ -70,35 +75,104 @@ def isothermal_check(r, params):
     return float(np.max(np.abs(out.spectrum - expected) / expected))
 
 
+def kernel_source_digest():
+    """sha256 over the single-point kernel's source files (their contents)."""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profiles(kernel="emissionKernelCached"):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch,
-    scaled by the copy-kernel calibration recorded with them)."""
+    scaled by the copy-kernel calibration recorded with them), provided the
+    summary was taken on these kernel sources; (bytes or None, note)."""
     path = os.path.join(ROOT, "profiles", "emission_c2_counters.json")
     if not os.path.exists(path):
-        return None
+        return None, "no counter profile"
+    d = json.load(open(path))
+    digest = kernel_source_digest()
+    if d.get("kernel_source_digest") != digest:
+        return None, (f"counter profile taken on kernel sources {d.get('kernel_source_digest')}, "
+                      f"these are {digest}: not attached")
+    for name, v in d.items():
+        if name.startswith(kernel):
+            return float(v["hbm_bytes_per_launch"]), f"counters of kernel sources {digest}"
+    return None, "kernel not in the counter profile"
+
+
+def _counter_rows(folder):
+    """(kernel name without its arguments, counter, value, dispatch id) of
+    every rocprofv3 counter_collection.csv under folder."""
+    import csv
+    import glob
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                name = row["Kernel_Name"].split("(")[0].strip()
+                if name.startswith("void "):
+                    name = name[5:]
+                yield name, row["Counter_Name"], float(row["Counter_Value"]), row.get("Dispatch_Id", "")
+
+
+def _per_dispatch(folder, counter):
+    """kernel -> (dispatches, mean counter value per dispatch)."""
+    acc = {}
+    for name, c, v, d in _counter_rows(folder):
+        if c != counter:
+            continue
+        e = acc.setdefault(name, {})
+        e[d] = e.get(d, 0.0) + v                 # per-XCD / per-channel rows of one dispatch summed
+    return {k: (len(v), sum(v.values()) / len(v)) for k, v in acc.items()}
+
+
+def summarize_counters(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes):
+    """The counter summary bench.py attaches as roofline.traffic: per kernel
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) from two rocprofv3 --pmc
+    runs of the bench, scaled by what the same counters read for a plain
+    device-to-device copy of copy_bytes, with the kernel sources' digest."""
+    import subprocess
+    copy_fetch = _per_dispatch(copy_fetch_dir, "FETCH_SIZE")
+    copy_write = _per_dispatch(copy_write_dir, "WRITE_SIZE")
+    ck = [k for k in copy_fetch if "copyBuffer" in k or "copy" in k.lower()]
+    if not ck:
+        raise SystemExit(f"no copy kernel in {copy_fetch_dir}: {sorted(copy_fetch)}")
+    f_kib, w_kib = copy_fetch[ck[0]][1], copy_write[ck[0]][1]
+    fetch_factor, write_factor = copy_bytes / (f_kib * 1024), copy_bytes / (w_kib * 1024)
     try:
-        d = json.load(open(path))
-        for name, v in d.items():
-            if name.startswith(kernel):
-                return float(v["hbm_bytes_per_launch"])
-    except Exception:
-        pass
-    return None
+        commit = subprocess.run(["git", "rev-parse", "HEAD"], cwd=ROOT, capture_output=True, text=True).stdout.strip()
+    except OSError:
+        commit = "unknown"
+    out = {"kernel_source_digest": kernel_source_digest(), "commit_before_profile": commit,
+           "calibration": {"copy_kernel": ck[0], "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
+                           "FETCH_SIZE_KiB": f_kib, "WRITE_SIZE_KiB": w_kib,
+                           "fetch_factor": fetch_factor, "write_factor": write_factor}}
+    fetch, write = _per_dispatch(fetch_dir, "FETCH_SIZE"), _per_dispatch(write_dir, "WRITE_SIZE")
+    for k in sorted(fetch, key=lambda k: -fetch[k][1]):
+        if k not in write:
+            continue
+        rd, wr = fetch[k][1] * 1024 * fetch_factor, write[k][1] * 1024 * write_factor
+        out[k] = {"dispatches": fetch[k][0], "FETCH_SIZE_KiB": fetch[k][1], "WRITE_SIZE_KiB": write[k][1],
+                  "hbm_bytes_read": rd, "hbm_bytes_written": wr, "hbm_bytes_per_launch": rd + wr}
+    return out
 
 
 def batched_leg(r, points, batch_steps, plain_rate):
     """Batches of live points per launch sequence at each table precision:
     throughput, speedup over the single-point path, and the error against the
-    FP64 single-point path (pinned to the oracle by the parity tests) on the
-    first four points: high-resolution spectrum and bins (relative),
-    log-likelihood (absolute)."""
+    FP64 single-point path (equal to the oracle within 5e-11 relative at this
+    size, tests/test_gpu_full_size.py) on the first four points:
+    high-resolution spectrum and bins (relative), log-likelihood (absolute),
+    with whether each leg meets the spectral bar (1e-5 relative) and the
+    log-likelihood bar (1e-6 absolute)."""
     import torch
     pts = np.array(points)
     ref = [r.computeModel(p, True) for p in pts[:4]]
     ref_ll = [r.computeLikelihood(p) for p in pts[:4]]
     out = {"batch": len(pts)}
-    for precision in ("fp64", "fp32", "fp16", "bf16"):
+    for precision in ("fp64", "fp32", "fp16", "bf16", "int16"):
         r.set_table_precision(precision)
         r.computeLikelihoodBatch(pts)                   # tuning + graph capture
         r.computeLikelihoodBatch(pts)
@@ -121,6 +195,8 @@ def batched_leg(r, points, batch_steps, plain_rate):
                           "gpu_ms_per_batch": round(st["gpu_ms_total"] / batch_steps, 3),
                           "speedup_vs_single": round(rate / plain_rate, 2),
                           "max_spectrum_rel_err": err_spec, "max_bin_rel_err": err_bins, "max_abs_dloglike": err_ll,
+                          "meets_spectral_bar": bool(err_spec <= SPECTRUM_BAR and err_bins <= SPECTRUM_BAR),
+                          "meets_loglike_bar": bool(err_ll <= LOGLIKE_BAR),
                           "launch": {k: st[k] for k in ("block", "tile", "grid", "used_graph")}}
         log(f"batch {precision}: {rate:.1f} evals/s, spectrum err {err_spec:.2e}, |dlogL| {err_ll:.2e}")
     r.set_table_precision("fp64")
@@ -156,57 +232,84 @@ def batched_leg(r, points, batch_steps, plain_rate):
     return out
 
 
-def cpu_baseline(threads, scratch):
-    """The reference's own CPU path (oracle/_ref build) on a bounded sample:
-    the same configuration at 20,000 wavenumbers, scaled to 400,000 by the
-    per-wavenumber cost; the oracle restatement stands in when the reference
-    build is absent."""
-    from bear_amd import synth
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed_reference(folder, scratch, threads, n_evals):
+    """Median seconds of one evaluation on the reference build (the oracle
+    restatement where the build is absent): table rows preloaded by an
+    untimed first evaluation; (seconds, kind)."""
     from oracle import api as oracle_api
-    case = build_case(SAMPLE_N, "numpy")
-    folder = synth.write_case(case, os.path.join(scratch, "cpu_sample"))
-    truth = case.parameters.copy()
-    o = oracle_api.EmissionOracle(folder)
-    synth.inject_data(case, o.evaluate(truth)["bins"])
-    folder = synth.write_case(case, folder)
-    params = synth.perturbed(truth, case, seed=5)
-    with open(os.path.join(folder, "case.txt")) as f:
-        lines = [l for l in f.read().splitlines() if not l.startswith("parameters ")]
-    lines.append(f"parameters {params.size} " + " ".join(repr(float(v)) for v in params))
-    with open(os.path.join(folder, "case.txt"), "w") as f:
-        f.write("\n".join(lines) + "\n")
-    n_evals = 12
     if os.path.exists(oracle_api.REF_BIN):
-        out = oracle_api.run_reference(folder, os.path.join(scratch, "cpu_out"), n_timed=n_evals, threads=threads)
-        t = float(np.median(out["timings"]))
-        kind = "reference"
-    else:
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        o = oracle_api.EmissionOracle(folder)
+        out = oracle_api.run_reference(folder, os.path.join(scratch, f"cpu_out_{threads}"), n_timed=n_evals,
+                                       threads=threads)
+        return float(np.median(out["timings"])), "reference"
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = oracle_api.EmissionOracle(folder)
+    params = np.array(json.load(open(os.path.join(folder, "case.json")))["parameters"])
+    o.evaluate(params)
+    ts = []
+    for _ in range(n_evals):
+        t0 = time.perf_counter()
         o.evaluate(params)
-        ts = []
-        for _ in range(n_evals):
-            t0 = time.perf_counter()
-            o.evaluate(params)
-            ts.append(time.perf_counter() - t0)
-        t = float(np.median(ts))
-        kind = "port"
-    n_sample = o.nb_points if kind == "port" else oracle_api.EmissionOracle(folder).nb_points
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), "port"
+
+
+def write_cpu_case(case, params, folder):
+    """The case in the reference's file formats with `params` as its
+    parameter vector (the reference build evaluates case.txt's)."""
+    from bear_amd import synth
+    keep = case.parameters
+    case.parameters = np.asarray(params, dtype=np.float64)
+    try:
+        return synth.write_case(case, folder)
+    finally:
+        case.parameters = keep
+
+
+def cpu_baseline(case, params, scratch):
+    """The reference's own CPU path (oracle/_ref build, compiled here from
+    the reference's sources) on this workload in full: every wavenumber, the
+    same tables, observations and live point as the GPU line, CPU_THREADS
+    threads, the median of CPU_EVALS evaluations (no scaling).  Beside it the
+    plain CPU configuration (emission_c1: 60 levels, 2 absorbers, 20,001
+    wavenumbers, 100 bins) at CPU_THREADS threads and at one thread."""
+    from bear_amd import synth
+    folder = write_cpu_case(case, params, os.path.join(scratch, "cpu_c2"))
+    log(f"CPU baseline: full workload on {CPU_THREADS} threads, {CPU_EVALS} evaluations")
+    t, kind = timed_reference(folder, scratch, CPU_THREADS, CPU_EVALS)
+    c1 = synth.emission_c1()
+    c1_folder = synth.write_case(c1, os.path.join(scratch, "cpu_c1"))
+    t1_many, _ = timed_reference(c1_folder, scratch, CPU_THREADS, CPU_EVALS)
+    t1_one, _ = timed_reference(c1_folder, scratch, 1, CPU_EVALS)
+    log(f"CPU baseline: {t * 1e3:.1f} ms per evaluation ({kind}); emission_c1 {t1_many * 1e3:.2f} ms on "
+        f"{CPU_THREADS} threads, {t1_one * 1e3:.2f} ms on one")
     return {
-        "value": (1.0 / t) * n_sample / FULL_N,
+        "value": round(1.0 / t, 4),
         "unit": "evals/s",
-        "cores": threads,
+        "cores": CPU_THREADS,
         "kind": kind,
-        "sample": f"same emission configuration at {n_sample} wavenumbers (1000-1500 cm^-1), median of "
-                  f"{n_evals} evaluations = {t * 1e3:.1f} ms, scaled by {n_sample}/{FULL_N} wavenumbers",
+        "sample": f"the whole workload (all {FULL_N} wavenumbers, the same tables, observations and live point), "
+                  f"median of {CPU_EVALS} evaluations = {t * 1e3:.1f} ms, {CPU_THREADS} threads "
+                  f"(the box's CPU share per GPU; nproc {os.cpu_count()}), {cpu_model()}",
+        "emission_c1": {"evals_per_s_threads": round(1.0 / t1_many, 2), "evals_per_s_one_thread": round(1.0 / t1_one, 2),
+                        "threads": CPU_THREADS, "evaluations": CPU_EVALS},
     }
 
 
 CONFIG_WORKLOADS = {
     # name: (builder, live points, table precisions of the batched legs, what a result is)
-    "transmission_c3": ("transmission_c3", 256, ("fp64", "fp32", "fp16"), "ppm of transit depth"),
+    "transmission_c3": ("transmission_c3", 256, ("fp64", "fp32", "fp16", "int16"), "ppm of transit depth"),
     "emission_c4": ("emission_c4", 64, ("fp64",), "W m^-2 micron^-1"),
-    "eclipse_c5": ("eclipse_c5", 512, ("fp64", "fp16", "bf16"), "ppm of eclipse depth"),
+    "eclipse_c5": ("eclipse_c5", 512, ("fp64", "fp16", "bf16", "int16"), "ppm of eclipse depth"),
 }
 
 
@@ -312,11 +415,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128, help="live points per batch in the batched leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="emission_c2", choices=["emission_c2"] + sorted(CONFIG_WORKLOADS),
                     help="emission_c2: the headline line; the others print their own line (batch size "
                          "from --batch, default theirs)")
+    ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="write the counter summary (JSON, stdout) from rocprofv3 --pmc output folders; no GPU")
+    ap.add_argument("--copy-bytes", type=int, default=2 ** 31, help="bytes of the calibration copy")
     args = ap.parse_args()
+    if args.summarize_counters:
+        print(json.dumps(summarize_counters(*args.summarize_counters, args.copy_bytes), indent=1))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the likelihood path on a single GPU "
               "(the multi-GPU sharding is a design, not a run); use --gpus 1", file=sys.stderr)
@@ -339,7 +447,6 @@ def main():
         r, params = open_with_injected_data(case, os.path.join(scratch, "gpu"))
         from bear_amd import synth
         points = [params] + [synth.perturbed(params, case, seed=1000 + k) for k in range(max(args.batch, 1) - 1)]
-        del case
         iso_err = isothermal_check(r, params)
         log(f"isothermal identity at full size: max rel err {iso_err:.2e}")
 
@@ -363,23 +470,25 @@ def main():
         algorithmic = st["distinct_table_rows"] * n * st["table_value_bytes"] + n * 8 + n * 8 \
             + st["nb_levels"] * (32 + st["nb_species"] * 48)
         achieved = algorithmic / kernel_s / 1e9
+        traffic, traffic_note = traffic_from_profiles()
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
                     "kernel": "emissionKernelCached<8>", "kernel_ms": round(kernel_s * 1e3, 4),
-                    "algorithmic_bytes": int(algorithmic), "distinct_table_rows": st["distinct_table_rows"]}
+                    "algorithmic_bytes": int(algorithmic), "distinct_table_rows": st["distinct_table_rows"],
+                    "kernel_source_digest": kernel_source_digest()}
 
         batched = None
         if args.batch > 0:
             batched = batched_leg(r, points[:args.batch], args.batch_steps, args.steps / elapsed)
 
+        r.close()
         cpu = None
         if not args.no_cpu_baseline:
-            threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-            log(f"CPU baseline on {threads} threads")
             try:
-                cpu = cpu_baseline(threads, scratch)
+                cpu = cpu_baseline(case, params, scratch)
             except Exception as e:  # the GPU figure stands without it
                 log(f"CPU baseline failed: {e}")
+        del case
 
     value = args.steps / elapsed
     out = {

@@ -57,6 +57,21 @@ typedef struct bear_global_config {
   int spectral_discretisation;      /* 0 const_wavenumber, 1 const_wavelength, 2 const_resolution */
   double spectral_resolution;
   int use_error_inflation;          /* last parameter is log10 of the added variance */
+  /* Storage of the cross-section tables on the device (BEAR_TABLE_*, defined
+   * below; no counterpart in the reference, whose tables are FP64 in
+   * memory).  BEAR_TABLE_FP64 (0) keeps the FP64 rows, which every call can
+   * use and from which bear_retrieval_set_table_precision splits
+   * reduced-precision copies.  FP32/FP16/BF16/INT16 split each species'
+   * rows into the per-wavenumber FP32 base (and scale) and the 16- or
+   * 32-bit values while the tables load (one species' FP64 rows at a time
+   * pass through host and device memory) and keep nothing else: the
+   * batched forward calls run at that precision,
+   * bear_retrieval_compute_likelihood and bear_retrieval_compute_model run
+   * as batches of one, and the calls that need the FP64 rows
+   * (bear_forward_model_calc_model_gpu and its no-allocation variant, the
+   * gradient, another table precision) return BEAR_ERROR_INVALID_INPUT.  The
+   * direct spectrum models accept FP64 only. */
+  int table_storage;
 } bear_global_config;
 
 /* Forward-model configuration (EmissionModelConfig, forward_model/emission/emission.h:51-102;
@@ -200,11 +215,18 @@ int bear_forward_model_calc_model_gpu_noalloc(bear_retrieval* r, const double* p
  *                    radiative transfer, transit-depth integral, bins and
  *                    log-likelihood in FP64.  The residuals' precision
  *                    follows each species' range of log10 sigma over its
- *                    rows at a wavenumber. */
+ *                    rows at a wavenumber.
+ *   BEAR_TABLE_INT16 16-bit integers q with log10 sigma = base + scale q,
+ *                    base and scale FP32 per (species, wavenumber): the
+ *                    same 2 bytes per value as FP16 with an absolute
+ *                    error of at most range / 131068 at every value
+ *                    (FP16 residuals: up to range / 4096); arithmetic as
+ *                    above. */
 #define BEAR_TABLE_FP64 0
 #define BEAR_TABLE_FP32 1
 #define BEAR_TABLE_FP16 2
 #define BEAR_TABLE_BF16 3
+#define BEAR_TABLE_INT16 4
 
 int bear_retrieval_set_table_precision(bear_retrieval* r, int precision);
 
@@ -337,6 +359,7 @@ typedef struct bear_eval_stats {
   size_t table_value_bytes;      /* bytes per stored table value */
   size_t kernel_launches;        /* timed launches since profiling was enabled */
   double kernel_ms_total;        /* their summed duration */
+  size_t table_device_bytes;     /* device memory of the resident tables (FP64 rows, reduced copies) */
 } bear_eval_stats;
 
 int bear_retrieval_set_profiling(bear_retrieval* r, int enable);

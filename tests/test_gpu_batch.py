@@ -28,6 +28,7 @@ REDUCED_BOUNDS = {
     "fp32": dict(spectrum=1e-5, bins=1e-6, ppm=0.01, loglike=3e-3),
     "fp16": dict(spectrum=3e-3, bins=2e-3, ppm=6.0, loglike=2.0),
     "bf16": dict(spectrum=2e-2, bins=1.5e-2, ppm=40.0, loglike=10.0),
+    "int16": dict(spectrum=2e-4, bins=1e-4, ppm=0.4, loglike=0.1),
 }
 
 _CACHE = {}
@@ -132,7 +133,7 @@ def test_uneven_batch_sizes_and_launch_shapes(case_factory, name):
     np.testing.assert_allclose(tuned, expect, rtol=0, atol=LOGLIKE_ATOL)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16", "int16"])
 @pytest.mark.parametrize("name", ["emission_c1", "eclipse_small", "transmission_plain"])
 def test_reduced_precision_error_against_fp64_oracle(case_factory, name, precision):
     folder, pts = setup(case_factory, name, 4)

@@ -222,7 +222,8 @@ def test_full_size_closed_forms(full_factory, name):
 
 # ---------------------------------------------------------------- reduced precision
 REDUCED = [("emission_c2", "fp32"), ("transmission_c3", "fp32"), ("transmission_c3", "fp16"),
-           ("transmission_c3", "bf16"), ("eclipse_c5", "fp16"), ("eclipse_c5", "bf16")]
+           ("transmission_c3", "bf16"), ("transmission_c3", "int16"), ("eclipse_c5", "fp16"),
+           ("eclipse_c5", "bf16"), ("eclipse_c5", "int16")]
 
 
 def _reduced(f, precision):
@@ -238,7 +239,7 @@ def _reduced(f, precision):
 
 @pytest.mark.parametrize("name,precision", REDUCED)
 def test_full_size_reduced_precision_spectral_bar(full_factory, name, precision):
-    """Batched evaluation from FP32 / FP16 / BF16 tables against the FP64
+    """Batched evaluation from FP32 / FP16 / BF16 / scaled INT16 tables against the FP64
     oracle.  Transit and eclipse depths: at most 10 ppm of depth at every
     wavenumber and in every bin.  Emission (FP32 tables): the spectra bar,
     1e-5 relative."""

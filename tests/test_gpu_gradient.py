@@ -199,7 +199,7 @@ def test_transmission_whole_and_chunked_batches_agree(case_factory, tmp_path_fac
     n, L = st["nb_spectral_points"], st["nb_levels"]
     stride = (n + 63) // 64 * 64
     slots = (st["nb_species"] + 3) // 4 * 4 + 4
-    rows = (L - 1 + 15) // 16 * 16
+    rows = (L - 1 + 31) // 32 * 32
     per_point = 4 * L * 3 * stride + 8 * (L * slots * ((n + 63) // 64) + stride + 17 * (L - 1) ** 2 + (L - 1) * rows)
     try:
         ll_s, g_s = r.gradientBatch(pts)
