@@ -111,9 +111,11 @@ def _counter_rows(folder):
     for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
         with open(path) as fh:
             for row in csv.DictReader(fh):
-                name = row["Kernel_Name"].split("(")[0].strip()
+                name = row["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].strip()
                 if name.startswith("void "):
                     name = name[5:]
+                if name.startswith("bear_amd::"):
+                    name = name[len("bear_amd::"):]
                 yield name, row["Counter_Name"], float(row["Counter_Value"]), row.get("Dispatch_Id", "")
 
 
@@ -151,7 +153,7 @@ def summarize_counters(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, cop
                            "fetch_factor": fetch_factor, "write_factor": write_factor}}
     fetch, write = _per_dispatch(fetch_dir, "FETCH_SIZE"), _per_dispatch(write_dir, "WRITE_SIZE")
     for k in sorted(fetch, key=lambda k: -fetch[k][1]):
-        if k not in write:
+        if k not in write or "Kernel" not in k:         # this library's kernels only
             continue
         rd, wr = fetch[k][1] * 1024 * fetch_factor, write[k][1] * 1024 * write_factor
         out[k] = {"dispatches": fetch[k][0], "FETCH_SIZE_KiB": fetch[k][1], "WRITE_SIZE_KiB": write[k][1],
