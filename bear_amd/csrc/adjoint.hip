@@ -659,9 +659,12 @@ __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int n
 }
 
 // Step 2: one workgroup of 256 threads per (64 x 64 block of (q, t) with
-// t-block <= q-block, point, range of wavenumbers), 4 x 4 pairs per thread,
-// the wavenumbers in steps of 32 through LDS;
-// partial[point][range][q][t] for P = L - 1 segments and tangent paths.
+// t-block <= q-block, point, range of wavenumbers), 4 x 4 pairs per thread
+// (four consecutive segments, so each wavefront holds 16 consecutive ones,
+// times four paths), the wavenumbers in steps of 32 through LDS; a
+// wavefront skips a step in which no path of the block reaches its
+// segments.  partial[point][range][q][t] for P = L - 1 segments and tangent
+// paths.
 constexpr int kChordBlock = 64, kChordNu = 32, kChordRanges = 16;
 
 __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restrict__ keep, int nb_levels, int n,
@@ -669,7 +672,8 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
                                                           double* __restrict__ partial) {
   __shared__ float s_sh[kChordBlock][kChordNu + 1];           // S_q in FP32 (the gradient's inputs are FP32)
   __shared__ float a_sh[kChordBlock][kChordNu + 1];
-  __shared__ float l_sh[kChordBlock][kChordNu + 1];
+  __shared__ int l_sh[kChordBlock][kChordNu + 1];             // last segment of each path (-1: none)
+  __shared__ int s_reach[2];                                   // bit w: some path reaches wavefront w's segments
   const int L = nb_levels, P = L - 1;
   int bq = 0, bt = int(blockIdx.x);                                 // triangular block index
   while (bt > bq) {
@@ -678,56 +682,68 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
   }
   const int m = int(blockIdx.y), range = int(blockIdx.z);
   const int q0 = bq * kChordBlock, t0 = bt * kChordBlock;
-  const int tid = int(threadIdx.x), tq = tid >> 4, tt = tid & 15;
+  const int tid = int(threadIdx.x), tq = tid >> 4, tt = tid & 15, wave = tid >> 6;
+  const int q_first = q0 + 4 * tq;                                  // this thread's segments q_first .. + 3
   double acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  if (tid < 2) s_reach[tid] = 0;
+  __syncthreads();
   const int nu0 = range * nu_per_range, nu1 = min(n, nu0 + nu_per_range);
-  for (int v0 = nu0; v0 < nu1; v0 += kChordNu) {
-    bool reach = false;                              // some path of the block sums a segment >= q0
+  for (int v0 = nu0, step = 0; v0 < nu1; v0 += kChordNu, ++step) {
+    int reach = 0;
     for (int e = tid; e < kChordBlock * kChordNu; e += 256) {
       const int row = e / kChordNu, col = e % kChordNu;
       const int v = v0 + col, q = q0 + row, t = t0 + row;
-      float sv = 0.0f, av = 0.0f, lv = -1.0f;
+      float sv = 0.0f, av = 0.0f;
+      int lv = -1;
       if (v < nu1) {
         const float* col = keep + keptColumn(m, stride, L, v);
         if (q < P) sv = float(double(col[size_t(q) * kKeptLevel]) + double(col[size_t(q + 1) * kKeptLevel]));
         if (t < P) {
           av = col[size_t(t) * kKeptLevel + kKeptValue];
-          lv = col[size_t(t) * kKeptLevel + 2 * kKeptValue];
+          lv = int(col[size_t(t) * kKeptLevel + 2 * kKeptValue]);
         }
       }
       s_sh[row][col] = sv;
       a_sh[row][col] = av;
       l_sh[row][col] = lv;
-      reach = reach || lv >= float(q0);
+      // wavefronts 0 .. (lv - q0) / 16, whose first segments this path reaches
+      if (lv >= q0) reach |= (2 << min(3, (lv - q0) / 16)) - 1;
     }
-    if (!__syncthreads_or(reach)) continue;          // every product of this step is masked
-    for (int v = 0; v < kChordNu; ++v) {
-      double sv[4], av[4];
-      float lv[4];
+    if (reach) atomicOr(&s_reach[step & 1], reach);
+    __syncthreads();
+    const int bits = s_reach[step & 1];
+    if ((bits >> wave) & 1) {                                       // uniform over the wavefront
+      for (int v = 0; v < kChordNu; ++v) {
+        double sv[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) sv[a] = double(s_sh[tq + 16 * a][v]);
+        for (int a = 0; a < 4; ++a) sv[a] = double(s_sh[4 * tq + a][v]);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        av[b] = double(a_sh[tt + 16 * b][v]);
-        lv[b] = l_sh[tt + 16 * b][v];
-      }
+        for (int b = 0; b < 4; ++b) {
+          const double av = double(a_sh[tt + 16 * b][v]);
+          const int lv = l_sh[tt + 16 * b][v];
+          // past the path's last segment only the high word of tau-bar is
+          // cleared: a subnormal below 2.2e-314 whose products with S (FP32
+          // extinction sums) vanish in the sums; one select per product
+          const int hi = __double2hiint(av), lo = __double2loint(av);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const float q = float(q0 + tq + 16 * a);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = fma(q <= lv[b] ? av[b] : 0.0, sv[a], acc[a][b]);
+          for (int a = 0; a < 4; ++a)
+            acc[a][b] = fma(__hiloint2double(q_first + a <= lv ? hi : 0, lo), sv[a], acc[a][b]);
+        }
       }
     }
     __syncthreads();
+    // every thread has read this step's bits: clear them for the step after
+    // next (the next step ORs into the other word, cleared a step ago)
+    if (tid == 0) s_reach[step & 1] = 0;
   }
   double* out = partial + (size_t(m) * ranges + range) * size_t(P) * P;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int q = q0 + tq + 16 * a;
+    const int q = q_first + a;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int t = t0 + tt + 16 * b;
@@ -777,7 +793,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
-    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    if (!workItem(item, batch, tiles, m, tile, 1)) continue;   // uniform; tiles of one point together (its chords)
     const int i = tile * kWave + lane;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
@@ -796,31 +812,32 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
 #pragma unroll
       for (int u = 0; u < kAdjointSegments; ++u) sb[u] = 0.0;
       const int t_end = min(qb + kAdjointSegments, L - 1);
-      const float q_first = float(qb), q_last = float(qb + kAdjointSegments - 1);
-      for (int t0 = 0; t0 < t_end; t0 += 4) {          // four paths' values loaded together
-        double tb[4];
-        float last[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float* col = kp + size_t(min(t0 + v, t_end - 1)) * level_step;
-          tb[v] = t0 + v < t_end ? double(col[kKeptValue]) : 0.0;
-          last[v] = col[2 * kKeptValue];
+      // this lane's tau-bar and last segment of path t; path t + 1's are
+      // loaded while path t's products run
+      auto column = [&](int t) { return kp + size_t(t) * level_step; };
+      double tb = double(column(0)[kKeptValue]);
+      int last = int(column(0)[2 * kKeptValue]);
+      for (int t = 0; t < t_end; ++t) {
+        double tb_next = 0.0;
+        int last_next = 0;
+        if (t + 1 < t_end) {                           // uniform
+          tb_next = double(column(t + 1)[kKeptValue]);
+          last_next = int(column(t + 1)[2 * kKeptValue]);
         }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int t = t0 + v;
-          if (t >= t_end) break;                       // uniform
-          if (!__any(last[v] >= q_first)) continue;    // uniform: the path ends below every segment of the pass
+        if (__any(last >= qb)) {                       // uniform: else the path ends below every segment of the pass
           const double* ct = c + size_t(t) * rows + qb;
-          if (__all(last[v] >= q_last)) {
+          const int k = last - qb;                     // this lane sums segments qb .. qb + k
+          // Past segment qb + k only the high word of tau-bar is cleared:
+          // what remains is a subnormal below 2.2e-314, whose products
+          // with chords (below 1e10 cm) stay below 3e-304 and vanish in
+          // the sums.  One select per product instead of two.
+          const int hi = __double2hiint(tb), lo = __double2loint(tb);
 #pragma unroll
-            for (int u = 0; u < kAdjointSegments; ++u) sb[u] = fma(tb[v], ct[u], sb[u]);
-          } else {
-#pragma unroll
-            for (int u = 0; u < kAdjointSegments; ++u)
-              sb[u] = fma(float(qb + u) <= last[v] ? tb[v] : 0.0, ct[u], sb[u]);
-          }
+          for (int u = 0; u < kAdjointSegments; ++u)
+            sb[u] = fma(__hiloint2double(u <= k ? hi : 0, lo), ct[u], sb[u]);
         }
+        tb = tb_next;
+        last = last_next;
       }
 #pragma unroll
       for (int u = kAdjointSegments - 1; u >= 0; --u) {
