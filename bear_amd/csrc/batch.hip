@@ -72,13 +72,14 @@ struct TabF16 {
 };
 struct TabBF16 {
   using T = uint16_t;
-  using R = uint32_t;
-  static __device__ __forceinline__ R raw(const T* p) { return *p; }
-  static __device__ __forceinline__ float value(R r) { return __uint_as_float(r << 16); }
+  using R = float;                                 // widened when a row is read
+  static __device__ __forceinline__ R raw(const T* p) { return __uint_as_float(uint32_t(*p) << 16); }
+  static __device__ __forceinline__ float value(R r) { return r; }
   static __device__ __forceinline__ T store(double v) {   // round to nearest even
     const uint32_t u = __float_as_uint(float(v));
     return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-  }  static constexpr bool kScaled = false;
+  }
+  static constexpr bool kScaled = false;
 };
 // 16-bit integers q in [-32767, 32767] with value = base + scale q, base and
 // scale per (species, wavenumber): the 16 bits cover each wavenumber's own
@@ -87,9 +88,8 @@ struct TabBF16 {
 // slot the base row and the scale row.
 struct TabS16 {
   using T = int16_t;
-  using R = int32_t;
-  static __device__ __forceinline__ R raw(const T* p) { return *p; }
-  static __device__ __forceinline__ float value(R r) { return float(r); }
+  using R = float;                                 // scale q, formed when a row is read
+  static __device__ __forceinline__ float value(R r) { return r; }
   static constexpr bool kScaled = true;
   static constexpr int kMax = 32767;
 };
@@ -177,20 +177,16 @@ struct Opacity32 {
       }
     }
   }
-  // x = bz + log2 n + scale sum_k w_k v_k (scale 1 unless TabS16)
+  // x = bz + log2 n + sum_k w_k v_k (TabS16: v = scale q, formed at the row's read)
   __device__ __forceinline__ float exponent(int s, float start, const float* w) const {
-    if constexpr (Tab::kScaled) {
-      float a = w[0] * Tab::value(v[s][0]);
-      a = fmaf(w[1], Tab::value(v[s][1]), a);
-      a = fmaf(w[2], Tab::value(v[s][2]), a);
-      a = fmaf(w[3], Tab::value(v[s][3]), a);
-      return fmaf(sc[s], a, start);
-    } else {
-      float x = fmaf(w[0], Tab::value(v[s][0]), start);
-      x = fmaf(w[1], Tab::value(v[s][1]), x);
-      x = fmaf(w[2], Tab::value(v[s][2]), x);
-      return fmaf(w[3], Tab::value(v[s][3]), x);
-    }
+    float x = fmaf(w[0], Tab::value(v[s][0]), start);
+    x = fmaf(w[1], Tab::value(v[s][1]), x);
+    x = fmaf(w[2], Tab::value(v[s][2]), x);
+    return fmaf(w[3], Tab::value(v[s][3]), x);
+  }
+  __device__ __forceinline__ typename Tab::R load(const typename Tab::T* p, int s) const {
+    if constexpr (Tab::kScaled) return sc[s] * float(*p);
+    else return Tab::raw(p);
   }
   // grouped stencils: the level's shared weights and contiguous densities
   // (FP32 copies), as kappaTerms does for the FP64 tables
@@ -200,7 +196,7 @@ struct Opacity32 {
 #pragma unroll
       for (int s = 0; s < NS; ++s)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[s][q] = Tab::raw(tab + (size_t(d[s].block[q]) << 6));
+        for (int q = 0; q < 4; ++q) v[s][q] = load(tab + (size_t(d[s].block[q]) << 6), s);
     }
     float k = 0.0f;
     if (groups.grouped) {

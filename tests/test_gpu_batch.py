@@ -23,12 +23,13 @@ LOGLIKE_ATOL = 1e-6
 # high-resolution spectrum and of the bins, absolute error in ppm of depth
 # (eclipse and transmission), absolute error of the log-likelihood.
 # Measured on MI355X: fp32 4.3e-6 / 1.6e-7 / 1.2e-3 ppm / 6.3e-4;
-# fp16 1.1e-3 / 7.0e-4 / 2.05 ppm / 0.50; bf16 7.5e-3 / 4.7e-3 / 13.7 ppm / 3.3.
+# fp16 1.1e-3 / 7.0e-4 / 2.05 ppm / 0.50; bf16 7.5e-3 / 4.7e-3 / 13.7 ppm / 3.3;
+# int16 4.0e-5 / 5.5e-7 / 0.014 ppm / 7.8e-4.
 REDUCED_BOUNDS = {
     "fp32": dict(spectrum=1e-5, bins=1e-6, ppm=0.01, loglike=3e-3),
     "fp16": dict(spectrum=3e-3, bins=2e-3, ppm=6.0, loglike=2.0),
     "bf16": dict(spectrum=2e-2, bins=1.5e-2, ppm=40.0, loglike=10.0),
-    "int16": dict(spectrum=2e-4, bins=1e-4, ppm=0.4, loglike=0.1),
+    "int16": dict(spectrum=1.2e-4, bins=1.5e-6, ppm=0.05, loglike=3e-3),
 }
 
 _CACHE = {}
