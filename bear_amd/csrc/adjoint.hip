@@ -42,6 +42,7 @@
 #include <string>
 
 #include "device_math.h"
+#include "tile_window.h"
 #include "engine.h"
 
 namespace bear_amd {
@@ -504,15 +505,16 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
 template <int NS>
 __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __restrict__ tables, size_t stride,
                                                               Records rec, int nb_levels, int n, int tiles,
-                                                              int batch, const float* __restrict__ keep,
+                                                              TileWindow win, int batch,
+                                                              const float* __restrict__ keep,
                                                               double* __restrict__ partial) {
   constexpr int K = NS + 4;
   const int L = nb_levels;
   const int lane = int(threadIdx.x);
-  const size_t total = nbItems(tiles, batch);
+  const size_t total = nbItems(win.count, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
-    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    if (!windowItem(item, batch, win, m, tile)) continue;          // uniform over the wavefront
     const int i = tile * kWave + lane;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
     const LevelSpecies* species = reinterpret_cast<const LevelSpecies*>(r + rec.species);
     const LevelSpeciesAdjoint* extra = reinterpret_cast<const LevelSpeciesAdjoint*>(r + rec.adjoint);
-    const float* kb = keep + keptColumn(m, stride, L, ic) + 2 * kKeptValue;
+    const float* kb = keep + keptAt(win, m, stride, L, ic) + 2 * kKeptValue;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double* tab = tables + ic;
     double v[NS][4];
@@ -590,8 +592,8 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
     float* kept_values = reinterpret_cast<float*>(buffer);
     hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, fbar,
                        kept_values, partial);
-    hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch,
-                       kept_values, partial);
+    hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
+                       wholeGrid(tiles), batch, kept_values, partial);
   } else {
     hipLaunchKernelGGL(emissionAdjointKernel<NS>, grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n, tiles,
                        batch, fbar, buffer, partial);
@@ -622,23 +624,24 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 // The host carries d logL / d chord and d logL / d c_t to the altitudes and
 // the radius.
 __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int nb_levels, int n, size_t stride, int tiles,
-                                                              int batch, int ns, const double* __restrict__ fbar,
+                                                              TileWindow win, int batch, int ns,
+                                                              const double* __restrict__ fbar,
                                                               const double* __restrict__ depth,
                                                               float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels, K = ns + 4;
   const int lane = int(threadIdx.x);
   const size_t level_step = kKeptLevel;
-  const size_t total = nbItems(tiles, batch);
+  const size_t total = nbItems(win.count, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
-    if (!workItem(item, batch, tiles, m, tile)) continue;          // uniform over the wavefront
+    if (!windowItem(item, batch, win, m, tile)) continue;          // uniform over the wavefront
     const int i = tile * kWave + lane;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;                                // idle lanes: zero adjoints, no stores
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
-    float* kp = keep + keptColumn(m, stride, L, ic);
+    float* kp = keep + keptAt(win, m, stride, L, ic);
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double e = fb * 1e6 / (bm->star_radius * bm->star_radius);
@@ -666,9 +669,13 @@ __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int n
 // segments.  partial[point][range][q][t] for P = L - 1 segments and tangent
 // paths.
 constexpr int kChordBlock = 64, kChordNu = 32, kChordRanges = 16;
+// chord ranges (and hence adjoint windows) start at multiples of this many
+// wavenumbers: the widest tile of the batched transmission kernel
+constexpr int kWindowAlign = 128;
 
 __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restrict__ keep, int nb_levels, int n,
-                                                          size_t stride, int ranges, int nu_per_range,
+                                                          size_t stride, int ranges, int range_first,
+                                                          int nu_per_range, TileWindow win,
                                                           double* __restrict__ partial) {
   __shared__ float s_sh[kChordBlock][kChordNu + 1];           // S_q in FP32 (the gradient's inputs are FP32)
   __shared__ float a_sh[kChordBlock][kChordNu + 1];
@@ -680,7 +687,7 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
     bt -= bq + 1;
     ++bq;
   }
-  const int m = int(blockIdx.y), range = int(blockIdx.z);
+  const int m = int(blockIdx.y), range = range_first + int(blockIdx.z);
   const int q0 = bq * kChordBlock, t0 = bt * kChordBlock;
   const int tid = int(threadIdx.x), tq = tid >> 4, tt = tid & 15, wave = tid >> 6;
   const int q_first = q0 + 4 * tq;                                  // this thread's segments q_first .. + 3
@@ -700,7 +707,7 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
       float sv = 0.0f, av = 0.0f;
       int lv = -1;
       if (v < nu1) {
-        const float* col = keep + keptColumn(m, stride, L, v);
+        const float* col = keep + keptAt(win, m, stride, L, v);
         if (q < P) sv = float(double(col[size_t(q) * kKeptLevel]) + double(col[size_t(q + 1) * kKeptLevel]));
         if (t < P) {
           av = col[size_t(t) * kKeptLevel + kKeptValue];
@@ -784,20 +791,21 @@ __global__ __launch_bounds__(kBlock) void chordTransposeKernel(const double* __r
 
 __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const double* __restrict__ chord_t,
                                                               const double* __restrict__ rayleigh, int nb_levels,
-                                                              int n, size_t stride, int tiles, int batch, int ns,
-                                                              float* __restrict__ keep, double* __restrict__ partial) {
+                                                              int n, size_t stride, int tiles, TileWindow win,
+                                                              int batch, int ns, float* __restrict__ keep,
+                                                              double* __restrict__ partial) {
   const int L = nb_levels, K = ns + 4;
   const int rows = segmentRows(L);
   const int lane = int(threadIdx.x);
   const size_t level_step = kKeptLevel;
-  const size_t total = nbItems(tiles, batch);
+  const size_t total = nbItems(win.count, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tile;
-    if (!workItem(item, batch, tiles, m, tile, 1)) continue;   // uniform; tiles of one point together (its chords)
+    if (!windowItem(item, batch, win, m, tile, 1)) continue;   // uniform; tiles of one point together (its chords)
     const int i = tile * kWave + lane;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;
-    float* kp = keep + keptColumn(m, stride, L, ic);
+    float* kp = keep + keptAt(win, m, stride, L, ic);
     const double* c = chord_t + size_t(m) * (L - 1) * rows;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
     const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
@@ -859,26 +867,28 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
   }
 }
 
+// The transmission adjoint over the wavenumbers of chord ranges [r0, r1)
+// (win: their tiles of 64; keep holds those tiles' kept quantities): path,
+// chord, segment and table adjoints.  The chord products of the ranges go
+// to chord_partial[point][range]; chordReduceKernel sums the ranges once
+// every window is done.  chord_t (chordTransposeKernel) is made before the
+// first window.
 template <int NS>
 void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const double* tables, size_t stride,
-                               const double* rayleigh, Records rec, const double* chord, int L, int n, int tiles,
+                               const double* rayleigh, Records rec, int L, int n, int tiles, TileWindow win,
                                int batch, const double* fbar, const double* depth, float* keep, double* partial,
-                               int ranges, int nu_per_range, double* chord_partial, double* chord_out,
-                               double* chord_t) {
-  hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, batch, NS, fbar,
-                     depth, keep, partial);
+                               int ranges, int r0, int r1, int nu_per_range, double* chord_partial,
+                               const double* chord_t) {
+  hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, win, batch, NS,
+                     fbar, depth, keep, partial);
   const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
-  hipLaunchKernelGGL(chordAdjointKernel, dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(ranges)),
-                     dim3(256), 0, s, keep, L, n, stride, ranges, nu_per_range, chord_partial);
-  hipLaunchKernelGGL(chordReduceKernel, dim3(unsigned((P * P + kBlock - 1) / kBlock), unsigned(batch)), dim3(kBlock),
-                     0, s, chord_partial, P, ranges, chord_out);
-  const int rows = segmentRows(L);
-  hipLaunchKernelGGL(chordTransposeKernel, dim3(unsigned((P * rows + kBlock - 1) / kBlock), unsigned(batch)),
-                     dim3(kBlock), 0, s, chord, L, chord_t);
+  hipLaunchKernelGGL(chordAdjointKernel,
+                     dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(r1 - r0)), dim3(256), 0, s,
+                     keep, L, n, stride, ranges, r0, nu_per_range, win, chord_partial);
   hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord_t, rayleigh, L, n, stride, tiles,
-                     batch, NS, keep, partial);
-  hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, batch, keep,
-                     partial);
+                     win, batch, NS, keep, partial);
+  hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, win,
+                     batch, keep, partial);
 }
 
 size_t align8(size_t x) { return (x + 7) / 8 * 8; }
@@ -900,6 +910,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   if (transmission != transmission_) raise(kInternal, "adjoint model kind differs from the configured model");
   const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
+  if (hminus_) raise(kInvalidInput, "the adjoint does not take the H- continuum");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
   const int NS = int(species_stride_);
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
@@ -912,32 +923,60 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
 
   // points per chunk: kept forward quantities within the budget (and 80 % of
   // the free memory, checked when the buffers grow), else recomputed
-  // (emission and eclipse only); transmission also holds per point the
-  // chord gradients and their per-range partial sums
+  // (emission and eclipse); transmission also holds per point the chord
+  // gradients and their per-range partial sums, and when one point's kept
+  // quantities exceed the budget it takes one point at a time with its grid
+  // in windows of whole chord ranges, running the forward kernel again with
+  // the kept quantities for each window
   const int P = L - 1;
+  // chord ranges: fixed by the grid, none empty, each a multiple of 128
+  // wavenumbers (the batched transmission kernel's widest tile)
+  int ranges = transmission ? std::max(1, std::min(kChordRanges, (n + 511) / 512)) : 1;
+  int nu_per_range = 0;
+  for (;;) {
+    nu_per_range = int(((n + ranges - 1) / ranges + kWindowAlign - 1) / kWindowAlign * kWindowAlign);
+    if (ranges == 1 || (ranges - 1) * nu_per_range < n) break;
+    --ranges;
+  }
   const size_t chord_point =
       transmission ? size_t(P) * P * (kChordRanges + 1) + size_t(P) * segmentRows(L) : 0;   // doubles per point
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
   const size_t kept_values = size_t(L) * 3 * stride_;                          // FP32 values per point
   const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_ + chord_point;  // doubles per point
+  const size_t kept_range = size_t(L) * 3 * size_t(nu_per_range);             // FP32 values per chord range
   size_t budget = adjoint_budget_;
   size_t chunk = 0, buffer_size = 0, partial_size = 0;
+  int window_ranges = ranges;                                                  // chord ranges per window
   bool kept = false;
   auto plan = [&] {
     chunk = std::min(batch, budget / (kept_point * sizeof(double)));
     kept = chunk > 0;
+    window_ranges = ranges;
+    if (!kept && transmission) {
+      const size_t fixed = partial_point + stride_ + chord_point;             // doubles, one point
+      const size_t room = budget / sizeof(double);
+      window_ranges = room > fixed ? int(std::min<size_t>(ranges, (room - fixed) / ((kept_range + 1) / 2))) : 0;
+      if (window_ranges > 0) {
+        chunk = 1;
+        kept = true;
+      }
+    }
     if (!kept)
       chunk = std::min(batch, std::max<size_t>(1, kPartialBudget / ((partial_point + stride_) * sizeof(double))));
     const size_t chunks = (batch + chunk - 1) / chunk;                      // equal chunks, no small tail
     chunk = (batch + chunks - 1) / chunks;
-    buffer_size = kept ? (chunk * kept_values + 1) / 2 : size_t(grid) * L * 2 * kWave;   // doubles
+    if (!kept) buffer_size = size_t(grid) * L * 2 * kWave;                   // doubles
+    else if (window_ranges < ranges) buffer_size = (size_t(window_ranges) * kept_range + 1) / 2;
+    else buffer_size = (chunk * kept_values + 1) / 2;
     partial_size = chunk * (partial_point + chord_point);
   };
   plan();
   if (transmission && !kept)
-    raise(kInvalidInput, "the transmission adjoint keeps each point's opacity and path adjoints (" +
-                             std::to_string(kept_point * sizeof(double)) +
-                             " bytes per point): raise the adjoint memory");
+    raise(kInvalidInput, "the transmission adjoint keeps one point's opacity and path adjoints for at least 1/" +
+                             std::to_string(ranges) + " of its grid (" +
+                             std::to_string((partial_point + stride_ + chord_point + (kept_range + 1) / 2) *
+                                            sizeof(double)) +
+                             " bytes): raise the adjoint memory");
   const bool grow = batch_capacity_ > adj_capacity_ || size_t(grid) != adj_grid_ ||
                     partial_size > adj_partial_size_ || buffer_size > adj_intensity_size_;
   if (grow) {
@@ -970,7 +1009,9 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                       rec_shifts_, groups_};
     float* keep = kept ? reinterpret_cast<float*>(adj_intensity_) : nullptr;
-    launchBatchSpectrum(transmission ? ModelKind::transmission : ModelKind::emission, size_t(cb), launch, first, keep);
+    const bool windowed = transmission && window_ranges < ranges;   // the kept forward runs per window
+    launchBatchSpectrum(transmission ? ModelKind::transmission : ModelKind::emission, size_t(cb), launch, first,
+                        windowed ? nullptr : keep);
     launchBatchRest(kind, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
     double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;
@@ -992,29 +1033,41 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
                        s, adj_ptr_, adj_bin_, adj_weight_, fscale, nb, n, stride_, adj_fbar_);
     check(hipGetLastError(), "adjoint flux kernel");
     if (transmission) {
-      // wavenumber ranges of the chord products, from the grid alone (not
-      // the chunk's points), so that a point's summation order and hence
-      // its gradient do not depend on the batch it came in
-      const int ranges = std::max(1, std::min(kChordRanges, (n + 511) / 512));
-      const int per_range = (n + ranges - 1) / ranges;
-      const int nu_per_range = (per_range + kChordNu - 1) / kChordNu * kChordNu;
       double* chord_partial = adj_partial_ + size_t(cb) * partial_point;
       double* chord_out = chord_partial + size_t(cb) * P * P * kChordRanges;
       double* chord_t = chord_out + size_t(cb) * P * P;
-      switch (NS) {
+      const int rows = segmentRows(L);
+      hipLaunchKernelGGL(chordTransposeKernel, dim3(unsigned((P * rows + kBlock - 1) / kBlock), unsigned(cb)),
+                         dim3(kBlock), 0, s, batch_chord_, L, chord_t);
+      for (int r0 = 0; r0 < ranges; r0 += window_ranges) {
+        const int r1 = std::min(ranges, r0 + window_ranges);
+        TileWindow win = wholeGrid(tiles);
+        if (windowed) {                                   // one point: rerun its forward over the window, kept
+          const size_t nu_first = size_t(r0) * nu_per_range;
+          const size_t nu_end = std::min(size_t(n), size_t(r1) * nu_per_range);
+          win.first = int(nu_first / kWave);
+          win.count = int((nu_end + kWave - 1) / kWave) - win.first;
+          win.kept_origin = nu_first / kWave * size_t(L) * kKeptLevel;
+          launchBatchSpectrum(ModelKind::transmission, 1, launch, first, keep, nu_first, nu_end - nu_first);
+        }
+        switch (NS) {
 #define BEAR_TADJ(NSV)                                                                                             \
   case NSV:                                                                                                        \
     launchTransmissionAdjoint<NSV>(dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, tables_, stride_, rayleigh_,  \
-                                   rec, batch_chord_, L, n, tiles, cb, adj_fbar_, batch_flux_ + first * stride_,  \
-                                   keep, adj_partial_, ranges, nu_per_range, chord_partial, chord_out, chord_t);  \
+                                   rec, L, n, tiles, win, cb, adj_fbar_, batch_flux_ + first * stride_, keep,     \
+                                   adj_partial_, ranges, r0, r1, nu_per_range, chord_partial, chord_t);           \
     break;
-        BEAR_TADJ(4)
-        BEAR_TADJ(8)
-        BEAR_TADJ(12)
-        BEAR_TADJ(16)
+          BEAR_TADJ(4)
+          BEAR_TADJ(8)
+          BEAR_TADJ(12)
+          BEAR_TADJ(16)
 #undef BEAR_TADJ
+        }
+        check(hipGetLastError(), "transmission adjoint kernels");
       }
-      check(hipGetLastError(), "transmission adjoint kernels");
+      hipLaunchKernelGGL(chordReduceKernel, dim3(unsigned((P * P + kBlock - 1) / kBlock), unsigned(cb)),
+                         dim3(kBlock), 0, s, chord_partial, P, ranges, chord_out);
+      check(hipGetLastError(), "chord reduction kernel");
       check(hipMemcpyAsync(bin_terms.data() + first * P * P, chord_out, size_t(cb) * P * P * sizeof(double),
                            hipMemcpyDeviceToHost, s),
             "read chord gradients");

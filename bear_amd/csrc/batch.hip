@@ -22,6 +22,7 @@
 #include <string>
 
 #include "device_math.h"
+#include "tile_window.h"
 #include "engine.h"
 
 namespace bear_amd {
@@ -332,7 +333,7 @@ template <class Op, bool kKeep = false>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
-    int tiles, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep) {
+    TileWindow win, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep) {
   using K = typename Op::K;
   using Species = typename Op::Species;
   extern __shared__ double lds[];
@@ -346,10 +347,10 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
   const int chunk = (L + groups - 1) / groups;
   const int l0 = g * chunk;
   const int l1 = min(L, l0 + chunk);
-  const size_t total = nbItems(tiles, batch);
+  const size_t total = nbItems(win.count, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
     int m, tl;
-    if (!workItem(item, batch, tiles, m, tl, point_group)) continue;      // uniform over the workgroup
+    if (!windowItem(item, batch, win, m, tl, point_group)) continue;      // uniform over the workgroup
     const unsigned char* r = rec.at(m);
     const BatchModel* bm = reinterpret_cast<const BatchModel*>(r);
     const LevelState* levels = reinterpret_cast<const LevelState*>(r + rec.levels);
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
         const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
                                           l == l0 || levels[l].reload != 0));
         const K k = K(kd + sigma_r * levels[l].rayleigh_density);
-        if constexpr (kKeep) keep[keptColumn(m, stride, L, i) + size_t(l) * kKeptLevel] = float(k);
+        if constexpr (kKeep) keep[keptAt(win, m, stride, L, i) + size_t(l) * kKeptLevel] = float(k);
         if (l == l0) part[g * tile + j] = double(k);
         else S[(l - 1) * tile + j] = last + k;
         last = k;
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
           for (int k = 0; k < kTangentBlock; ++k) {
             const int t = t0 + k;
             if (t < L - 1) {
-              float* kt = keep + keptColumn(m, stride, L, i) + size_t(t) * kKeptLevel;
+              float* kt = keep + keptAt(win, m, stride, L, i) + size_t(t) * kKeptLevel;
               kt[kKeptValue] = float(tau[k]);
               kt[2 * kKeptValue] = float(stop[k]);
             }
@@ -504,33 +505,33 @@ void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, cons
 template <int NS>
 void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
                         const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
-                        const double* chord, const float* chord32, int L, int n, int tiles, int tile, int batch,
-                        int group, double* depth, float* keep) {
+                        const double* chord, const float* chord32, int L, int n, TileWindow win, int tile,
+                        int batch, int group, double* depth, float* keep) {
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       if (keep)
         hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true>), grid, block, lds, s, tab64, base, stride,
-                           rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, keep);
+                           rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep);
       else
         hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                           rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+                           rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
       break;
     case TablePrecision::int16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, tiles, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
       break;
   }
 }
@@ -677,7 +678,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
 
 // ---------------------------------------------------------------- launches
 void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc, size_t first,
-                                       float* keep) {
+                                       float* keep, size_t nu_first, size_t nu_count) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                     rec_shifts_, groups_};
@@ -689,12 +690,19 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
                        wide ? batch_chord_ : nullptr, wide ? nullptr : batch_chord32_);
     check(hipGetLastError(), "chord kernel");
     const int tiles = (n + lc.tile - 1) / lc.tile;
+    TileWindow win = wholeGrid(tiles);
+    if (nu_count > 0) {                                   // wavenumbers [nu_first, nu_first + nu_count)
+      if (nu_first % size_t(lc.tile) != 0 || nu_first >= size_t(n)) raise(kInternal, "misaligned wavenumber window");
+      win.first = int(nu_first / size_t(lc.tile));
+      win.count = int((std::min(size_t(n), nu_first + nu_count) + size_t(lc.tile) - 1) / size_t(lc.tile)) - win.first;
+      win.kept_origin = nu_first / kKeptValue * size_t(L) * kKeptLevel;
+    }
     const size_t lds = transmissionLds(L, lc.tile, wide);
     switch (species_stride_) {
 #define BEAR_T(NS)                                                                                                  \
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
-                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, tiles, lc.tile, B, lc.group, flux, \
+                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, win, lc.tile, B, lc.group, flux,   \
                            keep);                                                                              \
     break;
       BEAR_T(4)
@@ -820,6 +828,8 @@ void DeviceEngine::releaseGraph() {
 }
 
 void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
+  if (species_stride_ > 16) raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
+  if (hminus_) raise(kInvalidInput, "the H- continuum is computed by the single-point kernels only");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "batch larger than the staged records");
   if ((kind == ModelKind::transmission) != transmission_) raise(kInvalidInput, "model kind does not match the engine");
   check(hipSetDevice(device_), "hipSetDevice");

@@ -192,6 +192,22 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
   return k;
 }
 
+// The H- continuum (h_m.cpp:38-53, 56-82, 86-137): per wavenumber the
+// bound-free cross-section and the six free-free wavelength polynomials
+// (rows [7][n]), per level their weights LevelState::hminus_c, so
+// kappa_H- = c0 sigma_bf + sum_i c_i P_i.
+__device__ __forceinline__ void loadHminus(const double* __restrict__ rows, int i, int n, double (&hm)[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) hm[r] = rows[size_t(r) * n + i];
+}
+
+__device__ __forceinline__ double hminusKappa(const double (&hm)[7], const LevelState& lev) {
+  double k = lev.hminus_c[0] * hm[0];
+#pragma unroll
+  for (int r = 1; r < 7; ++r) k = fma(lev.hminus_c[r], hm[r], k);
+  return k;
+}
+
 // The same without a register cache, for any number of species: chunks of
 // kChunk slots with every row of a chunk loaded before any is used; weights
 // and densities from each slot's record.

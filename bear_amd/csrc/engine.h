@@ -33,6 +33,10 @@ struct LevelState {
   float log2_n32[16];
   // LevelSpeciesAdjoint::u shared by the slots of stencil group 0 and 1
   double group_u[2][4];
+  // the H- continuum's weights of its seven per-wavenumber rows: the H-
+  // density, then 1e-29 (5040/T)^((i+1)/2) p_e n_H for i = 1..6 (all zero
+  // where the level holds no H-)
+  double hminus_c[7];
 };
 
 // log2 of a zero effective density in the FP64 stencil records: finite, so
@@ -186,6 +190,10 @@ class DeviceEngine {
   void uploadSpeciesRows(int slot, size_t first, size_t count, const double* rows);
   size_t tableBytes() const;                     // device bytes of the resident tables
   void uploadRayleigh(const std::vector<double>& cross_section);   // [N] cm^2, empty: no scatterer
+  // the H- continuum's rows [7][N] (bound-free cross-section, six free-free
+  // polynomials); empty: no H-.  The single-point kernels only.
+  void uploadHminus(const std::vector<double>& rows);
+  bool hasHminus() const { return hminus_ != nullptr; }
   void uploadObservations(const std::vector<ObservationSetup>& observations);
   void configure(size_t nb_levels, size_t nb_species, bool transmission);
   void setStencilGroups(StencilGroups g) { groups_ = g; }
@@ -319,6 +327,7 @@ class DeviceEngine {
   double* wavelength_ = nullptr;
   double* tables_ = nullptr;
   double* rayleigh_ = nullptr;
+  double* hminus_ = nullptr;                     // [7][N], or none
   double* flux_ = nullptr;
   double* spectrum_ = nullptr;
   double* star_ = nullptr;                       // tabulated stellar flux [N], or none
@@ -356,8 +365,11 @@ class DeviceEngine {
   // every level's I - B of both angles and opacity, FP32
   // [batch][64-wavenumber tile][levels][3][64] (emission; transmission: the extinction in
   // the first of the three values)
+  // nu_count > 0 (transmission): only wavenumbers [nu_first, nu_first +
+  // nu_count) of one point, kept from the start of `keep` (nu_first a
+  // multiple of the launch's tile)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
-                           float* keep = nullptr);
+                           float* keep = nullptr, size_t nu_first = 0, size_t nu_count = 0);
   void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);
   BatchLaunch tuned(ModelKind kind, size_t batch);
   BatchLaunch tune(ModelKind kind, size_t batch);

@@ -88,11 +88,12 @@ __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2
 // values of every stencil slot stay in registers from level to level and are
 // reloaded only where the rows change, so each table row is read about once
 // per thread; kappa and the intensities never leave registers.  The stencil
-// fields are wave-uniform (scalar loads).
-template <int NS>
+// fields are wave-uniform (scalar loads).  HM: the H- continuum's seven rows
+// [7][n] (hminus) are read once per thread and weighted per level.
+template <int NS, bool HM>
 __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
-    const double* __restrict__ tables, size_t stride,
+    const double* __restrict__ tables, size_t stride, const double* __restrict__ hminus,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups groups,
     int nb_levels, int n, double scaling, SpectrumOut out, double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -102,13 +103,17 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   double v[NS][4];
+  double hm[7];
+  if (HM) loadHminus(hminus, i, n, hm);
   double k_prev = kappaTerms<NS>(tab, stride, species, levels[0], groups, true, v);
+  if (HM) k_prev += hminusKappa(hm, levels[0]);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], groups,
-                                         levels[l].reload != 0, v);
+    double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], groups,
+                                   levels[l].reload != 0, v);
+    if (HM) k_next += hminusKappa(hm, levels[l]);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -119,10 +124,11 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   writeFlux(i, nu, i1, i2, scaling, wavelength, out, flux, spectrum);
 }
 
-// The same for any number of species, without the register cache.
+// The same for any number of species, without the register cache (hminus:
+// the H- rows, or none).
 __global__ __launch_bounds__(kBlock) void emissionKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
-    const double* __restrict__ tables, size_t stride,
+    const double* __restrict__ tables, size_t stride, const double* __restrict__ hminus,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
     int nb_levels, int nb_chunks, int n, double scaling, SpectrumOut out,
     double* __restrict__ flux, double* __restrict__ spectrum) {
@@ -133,12 +139,16 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   const int stride_species = nb_chunks * kChunk;
+  double hm[7];
+  if (hminus) loadHminus(hminus, i, n, hm);
   double k_prev = chunkedKappa(tab, stride, species, levels[0], nb_chunks);
+  if (hminus) k_prev += hminusKappa(hm, levels[0]);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    const double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], nb_chunks);
+    double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], nb_chunks);
+    if (hminus) k_next += hminusKappa(hm, levels[l]);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
@@ -164,10 +174,12 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
 //     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R and
 //     the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
-// The transposed chord matrix chord[q][t] comes from the host.
-template <int NS>
+// The transposed chord matrix chord[q][t] comes from the host.  HM: the H-
+// continuum's rows [7][n] (hminus) join the absorption.
+template <int NS, bool HM>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
+    const double* __restrict__ hminus,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups stencil_groups,
     const double* __restrict__ chord, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
     double* __restrict__ depth, double* __restrict__ spectrum) {
@@ -188,10 +200,13 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* tab = tables + i;
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
+    double hm[7];
+    if (HM) loadHminus(hminus, i, n, hm);
     for (int l = l0; l < l1; ++l) {
-      const double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], stencil_groups,
-                                      l == l0 || levels[l].reload != 0, v) +
-                       sigma_r * levels[l].rayleigh_density;
+      double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], stencil_groups,
+                                l == l0 || levels[l].reload != 0, v) +
+                 sigma_r * levels[l].rayleigh_density;
+      if (HM) k += hminusKappa(hm, levels[l]);
       if (l == l0) part[g * tile + j] = k;
       else S[(l - 1) * tile + j] = last + k;
       last = k;
@@ -344,9 +359,9 @@ DeviceEngine::~DeviceEngine() {
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
   releaseBatch();
   releaseAdjoint();
-  for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)flux_,
-                  (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_, (void*)chunk_sums_, (void*)ll_,
-                  (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,
+  for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)hminus_,
+                  (void*)flux_, (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_,
+                  (void*)chunk_sums_, (void*)ll_, (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,
                   (void*)star_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
@@ -501,6 +516,19 @@ void DeviceEngine::uploadRayleigh(const std::vector<double>& sigma) {
   check(hipStreamSynchronize(s), "copy Rayleigh cross-sections");
 }
 
+void DeviceEngine::uploadHminus(const std::vector<double>& rows) {
+  check(hipSetDevice(device_), "hipSetDevice");
+  deviceFree(hminus_);
+  hminus_ = nullptr;
+  if (rows.empty()) return;
+  if (rows.size() != 7 * n_points_) raise(kInvalidInput, "H- rows do not match the grid");
+  hminus_ = deviceAlloc<double>(rows.size(), "H- continuum rows");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(hminus_, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, s),
+        "copy H- continuum rows");
+  check(hipStreamSynchronize(s), "copy H- continuum rows");
+}
+
 void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) {
   check(hipSetDevice(device_), "hipSetDevice");
   std::vector<BinDesc> desc;
@@ -653,14 +681,18 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
 
 namespace {
 void launchEmission(hipStream_t s, size_t species_stride, int blocks, const double* nu, const double* lam,
-                    const double* tables, size_t stride, const LevelState* lev, const LevelSpecies* spc,
-                    StencilGroups groups, int L, int n, double scaling, SpectrumOut out, double* flux,
-                    double* spectrum) {
+                    const double* tables, size_t stride, const double* hminus, const LevelState* lev,
+                    const LevelSpecies* spc, StencilGroups groups, int L, int n, double scaling, SpectrumOut out,
+                    double* flux, double* spectrum) {
   switch (species_stride) {
 #define BEAR_CACHED(NS)                                                                                             \
   case NS:                                                                                                          \
-    hipLaunchKernelGGL(emissionKernelCached<NS>, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev,   \
-                       spc, groups, L, n, scaling, out, flux, spectrum);                                            \
+    if (hminus)                                                                                                     \
+      hipLaunchKernelGGL((emissionKernelCached<NS, true>), dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables,      \
+                         stride, hminus, lev, spc, groups, L, n, scaling, out, flux, spectrum);                     \
+    else                                                                                                            \
+      hipLaunchKernelGGL((emissionKernelCached<NS, false>), dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables,     \
+                         stride, nullptr, lev, spc, groups, L, n, scaling, out, flux, spectrum);                    \
     break;
     BEAR_CACHED(4)
     BEAR_CACHED(8)
@@ -668,8 +700,8 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
     BEAR_CACHED(16)
 #undef BEAR_CACHED
     default:
-      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, lev, spc, L,
-                         int(species_stride / kChunk), n, scaling, out, flux, spectrum);
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, hminus, lev,
+                         spc, L, int(species_stride / kChunk), n, scaling, out, flux, spectrum);
   }
 }
 }  // namespace
@@ -684,8 +716,8 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
   const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
   const int n = int(n_points_);
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0, nullptr}, buf.flux, spectrum_dev);
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+                 lev, spc, groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(0, 0.0, 0.0, inflation, bins_dev);
@@ -703,8 +735,8 @@ void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio,
   const int n = int(n_points_);
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale, nullptr}, buf.flux, spectrum_dev);
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+                 lev, spc, groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
   launchBins(1, star_temperature, depth_scale, inflation, bins_dev);
@@ -722,8 +754,8 @@ void DeviceEngine::evaluateEclipseTabulatedStar(double radius_ratio, double* spe
   const int n = int(n_points_);
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, lev, spc,
-                 groups_, int(nb_levels_), n, 1.0, SpectrumOut{2, 0.0, depth_scale, star_}, buf.flux, spectrum_dev);
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+                 lev, spc, groups_, int(nb_levels_), n, 1.0, SpectrumOut{2, 0.0, depth_scale, star_}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
 }
@@ -767,9 +799,14 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
   switch (species_stride_) {
 #define BEAR_TRANSMISSION(NS)                                                                                       \
   case NS:                                                                                                          \
-    hipLaunchKernelGGL(transmissionKernel<NS>, dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_, stride_,    \
-                       rayleigh_, lev, spc, groups_, path, L, n, tile, bottom_radius, star_radius, buf.flux,       \
-                       spectrum_dev);                                                                               \
+    if (hminus_)                                                                                                    \
+      hipLaunchKernelGGL((transmissionKernel<NS, true>), dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_,  \
+                         stride_, rayleigh_, hminus_, lev, spc, groups_, path, L, n, tile, bottom_radius,          \
+                         star_radius, buf.flux, spectrum_dev);                                                      \
+    else                                                                                                            \
+      hipLaunchKernelGGL((transmissionKernel<NS, false>), dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_, \
+                         stride_, rayleigh_, nullptr, lev, spc, groups_, path, L, n, tile, bottom_radius,          \
+                         star_radius, buf.flux, spectrum_dev);                                                      \
     break;
     BEAR_TRANSMISSION(4)
     BEAR_TRANSMISSION(8)

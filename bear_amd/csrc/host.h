@@ -151,6 +151,10 @@ class OpacitySpecies {
   std::vector<int> cia_partners;
   bool rayleigh = false;
   std::vector<double> rayleigh_cross_section;        // [N], cm^2
+  // the H- continuum (no tables): per wavenumber the bound-free cross
+  // section (cm^2) and the six free-free wavelength polynomials, [7][N]
+  bool hminus = false;
+  std::vector<double> hminus_rows;
   std::vector<std::vector<TableRow>> by_temperature;  // ascending T, each ascending p
 
   bool hasTable() const { return !by_temperature.empty(); }
@@ -195,6 +199,10 @@ TableSource scanSpeciesTable(OpacitySpecies& s, const std::string& cross_section
 // (cross_section_file.cpp:33-62, sampled_data.cpp:48-87)
 void sampleSpeciesTable(const TableSource& src, const SpectralGrid& grid, std::vector<double>& out);
 OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& folder, const SpectralGrid& grid);
+// the H- continuum's per-level coefficients of those rows: [0] the H-
+// density, [1..6] 1e-29 (5040/T)^((i+1)/2) p_e n_H (h_m.cpp:38-53, 86-137);
+// false when the level has no H- (the reference then skips the species)
+bool hminusCoefficients(double temperature, double n_hminus, double n_h, double n_e, double (&c)[7]);
 
 // ------------------------------------------------------------------ atmosphere
 // Piecewise Lagrange polynomial over log10(p) (additional/piecewise_poly.cpp:49-229)

@@ -52,6 +52,60 @@ void clearRegisteredTables() {
 // ---------------------------------------------------------------- species
 // TransportCoefficients::addOpacitySpecies (transport_coeff.cpp:98-237):
 // CIA pairs, H2 Rayleigh and generic line absorbers
+// The H- continuum (h_m.cpp, T. L. John 1988, A&A 193, 189): the bound-free
+// photo-detachment cross section up to the 1.6419 micron threshold
+// (h_m.cpp:56-82) and, per wavenumber, the six wavelength polynomials
+// lam^2 A_i + B_i + C_i / lam + ... + F_i / lam^4 of the free-free fit
+// (h_m.cpp:86-137: one coefficient set above 0.3645 micron, another down to
+// 0.1823 micron), which the kernels weight per level with
+// 1e-29 (5040 / T)^((i+1)/2) p_e n_H.
+namespace {
+void hminusRows(const SpectralGrid& grid, std::vector<double>& rows) {
+  const double lambda_0 = 1.6419;
+  const double cb[7] = {0.0, 152.519, 49.534, -118.858, 92.536, -34.194, 4.982};
+  // [set][coefficient A..F][i]
+  const double ff[2][6][7] = {
+      {{0.0, 0.0, 2483.3460, -3449.8890, 2200.0400, -696.2710, 88.2830},
+       {0.0, 0.0, 285.8270, -1158.3820, 2427.7190, -1841.4000, 444.5170},
+       {0.0, 0.0, -2054.2910, 8746.5230, -13651.1050, 8624.9700, -1863.8650},
+       {0.0, 0.0, 2827.7760, -11485.6320, 16755.5240, -10051.5300, 2095.2880},
+       {0.0, 0.0, -1341.5370, 5303.6090, -7510.4940, 4400.0670, -901.7880},
+       {0.0, 0.0, 208.9520, -812.9390, 1132.7380, -655.0200, 132.9850}},
+      {{0.0, 518.1021, 473.2636, -482.2089, 115.5291, 0.0, 0.0},
+       {0.0, -734.8666, 1443.4137, -737.1616, 169.6374, 0.0, 0.0},
+       {0.0, 1021.1775, -1977.3395, 1096.8827, -245.6490, 0.0, 0.0},
+       {0.0, -479.0721, 922.3575, -521.1341, 114.2430, 0.0, 0.0},
+       {0.0, 93.1373, -178.9275, 101.7963, -21.9972, 0.0, 0.0},
+       {0.0, -6.4285, 12.3600, -7.0571, 1.5097, 0.0, 0.0}}};
+  const size_t n = grid.size();
+  rows.assign(7 * n, 0.0);
+  for (size_t j = 0; j < n; ++j) {
+    const double lam = grid.wavelength[j];
+    if (lam <= lambda_0 && lam >= 0.125) {
+      const double x = 1.0 / lam - 1.0 / lambda_0;
+      double f = 0;
+      for (int i = 1; i < 7; ++i) f += cb[i] * std::pow(x, (i - 1) / 2.0);
+      rows[j] = 1e-18 * lam * lam * lam * std::pow(x, 1.5) * f;
+    }
+    const int set = lam > 0.3645 ? 0 : lam >= 0.1823 ? 1 : -1;
+    if (set < 0) continue;
+    const double(&c)[6][7] = ff[set];
+    for (int i = 1; i < 7; ++i)
+      rows[size_t(i) * n + j] = lam * lam * c[0][i] + c[1][i] + c[2][i] / lam + c[3][i] / lam / lam +
+                                c[4][i] / lam / lam / lam + c[5][i] / lam / lam / lam / lam;
+  }
+}
+}  // namespace
+
+bool hminusCoefficients(double temperature, double n_hminus, double n_h, double n_e, double (&c)[7]) {
+  for (double& v : c) v = 0.0;
+  if (n_hminus == 0) return false;                   // opacity_species.cpp:407-412
+  const double pe_nh = n_e * phys::boltzmann_k * temperature * n_h;   // electron pressure x H density
+  c[0] = n_hminus;
+  for (int i = 1; i < 7; ++i) c[i] = 1e-29 * std::pow(5040.0 / temperature, (i + 1) / 2.0) * pe_nh;
+  return true;
+}
+
 OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& folder, const SpectralGrid& grid) {
   OpacitySpecies s;
   s.symbol = symbol;
@@ -72,8 +126,10 @@ OpacitySpecies makeOpacitySpecies(const std::string& symbol, const std::string& 
       const double f = (n * n - 1) / (n * n + 2);
       s.rayleigh_cross_section[j] = 24. * phys::pi * phys::pi * phys::pi * nu * nu * nu * nu / (n_ref * n_ref) * f * f;
     }
-  } else if (symbol == "H-") {
-    raise(kInvalidInput, "the H- continuum is not available in this engine");
+  } else if (symbol == "H-") {                         // transport_coeff.cpp:145-150
+    s.species_id = speciesId("H-");
+    s.hminus = true;
+    hminusRows(grid, s.hminus_rows);
   } else {
     s.species_id = speciesId(symbol);
   }

@@ -16,11 +16,20 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
     : observations_(observations),
       engine_(engine),
       atmosphere_(m.nb_levels, m.bottom_pressure, m.top_pressure, m.temperature, m.chemistry) {
-  std::vector<double> rayleigh;
+  std::vector<double> rayleigh, hminus;
   std::vector<TableSource> sources;
   uint32_t nb_rows = 0;
   for (auto& e : m.opacity) {
     OpacitySpecies s = makeOpacitySpecies(e.first, e.second, grid);
+    if (s.hminus) {                                  // absorption of every model (h_m.cpp:30-54)
+      if (hminus_) raise(kInvalidInput, "the H- continuum is listed twice");
+      hminus_ = true;
+      hminus_ids_[0] = s.species_id;
+      hminus_ids_[1] = sid::H;
+      hminus_ids_[2] = speciesId("e-");
+      hminus = std::move(s.hminus_rows);
+      continue;
+    }
     if (s.rayleigh) {
       // scattering enters the transmission model only: emission's short
       // characteristics take the absorption coefficients alone
@@ -46,6 +55,7 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
   rows.clear();
   rows.shrink_to_fit();
   engine_.uploadRayleigh(rayleigh);
+  engine_.uploadHminus(hminus);
   for (OpacitySpecies& s : species_) s.preparePressures(atmosphere_.pressure);   // fixed pressure grid
   engine_.configure(m.nb_levels, species_.size(), transmission);
   // stencil groups: the species' table grids, at most two distinct ones
@@ -83,6 +93,9 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     lev[l].inv_temperature = 1.0 / t;
     lev[l].altitude = atm.altitude[l];
     lev[l].rayleigh_density = n_r;
+    if (hminus_)                                    // H-, H and electron densities
+      hminusCoefficients(t, atm.nd(l, hminus_ids_[0]), atm.nd(l, hminus_ids_[1]), atm.nd(l, hminus_ids_[2]),
+                         lev[l].hminus_c);
     for (int g = 0; g < 2; ++g) {
       if (group_rep_[g] < 0) continue;
       const Stencil st = species_[size_t(group_rep_[g])].bracketLevel(l, t);

@@ -111,6 +111,8 @@ class TabulatedModel : public ForwardModel {
   std::vector<std::unique_ptr<Atmosphere>> atmosphere_pool_;   // one per staging thread
   std::vector<OpacitySpecies> species_;     // tabulated absorbers
   int rayleigh_species_ = -1;               // species id of the Rayleigh scatterer, -1: none
+  bool hminus_ = false;                     // the H- continuum is listed
+  int hminus_ids_[3] = {-1, -1, -1};        // species ids of H-, H and e-
   int group_rep_[2] = {-1, -1};             // a species of each stencil group (StencilGroups)
   size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
 };

@@ -98,6 +98,7 @@ class Case:
     # GlobalConfig::spectral_disecretisation: 0 const_wavenumber (spectral
     # resolution in cm^-1), 1 const_wavelength (micron), 2 const_resolution (R)
     discretisation: int = 0
+    continuum: list = field(default_factory=list)   # ["H-"]: species without tables
 
     @property
     def opacity_entries(self):
@@ -107,6 +108,8 @@ class Case:
             out.append((s.symbol, s.folder))
         for r in self.rayleigh:
             out.append((r, "Rayleigh"))
+        for c in self.continuum:
+            out.append((c, "Continuum"))
         return out
 
 
@@ -589,7 +592,97 @@ def resampled(case: Case, discretisation, resolution, name):
     return case
 
 
+def hminus_case(seed=23):
+    """The H- continuum (bound-free below 1.6419 micron, free-free at every
+    wavelength above 0.1823 micron) with H2O and H2-H2 CIA: a grid from
+    0.5 to 5 micron (2,000-20,000 cm^-1 at 4 cm^-1), iso chemistry of H2O,
+    H-, H and e- over the H2/He background, 16 levels, spectroscopy and
+    band spectroscopy across the bound-free edge."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(2000.0, 20000.0, 4.0)
+    temps = np.array([1000.0, 1500.0, 2000.0, 2500.0, 3000.0])
+    press = 10.0 ** np.arange(-5.0, 2.5, 2.5)
+    species = [line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=300, mass=MOLECULAR_WEIGHT["H2O"]),
+               cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps)]
+    scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
+    wl = np.linspace(0.6, 4.5, 40)
+    d, e = noisy(rng, blackbody_flux_lambda(wl, 2200.0, scale))
+    obs = [ObservationSpec("spec", "spectroscopy", wavelengths=wl, data=d, error=e)]
+    edges = np.linspace(4.9, 0.55, 12)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    d, e = noisy(rng, blackbody_flux_lambda(be.mean(axis=1), 2200.0, scale))
+    obs.append(ObservationSpec("bands", "band-spectroscopy", bin_edges=be, data=d, error=e))
+    log_g = rng.uniform(4.5, 5.2)
+    # H2O, H-, H, e- (the electrons and H- of a ~2000-3000 K atmosphere)
+    mixing = np.array([10 ** rng.uniform(-4, -3), 10 ** rng.uniform(-9.5, -8.5), 10 ** rng.uniform(-3, -2),
+                       10 ** rng.uniform(-6, -5)])
+    theta = poly_temperature_params(rng, 7)
+    theta[0] = 2900.0                               # hot enough for the continuum to matter
+    params = np.concatenate([[log_g, 1.0, 10.0 * PARSEC], mixing, theta])
+    case = Case("hminus", nu, 4.0, 16, 100.0, 1e-5, ("poly", 6, 1),
+                [("iso", ["H2O", "H-", "H", "e-"]), ("bg", ["H2He"])], species, [], obs, params)
+    case.continuum = ["H-"]
+    return case
+
+
+def transmission_hminus_case(seed=29):
+    """Transmission through a 2,800 K isothermal atmosphere with the H-
+    continuum: the grid of hminus_case (0.5-5 micron at 4 cm^-1), H2O, H2-H2
+    CIA and H2 Rayleigh, iso chemistry of H2O, H-, H and e-, 30 levels
+    1e2..1e-6 bar, two band-spectroscopy instruments with line-spread
+    functions, one each side of the bound-free edge."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(2000.0, 20000.0, 4.0)
+    temps = np.array([1000.0, 1500.0, 2000.0, 2500.0, 3000.0])
+    press = 10.0 ** np.arange(-6.0, 2.5, 2.0)
+    species = [line_table(rng, "H2O", "H2O", nu, temps, press, n_lines=300, mass=MOLECULAR_WEIGHT["H2O"]),
+               cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps)]
+    obs = []
+    for k, (lo, hi, nb, r) in enumerate([(0.65, 1.55, 18, 100.0), (1.8, 4.4, 26, 150.0)]):
+        be, d, e, fw = _transit_data(lo, hi, nb, r, rng)
+        obs.append(ObservationSpec(f"instrument{k}", "band-spectroscopy", bin_edges=be, data=d, error=e, fwhm=fw))
+    log_g = rng.uniform(2.8, 3.4)
+    radius = RADIUS_JUPITER * rng.uniform(1.2, 1.4)
+    mixing = np.array([10 ** rng.uniform(-4, -3), 10 ** rng.uniform(-9.5, -8.5), 10 ** rng.uniform(-3, -2),
+                       10 ** rng.uniform(-6, -5)])
+    params = np.concatenate([[log_g, radius, RADIUS_SUN], mixing, [2800.0]])
+    case = Case("transmission_hminus", nu, 4.0, 30, 100.0, 1e-6, ("const",),
+                [("iso", ["H2O", "H-", "H", "e-"]), ("bg", ["H2He"])], species, ["H2"], obs, params,
+                model="transmission", extra={"variant": "plain"})
+    case.continuum = ["H-"]
+    return case
+
+
+MANY_ABSORBERS = ABSORBERS + ["C2H2", "Na", "SH", "OH", "MgO", "AlO", "CaO", "SiO"]
+
+
+def many_species_case(seed=19):
+    """Emission with 20 absorbers + H2-H2/H2-He CIA (22 tabulated species,
+    beyond the 16 stencil slots of the register-cached kernels: the generic
+    single-point kernel), small tables, 20 levels, 2,001 wavenumbers,
+    spectroscopy and band spectroscopy."""
+    rng = np.random.default_rng(seed)
+    nu = _grid(2500.0, 6500.0, 2.0)
+    temps = np.array([400.0, 900.0, 1400.0, 1900.0, 2400.0])
+    press = 10.0 ** np.arange(-5.0, 2.5, 2.5)
+    species = [line_table(rng, s, s, nu, temps, press, n_lines=150) for s in MANY_ABSORBERS]
+    species += [cia_table(rng, "CIA-H2-H2", "CIA/H2-H2", nu, temps),
+                cia_table(rng, "CIA-H2-He", "CIA/H2-He", nu, temps)]
+    scale = (RADIUS_JUPITER / (10 * PARSEC)) ** 2
+    wl = np.linspace(1.6, 3.9, 30)
+    d, e = noisy(rng, blackbody_flux_lambda(wl, 1300.0, scale))
+    obs = [ObservationSpec("spec", "spectroscopy", wavelengths=wl, data=d, error=e)]
+    edges = np.linspace(3.95, 1.6, 8)
+    be = np.stack([edges[:-1], edges[1:]], axis=1)
+    d, e = noisy(rng, blackbody_flux_lambda(be.mean(axis=1), 1300.0, scale))
+    obs.append(ObservationSpec("bands", "band-spectroscopy", bin_edges=be, data=d, error=e))
+    params = emission_parameters(rng, len(MANY_ABSORBERS), 7)
+    return Case("many_species", nu, 2.0, 20, 100.0, 1e-5, ("poly", 6, 1),
+                [("iso", list(MANY_ABSORBERS)), ("bg", ["H2He"])], species, ["H2"], obs, params)
+
+
 CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_c2,
+         "many_species": many_species_case, "hminus": hminus_case, "transmission_hminus": transmission_hminus_case,
          # tiny at constant wavelength steps of 2 nm and at R = 1500
          "tiny_const_wavelength": lambda: resampled(tiny_case(), 1, 2e-3, "tiny_const_wavelength"),
          "tiny_const_resolution": lambda: resampled(tiny_case(), 2, 1500.0, "tiny_const_resolution"),

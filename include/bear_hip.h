@@ -266,12 +266,14 @@ int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, s
  * per live point, plus per-point partial sums; transmission: the same 12
  * bytes per level and spectral point for the extinction and the tangent
  * paths' adjoints, plus 8 bytes x (levels - 1) x (17 (levels - 1) + the
- * levels - 1 rounded up to 16) per point).
+ * levels - 1 rounded up to 32) per point).
  * Batches are processed in chunks of as many points as fit; with 0, or when
- * a single point does not fit, the backward kernel recomputes the forward
- * pass instead (emission and eclipse; the transmission gradient then
- * returns BEAR_ERROR_INVALID_INPUT).  Default 96 GiB, never more than 80 %
- * of the free memory.
+ * a single point does not fit, the emission and eclipse backward kernels
+ * recompute the forward pass instead, and the transmission gradient takes
+ * one point at a time with its spectral grid in windows (at least 1/16 of
+ * the grid each, the forward rerun per window; the same result bit for
+ * bit), returning BEAR_ERROR_INVALID_INPUT when not even that fits.
+ * Default 96 GiB, never more than 80 % of the free memory.
  * No counterpart in the reference. */
 int bear_retrieval_set_adjoint_memory(bear_retrieval* r, size_t bytes);
 

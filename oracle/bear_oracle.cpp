@@ -575,6 +575,8 @@ struct OpacitySpecies {
   int index = 0;
   std::vector<int> partners;
   bool rayleigh = false;
+  bool hminus = false;                            // the H- continuum (GasHm, no tables)
+  std::vector<double> hminus_bf;                  // bound-free cross section per wavenumber
   double mass = 0;
   bool log_storage = false;
   std::vector<Row> rows;
@@ -714,6 +716,44 @@ struct OpacitySpecies {
 double rayleighGeneral(double ref_density, double n_refr, double king, double nu) {
   return 24. * std::pow(kPi, 3) * std::pow(nu, 4) / (ref_density * ref_density)
          * std::pow((n_refr * n_refr - 1) / (n_refr * n_refr + 2), 2) * king;
+}
+
+// The H- continuum (h_m.cpp:38-137, John 1988 fits): the bound-free
+// photo-detachment cross section (cm^2) at wavelength lam (micron) and the
+// free-free absorption per H atom and unit electron pressure (cm^4 / dyne)
+// at temperature t.
+double hminusBoundFree(double lam) {                        // h_m.cpp:56-82
+  const double lambda_0 = 1.6419;
+  const double c[7] = {0.0, 152.519, 49.534, -118.858, 92.536, -34.194, 4.982};
+  if (!(lam <= lambda_0 && lam >= 0.125)) return 0.0;
+  double f = 0;
+  for (unsigned i = 1; i < 7; ++i) f += c[i] * std::pow(1.0 / lam - 1.0 / lambda_0, (i - 1) / 2.0);
+  return 1e-18 * lam * lam * lam * std::pow(1.0 / lam - 1.0 / lambda_0, 1.5) * f;
+}
+
+double hminusFreeFree(double lam, double t) {              // h_m.cpp:86-137
+  static const double a1[7] = {0.0, 0.0, 2483.3460, -3449.8890, 2200.0400, -696.2710, 88.2830};
+  static const double b1[7] = {0.0, 0.0, 285.8270, -1158.3820, 2427.7190, -1841.4000, 444.5170};
+  static const double c1[7] = {0.0, 0.0, -2054.2910, 8746.5230, -13651.1050, 8624.9700, -1863.8650};
+  static const double d1[7] = {0.0, 0.0, 2827.7760, -11485.6320, 16755.5240, -10051.5300, 2095.2880};
+  static const double e1[7] = {0.0, 0.0, -1341.5370, 5303.6090, -7510.4940, 4400.0670, -901.7880};
+  static const double f1[7] = {0.0, 0.0, 208.9520, -812.9390, 1132.7380, -655.0200, 132.9850};
+  static const double a2[7] = {0.0, 518.1021, 473.2636, -482.2089, 115.5291, 0.0, 0.0};
+  static const double b2[7] = {0.0, -734.8666, 1443.4137, -737.1616, 169.6374, 0.0, 0.0};
+  static const double c2[7] = {0.0, 1021.1775, -1977.3395, 1096.8827, -245.6490, 0.0, 0.0};
+  static const double d2[7] = {0.0, -479.0721, 922.3575, -521.1341, 114.2430, 0.0, 0.0};
+  static const double e2[7] = {0.0, 93.1373, -178.9275, 101.7963, -21.9972, 0.0, 0.0};
+  static const double f2[7] = {0.0, -6.4285, 12.3600, -7.0571, 1.5097, 0.0, 0.0};
+  const double *a, *b, *c, *d, *e, *f;
+  if (lam > 0.3645) { a = a1; b = b1; c = c1; d = d1; e = e1; f = f1; }
+  else if (lam >= 0.1823) { a = a2; b = b2; c = c2; d = d2; e = e2; f = f2; }
+  else return 0.0;
+  double x = 0;
+  for (unsigned i = 1; i < 7; ++i)
+    x += std::pow(5040.0 / t, (i + 1) / 2.0) *
+         (lam * lam * a[i] + b[i] + c[i] / lam + d[i] / lam / lam + e[i] / lam / lam / lam +
+          f[i] / lam / lam / lam / lam);
+  return x * 1e-29;
 }
 
 // One tangent path's optical depth (transmission_spectrum.cpp:110-140):
@@ -939,8 +979,12 @@ struct EmissionOracle {
       else if (e.first == "CIA-H2-He") { s.index = kH2; s.partners = {kHe}; }
       else if (e.first == "CIA-H-He") { s.index = kH; s.partners = {kHe}; }
       else if (e.first == "H2" && e.second == "Rayleigh") { s.index = kH2; s.rayleigh = true; }
+      else if (e.first == "H-") { s.index = speciesId("H-"); s.hminus = true; }   // transport_coeff.cpp:145-150
       else s.index = speciesId(e.first);
-      if (s.rayleigh) {
+      if (s.hminus) {
+        s.hminus_bf.resize(grid.nu.size());
+        for (size_t j = 0; j < grid.nu.size(); ++j) s.hminus_bf[j] = hminusBoundFree(grid.lam[j]);
+      } else if (s.rayleigh) {
         s.rayleigh_sigma.resize(grid.nu.size());
         for (size_t j = 0; j < grid.nu.size(); ++j) {
           const double n = (13.58e-5 * (1. + 7.52e-3 / std::pow(grid.lam[j], 2))) + 1.;
@@ -1131,6 +1175,12 @@ struct EmissionOracle {
         }
         if (s.rayleigh)
           for (size_t i = 0; i < N; ++i) scattering[l][i] += s.rayleigh_sigma[i] * nd[l][s.index];
+        if (s.hminus) {                            // h_m.cpp:38-53
+          const double pe = nd[l][speciesId("e-")] * kBoltzmannK * temperature[l];
+          const double nh = nd[l][kH];
+          for (size_t i = 0; i < N; ++i)
+            kappa[l][i] += s.hminus_bf[i] * n + hminusFreeFree(grid.lam[i], temperature[l]) * pe * nh;
+        }
       }
     }
   }

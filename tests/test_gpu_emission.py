@@ -42,7 +42,7 @@ def test_grid_matches_oracle(case_factory, name):
 
 
 @pytest.mark.parametrize("name", ["tiny", "emission_c1", "emission_wide", "tiny_const_wavelength",
-                                  "tiny_const_resolution"])
+                                  "tiny_const_resolution", "many_species", "hminus"])
 def test_emission_parity(case_factory, name):
     r, o, params, _ = retrieval(case_factory, name)
     for k, p in enumerate(params):
@@ -135,3 +135,35 @@ def test_first_evaluation_after_setup(case_factory):
         rel = np.max(np.abs(out.spectrum - ref["spectrum"]) / np.abs(ref["spectrum"]))
         assert rel <= SPECTRUM_RTOL, (k, rel)
         r.close()
+
+
+def test_many_species_limits(case_factory):
+    """22 tabulated species run the generic single-point kernel (parity in
+    test_emission_parity); the register-cached batched and adjoint kernels
+    hold at most 16 stencil slots and refuse the case with InvalidInput."""
+    r, o, params, _ = retrieval(case_factory, "many_species")
+    assert r.stats()["nb_species"] == 22
+    pts = np.array(params[:2])
+    with pytest.raises(api.BearError) as e:
+        r.computeLikelihoodBatch(pts)
+    assert e.value.code == -1 and "16" in str(e.value)
+    with pytest.raises(api.BearError) as e:
+        r.gradientBatch(pts)
+    assert e.value.code == -1
+    # the single-point path still works after the refusals
+    assert abs(r.computeLikelihood(params[0]) - o.evaluate(params[0])["loglike"]) <= LOGLIKE_ATOL
+
+
+def test_hminus_limits(case_factory):
+    """The H- continuum (rows per wavenumber, weights per level) enters the
+    single-point kernels (parity in test_emission_parity); the batched and
+    adjoint kernels do not take it and refuse the case with InvalidInput."""
+    r, o, params, _ = retrieval(case_factory, "hminus")
+    pts = np.array(params[:2])
+    with pytest.raises(api.BearError) as e:
+        r.computeLikelihoodBatch(pts)
+    assert e.value.code == -1 and "H-" in str(e.value)
+    with pytest.raises(api.BearError) as e:
+        r.gradientBatch(pts)
+    assert e.value.code == -1 and "H-" in str(e.value)
+    assert abs(r.computeLikelihood(params[0]) - o.evaluate(params[0])["loglike"]) <= LOGLIKE_ATOL

@@ -170,6 +170,15 @@ def test_autograd_function_backward(case_factory, tmp_path_factory):
     np.testing.assert_allclose(x.grad.numpy(), w.numpy()[:, None] * grad_ref, rtol=1e-15, atol=0)
     plain = log_likelihood(r, torch.tensor(pts, dtype=torch.float64))
     np.testing.assert_array_equal(plain.numpy(), r.computeLikelihoodBatch(pts))
+    # lazy: the forward is the plain batch, the adjoint runs in backward
+    y = torch.tensor(pts, dtype=torch.float64, requires_grad=True)
+    ll_lazy = log_likelihood(r, y, gradient="lazy")
+    np.testing.assert_array_equal(ll_lazy.detach().numpy(), r.computeLikelihoodBatch(pts))
+    (ll_lazy * w).sum().backward()
+    np.testing.assert_allclose(y.grad.numpy(), w.numpy()[:, None] * grad_ref, rtol=1e-15, atol=0)
+    z = torch.tensor(pts[0], dtype=torch.float64, requires_grad=True)
+    log_likelihood(r, z).backward()
+    np.testing.assert_allclose(z.grad.numpy(), grad_ref[0], rtol=1e-12, atol=0)
 
 
 def test_gradient_refuses_other_models_and_precisions(case_factory, tmp_path_factory):
@@ -211,6 +220,47 @@ def test_transmission_whole_and_chunked_batches_agree(case_factory, tmp_path_fac
         r.set_adjoint_memory(96 << 30)
     np.testing.assert_array_equal(ll_c, ll_s)
     np.testing.assert_array_equal(g_c, g_s)
+
+
+def test_transmission_windows_match_the_whole_grid(case_factory, tmp_path_factory):
+    """An adjoint memory below one point's kept forward quantities: one
+    point at a time, its grid in windows of three chord ranges, the forward
+    kernel rerun per window.  The same sums in the same order: the gradients
+    equal the whole-grid path's bit for bit; below one range, an error."""
+    folder, params, r = gradient_case(case_factory, tmp_path_factory, "transmission_plain")
+    r.set_table_precision("fp64")
+    pts = np.array(params)
+    st = r.stats()
+    n, L = st["nb_spectral_points"], st["nb_levels"]
+    stride = (n + 63) // 64 * 64
+    tiles = (n + 63) // 64
+    slots = (st["nb_species"] + 3) // 4 * 4 + 4
+    P = L - 1
+    rows = (P + 31) // 32 * 32
+    ranges = max(1, min(16, (n + 511) // 512))
+    while True:                                          # as the engine: no empty range, multiples of 128
+        npr = -(-(-(-n // ranges)) // 128) * 128
+        if ranges == 1 or (ranges - 1) * npr < n:
+            break
+        ranges -= 1
+    assert ranges >= 4
+    fixed = 8 * (L * slots * tiles + stride + 17 * P * P + P * rows)
+    kept_range = 4 * L * 3 * npr
+    try:
+        ll_s, g_s = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == len(pts)
+        r.set_adjoint_memory(fixed + 3 * kept_range + kept_range // 2)
+        print(f"[gpu] transmission windows: {ranges} ranges of {npr} wavenumbers, 3 per window", flush=True)
+        ll_w, g_w = r.gradientBatch(pts)
+        assert r.batch_stats()["adjoint_chunk"] == 1
+        np.testing.assert_array_equal(ll_w, ll_s)
+        np.testing.assert_array_equal(g_w, g_s)
+        r.set_adjoint_memory(fixed + kept_range // 2)
+        with pytest.raises(api.BearError) as e:
+            r.gradientBatch(pts)
+        assert e.value.code == -1 and "adjoint memory" in str(e.value)
+    finally:
+        r.set_adjoint_memory(96 << 30)
 
 
 def test_neglected_point_has_zero_gradient(case_factory, tmp_path_factory):

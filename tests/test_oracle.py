@@ -22,7 +22,7 @@ def _golden(name):
 
 ALL_CASES = ["tiny", "emission_c1", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
              "transmission_fit_mmw", "transmission_fit_scale_height", "tiny_const_wavelength",
-             "tiny_const_resolution"]
+             "tiny_const_resolution", "many_species", "hminus", "transmission_hminus"]
 
 
 @pytest.mark.parametrize("name", ALL_CASES)
@@ -73,7 +73,8 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
 
 @pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
 @pytest.mark.parametrize("name", ["tiny", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
-                                  "tiny_const_wavelength", "tiny_const_resolution"])
+                                  "tiny_const_wavelength", "tiny_const_resolution", "hminus", "many_species",
+                                  "transmission_hminus"])
 def test_oracle_intermediates_match_live_reference(case_factory, tmp_path, name):
     folder, params = case_factory(name)
     ref = api.run_reference(folder, str(tmp_path / "ref"))
