@@ -41,7 +41,8 @@ class _ModelConfig(C.Structure):
                 ("atmos_top_pressure", C.c_double), ("temperature_profile", C.c_char_p),
                 ("radiative_transfer", C.c_char_p), ("chemistry", C.c_char_p), ("opacity_species", C.c_char_p),
                 ("fit_mean_molecular_weight", C.c_int), ("fit_scale_height", C.c_int),
-                ("use_variable_gravity", C.c_int), ("stellar_spectrum_model", C.c_char_p)]
+                ("use_variable_gravity", C.c_int), ("stellar_spectrum_model", C.c_char_p),
+                ("cloud_model", C.c_char_p)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -181,16 +182,17 @@ class ModelConfig:
     fit_scale_height: bool = False           # transmission
     use_variable_gravity: bool = False       # transmission
     stellar_spectrum_model: str = "blackbody"  # secondary eclipse
+    cloud_model: list = field(default_factory=list)   # ["grey", "power_law 1.0 fb", "KHnongrey 1.0"]
 
     def _c(self):
         chem = "\n".join(self.chemistry)
         opac = "\n".join(f"{s} {f}" for s, f in self.opacity_species)
         self._keep = [_b(self.temperature_profile), _b(self.radiative_transfer), _b(chem), _b(opac),
-                      _b(self.stellar_spectrum_model)]
+                      _b(self.stellar_spectrum_model), _b("\n".join(self.cloud_model))]
         return _ModelConfig(int(self.nb_grid_points), float(self.atmos_bottom_pressure),
                             float(self.atmos_top_pressure), self._keep[0], self._keep[1], self._keep[2], self._keep[3],
                             int(self.fit_mean_molecular_weight), int(self.fit_scale_height),
-                            int(self.use_variable_gravity), self._keep[4])
+                            int(self.use_variable_gravity), self._keep[4], self._keep[5])
 
 
 @dataclass

@@ -911,6 +911,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
   if (hminus_) raise(kInvalidInput, "the adjoint does not take the H- continuum");
+  if (cloud_models_ > 0) raise(kInvalidInput, "the adjoint does not take cloud layers");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
   const int NS = int(species_stride_);
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");

@@ -830,6 +830,7 @@ void DeviceEngine::releaseGraph() {
 void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
   if (species_stride_ > 16) raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
   if (hminus_) raise(kInvalidInput, "the H- continuum is computed by the single-point kernels only");
+  if (cloud_models_ > 0) raise(kInvalidInput, "cloud layers are computed by the single-point kernels only");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "batch larger than the staged records");
   if ((kind == ModelKind::transmission) != transmission_) raise(kInvalidInput, "model kind does not match the engine");
   check(hipSetDevice(device_), "hipSetDevice");

@@ -81,11 +81,10 @@ std::vector<int> structureSpecies(const bear_spectrum_model* m, const double* pr
   if (nb_levels != m->impl->nbLevels())
     bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "expected " + std::to_string(m->impl->nbLevels()) + " levels, got " +
                                                   std::to_string(nb_levels));
-  if (cloud) {                                     // emission.cpp:344-360: an all-zero cloud is no cloud
-    const size_t n = (nb_levels - 1) * m->impl->grid().size();
-    for (size_t i = 0; i < n; ++i)
-      if (cloud[i] > 0) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "cloud optical depths are not available");
-  }
+  // EmissionModel::setCloudProperties and its transmission and eclipse
+  // forms (emission.cpp:344-367): the caller's layer optical depths, used
+  // when any is positive
+  m->impl->engine().setFixedCloud(cloud);
   std::vector<int> ids;
   for (size_t j = 0; j < nb_species; ++j) ids.push_back(bear_amd::speciesIdOrFastChem(str(species[j])));
   return ids;
@@ -136,6 +135,7 @@ int bear_retrieval_create(const bear_global_config* config, const char* forward_
     m.use_variable_gravity = model_config->use_variable_gravity != 0;
     if (model_config->stellar_spectrum_model) m.stellar_model = model_config->stellar_spectrum_model;
     m.opacity = opacityEntries(model_config->opacity_species);
+    m.clouds = bear_amd::parseCloudModels(str(model_config->cloud_model));
 
     std::vector<bear_amd::ObservationInputView> inputs;
     for (size_t i = 0; i < nb_observations; ++i) {

@@ -194,18 +194,50 @@ __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, siz
 
 // The H- continuum (h_m.cpp:38-53, 56-82, 86-137): per wavenumber the
 // bound-free cross-section and the six free-free wavelength polynomials
-// (rows [7][n]), per level their weights LevelState::hminus_c, so
-// kappa_H- = c0 sigma_bf + sum_i c_i P_i.
-__device__ __forceinline__ void loadHminus(const double* __restrict__ rows, int i, int n, double (&hm)[7]) {
-#pragma unroll
-  for (int r = 0; r < 7; ++r) hm[r] = rows[size_t(r) * n + i];
-}
-
+// (rows [7][n], read once per thread by loadExtras), per level their weights
+// LevelState::hminus_c, so kappa_H- = c0 sigma_bf + sum_i c_i P_i.
 __device__ __forceinline__ double hminusKappa(const double (&hm)[7], const LevelState& lev) {
   double k = lev.hminus_c[0] * hm[0];
 #pragma unroll
   for (int r = 1; r < 7; ++r) k = fma(lev.hminus_c[r], hm[r], k);
   return k;
+}
+
+// per thread, once: the H- rows and the cloud models' spectral factors at
+// wavelength lam (micron) (power_law_cloud_model.cpp:114-120,
+// kh_cloud_model.cpp:116-132); unused slots are zero
+__device__ __forceinline__ void loadExtras(const Extras& ex, int i, int n, double lam, double (&hm)[7],
+                                           double (&cf)[kMaxClouds]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) hm[r] = ex.hminus ? ex.hminus[size_t(r) * n + i] : 0.0;
+#pragma unroll
+  for (int m = 0; m < kMaxClouds; ++m) {
+    double f = 0.0;
+    if (m < ex.clouds.count) {
+      const double* c = ex.clouds.c[m];
+      if (ex.clouds.kind[m] == kCloudPowerLaw) {
+        f = pow(lam, c[0]) * c[1];
+      } else if (ex.clouds.kind[m] == kCloudKH) {
+        const double x = c[2] / lam;
+        f = c[3] / (c[0] * pow(x, -c[1]) + pow(x, 0.2));
+      } else {
+        f = 1.0;
+      }
+    }
+    cf[m] = f;
+  }
+}
+
+// the clouds' optical depth in the layer below level l (l >= 1): the cloud
+// models' weights times their factors, plus the caller-given depth
+// (opacity_calc.h:221-237 sums them)
+__device__ __forceinline__ double cloudLayer(const Extras& ex, const double (&cf)[kMaxClouds], const LevelState& lev,
+                                             int l, int i, int n) {
+  double t = 0.0;
+#pragma unroll
+  for (int m = 0; m < kMaxClouds; ++m) t = fma(lev.cloud_w[m], cf[m], t);
+  if (ex.cloud_tau) t += ex.cloud_tau[size_t(l - 1) * n + i];
+  return t;
 }
 
 // The same without a register cache, for any number of species: chunks of

@@ -11,6 +11,29 @@
 
 namespace bear_amd {
 
+// Cloud layers (cloud_model/*.cpp): at most kMaxClouds parametric cloud
+// models per forward model.  Model m's optical depth in a layer is its
+// per-layer weight (LevelState::cloud_w) times a spectral factor f_m(lambda)
+// that the kernels form per wavenumber: 1 (grey), lambda^c0 c1 (power law:
+// c0 the exponent, c1 = lambda_ref^-c0), c3 / (c0 x^-c1 + x^0.2) with
+// x = c2 / lambda (KHnongrey: q0, a0, 2 pi a, the reference efficiency).
+constexpr int kMaxClouds = 4;
+enum CloudKind : int { kCloudGrey = 0, kCloudKH = 1, kCloudPowerLaw = 2 };
+struct CloudSpec {
+  int count = 0;
+  int kind[kMaxClouds] = {};
+  double c[kMaxClouds][4] = {};
+};
+
+// What the single-point kernels add to the tabulated opacity: the H-
+// continuum (rows [7][N], or none), parametric cloud layers and caller-given
+// cloud optical depths [levels - 1][N] (or none).
+struct Extras {
+  const double* hminus;
+  const double* cloud_tau;
+  CloudSpec clouds;
+};
+
 // Per level: temperature and altitude of the model atmosphere, and whether
 // any stencil slot's table rows differ from the previous level's (always at
 // level 0): the kernels then reload every slot's rows at once, so one scalar
@@ -37,6 +60,12 @@ struct LevelState {
   // density, then 1e-29 (5040/T)^((i+1)/2) p_e n_H for i = 1..6 (all zero
   // where the level holds no H-)
   double hminus_c[7];
+  // clouds: per cloud model its optical-depth weight in the layer below this
+  // level (zero at level 0 and outside the cloud), and the factor that turns
+  // that layer's cloud optical depth into this level's term: 1 for emission,
+  // 1 / (z_l - z_{l-1}) for transmission (an extinction, cloud_model.h:67-88)
+  double cloud_w[kMaxClouds];
+  double cloud_scale;
 };
 
 // log2 of a zero effective density in the FP64 stencil records: finite, so
@@ -194,6 +223,13 @@ class DeviceEngine {
   // polynomials); empty: no H-.  The single-point kernels only.
   void uploadHminus(const std::vector<double>& rows);
   bool hasHminus() const { return hminus_ != nullptr; }
+  // parametric cloud models of the configured forward model (their number),
+  // and the spectral factors of the next single-point evaluation
+  void setCloudModels(int count) { cloud_models_ = count; }
+  void setCloudSpec(const CloudSpec& spec) { cloud_spec_ = spec; }
+  // the direct models' caller-given layer optical depths [levels - 1][N]
+  // for the next evaluations (FixedCloudModel); NULL or all zero: none
+  void setFixedCloud(const double* tau);
   void uploadObservations(const std::vector<ObservationSetup>& observations);
   void configure(size_t nb_levels, size_t nb_species, bool transmission);
   void setStencilGroups(StencilGroups g) { groups_ = g; }
@@ -328,6 +364,11 @@ class DeviceEngine {
   double* tables_ = nullptr;
   double* rayleigh_ = nullptr;
   double* hminus_ = nullptr;                     // [7][N], or none
+  int cloud_models_ = 0;
+  CloudSpec cloud_spec_;
+  double* cloud_tau_ = nullptr;                  // [L-1][N] caller-given cloud optical depths
+  bool fixed_cloud_ = false;
+  Extras extras() const { return Extras{hminus_, fixed_cloud_ ? cloud_tau_ : nullptr, cloud_spec_}; }
   double* flux_ = nullptr;
   double* spectrum_ = nullptr;
   double* star_ = nullptr;                       // tabulated stellar flux [N], or none

@@ -88,12 +88,14 @@ __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2
 // values of every stencil slot stay in registers from level to level and are
 // reloaded only where the rows change, so each table row is read about once
 // per thread; kappa and the intensities never leave registers.  The stencil
-// fields are wave-uniform (scalar loads).  HM: the H- continuum's seven rows
-// [7][n] (hminus) are read once per thread and weighted per level.
-template <int NS, bool HM>
+// fields are wave-uniform (scalar loads).  EX: the extras (H- continuum,
+// clouds) are formed once per thread and added per level and layer: the H-
+// term to kappa, the cloud optical depth to the layer's
+// (short_characteristics.cpp:75-76).
+template <int NS, bool EX>
 __global__ __launch_bounds__(kBlock) void emissionKernelCached(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
-    const double* __restrict__ tables, size_t stride, const double* __restrict__ hminus,
+    const double* __restrict__ tables, size_t stride, Extras ex,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups groups,
     int nb_levels, int n, double scaling, SpectrumOut out, double* __restrict__ flux, double* __restrict__ spectrum) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -103,20 +105,22 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   double v[NS][4];
-  double hm[7];
-  if (HM) loadHminus(hminus, i, n, hm);
+  double hm[7], cf[kMaxClouds];
+  if (EX) loadExtras(ex, i, n, wavelength[i], hm, cf);
   double k_prev = kappaTerms<NS>(tab, stride, species, levels[0], groups, true, v);
-  if (HM) k_prev += hminusKappa(hm, levels[0]);
+  if (EX && ex.hminus) k_prev += hminusKappa(hm, levels[0]);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
     double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], groups,
                                    levels[l].reload != 0, v);
-    if (HM) k_next += hminusKappa(hm, levels[l]);
+    if (EX && ex.hminus) k_next += hminusKappa(hm, levels[l]);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
-    shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+    double tau = (z_next - z_prev) * (k_next + k_prev) * 0.5;
+    if (EX) tau += cloudLayer(ex, cf, levels[l], l, i, n);
+    shortCharacteristicsLayer(tau, b_prev, b_next, i1, i2);
     k_prev = k_next;
     b_prev = b_next;
     z_prev = z_next;
@@ -124,11 +128,10 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   writeFlux(i, nu, i1, i2, scaling, wavelength, out, flux, spectrum);
 }
 
-// The same for any number of species, without the register cache (hminus:
-// the H- rows, or none).
+// The same for any number of species, without the register cache.
 __global__ __launch_bounds__(kBlock) void emissionKernel(
     const double* __restrict__ wavenumber, const double* __restrict__ wavelength,
-    const double* __restrict__ tables, size_t stride, const double* __restrict__ hminus,
+    const double* __restrict__ tables, size_t stride, Extras ex,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species,
     int nb_levels, int nb_chunks, int n, double scaling, SpectrumOut out,
     double* __restrict__ flux, double* __restrict__ spectrum) {
@@ -139,19 +142,22 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
   const double b_num = kPlanckC1 * nu * nu * nu;
   const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
   const int stride_species = nb_chunks * kChunk;
-  double hm[7];
-  if (hminus) loadHminus(hminus, i, n, hm);
+  const bool clouds = ex.clouds.count > 0 || ex.cloud_tau;
+  double hm[7], cf[kMaxClouds];
+  loadExtras(ex, i, n, wavelength[i], hm, cf);
   double k_prev = chunkedKappa(tab, stride, species, levels[0], nb_chunks);
-  if (hminus) k_prev += hminusKappa(hm, levels[0]);
+  if (ex.hminus) k_prev += hminusKappa(hm, levels[0]);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
     double k_next = chunkedKappa(tab, stride, species + size_t(l) * stride_species, levels[l], nb_chunks);
-    if (hminus) k_next += hminusKappa(hm, levels[l]);
+    if (ex.hminus) k_next += hminusKappa(hm, levels[l]);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
-    shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+    double tau = (z_next - z_prev) * (k_next + k_prev) * 0.5;
+    if (clouds) tau += cloudLayer(ex, cf, levels[l], l, i, n);
+    shortCharacteristicsLayer(tau, b_prev, b_next, i1, i2);
     k_prev = k_next;
     b_prev = b_next;
     z_prev = z_next;
@@ -174,12 +180,13 @@ __global__ __launch_bounds__(kBlock) void emissionKernel(
 //     reference's trapezoid (one-sided at t = 0; T = 1 at the top level);
 //  3. per wavenumber, the partials in a fixed order, h = sqrt(H + R^2) - R and
 //     the depth (R_p + h)^2 / R_*^2 1e6 in FP64.
-// The transposed chord matrix chord[q][t] comes from the host.  HM: the H-
-// continuum's rows [7][n] (hminus) join the absorption.
-template <int NS, bool HM>
+// The transposed chord matrix chord[q][t] comes from the host.  EX: the H-
+// continuum joins the absorption and the clouds' layer optical depths join
+// as level extinction (transmission_spectrum.cpp:121-130).
+template <int NS, bool EX>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* __restrict__ tables, size_t stride, const double* __restrict__ rayleigh,
-    const double* __restrict__ hminus,
+    const double* __restrict__ wavelength, Extras ex,
     const LevelState* __restrict__ levels, const LevelSpecies* __restrict__ species, StencilGroups stencil_groups,
     const double* __restrict__ chord, int nb_levels, int n, int tile, double bottom_radius, double star_radius,
     double* __restrict__ depth, double* __restrict__ spectrum) {
@@ -200,13 +207,16 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionKernel(
     const double* tab = tables + i;
     const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
     double v[NS][4];
-    double hm[7];
-    if (HM) loadHminus(hminus, i, n, hm);
+    double hm[7], cf[kMaxClouds];
+    if (EX) loadExtras(ex, i, n, wavelength[i], hm, cf);
     for (int l = l0; l < l1; ++l) {
       double k = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], stencil_groups,
                                 l == l0 || levels[l].reload != 0, v) +
                  sigma_r * levels[l].rayleigh_density;
-      if (HM) k += hminusKappa(hm, levels[l]);
+      if (EX) {
+        if (ex.hminus) k += hminusKappa(hm, levels[l]);
+        if (l > 0) k += cloudLayer(ex, cf, levels[l], l, i, n) * levels[l].cloud_scale;
+      }
       if (l == l0) part[g * tile + j] = k;
       else S[(l - 1) * tile + j] = last + k;
       last = k;
@@ -361,7 +371,7 @@ DeviceEngine::~DeviceEngine() {
   releaseAdjoint();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)hminus_,
                   (void*)flux_, (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_,
-                  (void*)chunk_sums_, (void*)ll_, (void*)staging_dev_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,
+                  (void*)chunk_sums_, (void*)ll_, (void*)staging_dev_, (void*)cloud_tau_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,
                   (void*)star_})
     deviceFree(p);
   if (staging_host_) (void)hipHostFree(staging_host_);
@@ -529,6 +539,21 @@ void DeviceEngine::uploadHminus(const std::vector<double>& rows) {
   check(hipStreamSynchronize(s), "copy H- continuum rows");
 }
 
+void DeviceEngine::setFixedCloud(const double* tau) {
+  fixed_cloud_ = false;
+  if (!tau || nb_levels_ < 2) return;
+  const size_t count = (nb_levels_ - 1) * n_points_;
+  bool any = false;                               // emission.cpp:344-360: an all-zero cloud is no cloud
+  for (size_t i = 0; i < count && !any; ++i) any = tau[i] > 0;
+  if (!any) return;
+  check(hipSetDevice(device_), "hipSetDevice");
+  if (!cloud_tau_) cloud_tau_ = deviceAlloc<double>(count, "cloud optical depths");
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  check(hipMemcpyAsync(cloud_tau_, tau, count * sizeof(double), hipMemcpyHostToDevice, s), "copy cloud optical depths");
+  check(hipStreamSynchronize(s), "copy cloud optical depths");
+  fixed_cloud_ = true;
+}
+
 void DeviceEngine::uploadObservations(const std::vector<ObservationSetup>& obs) {
   check(hipSetDevice(device_), "hipSetDevice");
   std::vector<BinDesc> desc;
@@ -680,19 +705,21 @@ void DeviceEngine::launchBins(int mode, double star_temperature, double depth_sc
 }
 
 namespace {
+bool hasExtras(const Extras& ex) { return ex.hminus || ex.cloud_tau || ex.clouds.count > 0; }
+
 void launchEmission(hipStream_t s, size_t species_stride, int blocks, const double* nu, const double* lam,
-                    const double* tables, size_t stride, const double* hminus, const LevelState* lev,
+                    const double* tables, size_t stride, const Extras& ex, const LevelState* lev,
                     const LevelSpecies* spc, StencilGroups groups, int L, int n, double scaling, SpectrumOut out,
                     double* flux, double* spectrum) {
   switch (species_stride) {
 #define BEAR_CACHED(NS)                                                                                             \
   case NS:                                                                                                          \
-    if (hminus)                                                                                                     \
+    if (hasExtras(ex))                                                                                              \
       hipLaunchKernelGGL((emissionKernelCached<NS, true>), dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables,      \
-                         stride, hminus, lev, spc, groups, L, n, scaling, out, flux, spectrum);                     \
+                         stride, ex, lev, spc, groups, L, n, scaling, out, flux, spectrum);                         \
     else                                                                                                            \
       hipLaunchKernelGGL((emissionKernelCached<NS, false>), dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables,     \
-                         stride, nullptr, lev, spc, groups, L, n, scaling, out, flux, spectrum);                    \
+                         stride, ex, lev, spc, groups, L, n, scaling, out, flux, spectrum);                         \
     break;
     BEAR_CACHED(4)
     BEAR_CACHED(8)
@@ -700,7 +727,7 @@ void launchEmission(hipStream_t s, size_t species_stride, int blocks, const doub
     BEAR_CACHED(16)
 #undef BEAR_CACHED
     default:
-      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, hminus, lev,
+      hipLaunchKernelGGL(emissionKernel, dim3(blocks), dim3(kBlock), 0, s, nu, lam, tables, stride, ex, lev,
                          spc, L, int(species_stride / kChunk), n, scaling, out, flux, spectrum);
   }
 }
@@ -716,7 +743,7 @@ void DeviceEngine::evaluateEmission(double scaling, double inflation, double* sp
   const LevelSpecies* spc = reinterpret_cast<const LevelSpecies*>(buf.staging + off_species_);
   const int n = int(n_points_);
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, extras(),
                  lev, spc, groups_, int(nb_levels_), n, scaling, SpectrumOut{0, 0.0, 0.0, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
@@ -735,7 +762,7 @@ void DeviceEngine::evaluateEclipse(double star_temperature, double radius_ratio,
   const int n = int(n_points_);
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, extras(),
                  lev, spc, groups_, int(nb_levels_), n, 1.0, SpectrumOut{1, 1.0 / star_temperature, depth_scale, nullptr}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
@@ -754,7 +781,7 @@ void DeviceEngine::evaluateEclipseTabulatedStar(double radius_ratio, double* spe
   const int n = int(n_points_);
   const double depth_scale = radius_ratio * radius_ratio * 1e6;
   beginTimed();
-  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, hminus_,
+  launchEmission(s, species_stride_, (n + kBlock - 1) / kBlock, wavenumber_, wavelength_, tables_, stride_, extras(),
                  lev, spc, groups_, int(nb_levels_), n, 1.0, SpectrumOut{2, 0.0, depth_scale, star_}, buf.flux, spectrum_dev);
   check(hipGetLastError(), "emission kernel");
   endTimed();
@@ -795,17 +822,18 @@ void DeviceEngine::evaluateTransmission(double bottom_radius, double star_radius
   const int tile = transmissionLds(L, 128) <= lds_limit_ ? 128 : 64;
   const size_t lds = transmissionLds(L, tile);
   const int blocks = (n + tile - 1) / tile;
+  const Extras ex = extras();
   beginTimed();
   switch (species_stride_) {
 #define BEAR_TRANSMISSION(NS)                                                                                       \
   case NS:                                                                                                          \
-    if (hminus_)                                                                                                    \
+    if (hasExtras(ex))                                                                                              \
       hipLaunchKernelGGL((transmissionKernel<NS, true>), dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_,  \
-                         stride_, rayleigh_, hminus_, lev, spc, groups_, path, L, n, tile, bottom_radius,          \
+                         stride_, rayleigh_, wavelength_, ex, lev, spc, groups_, path, L, n, tile, bottom_radius,  \
                          star_radius, buf.flux, spectrum_dev);                                                      \
     else                                                                                                            \
       hipLaunchKernelGGL((transmissionKernel<NS, false>), dim3(blocks), dim3(kTransmissionBlock), lds, s, tables_, \
-                         stride_, rayleigh_, nullptr, lev, spc, groups_, path, L, n, tile, bottom_radius,          \
+                         stride_, rayleigh_, wavelength_, ex, lev, spc, groups_, path, L, n, tile, bottom_radius,  \
                          star_radius, buf.flux, spectrum_dev);                                                      \
     break;
     BEAR_TRANSMISSION(4)

@@ -224,6 +224,26 @@ class PiecewisePolynomial {
 
 struct ChemistryModule { bool background = false; std::vector<int> species; bool sodium_free = false; };
 
+// A cloud model of forward_model.config (select_cloud_model.h:44-99):
+// "grey", "KHnongrey <lambda_ref>" or "power_law <lambda_ref>" (micron),
+// each optionally followed by "fb" (the bottom one scale height below the
+// top instead of a fitted bottom pressure).  Parameters: grey [tau, p_top,
+// p_bottom / p_top], KHnongrey [tau, q0, a0, particle size (micron), p_top,
+// p_bottom / p_top], power law [tau, exponent, p_top, p_bottom / p_top],
+// without the last with "fb" (grey_cloud_model.cpp:40-52,
+// kh_cloud_model.cpp:41-62, power_law_cloud_model.cpp:39-60).
+struct CloudModelSpec {
+  int kind = 0;                     // CloudKind: 0 grey, 1 KHnongrey, 2 power law
+  double reference_wavelength = 0;
+  bool fixed_bottom = false;
+  size_t nbParameters() const {
+    const size_t n = kind == 0 ? 3 : kind == 1 ? 6 : 4;
+    return fixed_bottom ? n - 1 : n;
+  }
+};
+// one model per line; throws InvalidInput for an unknown model
+std::vector<CloudModelSpec> parseCloudModels(const std::string& text);
+
 // Variants of the altitude computation (atmosphere.cpp:63-241): radius-
 // dependent gravity, a fixed mean molecular weight, or a fixed scale height.
 struct AltitudeOptions {
@@ -257,6 +277,13 @@ class Atmosphere {
   void setStructure(double surface_gravity, double bottom_radius, bool variable_gravity, const double* pressure,
                     const double* temperature, const std::vector<int>& species, const double* mixing_ratios);
   double nd(size_t level, int id) const { return number_density[level * n_ids_ + id]; }
+  // Atmosphere::calcScaleHeight (atmosphere.cpp:353-360) of the last
+  // compute(): R T / (mu g) with the surface gravity, or the fixed scale
+  // height (km) of that variant (atmosphere.cpp:232)
+  double scaleHeight(size_t level, double surface_gravity, double scale_height_km) const {
+    if (scale_height_km > 0) return scale_height_km * 100000.0;
+    return phys::gas_constant * temperature[level] / (mean_molecular_weight[level] * surface_gravity);
+  }
   // Reverse-mode derivative of the last compute(), made with `options`:
   // from dL/dT[l] (direct effects), dL/dn[l][id] and dL/dz[l] (consumed) to
   // dL/dg and dL/d(parameters), adding to *radius_bar (variable gravity: the

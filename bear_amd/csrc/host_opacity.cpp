@@ -97,6 +97,35 @@ void hminusRows(const SpectralGrid& grid, std::vector<double>& rows) {
 }
 }  // namespace
 
+std::vector<CloudModelSpec> parseCloudModels(const std::string& text) {
+  std::vector<CloudModelSpec> out;
+  std::istringstream lines(text);
+  std::string line;
+  while (std::getline(lines, line)) {
+    std::istringstream in(line);
+    std::string type, tok;
+    if (!(in >> type)) continue;
+    std::vector<std::string> args;
+    while (in >> tok) args.push_back(tok);
+    CloudModelSpec c;
+    if (type == "grey") {
+      c.kind = 0;
+      c.fixed_bottom = args.size() == 1 && args[0] == "fb";
+    } else if (type == "KHnongrey" || type == "power_law") {
+      c.kind = type == "KHnongrey" ? 1 : 2;
+      if (args.empty()) raise(kInvalidInput, "the " + type + " cloud model needs a reference wavelength");
+      c.reference_wavelength = std::stod(args[0]);
+      c.fixed_bottom = args.size() == 2 && args[1] == "fb";
+    } else if (type == "none") {
+      continue;
+    } else {
+      raise(kInvalidInput, "cloud model " + type + " unknown");
+    }
+    out.push_back(c);
+  }
+  return out;
+}
+
 bool hminusCoefficients(double temperature, double n_hminus, double n_h, double n_e, double (&c)[7]) {
   for (double& v : c) v = 0.0;
   if (n_hminus == 0) return false;                   // opacity_species.cpp:407-412

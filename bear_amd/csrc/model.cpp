@@ -74,6 +74,14 @@ TabulatedModel::TabulatedModel(const GlobalSettings& global, const ModelSettings
   engine_.setStencilGroups(groups);
   nb_chem_ = atmosphere_.nbChemistryParameters();
   nb_temp_ = atmosphere_.nbTemperatureParameters();
+  // cloud modules (emission_init.cpp:72-89); their parameters follow the
+  // temperature's (emission.cpp:123-153)
+  clouds_ = m.clouds;
+  if (clouds_.size() > size_t(kMaxClouds))
+    raise(kInvalidInput, "at most " + std::to_string(kMaxClouds) + " cloud models");
+  for (const auto& c : clouds_) nb_cloud_ += c.nbParameters();
+  transmission_ = transmission;
+  engine_.setCloudModels(int(clouds_.size()));
   for (auto& o : observations_) nb_modifier_ += size_t(o.modifier_parameters);
 }
 
@@ -93,6 +101,8 @@ void TabulatedModel::stageLevels(const Atmosphere& atm, const double* mods, Leve
     lev[l].inv_temperature = 1.0 / t;
     lev[l].altitude = atm.altitude[l];
     lev[l].rayleigh_density = n_r;
+    // a layer's cloud optical depth as this level's term (cloud_model.h:67-88)
+    if (l > 0) lev[l].cloud_scale = transmission_ ? 1.0 / (atm.altitude[l] - atm.altitude[l - 1]) : 1.0;
     if (hminus_)                                    // H-, H and electron densities
       hminusCoefficients(t, atm.nd(l, hminus_ids_[0]), atm.nd(l, hminus_ids_[1]), atm.nd(l, hminus_ids_[2]),
                          lev[l].hminus_c);
@@ -193,7 +203,83 @@ bool TabulatedModel::stage(const double* p, BatchModel& scalars) {
   const bool neglect = setup(p, atmosphere_, scalars);
   stageLevels(atmosphere_, modifiers(p), engine_.levels(), engine_.levelSpecies(), nullptr, nullptr,
               engine_.shifts());
+  CloudSpec spec;
+  if (!clouds_.empty()) stageClouds(atmosphere_, clouds(p), std::pow(10, p[0]), fixedScaleHeight(p), engine_.levels(), spec);
+  engine_.setCloudSpec(spec);
   return neglect;
+}
+
+// GreyCloudModel::cloudPosition, both forms (grey_cloud_model.cpp:118-197):
+// the levels bracketing the top (and bottom) pressure, or the bottom one
+// scale height below the top; at least two layers thick.  Unsigned
+// arithmetic as in the reference.
+namespace {
+void cloudPosition(const Atmosphere& atm, double top_pressure, double bottom_pressure, bool fixed_bottom,
+                   double g, double scale_height_km, unsigned& top, unsigned& bottom) {
+  const size_t L = atm.nb_levels;
+  const std::vector<double>& p = atm.pressure;
+  top = 0;
+  bottom = 0;
+  for (size_t i = 0; i + 1 < L; ++i)
+    if ((p[i] > top_pressure && p[i + 1] < top_pressure) || p[i] == top_pressure) top = unsigned(i);
+  if (!fixed_bottom) {
+    for (size_t i = 0; i + 1 < L; ++i)
+      if ((p[i] > bottom_pressure && p[i + 1] < bottom_pressure) || p[i] == bottom_pressure) bottom = unsigned(i);
+    if (top_pressure <= p.back()) top = unsigned(L - 1);
+    if (bottom_pressure >= p[0]) bottom = 0;
+  } else {
+    if (top_pressure < p.back()) top = unsigned(L - 1);
+    const std::vector<double>& z = atm.altitude;
+    double bottom_altitude = z[top] - atm.scaleHeight(top, g, scale_height_km);
+    if (bottom_altitude < 0) bottom_altitude = 0;
+    for (size_t i = 0; i + 1 < L; ++i)
+      if ((z[i] < bottom_altitude && z[i + 1] > bottom_altitude) || z[i] == bottom_altitude) bottom = unsigned(i);
+  }
+  if (top == bottom) bottom -= 2;
+  if (top < 2) {
+    top = 2;
+    bottom = 0;
+  }
+}
+}  // namespace
+
+// The cloud models' layers (grey_cloud_model.cpp:56-98,
+// power_law_cloud_model.cpp:70-121, kh_cloud_model.cpp:69-133): tau / (top -
+// bottom) on the layers bottom+1 .. top, in LevelState::cloud_w of the level
+// above each layer, times the model's spectral factor, which the kernels form
+// from spec.  A top at the uppermost level names a layer above the grid (the
+// reference writes past its array there); that layer is left out.
+void TabulatedModel::stageClouds(const Atmosphere& atm, const double* cp, double g, double scale_height_km,
+                                 LevelState* lev, CloudSpec& spec) const {
+  const size_t L = atm.nb_levels;
+  const double two_pi = 2.0 * phys::pi;
+  spec = CloudSpec{};
+  spec.count = int(clouds_.size());
+  for (size_t m = 0; m < clouds_.size(); ++m) {
+    const CloudModelSpec& c = clouds_[m];
+    const double* q = cp;
+    cp += c.nbParameters();
+    const size_t at_top = c.kind == kCloudGrey ? 1 : c.kind == kCloudKH ? 4 : 2;
+    const double top_pressure = q[at_top];
+    const double bottom_pressure = c.fixed_bottom ? 0.0 : top_pressure * q[at_top + 1];
+    const double tau = q[0] < 0 ? 0.0 : q[0];
+    unsigned top = 0, bottom = 0;
+    cloudPosition(atm, top_pressure, bottom_pressure, c.fixed_bottom, g, scale_height_km, top, bottom);
+    const double layer = tau / static_cast<double>(top - bottom);
+    for (unsigned i = top; i > bottom; --i)
+      if (i + 1 < L) lev[i + 1].cloud_w[m] = layer;
+    spec.kind[m] = c.kind;
+    if (c.kind == kCloudPowerLaw) {
+      spec.c[m][0] = q[1];
+      spec.c[m][1] = 1.0 / std::pow(c.reference_wavelength, q[1]);
+    } else if (c.kind == kCloudKH) {
+      const double x_ref = two_pi * q[3] / c.reference_wavelength;
+      spec.c[m][0] = q[1];
+      spec.c[m][1] = q[2];
+      spec.c[m][2] = two_pi * q[3];
+      spec.c[m][3] = q[1] * std::pow(x_ref, -q[2]) + std::pow(x_ref, 0.2);
+    }
+  }
 }
 
 void TabulatedModel::stageBatch(const double* params, size_t batch, size_t stride, const double* inflation,
@@ -348,7 +434,7 @@ void TabulatedModel::densityGradient(const Atmosphere& atm, const double* G, std
 // shift_const: one parameter per observation, added to its bins
 // (forward_model.cpp:161-180)
 void TabulatedModel::shiftGradient(const double* bin_grad, double* out) const {
-  size_t k = nb_general_ + nb_chem_ + nb_temp_, j = 0;
+  size_t k = nb_general_ + nb_chem_ + nb_temp_ + nb_cloud_, j = 0;
   for (const auto& o : observations_) {
     double sum = 0;
     for (size_t q = 0; q < o.nbPoints(); ++q) sum += bin_grad[j + q];

@@ -30,6 +30,7 @@ struct ModelSettings {
   std::vector<std::pair<std::string, std::string>> opacity;   // (symbol, folder)
   bool fit_mean_molecular_weight = false, fit_scale_height = false, use_variable_gravity = false;
   std::string stellar_model = "blackbody";
+  std::vector<CloudModelSpec> clouds;
 };
 
 class ForwardModel {
@@ -89,7 +90,18 @@ class TabulatedModel : public ForwardModel {
                    LevelSpeciesAdjoint* species_adjoint, LevelSpecies32* species32, double* shifts) const;
   // the single-model path: setup + stageLevels into the engine's staging
   bool stage(const double* parameters, BatchModel& scalars);
-  const double* modifiers(const double* p) const { return p + nb_general_ + nb_chem_ + nb_temp_; }
+  const double* clouds(const double* p) const { return p + nb_general_ + nb_chem_ + nb_temp_; }
+  const double* modifiers(const double* p) const { return clouds(p) + nb_cloud_; }
+  // the cloud layers of one computed atmosphere (cloud parameters cp,
+  // surface gravity g): per level LevelState::cloud_w, and the models'
+  // spectral factors
+  void stageClouds(const Atmosphere& atmosphere, const double* cp, double g, double scale_height_km,
+                   LevelState* levels, CloudSpec& spec) const;
+  // the fixed scale height (km) of the transmission variant that fits one, else 0
+  virtual double fixedScaleHeight(const double* p) const {
+    (void)p;
+    return 0.0;
+  }
   // emission and eclipse: the chain from the level gradients through the
   // atmosphere to [log g, chemistry..., temperature..., modifiers...];
   // generalGradient fills the other two general parameters
@@ -114,7 +126,9 @@ class TabulatedModel : public ForwardModel {
   bool hminus_ = false;                     // the H- continuum is listed
   int hminus_ids_[3] = {-1, -1, -1};        // species ids of H-, H and e-
   int group_rep_[2] = {-1, -1};             // a species of each stencil group (StencilGroups)
-  size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_modifier_ = 0;
+  std::vector<CloudModelSpec> clouds_;
+  bool transmission_ = false;
+  size_t nb_general_ = 3, nb_chem_ = 0, nb_temp_ = 0, nb_cloud_ = 0, nb_modifier_ = 0;
 };
 
 // EmissionModel (forward_model/emission/emission.h:127-250): parameters
@@ -123,7 +137,7 @@ class EmissionModel : public TabulatedModel {
  public:
   EmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
                 const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
-  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_modifier_; }
+  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_cloud_ + nb_modifier_; }
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
@@ -151,7 +165,7 @@ class OccultationModel : public TabulatedModel {
  public:
   OccultationModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
                    const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
-  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_modifier_; }
+  size_t parametersNumber() const override { return 3 + nb_chem_ + nb_temp_ + nb_cloud_ + nb_modifier_; }
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return true; }
@@ -183,7 +197,7 @@ class TransmissionModel : public TabulatedModel {
  public:
   TransmissionModel(const GlobalSettings& global, const ModelSettings& model, const SpectralGrid& grid,
                     const std::vector<ObservationSetup>& observations, DeviceEngine& engine);
-  size_t parametersNumber() const override { return nb_general_ + nb_chem_ + nb_temp_ + nb_modifier_; }
+  size_t parametersNumber() const override { return nb_general_ + nb_chem_ + nb_temp_ + nb_cloud_ + nb_modifier_; }
   bool calcModelGPU(const double* parameters, double error_inflation, double* spectrum_dev,
                     double* const* spectrum_obs_dev) override;
   bool binsAreFluxes() const override { return false; }
@@ -202,6 +216,7 @@ class TransmissionModel : public TabulatedModel {
 
  private:
   AltitudeOptions altitudeOptions(const double* p) const;
+  double fixedScaleHeight(const double* p) const override { return fit_scale_height_ ? p[3] : 0.0; }
   // the chord lengths of the staged atmosphere into the engine's staging
   void stageChords(double bottom_radius);
   bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;

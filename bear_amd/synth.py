@@ -99,6 +99,9 @@ class Case:
     # resolution in cm^-1), 1 const_wavelength (micron), 2 const_resolution (R)
     discretisation: int = 0
     continuum: list = field(default_factory=list)   # ["H-"]: species without tables
+    # cloud models as forward_model.config lines ("grey", "power_law 1.0 fb",
+    # "KHnongrey 1.0"); their parameters follow the temperature's
+    clouds: list = field(default_factory=list)
 
     @property
     def opacity_entries(self):
@@ -592,6 +595,21 @@ def resampled(case: Case, discretisation, resolution, name):
     return case
 
 
+def cloudy(case: Case, name, clouds, values):
+    """The case with cloud models (forward_model.config lines, e.g. "grey",
+    "power_law 1.0 fb", "KHnongrey 1.0") and their parameter values, which
+    go after the temperature parameters (emission.cpp:123-153)."""
+    nb_mix = sum(len(c[1]) for c in case.chemistry if c[0] == "iso")
+    tm = case.temperature_model
+    nb_temp = 1 if tm[0] == "const" else tm[1] * tm[2] + 1
+    at = nb_general_parameters(case) + nb_mix + nb_temp
+    case.parameters = np.concatenate([case.parameters[:at], np.asarray(values, dtype=np.float64),
+                                      case.parameters[at:]])
+    case.clouds = list(clouds)
+    case.name = name
+    return case
+
+
 def hminus_case(seed=23):
     """The H- continuum (bound-free below 1.6419 micron, free-free at every
     wavelength above 0.1823 micron) with H2O and H2-H2 CIA: a grid from
@@ -694,7 +712,18 @@ CASES = {"tiny": tiny_case, "emission_c1": emission_c1, "emission_c2": emission_
          "transmission_variable_gravity": lambda: transmission_small(variant="variable_gravity"),
          "transmission_fit_mmw": lambda: transmission_small(variant="fit_mmw"),
          "transmission_fit_scale_height": lambda: transmission_small(variant="fit_scale_height"),
-         "eclipse_small": eclipse_small}
+         "eclipse_small": eclipse_small,
+         # grey cloud [tau, p_top, p_bottom / p_top] and a power-law cloud one
+         # scale height thick [tau, exponent, p_top] (lambda_ref 1 micron)
+         "cloudy_tiny": lambda: cloudy(tiny_case(), "cloudy_tiny", ["grey", "power_law 1.0 fb"],
+                                       [2.0, 0.1, 20.0, 1.5, -2.0, 1e-3]),
+         # KHnongrey [tau, q0, a0, particle size (micron), p_top, p_bottom / p_top]
+         # (lambda_ref 1 micron) and a grey cloud one scale height thick
+         "cloudy_eclipse": lambda: cloudy(eclipse_small(), "cloudy_eclipse", ["KHnongrey 1.0", "grey fb"],
+                                          [3.0, 10.0, 4.0, 0.5, 0.05, 30.0, 1.0, 1e-2]),
+         "transmission_cloudy": lambda: cloudy(transmission_small(variant="plain"), "transmission_cloudy",
+                                               ["grey", "KHnongrey 2.0 fb"],
+                                               [5.0, 1e-2, 50.0, 2.0, 8.0, 4.5, 0.3, 1e-4])}
 
 
 # ---------------------------------------------------------------------------
@@ -799,6 +828,7 @@ def write_case(case: Case, folder: str, tables: bool = True) -> str:
         "use_error_inflation": case.use_error_inflation,
         "extra": case.extra,
         "discretisation": case.discretisation,
+        "clouds": list(case.clouds),
     }
     with open(folder + "case.json", "w") as f:
         json.dump(meta, f, indent=1)
@@ -815,6 +845,8 @@ def write_case(case: Case, folder: str, tables: bool = True) -> str:
             f.write(f"chemistry {kind} " + " ".join(sp) + "\n")
         for s, fo in case.opacity_entries:
             f.write(f"opacity {s} {fo}\n")
+        for c in case.clouds:
+            f.write(f"cloud {c}\n")
         f.write(f"error_inflation {int(case.use_error_inflation)}\n")
         variant = case.extra.get("variant", "plain")
         f.write(f"use_variable_gravity {int(variant == 'variable_gravity')}\n")
@@ -877,7 +909,8 @@ def _model_config(meta):
                            [tuple(o) for o in meta["opacity"]],
                            fit_mean_molecular_weight=variant == "fit_mmw",
                            fit_scale_height=variant == "fit_scale_height",
-                           use_variable_gravity=variant == "variable_gravity")
+                           use_variable_gravity=variant == "variable_gravity",
+                           cloud_model=list(meta.get("clouds", [])))
 
 
 def open_retrieval(folder, device=0, table_storage="fp64"):
@@ -909,6 +942,7 @@ def case_meta(case: Case):
         "use_error_inflation": case.use_error_inflation,
         "model": case.model,
         "extra": case.extra,
+        "clouds": list(case.clouds),
     }
 
 

@@ -97,6 +97,17 @@ typedef struct bear_model_config {
   int use_variable_gravity;
   /* secondary eclipse (OccultationModelConfig): "blackbody" (NULL: blackbody) */
   const char* stellar_spectrum_model;
+  /* cloud models (EmissionModelConfig::cloud_model and cloud_model_parameters,
+   * emission.h:66-67; select_cloud_model.h:44-99), one per line, NULL or
+   * empty for none: "grey", "KHnongrey <lambda_ref>", "power_law <lambda_ref>"
+   * (micron), each optionally followed by "fb" (bottom one scale height
+   * below the top).  Their parameters follow the temperature's: grey [tau,
+   * p_top, p_bottom / p_top], KHnongrey [tau, q0, a0, particle size (micron),
+   * p_top, p_bottom / p_top], power_law [tau, exponent, p_top,
+   * p_bottom / p_top], without the last with "fb".  At most 4 models; the
+   * single-point calls compute them, the batched and gradient calls refuse
+   * such a model (BEAR_ERROR_INVALID_INPUT). */
+  const char* cloud_model;
 } bear_model_config;
 
 /* ObservationInput (observations/observations.h:43-83).  For "spectroscopy"
@@ -322,9 +333,12 @@ int bear_spectrum_model_spectral_grid(const bear_spectrum_model* m, double* wave
  * the flux in W m^-2 cm times radius^2 / distance^2 (same units for both).
  * surface_gravity in cm s^-2; radius is also the bottom radius of the
  * variable gravity.  mixing_ratios [nb_levels][nb_species].
- * cloud_optical_depth [nb_levels - 1][points] may be NULL; a cloud with
- * non-zero optical depth is refused (the reference's FixedCloudModel is not
- * part of this engine). */
+ * cloud_optical_depth [nb_levels - 1][points] (layer above level l in row
+ * l, points in the grid's order) may be NULL; when any value is positive it
+ * is the clouds' optical depth of each layer (FixedCloudModel,
+ * fixed_cloud_model.cpp:36-75; emission.cpp:344-367), added to the gas's
+ * (emission, eclipse) or, divided by the layer's thickness, to the upper
+ * level's extinction (transmission, cloud_model.h:67-88). */
 int bear_emission_calc_spectrum(bear_spectrum_model* m, double surface_gravity, double radius, double distance,
                                 const double* pressure, const double* temperature, size_t nb_levels,
                                 const char* const* species, size_t nb_species, const double* mixing_ratios,

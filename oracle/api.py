@@ -61,7 +61,12 @@ def lib():
         L.oracle_transmission_paths.argtypes = [vp, sz, dp, i32]
         L.oracle_direct_spectrum.restype = C.c_int
         L.oracle_direct_spectrum.argtypes = [vp, C.c_double, C.c_double, C.c_double, dp, dp, C.c_char_p, dp, C.c_int,
-                                             dp]
+                                             dp, dp]
+        L.oracle_cloud_optical_depths.restype = C.c_int
+        L.oracle_cloud_optical_depths.argtypes = [vp, dp]
+        L.oracle_scale_height.argtypes = [vp, dp]
+        L.oracle_cloud_extinction.restype = C.c_int
+        L.oracle_cloud_extinction.argtypes = [vp, dp]
         L.oracle_tangent_sum.restype = C.c_double
         L.oracle_tangent_sum.argtypes = [dp, dp, sz, C.c_int32, i32]
         _lib = L
@@ -154,17 +159,23 @@ class EmissionOracle:
         return dict(spectrum_flux=flux, spectrum=spec, bins=out, loglike=ll.value, neglect=bool(neg.value))
 
     def direct_spectrum(self, surface_gravity, a, b, pressure, temperature, species, mixing_ratios,
-                        use_variable_gravity=False):
+                        use_variable_gravity=False, cloud_optical_depth=None):
         """The direct models' calcSpectrum (a case written with a direct
         range): emission a = radius, b = distance; transmission a = planet
         radius, b = radius ratio; eclipse a = radius ratio (b unused).
-        mixing_ratios [levels][species]."""
+        mixing_ratios [levels][species]; cloud_optical_depth [levels - 1][points]
+        or None."""
         p = np.ascontiguousarray(pressure, dtype=np.float64)
         t = np.ascontiguousarray(temperature, dtype=np.float64)
         x = np.ascontiguousarray(mixing_ratios, dtype=np.float64)
+        cloud = None
+        if cloud_optical_depth is not None:
+            cloud = np.ascontiguousarray(cloud_optical_depth, dtype=np.float64)
+            assert cloud.size == (p.size - 1) * self.nb_points
         out = np.empty(self.nb_points)
         rc = self._L.oracle_direct_spectrum(self._h, float(surface_gravity), float(a), float(b), _p(p), _p(t),
-                                            "\n".join(species).encode(), _p(x), int(use_variable_gravity), _p(out))
+                                            "\n".join(species).encode(), _p(x), int(use_variable_gravity),
+                                            None if cloud is None else _p(cloud), _p(out))
         if rc != 0:
             raise RuntimeError(self._L.oracle_last_error().decode())
         return out
@@ -212,6 +223,24 @@ class EmissionOracle:
         s = np.empty_like(k)
         self._L.oracle_coefficients(self._h, _p(k), _p(s))
         return k, s
+
+    def cloud_optical_depths(self):
+        """The last evaluation's cloud layer optical depths [levels - 1, points],
+        or None without a cloud."""
+        out = np.empty((self.nb_levels - 1, self.nb_points))
+        return out if self._L.oracle_cloud_optical_depths(self._h, _p(out)) else None
+
+    def cloud_extinction(self):
+        """The last transmission evaluation's cloud extinction [levels, points]
+        (1/cm), or None without a cloud."""
+        out = np.empty((self.nb_levels, self.nb_points))
+        return out if self._L.oracle_cloud_extinction(self._h, _p(out)) else None
+
+    def scale_height(self):
+        """The last evaluation's scale height per level (cm)."""
+        out = np.empty(self.nb_levels)
+        self._L.oracle_scale_height(self._h, _p(out))
+        return out
 
 
 def tangent_sum(path, extinction, frozen=-1):

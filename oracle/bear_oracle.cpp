@@ -833,6 +833,22 @@ struct PiecewisePoly {                             // additional/piecewise_poly.
 
 struct Chemistry { bool background = false; std::vector<int> species; bool sodium_free = false; };
 
+// Cloud layers (cloud_model/grey_cloud_model.cpp, power_law_cloud_model.cpp,
+// kh_cloud_model.cpp; select_cloud_model.h:44-99): kind 0 "grey" [tau,
+// p_top, p_bottom / p_top], 1 "KHnongrey" [tau, q0, a0, particle size,
+// p_top, p_bottom / p_top], 2 "power_law" [tau, exponent, p_top,
+// p_bottom / p_top]; "fb" drops the last parameter and puts the bottom one
+// scale height below the top.
+struct CloudLayer {
+  int kind = 0;
+  double reference_wavelength = 0;
+  bool fixed_bottom = false;
+  size_t nbParameters() const {
+    const size_t n = kind == 0 ? 3 : kind == 1 ? 6 : 4;
+    return fixed_bottom ? n - 1 : n;
+  }
+};
+
 // ---------------------------------------------------------------- the model
 struct EmissionOracle {
   std::string model = "emission";
@@ -847,6 +863,8 @@ struct EmissionOracle {
   size_t nb_temp_param = 0;
   std::vector<Chemistry> chemistry;
   size_t nb_chem_param = 0, nb_modifier = 0;
+  std::vector<CloudLayer> clouds;
+  size_t nb_cloud_param = 0;
   std::vector<OpacitySpecies> species;
   bool error_inflation = false;
   std::vector<double> default_parameters;
@@ -857,7 +875,14 @@ struct EmissionOracle {
   std::vector<double> stellar_flux;
 
   // per-evaluation state
-  std::vector<double> temperature, altitude, mmw;
+  std::vector<double> temperature, altitude, mmw, scale_height;
+  // layer optical depths of the clouds [L-1][N] (empty: no cloud), the sum
+  // over the cloud models (opacity_calc.h:195-240), or the direct models'
+  // caller-given ones (fixed_cloud_model.cpp:36-75)
+  std::vector<std::vector<double>> cloud_tau;
+  // transmission: the same as level extinction [L][N], 0 at the bottom,
+  // tau[l-1] / (z_l - z_{l-1}) above (cloud_model.h:67-88)
+  std::vector<std::vector<double>> cloud_ext;
   std::vector<std::vector<double>> nd;                 // [L][46]
   std::vector<std::vector<double>> kappa;              // [L][N]
   std::vector<std::vector<double>> scattering;         // [L][N]
@@ -926,6 +951,21 @@ struct EmissionOracle {
       }
       if (key == "opacity") opac.push_back({r[0], r[1]});
       if (key == "error_inflation") error_inflation = r[0] == "1";
+      if (key == "cloud") {                               // forward_model.config cloud lines
+        CloudLayer c;
+        const std::vector<std::string> a(r.begin() + 1, r.end());
+        if (r[0] == "grey") {
+          c.kind = 0;
+          c.fixed_bottom = a.size() == 1 && a[0] == "fb";
+        } else if (r[0] == "KHnongrey" || r[0] == "power_law") {
+          c.kind = r[0] == "KHnongrey" ? 1 : 2;
+          if (a.empty()) throw std::runtime_error("cloud model " + r[0] + " needs a reference wavelength");
+          c.reference_wavelength = std::stod(a[0]);
+          c.fixed_bottom = a.size() == 2 && a[1] == "fb";
+        } else throw std::runtime_error("cloud model not restated: " + r[0]);
+        clouds.push_back(c);
+        nb_cloud_param += c.nbParameters();
+      }
       if (key == "parameters")
         for (size_t i = 0; i < std::stoul(r[0]); ++i) default_parameters.push_back(std::stod(r[1 + i]));
       if (key == "direct_range") {
@@ -1008,7 +1048,7 @@ struct EmissionOracle {
     return 3;
   }
   size_t nbParameters() const {
-    return nbGeneral() + nb_chem_param + nb_temp_param + nb_modifier + (error_inflation ? 1 : 0);
+    return nbGeneral() + nb_chem_param + nb_temp_param + nb_cloud_param + nb_modifier + (error_inflation ? 1 : 0);
   }
 
   // Atmosphere::calcAtmosphereStructure, its three overloads (atmosphere.cpp:63-241):
@@ -1076,10 +1116,12 @@ struct EmissionOracle {
     if (scale_height_km > 0) {                                 // atmosphere.cpp:338-350
       const double h = scale_height_km * 100000.0;
       for (size_t i = 1; i < L; ++i) altitude[i] = -h * std::log(pressure[i] / pressure[0]);
+      scale_height.assign(L, h);                               // atmosphere.cpp:232
       return neg;
     }
     if (fixed_mmw > 0) mmw.assign(L, fixed_mmw);               // atmosphere.cpp:168
     hydrostaticAltitude(g, bottom_radius, variable_gravity);
+    scaleHeight(g);
     return neg;
   }
 
@@ -1103,6 +1145,91 @@ struct EmissionOracle {
     }
     altitude.assign(L, 0.0);
     hydrostaticAltitude(g, bottom_radius, var_gravity);
+    scaleHeight(g);
+  }
+
+  // Atmosphere::calcScaleHeight (atmosphere.cpp:353-360): R T / (mu g) with
+  // the surface gravity
+  void scaleHeight(double g) {
+    scale_height.assign(nb_levels, 0.0);
+    for (size_t i = 0; i < nb_levels; ++i) scale_height[i] = kGasConstant * temperature[i] / (mmw[i] * g);
+  }
+
+  // GreyCloudModel::cloudPosition, both forms (grey_cloud_model.cpp:118-197):
+  // the levels bracketing the cloud top (and bottom) pressure, or the bottom
+  // one scale height below the top; at least two layers thick.  Unsigned
+  // arithmetic as in the reference.
+  void cloudPosition(double top_pressure, double bottom_pressure, bool fixed_bottom, unsigned& top,
+                     unsigned& bottom) const {
+    const size_t L = nb_levels;
+    top = 0;
+    bottom = 0;
+    for (size_t i = 0; i < L - 1; ++i)
+      if ((pressure[i] > top_pressure && pressure[i + 1] < top_pressure) || pressure[i] == top_pressure)
+        top = unsigned(i);
+    if (!fixed_bottom) {
+      for (size_t i = 0; i < L - 1; ++i)
+        if ((pressure[i] > bottom_pressure && pressure[i + 1] < bottom_pressure) || pressure[i] == bottom_pressure)
+          bottom = unsigned(i);
+      if (top_pressure <= pressure.back()) top = unsigned(L - 1);
+      if (bottom_pressure >= pressure[0]) bottom = 0;
+    } else {
+      if (top_pressure < pressure.back()) top = unsigned(L - 1);
+      double bottom_altitude = altitude[top] - scale_height[top];
+      if (bottom_altitude < 0) bottom_altitude = 0;
+      for (size_t i = 0; i < L - 1; ++i)
+        if ((altitude[i] < bottom_altitude && altitude[i + 1] > bottom_altitude) || altitude[i] == bottom_altitude)
+          bottom = unsigned(i);
+    }
+    if (top == bottom) bottom -= 2;
+    if (top < 2) {
+      top = 2;
+      bottom = 0;
+    }
+  }
+
+  // the cloud models' layer optical depths, summed (opacity_calc.h:195-240;
+  // grey_cloud_model.cpp:56-98, power_law_cloud_model.cpp:70-121,
+  // kh_cloud_model.cpp:69-133): tau / (top - bottom) on the layers
+  // bottom+1 .. top, times (lambda / lambda_ref)^exponent for the power law
+  // and Q(x_ref) / (q0 x^-a0 + x^0.2), x = 2 pi a / lambda, for KHnongrey.
+  // A top at the uppermost level would name the layer above the grid (the
+  // reference writes past its array there); that layer is left out.
+  void cloudOpticalDepths(const std::vector<double>& cp) {
+    const size_t N = grid.nu.size(), L = nb_levels;
+    cloud_tau.clear();
+    if (clouds.empty()) return;
+    cloud_tau.assign(L - 1, std::vector<double>(N, 0.0));
+    size_t k = 0;
+    for (const CloudLayer& c : clouds) {
+      const double* q = cp.data() + k;
+      k += c.nbParameters();
+      double tau = q[0];
+      const size_t at_top = c.kind == 0 ? 1 : c.kind == 1 ? 4 : 2;
+      const double top_pressure = q[at_top];
+      const double bottom_pressure = c.fixed_bottom ? 0.0 : top_pressure * q[at_top + 1];
+      if (tau < 0) tau = 0;
+      unsigned top = 0, bottom = 0;
+      cloudPosition(top_pressure, bottom_pressure, c.fixed_bottom, top, bottom);
+      const double layer = tau / static_cast<double>(top - bottom);
+      std::vector<double> d(N, layer);                    // this model's depth per wavenumber
+      if (c.kind == 2) {
+        const double reference_value = std::pow(c.reference_wavelength, q[1]);
+        for (size_t j = 0; j < N; ++j) d[j] = layer * std::pow(grid.lam[j], q[1]) / reference_value;
+      } else if (c.kind == 1) {
+        const double q0 = q[1], a0 = q[2], size = q[3];
+        const double x_ref = 2 * kPi * size / c.reference_wavelength;
+        const double q_ref = q0 * std::pow(x_ref, -a0) + std::pow(x_ref, 0.2);
+        for (size_t j = 0; j < N; ++j) {
+          const double x = 2 * kPi * size / grid.lam[j];
+          d[j] = layer * q_ref / (q0 * std::pow(x, -a0) + std::pow(x, 0.2));
+        }
+      }
+      for (unsigned i = top; i > bottom; --i) {
+        if (i >= L - 1) continue;
+        for (size_t j = 0; j < N; ++j) cloud_tau[i][j] += d[j];
+      }
+    }
   }
 
   // hydrostatic altitude from mmw, T and p (atmosphere.cpp:288-334)
@@ -1124,9 +1251,19 @@ struct EmissionOracle {
   // the direct models' calcSpectrum: emission (emission.cpp:268-340, a =
   // radius, b = distance), transmission (transmission.cpp:371-465, a = planet
   // radius, b = radius ratio), eclipse (secondary_eclipse.cpp:395-482, a =
-  // radius ratio)
+  // radius ratio).  cloud: the caller's layer optical depths [L-1][N] or
+  // empty; used when any is positive (emission.cpp:344-367,
+  // fixed_cloud_model.cpp:36-75), summed onto zeros (opacity_calc.h:221-237)
   void directSpectrum(double g, double a, double b, const std::vector<double>& p, const std::vector<double>& t,
-                      const std::vector<int>& ids, const std::vector<double>& x, bool var_gravity) {
+                      const std::vector<int>& ids, const std::vector<double>& x, bool var_gravity,
+                      const std::vector<double>& cloud = {}) {
+    const size_t N = grid.nu.size(), L = nb_levels;
+    cloud_tau.clear();
+    if (std::any_of(cloud.begin(), cloud.end(), [](double v) { return v > 0; })) {
+      cloud_tau.assign(L - 1, std::vector<double>(N, 0.0));
+      for (size_t l = 0; l + 1 < L; ++l)
+        for (size_t i = 0; i < N; ++i) cloud_tau[l][i] += cloud[l * N + i];
+    }
     if (model == "transmission") {
       setStructure(g, a, var_gravity, p, t, ids, x);
       opacity();
@@ -1195,6 +1332,8 @@ struct EmissionOracle {
     for (size_t i = 0; i < N; ++i) {
       std::vector<double> tau(L - 1), b(L);
       for (size_t l = 0; l + 1 < L; ++l) tau[l] = (altitude[l + 1] - altitude[l]) * (kappa[l + 1][i] + kappa[l][i]) / 2.;
+      if (!cloud_tau.empty())                                   // short_characteristics.cpp:75-76
+        for (size_t l = 0; l + 1 < L; ++l) tau[l] += cloud_tau[l][i];
       for (size_t l = 0; l < L; ++l) b[l] = planck(temperature[l], grid.nu[i]);
       double i1 = b[0], i2 = b[0];
       for (size_t l = 0; l + 1 < L; ++l) {
@@ -1246,6 +1385,8 @@ struct EmissionOracle {
     const int32_t* frozen = frozen_stops.empty() ? nullptr : frozen_stops.data() + w * (L - 1);
     std::vector<double> ext(L);
     for (size_t i = 0; i < L; ++i) ext[i] = kappa[i][w] + scattering[i][w];
+    if (!cloud_ext.empty())                                     // transmission_spectrum.cpp:121-130
+      for (size_t i = 0; i < L; ++i) ext[i] += cloud_ext[i][w];
     for (int t = int(L) - 2; t > -1; --t) {
       int32_t last = 0;
       tau[t] = tangentSum(path[size_t(t)].data(), ext.data() + t, path[size_t(t)].size(), frozen ? frozen[t] - t : -1,
@@ -1254,8 +1395,20 @@ struct EmissionOracle {
     }
   }
 
+  void cloudExtinction() {
+    const size_t N = grid.nu.size(), L = nb_levels;
+    cloud_ext.clear();
+    if (cloud_tau.empty()) return;
+    cloud_ext.assign(L, std::vector<double>(N, 0.0));
+    for (size_t j = 1; j < L; ++j) {
+      const double dz = altitude[j] - altitude[j - 1];
+      for (size_t i = 0; i < N; ++i) cloud_ext[j][i] = cloud_tau[j - 1][i] / dz;
+    }
+  }
+
   void transmissionSpectrum(double bottom_radius, double star_radius) {
     const size_t N = grid.nu.size(), L = nb_levels;
+    cloudExtinction();
     spectrum_flux.assign(N, 0.0);
     if (record_stops) recorded_stops.assign(N * (L - 1), 0);
     const std::vector<std::vector<double>> path = pathLengths(bottom_radius);
@@ -1292,13 +1445,16 @@ struct EmissionOracle {
     const size_t g0 = nbGeneral();
     const std::vector<double> cp(p.begin() + g0, p.begin() + g0 + nb_chem_param);
     const std::vector<double> tp(p.begin() + g0 + nb_chem_param, p.begin() + g0 + nb_chem_param + nb_temp_param);
-    const size_t mods = g0 + nb_chem_param + nb_temp_param;
+    const size_t clp = g0 + nb_chem_param + nb_temp_param;
+    const std::vector<double> cl(p.begin() + clp, p.begin() + clp + nb_cloud_param);
+    const size_t mods = clp + nb_cloud_param;
     bins.resize(obs.size());
     if (model == "transmission") {
       const double fit = (fit_mmw || fit_scale_height) ? p[3] : 0.0;
       neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp, p[1], fit_mmw ? fit : 0.0,
                                     fit_scale_height ? fit : 0.0);
       opacity();
+      cloudOpticalDepths(cl);
       last_bottom_radius = p[1];
       transmissionSpectrum(p[1], p[2]);
       for (size_t i = 0; i < obs.size(); ++i) bins[i] = processSpectrum(grid, obs[i], spectrum_flux, false);
@@ -1307,6 +1463,7 @@ struct EmissionOracle {
     } else if (model == "secondary_eclipse") {
       neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp);
       opacity();
+      cloudOpticalDepths(cl);
       radiativeTransfer(1.0);
       std::vector<double> star(grid.nu.size());                     // star_blackbody.cpp:40-54
       for (size_t i = 0; i < star.size(); ++i) star[i] = planck(p[2], grid.nu[i]) * kPi * 1e-3;
@@ -1324,6 +1481,7 @@ struct EmissionOracle {
     } else {
       neglect = atmosphereStructure(std::pow(10, p[0]), tp, cp);
       opacity();
+      cloudOpticalDepths(cl);
       double scaling = kRadiusJupiter / p[2];                          // emission.cpp:108-120
       scaling = scaling * scaling * p[1];
       radiativeTransfer(scaling);
@@ -1444,10 +1602,39 @@ void oracle_coefficients(void* h, double* kappa, double* scattering) {
 
 int oracle_nb_species_ids() { return kNbSpecies; }
 
+// the last evaluation's cloud layer optical depths [L-1][N]; 1 when there is
+// a cloud, 0 (nothing written) when not
+int oracle_cloud_optical_depths(void* h, double* tau) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  if (m->cloud_tau.empty()) return 0;
+  const size_t N = m->grid.nu.size();
+  for (size_t l = 0; l < m->cloud_tau.size(); ++l)
+    for (size_t i = 0; i < N; ++i) tau[l * N + i] = m->cloud_tau[l][i];
+  return 1;
+}
+
+// the last transmission evaluation's cloud extinction [L][N]; 1 when there
+// is a cloud, 0 (nothing written) when not
+int oracle_cloud_extinction(void* h, double* ext) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  if (m->cloud_ext.empty()) return 0;
+  const size_t N = m->grid.nu.size();
+  for (size_t l = 0; l < m->cloud_ext.size(); ++l)
+    for (size_t i = 0; i < N; ++i) ext[l * N + i] = m->cloud_ext[l][i];
+  return 1;
+}
+
+// the last evaluation's scale height per level (cm)
+void oracle_scale_height(void* h, double* out) {
+  auto* m = static_cast<EmissionOracle*>(h);
+  for (size_t l = 0; l < m->scale_height.size(); ++l) out[l] = m->scale_height[l];
+}
+
 // the direct models' calcSpectrum (EmissionOracle::directSpectrum): species
 // as newline-separated symbols, mixing ratios [levels][species], spectrum [N]
 int oracle_direct_spectrum(void* h, double g, double a, double b, const double* p, const double* t,
-                           const char* species, const double* x, int variable_gravity, double* spectrum) {
+                           const char* species, const double* x, int variable_gravity, const double* cloud,
+                           double* spectrum) {
   try {
     auto* m = static_cast<EmissionOracle*>(h);
     if (!m->direct) throw std::runtime_error("not a direct model (no direct_range)");
@@ -1456,8 +1643,10 @@ int oracle_direct_spectrum(void* h, double g, double a, double b, const double* 
     std::string sym;
     while (in >> sym) ids.push_back(fixedChemistryId(sym));
     const size_t L = m->nb_levels;
+    std::vector<double> c;
+    if (cloud) c.assign(cloud, cloud + (L - 1) * m->grid.nu.size());
     m->directSpectrum(g, a, b, std::vector<double>(p, p + L), std::vector<double>(t, t + L), ids,
-                      std::vector<double>(x, x + L * ids.size()), variable_gravity != 0);
+                      std::vector<double>(x, x + L * ids.size()), variable_gravity != 0, c);
     std::copy(m->spectrum.begin(), m->spectrum.end(), spectrum);
     return 0;
   } catch (std::exception& e) {

@@ -32,6 +32,8 @@ SRCS := config/global_config.cpp \
         chemistry/isoprofile_chemistry.cpp chemistry/background_chemistry.cpp \
         chemistry/fixed_chemistry.cpp \
         radiative_transfer/short_characteristics.cpp \
+        cloud_model/grey_cloud_model.cpp cloud_model/power_law_cloud_model.cpp \
+        cloud_model/kh_cloud_model.cpp cloud_model/fixed_cloud_model.cpp \
         forward_model/stellar_spectrum/star_blackbody.cpp \
         forward_model/stellar_spectrum/star_file_spectrum.cpp
 

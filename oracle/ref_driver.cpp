@@ -41,6 +41,8 @@
 #include "chemistry/isoprofile_chemistry.h"
 #include "chemistry/background_chemistry.h"
 #include "radiative_transfer/short_characteristics.h"
+#include "cloud_model/select_cloud_model.h"
+#include "cloud_model/fixed_cloud_model.h"
 #include "forward_model/stellar_spectrum/star_blackbody.h"
 #include "forward_model/stellar_spectrum/star_file_spectrum.h"
 #include "additional/physical_const.h"
@@ -74,6 +76,7 @@ struct CaseText {
   std::vector<std::string> temperature;
   std::vector<std::vector<std::string>> chemistry;
   std::vector<std::string> opac_symbol, opac_folder;
+  std::vector<std::vector<std::string>> clouds;       // "type args..." of each cloud model
   bool error_inflation = false;
   std::vector<double> parameters;
   bool direct = false;                     // pyBeAR's direct models over direct_range (micron)
@@ -89,6 +92,7 @@ struct DirectInput {
   std::vector<std::string> species;
   std::vector<double> pressure, temperature;
   std::vector<std::vector<double>> mixing_ratios;
+  std::vector<std::vector<double>> cloud;      // "cloud" lines: one layer's optical depth per grid point
 };
 
 DirectInput readDirectInput(const std::string& path) {
@@ -112,6 +116,11 @@ DirectInput readDirectInput(const std::string& path) {
       d.pressure.push_back(p);
       d.temperature.push_back(t);
       d.mixing_ratios.push_back(x);
+    } else if (key == "cloud") {
+      std::vector<double> row;
+      double v;
+      while (in >> v) row.push_back(v);
+      d.cloud.push_back(row);
     }
   }
   return d;
@@ -141,6 +150,7 @@ CaseText readCase(const std::string& folder) {
     else if (key == "chemistry") c.chemistry.push_back(rest);
     else if (key == "opacity") { c.opac_symbol.push_back(rest[0]); c.opac_folder.push_back(rest[1]); }
     else if (key == "error_inflation") c.error_inflation = rest[0] == "1";
+    else if (key == "cloud") c.clouds.push_back(rest);
     else if (key == "direct_range") {
       c.direct = true;
       c.direct_range[0] = std::stod(rest[0]);
@@ -173,8 +183,17 @@ int runDirect(const CaseText& cs, bear::GlobalConfig& config, const std::string&
   const double radius = transmission || !eclipse ? d.a : 1.0;
   atmosphere.setAtmosphericStructure(d.g, radius, eclipse ? false : d.variable_gravity, d.pressure, d.temperature,
                                      d.species, d.mixing_ratios);
-  std::vector<CloudModel*> no_clouds;
-  opacity.calculate(no_clouds, std::vector<double>{});
+  // EmissionModel::setCloudProperties (emission.cpp:344-367): a fixed cloud
+  // when any optical depth is positive
+  std::vector<CloudModel*> clouds;
+  bool use_cloud = false;
+  for (auto& row : d.cloud)
+    for (double v : row) use_cloud = use_cloud || v > 0;
+  if (use_cloud) {
+    const std::vector<std::vector<double>> zeros(d.cloud.size(), std::vector<double>(d.cloud[0].size(), 0.0));
+    clouds.push_back(new FixedCloudModel(d.cloud, zeros, zeros));
+  }
+  opacity.calculate(clouds, std::vector<double>{});
   std::vector<double> spectrum;
   if (!transmission) {
     spectrum.assign(grid.nbSpectralPoints(), 0.0);
@@ -206,6 +225,14 @@ int runDirect(const CaseText& cs, bear::GlobalConfig& config, const std::string&
   writeNpy(out + "absorption_coeff.npy", k, opacity.absorption_coeff.size());
   writeNpy(out + "scattering_coeff.npy", sc, opacity.scattering_coeff.size());
   if (!spectrum.empty()) writeNpy(out + "spectrum.npy", spectrum);
+  if (use_cloud) {
+    std::vector<std::vector<double>> ext(grid.nbSpectralPoints(), std::vector<double>(cs.nb_levels, 0.0));
+    clouds[0]->convertOpticalDepth(opacity.cloud_optical_depths, ext, atmosphere.altitude);
+    std::vector<double> flat;
+    for (size_t l = 0; l < cs.nb_levels; ++l)
+      for (size_t i = 0; i < grid.nbSpectralPoints(); ++i) flat.push_back(ext[i][l]);
+    writeNpy(out + "cloud_extinction.npy", flat, cs.nb_levels);
+  }
   std::printf("direct %s N=%zu\n", cs.model.c_str(), grid.nbSpectralPoints());
   return 0;
 }
@@ -259,8 +286,16 @@ int main(int argc, char** argv) {
   // the emission model's modules (EmissionModel ctor + initModules)
   const std::vector<double> boundaries = {cs.p_bottom, cs.p_top};
   Atmosphere atmosphere(cs.nb_levels, boundaries, false);
-  OpacityCalculation opacity(&config, &grid, &atmosphere, cs.opac_symbol, cs.opac_folder, false, false);
+  OpacityCalculation opacity(&config, &grid, &atmosphere, cs.opac_symbol, cs.opac_folder, false,
+                             !cs.clouds.empty());
   ShortCharacteristics rt(&grid);
+  // the cloud modules (EmissionModel::initModules, emission_init.cpp:72-89)
+  std::vector<CloudModel*> cloud_models;
+  size_t nb_cloud_param = 0;
+  for (auto& c : cs.clouds) {
+    cloud_models.push_back(selectCloudModel(c[0], std::vector<std::string>(c.begin() + 1, c.end())));
+    nb_cloud_param += cloud_models.back()->nbParameters();
+  }
   std::unique_ptr<Temperature> temperature_profile;
   if (cs.temperature[0] == "const") temperature_profile.reset(new ConstantTemperature());
   else temperature_profile.reset(new PiecewisePolynomialTemperature(std::stoul(cs.temperature[1]),
@@ -285,15 +320,17 @@ int main(int argc, char** argv) {
   const std::vector<double> chem_param(p.begin() + nb_general, p.begin() + nb_general + nb_chem_param);
   const std::vector<double> temp_param(p.begin() + nb_general + nb_chem_param,
                                        p.begin() + nb_general + nb_chem_param + nb_temp_param);
-  const std::vector<double> modifier_param(p.begin() + nb_general + nb_chem_param + nb_temp_param,
-                                           p.begin() + nb_general + nb_chem_param + nb_temp_param + nb_modifier);
+  const size_t at_cloud = nb_general + nb_chem_param + nb_temp_param;
+  const std::vector<double> cloud_param(p.begin() + at_cloud, p.begin() + at_cloud + nb_cloud_param);
+  const std::vector<double> modifier_param(p.begin() + at_cloud + nb_cloud_param,
+                                           p.begin() + at_cloud + nb_cloud_param + nb_modifier);
+  std::vector<std::vector<double>> cloud_extinction;
   StarBlackBody star(&grid);
 
   std::vector<double> spectrum, spectrum_high_res_flux;
   std::vector<std::vector<double>> spectrum_obs(observations.size());
   bool neglect = false;
   double log_like = 0;
-  std::vector<CloudModel*> no_clouds;
 
   auto evaluate = [&]() {
     const double surface_gravity = std::pow(10, model_param[0]);
@@ -308,7 +345,11 @@ int main(int argc, char** argv) {
       else
         neglect = atmosphere.calcAtmosphereStructure(surface_gravity, model_param[1], cs.variable_gravity,
                                                      &temperature, temp_param, chemistry, chem_param);
-      opacity.calculate(no_clouds, std::vector<double>{});
+      opacity.calculate(cloud_models, cloud_param);
+      // TransmissionModel::calcModelCPU's cloud extinction (transmission.cpp:213-223)
+      cloud_extinction.assign(grid.nbSpectralPoints(), std::vector<double>(cs.nb_levels, 0.0));
+      if (!cloud_models.empty())
+        cloud_models[0]->convertOpticalDepth(opacity.cloud_optical_depths, cloud_extinction, atmosphere.altitude);
       spectrum.clear();
       return;
     }
@@ -316,7 +357,7 @@ int main(int argc, char** argv) {
     // model's is the same call (secondary_eclipse.cpp:159-176)
     neglect = atmosphere.calcAtmosphereStructure(surface_gravity, 1.0, false, &temperature, temp_param,
                                                  chemistry, chem_param);
-    opacity.calculate(no_clouds, std::vector<double>{});
+    opacity.calculate(cloud_models, cloud_param);
     spectrum.assign(grid.nbSpectralPoints(), 0.0);
     double scaling = 1.0;
     if (!eclipse) {
@@ -380,6 +421,21 @@ int main(int argc, char** argv) {
   writeNpy(out + "temperature.npy", atmosphere.temperature);
   writeNpy(out + "pressure.npy", atmosphere.pressure);
   writeNpy(out + "altitude.npy", atmosphere.altitude);
+  writeNpy(out + "scale_height.npy", atmosphere.scale_height);
+  if (!cloud_models.empty()) {
+    // [layer][wavenumber], and for transmission the level extinction [level][wavenumber]
+    const size_t N = grid.nbSpectralPoints();
+    std::vector<double> tau((cs.nb_levels - 1) * N), ext;
+    for (size_t i = 0; i < N; ++i)
+      for (size_t l = 0; l + 1 < cs.nb_levels; ++l) tau[l * N + i] = opacity.cloud_optical_depths[i][l];
+    writeNpy(out + "cloud_optical_depth.npy", tau, cs.nb_levels - 1);
+    if (transmission) {
+      ext.resize(cs.nb_levels * N);
+      for (size_t i = 0; i < N; ++i)
+        for (size_t l = 0; l < cs.nb_levels; ++l) ext[l * N + i] = cloud_extinction[i][l];
+      writeNpy(out + "cloud_extinction.npy", ext, cs.nb_levels);
+    }
+  }
   {
     std::vector<double> nd;
     for (auto& l : atmosphere.number_densities) nd.insert(nd.end(), l.begin(), l.end());

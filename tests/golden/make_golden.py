@@ -85,6 +85,13 @@ def main():
                 pre = f"p{k}_"
                 for key in ("temperature", "altitude"):
                     fixture[pre + key] = out[key]
+                if "cloud_optical_depth" in out:
+                    # the clouds: scale heights, layer optical depths and (transmission)
+                    # level extinctions on every 10th wavenumber
+                    fixture[pre + "scale_height"] = out["scale_height"]
+                    fixture[pre + "cloud_optical_depth"] = out["cloud_optical_depth"][:, ::10]
+                    if "cloud_extinction" in out:
+                        fixture[pre + "cloud_extinction"] = out["cloud_extinction"][:, ::10]
                 if transmission:
                     if name == "transmission_plain" and k == 0:
                         # extinction inputs on every 5th wavenumber
@@ -137,6 +144,8 @@ def make_direct():
                 fixture[pre + "scattering_coeff"] = out["scattering_coeff"][::7]
                 if "spectrum" in out:
                     fixture[pre + "spectrum"] = out["spectrum"]
+                if "cloud_extinction" in out and model == "transmission":
+                    fixture[pre + "cloud_extinction"] = out["cloud_extinction"][:, ::7]
             fixture["wavenumber"] = out["wavenumber"]
             path = os.path.join(ROOT, "tests", "golden", f"direct_{model}.npz")
             np.savez_compressed(path, **fixture)

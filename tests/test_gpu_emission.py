@@ -42,7 +42,7 @@ def test_grid_matches_oracle(case_factory, name):
 
 
 @pytest.mark.parametrize("name", ["tiny", "emission_c1", "emission_wide", "tiny_const_wavelength",
-                                  "tiny_const_resolution", "many_species", "hminus"])
+                                  "tiny_const_resolution", "many_species", "hminus", "cloudy_tiny"])
 def test_emission_parity(case_factory, name):
     r, o, params, _ = retrieval(case_factory, name)
     for k, p in enumerate(params):
@@ -167,3 +167,28 @@ def test_hminus_limits(case_factory):
         r.gradientBatch(pts)
     assert e.value.code == -1 and "H-" in str(e.value)
     assert abs(r.computeLikelihood(params[0]) - o.evaluate(params[0])["loglike"]) <= LOGLIKE_ATOL
+
+
+def test_cloud_limits(case_factory):
+    """Cloud layers (a grey and a power-law model) enter the single-point
+    kernels (parity in test_emission_parity); the batched and adjoint
+    kernels do not take them and refuse the case with InvalidInput.  A
+    cloud with zero optical depth leaves the spectrum as without it."""
+    r, o, params, _ = retrieval(case_factory, "cloudy_tiny")
+    pts = np.array(params[:2])
+    with pytest.raises(api.BearError) as e:
+        r.computeLikelihoodBatch(pts)
+    assert e.value.code == -1 and "cloud" in str(e.value)
+    with pytest.raises(api.BearError) as e:
+        r.gradientBatch(pts)
+    assert e.value.code == -1 and "cloud" in str(e.value)
+    p = np.array(params[0])
+    clear = p.copy()
+    clear[3 + 2 + 7] = 0.0                     # grey tau
+    clear[3 + 2 + 7 + 3] = 0.0                 # power-law tau
+    ref = o.evaluate(clear)
+    out = r.computeModel(clear, True)
+    rel = np.max(np.abs(out.spectrum - ref["spectrum"]) / np.abs(ref["spectrum"]))
+    assert rel <= SPECTRUM_RTOL
+    cloudy = r.computeModel(p, True)
+    assert np.max(np.abs(cloudy.spectrum / out.spectrum - 1)) > 1e-3   # the clouds matter

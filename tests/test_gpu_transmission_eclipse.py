@@ -14,7 +14,8 @@ SPECTRUM_RTOL = 1e-5
 LOGLIKE_ATOL = 1e-6
 
 MODELS = ["eclipse_small", "eclipse_wide", "transmission_plain", "transmission_variable_gravity",
-          "transmission_fit_mmw", "transmission_fit_scale_height", "transmission_hminus"]
+          "transmission_fit_mmw", "transmission_fit_scale_height", "transmission_hminus",
+          "cloudy_eclipse", "transmission_cloudy"]
 
 
 @pytest.mark.parametrize("name", MODELS)

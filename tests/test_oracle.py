@@ -22,7 +22,8 @@ def _golden(name):
 
 ALL_CASES = ["tiny", "emission_c1", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
              "transmission_fit_mmw", "transmission_fit_scale_height", "tiny_const_wavelength",
-             "tiny_const_resolution", "many_species", "hminus", "transmission_hminus"]
+             "tiny_const_resolution", "many_species", "hminus", "transmission_hminus", "cloudy_tiny",
+             "cloudy_eclipse", "transmission_cloudy"]
 
 
 @pytest.mark.parametrize("name", ALL_CASES)
@@ -55,6 +56,11 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
         a = o.atmosphere()
         np.testing.assert_allclose(a["temperature"], g[pre + "temperature"], rtol=1e-14)
         np.testing.assert_allclose(a["altitude"], g[pre + "altitude"], rtol=1e-13)
+        if pre + "cloud_optical_depth" in g:
+            np.testing.assert_allclose(o.scale_height(), g[pre + "scale_height"], rtol=1e-13)
+            np.testing.assert_allclose(o.cloud_optical_depths()[:, ::10], g[pre + "cloud_optical_depth"], rtol=1e-13)
+            if transmission:
+                np.testing.assert_allclose(o.cloud_extinction()[:, ::10], g[pre + "cloud_extinction"], rtol=1e-13)
         if transmission:
             if pre + "absorption_coeff" in g:
                 kap, sca = o.coefficients()
@@ -74,7 +80,7 @@ def test_oracle_matches_reference_fixtures(case_factory, name):
 @pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
 @pytest.mark.parametrize("name", ["tiny", "eclipse_small", "transmission_plain", "transmission_variable_gravity",
                                   "tiny_const_wavelength", "tiny_const_resolution", "hminus", "many_species",
-                                  "transmission_hminus"])
+                                  "transmission_hminus", "cloudy_tiny", "cloudy_eclipse", "transmission_cloudy"])
 def test_oracle_intermediates_match_live_reference(case_factory, tmp_path, name):
     folder, params = case_factory(name)
     ref = api.run_reference(folder, str(tmp_path / "ref"))
@@ -86,6 +92,11 @@ def test_oracle_intermediates_match_live_reference(case_factory, tmp_path, name)
     np.testing.assert_array_equal(s.T, ref["scattering_coeff"])
     np.testing.assert_array_equal(o.atmosphere()["number_densities"], ref["number_densities"])
     np.testing.assert_array_equal(o.atmosphere()["altitude"], ref["altitude"])
+    if "cloud_optical_depth" in ref:
+        np.testing.assert_array_equal(o.scale_height(), ref["scale_height"])
+        np.testing.assert_array_equal(o.cloud_optical_depths(), ref["cloud_optical_depth"])
+        if name.startswith("transmission"):
+            np.testing.assert_array_equal(o.cloud_extinction(), ref["cloud_extinction"])
     if not name.startswith("transmission"):
         np.testing.assert_array_equal(r["spectrum"], ref["spectrum"])
 
@@ -426,7 +437,8 @@ def test_oracle_direct_matches_reference_fixtures(tmp_path, model):
     for k, d in enumerate(inputs):
         pre = f"d{k}_"
         np.testing.assert_array_equal(d["x"], g[pre + "mixing_ratios"])
-        spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"])
+        spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"],
+                                 d["cloud"])
         atm = o.atmosphere()
         np.testing.assert_allclose(atm["altitude"], g[pre + "altitude"], rtol=1e-13)
         np.testing.assert_allclose(atm["number_densities"], g[pre + "number_densities"], rtol=1e-14)
@@ -435,17 +447,25 @@ def test_oracle_direct_matches_reference_fixtures(tmp_path, model):
         np.testing.assert_allclose(sca.T[::7], g[pre + "scattering_coeff"], rtol=1e-13)
         if pre + "spectrum" in g:
             np.testing.assert_allclose(spec, g[pre + "spectrum"], rtol=1e-12)
+        if pre + "cloud_extinction" in g:
+            np.testing.assert_allclose(o.cloud_extinction()[:, ::7], g[pre + "cloud_extinction"], rtol=1e-13)
 
 
 @pytest.mark.skipif(not os.path.exists(api.REF_BIN), reason="reference build (oracle/ref.mk) not present")
-@pytest.mark.parametrize("model", ["emission", "secondary_eclipse"])
-def test_oracle_direct_matches_live_reference(tmp_path, model):
+@pytest.mark.parametrize("model", ["emission", "secondary_eclipse", "transmission"])
+@pytest.mark.parametrize("which", [1, 2])
+def test_oracle_direct_matches_live_reference(tmp_path, model, which):
+    """Input 2 carries a caller-made cloud (the reference's FixedCloudModel)."""
     import cases
     folder, inputs = cases.direct_case(model, str(tmp_path / model))
     o = api.EmissionOracle(folder)
-    d = inputs[1]
+    d = inputs[which]
     cases.write_direct_input(folder, d)
     ref = api.run_reference(folder, str(tmp_path / "ref"))
-    spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"])
-    np.testing.assert_array_equal(spec, ref["spectrum"])
+    spec = o.direct_spectrum(d["g"], d["a"], d["b"], d["p"], d["t"], d["species"], d["x"], d["variable_gravity"],
+                             d["cloud"])
+    if model != "transmission":
+        np.testing.assert_array_equal(spec, ref["spectrum"])
     np.testing.assert_array_equal(o.atmosphere()["altitude"], ref["altitude"])
+    if d["cloud"] is not None and model == "transmission":
+        np.testing.assert_array_equal(o.cloud_extinction(), ref["cloud_extinction"])
