@@ -610,6 +610,10 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 // (last_t: the segment at which the sum first passes 10, else L - 2):
 //  0. the kept forward (transmissionBatchKernel) stores tau_t in slot 1 and
 //     last_t in slot 2 of level t;
+//     The kept forward also stores per (point, tile of 64 wavenumbers, path)
+//     the tile's largest last_t (reach), with which steps 2 and 3 pass over
+//     paths and wavenumber steps that end below their segments without
+//     loading them;
 //  1. tangentAdjointKernel: tau-bar_t = e c_t e^-tau_t over slot 1; sums over wavenumbers of
 //     e (1 - e^-tau_t) (d logL / d c_t) in slot NS+1 of level t, of e (the
 //     R^2 term) in slot NS+1 of level L-1 and of d logL / d depth x depth
@@ -675,7 +679,8 @@ constexpr int kWindowAlign = 128;
 
 __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restrict__ keep, int nb_levels, int n,
                                                           size_t stride, int ranges, int range_first,
-                                                          int nu_per_range, TileWindow win,
+                                                          int nu_per_range, TileWindow win, int tiles,
+                                                          const int* __restrict__ reach,
                                                           double* __restrict__ partial) {
   __shared__ float s_sh[kChordBlock][kChordNu + 1];           // S_q in FP32 (the gradient's inputs are FP32)
   __shared__ float a_sh[kChordBlock][kChordNu + 1];
@@ -699,7 +704,13 @@ __global__ __launch_bounds__(256) void chordAdjointKernel(const float* __restric
   if (tid < 2) s_reach[tid] = 0;
   __syncthreads();
   const int nu0 = range * nu_per_range, nu1 = min(n, nu0 + nu_per_range);
+  const int* rl = reach + size_t(m) * tiles * P;
   for (int v0 = nu0, step = 0; v0 < nu1; v0 += kChordNu, ++step) {
+    // a step (within one tile of 64) in which no path of the block reaches
+    // its first segment adds nothing: nothing is loaded for it
+    const int tile_v = v0 / kWave;
+    const bool live = tid < kChordBlock && t0 + tid < P && rl[size_t(tile_v) * P + t0 + tid] >= q0;
+    if (!__syncthreads_or(live)) continue;                         // uniform over the workgroup
     int reach = 0;
     for (int e = tid; e < kChordBlock * kChordNu; e += 256) {
       const int row = e / kChordNu, col = e % kChordNu;
@@ -793,7 +804,8 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
                                                               const double* __restrict__ rayleigh, int nb_levels,
                                                               int n, size_t stride, int tiles, TileWindow win,
                                                               int batch, int ns, float* __restrict__ keep,
-                                                              double* __restrict__ partial) {
+                                                              double* __restrict__ partial,
+                                                              const int* __restrict__ reach) {
   const int L = nb_levels, K = ns + 4;
   const int rows = segmentRows(L);
   const int lane = int(threadIdx.x);
@@ -809,6 +821,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
     const double* c = chord_t + size_t(m) * (L - 1) * rows;
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;
     const double sr = (rayleigh && valid) ? rayleigh[i] : 0.0;
+    const int* rl = reach + (size_t(m) * tiles + tile) * (L - 1);   // per path: the wavefront's largest last_t
     double s_hi = 0.0;                                               // S-bar of the segment above
     // kAdjointSegments segments per pass over the paths (one load of
     // tau-bar_t and last_t for all of them); chord[q][t] = 0 for t > q and
@@ -820,19 +833,30 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
 #pragma unroll
       for (int u = 0; u < kAdjointSegments; ++u) sb[u] = 0.0;
       const int t_end = min(qb + kAdjointSegments, L - 1);
-      // this lane's tau-bar and last segment of path t; path t + 1's are
-      // loaded while path t's products run
+      // this lane's tau-bar and last segment of path t, for the paths that
+      // reach the pass (the next one's are loaded while a path's products
+      // run); the others end below every segment of the pass
       auto column = [&](int t) { return kp + size_t(t) * level_step; };
-      double tb = double(column(0)[kKeptValue]);
-      int last = int(column(0)[2 * kKeptValue]);
-      for (int t = 0; t < t_end; ++t) {
+      auto next = [&](int t) {                         // the first path from t on that reaches the pass
+        while (t < t_end && rl[t] < qb) ++t;
+        return t;
+      };
+      int t = next(0);
+      double tb = 0.0;
+      int last = -1;
+      if (t < t_end) {
+        tb = double(column(t)[kKeptValue]);
+        last = int(column(t)[2 * kKeptValue]);
+      }
+      while (t < t_end) {
+        const int t_next = next(t + 1);
         double tb_next = 0.0;
-        int last_next = 0;
-        if (t + 1 < t_end) {                           // uniform
-          tb_next = double(column(t + 1)[kKeptValue]);
-          last_next = int(column(t + 1)[2 * kKeptValue]);
+        int last_next = -1;
+        if (t_next < t_end) {                          // uniform
+          tb_next = double(column(t_next)[kKeptValue]);
+          last_next = int(column(t_next)[2 * kKeptValue]);
         }
-        if (__any(last >= qb)) {                       // uniform: else the path ends below every segment of the pass
+        {
           const double* ct = c + size_t(t) * rows + qb;
           const int k = last - qb;                     // this lane sums segments qb .. qb + k
           // Past segment qb + k only the high word of tau-bar is cleared:
@@ -846,6 +870,7 @@ __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const
         }
         tb = tb_next;
         last = last_next;
+        t = t_next;
       }
 #pragma unroll
       for (int u = kAdjointSegments - 1; u >= 0; --u) {
@@ -878,15 +903,15 @@ void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const dou
                                const double* rayleigh, Records rec, int L, int n, int tiles, TileWindow win,
                                int batch, const double* fbar, const double* depth, float* keep, double* partial,
                                int ranges, int r0, int r1, int nu_per_range, double* chord_partial,
-                               const double* chord_t) {
+                               const double* chord_t, int* reach) {
   hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, win, batch, NS,
                      fbar, depth, keep, partial);
   const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
   hipLaunchKernelGGL(chordAdjointKernel,
                      dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(r1 - r0)), dim3(256), 0, s,
-                     keep, L, n, stride, ranges, r0, nu_per_range, win, chord_partial);
+                     keep, L, n, stride, ranges, r0, nu_per_range, win, tiles, reach, chord_partial);
   hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord_t, rayleigh, L, n, stride, tiles,
-                     win, batch, NS, keep, partial);
+                     win, batch, NS, keep, partial, reach);
   hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, win,
                      batch, keep, partial);
 }
@@ -939,8 +964,11 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     if (ranges == 1 || (ranges - 1) * nu_per_range < n) break;
     --ranges;
   }
-  const size_t chord_point =
-      transmission ? size_t(P) * P * (kChordRanges + 1) + size_t(P) * segmentRows(L) : 0;   // doubles per point
+  // chord partial sums, chord gradients, transposed chords and the reach
+  // table (int32 [tiles][P]) of each point
+  const size_t chord_point = transmission ? size_t(P) * P * (kChordRanges + 1) + size_t(P) * segmentRows(L) +
+                                                (size_t(tiles) * P + 1) / 2
+                                          : 0;                                   // doubles per point
   const size_t partial_point = size_t(L) * K * size_t(tiles);               // doubles per point
   const size_t kept_values = size_t(L) * 3 * stride_;                          // FP32 values per point
   const size_t kept_point = (kept_values + 1) / 2 + partial_point + stride_ + chord_point;  // doubles per point
@@ -1011,8 +1039,13 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
                       rec_shifts_, groups_};
     float* keep = kept ? reinterpret_cast<float*>(adj_intensity_) : nullptr;
     const bool windowed = transmission && window_ranges < ranges;   // the kept forward runs per window
+    // transmission: chord partial sums, chord gradients, transposed chords, reach
+    double* chord_partial = adj_partial_ + size_t(cb) * partial_point;
+    double* chord_out = chord_partial + size_t(cb) * P * P * kChordRanges;
+    double* chord_t = chord_out + size_t(cb) * P * P;
+    int* reach = reinterpret_cast<int*>(chord_t + size_t(cb) * P * segmentRows(L));
     launchBatchSpectrum(transmission ? ModelKind::transmission : ModelKind::emission, size_t(cb), launch, first,
-                        windowed ? nullptr : keep);
+                        windowed ? nullptr : keep, 0, 0, transmission && !windowed ? reach : nullptr);
     launchBatchRest(kind, size_t(cb), first);
     double* gbar = adj_gbar_ + first * n_bins_;
     double* fscale = eclipse ? adj_fscale_ + first * n_bins_ : gbar;
@@ -1034,9 +1067,6 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
                        s, adj_ptr_, adj_bin_, adj_weight_, fscale, nb, n, stride_, adj_fbar_);
     check(hipGetLastError(), "adjoint flux kernel");
     if (transmission) {
-      double* chord_partial = adj_partial_ + size_t(cb) * partial_point;
-      double* chord_out = chord_partial + size_t(cb) * P * P * kChordRanges;
-      double* chord_t = chord_out + size_t(cb) * P * P;
       const int rows = segmentRows(L);
       hipLaunchKernelGGL(chordTransposeKernel, dim3(unsigned((P * rows + kBlock - 1) / kBlock), unsigned(cb)),
                          dim3(kBlock), 0, s, batch_chord_, L, chord_t);
@@ -1049,14 +1079,14 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
           win.first = int(nu_first / kWave);
           win.count = int((nu_end + kWave - 1) / kWave) - win.first;
           win.kept_origin = nu_first / kWave * size_t(L) * kKeptLevel;
-          launchBatchSpectrum(ModelKind::transmission, 1, launch, first, keep, nu_first, nu_end - nu_first);
+          launchBatchSpectrum(ModelKind::transmission, 1, launch, first, keep, nu_first, nu_end - nu_first, reach);
         }
         switch (NS) {
 #define BEAR_TADJ(NSV)                                                                                             \
   case NSV:                                                                                                        \
     launchTransmissionAdjoint<NSV>(dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, tables_, stride_, rayleigh_,  \
                                    rec, L, n, tiles, win, cb, adj_fbar_, batch_flux_ + first * stride_, keep,     \
-                                   adj_partial_, ranges, r0, r1, nu_per_range, chord_partial, chord_t);           \
+                                   adj_partial_, ranges, r0, r1, nu_per_range, chord_partial, chord_t, reach);    \
     break;
           BEAR_TADJ(4)
           BEAR_TADJ(8)

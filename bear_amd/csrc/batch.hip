@@ -333,7 +333,8 @@ template <class Op, bool kKeep = false>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
     Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
-    TileWindow win, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep) {
+    TileWindow win, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep,
+    int* __restrict__ reach) {
   using K = typename Op::K;
   using Species = typename Op::Species;
   extern __shared__ double lds[];
@@ -392,6 +393,19 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
       if constexpr (kKeep) {
         int stop[kTangentBlock];
         tangentBlock<K, true>(S + j, tile, chords, tangents, t0, L - 1, tau, stop);
+        // per tile of 64 wavenumbers (a wavefront) and path: the largest
+        // last segment, with which the adjoint passes over paths and steps
+        // that end below its segments
+        const int first64 = tl * tile + (j & ~63);
+        int* rl = reach + (size_t(m) * ((n + 63) / 64) + first64 / 64) * (L - 1);
+#pragma unroll
+        for (int k = 0; k < kTangentBlock; ++k) {
+          const int t = t0 + k;
+          if (t < L - 1) {                                                 // uniform
+            const int mx = waveMaxInt(i < n ? stop[k] : -1);
+            if ((j & 63) == 0 && first64 < n) rl[t] = mx;
+          }
+        }
         if (i < n)
 #pragma unroll
           for (int k = 0; k < kTangentBlock; ++k) {
@@ -506,32 +520,36 @@ template <int NS>
 void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
                         const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
                         const double* chord, const float* chord32, int L, int n, TileWindow win, int tile,
-                        int batch, int group, double* depth, float* keep) {
+                        int batch, int group, double* depth, float* keep, int* reach) {
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       if (keep)
         hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true>), grid, block, lds, s, tab64, base, stride,
-                           rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep);
+                           rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep, reach);
       else
         hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                           rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
+                           rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr, nullptr);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         nullptr);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         nullptr);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         nullptr);
       break;
     case TablePrecision::int16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr);
+                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         nullptr);
       break;
   }
 }
@@ -678,7 +696,7 @@ void DeviceEngine::reserveBatch(size_t batch) {
 
 // ---------------------------------------------------------------- launches
 void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& lc, size_t first,
-                                       float* keep, size_t nu_first, size_t nu_count) {
+                                       float* keep, size_t nu_first, size_t nu_count, int* reach) {
   hipStream_t s = static_cast<hipStream_t>(stream_);
   const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,
                     rec_shifts_, groups_};
@@ -703,7 +721,7 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
                            rayleigh_, rec, batch_chord_, batch_chord32_, L, n, win, lc.tile, B, lc.group, flux,   \
-                           keep);                                                                              \
+                           keep, reach);                                                                       \
     break;
       BEAR_T(4)
       BEAR_T(8)

@@ -152,16 +152,33 @@ __device__ __forceinline__ void shortCharacteristicsLayer(double tau, double b_p
 // shared sets, group1 holding these NH slots' group bits, and the densities
 // from the level's contiguous copy (NH <= 16, the first NH slots); otherwise
 // both from each slot's record.
+// The two halves are also available apart (loadRows, kappaFrom), so that a
+// kernel can issue the next level's row loads before this level's other
+// work and find them arrived when it needs them.
+template <int NH>
+__device__ __forceinline__ void loadRows(const double* __restrict__ tab, const LevelSpecies* __restrict__ d,
+                                         double (&v)[NH][4]) {
+#pragma unroll
+  for (int s = 0; s < NH; ++s)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
+}
+
+template <int NH>
+__device__ __forceinline__ double kappaFrom(const double (&v)[NH][4], const LevelSpecies* __restrict__ d,
+                                            const LevelState& lev, StencilGroups groups);
+
 template <int NH>
 __device__ __forceinline__ double kappaTerms(const double* __restrict__ tab, size_t stride,
                                              const LevelSpecies* __restrict__ d, const LevelState& lev,
                                              StencilGroups groups, bool reload, double (&v)[NH][4]) {
-  if (reload) {
-#pragma unroll
-    for (int s = 0; s < NH; ++s)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[s][q] = tab[size_t(d[s].block[q]) << 6];
-  }
+  if (reload) loadRows<NH>(tab, d, v);
+  return kappaFrom<NH>(v, d, lev, groups);
+}
+
+template <int NH>
+__device__ __forceinline__ double kappaFrom(const double (&v)[NH][4], const LevelSpecies* __restrict__ d,
+                                            const LevelState& lev, StencilGroups groups) {
   double k = 0.0;
   if (groups.grouped) {
     double a[4], b[4];
@@ -273,6 +290,21 @@ __device__ __forceinline__ double dppMove(double v) {
 
 __device__ __forceinline__ double joinHalves(unsigned lo, unsigned hi) {
   return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+// the largest of an int over the 64 lanes of a wavefront, in every lane:
+// v_permlane32_swap and v_permlane16_swap pair lanes j, j+32 and then j,
+// j+16, DPP moves finish within rows of 16 (as waveTotals does for sums)
+__device__ __forceinline__ int waveMaxInt(int v) {
+  const auto a = __builtin_amdgcn_permlane32_swap(unsigned(v), unsigned(v), false, false);
+  v = max(int(a[0]), int(a[1]));
+  const auto b = __builtin_amdgcn_permlane16_swap(unsigned(v), unsigned(v), false, false);
+  v = max(int(b[0]), int(b[1]));
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));      // quad_perm [1,0,3,2]
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));      // quad_perm [2,3,0,1]
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));     // row_half_mirror
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));     // row_mirror
+  return v;
 }
 
 // Sums over the 64 lanes of a wavefront of K values at once (K a multiple of

@@ -409,8 +409,10 @@ class DeviceEngine {
   // nu_count > 0 (transmission): only wavenumbers [nu_first, nu_first +
   // nu_count) of one point, kept from the start of `keep` (nu_first a
   // multiple of the launch's tile)
+  // (transmission with keep: also each kept tile's reach, int32
+  // [point][tiles of 64][paths], the largest last segment of every path)
   void launchBatchSpectrum(ModelKind kind, size_t batch, const BatchLaunch& launch, size_t first = 0,
-                           float* keep = nullptr, size_t nu_first = 0, size_t nu_count = 0);
+                           float* keep = nullptr, size_t nu_first = 0, size_t nu_count = 0, int* reach = nullptr);
   void launchBatchRest(ModelKind kind, size_t batch, size_t first = 0);
   BatchLaunch tuned(ModelKind kind, size_t batch);
   BatchLaunch tune(ModelKind kind, size_t batch);

@@ -87,8 +87,10 @@ __device__ __forceinline__ void writeFlux(int i, double nu, double i1, double i2
 // levels bottom to top (short_characteristics.cpp:39-125).  The four table
 // values of every stencil slot stay in registers from level to level and are
 // reloaded only where the rows change, so each table row is read about once
-// per thread; kappa and the intensities never leave registers.  The stencil
-// fields are wave-uniform (scalar loads).  EX: the extras (H- continuum,
+// per thread; kappa and the intensities never leave registers.  The next
+// level's rows are requested as soon as this level's kappa is formed, so
+// that their latency passes under this layer's Planck function and
+// attenuation factors.  The stencil fields are wave-uniform (scalar loads).  EX: the extras (H- continuum,
 // clouds) are formed once per thread and added per level and layer: the H-
 // term to kappa, the cloud optical depth to the layer's
 // (short_characteristics.cpp:75-76).
@@ -107,15 +109,18 @@ __global__ __launch_bounds__(kBlock) void emissionKernelCached(
   double v[NS][4];
   double hm[7], cf[kMaxClouds];
   if (EX) loadExtras(ex, i, n, wavelength[i], hm, cf);
-  double k_prev = kappaTerms<NS>(tab, stride, species, levels[0], groups, true, v);
+  loadRows<NS>(tab, species, v);
+  double k_prev = kappaFrom<NS>(v, species, levels[0], groups);
   if (EX && ex.hminus) k_prev += hminusKappa(hm, levels[0]);
+  if (nb_levels > 1 && levels[1].reload) loadRows<NS>(tab, species + NS, v);
   double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
   double z_prev = levels[0].altitude;
   double i1 = b_prev, i2 = b_prev;
   for (int l = 1; l < nb_levels; ++l) {
-    double k_next = kappaTerms<NS>(tab, stride, species + size_t(l) * NS, levels[l], groups,
-                                   levels[l].reload != 0, v);
+    const LevelSpecies* d = species + size_t(l) * NS;
+    double k_next = kappaFrom<NS>(v, d, levels[l], groups);
     if (EX && ex.hminus) k_next += hminusKappa(hm, levels[l]);
+    if (l + 1 < nb_levels && levels[l + 1].reload) loadRows<NS>(tab, d + NS, v);
     const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
     const double z_next = levels[l].altitude;
     double tau = (z_next - z_prev) * (k_next + k_prev) * 0.5;
