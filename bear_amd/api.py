@@ -105,6 +105,7 @@ def lib():
         L.bear_forward_model_calc_model_gpu_noalloc.argtypes = [vp, _dp, sz, vp, C.POINTER(C.c_void_p),
                                                                 C.POINTER(C.c_int)]
         L.bear_retrieval_set_table_precision.argtypes = [vp, C.c_int]
+        L.bear_retrieval_set_device_staging.argtypes = [vp, C.c_int]
         L.bear_retrieval_set_batch_options.argtypes = [vp, C.c_int, C.c_int]
         L.bear_retrieval_compute_likelihood_batch.argtypes = [vp, _dp, sz, sz, _dp]
         L.bear_retrieval_compute_model_batch.argtypes = [vp, _dp, sz, sz, _dp, _dp, C.POINTER(C.c_int)]
@@ -305,6 +306,11 @@ class Retrieval:
         return ForwardModelOutput(bool(neg.value), spec if spec is not None else np.zeros(0), out)
 
     # ---- batches: parameters [batch, nb_parameters]
+    def set_device_staging(self, enable=True):
+        """Form the batched calls' atmospheres and stencils on the device
+        (the default) or on the host."""
+        _check(self._L.bear_retrieval_set_device_staging(self._h, int(bool(enable))))
+
     def set_table_precision(self, precision="fp64"):
         """Table storage of the batched path: "fp64", "fp32", "fp16", "bf16" or
         "int16" (16-bit integers scaled per species and wavenumber)."""

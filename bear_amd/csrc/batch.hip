@@ -845,7 +845,10 @@ void DeviceEngine::releaseGraph() {
   graph_key_ = TuneKey{-1, -1, 0};
 }
 
-void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
+void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) { evaluateBatch(kind, batch, false); }
+
+void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch, bool staged) {
+  if (staged && (!stage_blob_ || batch > stage_capacity_)) raise(kInternal, "device staging not prepared");
   if (species_stride_ > 16) raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
   if (hminus_) raise(kInvalidInput, "the H- continuum is computed by the single-point kernels only");
   if (cloud_models_ > 0) raise(kInvalidInput, "cloud layers are computed by the single-point kernels only");
@@ -859,13 +862,17 @@ void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
   };
   const BatchLaunch launch = tuned(kind, batch);
   auto sequence = [&] {
-    check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+    if (staged) launchStage(batch);
+    else check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
     launchBatchSpectrum(kind, batch, launch);
     launchBatchRest(kind, batch);
+    if (staged)
+      check(hipMemcpyAsync(stage_neglect_host_, stage_neglect_, batch * sizeof(int), hipMemcpyDeviceToHost, s),
+            "read neglect flags");
   };
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
   if (use_graph_) {
-    if (!graph_exec_ || !same(graph_key_, key)) {
+    if (!graph_exec_ || !same(graph_key_, key) || graph_staged_ != staged) {
       releaseGraph();
       check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
       hipGraph_t graph = nullptr;
@@ -883,6 +890,7 @@ void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) {
       check(e, "instantiate graph");
       graph_exec_ = exec;
       graph_key_ = key;
+      graph_staged_ = staged;
     }
     check(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec_), s), "launch graph");
   } else {

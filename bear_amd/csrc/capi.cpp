@@ -242,6 +242,13 @@ int bear_retrieval_set_table_precision(bear_retrieval* r, int precision) {
   });
 }
 
+int bear_retrieval_set_device_staging(bear_retrieval* r, int enable) {
+  return guarded([&] {
+    if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");
+    r->impl->setDeviceStaging(enable != 0);
+  });
+}
+
 int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autotune) {
   return guarded([&] {
     if (!r) bear_amd::raise(BEAR_ERROR_INVALID_INPUT, "null retrieval");

@@ -168,6 +168,34 @@ struct BatchModel {
   double pad[2];
 };
 
+// Device staging of a batch: each live point's atmosphere (temperature
+// profile, chemistry, altitudes), stencils and scalars formed on the device
+// from its parameters, the model's fixed data uploaded once as one blob
+// (this header's counts and byte offsets into it).  The same steps as the
+// host's Atmosphere::compute and TabulatedModel::stageLevels.
+struct StageModule { int background, sodium_free, first, count; };        // chemistry modules in order
+struct StageSpecies { int species_id, partner0, partner1, grid, row_offset, pad[3]; };
+// a table grid: temperature rows (their temperatures at t_offset), where
+// each row's pressures start in the [t][p] flattening (tstart_offset), the
+// rows' temperatures in that flattening (temps_offset) and the picks [L][nt]
+struct StageGrid { int nt, t_offset, picks_offset, tstart_offset, temps_offset, pad[3]; };
+struct StagePick { uint32_t lower, upper; double fraction; };           // per level and temperature row
+struct StagePlan {
+  int L = 0, S = 0, NS = 0, kind = 0;             // kind: ModelKind
+  int isothermal = 0, basis_entries = 0, nb_temp = 0, nb_chem = 0;
+  int g0 = 0, nb_params = 0, error_inflation = 0, nb_obs = 0;
+  int alt_kind = 0;                               // 0 hydrostatic, 1 fixed mean molecular weight, 2 scale height
+  int variable_gravity = 0, grouped = 0, rayleigh_id = -1;
+  uint32_t group1 = 0, blocks_per_row = 0;
+  int group_grid[2] = {-1, -1};
+  int nb_modules = 0, nb_active = 0, n_ids = 0, mods_offset = 0;
+  int na_id = 0, k_id = 0, h2_id = 0, he_id = 0;
+  // byte offsets into the blob
+  size_t pressure = 0, log_ratio = 0, basis_index = 0, basis_weight = 0, modules = 0, module_species = 0,
+         active = 0, id_slot = 0, mol_weight = 0, species = 0, grids = 0, grid_t = 0, grid_tstart = 0, row_ids = 0,
+         row_temps = 0, picks = 0, obs_mods = 0;
+};
+
 // Launch shape of the batched spectrum kernel, picked by the autotuner.
 struct BatchLaunch {
   int block = 256;               // threads per workgroup
@@ -292,6 +320,17 @@ class DeviceEngine {
   double* batchShifts(size_t b) { return reinterpret_cast<double*>(record(b) + rec_shifts_); }
   // spectra, bins and log-likelihoods of live points [0, batch)
   void evaluateBatch(ModelKind kind, size_t batch);
+  // Device staging (StagePlan): the model's fixed data once, then batches
+  // whose records the device forms from the parameters placed in
+  // stagedParameters() ([batch][nb_params], pinned); evaluateBatch(kind,
+  // batch, true) stages and evaluates, and stagedNeglect() holds each
+  // point's neglect flag afterwards.
+  void uploadStagePlan(const StagePlan& plan, const std::vector<unsigned char>& blob);
+  bool hasStagePlan() const { return stage_blob_ != nullptr; }
+  void reserveStage(size_t batch);
+  double* stagedParameters() { return stage_params_host_; }
+  const int* stagedNeglect() const { return stage_neglect_host_; }
+  void evaluateBatch(ModelKind kind, size_t batch, bool staged);
   const double* batchLogLikelihood() const { return batch_ll_host_; }
   void copyBatchBins(size_t batch, std::vector<double>& out);          // [batch][bins], internal order
   void copyBatchSpectrum(ModelKind kind, size_t batch, double* host);  // [batch][N], output units
@@ -430,6 +469,15 @@ class DeviceEngine {
   float* table_base_ = nullptr;                  // [species stride][stride]
   bool use_graph_ = true, autotune_ = true;
   size_t batch_capacity_ = 0, rec_bytes_ = 0, rec_levels_ = 0, rec_species_ = 0, rec_adjoint_ = 0, rec_shifts_ = 0;
+  StagePlan stage_plan_;
+  unsigned char* stage_blob_ = nullptr;
+  double* stage_params_ = nullptr;               // [capacity][nb_params]
+  double* stage_params_host_ = nullptr;          // pinned
+  int* stage_neglect_ = nullptr;
+  int* stage_neglect_host_ = nullptr;            // pinned
+  size_t stage_capacity_ = 0;
+  void launchStage(size_t batch);
+  void releaseStage();
   unsigned char* batch_host_ = nullptr;          // pinned records
   unsigned char* batch_dev_ = nullptr;
   double* batch_flux_ = nullptr;                 // [capacity][stride]
@@ -443,6 +491,7 @@ class DeviceEngine {
   std::vector<std::pair<TuneKey, BatchLaunch>> tuned_;
   void* graph_exec_ = nullptr;
   TuneKey graph_key_{-1, -1, 0};
+  bool graph_staged_ = false;
   BatchLaunch last_launch_;
   bool last_used_graph_ = false;
   size_t batch_evaluations_ = 0;

@@ -374,6 +374,7 @@ DeviceEngine::~DeviceEngine() {
   if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
   releaseBatch();
   releaseAdjoint();
+  releaseStage();
   for (void* p : {(void*)wavenumber_, (void*)wavelength_, (void*)tables_, (void*)rayleigh_, (void*)hminus_,
                   (void*)flux_, (void*)weights_, (void*)bins_desc_, (void*)bin_chunks_, (void*)bins_,
                   (void*)chunk_sums_, (void*)ll_, (void*)staging_dev_, (void*)cloud_tau_, (void*)spectrum_, (void*)adj_ptr_, (void*)adj_bin_, (void*)adj_weight_,

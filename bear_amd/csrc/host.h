@@ -171,9 +171,12 @@ class OpacitySpecies {
   // same temperatures and, per temperature, the same pressures: bracket()
   // then gives the same fractions for every (p, T)
   bool sameGrid(const OpacitySpecies& other) const;
+  // preparePressures' picks: per level and temperature row the two pressure
+  // rows and the log-pressure fraction
+  struct PressurePick { uint16_t lower, upper; double fraction; };
+  const std::vector<PressurePick>& pressurePicks() const { return pressure_picks_; }
 
  private:
-  struct PressurePick { uint16_t lower, upper; double fraction; };
   std::vector<PressurePick> pressure_picks_;          // [level][temperature row]
   void bracketTemperature(double t, size_t& lo_i, size_t& hi_i) const;
 };
@@ -296,6 +299,12 @@ class Atmosphere {
                double* radius_bar = nullptr, double* option_bar = nullptr) const;
   // copies are independent (each owns its temperature profile's state)
   Atmosphere(const Atmosphere& other);
+  // the profile and chemistry as plain data (device staging)
+  bool isothermal() const { return isothermal_; }
+  const std::vector<PiecewisePolynomial::Basis>& temperatureBasis() const { return temperature_basis_; }
+  const std::vector<ChemistryModule>& chemistryModules() const { return chemistry_; }
+  const std::vector<int>& activeIds() const { return active_ids_; }
+  size_t nbIds() const { return n_ids_; }
  private:
   size_t n_ids_;
   std::unique_ptr<PiecewisePolynomial> poly_;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 
 namespace bear_amd {
@@ -310,6 +311,146 @@ void TabulatedModel::stageBatch(const double* params, size_t batch, size_t strid
     }
   }
   if (!failure.empty()) raise(failure_code, failure);
+}
+
+// The model's fixed data for the device staging (StagePlan): pressures,
+// the temperature profile's basis per level, the chemistry modules, the
+// active species, and per table grid its temperature rows, pressure picks
+// and row ids, laid out as one blob.
+bool TabulatedModel::prepareStaging(size_t nb_params, bool error_inflation) {
+  if (staged_params_ == nb_params) return true;
+  if (hminus_ || !clouds_.empty()) return false;
+  const Atmosphere& atm = atmosphere_;
+  const size_t L = atm.nb_levels, S = species_.size(), NS = engine_.speciesStride();
+  StagePlan P;
+  std::vector<unsigned char> blob;
+  auto put = [&](const void* data, size_t bytes) {
+    const size_t off = (blob.size() + 15) / 16 * 16;
+    blob.resize(off + std::max<size_t>(bytes, 1), 0);
+    if (bytes) std::memcpy(blob.data() + off, data, bytes);
+    return off;
+  };
+  auto putv = [&](const auto& v) { return put(v.data(), v.size() * sizeof(v[0])); };
+  P.L = int(L);
+  P.S = int(S);
+  P.NS = int(NS);
+  P.kind = int(kind());
+  P.isothermal = atm.isothermal();
+  P.nb_temp = int(nb_temp_);
+  P.nb_chem = int(nb_chem_);
+  P.g0 = int(nb_general_);
+  P.nb_params = int(nb_params);
+  P.error_inflation = error_inflation;
+  P.nb_obs = int(observations_.size());
+  P.mods_offset = int(nb_general_ + nb_chem_ + nb_temp_ + nb_cloud_);
+  P.alt_kind = altitudeKind();
+  P.variable_gravity = variableGravity();
+  const StencilGroups groups = engine_.stencilGroups();
+  P.grouped = int(groups.grouped);
+  P.group1 = groups.group1;
+  P.rayleigh_id = rayleigh_species_;
+  P.blocks_per_row = uint32_t(engine_.tableStride() / 64);
+  P.na_id = sid::Na;
+  P.k_id = sid::K;
+  P.h2_id = sid::H2;
+  P.he_id = sid::He;
+  P.pressure = putv(atm.pressure);
+  std::vector<double> log_ratio(L, 0.0);
+  for (size_t l = 1; l < L; ++l) log_ratio[l] = std::log(atm.pressure[l] / atm.pressure[0]);
+  P.log_ratio = putv(log_ratio);
+  if (!atm.isothermal()) {
+    const auto& basis = atm.temperatureBasis();
+    size_t e = 1;
+    for (const auto& b : basis) e = std::max(e, b.index.size());
+    std::vector<int> bi(L * e, -1);
+    std::vector<double> bw(L * e, 0.0);
+    for (size_t l = 0; l < L; ++l)
+      for (size_t k = 0; k < basis[l].index.size(); ++k) {
+        bi[l * e + k] = int(basis[l].index[k]);
+        bw[l * e + k] = basis[l].weight[k];
+      }
+    P.basis_entries = int(e);
+    P.basis_index = putv(bi);
+    P.basis_weight = putv(bw);
+  }
+  std::vector<StageModule> modules;
+  std::vector<int> module_species;
+  for (const auto& m : atm.chemistryModules()) {
+    modules.push_back(StageModule{m.background, m.sodium_free, int(module_species.size()), int(m.species.size())});
+    module_species.insert(module_species.end(), m.species.begin(), m.species.end());
+  }
+  P.nb_modules = int(modules.size());
+  P.modules = putv(modules);
+  P.module_species = putv(module_species);
+  const std::vector<int>& active = atm.activeIds();
+  P.nb_active = int(active.size());
+  P.n_ids = int(atm.nbIds());
+  std::vector<int> id_slot(atm.nbIds(), -1);
+  std::vector<double> weight(atm.nbIds());
+  for (size_t a = 0; a < active.size(); ++a) id_slot[size_t(active[a])] = int(a);
+  for (size_t j = 0; j < atm.nbIds(); ++j) weight[j] = molecularWeight(int(j));
+  P.active = putv(active);
+  P.id_slot = putv(id_slot);
+  P.mol_weight = putv(weight);
+  // table grids: the stencil groups' representatives, or each species its own
+  std::vector<int> grid_rep, species_grid(S, -1);
+  if (groups.grouped) {
+    for (int g = 0; g < 2; ++g)
+      if (group_rep_[g] >= 0) {
+        P.group_grid[g] = int(grid_rep.size());
+        grid_rep.push_back(group_rep_[g]);
+      }
+    for (size_t s = 0; s < S; ++s) species_grid[s] = P.group_grid[(groups.group1 >> s) & 1u];
+  } else {
+    for (size_t s = 0; s < S; ++s) {
+      species_grid[s] = int(grid_rep.size());
+      grid_rep.push_back(int(s));
+    }
+  }
+  std::vector<StageGrid> grids;
+  std::vector<double> grid_t, row_temps;
+  std::vector<int> tstart;
+  std::vector<StagePick> picks;
+  for (int r : grid_rep) {
+    const OpacitySpecies& sp = species_[size_t(r)];
+    const size_t nt = sp.by_temperature.size();
+    StageGrid G{int(nt), int(grid_t.size()), int(picks.size()), int(tstart.size()), int(row_temps.size()), {0, 0, 0}};
+    int start = 0;
+    for (size_t i = 0; i < nt; ++i) {
+      grid_t.push_back(sp.by_temperature[i].front().temperature);
+      tstart.push_back(start);
+      for (const TableRow& row : sp.by_temperature[i]) row_temps.push_back(row.temperature);
+      start += int(sp.by_temperature[i].size());
+    }
+    for (const auto& pk : sp.pressurePicks()) picks.push_back(StagePick{pk.lower, pk.upper, pk.fraction});
+    if (picks.size() != size_t(G.picks_offset) + L * nt) raise(kInternal, "pressure picks do not match the levels");
+    grids.push_back(G);
+  }
+  std::vector<StageSpecies> species(S);
+  std::vector<uint32_t> row_ids;
+  for (size_t s = 0; s < S; ++s) {
+    const OpacitySpecies& sp = species_[s];
+    const int p0 = sp.cia_partners.size() > 0 ? sp.cia_partners[0] : -1;
+    const int p1 = sp.cia_partners.size() > 1 ? sp.cia_partners[1] : -1;
+    if (sp.cia_partners.size() > 2) return false;
+    species[s] = StageSpecies{sp.species_id, p0, p1, species_grid[s], int(row_ids.size()), {0, 0, 0}};
+    for (const auto& row : sp.by_temperature)
+      for (const TableRow& r : row) row_ids.push_back(r.id);
+  }
+  P.grids = putv(grids);
+  P.grid_t = putv(grid_t);
+  P.grid_tstart = putv(tstart);
+  P.row_temps = putv(row_temps);
+  P.picks = putv(picks);
+  P.species = putv(species);
+  P.row_ids = putv(row_ids);
+  std::vector<int> obs_mods;
+  for (const auto& o : observations_) obs_mods.push_back(o.modifier_parameters);
+  P.obs_mods = putv(obs_mods);
+  if ((L * (5 + active.size()) + nb_temp_) * sizeof(double) > 60 * 1024) return false;   // the kernel's LDS
+  engine_.uploadStagePlan(P, blob);
+  staged_params_ = nb_params;
+  return true;
 }
 
 std::vector<std::pair<uint32_t, uint32_t>> TabulatedModel::speciesRows() const {
@@ -752,10 +893,25 @@ std::vector<char> Retrieval::stageBatch(const double* params, size_t batch, size
   return neglect;
 }
 
+void Retrieval::evaluateStaged(const double* params, size_t batch, size_t n, std::vector<char>& neglect) {
+  checkSize(n);
+  if (device_staging_ && model_->prepareStaging(n, global_.error_inflation)) {
+    engine_->reserveBatch(batch);
+    engine_->reserveStage(batch);
+    std::memcpy(engine_->stagedParameters(), params, batch * n * sizeof(double));
+    engine_->evaluateBatch(model_->kind(), batch, true);
+    neglect.resize(batch);
+    for (size_t b = 0; b < batch; ++b) neglect[b] = engine_->stagedNeglect()[b] != 0;
+    return;
+  }
+  neglect = stageBatch(params, batch, n);
+  engine_->evaluateBatch(model_->kind(), batch);
+}
+
 void Retrieval::computeLikelihoodBatch(const double* params, size_t batch, size_t n, double* ll) {
   if (batch == 0) return;
-  const std::vector<char> neglect = stageBatch(params, batch, n);
-  engine_->evaluateBatch(model_->kind(), batch);
+  std::vector<char> neglect;
+  evaluateStaged(params, batch, n, neglect);
   const double* v = engine_->batchLogLikelihood();
   for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
 }
@@ -763,8 +919,8 @@ void Retrieval::computeLikelihoodBatch(const double* params, size_t batch, size_
 void Retrieval::computeModelBatch(const double* params, size_t batch, size_t n, double* spectrum,
                                   double* spectrum_obs, int* neglect_out) {
   if (batch == 0) return;
-  const std::vector<char> neglect = stageBatch(params, batch, n);
-  engine_->evaluateBatch(model_->kind(), batch);
+  std::vector<char> neglect;
+  evaluateStaged(params, batch, n, neglect);
   std::vector<double> bins;
   engine_->copyBatchBins(batch, bins);
   const size_t nb = engine_->nbBins();

@@ -52,6 +52,13 @@ class ForwardModel {
                           char* neglect) = 0;
   // [first, end) of each tabulated species' rows in the table store
   virtual std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const = 0;
+  // device staging (StagePlan) of this model's batches with parameter rows
+  // of nb_params: uploaded once; false where the model has what the device
+  // staging does not form (the H- continuum, clouds)
+  virtual bool prepareStaging(size_t nb_params, bool error_inflation) {
+    (void)nb_params, (void)error_inflation;
+    return false;
+  }
   // gradient rows [batch][stride] of the model's parameters (and observation
   // modifiers) from the engine's adjoint: level gradients [batch][levels][slots],
   // bin gradients [batch][bins] and the model's terms (eclipse: per bin
@@ -73,6 +80,7 @@ class TabulatedModel : public ForwardModel {
   void stageBatch(const double* parameters, size_t batch, size_t stride, const double* error_inflation,
                   char* neglect) override;
   std::vector<std::pair<uint32_t, uint32_t>> speciesRows() const override;
+  bool prepareStaging(size_t nb_params, bool error_inflation) override;
   // pyBeAR's direct spectra: the atmosphere set from the caller's structure
   // (Atmosphere::setStructure: pressure, temperature, mixing ratios [levels]
   // [species] of the given species ids) and staged for one evaluation
@@ -102,6 +110,10 @@ class TabulatedModel : public ForwardModel {
     (void)p;
     return 0.0;
   }
+  // the altitude variant (StagePlan::alt_kind) and radius-dependent gravity
+  virtual int altitudeKind() const { return 0; }
+  virtual bool variableGravity() const { return false; }
+  size_t staged_params_ = 0;                // parameter row length of the uploaded StagePlan, 0: none
   // emission and eclipse: the chain from the level gradients through the
   // atmosphere to [log g, chemistry..., temperature..., modifiers...];
   // generalGradient fills the other two general parameters
@@ -217,6 +229,8 @@ class TransmissionModel : public TabulatedModel {
  private:
   AltitudeOptions altitudeOptions(const double* p) const;
   double fixedScaleHeight(const double* p) const override { return fit_scale_height_ ? p[3] : 0.0; }
+  int altitudeKind() const override { return fit_mmw_ ? 1 : fit_scale_height_ ? 2 : 0; }
+  bool variableGravity() const override { return variable_gravity_; }
   // the chord lengths of the staged atmosphere into the engine's staging
   void stageChords(double bottom_radius);
   bool fit_mmw_ = false, fit_scale_height_ = false, variable_gravity_ = false;
@@ -243,6 +257,9 @@ class Retrieval {
   void computeModelBatch(const double* parameters, size_t batch, size_t n, double* spectrum, double* spectrum_obs,
                          int* neglect);
   void setTablePrecision(TablePrecision p);
+  // batches staged on the device where the model allows it (the default),
+  // or on the host
+  void setDeviceStaging(bool on) { device_staging_ = on; }
   // log-likelihoods [batch] and their gradients [batch][n] (FP64 tables)
   void gradientBatch(const double* parameters, size_t batch, size_t n, double* log_likelihood, double* gradient);
   DeviceEngine& engine() const { return *engine_; }
@@ -251,6 +268,9 @@ class Retrieval {
   double errorInflation(const double* parameters, size_t n) const;
   void checkSize(size_t n) const;
   std::vector<char> stageBatch(const double* parameters, size_t batch, size_t n);
+  // evaluate a batch, staged on the device when possible; neglect flags out
+  void evaluateStaged(const double* parameters, size_t batch, size_t n, std::vector<char>& neglect);
+  bool device_staging_ = true;
   GlobalSettings global_;
   SpectralGrid grid_;
   std::vector<ObservationSetup> observations_;

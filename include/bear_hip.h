@@ -247,6 +247,16 @@ int bear_retrieval_set_table_precision(bear_retrieval* r, int precision);
  * shape).  Both default to 1. */
 int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autotune);
 
+/* Where the batched forward calls form each live point's atmosphere
+ * (temperature profile, chemistry, altitudes) and table stencils: on the
+ * device from the parameters (enable = 1, the default; the parameters are
+ * the only per-batch upload), or on the host in parallel over points with
+ * the records uploaded (enable = 0).  No counterpart in the reference, which
+ * has no batches.  Models with the H- continuum or clouds, which the
+ * batched kernels refuse, and the gradient calls, whose chain rule needs
+ * the host's atmospheres, stage on the host. */
+int bear_retrieval_set_device_staging(bear_retrieval* r, int enable);
+
 /* log_likelihood [batch]; -1e30 for neglected points. */
 int bear_retrieval_compute_likelihood_batch(bear_retrieval* r, const double* parameters, size_t batch,
                                             size_t nb_parameters, double* log_likelihood);

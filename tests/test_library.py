@@ -72,6 +72,7 @@ def test_batch_entry_points_refuse_null_arguments():
     L = api.lib()
     assert L.bear_retrieval_set_table_precision(None, 0) == -1
     assert L.bear_retrieval_set_batch_options(None, 1, 1) == -1
+    assert L.bear_retrieval_set_device_staging(None, 1) == -1
     assert L.bear_retrieval_compute_likelihood_batch(None, None, 1, 4, None) == -1
     assert L.bear_retrieval_compute_model_batch(None, None, 1, 4, None, None, None) == -1
     assert L.bear_retrieval_batch_stats(None, None) == -1
