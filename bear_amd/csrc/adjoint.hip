@@ -930,7 +930,8 @@ void DeviceEngine::releaseAdjoint() {
 
 void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
                                         std::vector<double>& bin_grad, std::vector<double>& bins,
-                                        std::vector<double>& bin_terms) {
+                                        std::vector<double>& bin_terms, bool staged) {
+  if (staged && (!stage_blob_ || batch > stage_capacity_)) raise(kInternal, "device staging not prepared");
   const bool transmission = kind == ModelKind::transmission;
   if (transmission != transmission_) raise(kInternal, "adjoint model kind differs from the configured model");
   const bool eclipse = kind == ModelKind::eclipse;
@@ -1032,7 +1033,13 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   const BatchLaunch launch = tuned(transmission ? ModelKind::transmission : ModelKind::emission, chunk);
   bin_terms.assign(transmission ? batch * P * P : eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
-  check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+  if (staged) {
+    launchStage(batch);
+    check(hipMemcpyAsync(stage_neglect_host_, stage_neglect_, batch * sizeof(int), hipMemcpyDeviceToHost, s),
+          "read neglect flags");
+  } else {
+    check(hipMemcpyAsync(batch_dev_, batch_host_, batch * rec_bytes_, hipMemcpyHostToDevice, s), "upload batch");
+  }
   for (size_t first = 0; first < batch; first += chunk) {
     const int cb = int(std::min(chunk, batch - first));
     const Records rec{batch_dev_ + first * rec_bytes_, rec_bytes_, rec_levels_, rec_species_, rec_adjoint_,

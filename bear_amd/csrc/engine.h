@@ -354,8 +354,11 @@ class DeviceEngine {
   // depth, and dL/dn_R of the Rayleigh scatterer; bin_terms gets dL/dchord
   // [batch][L-1 segments][L-1 tangents], zero where the segment lies below
   // the tangent level).
+  // (staged: the records formed on the device from stagedParameters(), the
+  // neglect flags in stagedNeglect() afterwards)
   void evaluateAdjointBatch(ModelKind kind, size_t batch, std::vector<double>& level_grad,
-                            std::vector<double>& bin_grad, std::vector<double>& bins, std::vector<double>& bin_terms);
+                            std::vector<double>& bin_grad, std::vector<double>& bins, std::vector<double>& bin_terms,
+                            bool staged = false);
   // HBM budget (bytes) of the kept forward quantities; 0: always recompute
   void setAdjointMemory(size_t bytes) { adjoint_budget_ = bytes; }
   size_t adjointMemory() const { return adjoint_budget_; }

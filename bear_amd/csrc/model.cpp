@@ -949,9 +949,22 @@ void Retrieval::gradientBatch(const double* params, size_t batch, size_t n, doub
   if (batch == 0) return;
   if (engine_->tablePrecision() != TablePrecision::fp64)
     raise(kInvalidInput, "the gradient needs the FP64 tables (table precision fp64)");
-  const std::vector<char> neglect = stageBatch(params, batch, n);
+  // the records staged on the device where the model allows it; the chain
+  // rule below computes each point's atmosphere on the host either way
+  std::vector<char> neglect;
   std::vector<double> level_grad, bin_grad, bins, bin_terms;
-  engine_->evaluateAdjointBatch(model_->kind(), batch, level_grad, bin_grad, bins, bin_terms);
+  checkSize(n);
+  if (device_staging_ && model_->prepareStaging(n, global_.error_inflation)) {
+    engine_->reserveBatch(batch);
+    engine_->reserveStage(batch);
+    std::memcpy(engine_->stagedParameters(), params, batch * n * sizeof(double));
+    engine_->evaluateAdjointBatch(model_->kind(), batch, level_grad, bin_grad, bins, bin_terms, true);
+    neglect.resize(batch);
+    for (size_t b = 0; b < batch; ++b) neglect[b] = engine_->stagedNeglect()[b] != 0;
+  } else {
+    neglect = stageBatch(params, batch, n);
+    engine_->evaluateAdjointBatch(model_->kind(), batch, level_grad, bin_grad, bins, bin_terms);
+  }
   const double* v = engine_->batchLogLikelihood();
   for (size_t b = 0; b < batch; ++b) ll[b] = neglect[b] ? -1e30 : v[b];
   std::fill(gradient, gradient + batch * n, 0.0);

@@ -252,9 +252,9 @@ int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autot
  * device from the parameters (enable = 1, the default; the parameters are
  * the only per-batch upload), or on the host in parallel over points with
  * the records uploaded (enable = 0).  No counterpart in the reference, which
- * has no batches.  Models with the H- continuum or clouds, which the
- * batched kernels refuse, and the gradient calls, whose chain rule needs
- * the host's atmospheres, stage on the host. */
+ * has no batches.  The gradient calls stage the same way (their chain rule
+ * then computes each point's atmosphere on the host).  Models with the H-
+ * continuum or clouds, which the batched kernels refuse, stage on the host. */
 int bear_retrieval_set_device_staging(bear_retrieval* r, int enable);
 
 /* log_likelihood [batch]; -1e30 for neglected points. */

@@ -67,3 +67,23 @@ def test_device_staging_flags_neglected_points(case_factory):
     assert (ll == -1e30).any() and (ll > -1e29).any()
     np.testing.assert_array_equal(ll == -1e30, ll_host == -1e30)
     r.close()
+
+
+@pytest.mark.parametrize("name", ["tiny", "transmission_variable_gravity"])
+def test_device_staged_gradient_matches_host_staging(case_factory, name):
+    """The gradient calls' records staged on the device give the gradient of
+    the host-staged records (the chain rule computes the atmospheres on the
+    host either way)."""
+    folder, params = case_factory(name)
+    r = synth.open_retrieval(folder)
+    pts = np.array(params, dtype=np.float64)
+    r.set_device_staging(False)
+    ll_h, g_h = r.gradientBatch(pts)
+    r.set_device_staging(True)
+    ll_d, g_d = r.gradientBatch(pts)
+    live = ll_h > -1e29
+    np.testing.assert_array_equal(ll_d > -1e29, live)
+    np.testing.assert_allclose(ll_d[live], ll_h[live], rtol=1e-9, atol=0)
+    scale = np.max(np.abs(g_h[live]), axis=0) + 1e-300
+    assert np.max(np.abs(g_d[live] - g_h[live]) / scale) <= 1e-8
+    r.close()
