@@ -75,6 +75,22 @@ def isothermal_check(r, params):
     return float(np.max(np.abs(out.spectrum - expected) / expected))
 
 
+def transcendentals(st, transmission, evals_per_s):
+    """Algorithmic transcendental count of one evaluation and the rate it
+    implies: one 10^x per (level, tabulated species, wavenumber) for the
+    opacity interpolation, and per (level, wavenumber) two exponentials
+    (the short-characteristics attenuation of both sweeps) and one Planck
+    exponential for emission, or one exp(-tau) per (tangent path,
+    wavenumber) for a transit spectrum."""
+    nl, ns, n = st["nb_levels"], st["nb_species"], st["nb_spectral_points"]
+    exp10 = nl * ns * n
+    rt = (nl - 1) * n if transmission else 2 * nl * n
+    planck = 0 if transmission else nl * n
+    total = exp10 + rt + planck
+    return {"per_eval": int(total), "exp10": int(exp10), "rt_exp": int(rt), "planck": int(planck),
+            "rate_per_s": float(f"{total * evals_per_s:.4g}")}
+
+
 def kernel_source_digest():
     """sha256 over the single-point kernel's source files (their contents)."""
     h = hashlib.sha256()
@@ -378,6 +394,8 @@ def config_workload(name, batch, steps):
                                "launch": {k: bst[k] for k in ("block", "tile", "grid", "point_group", "used_graph")}}
             log(f"{name} batch {precision}: {rate:.1f} evals/s, bins err {err_bins:.3e} {unit}, |dlogL| {err_ll:.2e}")
         out["batched"] = legs
+        best = max(v["evals_per_s"] for v in legs.values())
+        out["transcendentals"] = transcendentals(st, name.startswith("transmission"), best)
         if "fp64" in precisions:
             r.set_table_precision("fp64")
             r.gradientBatch(points)
@@ -513,6 +531,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "transcendentals": transcendentals(st, False, value),
         "checks": {"isothermal_max_rel_err": iso_err},
     }
     print(json.dumps(out), flush=True)
