@@ -13,10 +13,13 @@ optimisers).  gradient="lazy" runs only the batched likelihood forward and
 the value + gradient sequence in backward, for graphs where backward often
 does not run; a backward then costs one extra forward pass.
 
-The parameters cross to the host either way: the engine forms each point's
-atmosphere and table stencils on the host (a few microseconds per point)
-before its device work, so the [batch, nb_parameters] values are copied
-once per call (a few KB).
+The [batch, nb_parameters] values cross to the host once per call (a few
+KB) and go back to the device with the batch's other inputs: the batched
+calls form each point's atmosphere and table stencils on the device from
+them (device staging, the default), and the chain rule from the level
+adjoints to the parameters runs on the host in FP64 (a few microseconds per
+point).  The returned log-likelihoods and gradients are copied to the
+parameters' device.
 """
 
 from __future__ import annotations
