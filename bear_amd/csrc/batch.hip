@@ -230,11 +230,14 @@ struct Opacity32 {
 // keep (keptColumn layout) in FP32, for the adjoint's backward sweep
 // (which recomputes B in FP64: the difference it needs keeps its own
 // relative precision, and the gradient's bar is 1e-4).
-template <class Op>
+// EX (FP64 tables): the H- continuum joins each level's opacity and the
+// point's cloud models (BatchModel::clouds, weights in its LevelState) each
+// layer's optical depth, as in the single-point kernel.
+template <class Op, bool EX = false>
 __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
-    const double* __restrict__ wavenumber, const void* __restrict__ tables, const float* __restrict__ base,
-    size_t stride, Records rec, int nb_levels, int n, int tiles, int batch, double* __restrict__ flux,
-    float* __restrict__ keep) {
+    const double* __restrict__ wavenumber, const double* __restrict__ wavelength, const void* __restrict__ tables,
+    const float* __restrict__ base, size_t stride, Extras ex, Records rec, int nb_levels, int n, int tiles, int batch,
+    double* __restrict__ flux, float* __restrict__ keep) {
   using Species = typename Op::Species;
   const size_t total = nbItems(tiles, batch);
   for (size_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -251,7 +254,14 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
     const double nu = wavenumber[i];
     const double b_num = kPlanckC1 * nu * nu * nu;
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
+    Extras e = ex;
+    double hm[7], cf[kMaxClouds];
+    if constexpr (EX) {
+      e.clouds = bm->clouds;
+      loadExtras(e, i, n, wavelength[i], hm, cf);
+    }
     double k_prev = double(op.kappa(species, levels[0], rec.groups, true));
+    if (EX && e.hminus) k_prev += hminusKappa(hm, levels[0]);
     double b_prev = planckAt(b_num, x_nu, levels[0].inv_temperature);
     double z_prev = levels[0].altitude;
     double i1 = b_prev, i2 = b_prev;
@@ -262,10 +272,13 @@ __global__ __launch_bounds__(kBatchBlock) void emissionBatchKernel(
       kept[2 * kKeptValue] = float(k_prev);
     }
     for (int l = 1; l < nb_levels; ++l) {
-      const double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups, levels[l].reload != 0));
+      double k_next = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups, levels[l].reload != 0));
+      if (EX && e.hminus) k_next += hminusKappa(hm, levels[l]);
       const double b_next = planckAt(b_num, x_nu, levels[l].inv_temperature);
       const double z_next = levels[l].altitude;
-      shortCharacteristicsLayer((z_next - z_prev) * (k_next + k_prev) * 0.5, b_prev, b_next, i1, i2);
+      double tau = (z_next - z_prev) * (k_next + k_prev) * 0.5;
+      if (EX) tau += cloudLayer(e, cf, levels[l], l, i, n);
+      shortCharacteristicsLayer(tau, b_prev, b_next, i1, i2);
       if (kept) {
         float* at = kept + size_t(l) * kKeptLevel;
         at[0] = float(i1 - b_next);
@@ -329,12 +342,15 @@ __device__ __forceinline__ double transmittance(float tau) {
 //     and depth (R + h)^2 / R_*^2 1e6 in FP64.
 // kKeep (FP64 tables, for the adjoint): also, in the keptColumn layout,
 // {k_l, tau_t, last segment summed by path t} per level in FP32.
-template <class Op, bool kKeep = false>
+// EX (FP64 tables): the H- continuum and the point's cloud extinction
+// (BatchModel::clouds, the layer below each level over its thickness) join
+// each level's extinction, as in the single-point kernel.
+template <class Op, bool kKeep = false, bool EX = false>
 __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
     const void* __restrict__ tables, const float* __restrict__ base, size_t stride, const double* __restrict__ rayleigh,
-    Records rec, const double* __restrict__ chord64, const float* __restrict__ chord32, int nb_levels, int n,
-    TileWindow win, int tile, int batch, int point_group, double* __restrict__ depth, float* __restrict__ keep,
-    int* __restrict__ reach) {
+    const double* __restrict__ wavelength, Extras ex, Records rec, const double* __restrict__ chord64,
+    const float* __restrict__ chord32, int nb_levels, int n, TileWindow win, int tile, int batch, int point_group,
+    double* __restrict__ depth, float* __restrict__ keep, int* __restrict__ reach) {
   using K = typename Op::K;
   using Species = typename Op::Species;
   extern __shared__ double lds[];
@@ -362,10 +378,21 @@ __global__ __launch_bounds__(kTransmissionBlock) void transmissionBatchKernel(
       Op op;
       op.init(tables, base, stride, i, 0);
       const double sigma_r = rayleigh ? rayleigh[i] : 0.0;
+      Extras e = ex;
+      double hm[7], cf[kMaxClouds];
+      if constexpr (EX) {
+        e.clouds = bm->clouds;
+        loadExtras(e, i, n, wavelength[i], hm, cf);
+      }
       for (int l = l0; l < l1; ++l) {
         const double kd = double(op.kappa(species + size_t(l) * Op::kNS, levels[l], rec.groups,
                                           l == l0 || levels[l].reload != 0));
-        const K k = K(kd + sigma_r * levels[l].rayleigh_density);
+        double ke = kd + sigma_r * levels[l].rayleigh_density;
+        if constexpr (EX) {
+          if (e.hminus) ke += hminusKappa(hm, levels[l]);
+          if (l > 0) ke += cloudLayer(e, cf, levels[l], l, i, n) * levels[l].cloud_scale;
+        }
+        const K k = K(ke);
         if constexpr (kKeep) keep[keptAt(win, m, stride, L, i) + size_t(l) * kKeptLevel] = float(k);
         if (l == l0) part[g * tile + j] = double(k);
         else S[(l - 1) * tile + j] = last + k;
@@ -488,67 +515,78 @@ __global__ __launch_bounds__(kBatchBlock) void batchSpectrumKernel(
 }
 
 // ---------------------------------------------------------------- launch helpers
+// ex: the H- rows and whether the points carry cloud models (the extras
+// variants, FP64 tables only; the callers refuse the others)
 template <int NS>
-void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const void* tab64,
-                    const void* tab_low, const float* base, size_t stride, Records rec, int L, int n, int tiles,
-                    int batch, double* flux, float* keep) {
+void launchEmission(TablePrecision p, dim3 grid, dim3 block, hipStream_t s, const double* nu, const double* lam,
+                    const void* tab64, const void* tab_low, const float* base, size_t stride, const Extras& ex,
+                    bool extras, Records rec, int L, int n, int tiles, int batch, double* flux, float* keep) {
   switch (p) {
     case TablePrecision::fp64:
-      hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, tab64, base, stride, rec, L, n,
-                         tiles, batch, flux, keep);
+      if (extras)
+        hipLaunchKernelGGL((emissionBatchKernel<Opacity64<NS>, true>), grid, block, 0, s, nu, lam, tab64, base, stride,
+                           ex, rec, L, n, tiles, batch, flux, keep);
+      else
+        hipLaunchKernelGGL(emissionBatchKernel<Opacity64<NS>>, grid, block, 0, s, nu, lam, tab64, base, stride, ex,
+                           rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::fp32:
-      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, keep);
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, 0, s, nu, lam, tab_low, base,
+                         stride, ex, rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::fp16:
-      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, keep);
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, 0, s, nu, lam, tab_low, base,
+                         stride, ex, rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::bf16:
-      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, keep);
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, 0, s, nu, lam, tab_low, base,
+                         stride, ex, rec, L, n, tiles, batch, flux, keep);
       break;
     case TablePrecision::int16:
-      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, 0, s, nu, tab_low, base, stride,
-                         rec, L, n, tiles, batch, flux, keep);
+      hipLaunchKernelGGL((emissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, 0, s, nu, lam, tab_low, base,
+                         stride, ex, rec, L, n, tiles, batch, flux, keep);
       break;
   }
 }
 
 template <int NS>
 void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, const void* tab64,
-                        const void* tab_low, const float* base, size_t stride, const double* rayleigh, Records rec,
-                        const double* chord, const float* chord32, int L, int n, TileWindow win, int tile,
-                        int batch, int group, double* depth, float* keep, int* reach) {
+                        const void* tab_low, const float* base, size_t stride, const double* rayleigh,
+                        const double* lam, const Extras& ex, bool extras, Records rec, const double* chord,
+                        const float* chord32, int L, int n, TileWindow win, int tile, int batch, int group,
+                        double* depth, float* keep, int* reach) {
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
       if (keep)
         hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true>), grid, block, lds, s, tab64, base, stride,
-                           rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep, reach);
+                           rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep, reach);
+      else if (extras)
+        hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, false, true>), grid, block, lds, s, tab64, base,
+                           stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth,
+                           nullptr, nullptr);
       else
         hipLaunchKernelGGL(transmissionBatchKernel<Opacity64<NS>>, grid, block, lds, s, tab64, base, stride, rayleigh,
-                           rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr, nullptr);
+                           lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr, nullptr);
       break;
     case TablePrecision::fp32:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF32>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
                          nullptr);
       break;
     case TablePrecision::fp16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
                          nullptr);
       break;
     case TablePrecision::bf16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabBF16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
                          nullptr);
       break;
     case TablePrecision::int16:
       hipLaunchKernelGGL((transmissionBatchKernel<Opacity32<NS, TabS16>>), grid, block, lds, s, tab_low, base,
-                         stride, rayleigh, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
+                         stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, nullptr,
                          nullptr);
       break;
   }
@@ -703,6 +741,8 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
   double* flux = batch_flux_ + first * stride_;
   const int n = int(n_points_), L = int(nb_levels_), B = int(batch);
   const bool wide = precision_ == TablePrecision::fp64;
+  const Extras ex{hminus_, nullptr, CloudSpec{}};       // the clouds' parameters come with each point's record
+  const bool extras = hminus_ != nullptr || cloud_models_ > 0;
   if (kind == ModelKind::transmission) {
     hipLaunchKernelGGL(chordKernel, dim3(unsigned(chordSegments(L)), unsigned(B)), dim3(kBatchBlock), 0, s, rec, L,
                        wide ? batch_chord_ : nullptr, wide ? nullptr : batch_chord32_);
@@ -720,8 +760,8 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
 #define BEAR_T(NS)                                                                                                  \
   case NS:                                                                                                          \
     launchTransmission<NS>(precision_, dim3(unsigned(lc.grid)), lds, s, tables_, tables_low_, table_base_, stride_, \
-                           rayleigh_, rec, batch_chord_, batch_chord32_, L, n, win, lc.tile, B, lc.group, flux,   \
-                           keep, reach);                                                                       \
+                           rayleigh_, wavelength_, ex, extras, rec, batch_chord_, batch_chord32_, L, n, win,       \
+                           lc.tile, B, lc.group, flux, keep, reach);                                               \
     break;
       BEAR_T(4)
       BEAR_T(8)
@@ -738,8 +778,8 @@ void DeviceEngine::launchBatchSpectrum(ModelKind kind, size_t batch, const Batch
   switch (species_stride_) {
 #define BEAR_E(NS)                                                                                                   \
   case NS:                                                                                                           \
-    launchEmission<NS>(precision_, dim3(unsigned(lc.grid)), dim3(unsigned(lc.block)), s, wavenumber_, tables_,       \
-                       tables_low_, table_base_, stride_, rec, L, n, tiles, B, flux, keep);                          \
+    launchEmission<NS>(precision_, dim3(unsigned(lc.grid)), dim3(unsigned(lc.block)), s, wavenumber_, wavelength_,   \
+                       tables_, tables_low_, table_base_, stride_, ex, extras, rec, L, n, tiles, B, flux, keep);    \
     break;
     BEAR_E(4)
     BEAR_E(8)
@@ -850,8 +890,8 @@ void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch) { evaluateBatch(k
 void DeviceEngine::evaluateBatch(ModelKind kind, size_t batch, bool staged) {
   if (staged && (!stage_blob_ || batch > stage_capacity_)) raise(kInternal, "device staging not prepared");
   if (species_stride_ > 16) raise(kInvalidInput, "the batched kernels take at most 16 tabulated species");
-  if (hminus_) raise(kInvalidInput, "the H- continuum is computed by the single-point kernels only");
-  if (cloud_models_ > 0) raise(kInvalidInput, "cloud layers are computed by the single-point kernels only");
+  if ((hminus_ || cloud_models_ > 0) && precision_ != TablePrecision::fp64)
+    raise(kInvalidInput, "the H- continuum and cloud layers take FP64 tables");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "batch larger than the staged records");
   if ((kind == ModelKind::transmission) != transmission_) raise(kInvalidInput, "model kind does not match the engine");
   check(hipSetDevice(device_), "hipSetDevice");

@@ -166,6 +166,7 @@ struct BatchModel {
   double star_radius;            // transmission: R_* (cm)
   double inflation;              // added variance (error inflation), or 0
   double pad[2];
+  CloudSpec clouds;              // the point's cloud models (their weights per level are in its LevelState)
 };
 
 // Device staging of a batch: each live point's atmosphere (temperature

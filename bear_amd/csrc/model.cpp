@@ -302,6 +302,8 @@ void TabulatedModel::stageBatch(const double* params, size_t batch, size_t strid
       stageLevels(atm, modifiers(p), engine_.batchLevels(b), wide ? engine_.batchSpecies(b) : nullptr,
                   wide ? engine_.batchSpeciesAdjoint(b) : nullptr, wide ? nullptr : engine_.batchSpecies32(b),
                   engine_.batchShifts(b));
+      if (!clouds_.empty())
+        stageClouds(atm, clouds(p), std::pow(10, p[0]), fixedScaleHeight(p), engine_.batchLevels(b), scalars.clouds);
     } catch (const Error& e) {
 #pragma omp critical(bear_stage_failure)
       if (failure.empty()) { failure = e.what(); failure_code = e.code; }

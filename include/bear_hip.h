@@ -105,8 +105,9 @@ typedef struct bear_model_config {
    * p_top, p_bottom / p_top], KHnongrey [tau, q0, a0, particle size (micron),
    * p_top, p_bottom / p_top], power_law [tau, exponent, p_top,
    * p_bottom / p_top], without the last with "fb".  At most 4 models; the
-   * single-point calls compute them, the batched and gradient calls refuse
-   * such a model (BEAR_ERROR_INVALID_INPUT). */
+   * single-point calls and the batched calls with FP64 tables compute them,
+   * the gradient calls and reduced-precision batches refuse such a model
+   * (BEAR_ERROR_INVALID_INPUT). */
   const char* cloud_model;
 } bear_model_config;
 
@@ -254,7 +255,7 @@ int bear_retrieval_set_batch_options(bear_retrieval* r, int use_graph, int autot
  * the records uploaded (enable = 0).  No counterpart in the reference, which
  * has no batches.  The gradient calls stage the same way (their chain rule
  * then computes each point's atmosphere on the host).  Models with the H-
- * continuum or clouds, which the batched kernels refuse, stage on the host. */
+ * continuum or clouds stage on the host. */
 int bear_retrieval_set_device_staging(bear_retrieval* r, int enable);
 
 /* log_likelihood [batch]; -1e30 for neglected points. */

@@ -10,7 +10,7 @@ points inside a batch, and launch-shape independence."""
 import numpy as np
 import pytest
 
-from bear_amd import synth
+from bear_amd import api, synth
 from oracle import api as oracle_api
 
 pytestmark = pytest.mark.gpu
@@ -81,6 +81,46 @@ def test_fp64_batch_matches_oracle_and_single_calls(case_factory, name):
         assert abs(ll[k] - ref["loglike"]) <= LOGLIKE_ATOL, (k, ll[k], ref["loglike"])
         single = r.computeLikelihood(p)
         assert abs(ll[k] - single) <= 1e-9 * max(1.0, abs(single)), (k, ll[k], single)
+
+
+@pytest.mark.parametrize("name", ["hminus", "cloudy_tiny", "cloudy_eclipse", "transmission_hminus",
+                                  "transmission_cloudy"])
+def test_fp64_batch_with_hminus_and_clouds(case_factory, name):
+    """The H- continuum and the cloud models in the batched kernels (FP64
+    tables; each point's clouds travel with its record): a batch of the
+    case's own parameter vectors (clouds at several heights and depths) and
+    two perturbed ones matches the oracle at the single-point bars and the
+    single-point calls; reduced-precision tables refuse these models."""
+    folder, params = case_factory(name)
+    case = synth.CASES[name]()
+    pts = np.array(list(params) + [synth.perturbed(params[0], case, seed=300 + k, scale=2.0) for k in range(2)])
+    r, o = opened(folder)
+    print(f"[gpu] batch fp64 with extras {name} ({len(pts)} points)", flush=True)
+    r.set_table_precision("fp64")
+    ll = r.computeLikelihoodBatch(pts)
+    spec, bins, neglect = r.computeModelBatch(pts, True)
+    for k, p in enumerate(pts):
+        ref = o.evaluate(p)
+        assert bool(neglect[k]) == ref["neglect"]
+        if ref["neglect"]:
+            assert ll[k] == -1e30
+            continue
+        rel = np.max(np.abs(spec[k] - ref["spectrum"]) / np.abs(ref["spectrum"]))
+        assert rel <= SPECTRUM_RTOL, (k, rel)
+        for i, expect in enumerate(oracle_bins(o, ref)):
+            np.testing.assert_allclose(bins[i][k], expect, rtol=SPECTRUM_RTOL, atol=0)
+        assert abs(ll[k] - ref["loglike"]) <= LOGLIKE_ATOL, (k, ll[k], ref["loglike"])
+        single = r.computeModel(p, True)
+        np.testing.assert_allclose(spec[k], single.spectrum, rtol=1e-11, atol=0)
+        one = r.computeLikelihood(p)
+        assert abs(ll[k] - one) <= 1e-9 * max(1.0, abs(one)), (k, ll[k], one)
+    r.set_table_precision("fp32")
+    try:
+        with pytest.raises(api.BearError) as e:
+            r.computeLikelihoodBatch(pts[:2])
+        assert "FP64 tables" in str(e.value)
+    finally:
+        r.set_table_precision("fp64")
 
 
 def test_neglected_point_inside_a_batch(case_factory):

@@ -156,13 +156,11 @@ def test_many_species_limits(case_factory):
 
 def test_hminus_limits(case_factory):
     """The H- continuum (rows per wavenumber, weights per level) enters the
-    single-point kernels (parity in test_emission_parity); the batched and
-    adjoint kernels do not take it and refuse the case with InvalidInput."""
+    single-point and batched kernels (parity in test_emission_parity and
+    test_fp64_batch_with_hminus_and_clouds); the adjoint kernels do not take
+    it and refuse the case with InvalidInput."""
     r, o, params, _ = retrieval(case_factory, "hminus")
     pts = np.array(params[:2])
-    with pytest.raises(api.BearError) as e:
-        r.computeLikelihoodBatch(pts)
-    assert e.value.code == -1 and "H-" in str(e.value)
     with pytest.raises(api.BearError) as e:
         r.gradientBatch(pts)
     assert e.value.code == -1 and "H-" in str(e.value)
@@ -171,14 +169,12 @@ def test_hminus_limits(case_factory):
 
 def test_cloud_limits(case_factory):
     """Cloud layers (a grey and a power-law model) enter the single-point
-    kernels (parity in test_emission_parity); the batched and adjoint
-    kernels do not take them and refuse the case with InvalidInput.  A
-    cloud with zero optical depth leaves the spectrum as without it."""
+    and batched kernels (parity in test_emission_parity and
+    test_fp64_batch_with_hminus_and_clouds); the adjoint kernels do not take
+    them and refuse the case with InvalidInput.  A cloud with zero optical
+    depth leaves the spectrum as without it."""
     r, o, params, _ = retrieval(case_factory, "cloudy_tiny")
     pts = np.array(params[:2])
-    with pytest.raises(api.BearError) as e:
-        r.computeLikelihoodBatch(pts)
-    assert e.value.code == -1 and "cloud" in str(e.value)
     with pytest.raises(api.BearError) as e:
         r.gradientBatch(pts)
     assert e.value.code == -1 and "cloud" in str(e.value)
