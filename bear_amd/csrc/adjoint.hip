@@ -420,10 +420,10 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 // for opacityAdjointKernel.  ns: stencil slots per level.
 __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restrict__ wavenumber, Records rec,
                                                          int nb_levels, int n, size_t stride, int tiles, int batch,
-                                                         int ns, const double* __restrict__ fbar,
+                                                         int ns, int kslots, const double* __restrict__ fbar,
                                                          float* __restrict__ keep, double* __restrict__ partial) {
   const int L = nb_levels;
-  const int K = ns + 4;
+  const int K = kslots;
   const int lane = int(threadIdx.x);
   const size_t level_step = kKeptLevel;
   const size_t total = nbItems(tiles, batch);
@@ -502,13 +502,18 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
 // forward kernels do (register-cached rows, reloaded when they change),
 // n_s d logL / d n_s = sum over wavenumbers of kappa-bar t_s and the
 // temperature's share through the tables, kappa-bar d kappa / dT.
-template <int NS>
+// HM: also, in slots NS+4 .. NS+10, kappa-bar times each of the H-
+// continuum's seven rows (hminus [7][n]): the gradients of its per-level
+// weights LevelState::hminus_c.
+template <int NS, bool HM = false>
 __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __restrict__ tables, size_t stride,
                                                               Records rec, int nb_levels, int n, int tiles,
                                                               TileWindow win, int batch,
                                                               const float* __restrict__ keep,
+                                                              const double* __restrict__ hminus,
                                                               double* __restrict__ partial) {
-  constexpr int K = NS + 4;
+  constexpr int KX = NS + 4;
+  constexpr int K = NS + (HM ? 12 : 4);
   const int L = nb_levels;
   const int lane = int(threadIdx.x);
   const size_t total = nbItems(win.count, batch);
@@ -526,6 +531,10 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
     double* out = partial + (size_t(m) * L * size_t(tiles) + tile) * K;         // level l: + l * tiles * K
     const double* tab = tables + ic;
     double v[NS][4];
+    double hm[7];
+    if constexpr (HM)
+#pragma unroll
+      for (int r = 0; r < 7; ++r) hm[r] = hminus[size_t(r) * n + ic];
     // d logL / d kappa streams from HBM: read two levels ahead
     double kb1 = kb[0], kb2 = L > 1 ? kb[kKeptLevel] : 0.0;
     for (int l = 0; l < L; ++l) {
@@ -540,7 +549,7 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
       }
       kb1 = kb2;
       if (l + 2 < L) kb2 = kb[size_t(l + 2) * kKeptLevel];
-      double x[K];
+      double x[KX];
       double dk = 0.0;
       // the terms to 1.2e-7 (pow2Gradient): they only weight d logL / d kappa
       slotTerms<NS, true>(d, a, levels[l], rec.groups, v, [&](int s, double ts, double ys) {
@@ -549,8 +558,15 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
       });
       x[NS] = kbar * dk * kLn2;
 #pragma unroll
-      for (int s = NS + 1; s < K; ++s) x[s] = 0.0;
-      emitSums<K>(out + size_t(l) * tiles * K, lane, x, 0, NS + 1);
+      for (int s = NS + 1; s < KX; ++s) x[s] = 0.0;
+      emitSums<KX>(out + size_t(l) * tiles * K, lane, x, 0, NS + 1);
+      if constexpr (HM) {
+        double xh[8];
+#pragma unroll
+        for (int r = 0; r < 7; ++r) xh[r] = kbar * hm[r];
+        xh[7] = 0.0;
+        emitSums<8>(out + size_t(l) * tiles * K, lane, xh, NS + 4, 8);
+      }
     }
   }
 }
@@ -584,16 +600,22 @@ __global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __re
   }
 }
 
+// hminus: the H- continuum's rows (kept forward only), its sums in slots
+// NS+4 .. NS+10 of kslots per level
 template <int NS>
 void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const double* nu, const double* tables,
                    size_t stride, Records rec, int L, int n, int tiles, int batch, const double* fbar,
-                   double* buffer, double* partial) {
+                   double* buffer, double* partial, const double* hminus, int kslots) {
   if (kept) {
     float* kept_values = reinterpret_cast<float*>(buffer);
-    hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, fbar,
-                       kept_values, partial);
-    hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
-                       wholeGrid(tiles), batch, kept_values, partial);
+    hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, kslots,
+                       fbar, kept_values, partial);
+    if (hminus)
+      hipLaunchKernelGGL((opacityAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
+                         wholeGrid(tiles), batch, kept_values, hminus, partial);
+    else
+      hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
+                         wholeGrid(tiles), batch, kept_values, hminus, partial);
   } else {
     hipLaunchKernelGGL(emissionAdjointKernel<NS>, grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n, tiles,
                        batch, fbar, buffer, partial);
@@ -628,11 +650,11 @@ void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const doub
 // The host carries d logL / d chord and d logL / d c_t to the altitudes and
 // the radius.
 __global__ __launch_bounds__(kWave) void tangentAdjointKernel(Records rec, int nb_levels, int n, size_t stride, int tiles,
-                                                              TileWindow win, int batch, int ns,
+                                                              TileWindow win, int batch, int ns, int kslots,
                                                               const double* __restrict__ fbar,
                                                               const double* __restrict__ depth,
                                                               float* __restrict__ keep, double* __restrict__ partial) {
-  const int L = nb_levels, K = ns + 4;
+  const int L = nb_levels, K = kslots;
   const int lane = int(threadIdx.x);
   const size_t level_step = kKeptLevel;
   const size_t total = nbItems(win.count, batch);
@@ -803,10 +825,10 @@ __global__ __launch_bounds__(kBlock) void chordTransposeKernel(const double* __r
 __global__ __launch_bounds__(kWave) void segmentAdjointKernel(Records rec, const double* __restrict__ chord_t,
                                                               const double* __restrict__ rayleigh, int nb_levels,
                                                               int n, size_t stride, int tiles, TileWindow win,
-                                                              int batch, int ns, float* __restrict__ keep,
+                                                              int batch, int ns, int kslots, float* __restrict__ keep,
                                                               double* __restrict__ partial,
                                                               const int* __restrict__ reach) {
-  const int L = nb_levels, K = ns + 4;
+  const int L = nb_levels, K = kslots;
   const int rows = segmentRows(L);
   const int lane = int(threadIdx.x);
   const size_t level_step = kKeptLevel;
@@ -903,17 +925,21 @@ void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const dou
                                const double* rayleigh, Records rec, int L, int n, int tiles, TileWindow win,
                                int batch, const double* fbar, const double* depth, float* keep, double* partial,
                                int ranges, int r0, int r1, int nu_per_range, double* chord_partial,
-                               const double* chord_t, int* reach) {
+                               const double* chord_t, int* reach, const double* hminus, int kslots) {
   hipLaunchKernelGGL(tangentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, L, n, stride, tiles, win, batch, NS,
-                     fbar, depth, keep, partial);
+                     kslots, fbar, depth, keep, partial);
   const int P = L - 1, blocks = (P + kChordBlock - 1) / kChordBlock;
   hipLaunchKernelGGL(chordAdjointKernel,
                      dim3(unsigned(blocks * (blocks + 1) / 2), unsigned(batch), unsigned(r1 - r0)), dim3(256), 0, s,
                      keep, L, n, stride, ranges, r0, nu_per_range, win, tiles, reach, chord_partial);
   hipLaunchKernelGGL(segmentAdjointKernel, rt_grid, dim3(kWave), 0, s, rec, chord_t, rayleigh, L, n, stride, tiles,
-                     win, batch, NS, keep, partial, reach);
-  hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, win,
-                     batch, keep, partial);
+                     win, batch, NS, kslots, keep, partial, reach);
+  if (hminus)
+    hipLaunchKernelGGL((opacityAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
+                       win, batch, keep, hminus, partial);
+  else
+    hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, win,
+                       batch, keep, hminus, partial);
 }
 
 size_t align8(size_t x) { return (x + 7) / 8 * 8; }
@@ -936,14 +962,14 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   if (transmission != transmission_) raise(kInternal, "adjoint model kind differs from the configured model");
   const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
-  if (hminus_) raise(kInvalidInput, "the adjoint does not take the H- continuum");
   if (cloud_models_ > 0) raise(kInvalidInput, "the adjoint does not take cloud layers");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
   const int NS = int(species_stride_);
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
   check(hipSetDevice(device_), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + 4;
+  // level slots: NS + 4, and 8 more for the H- continuum's weights
+  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + (hminus_ ? 12 : 4);
   const int tiles = (n + kWave - 1) / kWave;
   const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
   const int rt_grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kRtWavesPerCu)));
@@ -1030,6 +1056,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     adj_intensity_ = deviceAlloc<double>(buffer_size, "adjoint forward quantities");
     adj_partial_ = deviceAlloc<double>(partial_size, "adjoint partial sums");
   }
+  if (hminus_ && !kept)
+    raise(kInvalidInput, "the H- continuum's gradient needs the kept forward quantities: raise the adjoint memory");
   const BatchLaunch launch = tuned(transmission ? ModelKind::transmission : ModelKind::emission, chunk);
   bin_terms.assign(transmission ? batch * P * P : eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
@@ -1093,7 +1121,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   case NSV:                                                                                                        \
     launchTransmissionAdjoint<NSV>(dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, tables_, stride_, rayleigh_,  \
                                    rec, L, n, tiles, win, cb, adj_fbar_, batch_flux_ + first * stride_, keep,     \
-                                   adj_partial_, ranges, r0, r1, nu_per_range, chord_partial, chord_t, reach);    \
+                                   adj_partial_, ranges, r0, r1, nu_per_range, chord_partial, chord_t, reach,     \
+                                   hminus_, K);                                                                   \
     break;
           BEAR_TADJ(4)
           BEAR_TADJ(8)
@@ -1114,7 +1143,7 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
 #define BEAR_ADJ(NSV)                                                                                              \
   case NSV:                                                                                                        \
     launchAdjoint<NSV>(kept, dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, wavenumber_, tables_, stride_, rec, \
-                       L, n, tiles, cb, adj_fbar_, adj_intensity_, adj_partial_);                                  \
+                       L, n, tiles, cb, adj_fbar_, adj_intensity_, adj_partial_, hminus_, K);                      \
     break;
       BEAR_ADJ(4)
       BEAR_ADJ(8)
@@ -1135,6 +1164,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
       BEAR_RED(12)
       BEAR_RED(16)
       BEAR_RED(20)
+      BEAR_RED(24)
+      BEAR_RED(28)
 #undef BEAR_RED
     }
     check(hipGetLastError(), "adjoint reduction kernel");

@@ -558,7 +558,11 @@ void launchTransmission(TablePrecision p, dim3 grid, size_t lds, hipStream_t s, 
   const dim3 block(kTransmissionBlock);
   switch (p) {
     case TablePrecision::fp64:
-      if (keep)
+      if (keep && extras)
+        hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true, true>), grid, block, lds, s, tab64, base,
+                           stride, rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep,
+                           reach);
+      else if (keep)
         hipLaunchKernelGGL((transmissionBatchKernel<Opacity64<NS>, true>), grid, block, lds, s, tab64, base, stride,
                            rayleigh, lam, ex, rec, chord, chord32, L, n, win, tile, batch, group, depth, keep, reach);
       else if (extras)

@@ -512,7 +512,7 @@ void EmissionModel::calcSpectrum(double g, double radius, double distance, const
 void TabulatedModel::chainGradient(const double* params, size_t batch, size_t stride,
                                    const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
                                    const std::vector<double>& bin_terms, const char* neglect, double* gradient) {
-  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = NS + 4;
+  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = levelSlots();
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
@@ -535,7 +535,7 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
         t_bar[l] = G[l * K + NS] + G[l * K + NS + 3];
         z_bar[l] = G[l * K + NS + 1];
       }
-      densityGradient(atm, G, n_bar);
+      densityGradient(atm, G, n_bar, t_bar);
       double* out = gradient + b * stride;
       double g_bar = 0;
       atm.adjoint(g, temp, chem, t_bar, n_bar, z_bar, g_bar, out + 3 + nb_chem_, out + 3);
@@ -553,8 +553,25 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
   if (!failure.empty()) raise(failure_code, failure);
 }
 
-void TabulatedModel::densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar) const {
-  const size_t L = atm.nb_levels, K = engine_.speciesStride() + 4, ids = size_t(nbSpeciesIds());
+void TabulatedModel::densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar,
+                                     std::vector<double>& t_bar) const {
+  const size_t L = atm.nb_levels, NS = engine_.speciesStride(), K = levelSlots(), ids = size_t(nbSpeciesIds());
+  // the H- continuum: slots NS+4 .. NS+10 hold d logL / d c_i for the
+  // weights of hminusCoefficients, c_0 = n_H- and, for i >= 1,
+  // c_i = a_i T n_e n_H with a_i = 1e-29 (5040 / T)^((i+1)/2) k
+  if (hminus_)
+    for (size_t l = 0; l < L; ++l) {
+      if (atm.nd(l, hminus_ids_[0]) == 0) continue;                   // no H- term at this level
+      const double* gh = G + l * K + NS + 4;
+      const double t = atm.temperature[l], n_h = atm.nd(l, hminus_ids_[1]), n_e = atm.nd(l, hminus_ids_[2]);
+      n_bar[l * ids + size_t(hminus_ids_[0])] += gh[0];
+      for (int i = 1; i < 7; ++i) {
+        const double a = 1e-29 * std::pow(5040.0 / t, (i + 1) / 2.0) * phys::boltzmann_k;
+        n_bar[l * ids + size_t(hminus_ids_[1])] += gh[i] * a * t * n_e;
+        n_bar[l * ids + size_t(hminus_ids_[2])] += gh[i] * a * t * n_h;
+        t_bar[l] += gh[i] * a * n_e * n_h * (1.0 - (i + 1) / 2.0);
+      }
+    }
   for (size_t l = 0; l < L; ++l)
     for (size_t s = 0; s < species_.size(); ++s) {
       const OpacitySpecies& sp = species_[s];
@@ -695,7 +712,7 @@ bool TransmissionModel::setup(const double* p, Atmosphere& atm, BatchModel& scal
 void TransmissionModel::gradientBatch(const double* params, size_t batch, size_t stride,
                                       const std::vector<double>& level_grad, const std::vector<double>& bin_grad,
                                       const std::vector<double>& chord_grad, const char* neglect, double* gradient) {
-  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = NS + 4, P = L - 1;
+  const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = levelSlots(), P = L - 1;
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
@@ -716,7 +733,7 @@ void TransmissionModel::gradientBatch(const double* params, size_t batch, size_t
       const double* G = level_grad.data() + b * L * K;
       std::vector<double> t_bar(L), z_bar(L, 0.0), n_bar(L * ids, 0.0);
       for (size_t l = 0; l < L; ++l) t_bar[l] = G[l * K + NS];
-      densityGradient(atm, G, n_bar);
+      densityGradient(atm, G, n_bar, t_bar);
       if (rayleigh_species_ >= 0)
         for (size_t l = 0; l < L; ++l) n_bar[l * ids + size_t(rayleigh_species_)] += G[l * K + NS + 3];
       const std::vector<double>& z = atm.altitude;

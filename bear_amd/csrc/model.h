@@ -126,7 +126,12 @@ class TabulatedModel : public ForwardModel {
   }
   // dL/dn[level][species id] from the level gradients G [levels][slots]
   // (n_eff dL/dn_eff per stencil slot) through the CIA products
-  void densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar) const;
+  // per level and species id d logL / d n from the kernels' level slots (the
+  // H- continuum's weights also give temperature terms, added to t_bar)
+  void densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar,
+                       std::vector<double>& t_bar) const;
+  // the adjoint's level slots: NS + 4, and 8 more with the H- continuum
+  size_t levelSlots() const { return engine_.speciesStride() + (hminus_ ? 12 : 4); }
   // the observations' shift parameters: per observation the sum of its bins' dL/dbin
   void shiftGradient(const double* bin_grad, double* out) const;
   const std::vector<ObservationSetup>& observations_;

@@ -156,14 +156,20 @@ def test_many_species_limits(case_factory):
 
 def test_hminus_limits(case_factory):
     """The H- continuum (rows per wavenumber, weights per level) enters the
-    single-point and batched kernels (parity in test_emission_parity and
-    test_fp64_batch_with_hminus_and_clouds); the adjoint kernels do not take
-    it and refuse the case with InvalidInput."""
+    single-point, batched and adjoint kernels (parity in
+    test_emission_parity, test_fp64_batch_with_hminus_and_clouds and the
+    gradient tests); the adjoint that recomputes its forward sweep (when the
+    kept quantities exceed the adjoint memory) does not take it and refuses
+    the case with InvalidInput."""
     r, o, params, _ = retrieval(case_factory, "hminus")
     pts = np.array(params[:2])
-    with pytest.raises(api.BearError) as e:
-        r.gradientBatch(pts)
-    assert e.value.code == -1 and "H-" in str(e.value)
+    r.set_adjoint_memory(0)                    # no kept quantities: the recomputing adjoint
+    try:
+        with pytest.raises(api.BearError) as e:
+            r.gradientBatch(pts)
+        assert e.value.code == -1 and "H-" in str(e.value)
+    finally:
+        r.set_adjoint_memory(96 << 30)
     assert abs(r.computeLikelihood(params[0]) - o.evaluate(params[0])["loglike"]) <= LOGLIKE_ATOL
 
 
