@@ -273,12 +273,14 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
  * secondary-eclipse and transmission models, FP64 tables).  No counterpart
  * in the reference, which has no derivatives: a hand-written adjoint of the
  * whole chain (parameters -> temperature profile, chemistry, altitudes ->
- * opacity interpolation (transmission: with Rayleigh scattering) -> short
- * characteristics, or the tangent paths' optical depths, chord lengths and
- * transit-depth integral -> folded observation operator, for the eclipse
- * against the blackbody star -> Gaussian likelihood).  log_likelihood
- * [batch] as bear_retrieval_compute_likelihood_batch; gradient
- * [batch][nb_parameters] (rows of neglected points are zero). */
+ * opacity interpolation (transmission: with Rayleigh scattering; the H-
+ * continuum when the adjoint memory holds the kept forward quantities) ->
+ * short characteristics, or the tangent paths' optical depths, chord lengths
+ * and transit-depth integral -> folded observation operator, for the eclipse
+ * against the blackbody star -> Gaussian likelihood).  Models with cloud
+ * layers return BEAR_ERROR_INVALID_INPUT.  log_likelihood [batch] as
+ * bear_retrieval_compute_likelihood_batch; gradient [batch][nb_parameters]
+ * (rows of neglected points are zero). */
 int bear_retrieval_gradient_batch(bear_retrieval* r, const double* parameters, size_t batch, size_t nb_parameters,
                                   double* log_likelihood, double* gradient);
 
