@@ -418,10 +418,17 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 // the altitude, the flux scaling (level 0) and the temperature through the
 // Planck function, and overwrites each level's kappa with d logL / d kappa
 // for opacityAdjointKernel.  ns: stencil slots per level.
+// CL: the point's grey and power-law cloud models (BatchModel::clouds, layer
+// weights LevelState::cloud_w) add their optical depth to each layer's, and
+// per layer and model the sums of d logL / d tau times the model's spectral
+// factor f and times d f / d exponent = f (ln lambda - ln lambda_ref) go to
+// slots cloud_slot0 + 2 m and + 2 m + 1.
+template <bool CL = false>
 __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restrict__ wavenumber, Records rec,
                                                          int nb_levels, int n, size_t stride, int tiles, int batch,
                                                          int ns, int kslots, const double* __restrict__ fbar,
-                                                         float* __restrict__ keep, double* __restrict__ partial) {
+                                                         float* __restrict__ keep, double* __restrict__ partial,
+                                                         const double* __restrict__ wavelength, int cloud_slot0) {
   const int L = nb_levels;
   const int K = kslots;
   const int lane = int(threadIdx.x);
@@ -443,6 +450,16 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double unit = 2.0 * kPi * 1e-3;
+    double cf[kMaxClouds], cl[kMaxClouds];
+    if constexpr (CL) {
+      double hm[7];
+      const Extras e{nullptr, nullptr, bm->clouds};
+      const double lam = wavelength[ic];
+      loadExtras(e, ic, n, lam, hm, cf);
+#pragma unroll
+      for (int c = 0; c < kMaxClouds; ++c)
+        cl[c] = c < e.clouds.count && e.clouds.kind[c] == kCloudPowerLaw ? cf[c] * (log(lam) - e.clouds.c[c][2]) : 0.0;
+    }
     // kept: d = I - B of both angles and kappa per level, FP32
     const float* top = kp + size_t(L - 1) * level_step;
     double b_hi, db_hi;
@@ -468,7 +485,10 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       const double z_lo = levels[l - 1].altitude;
       const double dz = z_hi - z_lo;
       const double k_mean = (k_hi + k_lo) * 0.5;
-      const double tau = dz * k_mean;
+      double tau = dz * k_mean;
+      if constexpr (CL)
+#pragma unroll
+        for (int c = 0; c < kMaxClouds; ++c) tau = fma(levels[l].cloud_w[c], cf[c], tau);
       double tbar = 0.0, bbar_lo = 0.0;
       if (tau != 0) {
         const double rt = recip(tau);
@@ -482,6 +502,15 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       if (valid) kp[size_t(l) * level_step + 2 * kKeptValue] = float(kbar_hi);
       double x[4] = {zbar_hi, 0.0, bbar_hi * db_hi, 0.0};
       emitSums<4>(out + size_t(l) * tiles * K, lane, x, ns + 1, 3);
+      if constexpr (CL) {
+        double xc[2 * kMaxClouds];
+#pragma unroll
+        for (int c = 0; c < kMaxClouds; ++c) {
+          xc[2 * c] = tbar * cf[c];
+          xc[2 * c + 1] = tbar * cl[c];
+        }
+        emitSums<2 * kMaxClouds>(out + size_t(l) * tiles * K, lane, xc, cloud_slot0, 2 * kMaxClouds);
+      }
       k_hi = k_lo;
       b_hi = b_lo;
       db_hi = db_lo;
@@ -494,6 +523,10 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     if (valid) kp[2 * kKeptValue] = float(kbar_hi);
     double x[4] = {zbar_hi, s_bar, bbar_hi * db_hi, 0.0};
     emitSums<4>(out, lane, x, ns + 1, 3);
+    if constexpr (CL) {                                              // no layer below level 0
+      double xc[2 * kMaxClouds] = {};
+      emitSums<2 * kMaxClouds>(out, lane, xc, cloud_slot0, 2 * kMaxClouds);
+    }
   }
 }
 
@@ -510,10 +543,10 @@ __global__ __launch_bounds__(kWave) void opacityAdjointKernel(const double* __re
                                                               Records rec, int nb_levels, int n, int tiles,
                                                               TileWindow win, int batch,
                                                               const float* __restrict__ keep,
-                                                              const double* __restrict__ hminus,
+                                                              const double* __restrict__ hminus, int kslots,
                                                               double* __restrict__ partial) {
   constexpr int KX = NS + 4;
-  constexpr int K = NS + (HM ? 12 : 4);
+  const int K = kslots;                                             // partial row: every level slot
   const int L = nb_levels;
   const int lane = int(threadIdx.x);
   const size_t total = nbItems(win.count, batch);
@@ -603,19 +636,24 @@ __global__ __launch_bounds__(kBlock) void adjointReduceKernel(const double* __re
 // hminus: the H- continuum's rows (kept forward only), its sums in slots
 // NS+4 .. NS+10 of kslots per level
 template <int NS>
-void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const double* nu, const double* tables,
-                   size_t stride, Records rec, int L, int n, int tiles, int batch, const double* fbar,
-                   double* buffer, double* partial, const double* hminus, int kslots) {
+void launchAdjoint(bool kept, dim3 grid, dim3 rt_grid, hipStream_t s, const double* nu, const double* lam,
+                   const double* tables, size_t stride, Records rec, int L, int n, int tiles, int batch,
+                   const double* fbar, double* buffer, double* partial, const double* hminus, int kslots,
+                   int cloud_slot0) {
   if (kept) {
     float* kept_values = reinterpret_cast<float*>(buffer);
-    hipLaunchKernelGGL(rtAdjointKernel, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS, kslots,
-                       fbar, kept_values, partial);
+    if (cloud_slot0 >= 0)
+      hipLaunchKernelGGL(rtAdjointKernel<true>, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS,
+                         kslots, fbar, kept_values, partial, lam, cloud_slot0);
+    else
+      hipLaunchKernelGGL(rtAdjointKernel<false>, rt_grid, dim3(kWave), 0, s, nu, rec, L, n, stride, tiles, batch, NS,
+                         kslots, fbar, kept_values, partial, lam, cloud_slot0);
     if (hminus)
       hipLaunchKernelGGL((opacityAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
-                         wholeGrid(tiles), batch, kept_values, hminus, partial);
+                         wholeGrid(tiles), batch, kept_values, hminus, kslots, partial);
     else
       hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
-                         wholeGrid(tiles), batch, kept_values, hminus, partial);
+                         wholeGrid(tiles), batch, kept_values, hminus, kslots, partial);
   } else {
     hipLaunchKernelGGL(emissionAdjointKernel<NS>, grid, dim3(kWave), 0, s, nu, tables, stride, rec, L, n, tiles,
                        batch, fbar, buffer, partial);
@@ -936,10 +974,10 @@ void launchTransmissionAdjoint(dim3 grid, dim3 rt_grid, hipStream_t s, const dou
                      win, batch, NS, kslots, keep, partial, reach);
   if (hminus)
     hipLaunchKernelGGL((opacityAdjointKernel<NS, true>), grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles,
-                       win, batch, keep, hminus, partial);
+                       win, batch, keep, hminus, kslots, partial);
   else
     hipLaunchKernelGGL(opacityAdjointKernel<NS>, grid, dim3(kWave), 0, s, tables, stride, rec, L, n, tiles, win,
-                       batch, keep, hminus, partial);
+                       batch, keep, hminus, kslots, partial);
 }
 
 size_t align8(size_t x) { return (x + 7) / 8 * 8; }
@@ -962,14 +1000,18 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   if (transmission != transmission_) raise(kInternal, "adjoint model kind differs from the configured model");
   const bool eclipse = kind == ModelKind::eclipse;
   if (precision_ != TablePrecision::fp64) raise(kInvalidInput, "the adjoint needs the FP64 tables");
-  if (cloud_models_ > 0) raise(kInvalidInput, "the adjoint does not take cloud layers");
+  if (cloud_models_ > 0 && kind == ModelKind::transmission)
+    raise(kInvalidInput, "the transmission adjoint does not take cloud layers");
   if (batch == 0 || batch > batch_capacity_) raise(kInternal, "adjoint batch larger than the staged records");
   const int NS = int(species_stride_);
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
   check(hipSetDevice(device_), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  // level slots: NS + 4, and 8 more for the H- continuum's weights
-  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_), K = NS + (hminus_ ? 12 : 4);
+  // level slots: NS + 4, 8 more for the H- continuum's weights, 8 more for
+  // the cloud models' layer sums
+  const int cloud_slot0 = cloud_models_ > 0 ? NS + (hminus_ ? 12 : 4) : -1;
+  const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_),
+            K = NS + (hminus_ ? 12 : 4) + (cloud_models_ > 0 ? 2 * kMaxClouds : 0);
   const int tiles = (n + kWave - 1) / kWave;
   const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
   const int rt_grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kRtWavesPerCu)));
@@ -1058,6 +1100,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   }
   if (hminus_ && !kept)
     raise(kInvalidInput, "the H- continuum's gradient needs the kept forward quantities: raise the adjoint memory");
+  if (cloud_models_ > 0 && !kept)
+    raise(kInvalidInput, "the clouds' gradient needs the kept forward quantities: raise the adjoint memory");
   const BatchLaunch launch = tuned(transmission ? ModelKind::transmission : ModelKind::emission, chunk);
   bin_terms.assign(transmission ? batch * P * P : eclipse ? batch * n_bins_ * 2 : 0, 0.0);
   if (profiling_) check(hipEventRecord(static_cast<hipEvent_t>(event_start_), s), "hipEventRecord");
@@ -1142,8 +1186,9 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
     switch (NS) {
 #define BEAR_ADJ(NSV)                                                                                              \
   case NSV:                                                                                                        \
-    launchAdjoint<NSV>(kept, dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, wavenumber_, tables_, stride_, rec, \
-                       L, n, tiles, cb, adj_fbar_, adj_intensity_, adj_partial_, hminus_, K);                      \
+    launchAdjoint<NSV>(kept, dim3(unsigned(grid)), dim3(unsigned(rt_grid)), s, wavenumber_, wavelength_, tables_,   \
+                       stride_, rec, L, n, tiles, cb, adj_fbar_, adj_intensity_, adj_partial_, hminus_, K,         \
+                       cloud_slot0);                                                                               \
     break;
       BEAR_ADJ(4)
       BEAR_ADJ(8)
@@ -1166,6 +1211,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
       BEAR_RED(20)
       BEAR_RED(24)
       BEAR_RED(28)
+      BEAR_RED(32)
+      BEAR_RED(36)
 #undef BEAR_RED
     }
     check(hipGetLastError(), "adjoint reduction kernel");

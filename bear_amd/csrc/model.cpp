@@ -273,6 +273,7 @@ void TabulatedModel::stageClouds(const Atmosphere& atm, const double* cp, double
     if (c.kind == kCloudPowerLaw) {
       spec.c[m][0] = q[1];
       spec.c[m][1] = 1.0 / std::pow(c.reference_wavelength, q[1]);
+      spec.c[m][2] = std::log(c.reference_wavelength);               // for the exponent's gradient
     } else if (c.kind == kCloudKH) {
       const double x_ref = two_pi * q[3] / c.reference_wavelength;
       spec.c[m][0] = q[1];
@@ -515,6 +516,8 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
   const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = levelSlots();
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
+  for (const CloudModelSpec& c : clouds_)
+    if (c.kind == kCloudKH) raise(kInvalidInput, "the gradient does not take the KHnongrey cloud model");
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
   while (atmosphere_pool_.size() < size_t(threads)) atmosphere_pool_.emplace_back(new Atmosphere(atmosphere_));
   std::string failure;
@@ -542,6 +545,7 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
       out[0] = g_bar * g * std::log(10.0);
       generalGradient(p, G[NS + 2], bin_terms.empty() ? nullptr : bin_terms.data() + b * nb * 2, out);
       shiftGradient(bin_grad.data() + b * nb, out);
+      if (!clouds_.empty()) cloudGradient(atm, p, g, G, out);
     } catch (const Error& e) {
 #pragma omp critical(bear_gradient_failure)
       if (failure.empty()) { failure = e.what(); failure_code = e.code; }
@@ -551,6 +555,40 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
     }
   }
   if (!failure.empty()) raise(failure_code, failure);
+}
+
+// The clouds' layer weights are w_m = tau / (top - bottom) on their layers
+// (stageClouds; tau < 0 counts as 0) and the power-law factor is
+// (lambda / lambda_ref)^a: d logL / d tau = sum over layers of
+// (d logL / d w_m) / (top - bottom), d logL / d a = sum of w_m times the
+// layer's exponent sum.  The cloud's position in pressure (or one scale
+// height below its top) moves in whole levels: no gradient.
+void TabulatedModel::cloudGradient(const Atmosphere& atm, const double* p, double g, const double* G,
+                                   double* out) const {
+  const size_t L = atm.nb_levels, K = levelSlots(), c0 = cloudSlot();
+  const double* cp = clouds(p);
+  std::vector<LevelState> lev(L), unit(L);
+  CloudSpec spec;
+  stageClouds(atm, cp, g, fixedScaleHeight(p), lev.data(), spec);
+  std::vector<double> q(cp, cp + nb_cloud_);                       // unit optical depths: the weights' slopes
+  size_t off = 0;
+  for (const CloudModelSpec& c : clouds_) {
+    q[off] = 1.0;
+    off += c.nbParameters();
+  }
+  stageClouds(atm, q.data(), g, fixedScaleHeight(p), unit.data(), spec);
+  double* gc = out + (cp - p);
+  off = 0;
+  for (size_t m = 0; m < clouds_.size(); ++m) {
+    double d_tau = 0.0, d_exp = 0.0;
+    for (size_t l = 1; l < L; ++l) {
+      d_tau += G[l * K + c0 + 2 * m] * unit[l].cloud_w[m];
+      d_exp += G[l * K + c0 + 2 * m + 1] * lev[l].cloud_w[m];
+    }
+    if (cp[off] >= 0) gc[off] += d_tau;
+    if (clouds_[m].kind == kCloudPowerLaw) gc[off + 1] += d_exp;
+    off += clouds_[m].nbParameters();
+  }
 }
 
 void TabulatedModel::densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar,
