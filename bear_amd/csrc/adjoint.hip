@@ -418,11 +418,14 @@ __global__ __launch_bounds__(kWave) void emissionAdjointKernel(
 // the altitude, the flux scaling (level 0) and the temperature through the
 // Planck function, and overwrites each level's kappa with d logL / d kappa
 // for opacityAdjointKernel.  ns: stencil slots per level.
-// CL: the point's grey and power-law cloud models (BatchModel::clouds, layer
-// weights LevelState::cloud_w) add their optical depth to each layer's, and
-// per layer and model the sums of d logL / d tau times the model's spectral
-// factor f and times d f / d exponent = f (ln lambda - ln lambda_ref) go to
-// slots cloud_slot0 + 2 m and + 2 m + 1.
+// CL: the point's cloud models (BatchModel::clouds, layer weights
+// LevelState::cloud_w) add their optical depth to each layer's, and per
+// layer and model four sums of d logL / d tau times g_k go to slots
+// cloud_slot0 + 4 m + k: g_0 = f, the model's spectral factor; power law
+// f = (lambda / lambda_ref)^a: g_1 = f (ln lambda - ln lambda_ref);
+// KHnongrey f = c3 / u, u = q0 x^-a0 + x^0.2, x = 2 pi r / lambda:
+// g_1 = f x^-a0 / u, g_2 = f q0 x^-a0 ln x / u,
+// g_3 = f (0.2 x^0.2 - a0 q0 x^-a0) / u (the host adds c3's own terms).
 template <bool CL = false>
 __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restrict__ wavenumber, Records rec,
                                                          int nb_levels, int n, size_t stride, int tiles, int batch,
@@ -450,15 +453,26 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     const double x_nu = -(6.62606896e-27 * nu * 2.99792458e10 / 1.3806504e-16) * kLog2E;
     const double fb = valid ? fbar[size_t(m) * stride + i] : 0.0;
     const double unit = 2.0 * kPi * 1e-3;
-    double cf[kMaxClouds], cl[kMaxClouds];
+    double cf[kMaxClouds], cg[kMaxClouds][3];
     if constexpr (CL) {
       double hm[7];
       const Extras e{nullptr, nullptr, bm->clouds};
       const double lam = wavelength[ic];
       loadExtras(e, ic, n, lam, hm, cf);
 #pragma unroll
-      for (int c = 0; c < kMaxClouds; ++c)
-        cl[c] = c < e.clouds.count && e.clouds.kind[c] == kCloudPowerLaw ? cf[c] * (log(lam) - e.clouds.c[c][2]) : 0.0;
+      for (int c = 0; c < kMaxClouds; ++c) {
+        cg[c][0] = cg[c][1] = cg[c][2] = 0.0;
+        if (c < e.clouds.count && e.clouds.kind[c] == kCloudPowerLaw) {
+          cg[c][0] = cf[c] * (log(lam) - e.clouds.c[c][2]);
+        } else if (c < e.clouds.count && e.clouds.kind[c] == kCloudKH) {
+          const double* k = e.clouds.c[c];
+          const double x = k[2] / lam, xm = pow(x, -k[1]), x2 = pow(x, 0.2);
+          const double fu = cf[c] / (k[0] * xm + x2);
+          cg[c][0] = fu * xm;
+          cg[c][1] = fu * k[0] * xm * log(x);
+          cg[c][2] = fu * (0.2 * x2 - k[1] * k[0] * xm);
+        }
+      }
     }
     // kept: d = I - B of both angles and kappa per level, FP32
     const float* top = kp + size_t(L - 1) * level_step;
@@ -503,13 +517,14 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
       double x[4] = {zbar_hi, 0.0, bbar_hi * db_hi, 0.0};
       emitSums<4>(out + size_t(l) * tiles * K, lane, x, ns + 1, 3);
       if constexpr (CL) {
-        double xc[2 * kMaxClouds];
+        double xc[4 * kMaxClouds];
 #pragma unroll
         for (int c = 0; c < kMaxClouds; ++c) {
-          xc[2 * c] = tbar * cf[c];
-          xc[2 * c + 1] = tbar * cl[c];
+          xc[4 * c] = tbar * cf[c];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) xc[4 * c + 1 + k] = tbar * cg[c][k];
         }
-        emitSums<2 * kMaxClouds>(out + size_t(l) * tiles * K, lane, xc, cloud_slot0, 2 * kMaxClouds);
+        emitSums<4 * kMaxClouds>(out + size_t(l) * tiles * K, lane, xc, cloud_slot0, 4 * kMaxClouds);
       }
       k_hi = k_lo;
       b_hi = b_lo;
@@ -524,8 +539,8 @@ __global__ __launch_bounds__(kWave) void rtAdjointKernel(const double* __restric
     double x[4] = {zbar_hi, s_bar, bbar_hi * db_hi, 0.0};
     emitSums<4>(out, lane, x, ns + 1, 3);
     if constexpr (CL) {                                              // no layer below level 0
-      double xc[2 * kMaxClouds] = {};
-      emitSums<2 * kMaxClouds>(out, lane, xc, cloud_slot0, 2 * kMaxClouds);
+      double xc[4 * kMaxClouds] = {};
+      emitSums<4 * kMaxClouds>(out, lane, xc, cloud_slot0, 4 * kMaxClouds);
     }
   }
 }
@@ -1007,11 +1022,11 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
   if (NS != 4 && NS != 8 && NS != 12 && NS != 16) raise(kInvalidInput, "the adjoint takes at most 16 tabulated species");
   check(hipSetDevice(device_), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream_);
-  // level slots: NS + 4, 8 more for the H- continuum's weights, 8 more for
+  // level slots: NS + 4, 8 more for the H- continuum's weights, 16 more for
   // the cloud models' layer sums
   const int cloud_slot0 = cloud_models_ > 0 ? NS + (hminus_ ? 12 : 4) : -1;
   const int n = int(n_points_), L = int(nb_levels_), nb = int(n_bins_),
-            K = NS + (hminus_ ? 12 : 4) + (cloud_models_ > 0 ? 2 * kMaxClouds : 0);
+            K = NS + (hminus_ ? 12 : 4) + (cloud_models_ > 0 ? 4 * kMaxClouds : 0);
   const int tiles = (n + kWave - 1) / kWave;
   const int grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kAdjointWavesPerCu)));
   const int rt_grid = int(align8(std::max<size_t>(8, size_t(nb_cus_) * kRtWavesPerCu)));
@@ -1213,6 +1228,8 @@ void DeviceEngine::evaluateAdjointBatch(ModelKind kind, size_t batch, std::vecto
       BEAR_RED(28)
       BEAR_RED(32)
       BEAR_RED(36)
+      BEAR_RED(40)
+      BEAR_RED(44)
 #undef BEAR_RED
     }
     check(hipGetLastError(), "adjoint reduction kernel");

@@ -516,8 +516,6 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
   const size_t L = atmosphere_.nb_levels, NS = engine_.speciesStride(), K = levelSlots();
   const size_t ids = size_t(nbSpeciesIds());
   const size_t nb = engine_.nbBins();
-  for (const CloudModelSpec& c : clouds_)
-    if (c.kind == kCloudKH) raise(kInvalidInput, "the gradient does not take the KHnongrey cloud model");
   const int threads = std::max(1, std::min(omp_get_max_threads(), int(batch)));
   while (atmosphere_pool_.size() < size_t(threads)) atmosphere_pool_.emplace_back(new Atmosphere(atmosphere_));
   std::string failure;
@@ -558,11 +556,14 @@ void TabulatedModel::chainGradient(const double* params, size_t batch, size_t st
 }
 
 // The clouds' layer weights are w_m = tau / (top - bottom) on their layers
-// (stageClouds; tau < 0 counts as 0) and the power-law factor is
-// (lambda / lambda_ref)^a: d logL / d tau = sum over layers of
-// (d logL / d w_m) / (top - bottom), d logL / d a = sum of w_m times the
-// layer's exponent sum.  The cloud's position in pressure (or one scale
-// height below its top) moves in whole levels: no gradient.
+// (stageClouds; tau < 0 counts as 0): d logL / d tau = sum over layers of
+// (d logL / d w_m) / (top - bottom).  The spectral parameters from the
+// device's per-layer sums S_k (slots 4 m + k, rtAdjointKernel<true>) times
+// w_m: power law, d / d a = sum w S_1; KHnongrey, f = c3 / u with
+// c3 = q0 x_ref^-a0 + x_ref^0.2 (x_ref = 2 pi r / lambda_ref), its own
+// derivatives times S_0 / c3 beside the device's u terms.  The cloud's
+// position in pressure (or one scale height below its top) moves in whole
+// levels: no gradient.
 void TabulatedModel::cloudGradient(const Atmosphere& atm, const double* p, double g, const double* G,
                                    double* out) const {
   const size_t L = atm.nb_levels, K = levelSlots(), c0 = cloudSlot();
@@ -580,13 +581,24 @@ void TabulatedModel::cloudGradient(const Atmosphere& atm, const double* p, doubl
   double* gc = out + (cp - p);
   off = 0;
   for (size_t m = 0; m < clouds_.size(); ++m) {
-    double d_tau = 0.0, d_exp = 0.0;
+    double d_tau = 0.0, s[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t l = 1; l < L; ++l) {
-      d_tau += G[l * K + c0 + 2 * m] * unit[l].cloud_w[m];
-      d_exp += G[l * K + c0 + 2 * m + 1] * lev[l].cloud_w[m];
+      const double* g = G + l * K + c0 + 4 * m;
+      d_tau += g[0] * unit[l].cloud_w[m];
+      for (int k = 0; k < 4; ++k) s[k] += g[k] * lev[l].cloud_w[m];
     }
-    if (cp[off] >= 0) gc[off] += d_tau;
-    if (clouds_[m].kind == kCloudPowerLaw) gc[off + 1] += d_exp;
+    const double* q = cp + off;
+    if (q[0] >= 0) gc[off] += d_tau;
+    if (clouds_[m].kind == kCloudPowerLaw) {
+      gc[off + 1] += s[1];
+    } else if (clouds_[m].kind == kCloudKH) {
+      const double q0 = q[1], a0 = q[2], r = q[3];
+      const double x_ref = 2.0 * phys::pi * r / clouds_[m].reference_wavelength;
+      const double xm = std::pow(x_ref, -a0), x2 = std::pow(x_ref, 0.2), c3 = q0 * xm + x2;
+      gc[off + 1] += s[0] * xm / c3 - s[1];
+      gc[off + 2] += -s[0] * q0 * xm * std::log(x_ref) / c3 + s[2];
+      gc[off + 3] += (s[0] * (0.2 * x2 - a0 * q0 * xm) / c3 - s[3]) / r;
+    }
     off += clouds_[m].nbParameters();
   }
 }

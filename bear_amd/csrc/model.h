@@ -130,11 +130,11 @@ class TabulatedModel : public ForwardModel {
   // H- continuum's weights also give temperature terms, added to t_bar)
   void densityGradient(const Atmosphere& atm, const double* G, std::vector<double>& n_bar,
                        std::vector<double>& t_bar) const;
-  // the adjoint's level slots: NS + 4, 8 more with the H- continuum, 8 more
+  // the adjoint's level slots: NS + 4, 8 more with the H- continuum, 16 more
   // with clouds (their layer sums from cloudSlot() on)
   size_t cloudSlot() const { return engine_.speciesStride() + (hminus_ ? 12 : 4); }
-  size_t levelSlots() const { return cloudSlot() + (clouds_.empty() ? 0 : 2 * kMaxClouds); }
-  // d logL / d the cloud models' optical depths and power-law exponents
+  size_t levelSlots() const { return cloudSlot() + (clouds_.empty() ? 0 : 4 * kMaxClouds); }
+  // d logL / d the cloud models' optical depths and spectral parameters
   void cloudGradient(const Atmosphere& atm, const double* p, double g, const double* G, double* out) const;
   // the observations' shift parameters: per observation the sum of its bins' dL/dbin
   void shiftGradient(const double* bin_grad, double* out) const;

@@ -106,8 +106,8 @@ typedef struct bear_model_config {
    * p_top, p_bottom / p_top], power_law [tau, exponent, p_top,
    * p_bottom / p_top], without the last with "fb".  At most 4 models; the
    * single-point calls and the batched calls with FP64 tables compute them,
-   * the gradient calls take grey and power-law models of emission and
-   * eclipse retrievals; the others and reduced-precision batches return
+   * the gradient calls take them in emission and eclipse retrievals;
+   * transmission gradients and reduced-precision batches with clouds return
    * BEAR_ERROR_INVALID_INPUT. */
   const char* cloud_model;
 } bear_model_config;
@@ -278,9 +278,9 @@ int bear_retrieval_compute_model_batch(bear_retrieval* r, const double* paramete
  * continuum when the adjoint memory holds the kept forward quantities) ->
  * short characteristics, or the tangent paths' optical depths, chord lengths
  * and transit-depth integral -> folded observation operator, for the eclipse
- * against the blackbody star -> Gaussian likelihood; the grey and power-law
- * clouds of emission and eclipse models likewise).  The KHnongrey model and
- * transmission models with clouds return BEAR_ERROR_INVALID_INPUT.
+ * against the blackbody star -> Gaussian likelihood; the cloud models of
+ * emission and eclipse retrievals likewise).  Transmission models with
+ * clouds return BEAR_ERROR_INVALID_INPUT.
  * log_likelihood [batch] as
  * bear_retrieval_compute_likelihood_batch; gradient [batch][nb_parameters]
  * (rows of neglected points are zero). */

@@ -86,7 +86,7 @@ TRANSMISSION = ["transmission_plain", "transmission_variable_gravity", "transmis
 
 
 @pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "emission_c1", "eclipse_small", "eclipse_wide", "hminus",
-                                  "cloudy_tiny"] + TRANSMISSION + ["transmission_hminus"])
+                                  "cloudy_tiny", "cloudy_eclipse"] + TRANSMISSION + ["transmission_hminus"])
 def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
@@ -102,7 +102,8 @@ def test_adjoint_matches_central_differences(case_factory, tmp_path_factory, nam
 
 
 @pytest.mark.parametrize("name", ["tiny", "tiny_inflated", "eclipse_small", "eclipse_wide", "transmission_plain",
-                                  "transmission_fit_scale_height", "hminus", "transmission_hminus", "cloudy_tiny"])
+                                  "transmission_fit_scale_height", "hminus", "transmission_hminus", "cloudy_tiny",
+                                  "cloudy_eclipse"])
 def test_adjoint_matches_oracle_fp64_gradient(case_factory, tmp_path_factory, name):
     folder, params, r = gradient_case(case_factory, tmp_path_factory, name)
     r.set_table_precision("fp64")
